@@ -1,0 +1,185 @@
+/*
+ * lsh.h -- C ABI of the MI355X (gfx950) stacking-engine backend.
+ *
+ * The backend fills the engine's backend seam as a third engine beside the
+ * CPU and CUDA ones.  Reference citations are to crates/lapstack-core/src/ of
+ * ragtux/LAPSTACK:
+ *   Backend enum and probe ........ align.rs:27-49, gpu.rs:307-310
+ *   Fuser contract (AnyFuser) ..... stack.rs:122-201, fuse.rs:349-521, gpu.rs:432-563
+ *   Frames contract (GpuFrames) ... stack.rs:206-230, gpu.rs:636-900
+ *   Depth contract ................ depth.rs:1119-1200, gpu.rs:1327-1468
+ *
+ * Conventions follow the reference's own C-ABI plugin (crates/lapstack-raw/
+ * src/ffi.rs): an ABI version check, every call returns NULL on success or a
+ * library-owned error string the caller releases with lsh_free_str, outputs
+ * are caller-provided at documented sizes, and nothing unwinds across the
+ * boundary (every C++ exception is caught and returned as a string).  A size
+ * mismatch is an error string, never an abort.
+ *
+ * Memory: every buffer argument comes with an lsh_mem saying where it lives
+ * (host memory, or device memory of the current HIP device).  Images are
+ * three consecutive row-major f32 planes (R, G, B) of w*h, values in [0, 1]
+ * (the reference's Img3, pyramid.rs:18-24).  Frames in 16-bit form are
+ * interleaved RGB u16 (h*w*3), the decoder's layout (io.rs:80-97).
+ *
+ * Threading: handles are single-thread-affine.  All work of the library runs
+ * in order on one HIP stream per device (lsh_stream), so a frames handle, a
+ * fuser and a depth pass hand device buffers to each other without waits.
+ */
+#ifndef LSH_H
+#define LSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSH_ABI_VERSION 1u
+
+typedef enum lsh_mem { LSH_HOST = 0, LSH_DEVICE = 1 } lsh_mem;
+
+/* Interpolation kernels, in the order of align.rs:146 Interp::ALL (Interp::id). */
+enum { LSH_NEAREST = 0, LSH_BILINEAR = 1, LSH_BICUBIC = 2, LSH_SPLINE4X4 = 3, LSH_SPLINE6X6 = 4, LSH_LANCZOS3 = 5 };
+
+/* fuse.rs:69-107 FuseParams.  levels <= 0: auto_levels(w, h, 32) (pyramid.rs:191).
+ * top_rule: 0 DevEntropy, 1 Deviation, 2 Average (fuse.rs:48-56). */
+typedef struct lsh_fuse_params {
+    int32_t levels;
+    uint32_t energy_radius;
+    uint32_t top_rule;
+    uint32_t top_radius;
+    uint32_t entropy_bins;
+    uint32_t use_chroma;
+    uint32_t depth_level;
+    float halo;
+} lsh_fuse_params;
+
+/* depth.rs:68-109 DepthParams.  focus_kind: 0 RDF{r_in, r_out}, 1 SML{sml_step};
+ * upsample_kind: 0 bilinear, 1 guided{up_radius, up_eps}.  tol is the PCG's
+ * relative-residual stop, fixed at 1e-5 in the reference (depth.rs:1054). */
+typedef struct lsh_depth_params {
+    uint32_t scale;
+    uint32_t focus_kind;
+    uint32_t r_in, r_out, sml_step;
+    uint32_t agg_radius;
+    float agg_eps;
+    float lambda;
+    float sigma_c;
+    uint32_t cg_iters;
+    uint32_t median;
+    uint32_t upsample_kind;
+    uint32_t up_radius;
+    float up_eps;
+    float gate;
+    float robust;
+    float tol;
+} lsh_depth_params;
+
+/* depth.rs:140-156 DepthMap's scalars plus the WLS report (depth.rs:1160, 1179). */
+typedef struct lsh_depth_info {
+    uint32_t dw, dh;
+    uint32_t iters1, iters2;
+    float rel1, rel2;
+    float p90;
+    float floor;
+} lsh_depth_info;
+
+typedef struct lsh_frames lsh_frames;
+typedef struct lsh_fuser lsh_fuser;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* ABI version; a client refuses a mismatch (raw.rs:31-32, ffi.rs:66-71). */
+uint32_t lsh_abi(void);
+/* Backend probe (gpu.rs:307-310 available): NULL when a gfx950 device is usable. */
+char *lsh_available(void);
+/* Release a string returned by the library (ffi.rs:143-152 lapstack_raw_free_str). */
+void lsh_free_str(char *s);
+/* The library's HIP stream on the current device (a hipStream_t), for callers
+ * that order their own work against it. */
+char *lsh_stream(void **stream_out);
+/* Wait for all queued work of the library on the current device. */
+char *lsh_sync(void);
+
+/* Per-kernel timing with HIP events on the library's stream: enable, read
+ * (names separated by '\n' into names; per kernel the total ms, the total
+ * algorithmic HBM bytes -- each array a launch must read or write, once, in
+ * its stored format -- and the launch count), reset.  Off by default; timing
+ * adds an event pair per launch. */
+char *lsh_prof_enable(uint32_t on);
+char *lsh_prof_read(char *names, size_t names_cap, double *total_ms, double *total_bytes, uint64_t *counts,
+                    uint32_t cap, uint32_t *n);
+char *lsh_prof_reset(void);
+
+/* ---- frames: registration, warp, brightness, focus slice ---------------- */
+
+/* gpu.rs:636 GpuFrames::new.  interp: the final resampling kernel (LSH_*). */
+char *lsh_frames_new(uint32_t w, uint32_t h, uint32_t interp, lsh_frames **out);
+
+/* gpu.rs:751-886 GpuFrames::process, with the CPU path's semantics
+ * (stack.rs:513-530 register, :638-706 AlignedFrames::get):
+ *   rgb           one frame, interleaved RGB u16 (h*w*3) in rgb_mem;
+ *   guess         the previous frame's Sim (8 doubles: xoff, yoff, scale, rot,
+ *                 aspect, shear, px, py; align.rs:199-253);
+ *   free8         which Sim parameters the search moves (align.rs:115-119);
+ *   coarsen       stop the search this many levels short of full resolution;
+ *   known_sim, known_gains  (both NULL, or both set) a re-pass's transform and gains;
+ *   brightness    bring the frame to frame 0's brightness (brightness.rs:57-61);
+ *   measure       when non-NULL, the focus slice (depth.rs:1097) is written to
+ *                 slice_out (dw*dh floats, dw = ceil(w/2^scale)) in slice_mem.
+ * The first frame processed sits at the identity and becomes the brightness
+ * reference.  Afterward the handle holds the frame (lsh_fuser_push_frames). */
+char *lsh_frames_process(lsh_frames *f, const uint16_t *rgb, lsh_mem rgb_mem, const double *guess,
+                         const uint8_t *free8, uint32_t coarsen, const double *known_sim, const float *known_gains,
+                         uint32_t brightness, const lsh_depth_params *measure, double *sim_out, float *gains_out,
+                         float *slice_out, lsh_mem slice_mem);
+
+/* stack.rs:361-419 LazyFrames::take for registered stacks (no alignment): the
+ * frame as decoded (io.rs:89), with brightness gains to frame 0's channel means
+ * (brightness.rs:107-117) when asked, and its focus slice when measure != NULL. */
+char *lsh_frames_ingest(lsh_frames *f, const uint16_t *rgb, lsh_mem rgb_mem, uint32_t brightness,
+                        const lsh_depth_params *measure, float *gains_out, float *slice_out, lsh_mem slice_mem);
+
+/* gpu.rs:889-900 planes/download: the held frame's device planes, or a copy. */
+char *lsh_frames_planes(lsh_frames *f, const float **planes_dev);
+char *lsh_frames_download(lsh_frames *f, float *img3, lsh_mem mem);
+/* Cost evaluations the last lsh_frames_process search ran (the reference logs
+ * the same count per frame pair, docs/engine.md:73-75). */
+uint64_t lsh_frames_last_evals(lsh_frames *f);
+void lsh_frames_free(lsh_frames *f);
+
+/* ---- fuser: Laplacian pyramid, region-energy fold, residual rule, collapse */
+
+/* fuse.rs:371 Fuser::new / gpu.rs GpuFuser::new. */
+char *lsh_fuser_new(uint32_t w, uint32_t h, const lsh_fuse_params *p, lsh_fuser **out);
+/* fuse.rs:351 Fuser::levels. */
+char *lsh_fuser_levels(lsh_fuser *f, uint32_t *levels, uint32_t *depth_level, uint32_t *dw, uint32_t *dh);
+/* fuse.rs:401 Fuser::push: one aligned frame (3 planes). */
+char *lsh_fuser_push(lsh_fuser *f, const float *img3, lsh_mem mem);
+/* gpu.rs:451 GpuFuser::push_device: the frame a frames handle holds. */
+char *lsh_fuser_push_frames(lsh_fuser *f, lsh_frames *src);
+/* fuse.rs:489 Fuser::shares: up to cap values (n = frames folded, or 0). */
+char *lsh_fuser_shares(lsh_fuser *f, float *out, uint32_t cap, uint32_t *n);
+/* fuse.rs:499 Fuser::finish: fused image (clamped to [0,1]) into img3 (img_mem);
+ * the winner map nearest-upsampled to w*h f32 (nullable, win_mem); the winner
+ * map at depth_level, dw*dh u16 in host memory (nullable).  The fuser can be
+ * finished once. */
+char *lsh_fuser_finish(lsh_fuser *f, float *img3, lsh_mem img_mem, float *winner_full, lsh_mem win_mem,
+                       uint16_t *winner_level);
+void lsh_fuser_free(lsh_fuser *f);
+
+/* ---- depth ------------------------------------------------------------ */
+
+/* depth.rs:1119-1200 depth_from_slices: n slices of dw*dh (contiguous, frame
+ * order) in slices_mem; the fused image (3*w*h) in fused_mem; depth and conf
+ * (w*h) into out_mem; the noise-floor plane (dw*dh, nullable) into out_mem. */
+char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
+                            lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
+                            float *conf, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

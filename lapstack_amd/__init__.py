@@ -1,0 +1,381 @@
+"""lapstack_amd -- the MI355X (gfx950) backend of lapstack-core's stacking engine.
+
+The host side mirrors the reference engine's backend seam
+(crates/lapstack-core/src/stack.rs:122-230, align.rs:27-49) over the C ABI of
+lapstack_amd/liblsh.so (declared in include/lsh.h):
+
+* ``available()``          -- the backend probe (gpu.rs:307-310).
+* ``Fuser``                -- fuse.rs:349-521 Fuser / gpu.rs GpuFuser: new, levels,
+                              push, push_frames (the device handoff), shares, finish.
+* ``HipFrames``            -- gpu.rs:636-900 GpuFrames: process (register, warp,
+                              brightness, focus slice), ingest (registered stacks).
+* ``depth_from_slices``    -- depth.rs:1119-1200.
+* ``fuse_and_depth``       -- stack.rs:740-853: the per-frame loop over a stack.
+
+Buffers are numpy arrays (host memory) or torch tensors on the HIP device
+(device memory, passed by pointer, no copy).  There is no CPU fallback: every
+call runs on the device, and a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import LSH_DEVICE, LSH_HOST, HipError
+
+__all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
+           "depth_from_slices", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+
+SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+
+
+class Backend:
+    """align.rs:29-36 Backend, with the third engine this package provides."""
+
+    CPU = "cpu"
+    CUDA = "cuda"
+    WGPU = "wgpu"
+    HIP = "hip"
+
+
+class Interp:
+    """align.rs:128-143, numbered as Interp::id (the order of Interp::ALL)."""
+
+    NEAREST, BILINEAR, BICUBIC, SPLINE4X4, SPLINE6X6, LANCZOS3 = range(6)
+    NAMES = {"nearest": 0, "nn": 0, "bilinear": 1, "linear": 1, "bicubic": 2, "cubic": 2, "spline4x4": 3,
+             "spline16": 3, "spline6x6": 4, "spline36": 4, "lanczos3": 5, "lanczos": 5}
+
+    @classmethod
+    def parse(cls, s) -> int:
+        if isinstance(s, int):
+            return s
+        v = cls.NAMES.get(str(s).strip().lower())
+        if v is None:
+            raise ValueError(f"unknown interpolation kernel {s!r}")
+        return v
+
+
+class TopRule:
+    """fuse.rs:48-67."""
+
+    DEV_ENTROPY, DEVIATION, AVERAGE = 0, 1, 2
+    NAMES = {"de": 0, "dev-entropy": 0, "dev": 1, "deviation": 1, "avg": 2, "average": 2}
+
+    @classmethod
+    def parse(cls, s) -> int:
+        return s if isinstance(s, int) else cls.NAMES[s]
+
+
+@dataclasses.dataclass
+class FuseParams:
+    """fuse.rs:69-107 with its defaults."""
+
+    levels: Optional[int] = None
+    energy_radius: int = 1
+    top_rule: object = TopRule.DEV_ENTROPY
+    top_radius: int = 2
+    entropy_bins: int = 256
+    use_chroma: bool = False
+    depth_level: int = 2
+    halo: float = 0.0
+
+    def _c(self):
+        return _lib.FuseParamsC(0 if self.levels is None else int(self.levels), self.energy_radius,
+                                TopRule.parse(self.top_rule), self.top_radius, self.entropy_bins,
+                                int(self.use_chroma), self.depth_level, float(self.halo))
+
+
+@dataclasses.dataclass
+class DepthParams:
+    """depth.rs:68-109 with its defaults; ``tol`` is the PCG stop (1e-5 in depth.rs:1054)."""
+
+    scale: int = 1
+    focus: str = "rdf"
+    r_in: int = 1
+    r_out: int = 3
+    sml_step: int = 1
+    agg_radius: int = 3
+    agg_eps: float = 1e-4
+    lam: float = 3.0
+    sigma_c: float = 0.04
+    cg_iters: int = 200
+    median: bool = True
+    upsample: str = "guided"
+    up_radius: int = 2
+    up_eps: float = 1e-2
+    gate: float = 1.0
+    robust: float = 1.0
+    tol: float = 1e-5
+
+    def _c(self):
+        return _lib.DepthParamsC(self.scale, 0 if self.focus == "rdf" else 1, self.r_in, self.r_out, self.sml_step,
+                                 self.agg_radius, self.agg_eps, self.lam, self.sigma_c, self.cg_iters,
+                                 int(self.median), 1 if self.upsample == "guided" else 0, self.up_radius,
+                                 self.up_eps, self.gate, self.robust, self.tol)
+
+    def grid(self, w, h):
+        k = 1 << self.scale
+        return -(-w // k), -(-h // k)
+
+
+@dataclasses.dataclass
+class AlignParams:
+    """align.rs:51-77 (similarity model by default: shift, scale, rotation free)."""
+
+    shift: bool = True
+    scale: bool = True
+    rotation: bool = True
+    model: str = "similarity"
+    coarsen: int = 0
+    interp: object = "spline4x4"
+
+    def free(self):
+        """align.rs:115-119 free_mask."""
+        affine = self.model != "similarity"
+        proj = self.model == "projective"
+        return [self.shift, self.shift, self.scale, self.rotation, affine, affine, proj, proj]
+
+
+def available() -> Optional[str]:
+    """None when a gfx950 device and the library are usable, else the reason."""
+    try:
+        lib = _lib.load()
+    except OSError as e:
+        return str(e)
+    return _lib.take_err(lib.lsh_available())
+
+
+def _buf(a, dtype):
+    """(pointer, lsh_mem, keepalive) of a numpy array or a device tensor."""
+    if isinstance(a, np.ndarray):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        return a.ctypes.data_as(C.c_void_p), LSH_HOST, a
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        torch = None
+    if torch is not None and isinstance(a, torch.Tensor):
+        if not a.is_cuda:
+            return _buf(a.numpy(), dtype)
+        tdt = {np.float32: torch.float32, np.uint16: torch.uint16}[dtype]
+        if a.dtype != tdt:
+            raise TypeError(f"device tensor must be {tdt}, got {a.dtype}")
+        if not a.is_contiguous():
+            raise ValueError("device tensor must be contiguous")
+        return C.c_void_p(a.data_ptr()), LSH_DEVICE, a
+    raise TypeError(f"unsupported buffer {type(a)}")
+
+
+class HipFrames:
+    """gpu.rs:636-900 GpuFrames: one frame at a time through the device."""
+
+    def __init__(self, w: int, h: int, interp="spline4x4"):
+        self._lib = _lib.load()
+        self.w, self.h = w, h
+        self.interp = Interp.parse(interp)
+        h_ = C.c_void_p()
+        _lib.check(self._lib.lsh_frames_new(w, h, self.interp, C.byref(h_)))
+        self._h = h_
+
+    def _slice_buf(self, measure, slice_out):
+        if measure is None:
+            return None, LSH_HOST, None
+        if slice_out is None:
+            dw, dh = measure.grid(self.w, self.h)
+            slice_out = np.zeros((dh, dw), np.float32)
+        p, mem, keep = _buf(slice_out, np.float32)
+        return p, mem, (keep, slice_out)
+
+    def process(self, rgb, guess=SIM_ID, free=(1, 1, 1, 1, 0, 0, 0, 0), coarsen=0, known=None, brightness=True,
+                measure: Optional[DepthParams] = None, slice_out=None):
+        """Register (against the previous frame), warp, equalize and measure one frame.
+
+        Returns (sim, gains, slice) -- slice is the output buffer (or None)."""
+        rp, rmem, rkeep = _buf(rgb, np.uint16)
+        g = (C.c_double * 8)(*guess)
+        fr = (C.c_uint8 * 8)(*[int(bool(v)) for v in free])
+        ks = kg = None
+        if known is not None:
+            ks = (C.c_double * 8)(*known[0])
+            kg = (C.c_float * 3)(*known[1])
+        sp, smem, skeep = self._slice_buf(measure, slice_out)
+        mp = C.byref(measure._c()) if measure is not None else None
+        sim = (C.c_double * 8)()
+        gains = (C.c_float * 3)()
+        _lib.check(self._lib.lsh_frames_process(self._h, rp, rmem, g, fr, coarsen, ks, kg, int(brightness), mp, sim,
+                                                gains, sp, smem))
+        del rkeep
+        return tuple(sim), tuple(gains), (skeep[1] if skeep else None)
+
+    def ingest(self, rgb, brightness=True, measure: Optional[DepthParams] = None, slice_out=None):
+        """stack.rs:361-419 for registered stacks: decode-equivalent planes, gains to frame 0's means."""
+        rp, rmem, rkeep = _buf(rgb, np.uint16)
+        sp, smem, skeep = self._slice_buf(measure, slice_out)
+        mp = C.byref(measure._c()) if measure is not None else None
+        gains = (C.c_float * 3)()
+        _lib.check(self._lib.lsh_frames_ingest(self._h, rp, rmem, int(brightness), mp, gains, sp, smem))
+        del rkeep
+        return tuple(gains), (skeep[1] if skeep else None)
+
+    @property
+    def last_evals(self) -> int:
+        return int(self._lib.lsh_frames_last_evals(self._h))
+
+    def download(self) -> np.ndarray:
+        out = np.zeros((3, self.h, self.w), np.float32)
+        _lib.check(self._lib.lsh_frames_download(self._h, out.ctypes.data_as(C.c_void_p), LSH_HOST))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lsh_frames_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+class Fuser:
+    """fuse.rs:349-521 Fuser on the device (gpu.rs GpuFuser's contract)."""
+
+    def __init__(self, w: int, h: int, params: Optional[FuseParams] = None, **kw):
+        self._lib = _lib.load()
+        self.params = params or FuseParams(**kw)
+        self.w, self.h = w, h
+        h_ = C.c_void_p()
+        _lib.check(self._lib.lsh_fuser_new(w, h, C.byref(self.params._c()), C.byref(h_)))
+        self._h = h_
+        lv, dl, dw, dh = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _lib.check(self._lib.lsh_fuser_levels(self._h, C.byref(lv), C.byref(dl), C.byref(dw), C.byref(dh)))
+        self.levels, self.depth_level, self.dw, self.dh = lv.value, dl.value, dw.value, dh.value
+        self.count = 0
+
+    def push(self, img3):
+        """Fold one aligned frame (3 planes in [0, 1])."""
+        p, mem, keep = _buf(img3, np.float32)
+        shape = tuple(img3.shape)
+        if shape != (3, self.h, self.w):
+            raise ValueError(f"frame size mismatch: {shape} vs (3, {self.h}, {self.w})")
+        _lib.check(self._lib.lsh_fuser_push(self._h, p, mem))
+        self.count += 1
+        del keep
+
+    def push_frames(self, frames: HipFrames):
+        """gpu.rs:451 push_device: the frame the frames handle holds, without a copy."""
+        _lib.check(self._lib.lsh_fuser_push_frames(self._h, frames._h))
+        self.count += 1
+
+    def shares(self) -> np.ndarray:
+        out = np.zeros(max(self.count, 1), np.float32)
+        n = C.c_uint32()
+        _lib.check(self._lib.lsh_fuser_shares(self._h, out.ctypes.data_as(C.c_void_p), len(out), C.byref(n)))
+        return out[: n.value]
+
+    def finish(self, img_out=None, winner_out=None, want_level=False):
+        """(image (3,h,w), winner map (h,w) f32[, winner at depth_level u16])."""
+        img_out = np.zeros((3, self.h, self.w), np.float32) if img_out is None else img_out
+        winner_out = np.zeros((self.h, self.w), np.float32) if winner_out is None else winner_out
+        ip, imem, ik = _buf(img_out, np.float32)
+        wp, wmem, wk = _buf(winner_out, np.float32)
+        wl = np.zeros((self.dh, self.dw), np.uint16) if want_level else None
+        _lib.check(self._lib.lsh_fuser_finish(self._h, ip, imem, wp, wmem,
+                                              wl.ctypes.data_as(C.c_void_p) if want_level else None))
+        del ik, wk
+        return (img_out, winner_out, wl) if want_level else (img_out, winner_out)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lsh_fuser_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+@dataclasses.dataclass
+class DepthMap:
+    """depth.rs:140-156, plus the WLS report."""
+
+    depth: object
+    conf: object
+    w: int
+    h: int
+    dw: int
+    dh: int
+    floor: object
+    info: dict
+
+
+def depth_from_slices(slices, n: int, fused, params: Optional[DepthParams] = None, depth_out=None, conf_out=None,
+                      floor_out=None) -> DepthMap:
+    """depth.rs:1119-1200 on the device.  ``slices``: (n, dh, dw) contiguous, host or device."""
+    lib = _lib.load()
+    p = params or DepthParams()
+    if n <= 0:
+        raise HipError("depth: no slices")
+    _, h, w = tuple(fused.shape)
+    dw, dh = p.grid(w, h)
+    if tuple(slices.shape) != (n, dh, dw):
+        raise HipError(f"depth: slices have shape {tuple(slices.shape)}, the working grid is {n}x{dh}x{dw}")
+    sp, smem, sk = _buf(slices, np.float32)
+    fp, fmem, fk = _buf(fused, np.float32)
+    if depth_out is None:
+        depth_out = np.zeros((h, w), np.float32)
+        conf_out = np.zeros((h, w), np.float32)
+        floor_out = np.zeros((dh, dw), np.float32)
+    dp, dmem, dk = _buf(depth_out, np.float32)
+    cp, cmem, ck = _buf(conf_out, np.float32)
+    flp = _buf(floor_out, np.float32)[0] if floor_out is not None else None
+    info = _lib.DepthInfoC()
+    _lib.check(lib.lsh_depth_from_slices(sp, smem, n, fp, fmem, w, h, C.byref(p._c()), dp, cp, dmem, flp,
+                                         C.byref(info)))
+    del sk, fk, dk, ck
+    d = {k: getattr(info, k) for k, _ in info._fields_}
+    return DepthMap(depth_out, conf_out, w, h, dw, dh, floor_out, d)
+
+
+def fuse_and_depth(frames: Sequence, fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,
+                   depth: Optional[DepthParams] = None, brightness: bool = True, log=None):
+    """stack.rs:740-853 on the device: the per-frame loop (register against the
+    previous aligned frame, warp, equalize, fold, measure), the collapse, then
+    the depth pass.  ``frames``: a sequence of interleaved RGB u16 frames
+    (h, w, 3), numpy or device tensors.  Returns a dict like stack.rs Output."""
+    fuse = fuse or FuseParams()
+    n = len(frames)
+    if n == 0:
+        raise HipError("no input images")
+    h, w, _ = tuple(frames[0].shape)
+    fr = HipFrames(w, h, align.interp if align else "spline4x4")
+    fu = Fuser(w, h, fuse)
+    sims, gains, slices = [], [], None
+    if depth is not None:
+        dw, dh = depth.grid(w, h)
+        slices = np.zeros((n, dh, dw), np.float32)
+    aligned = align is not None and n > 1
+    last = SIM_ID
+    for i, f in enumerate(frames):
+        so = slices[i] if slices is not None else None
+        if aligned:
+            sim, g, _ = fr.process(f, last, align.free(), align.coarsen, None, brightness, depth, so)
+            last = sim
+        else:
+            g, _ = fr.ingest(f, brightness, depth, so)
+            sim = SIM_ID
+        sims.append(sim)
+        gains.append(g)
+        fu.push_frames(fr)
+        if log:
+            log(f"  frame {i + 1:>3}/{n} folded")
+    shares = fu.shares()
+    image, winner = fu.finish()
+    out = dict(image=image, winner=winner, sims=np.array(sims), gains=np.array(gains), shares=shares,
+               levels=fu.levels, slices=slices)
+    if depth is not None:
+        dm = depth_from_slices(slices, n, image, depth)
+        out.update(depth=dm.depth, conf=dm.conf, floor=dm.floor, info=dm.info)
+    fr.close()
+    fu.close()
+    return out

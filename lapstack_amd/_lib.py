@@ -1,0 +1,133 @@
+"""ctypes binding of lapstack_amd/liblsh.so (declarations: include/lsh.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("LSH_LIB", os.path.join(HERE, "liblsh.so"))
+LSH_HOST, LSH_DEVICE = 0, 1
+ABI_VERSION = 1
+
+_p = C.c_void_p
+_u32 = C.c_uint32
+
+
+class HipError(RuntimeError):
+    """An error string returned by the library."""
+
+
+class FuseParamsC(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("energy_radius", _u32), ("top_rule", _u32), ("top_radius", _u32),
+                ("entropy_bins", _u32), ("use_chroma", _u32), ("depth_level", _u32), ("halo", C.c_float)]
+
+
+class DepthParamsC(C.Structure):
+    _fields_ = [("scale", _u32), ("focus_kind", _u32), ("r_in", _u32), ("r_out", _u32), ("sml_step", _u32),
+                ("agg_radius", _u32), ("agg_eps", C.c_float), ("lambda_", C.c_float), ("sigma_c", C.c_float),
+                ("cg_iters", _u32), ("median", _u32), ("upsample_kind", _u32), ("up_radius", _u32),
+                ("up_eps", C.c_float), ("gate", C.c_float), ("robust", C.c_float), ("tol", C.c_float)]
+
+
+class DepthInfoC(C.Structure):
+    _fields_ = [("dw", _u32), ("dh", _u32), ("iters1", _u32), ("iters2", _u32), ("rel1", C.c_float),
+                ("rel2", C.c_float), ("p90", C.c_float), ("floor", C.c_float)]
+
+
+# (name, restype, argtypes) of every entry point include/lsh.h declares
+SIGNATURES = [
+    ("lsh_abi", _u32, []),
+    ("lsh_available", _p, []),
+    ("lsh_free_str", None, [_p]),
+    ("lsh_stream", _p, [C.POINTER(_p)]),
+    ("lsh_sync", _p, []),
+    ("lsh_prof_enable", _p, [_u32]),
+    ("lsh_prof_read", _p, [_p, C.c_size_t, _p, _p, _p, _u32, C.POINTER(_u32)]),
+    ("lsh_prof_reset", _p, []),
+    ("lsh_frames_new", _p, [_u32, _u32, _u32, C.POINTER(_p)]),
+    ("lsh_frames_process", _p, [_p, _p, C.c_int, _p, _p, _u32, _p, _p, _u32, _p, _p, _p, _p, C.c_int]),
+    ("lsh_frames_ingest", _p, [_p, _p, C.c_int, _u32, _p, _p, _p, C.c_int]),
+    ("lsh_frames_planes", _p, [_p, C.POINTER(_p)]),
+    ("lsh_frames_download", _p, [_p, _p, C.c_int]),
+    ("lsh_frames_last_evals", C.c_uint64, [_p]),
+    ("lsh_frames_free", None, [_p]),
+    ("lsh_fuser_new", _p, [_u32, _u32, C.POINTER(FuseParamsC), C.POINTER(_p)]),
+    ("lsh_fuser_levels", _p, [_p, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    ("lsh_fuser_push", _p, [_p, _p, C.c_int]),
+    ("lsh_fuser_push_frames", _p, [_p, _p]),
+    ("lsh_fuser_shares", _p, [_p, _p, _u32, C.POINTER(_u32)]),
+    ("lsh_fuser_finish", _p, [_p, _p, C.c_int, _p, C.c_int, _p]),
+    ("lsh_fuser_free", None, [_p]),
+    ("lsh_depth_from_slices", _p, [_p, C.c_int, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p,
+                                   C.c_int, _p, C.POINTER(DepthInfoC)]),
+]
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (raises OSError when it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise OSError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = C.CDLL(path)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.lsh_abi() != ABI_VERSION:
+            raise OSError(f"{path}: ABI {lib.lsh_abi()} but this binding speaks {ABI_VERSION}")
+        _lib = lib
+        return lib
+
+
+def take_err(ptr):
+    """The library's error string (released), or None."""
+    if not ptr:
+        return None
+    s = C.cast(ptr, C.c_char_p).value.decode("utf-8", "replace")
+    _lib.lsh_free_str(ptr)
+    return s
+
+
+def check(ptr):
+    msg = take_err(ptr)
+    if msg is not None:
+        raise HipError(msg)
+
+
+def stream_ptr() -> int:
+    s = C.c_void_p()
+    check(load().lsh_stream(C.byref(s)))
+    return s.value or 0
+
+
+def sync():
+    check(load().lsh_sync())
+
+
+def prof_enable(on: bool):
+    check(load().lsh_prof_enable(int(on)))
+
+
+def prof_reset():
+    check(load().lsh_prof_reset())
+
+
+def prof_read():
+    """{kernel: (total ms, total algorithmic bytes, launches)}, timed with HIP events on the library's stream."""
+    lib = load()
+    cap = 256
+    names = C.create_string_buffer(1 << 16)
+    ms = (C.c_double * cap)()
+    by = (C.c_double * cap)()
+    cnt = (C.c_uint64 * cap)()
+    n = C.c_uint32()
+    check(lib.lsh_prof_read(names, len(names), ms, by, cnt, cap, C.byref(n)))
+    keys = names.value.decode().split("\n")[: n.value]
+    return {k: (ms[i], by[i], int(cnt[i])) for i, k in enumerate(keys)}

@@ -1,0 +1,184 @@
+// common.hpp -- runtime plumbing shared by the gfx950 kernels: errors, device
+// buffers, the library's stream, launches with optional event timing, and the
+// arithmetic helpers every kernel must share with the CPU path bit for bit.
+//
+// All sources are compiled with -ffp-contract=off: the CPU path sums taps
+// left to right without fused multiply-adds, and so do these kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lsh.h"
+
+namespace lsh {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define LSH_CHECK(expr)                                                                                    \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) throw ::lsh::Error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+#define LSH_REQUIRE(cond, msg)                 \
+    do {                                       \
+        if (!(cond)) throw ::lsh::Error(msg);  \
+    } while (0)
+
+// ---------------------------------------------------------------- runtime
+hipStream_t stream();          // the library's stream on the current device
+void sync();                   // wait for it
+void *pinned_scratch(size_t);  // a per-device pinned host block (reused; at least the size asked)
+
+// ------------------------------------------------------------- profiling
+struct Prof {
+    bool on = false;
+    struct Rec {
+        const char *name;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    struct Tot {
+        double ms = 0.0, bytes = 0.0;
+        uint64_t n = 0;
+    };
+    std::vector<Rec> pending;
+    std::map<std::string, Tot> totals;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get();
+    void flush();  // synchronizes and folds pending records into totals
+};
+Prof &prof();
+
+// Launch on the library's stream.  `bytes` is the launch's algorithmic HBM
+// traffic (each array it must read or write, once, in its stored format),
+// recorded with the event timing so that achieved GB/s per kernel comes out.
+template <typename K, typename... A>
+inline void launch_b(const char *name, double bytes, K kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    Prof &p = prof();
+    hipStream_t s = stream();
+    if (p.on) {
+        hipEvent_t a = p.get(), b = p.get();
+        LSH_CHECK(hipEventRecord(a, s));
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+        LSH_CHECK(hipGetLastError());
+        LSH_CHECK(hipEventRecord(b, s));
+        p.pending.push_back({name, bytes, a, b});
+        if (p.pending.size() > 4096) p.flush();
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+        LSH_CHECK(hipGetLastError());
+    }
+}
+
+template <typename K, typename... A>
+inline void launch(const char *name, K kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
+    launch_b(name, 0.0, kernel, grid, block, shmem, args...);
+}
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// --------------------------------------------------------------- buffers
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) LSH_CHECK(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void zero() {
+        if (n) LSH_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), stream()));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// copy count elements between any two lsh_mem locations on the library's stream
+void copy_mem(void *dst, lsh_mem dmem, const void *src, lsh_mem smem, size_t bytes);
+
+// ----------------------------------------------------- shared arithmetic
+// Reflect-101 (pyramid.rs:51-66): -1 -> 1, n -> n-2; loops for tiny n.
+__host__ __device__ __forceinline__ int rf101(int i, int n) {
+    if (n == 1) return 0;
+    for (;;) {
+        if (i < 0)
+            i = -i;
+        else if (i >= n)
+            i = 2 * (n - 1) - i;
+        else
+            return i;
+    }
+}
+
+// Rust f32::clamp
+__host__ __device__ __forceinline__ float rclampf(float v, float lo, float hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+
+// f32::total_cmp key
+__host__ __device__ __forceinline__ int32_t total_key(float v) {
+    int32_t b = __builtin_bit_cast(int32_t, v);
+    return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
+}
+
+// A Sim (align.rs:199-253) as 8 doubles; the homography math runs on the host.
+struct Sim {
+    double v[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+    bool is_id() const {
+        static const double id[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+        for (int k = 0; k < 8; k++)
+            if (v[k] != id[k]) return false;
+        return true;
+    }
+};
+void sim_matrix(const Sim &s, size_t w, size_t h, double *m9);  // align.rs:241-252
+void inverse9(const double *m, double *inv);                     // align.rs:258-286
+
+// A 3x3 inverse homography passed by value to kernels.
+struct Inv9 {
+    double m[9];
+};
+
+}  // namespace lsh

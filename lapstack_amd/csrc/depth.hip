@@ -1,0 +1,1098 @@
+// depth.hip -- depth from focus on gfx950 (depth.rs): the focus slice of each
+// frame, guided-filter aggregation fused with the streamed peak tracker, the
+// exact order statistics (noise floor, p90), the 3x3 median, the edge-aware WLS
+// solve (FGS guess, then CG preconditioned by a multigrid V-cycle, Huber
+// reweighting) and the guided upsampling, behind lsh_depth_from_slices.
+//
+// Box filters sum their window in f64 directly (the CPU path keeps an f64
+// running sum; both give the exactly rounded window sum whenever the f64 sums
+// are exact, which they are for these planes), then round to f32 and scale by
+// the f32 reciprocal count, as the CPU path does.
+#include <algorithm>
+
+#include "api.hpp"
+#include "common.hpp"
+#include "engine.hpp"
+
+namespace lsh {
+
+// ---------------------------------------------------------------- focus slice
+struct Taps {
+    int n, halo, kind, step;
+};
+
+// depth.rs:417-495 + :174-197: luma, ring difference filter (or SML), |.|, k x k
+// block mean, one output cell per thread, the luma tile staged in LDS.
+__global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k,
+                                                     const int32_t *__restrict__ tdy, const int32_t *__restrict__ tdx,
+                                                     const float *__restrict__ twt, Taps T, float *__restrict__ out,
+                                                     int dw, int dh) {
+    extern __shared__ float Y[];
+    const int TX = 64, TY = 4, R = T.halo;
+    const int PW = TX * k + 2 * R, PH = TY * k + 2 * R;
+    const int px0 = blockIdx.x * TX * k - R, py0 = blockIdx.y * TY * k - R;
+    const size_t n = (size_t)w * h;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    for (int q = tid; q < PW * PH; q += TX * TY) {
+        int ly = q / PW, lx = q - ly * PW;
+        size_t o = (size_t)rf101(py0 + ly, h) * w + rf101(px0 + lx, w);
+        Y[q] = 0.299f * planes[o] + 0.587f * planes[n + o] + 0.114f * planes[2 * n + o];  // depth.rs:160-165
+    }
+    __syncthreads();
+    int ox = blockIdx.x * TX + threadIdx.x, oy = blockIdx.y * TY + threadIdx.y;
+    if (ox >= dw || oy >= dh) return;
+    int x0 = ox * k, y0 = oy * k, x1 = min(x0 + k, w), y1 = min(y0 + k, h);
+    float o = 0.0f;
+    for (int y = y0; y < y1; y++) {
+        float rs = 0.0f;
+        int ly = y - py0;
+        for (int x = x0; x < x1; x++) {
+            int lx = x - px0;
+            float f;
+            if (T.kind == 0) {
+                float a = 0.0f;
+                for (int t = 0; t < T.n; t++) a += twt[t] * Y[(ly + tdy[t]) * PW + lx + tdx[t]];
+                f = fabsf(a);
+            } else {
+                int s = T.step;
+                float c = Y[ly * PW + lx];
+                f = fabsf(2.0f * c - Y[ly * PW + lx - s] - Y[ly * PW + lx + s]) +
+                    fabsf(2.0f * c - Y[(ly - s) * PW + lx] - Y[(ly + s) * PW + lx]);
+            }
+            rs += f;
+        }
+        o += rs;
+    }
+    out[(size_t)oy * dw + ox] = o / (float)((y1 - y0) * (x1 - x0));
+}
+
+// depth.rs:454-471 with the sparse convolution's order (depth.rs:417-451): rows
+// grouped by dy in order of first appearance, each row's taps in list order.
+static void rdf_taps_ordered(uint32_t r_in, uint32_t r_out, std::vector<int32_t> &dy, std::vector<int32_t> &dx,
+                             std::vector<float> &wt) {
+    long ri2 = (long)r_in * r_in, ro2 = (long)r_out * r_out, r = (long)r_out;
+    std::vector<std::pair<int, int>> disk, ring;
+    for (long y = -r; y <= r; y++)
+        for (long x = -r; x <= r; x++) {
+            long d2 = x * x + y * y;
+            if (d2 <= ri2)
+                disk.push_back({(int)y, (int)x});
+            else if (d2 <= ro2)
+                ring.push_back({(int)y, (int)x});
+        }
+    float wd = 1.0f / (float)disk.size(), wr = -1.0f / (float)std::max<size_t>(ring.size(), 1);
+    std::vector<std::tuple<int, int, float>> taps;
+    for (auto &p : disk) taps.emplace_back(p.first, p.second, wd);
+    for (auto &p : ring) taps.emplace_back(p.first, p.second, wr);
+    std::vector<int> rows;
+    for (auto &t : taps)
+        if (std::find(rows.begin(), rows.end(), std::get<0>(t)) == rows.end()) rows.push_back(std::get<0>(t));
+    dy.clear();
+    dx.clear();
+    wt.clear();
+    for (int ry : rows)
+        for (auto &t : taps)
+            if (std::get<0>(t) == ry) {
+                dy.push_back(std::get<0>(t));
+                dx.push_back(std::get<1>(t));
+                wt.push_back(std::get<2>(t));
+            }
+}
+
+void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out) {
+    LSH_REQUIRE(p.scale <= 2, "focus slice: scale above 2 is not supported");
+    LSH_REQUIRE(p.focus_kind <= 1, "focus slice: unknown focus measure");
+    int k = 1 << p.scale;
+    uint32_t key = p.focus_kind == 0 ? (p.r_in << 8 | p.r_out) : (0x10000u | p.sml_step);
+    if (key != f->taps_key) {
+        std::vector<int32_t> dy, dx;
+        std::vector<float> wt;
+        if (p.focus_kind == 0) {
+            LSH_REQUIRE(p.r_out > p.r_in && p.r_out <= 7, "focus slice: ring radii out of range");
+            rdf_taps_ordered(p.r_in, p.r_out, dy, dx, wt);
+        } else {
+            LSH_REQUIRE(p.sml_step <= 7, "focus slice: SML step out of range");
+            dy = {0};
+            dx = {0};
+            wt = {0.f};
+        }
+        f->taps_dy.alloc(dy.size());
+        f->taps_dx.alloc(dx.size());
+        f->taps_wt.alloc(wt.size());
+        copy_mem(f->taps_dy.p, LSH_DEVICE, dy.data(), LSH_HOST, dy.size() * sizeof(int32_t));
+        copy_mem(f->taps_dx.p, LSH_DEVICE, dx.data(), LSH_HOST, dx.size() * sizeof(int32_t));
+        copy_mem(f->taps_wt.p, LSH_DEVICE, wt.data(), LSH_HOST, wt.size() * sizeof(float));
+        f->ntaps = (uint32_t)wt.size();
+        f->taps_key = key;
+    }
+    Taps T;
+    T.kind = (int)p.focus_kind;
+    T.n = (int)f->ntaps;
+    T.step = (int)std::max<uint32_t>(p.sml_step, 1);
+    T.halo = p.focus_kind == 0 ? (int)p.r_out : T.step;
+    int dw = (int)((f->w + k - 1) / k), dh = (int)((f->h + k - 1) / k);
+    size_t shm = (size_t)(64 * k + 2 * T.halo) * (4 * k + 2 * T.halo) * sizeof(float);
+    launch_b("focus_slice", 12.0 * f->w * f->h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4)),
+             dim3(64, 4), shm, (const float *)planes,
+           (int)f->w, (int)f->h, k, (const int32_t *)f->taps_dy.p, (const int32_t *)f->taps_dx.p,
+           (const float *)f->taps_wt.p, T, out, dw, dh);
+}
+
+// ----------------------------------------------------------------- helpers
+__global__ void k_luma3(const float *__restrict__ planes, size_t n, float *__restrict__ y) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = 0.299f * planes[i] + 0.587f * planes[n + i] + 0.114f * planes[2 * n + i];
+}
+
+// depth.rs:174-197
+__global__ void k_block_mean(const float *__restrict__ src, int w, int h, int k, float *__restrict__ out, int dw,
+                             int dh) {
+    int ox = blockIdx.x * blockDim.x + threadIdx.x, oy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ox >= dw || oy >= dh) return;
+    int x0 = ox * k, y0 = oy * k, x1 = min(x0 + k, w), y1 = min(y0 + k, h);
+    float o = 0.0f;
+    for (int y = y0; y < y1; y++) {
+        float rs = 0.0f;
+        for (int x = x0; x < x1; x++) rs += src[(size_t)y * w + x];
+        o += rs;
+    }
+    out[(size_t)oy * dw + ox] = o / (float)((y1 - y0) * (x1 - x0));
+}
+
+// depth.rs:201-221
+__device__ __forceinline__ float bilinear_at(const float *__restrict__ g, int dw, int dh, int x, int y, float inv) {
+    float fy = rclampf(((float)y + 0.5f) * inv - 0.5f, 0.0f, (float)(dh - 1));
+    int y0 = (int)fy, y1 = min(y0 + 1, dh - 1);
+    float ty = fy - (float)y0;
+    float fx = rclampf(((float)x + 0.5f) * inv - 0.5f, 0.0f, (float)(dw - 1));
+    int x0 = (int)fx, x1 = min(x0 + 1, dw - 1);
+    float tx = fx - (float)x0;
+    const float *r0 = g + (size_t)y0 * dw, *r1 = g + (size_t)y1 * dw;
+    float top = r0[x0] + (r0[x1] - r0[x0]) * tx;
+    float bot = r1[x0] + (r1[x1] - r1[x0]) * tx;
+    return top + (bot - top) * ty;
+}
+
+__global__ void k_up_bilinear(const float *__restrict__ g, int dw, int dh, int w, int h, float inv, float lo, float hi,
+                              int clamp, float *__restrict__ out) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float v = bilinear_at(g, dw, dh, x, y, inv);
+    if (clamp) v = rclampf(v, lo, hi);
+    out[(size_t)y * w + x] = v;
+}
+
+__global__ void k_clamp(float *__restrict__ v, size_t n, float lo, float hi) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = rclampf(v[i], lo, hi);
+}
+
+// depth.rs:1187-1196: a*Y + b with both coefficients bilinear from the grid, clamped
+__global__ void k_up_guided(const float *__restrict__ a, const float *__restrict__ b, int dw, int dh,
+                            const float *__restrict__ yfull, int w, int h, float inv, float maxd,
+                            float *__restrict__ out) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    size_t o = (size_t)y * w + x;
+    float av = bilinear_at(a, dw, dh, x, y, inv), bv = bilinear_at(b, dw, dh, x, y, inv);
+    out[o] = rclampf(av * yfull[o] + bv, 0.0f, maxd);
+}
+
+// -------------------------------------------------------------- box filters
+// Horizontal window sums of two planes in f64, rounded to f32.  src1 is a
+// product plane when mul is set: src1[i] = g[i] * src0[i] (or g[i]*g[i] when
+// src0 is the guide itself).
+__global__ __launch_bounds__(256) void k_box_h2(const float *__restrict__ a, const float *__restrict__ b,
+                                                const float *__restrict__ g, int mode, int w, int h, int r,
+                                                float *__restrict__ oa, float *__restrict__ ob) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const size_t row = (size_t)y * w;
+    int lo = max(x - r, 0), hi = min(x + r, w - 1);
+    double sa = 0.0, sb = 0.0;
+    for (int i = lo; i <= hi; i++) {
+        float va = a[row + i];
+        float vb = mode == 0 ? b[row + i] : (mode == 1 ? g[row + i] * va : va * va);
+        sa += (double)va;
+        sb += (double)vb;
+    }
+    oa[row + x] = (float)sa;
+    ob[row + x] = (float)sb;
+}
+
+// reciprocal of the border-clipped window count (depth.rs:273-280)
+__device__ __forceinline__ float inv_count(int x, int y, int w, int h, int r) {
+    float cy = (float)(min(y + r + 1, h) - max(y - r, 0));
+    float cx = (float)(min(x + r + 1, w) - max(x - r, 0));
+    return 1.0f / (cy * cx);
+}
+
+// The per-cell peak tracker state (depth.rs:499-531).
+struct Peaks {
+    float *c1, *l1, *r1, *c2, *prev, *prev2, *sum, *cmin;
+    uint32_t *i1;
+};
+
+__device__ __forceinline__ void px_register(const Peaks &P, size_t i, float val, uint32_t idx, float l, float r,
+                                            float &c1, float &c2) {
+    if (val > c1) {
+        if (c1 >= 0.0f) c2 = c1;
+        c1 = val;
+        P.i1[i] = idx;
+        P.l1[i] = l;
+        P.r1[i] = r;
+    } else if (val > c2) {
+        c2 = val;
+    }
+}
+
+// depth.rs:547-564 for one cell and value v (slice m)
+__device__ __forceinline__ void peak_push(const Peaks &P, size_t i, float v, uint32_t m) {
+    float prev = P.prev[i], prev2 = P.prev2[i];
+    if (m >= 1) {
+        bool is_peak = (m == 1 || prev >= prev2) && prev > v;
+        if (is_peak) {
+            float c1 = P.c1[i], c2 = P.c2[i];
+            float l = m >= 2 ? prev2 : -1.0f;
+            px_register(P, i, prev, m - 1, l, v, c1, c2);
+            P.c1[i] = c1;
+            P.c2[i] = c2;
+        }
+    }
+    P.sum[i] += v;
+    P.cmin[i] = fminf(P.cmin[i], v);
+    P.prev2[i] = prev;
+    P.prev[i] = v;
+}
+
+// Vertical window sums of two planes, then per mode:
+//  0  guide statistics: oa = mean(I), ob = var(I) = max(mean(I^2) - mean(I)^2, 0)
+//  1  guided filter step 1 (depth.rs:382-389): oa = a, ob = b from mean(p), mean(I p)
+//  2  guided filter step 2 fused with the peak push: q = mean(a) I + mean(b), max(q, 0), push
+//  3  guided coefficients: oa = mean(a), ob = mean(b)
+//  4  guided filter output: oa = mean(a) I + mean(b)
+__global__ __launch_bounds__(256) void k_box_v2(const float *__restrict__ ta, const float *__restrict__ tb, int w,
+                                                int h, int r, int mode, const float *__restrict__ guide,
+                                                const float *__restrict__ mean_i, const float *__restrict__ var_i,
+                                                float eps, float *__restrict__ oa, float *__restrict__ ob, Peaks P,
+                                                uint32_t m) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    int lo = max(y - r, 0), hi = min(y + r, h - 1);
+    double sa = 0.0, sb = 0.0;
+    for (int i = lo; i <= hi; i++) {
+        sa += (double)ta[(size_t)i * w + x];
+        sb += (double)tb[(size_t)i * w + x];
+    }
+    float ic = inv_count(x, y, w, h, r);
+    float ma = (float)sa * ic, mb = (float)sb * ic;
+    size_t o = (size_t)y * w + x;
+    if (mode == 0) {
+        oa[o] = ma;
+        ob[o] = fmaxf(mb - ma * ma, 0.0f);
+    } else if (mode == 1) {
+        float cov = mb - mean_i[o] * ma;
+        float a = cov / (var_i[o] + eps);
+        oa[o] = a;
+        ob[o] = ma - a * mean_i[o];
+    } else if (mode == 2) {
+        float q = ma * guide[o] + mb;
+        peak_push(P, o, fmaxf(q, 0.0f), m);
+    } else if (mode == 3) {
+        oa[o] = ma;
+        ob[o] = mb;
+    } else {
+        oa[o] = ma * guide[o] + mb;
+    }
+}
+
+__global__ void k_peak_push_raw(const float *__restrict__ c, size_t n, Peaks P, uint32_t m) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) peak_push(P, i, c[i], m);
+}
+
+__global__ void k_peak_init(Peaks P, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    P.c1[i] = P.l1[i] = P.r1[i] = P.c2[i] = -1.0f;
+    P.prev[i] = P.prev2[i] = P.sum[i] = 0.0f;
+    P.cmin[i] = INFINITY;
+    P.i1[i] = 0;
+}
+
+// depth.rs:571-576
+__global__ void k_peak_close(Peaks P, size_t n, uint32_t ns) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float prev = P.prev[i], prev2 = P.prev2[i];
+    if (ns == 1 || prev >= prev2) {
+        float c1 = P.c1[i], c2 = P.c2[i];
+        float l = ns >= 2 ? prev2 : -1.0f;
+        px_register(P, i, prev, ns - 1, l, -1.0f, c1, c2);
+        P.c1[i] = c1;
+        P.c2[i] = c2;
+    }
+}
+
+// depth.rs:581-609
+__global__ void k_peak_out(Peaks P, size_t n, uint32_t ns, float gate, const float *__restrict__ floor_p,
+                           float *__restrict__ depth, float *__restrict__ conf, float *__restrict__ mins) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float floor_ = *floor_p;
+    float inv_n = 1.0f / (float)ns;
+    float c1 = P.c1[i], l = P.l1[i], r = P.r1[i];
+    float delta = 0.0f;
+    if (l >= 0.0f && r >= 0.0f && c1 > 0.0f) {
+        float ll = logf(fmaxf(l, 1e-12f)), lc = logf(c1), lr = logf(fmaxf(r, 1e-12f));
+        float den = ll - 2.0f * lc + lr;
+        if (den < 0.0f) delta = rclampf(0.5f * (ll - lr) / den, -0.5f, 0.5f);
+    }
+    depth[i] = (float)P.i1[i] + delta;
+    float cv = 0.0f;
+    if (c1 > 0.0f) {
+        float mean = P.sum[i] * inv_n;
+        float prom = rclampf(1.0f - mean / c1, 0.0f, 1.0f);
+        float c2 = P.c2[i];
+        float pkr = c2 >= 0.0f ? rclampf(1.0f - c2 / c1, 0.0f, 1.0f) : 1.0f;
+        float g = (gate > 0.0f && floor_ > 0.0f) ? rclampf((c1 - floor_) / (gate * floor_), 0.0f, 1.0f) : 1.0f;
+        cv = prom * pkr * g;
+    }
+    conf[i] = cv;
+    if (mins) {
+        float cm = P.cmin[i];
+        mins[i] = isfinite(cm) ? cm : 0.0f;
+    }
+}
+
+// ------------------------------------------------- exact order statistics
+// The k-th smallest (f32::total_cmp order) of every 7th element, by a
+// three-pass radix select over the order-preserving 32-bit key.
+__device__ __forceinline__ uint32_t okey(float v) {
+    uint32_t b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_radix_hist(const float *__restrict__ v, size_t n, size_t stride, uint32_t prefix, int pbits,
+                             int shift, int bits, unsigned *__restrict__ hist) {
+    __shared__ unsigned sh[4096];
+    int nb = 1 << bits;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    size_t ns = (n + stride - 1) / stride;
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += (size_t)gridDim.x * blockDim.x) {
+        uint32_t key = okey(v[s * stride]);
+        if (pbits > 0 && (key >> (32 - pbits)) != prefix) continue;
+        atomicAdd(&sh[(key >> shift) & (nb - 1)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+static float select_kth(const float *d, size_t n, size_t stride, size_t k, DBuf<unsigned> &hist) {
+    size_t ns = (n + stride - 1) / stride;
+    LSH_REQUIRE(k < ns, "select: rank out of range");
+    hist.ensure(4096);
+    std::vector<unsigned> hh(4096);
+    uint32_t prefix = 0;
+    int pbits = 0;
+    const int plan[3][2] = {{20, 12}, {8, 12}, {0, 8}};  // (shift, bits)
+    unsigned grid = std::max(1u, std::min<unsigned>(cdiv(ns, 256), 512));
+    for (auto &pl : plan) {
+        int shift = pl[0], bits = pl[1], nb = 1 << bits;
+        LSH_CHECK(hipMemsetAsync(hist.p, 0, nb * sizeof(unsigned), stream()));
+        launch("radix_select", k_radix_hist, dim3(grid), dim3(256), 0, d, n, stride, prefix, pbits, shift, bits, hist.p);
+        copy_mem(hh.data(), LSH_HOST, hist.p, LSH_DEVICE, nb * sizeof(unsigned));
+        size_t acc = 0;
+        int bin = 0;
+        for (; bin < nb; bin++) {
+            if (acc + hh[bin] > k) break;
+            acc += hh[bin];
+        }
+        LSH_REQUIRE(bin < nb, "select: histogram does not cover the rank");
+        k -= acc;
+        prefix = (prefix << bits) | (uint32_t)bin;
+        pbits += bits;
+    }
+    uint32_t key = prefix;
+    uint32_t b = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+    float out;
+    std::memcpy(&out, &b, 4);
+    return out;
+}
+
+// depth.rs:628-650: 3x3 median, reflect-101
+__global__ void k_median3(const float *__restrict__ src, int w, int h, float *__restrict__ out) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float v[9];
+    int k = 0;
+    for (int dy = -1; dy <= 1; dy++) {
+        const float *row = src + (size_t)rf101(y + dy, h) * w;
+        for (int dx = -1; dx <= 1; dx++) v[k++] = row[rf101(x + dx, w)];
+    }
+    for (int i = 1; i < 9; i++) {  // insertion sort by total order
+        float t = v[i];
+        int j = i;
+        while (j > 0 && total_key(v[j - 1]) > total_key(t)) {
+            v[j] = v[j - 1];
+            j--;
+        }
+        v[j] = t;
+    }
+    out[(size_t)y * w + x] = v[4];
+}
+
+// depth.rs:619-621
+__global__ void k_conf_norm(float *__restrict__ conf, size_t n, float p90) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) conf[i] = fminf(conf[i] / p90, 1.0f);
+}
+
+// ---------------------------------------------------------------- WLS
+#define MG_OMEGA 0.8f
+
+// depth.rs:653-672 (scaled by lambda into ax/ay as well, depth.rs:980-981)
+__global__ void k_edges(const float *__restrict__ g, int w, int h, float inv, float lambda, float *__restrict__ ex,
+                        float *__restrict__ ey, float *__restrict__ ax, float *__restrict__ ay) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    size_t o = (size_t)y * w + x;
+    float e1 = x + 1 < w ? expf(fabsf(g[o] - g[o + 1]) * inv) : 0.0f;
+    float e2 = y + 1 < h ? expf(fabsf(g[o] - g[o + w]) * inv) : 0.0f;
+    ex[o] = e1;
+    ey[o] = e2;
+    ax[o] = lambda * e1;
+    ay[o] = lambda * e2;
+}
+
+// depth.rs:852-898: 2x2 Galerkin aggregation.  what: 1 data weights, 2 edges, 3 both.
+__global__ void k_coarsen(const float *__restrict__ wd, const float *__restrict__ ax, const float *__restrict__ ay,
+                          int w, int h, float *__restrict__ cwd, float *__restrict__ cax, float *__restrict__ cay,
+                          int cw, int ch, int what) {
+    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (xx >= cw || yy >= ch) return;
+    size_t co = (size_t)yy * cw + xx;
+    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
+    if (what & 1) {
+        float o = 0.0f;
+        for (int y = 2 * yy; y < ye; y++) {
+            float s = 0.0f;
+            for (int x = 2 * xx; x < xe; x++) s += wd[(size_t)y * w + x];
+            o += s;
+        }
+        cwd[co] = o;
+    }
+    if (what & 2) {
+        float o = 0.0f;
+        if (xx + 1 < cw)
+            for (int y = 2 * yy; y < ye; y++) o += ax[(size_t)y * w + 2 * xx + 1];
+        cax[co] = o;
+        float s = 0.0f;
+        if (yy + 1 < ch) {
+            int y = 2 * yy + 1;
+            for (int x = 2 * xx; x < xe; x++) s += ay[(size_t)y * w + x];
+        }
+        cay[co] = s;
+    }
+}
+
+// depth.rs:812-834
+__global__ void k_dinv(const float *__restrict__ wd, const float *__restrict__ ax, const float *__restrict__ ay, int w,
+                       int h, float *__restrict__ dinv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h) return;
+    size_t k = (size_t)y * w + i;
+    float v = wd[k];
+    if (i > 0) v += ax[k - 1];
+    if (i + 1 < w) v += ax[k];
+    if (y > 0) v += ay[k - w];
+    if (y + 1 < h) v += ay[k];
+    dinv[k] = 1.0f / fmaxf(v, 1e-12f);
+}
+
+// depth.rs:994: W = max(conf, 0) + 1e-4; with robust: conf * min(tau / |d - u|, 1) first (depth.rs:1168)
+__global__ void k_data_w(const float *__restrict__ conf, const float *__restrict__ d, const float *__restrict__ u,
+                         float tau, size_t n, float *__restrict__ wd) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float c = conf[i];
+    if (u) c = c * fminf(tau / fabsf(d[i] - u[i]), 1.0f);
+    wd[i] = fmaxf(c, 0.0f) + 1e-4f;
+}
+
+__device__ __forceinline__ float apply_at(const float *__restrict__ wd, const float *__restrict__ ax,
+                                          const float *__restrict__ ay, const float *__restrict__ x, int w, int h,
+                                          int i, int y, size_t k) {
+    float xk = x[k];
+    float v = wd[k] * xk;
+    if (i > 0) v += ax[k - 1] * (xk - x[k - 1]);
+    if (i + 1 < w) v += ax[k] * (xk - x[k + 1]);
+    if (y > 0) v += ay[k - w] * (xk - x[k - w]);
+    if (y + 1 < h) v += ay[k] * (xk - x[k + w]);
+    return v;
+}
+
+// depth.rs:786-810 apply; mode 0: A x; 1: b - A x; 2: x + w D^-1 (b - A x) (depth.rs:837-847)
+__global__ __launch_bounds__(256) void k_stencil(const float *__restrict__ wd, const float *__restrict__ ax,
+                                                 const float *__restrict__ ay, const float *__restrict__ dinv,
+                                                 const float *__restrict__ x, const float *__restrict__ b, int w,
+                                                 int h, int mode, float *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h) return;
+    size_t k = (size_t)y * w + i;
+    float v = apply_at(wd, ax, ay, x, w, h, i, y, k);
+    if (mode == 0)
+        out[k] = v;
+    else if (mode == 1)
+        out[k] = b[k] - v;
+    else
+        out[k] = x[k] + MG_OMEGA * dinv[k] * (b[k] - v);
+}
+
+__global__ void k_jacobi0(const float *__restrict__ dinv, const float *__restrict__ b, size_t n, float *__restrict__ x) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = MG_OMEGA * dinv[i] * b[i];
+}
+
+// depth.rs:911-921
+__global__ void k_restrict(const float *__restrict__ r, int w, int h, float *__restrict__ bc, int cw, int ch) {
+    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (xx >= cw || yy >= ch) return;
+    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
+    float o = 0.0f;
+    for (int y = 2 * yy; y < ye; y++) {
+        float s = 0.0f;
+        for (int x = 2 * xx; x < xe; x++) s += r[(size_t)y * w + x];
+        o += s;
+    }
+    bc[(size_t)yy * cw + xx] = o;
+}
+
+// depth.rs:901-908
+__global__ void k_prolong(float *__restrict__ x, int w, int h, const float *__restrict__ xc, int cw) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h) return;
+    x[(size_t)y * w + i] += xc[(size_t)(y / 2) * cw + i / 2];
+}
+
+// depth.rs:677-702: Thomas solve of every row, one row per thread
+__global__ void k_solve_rows(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
+                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
+                             float *__restrict__ dp) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h) return;
+    size_t o = (size_t)y * w;
+    float prev_a = 0.0f, cprev = 0.0f, dprev = 0.0f;
+    for (int i = 0; i < w; i++) {
+        float ai = i + 1 < w ? a[o + i] : 0.0f;
+        float wdi = wd ? wd[o + i] : 1.0f;
+        float diag = wdi + lam * (prev_a + ai);
+        float lower = -lam * prev_a;
+        float upper = -lam * ai;
+        float m = diag - lower * cprev;
+        if (fabsf(m) < 1e-12f) m = 1e-12f;
+        cprev = upper / m;
+        dprev = (wdi * f[o + i] - lower * dprev) / m;
+        cp[o + i] = cprev;
+        dp[o + i] = dprev;
+        prev_a = ai;
+    }
+    float nx = dp[o + w - 1];
+    u[o + w - 1] = nx;
+    for (int i = w - 2; i >= 0; i--) {
+        nx = dp[o + i] - cp[o + i] * nx;
+        u[o + i] = nx;
+    }
+}
+
+// depth.rs:707-746: the same down every column, one column per thread
+__global__ void k_solve_cols(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
+                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
+                             float *__restrict__ dp) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w) return;
+    float cprev = 0.0f, dprev = 0.0f;
+    for (int y = 0; y < h; y++) {
+        size_t i = (size_t)y * w + x;
+        float prev_a = y > 0 ? a[i - w] : 0.0f;
+        float ai = y + 1 < h ? a[i] : 0.0f;
+        float wdi = wd ? wd[i] : 1.0f;
+        float diag = wdi + lam * (prev_a + ai);
+        float lower = -lam * prev_a;
+        float upper = -lam * ai;
+        float m = diag - lower * cprev;
+        if (fabsf(m) < 1e-12f) m = 1e-12f;
+        cprev = upper / m;
+        dprev = (wdi * f[i] - lower * dprev) / m;
+        cp[i] = cprev;
+        dp[i] = dprev;
+    }
+    float nx = dp[(size_t)(h - 1) * w + x];
+    u[(size_t)(h - 1) * w + x] = nx;
+    for (int y = h - 2; y >= 0; y--) {
+        size_t i = (size_t)y * w + x;
+        nx = dp[i] - cp[i] * nx;
+        u[i] = nx;
+    }
+}
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// fixed-order grid reduction of one double per thread (per-block partials, last block folds)
+__device__ void grid_sum1(double v, double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ out) {
+    __shared__ double sh[4];
+    __shared__ bool last;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double s = wsum(v);
+    if (lane == 0) sh[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += part[b];
+    __syncthreads();
+    s = wsum(acc);
+    if (lane == 0) sh[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *out = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        *ticket = 0;
+    }
+}
+
+// sum a*b in f64 (depth.rs:1037)
+__global__ __launch_bounds__(256) void k_dot(const float *__restrict__ a, const float *__restrict__ b, size_t n,
+                                             double *__restrict__ part, unsigned *__restrict__ ticket,
+                                             double *__restrict__ out) {
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    grid_sum1(s, part, ticket, out);
+}
+
+// ap = A p and sum p*ap
+__global__ __launch_bounds__(256) void k_spmv_dot(const float *__restrict__ wd, const float *__restrict__ ax,
+                                                  const float *__restrict__ ay, const float *__restrict__ p, int w,
+                                                  int h, float *__restrict__ ap, double *__restrict__ part,
+                                                  unsigned *__restrict__ ticket, double *__restrict__ out) {
+    double s = 0.0;
+    size_t n = (size_t)w * h;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        int y = (int)(k / w), i = (int)(k - (size_t)y * w);
+        float v = apply_at(wd, ax, ay, p, w, h, i, y, k);
+        ap[k] = v;
+        s += (double)p[k] * (double)v;
+    }
+    grid_sum1(s, part, ticket, out);
+}
+
+// u += alpha p; r -= alpha ap; sum r*r (depth.rs:1064-1066)
+__global__ __launch_bounds__(256) void k_cg_update(float *__restrict__ u, float *__restrict__ r,
+                                                   const float *__restrict__ p, const float *__restrict__ ap,
+                                                   float alpha, size_t n, double *__restrict__ part,
+                                                   unsigned *__restrict__ ticket, double *__restrict__ out) {
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        u[i] += alpha * p[i];
+        float ri = r[i] - alpha * ap[i];
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    grid_sum1(s, part, ticket, out);
+}
+
+// b = wd * d (depth.rs:1039)
+__global__ void k_mul(const float *__restrict__ a, const float *__restrict__ b, size_t n, float *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] * b[i];
+}
+
+// p = z + beta p (depth.rs:1074)
+__global__ void k_pupdate(float *__restrict__ p, const float *__restrict__ z, float beta, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+struct MgLev {
+    int w, h;
+    DBuf<float> wd, ax, ay, dinv, x, b, r;
+};
+
+struct Wls {
+    int w, h;
+    float lambda;
+    DBuf<float> ex, ey, r, z, p, ap, f, cp, dp;
+    std::vector<MgLev> L;
+    DBuf<double> part, out;
+    DBuf<unsigned> ticket;
+};
+
+static dim3 g2(int w, int h) { return dim3(cdiv(w, 64), cdiv(h, 4)); }
+static const dim3 B2(64, 4);
+static unsigned g1(size_t n) { return cdiv(n, 256); }
+static unsigned gred(size_t n) { return std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024)); }
+
+static double read_scalar(Wls &s) {
+    double *pin = (double *)pinned_scratch(64);
+    LSH_CHECK(hipMemcpyAsync(pin, s.out.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    return pin[0];
+}
+
+// depth.rs:976-988
+static void wls_new(Wls &s, const float *guide, int w, int h, float lambda, float sigma_c) {
+    s.w = w;
+    s.h = h;
+    s.lambda = lambda;
+    size_t n = (size_t)w * h;
+    for (auto *b : {&s.ex, &s.ey, &s.r, &s.z, &s.p, &s.ap, &s.f, &s.cp, &s.dp}) b->alloc(n);
+    s.part.alloc(1024);
+    s.out.alloc(1);
+    s.ticket.alloc(1);
+    s.ticket.zero();
+    s.L.clear();
+    s.L.emplace_back();
+    MgLev &t = s.L[0];
+    t.w = w;
+    t.h = h;
+    for (auto *b : {&t.wd, &t.ax, &t.ay, &t.dinv, &t.x, &t.b, &t.r}) b->alloc(n);
+    float inv = -1.0f / fmaxf(sigma_c, 1e-6f);
+    launch("wls_edges", k_edges, g2(w, h), B2, 0, guide, w, h, inv, lambda, s.ex.p, s.ey.p, t.ax.p, t.ay.p);
+    while (std::max(s.L.back().w, s.L.back().h) > 16) {  // MG_MIN (depth.rs:777)
+        const MgLev &fl = s.L.back();
+        MgLev c;
+        c.w = (fl.w + 1) / 2;
+        c.h = (fl.h + 1) / 2;
+        size_t cn = (size_t)c.w * c.h;
+        for (auto *b : {&c.wd, &c.ax, &c.ay, &c.dinv, &c.x, &c.b, &c.r}) b->alloc(cn);
+        launch("wls_coarsen", k_coarsen, g2(c.w, c.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
+               (const float *)fl.ay.p, fl.w, fl.h, c.wd.p, c.ax.p, c.ay.p, c.w, c.h, 2);
+        s.L.push_back(std::move(c));
+    }
+}
+
+// depth.rs:992-1002 (with the optional robust reweighting of depth.rs:1168)
+static void wls_set_weights(Wls &s, const float *conf, const float *d, const float *u, float tau) {
+    size_t n = (size_t)s.w * s.h;
+    launch("wls_weights", k_data_w, dim3(g1(n)), dim3(256), 0, conf, d, u, tau, n, s.L[0].wd.p);
+    for (size_t l = 0; l < s.L.size(); l++) {
+        MgLev &lv = s.L[l];
+        if (l > 0) {
+            MgLev &fl = s.L[l - 1];
+            launch("wls_coarsen", k_coarsen, g2(lv.w, lv.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
+                   (const float *)fl.ay.p, fl.w, fl.h, lv.wd.p, lv.ax.p, lv.ay.p, lv.w, lv.h, 1);
+        }
+        launch("wls_dinv", k_dinv, g2(lv.w, lv.h), B2, 0, (const float *)lv.wd.p, (const float *)lv.ax.p,
+               (const float *)lv.ay.p, lv.w, lv.h, lv.dinv.p);
+    }
+}
+
+static void jacobi(MgLev &t, bool zero) {
+    size_t n = (size_t)t.w * t.h;
+    if (zero) {
+        launch_b("mg_jacobi", 12.0 * n, k_jacobi0, dim3(g1(n)), dim3(256), 0, (const float *)t.dinv.p,
+                 (const float *)t.b.p, n, t.x.p);
+        return;
+    }
+    launch_b("mg_jacobi", 28.0 * n, k_stencil, g2(t.w, t.h), B2, 0, (const float *)t.wd.p, (const float *)t.ax.p,
+             (const float *)t.ay.p, (const float *)t.dinv.p, (const float *)t.x.p, (const float *)t.b.p, t.w, t.h, 2,
+             t.r.p);
+    std::swap(t.x, t.r);
+}
+
+// depth.rs:924-947
+static void vcycle(Wls &s, size_t l) {
+    MgLev &t = s.L[l];
+    if (l + 1 == s.L.size()) {
+        for (int k = 0; k < 32; k++) jacobi(t, k == 0);  // MG_COARSE
+        return;
+    }
+    jacobi(t, true);
+    double n = (double)t.w * t.h;
+    launch_b("mg_residual", 24.0 * n, k_stencil, g2(t.w, t.h), B2, 0, (const float *)t.wd.p, (const float *)t.ax.p,
+             (const float *)t.ay.p, (const float *)t.dinv.p, (const float *)t.x.p, (const float *)t.b.p, t.w, t.h, 1,
+             t.r.p);
+    MgLev &c = s.L[l + 1];
+    launch_b("mg_restrict", 4.0 * n + 4.0 * c.w * c.h, k_restrict, g2(c.w, c.h), B2, 0, (const float *)t.r.p, t.w, t.h,
+             c.b.p, c.w, c.h);
+    vcycle(s, l + 1);
+    launch_b("mg_prolong", 8.0 * n + 4.0 * c.w * c.h, k_prolong, g2(t.w, t.h), B2, 0, t.x.p, t.w, t.h,
+             (const float *)c.x.p, c.w);
+    jacobi(t, false);
+}
+
+static void precond(Wls &s, const float *r, float *z) {
+    size_t n = (size_t)s.w * s.h;
+    LSH_CHECK(hipMemcpyAsync(s.L[0].b.p, r, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    vcycle(s, 0);
+    LSH_CHECK(hipMemcpyAsync(z, s.L[0].x.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+}
+
+static double dot(Wls &s, const float *a, const float *b) {
+    size_t n = (size_t)s.w * s.h;
+    launch_b("cg_dot", 8.0 * n, k_dot, dim3(gred(n)), dim3(256), 0, a, b, n, s.part.p, s.ticket.p, s.out.p);
+    return read_scalar(s);
+}
+
+// depth.rs:1006-1077
+static uint32_t wls_solve(Wls &s, const float *d, uint32_t max_iters, float tol, float *u, float *rel_out) {
+    int w = s.w, h = s.h;
+    size_t n = (size_t)w * h;
+    const float *wd = s.L[0].wd.p;
+    if (w == 1 || h == 1) {
+        if (h == 1)
+            launch("fgs_rows", k_solve_rows, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ex.p, w, h, s.lambda, s.cp.p, s.dp.p);
+        else
+            launch("fgs_cols", k_solve_cols, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ey.p, w, h, s.lambda, s.cp.p, s.dp.p);
+        if (rel_out) *rel_out = 0.0f;
+        return 0;
+    }
+    const int T = 3;
+    LSH_CHECK(hipMemcpyAsync(s.f.p, d, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    for (int t = 1; t <= T; t++) {
+        float lam_t = 1.5f * s.lambda * std::pow(4.0f, (float)(T - t)) / (std::pow(4.0f, (float)T) - 1.0f);
+        launch("fgs_rows", k_solve_rows, dim3(cdiv(h, 64)), dim3(64), 0, u, (const float *)s.f.p, t == 1 ? wd : nullptr,
+               (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
+        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+        launch("fgs_cols", k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, u, (const float *)s.f.p, nullptr,
+               (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
+        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    }
+    float *b = s.f.p;
+    launch("wls_rhs", k_mul, dim3(g1(n)), dim3(256), 0, wd, d, n, b);
+    double bnorm = std::sqrt(dot(s, b, b));
+    if (bnorm < 1e-30) bnorm = 1e-30;
+    MgLev &t0 = s.L[0];
+    launch("cg_residual", k_stencil, g2(w, h), B2, 0, (const float *)t0.wd.p, (const float *)t0.ax.p,
+           (const float *)t0.ay.p, (const float *)t0.dinv.p, (const float *)u, (const float *)b, w, h, 1, s.r.p);
+    precond(s, s.r.p, s.z.p);
+    LSH_CHECK(hipMemcpyAsync(s.p.p, s.z.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    double rz = dot(s, s.r.p, s.z.p);
+    float rel = (float)(std::sqrt(dot(s, s.r.p, s.r.p)) / bnorm);
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < max_iters; it++) {
+        if (rel < tol) break;
+        iters++;
+        launch_b("cg_spmv", 20.0 * n, k_spmv_dot, dim3(gred(n)), dim3(256), 0, (const float *)t0.wd.p,
+                 (const float *)t0.ax.p, (const float *)t0.ay.p, (const float *)s.p.p, w, h, s.ap.p, s.part.p,
+                 s.ticket.p, s.out.p);
+        double pap = read_scalar(s);
+        if (pap <= 0.0) break;
+        float alpha = (float)(rz / pap);
+        launch_b("cg_update", 24.0 * n, k_cg_update, dim3(gred(n)), dim3(256), 0, u, s.r.p, (const float *)s.p.p,
+                 (const float *)s.ap.p, alpha, n, s.part.p, s.ticket.p, s.out.p);
+        rel = (float)(std::sqrt(read_scalar(s)) / bnorm);
+        if (rel < tol) break;
+        precond(s, s.r.p, s.z.p);
+        double rz_new = dot(s, s.r.p, s.z.p);
+        float beta = (float)(rz_new / rz);
+        rz = rz_new;
+        launch("cg_pupdate", k_pupdate, dim3(g1(n)), dim3(256), 0, s.p.p, (const float *)s.z.p, beta, n);
+    }
+    if (rel_out) *rel_out = rel;
+    return iters;
+}
+
+// Guided-filter plumbing on the working grid (depth.rs:349-412)
+struct GF {
+    int w, h, r;
+    float eps;
+    const float *guide;
+    DBuf<float> mean_i, var_i, t1, t2, s1, s2;
+};
+
+static void gf_init(GF &g, const float *guide, int w, int h, int r, float eps) {
+    g.w = w;
+    g.h = h;
+    g.r = r;
+    g.eps = eps;
+    g.guide = guide;
+    size_t n = (size_t)w * h;
+    for (auto *b : {&g.mean_i, &g.var_i, &g.t1, &g.t2, &g.s1, &g.s2}) b->alloc(n);
+    Peaks none{};
+    launch("box_h", k_box_h2, g2(w, h), B2, 0, guide, (const float *)nullptr, (const float *)nullptr, 2, w, h, r, g.t1.p,
+           g.t2.p);
+    launch("box_v", k_box_v2, g2(w, h), B2, 0, (const float *)g.t1.p, (const float *)g.t2.p, w, h, r, 0,
+           (const float *)nullptr, (const float *)nullptr, (const float *)nullptr, 0.0f, g.mean_i.p, g.var_i.p, none, 0u);
+}
+
+// step 1 of a filtering of p: s1 = a, s2 = b
+static void gf_step1(GF &g, const float *p) {
+    Peaks none{};
+    double M = (double)g.w * g.h;
+    launch_b("box_h", 16.0 * M, k_box_h2, g2(g.w, g.h), B2, 0, p, (const float *)nullptr, g.guide, 1, g.w, g.h, g.r,
+             g.t1.p, g.t2.p);
+    launch_b("box_v", 24.0 * M, k_box_v2, g2(g.w, g.h), B2, 0, (const float *)g.t1.p, (const float *)g.t2.p, g.w, g.h,
+             g.r, 1, g.guide, (const float *)g.mean_i.p, (const float *)g.var_i.p, g.eps, g.s1.p, g.s2.p, none, 0u);
+}
+
+// step 2: the box means of a and b, then per mode (2 peak push, 3 coefficients into oa/ob, 4 filtered into oa)
+static void gf_step2(GF &g, int mode, float *oa, float *ob, const Peaks &P, uint32_t m) {
+    double M = (double)g.w * g.h;
+    launch_b("box_h", 16.0 * M, k_box_h2, g2(g.w, g.h), B2, 0, (const float *)g.s1.p, (const float *)g.s2.p,
+             (const float *)nullptr, 0, g.w, g.h, g.r, g.t1.p, g.t2.p);
+    // mode 2: t1, t2 and the guide read, the nine-plane peak state read and written
+    double by = mode == 2 ? (12.0 + 72.0) * M : 16.0 * M;
+    launch_b(mode == 2 ? "box_v_peaks" : "box_v", by, k_box_v2, g2(g.w, g.h), B2, 0, (const float *)g.t1.p,
+             (const float *)g.t2.p, g.w, g.h, g.r, mode, g.guide, (const float *)nullptr, (const float *)nullptr, 0.0f,
+             oa, ob, P, m);
+}
+
+struct PeakBufs {
+    DBuf<float> c1, l1, r1, c2, prev, prev2, sum, cmin;
+    DBuf<uint32_t> i1;
+    Peaks view() { return Peaks{c1.p, l1.p, r1.p, c2.p, prev.p, prev2.p, sum.p, cmin.p, i1.p}; }
+};
+
+}  // namespace lsh
+
+using namespace lsh;
+
+extern "C" {
+
+char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
+                            lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
+                            float *conf_out, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(slices && fused3 && p && depth && conf_out, "lsh_depth_from_slices: null argument");
+    LSH_REQUIRE(n > 0, "depth: no slices");
+    LSH_REQUIRE(w > 0 && h > 0, "depth: empty frame");
+    LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
+    const int k = 1 << p->scale;
+    const int dw = (int)((w + k - 1) / k), dh = (int)((h + k - 1) / k);
+    const size_t N = (size_t)w * h, M = (size_t)dw * dh;
+    // the guide: the fused image's luma, block-averaged (depth.rs:1129-1131)
+    DBuf<float> fused_d;
+    const float *fz = fused3;
+    if (fused_mem == LSH_HOST) {
+        fused_d.alloc(3 * N);
+        copy_mem(fused_d.p, LSH_DEVICE, fused3, LSH_HOST, 3 * N * sizeof(float));
+        fz = fused_d.p;
+    }
+    DBuf<float> yfull(N), guide(M);
+    launch("luma", k_luma3, dim3(g1(N)), dim3(256), 0, fz, N, yfull.p);
+    launch("block_mean", k_block_mean, g2(dw, dh), B2, 0, (const float *)yfull.p, (int)w, (int)h, k, guide.p, dw, dh);
+    // aggregation and the streamed peak search (depth.rs:1137-1151)
+    PeakBufs PB;
+    for (auto *b : {&PB.c1, &PB.l1, &PB.r1, &PB.c2, &PB.prev, &PB.prev2, &PB.sum, &PB.cmin}) b->alloc(M);
+    PB.i1.alloc(M);
+    Peaks P = PB.view();
+    launch("peak_init", k_peak_init, dim3(g1(M)), dim3(256), 0, P, M);
+    GF gf;
+    if (p->agg_radius > 0) gf_init(gf, guide.p, dw, dh, (int)p->agg_radius, p->agg_eps);
+    DBuf<float> stage;
+    if (slices_mem == LSH_HOST) stage.alloc(M);
+    for (uint32_t m = 0; m < n; m++) {
+        const float *c = slices + (size_t)m * M;
+        if (slices_mem == LSH_HOST) {
+            copy_mem(stage.p, LSH_DEVICE, c, LSH_HOST, M * sizeof(float));
+            c = stage.p;
+        }
+        if (p->agg_radius > 0) {
+            gf_step1(gf, c);
+            gf_step2(gf, 2, nullptr, nullptr, P, m);
+        } else {
+            launch("peak_push", k_peak_push_raw, dim3(g1(M)), dim3(256), 0, c, M, P, m);
+        }
+    }
+    // close the profiles, the noise floor (depth.rs:568-580)
+    launch("peak_close", k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
+    DBuf<unsigned> hist;
+    size_t ns = (M + 6) / 7;
+    float floor_ = select_kth(PB.cmin.p, M, 7, ns / 2, hist);
+    DBuf<float> floor_d(1), depth_w(M), conf(M), mins(M);
+    copy_mem(floor_d.p, LSH_DEVICE, &floor_, LSH_HOST, sizeof(float));
+    launch("peak_out", k_peak_out, dim3(g1(M)), dim3(256), 0, P, M, n, p->gate, (const float *)floor_d.p, depth_w.p,
+           conf.p, mins.p);
+    if (p->median) {
+        DBuf<float> med(M);
+        launch("median3", k_median3, g2(dw, dh), B2, 0, (const float *)depth_w.p, dw, dh, med.p);
+        depth_w = std::move(med);
+    }
+    // p90 normalisation (depth.rs:615-623)
+    size_t kk = std::min(ns * 9 / 10, ns - 1);
+    float p90 = select_kth(conf.p, M, 7, kk, hist);
+    if (p90 > 1e-6f) launch("conf_norm", k_conf_norm, dim3(g1(M)), dim3(256), 0, conf.p, M, p90);
+    // WLS (depth.rs:1161-1178)
+    uint32_t it1 = 0, it2 = 0;
+    float rel1 = 0.0f, rel2 = 0.0f;
+    DBuf<float> u;
+    const float *dres = depth_w.p;
+    if (p->lambda > 0.0f) {
+        Wls s;
+        wls_new(s, guide.p, dw, dh, p->lambda, p->sigma_c);
+        wls_set_weights(s, conf.p, nullptr, nullptr, 0.0f);
+        u.alloc(M);
+        u.zero();
+        it1 = wls_solve(s, depth_w.p, p->cg_iters, p->tol, u.p, &rel1);
+        rel2 = rel1;
+        it2 = it1;
+        if (p->robust > 0.0f) {
+            wls_set_weights(s, conf.p, depth_w.p, u.p, p->robust);
+            it2 = wls_solve(s, depth_w.p, p->cg_iters, p->tol, u.p, &rel2);
+        }
+        dres = u.p;
+    }
+    // upsampling (depth.rs:1180-1197)
+    DBuf<float> dout, cout;
+    float *dd = depth, *cc = conf_out;
+    if (out_mem == LSH_HOST) {
+        dout.alloc(N);
+        cout.alloc(N);
+        dd = dout.p;
+        cc = cout.p;
+    }
+    float maxd = (float)(n - 1), inv = 1.0f / (float)k;
+    if (p->upsample_kind == 0) {
+        launch("upsample", k_up_bilinear, g2(w, h), B2, 0, dres, dw, dh, (int)w, (int)h, inv, 0.0f, maxd, 1, dd);
+    } else {
+        GF ug;
+        gf_init(ug, guide.p, dw, dh, (int)p->up_radius, p->up_eps);
+        gf_step1(ug, dres);
+        Peaks none{};
+        if (k == 1) {
+            gf_step2(ug, 4, dd, nullptr, none, 0);
+            launch("clamp", k_clamp, dim3(g1(N)), dim3(256), 0, dd, N, 0.0f, maxd);
+        } else {
+            DBuf<float> ca(M), cb(M);
+            gf_step2(ug, 3, ca.p, cb.p, none, 0);
+            launch("upsample", k_up_guided, g2(w, h), B2, 0, (const float *)ca.p, (const float *)cb.p, dw, dh,
+                   (const float *)yfull.p, (int)w, (int)h, inv, maxd, dd);
+        }
+    }
+    if (k == 1)
+        LSH_CHECK(hipMemcpyAsync(cc, conf.p, M * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    else
+        launch("upsample", k_up_bilinear, g2(w, h), B2, 0, (const float *)conf.p, dw, dh, (int)w, (int)h, inv, 0.0f, 0.0f, 0, cc);
+    if (out_mem == LSH_HOST) {
+        copy_mem(depth, LSH_HOST, dd, LSH_DEVICE, N * sizeof(float));
+        copy_mem(conf_out, LSH_HOST, cc, LSH_DEVICE, N * sizeof(float));
+    }
+    if (floor_plane) copy_mem(floor_plane, out_mem, mins.p, LSH_DEVICE, M * sizeof(float));
+    sync();
+    if (info) {
+        info->dw = (uint32_t)dw;
+        info->dh = (uint32_t)dh;
+        info->iters1 = it1;
+        info->iters2 = it2;
+        info->rel1 = rel1;
+        info->rel2 = rel2;
+        info->p90 = p90;
+        info->floor = floor_;
+    }
+    LSH_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// engine.hpp -- the device state behind the C ABI's opaque handles.
+#pragma once
+#include "common.hpp"
+
+// The frames handle (the reference's GpuFrames, gpu.rs:636-900): the current
+// frame's planes, its registration luma and pyramid, the previous aligned
+// frame's luma pyramid (the next frame's search reference), and frame 0's
+// brightness samples.
+struct lsh_frames {
+    size_t w = 0, h = 0;
+    uint32_t interp = LSH_SPLINE4X4;
+    lsh::DBuf<uint16_t> staging;  // a host frame uploaded here
+    lsh::DBuf<float> planes;      // 3*w*h: the frame after warp and gains
+    lsh::DBuf<float> yt;          // target luma pyramid: level 0 (full res) then the registration levels
+    lsh::DBuf<float> yr;          // reference (previous aligned frame's luma) pyramid, same layout
+    std::vector<size_t> lw, lh, loff;  // registration level dims and offsets (level 0 = full res)
+    bool have_ref = false;        // yr holds a reference
+    bool have_grid = false;       // frame 0's brightness grid is set
+    lsh::DBuf<float> grid;        // 3*gw*gh frame-0 samples on the step-4 grid
+    size_t gw = 0, gh = 0;
+    double ref_means[3] = {0, 0, 0};  // frame 0's channel means (registered stacks)
+    bool have_means = false;
+    lsh::DBuf<double> red;        // reduction partials
+    lsh::DBuf<double> red_out;    // final sums
+    lsh::DBuf<unsigned> ticket;   // last-block counter
+    lsh::DBuf<int32_t> taps_dy, taps_dx;
+    lsh::DBuf<float> taps_wt;
+    uint32_t ntaps = 0, taps_key = 0xffffffffu;
+    size_t processed = 0;
+    uint64_t evals = 0;           // cost evaluations of the last search
+};
+
+struct lsh_fuser {
+    size_t w = 0, h = 0, levels = 0, depth_level = 0, count = 0;
+    lsh_fuse_params p{};
+    bool halo_on = false, finished = false;
+    size_t guide = 0;
+    float hard = 0.f;
+    std::vector<size_t> lw, lh, off;  // level dims, per-plane offsets into a pyramid of S cells
+    size_t S = 0;
+    lsh::DBuf<float> G;        // 3*S Gaussian levels (level 0 used for host-pushed frames)
+    lsh::DBuf<float> acc;      // 3*S fused band-pass levels (and the weighted residual with halo control)
+    lsh::DBuf<float> best;     // S winning energies (weight sums with halo control)
+    lsh::DBuf<uint16_t> winner;
+    lsh::DBuf<float> tops;     // count * 3 * tsz residuals
+    size_t tops_cap = 0;
+    lsh::DBuf<float> en_guide, wgt_a, wgt_b, lap;  // halo control scratch
+    lsh::DBuf<float> ent_tab;  // p*ln(p) per window count (residual rule)
+    lsh::DBuf<unsigned> cnt;   // shares
+};

@@ -1,0 +1,220 @@
+// runtime.hip -- the library's stream, event timing, copies, homography math
+// and the library-level C ABI (probe, version, errors, stream, timing).
+#include <cstdio>
+#include <cstdlib>
+
+#include "api.hpp"
+#include "common.hpp"
+
+namespace lsh {
+
+namespace {
+std::mutex g_mu;
+struct DevState {
+    hipStream_t s = nullptr;
+    void *pinned = nullptr;
+    size_t pinned_n = 0;
+    Prof prof;
+};
+std::map<int, DevState> g_dev;
+
+DevState &dev_state() {
+    int d = 0;
+    LSH_CHECK(hipGetDevice(&d));
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevState &st = g_dev[d];
+    if (!st.s) LSH_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    return st;
+}
+}  // namespace
+
+hipStream_t stream() { return dev_state().s; }
+
+void sync() { LSH_CHECK(hipStreamSynchronize(stream())); }
+
+void *pinned_scratch(size_t n) {
+    DevState &st = dev_state();
+    if (st.pinned_n < n) {
+        sync();
+        if (st.pinned) (void)hipHostFree(st.pinned);
+        st.pinned = nullptr;
+        st.pinned_n = 0;
+        LSH_CHECK(hipHostMalloc(&st.pinned, n, hipHostMallocDefault));
+        st.pinned_n = n;
+    }
+    return st.pinned;
+}
+
+Prof &prof() { return dev_state().prof; }
+
+hipEvent_t Prof::get() {
+    if (pool.empty()) {
+        hipEvent_t e;
+        LSH_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+}
+
+void Prof::flush() {
+    if (pending.empty()) return;
+    LSH_CHECK(hipEventSynchronize(pending.back().b));
+    for (auto &r : pending) {
+        float ms = 0.f;
+        LSH_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        auto &t = totals[r.name];
+        t.ms += ms;
+        t.bytes += r.bytes;
+        t.n += 1;
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+}
+
+void copy_mem(void *dst, lsh_mem dmem, const void *src, lsh_mem smem, size_t bytes) {
+    if (!bytes) return;
+    hipMemcpyKind kind = dmem == LSH_DEVICE ? (smem == LSH_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice)
+                                            : (smem == LSH_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost);
+    LSH_CHECK(hipMemcpyAsync(dst, src, bytes, kind, stream()));
+    if (dmem == LSH_HOST || smem == LSH_HOST) sync();
+}
+
+// align.rs:241-252
+void sim_matrix(const Sim &sim, size_t w, size_t h, double *m) {
+    const double *s = sim.v;
+    double cx = (double)w / 2.0, cy = (double)h / 2.0;
+    double c = std::cos(s[3]), sn = std::sin(s[3]);
+    double sc = s[2];
+    double a = sc * c, b = sc * (c * s[5] - sn * s[4]), d = sc * sn, e = sc * (sn * s[5] + c * s[4]);
+    double gx = s[6] / (double)w, gy = s[7] / (double)h;
+    double a2 = a + cx * gx, b2 = b + cx * gy, d2 = d + cy * gx, e2 = e + cy * gy;
+    double tx = cx + s[0] * (double)w - (a2 * cx + b2 * cy);
+    double ty = cy + s[1] * (double)h - (d2 * cx + e2 * cy);
+    m[0] = a2;
+    m[1] = b2;
+    m[2] = tx;
+    m[3] = d2;
+    m[4] = e2;
+    m[5] = ty;
+    m[6] = gx;
+    m[7] = gy;
+    m[8] = 1.0 - gx * cx - gy * cy;
+}
+
+// align.rs:258-286
+void inverse9(const double *m, double *inv) {
+    if (m[6] == 0.0 && m[7] == 0.0 && m[8] == 1.0) {
+        double a = m[0], b = m[1], tx = m[2], d = m[3], e = m[4], ty = m[5];
+        double det = a * e - b * d;
+        double ia = e / det, ib = -b / det, id = -d / det, ie = a / det;
+        inv[0] = ia;
+        inv[1] = ib;
+        inv[2] = -(ia * tx + ib * ty);
+        inv[3] = id;
+        inv[4] = ie;
+        inv[5] = -(id * tx + ie * ty);
+        inv[6] = 0.0;
+        inv[7] = 0.0;
+        inv[8] = 1.0;
+        return;
+    }
+    double t[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+            t[i * 3 + j] = m[r0 * 3 + c0] * m[r1 * 3 + c1] - m[r0 * 3 + c1] * m[r1 * 3 + c0];
+        }
+    double k = t[8];
+    for (int i = 0; i < 9; i++) inv[i] = t[i] / k;
+}
+
+char *make_err(const std::string &msg) {
+    char *s = (char *)std::malloc(msg.size() + 1);
+    if (s) std::memcpy(s, msg.c_str(), msg.size() + 1);
+    return s;
+}
+
+}  // namespace lsh
+
+using namespace lsh;
+
+extern "C" {
+
+uint32_t lsh_abi(void) { return LSH_ABI_VERSION; }
+
+void lsh_free_str(char *s) { std::free(s); }
+
+char *lsh_available(void) {
+    LSH_API_BEGIN
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return make_err("no HIP device (is the amdgpu driver loaded?)");
+    int d = 0;
+    LSH_CHECK(hipGetDevice(&d));
+    hipDeviceProp_t prop;
+    LSH_CHECK(hipGetDeviceProperties(&prop, d));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0) return make_err("device " + std::to_string(d) + " is " + arch + ", not gfx950");
+    (void)stream();
+    LSH_API_END
+}
+
+char *lsh_stream(void **out) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(out, "lsh_stream: null output");
+    *out = (void *)stream();
+    LSH_API_END
+}
+
+char *lsh_sync(void) {
+    LSH_API_BEGIN
+    sync();
+    LSH_API_END
+}
+
+char *lsh_prof_enable(uint32_t on) {
+    LSH_API_BEGIN
+    Prof &p = prof();
+    if (!on) p.flush();
+    p.on = on != 0;
+    LSH_API_END
+}
+
+char *lsh_prof_reset(void) {
+    LSH_API_BEGIN
+    Prof &p = prof();
+    p.flush();
+    p.totals.clear();
+    LSH_API_END
+}
+
+char *lsh_prof_read(char *names, size_t names_cap, double *total_ms, double *total_bytes, uint64_t *counts, uint32_t cap,
+                    uint32_t *n) {
+    LSH_API_BEGIN
+    Prof &p = prof();
+    p.flush();
+    std::string all;
+    uint32_t k = 0;
+    for (auto &kv : p.totals) {
+        if (k < cap) {
+            if (total_ms) total_ms[k] = kv.second.ms;
+            if (total_bytes) total_bytes[k] = kv.second.bytes;
+            if (counts) counts[k] = kv.second.n;
+            all += kv.first;
+            all += '\n';
+        }
+        k++;
+    }
+    if (names && names_cap) {
+        size_t m = all.size() < names_cap - 1 ? all.size() : names_cap - 1;
+        std::memcpy(names, all.data(), m);
+        names[m] = 0;
+    }
+    if (n) *n = k < cap ? k : cap;
+    LSH_API_END
+}
+
+}  // extern "C"

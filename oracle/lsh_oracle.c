@@ -2172,6 +2172,7 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
     float *y_full = fzeros(w * h), *guide = fzeros(M);
     ora_luma(fused3, w * h, y_full);
     ora_block_mean(y_full, w, h, k, guide);
+    double t_agg = now_s();
     guided_t gf;
     guided_init(&gf, guide, dw, dh, p->agg_radius, p->agg_eps);
     float *agg = fzeros(M);
@@ -2188,6 +2189,8 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
     }
     guided_free(&gf);
     free(agg);
+    t_agg = now_s() - t_agg;
+    double t_peaks = now_s();
     float *depth_w = fzeros(M), *conf = fzeros(M), *mins = fzeros(M), floor_;
     ora_peaks_finish(tr, p->gate, depth_w, conf, mins, &floor_);
     ora_peaks_free(tr);
@@ -2198,6 +2201,8 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
         depth_w = m2;
     }
     float p90 = ora_normalize_conf(conf, M);
+    t_peaks = now_s() - t_peaks;
+    double t_wls = now_s();
     uint32_t it1 = 0, it2 = 0;
     float rel1 = 0.0f, rel2 = 0.0f;
     if (p->lambda > 0.0f) {
@@ -2218,6 +2223,8 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
         free(depth_w);
         depth_w = u;
     }
+    t_wls = now_s() - t_wls;
+    double t_up = now_s();
     float max_d = (float)(n - 1);
     if (p->upsample_kind == 0) {
         ora_upsample_bilinear(depth_w, dw, dh, w, h, k, depth);
@@ -2246,6 +2253,10 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
         info->rel2 = rel2;
         info->p90 = p90;
         info->floor = floor_;
+        info->agg_s = t_agg;
+        info->peaks_s = t_peaks;
+        info->wls_s = t_wls;
+        info->up_s = now_s() - t_up;
     }
     free(depth_w); free(conf); free(mins); free(guide); free(y_full);
     return 0;

@@ -152,6 +152,7 @@ typedef struct ora_depth_out {
     float rel1, rel2;
     float p90;
     float floor;
+    double agg_s, peaks_s, wls_s, up_s; /* wall time of the phases */
 } ora_depth_out;
 /* slices: n planes of dw*dh.  depth/conf: w*h.  floor_plane: dw*dh (nullable). */
 int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, size_t w, size_t h,

@@ -106,7 +106,8 @@ class StackParams(C.Structure):
 
 class DepthOut(C.Structure):
     _fields_ = [("dw", C.c_uint32), ("dh", C.c_uint32), ("iters1", C.c_uint32), ("iters2", C.c_uint32),
-                ("rel1", C.c_float), ("rel2", C.c_float), ("p90", C.c_float), ("floor", C.c_float)]
+                ("rel1", C.c_float), ("rel2", C.c_float), ("p90", C.c_float), ("floor", C.c_float),
+                ("agg_s", C.c_double), ("peaks_s", C.c_double), ("wls_s", C.c_double), ("up_s", C.c_double)]
 
 
 class StackTimes(C.Structure):
