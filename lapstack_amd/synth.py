@@ -101,7 +101,8 @@ class _Blurrer:
         self.pad = int(math.ceil(3 * SIGMA_MAX)) + 1
         c, h, w = img.shape
         self.h, self.w = h, w
-        x = F.pad(img[None], (self.pad,) * 4, mode="reflect")[0]
+        mode = "reflect" if self.pad < min(h, w) else "replicate"
+        x = F.pad(img[None], (self.pad,) * 4, mode=mode)[0]
         self.ph, self.pw = x.shape[1], x.shape[2]
         self.spec = torch.fft.rfft2(x)
         fy = torch.fft.fftfreq(self.ph, device=img.device)[:, None]
