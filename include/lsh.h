@@ -148,6 +148,10 @@ char *lsh_frames_download(lsh_frames *f, float *img3, lsh_mem mem);
 /* Cost evaluations the last lsh_frames_process search ran (the reference logs
  * the same count per frame pair, docs/engine.md:73-75). */
 uint64_t lsh_frames_last_evals(lsh_frames *f);
+/* The last search per registration level (level 0 = full resolution): cost
+ * evaluations, and the device wall-clock ticks (hipDeviceAttributeWallClockRate)
+ * the search spent there; up to cap levels. */
+char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, uint32_t cap);
 void lsh_frames_free(lsh_frames *f);
 
 /* ---- fuser: Laplacian pyramid, region-energy fold, residual rule, collapse */

@@ -6,13 +6,14 @@
 // 1/65535 exactly as the decoder does (io.rs:89), so the values match the
 // CPU path's planes bit for bit.
 //
-// The search is the reference's bounded Nelder-Mead, run on the host; each
-// cost evaluation is one kernel over the reference level that warps the
-// target level (f64 coordinates, f32 Spline4x4 taps), folds d, d^2 and the
-// count per thread in f64, reduces them across the block with wavefront
-// shuffles and across blocks in a fixed order in the last block to finish,
-// and hands back three numbers.
+// The search is the reference's bounded Nelder-Mead, run on the device by one
+// cooperative kernel per frame: each cost evaluation warps the target level
+// onto the reference level (f64 coordinates, f32 Spline4x4 taps), folds d, d^2
+// and the count per thread in f64, reduces them across each block with
+// wavefront shuffles and across blocks, after one grid barrier, in a fixed
+// order -- no host round trip per evaluation.
 #include <algorithm>
+#include <cstdlib>
 
 #include "api.hpp"
 #include "common.hpp"
@@ -135,17 +136,137 @@ __device__ void grid_sum(double (&v)[NV], double *__restrict__ part, unsigned *_
     }
 }
 
-// align.rs:581-635: one cost evaluation -> (sum d, sum d^2, count)
-__global__ __launch_bounds__(256) void k_cost(const float *__restrict__ rf, const float *__restrict__ tg, int tw, int th,
-                                              Inv9 inv, int affine, double *__restrict__ part,
-                                              unsigned *__restrict__ ticket, double *__restrict__ out) {
-    double v[3] = {0.0, 0.0, 0.0};
+// ------------------------------------------------------- the search on device
+// multiscale_align (align.rs:896-942) as one cooperative kernel: every block
+// runs the same bounded Nelder-Mead (align.rs:796-883) on the same numbers.
+// Each cost evaluation is a grid-strided warp-and-reduce; the last block to
+// arrive folds all blocks' partial sums in a fixed order and broadcasts the
+// cost, so all blocks hold the identical value and take the identical next
+// step, with no host round trip.
+#define NM_MAXL 16
+// the search's flag words, each on its own 128-byte line: arrivals of groups,
+// the error flag, the broadcast generation, then one arrival counter per group
+#define NM_GROUPS 16
+#define NM_LINE 32
+#define NM_FL_TOP 0
+#define NM_FL_ERR (1 * NM_LINE)
+#define NM_FL_BCAST (2 * NM_LINE)
+#define NM_FL_HAND (3 * NM_LINE)
+#define NM_FL_GROUP (4 * NM_LINE)
+#define NM_FL_WORDS ((4 + NM_GROUPS) * NM_LINE)
+struct NmConst {
+    const float *ref[NM_MAXL];
+    const float *tgt[NM_MAXL];
+    int lw[NM_MAXL], lh[NM_MAXL];
+    int nlev, finest, nfree;
+    int solo_max;  // levels of at most this many pixels are searched by block 0 alone
+    int free_idx[8];
+    double step[NM_MAXL][8], tol[NM_MAXL][8];
+    double span[8];
+};
+
+// align.rs:241-252 and :258-286 on the device (sin and cos from the device library)
+__device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
+    double cx = (double)w / 2.0, cy = (double)h / 2.0;
+    double c = cos(s[3]), sn = sin(s[3]);
+    double sc = s[2];
+    double a = sc * c, b = sc * (c * s[5] - sn * s[4]), d = sc * sn, e = sc * (sn * s[5] + c * s[4]);
+    double gx = s[6] / (double)w, gy = s[7] / (double)h;
+    double a2 = a + cx * gx, b2 = b + cx * gy, d2 = d + cy * gx, e2 = e + cy * gy;
+    double m[9] = {a2, b2, cx + s[0] * (double)w - (a2 * cx + b2 * cy), d2, e2,
+                   cy + s[1] * (double)h - (d2 * cx + e2 * cy), gx, gy, 1.0 - gx * cx - gy * cy};
+    if (m[6] == 0.0 && m[7] == 0.0 && m[8] == 1.0) {
+        double det = m[0] * m[4] - m[1] * m[3];
+        double ia = m[4] / det, ib = -m[1] / det, id = -m[3] / det, ie = m[0] / det;
+        inv[0] = ia;
+        inv[1] = ib;
+        inv[2] = -(ia * m[2] + ib * m[5]);
+        inv[3] = id;
+        inv[4] = ie;
+        inv[5] = -(id * m[2] + ie * m[5]);
+        inv[6] = 0.0;
+        inv[7] = 0.0;
+        inv[8] = 1.0;
+        return;
+    }
+    double t[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+            t[i * 3 + j] = m[r0 * 3 + c0] * m[r1 * 3 + c1] - m[r0 * 3 + c1] * m[r1 * 3 + c0];
+        }
+    for (int i = 0; i < 9; i++) inv[i] = t[i] / t[8];
+}
+
+#define NM_THREADS 1024
+#define NM_WAVES (NM_THREADS / 64)
+
+struct NmShared {
+    double S[9][8], FV[9], XR[8], XT[8], C[8], lo[8], hi[8], cur[8], inv[9], cost, red[3][NM_WAVES];
+    unsigned lev_evals[NM_MAXL];
+    uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
+    int stop, conv, folds;
+};
+
+// the block's three sums, folded over its waves in wave order
+__device__ __forceinline__ void block_sums(NmShared &sh, double v0, double v1, double v2, double *out) {
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    v2 = wave_sum(v2);
+    __syncthreads();
+    if (lane == 0) {
+        sh.red[0][wv] = v0;
+        sh.red[1][wv] = v1;
+        sh.red[2][wv] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            double t = 0.0;
+            for (int q = 0; q < NM_WAVES; q++) t += sh.red[k][q];
+            out[k] = t;
+        }
+}
+
+// align.rs:630-634
+__device__ __forceinline__ double rms_of(double sd, double sd2, double cnt) {
+    if (cnt < 16.0) return 1e9;
+    double vv = (sd2 - sd * sd / cnt) / cnt;
+    return sqrt(vv > 0.0 ? vv : 0.0);
+}
+
+// One collective cost evaluation (align.rs:581-635) of the point x (nfree
+// doubles, in LDS) at level lvl.  solo: block 0 alone, over all the level's
+// pixels; otherwise every block over its grid-strided share, the partial sums
+// folded by the block completing the arrival tree and the cost broadcast.
+__device__ double nm_cost(const NmConst &K, NmShared &sh, int lvl, const double *x, bool solo, double *part,
+                          unsigned *ctr, unsigned *err, unsigned &gen) {
+    const int tw = K.lw[lvl], th = K.lh[lvl];
+    uint64_t tq = wall_clock64();
+    if (threadIdx.x == 0) {
+        sh.ph[3] += tq - sh.tmark;
+        double v[8];
+        for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
+        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = x[k];
+        sim_inverse_dev(v, tw, th, sh.inv);
+    }
+    __syncthreads();
+    uint64_t tq1 = wall_clock64();
+    if (threadIdx.x == 0) sh.ph[0] += tq1 - tq;
+    Inv9 inv;
+    for (int k = 0; k < 9; k++) inv.m[k] = sh.inv[k];
+    int affine = inv.m[6] == 0.0 && inv.m[7] == 0.0 && inv.m[8] == 1.0;
+    const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
     size_t n = (size_t)tw * th;
     double xmax = (double)(tw - 1), ymax = (double)(th - 1);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int y = (int)(i / tw), x = (int)(i - (size_t)y * tw);
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    size_t start = solo ? threadIdx.x : (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = solo ? blockDim.x : (size_t)gridDim.x * blockDim.x;
+    for (size_t i = start; i < n; i += stride) {
+        int y = (int)(i / tw), x0p = (int)(i - (size_t)y * tw);
         double sx, sy;
-        src_point(inv, affine, x, y, sx, sy);
+        src_point(inv, affine, x0p, y, sx, sy);
         if (!(sx >= 0.0 && sx <= xmax && sy >= 0.0 && sy <= ymax)) continue;
         double fx = floor(sx), fy = floor(sy);
         int x0 = (int)fx, y0 = (int)fy;
@@ -162,11 +283,281 @@ __global__ __launch_bounds__(256) void k_cost(const float *__restrict__ rf, cons
             acc += wy[j] * r;
         }
         double d = (double)(rf[i] - acc);
-        v[0] += d;
-        v[1] += d * d;
-        v[2] += 1.0;
+        v0 += d;
+        v1 += d * d;
+        v2 += 1.0;
     }
-    grid_sum<3>(v, part, ticket, out);
+    double bs[3];
+    block_sums(sh, v0, v1, v2, bs);
+    uint64_t tq2 = wall_clock64();
+    if (threadIdx.x == 0) sh.ph[1] += tq2 - tq1;
+    if (solo) {
+        if (threadIdx.x == 0) {
+            sh.cost = rms_of(bs[0], bs[1], bs[2]);
+            sh.lev_evals[lvl]++;
+            sh.tmark = wall_clock64();
+            sh.ph[2] += sh.tmark - tq2;
+        }
+        __syncthreads();
+        return sh.cost;
+    }
+    unsigned slot = gen & 1;
+    double *pp = part + (size_t)slot * gridDim.x * 3;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; k++) pp[(size_t)blockIdx.x * 3 + k] = bs[k];
+        __threadfence();
+        // two-level arrival: NM_GROUPS counters (one cache line each), then one
+        // counter of completed groups; the block completing the last group folds
+        const unsigned gs = gridDim.x / NM_GROUPS;
+        unsigned t = atomicAdd(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u);
+        int folds = 0;
+        if (t == (gen + 1) * gs - 1) {
+            __threadfence();
+            unsigned t2 = atomicAdd(&ctr[NM_FL_TOP], 1u);
+            folds = t2 == (gen + 1) * NM_GROUPS - 1;
+        }
+        sh.folds = folds;
+    }
+    __syncthreads();
+    unsigned *bflag = ctr + NM_FL_BCAST;
+    double *bcost = part + (size_t)2 * gridDim.x * 3;
+    if (sh.folds) {
+        // the fold order depends on thread indices only, not on which block folds
+        __threadfence();
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            a0 += __hip_atomic_load(&pp[(size_t)b * 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a1 += __hip_atomic_load(&pp[(size_t)b * 3 + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a2 += __hip_atomic_load(&pp[(size_t)b * 3 + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        double tot[3];
+        block_sums(sh, a0, a1, a2, tot);
+        if (threadIdx.x == 0) {
+            double c = rms_of(tot[0], tot[1], tot[2]);
+            __hip_atomic_store(&bcost[slot], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            __hip_atomic_store(bflag, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            sh.cost = c;
+        }
+    } else if (threadIdx.x == 0) {
+        // wait for the broadcast; bounded, and left at once when any block has raised the error flag
+        unsigned spins = 0;
+        while (__hip_atomic_load(bflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen + 1) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 24)) {
+                atomicOr(err, 1u);
+                break;
+            }
+        }
+        __threadfence();
+        sh.cost = __hip_atomic_load(&bcost[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    gen++;
+    if (threadIdx.x == 0) {
+        sh.lev_evals[lvl]++;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) sh.stop = 1;
+        sh.tmark = wall_clock64();
+        sh.ph[2] += sh.tmark - tq2;
+    }
+    __syncthreads();
+    return sh.cost;
+}
+
+__device__ __forceinline__ double clampd_dev(double v, double lo, double hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+
+// One level's bounded Nelder-Mead (align.rs:796-883), collective over the
+// block (solo) or the grid; the simplex lives in LDS and thread 0 steers it.
+__device__ void nm_level(const NmConst &K, NmShared &sh, int lvl, bool solo, double *part, unsigned *ctr,
+                         unsigned *err, unsigned &gen) {
+    const int n = K.nfree;
+    const bool t0 = threadIdx.x == 0;
+    const double *step = K.step[lvl], *tol = K.tol[lvl];
+#define CLAMPK(v, k) clampd_dev((v), sh.lo[k], sh.hi[k])
+#define COST(x) nm_cost(K, sh, lvl, (x), solo, part, ctr, err, gen)
+    if (t0) {
+        for (int k = 0; k < n; k++) sh.S[0][k] = sh.cur[K.free_idx[k]];
+        for (int i = 0; i < n; i++) {
+            for (int k = 0; k < n; k++) sh.S[i + 1][k] = sh.S[0][k];
+            sh.S[i + 1][i] += step[i];
+            for (int k = 0; k < n; k++) sh.S[i + 1][k] = CLAMPK(sh.S[i + 1][k], k);
+        }
+    }
+    __syncthreads();
+    for (int i = 0; i <= n && !sh.stop; i++) {
+        double c = COST(sh.S[i]);
+        if (t0) sh.FV[i] = c;
+    }
+    __syncthreads();
+    for (int it = 0; it < 200 && !sh.stop; it++) {
+        if (t0) {
+            // stable insertion sort of the vertices by value
+            for (int i = 1; i <= n; i++) {
+                double f = sh.FV[i], sv[8];
+                for (int k = 0; k < n; k++) sv[k] = sh.S[i][k];
+                int j = i;
+                while (j > 0 && sh.FV[j - 1] > f) {
+                    sh.FV[j] = sh.FV[j - 1];
+                    for (int k = 0; k < n; k++) sh.S[j][k] = sh.S[j - 1][k];
+                    j--;
+                }
+                sh.FV[j] = f;
+                for (int k = 0; k < n; k++) sh.S[j][k] = sv[k];
+            }
+            // align.rs:888-894
+            int conv = !(fabs(sh.FV[n] - sh.FV[0]) > 1e-4 * (1.0 + fabs(sh.FV[0])));
+            for (int i = 1; i <= n && conv; i++)
+                for (int k = 0; k < n; k++)
+                    if (!(fabs(sh.S[i][k] - sh.S[0][k]) <= tol[k])) conv = 0;
+            sh.conv = conv;
+            // centroid of the best n and the reflection (align.rs:823-836)
+            for (int k = 0; k < n; k++) sh.C[k] = 0.0;
+            for (int i = 0; i < n; i++)
+                for (int k = 0; k < n; k++) sh.C[k] += sh.S[i][k];
+            for (int k = 0; k < n; k++) sh.C[k] /= (double)n;
+            for (int k = 0; k < n; k++) sh.XR[k] = CLAMPK(sh.C[k] + 1.0 * (sh.C[k] - sh.S[n][k]), k);
+        }
+        __syncthreads();
+        if (sh.conv) break;
+        double fr = COST(sh.XR);
+        if (fr < sh.FV[0]) {
+            if (t0)
+                for (int k = 0; k < n; k++) sh.XT[k] = CLAMPK(sh.C[k] + 2.0 * (sh.XR[k] - sh.C[k]), k);
+            __syncthreads();
+            double fe = COST(sh.XT);
+            if (t0) {
+                const double *src = fe < fr ? sh.XT : sh.XR;
+                for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
+                sh.FV[n] = fe < fr ? fe : fr;
+            }
+        } else if (fr < sh.FV[n - 1]) {
+            if (t0) {
+                for (int k = 0; k < n; k++) sh.S[n][k] = sh.XR[k];
+                sh.FV[n] = fr;
+            }
+        } else {
+            if (t0)
+                for (int k = 0; k < n; k++) sh.XT[k] = CLAMPK(sh.C[k] + 0.5 * (sh.S[n][k] - sh.C[k]), k);
+            __syncthreads();
+            double fc = COST(sh.XT);
+            if (fc < sh.FV[n]) {
+                if (t0) {
+                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XT[k];
+                    sh.FV[n] = fc;
+                }
+            } else {
+                for (int i = 1; i <= n && !sh.stop; i++) {
+                    if (t0)
+                        for (int k = 0; k < n; k++) sh.S[i][k] = CLAMPK(sh.S[0][k] + 0.5 * (sh.S[i][k] - sh.S[0][k]), k);
+                    __syncthreads();
+                    double fi = COST(sh.S[i]);
+                    if (t0) sh.FV[i] = fi;
+                }
+            }
+        }
+        __syncthreads();
+    }
+#undef COST
+#undef CLAMPK
+    if (t0) {
+        int best = 0;
+        for (int i = 1; i <= n; i++)
+            if (sh.FV[i] < sh.FV[best]) best = i;
+        for (int k = 0; k < n; k++) sh.cur[K.free_idx[k]] = sh.S[best][k];
+    }
+    __syncthreads();
+}
+
+// state: [0..8) the initial Sim (the guess) in, the found Sim out; [8] the
+// evaluations; [9 + l] the evaluations at level l; [9 + NM_MAXL + l] block 0's
+// wall-clock ticks at level l.  The coarse levels (up to
+// NM_SOLO_MAX pixels) are searched by block 0 alone; it then hands the Sim to
+// the grid (state[0..8) and the hand-off generation), and all blocks search the
+// fine levels together.
+__global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restrict__ Kp, double *__restrict__ state,
+                                                          double *__restrict__ part, unsigned *__restrict__ ctr,
+                                                          unsigned *__restrict__ err) {
+    __shared__ NmShared sh;
+    const NmConst &K = *Kp;
+    const int n = K.nfree;
+    unsigned gen = 0;
+    const bool t0 = threadIdx.x == 0;
+    if (t0) {
+        for (int k = 0; k < 8; k++) sh.cur[k] = state[k];
+        for (int k = 0; k < n; k++) {  // the search box about the guess (align.rs:912-913)
+            double iv = state[K.free_idx[k]];
+            sh.lo[k] = iv - K.span[K.free_idx[k]];
+            sh.hi[k] = iv + K.span[K.free_idx[k]];
+        }
+        sh.stop = 0;
+        for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = 0;
+        for (int q = 0; q < 4; q++) sh.ph[q] = 0;
+        sh.tmark = wall_clock64();
+        if (blockIdx.x == 0)
+            for (int l = 0; l < NM_MAXL; l++) state[9 + NM_MAXL + l] = 0.0;
+    }
+    __syncthreads();
+    int lvl = K.nlev - 1;
+    if (n > 0) {
+        // the coarse levels, block 0 alone
+        if (blockIdx.x == 0)
+            for (int l = lvl; l >= K.finest && (size_t)K.lw[l] * K.lh[l] <= (size_t)K.solo_max; l--) {
+                uint64_t t = wall_clock64();
+                nm_level(K, sh, l, true, part, ctr, err, gen);
+                if (t0) state[9 + NM_MAXL + l] = (double)(wall_clock64() - t);
+            }
+        while (lvl >= K.finest && (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.solo_max) lvl--;
+        if (lvl >= K.finest) {
+            // hand-off: block 0 publishes the Sim, the others take it
+            unsigned *hflag = ctr + NM_FL_HAND;
+            if (t0) {
+                if (blockIdx.x == 0) {
+                    for (int k = 0; k < 8; k++) __hip_atomic_store(&state[k], sh.cur[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __threadfence();
+                    __hip_atomic_store(hflag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(hflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > (1u << 24)) {
+                            atomicOr(err, 1u);
+                            break;
+                        }
+                    }
+                    __threadfence();
+                    for (int k = 0; k < 8; k++) sh.cur[k] = __hip_atomic_load(&state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) sh.stop = 1;
+                }
+            }
+            __syncthreads();
+            for (; lvl >= K.finest && !sh.stop; lvl--) {
+                uint64_t t = wall_clock64();
+                nm_level(K, sh, lvl, false, part, ctr, err, gen);
+                if (t0 && blockIdx.x == 0) state[9 + NM_MAXL + lvl] = (double)(wall_clock64() - t);
+            }
+        }
+    }
+    if (blockIdx.x == 0 && t0) {
+        double tot = 0.0;
+        for (int l = 0; l < NM_MAXL; l++) {
+            state[9 + l] = (double)sh.lev_evals[l];
+            tot += (double)sh.lev_evals[l];
+        }
+        for (int k = 0; k < 8; k++) state[k] = sh.cur[k];
+        state[8] = tot;
+        for (int q = 0; q < 4; q++) state[9 + 2 * NM_MAXL + q] = (double)sh.ph[q];
+    }
+}
+
+// clears the arrival counter, the error flag and the broadcast generation
+// before a search (a kernel, so the sequence can be captured in a graph)
+__global__ void k_nm_reset(unsigned *flags) {
+    for (int i = threadIdx.x; i < NM_FL_WORDS; i += blockDim.x) flags[i] = 0;
 }
 
 // align.rs:416-474 in f64: the interpolation weights of kernel `ik` at fraction t
@@ -421,157 +812,84 @@ static void build_reg_pyramid(lsh_frames *f, float *base) {
     }
 }
 
-// One cost evaluation at registration level lvl (align.rs:926-933)
-static double eval_cost(lsh_frames *f, size_t lvl, const Sim &s) {
-    size_t tw = f->lw[lvl], th = f->lh[lvl], n = tw * th;
-    int affine;
-    Inv9 inv = inv_for(s, tw, th, affine);
-    unsigned grid = std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024));
-    f->red.ensure((size_t)grid * 7);
-    launch_b("warp_cost", 8.0 * n, k_cost, dim3(grid), dim3(256), 0, (const float *)(f->yr.p + f->loff[lvl]),
-           (const float *)(f->yt.p + f->loff[lvl]), (int)tw, (int)th, inv, affine, f->red.p, f->ticket.p, f->red_out.p);
-    double *pin = (double *)pinned_scratch(64);
-    LSH_CHECK(hipMemcpyAsync(pin, f->red_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-    sync();
-    f->evals++;
-    double sd = pin[0], sd2 = pin[1], cnt = pin[2];
-    if (cnt < 16.0) return 1e9;
-    double v = (sd2 - sd * sd / cnt) / cnt;
-    return std::sqrt(v > 0.0 ? v : 0.0);
-}
-
+// align.rs:220: the search box's half-width around the guess, per parameter
 static const double SPAN[8] = {0.10, 0.10, 0.10, 0.087266462599716474, 0.05, 0.05, 0.05, 0.05};
 
-static double clampd(double v, double lo, double hi) {
-    if (v < lo) v = lo;
-    if (v > hi) v = hi;
-    return v;
-}
-
-// align.rs:888-894
-static bool nm_converged(double (*s)[8], const double *fv, const double *tol, size_t n) {
-    if (std::fabs(fv[n] - fv[0]) > 1e-4 * (1.0 + std::fabs(fv[0]))) return false;
-    for (size_t i = 1; i <= n; i++)
-        for (size_t k = 0; k < n; k++)
-            if (!(std::fabs(s[i][k] - s[0][k]) <= tol[k])) return false;
-    return true;
-}
-
-// align.rs:796-883: bounded Nelder-Mead, cost on the device
-template <typename F>
-static void nelder_mead(F &&f, size_t n, const double *x0, const double *lo, const double *hi, const double *step,
-                        const double *tol, double *out) {
-    if (n == 0) return;
-    double s[9][8] = {{0}}, fv[9] = {0}, tmp[9][8], tfv[9];
-    auto clampv = [&](double *v) {
-        for (size_t k = 0; k < n; k++) v[k] = clampd(v[k], lo[k], hi[k]);
-    };
-    for (size_t k = 0; k < n; k++) s[0][k] = x0[k];
-    for (size_t i = 0; i < n; i++) {
-        for (size_t k = 0; k < n; k++) s[i + 1][k] = x0[k];
-        s[i + 1][i] += step[i];
-        clampv(s[i + 1]);
-    }
-    for (size_t i = 0; i <= n; i++) fv[i] = f(s[i]);
-    const double alpha = 1.0, gamma = 2.0, rho = 0.5, sigma = 0.5;
-    for (int it = 0; it < 200; it++) {
-        size_t idx[9];
-        for (size_t i = 0; i <= n; i++) idx[i] = i;
-        for (size_t i = 1; i <= n; i++) {  // stable insertion sort by value
-            size_t v = idx[i], j = i;
-            while (j > 0 && fv[idx[j - 1]] > fv[v]) {
-                idx[j] = idx[j - 1];
-                j--;
-            }
-            idx[j] = v;
-        }
-        for (size_t i = 0; i <= n; i++) {
-            std::memcpy(tmp[i], s[idx[i]], sizeof(tmp[i]));
-            tfv[i] = fv[idx[i]];
-        }
-        std::memcpy(s, tmp, sizeof(s));
-        std::memcpy(fv, tfv, sizeof(fv));
-        if (nm_converged(s, fv, tol, n)) break;
-        double c[8] = {0};
-        for (size_t i = 0; i < n; i++)
-            for (size_t k = 0; k < n; k++) c[k] += s[i][k];
-        for (size_t k = 0; k < n; k++) c[k] /= (double)n;
-        double xr[8];
-        for (size_t k = 0; k < n; k++) xr[k] = c[k] + alpha * (c[k] - s[n][k]);
-        clampv(xr);
-        double fr = f(xr);
-        if (fr < fv[0]) {
-            double xe[8];
-            for (size_t k = 0; k < n; k++) xe[k] = c[k] + gamma * (xr[k] - c[k]);
-            clampv(xe);
-            double fe = f(xe);
-            if (fe < fr) {
-                std::memcpy(s[n], xe, sizeof(xe));
-                fv[n] = fe;
-            } else {
-                std::memcpy(s[n], xr, sizeof(xr));
-                fv[n] = fr;
-            }
-        } else if (fr < fv[n - 1]) {
-            std::memcpy(s[n], xr, sizeof(xr));
-            fv[n] = fr;
-        } else {
-            double xc[8];
-            for (size_t k = 0; k < n; k++) xc[k] = c[k] + rho * (s[n][k] - c[k]);
-            clampv(xc);
-            double fc = f(xc);
-            if (fc < fv[n]) {
-                std::memcpy(s[n], xc, sizeof(xc));
-                fv[n] = fc;
-            } else {
-                for (size_t i = 1; i <= n; i++) {
-                    for (size_t k = 0; k < n; k++) s[i][k] = s[0][k] + sigma * (s[i][k] - s[0][k]);
-                    clampv(s[i]);
-                    fv[i] = f(s[i]);
-                }
-            }
-        }
-    }
-    size_t best = 0;
-    for (size_t i = 1; i <= n; i++)
-        if (fv[i] < fv[best]) best = i;
-    for (size_t k = 0; k < n; k++) out[k] = s[best][k];
-}
-
-// align.rs:896-942 with the target pyramid in yt and the reference in yr
+// multiscale_align (align.rs:896-942) through the cooperative search kernel:
+// the target pyramid in yt, the reference in yr, the guess in, the Sim out.
 static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8, size_t coarsen) {
     size_t n = f->lw.size();  // levels 0..n-1, level 0 the full-resolution plane
-    size_t free_idx[8], nfree = 0;
-    for (size_t k = 0; k < 8; k++)
-        if (free8[k]) free_idx[nfree++] = k;
-    double lo[8], hi[8];
-    for (size_t k = 0; k < nfree; k++) {
-        lo[k] = init.v[free_idx[k]] - SPAN[free_idx[k]];
-        hi[k] = init.v[free_idx[k]] + SPAN[free_idx[k]];
+    LSH_REQUIRE(n <= NM_MAXL, "registration pyramid deeper than the search supports");
+    NmConst K{};
+    K.nlev = (int)n;
+    K.finest = (int)std::min(coarsen, n - 1);
+    K.solo_max = 4096;
+    if (const char *e = std::getenv("LSH_SEARCH_SOLO_MAX")) K.solo_max = std::atoi(e);
+    for (size_t l = 0; l < n; l++) {
+        K.ref[l] = f->yr.p + f->loff[l];
+        K.tgt[l] = f->yt.p + f->loff[l];
+        K.lw[l] = (int)f->lw[l];
+        K.lh[l] = (int)f->lh[l];
     }
-    Sim cur = init;
-    size_t finest = std::min(coarsen, n - 1);
-    double W = (double)f->w, H = (double)f->h;
-    double r = std::max(W, H) / 2.0;
+    for (int k = 0; k < 8; k++) {
+        K.span[k] = SPAN[k];
+        if (free8[k]) K.free_idx[K.nfree++] = k;
+    }
+    double W = (double)f->w, H = (double)f->h, r = std::max(W, H) / 2.0;
     const double gain[8] = {W, H, r, r, H / 2.0, H / 2.0, W / 4.0, H / 4.0};  // align.rs:766-770
-    for (size_t lvl = n; lvl-- > finest;) {
-        Sim snap = cur;
-        auto cost = [&](const double *xf) {
-            Sim v = snap;
-            for (size_t k = 0; k < nfree; k++) v.v[free_idx[k]] = xf[k];
-            return eval_cost(f, lvl, v);
-        };
-        double x0[8], step[8], tol[8], best[8];
-        double px = (double)((uint64_t)1 << lvl);
-        for (size_t k = 0; k < nfree; k++) {
-            x0[k] = cur.v[free_idx[k]];
-            step[k] = 1.0 * px / gain[free_idx[k]];  // align.rs:784-790
-            tol[k] = 0.1 * px / gain[free_idx[k]];
+    for (size_t l = 0; l < n; l++) {
+        double px = (double)((uint64_t)1 << l);
+        for (int k = 0; k < K.nfree; k++) {
+            K.step[l][k] = 1.0 * px / gain[K.free_idx[k]];  // align.rs:784-790
+            K.tol[l][k] = 0.1 * px / gain[K.free_idx[k]];
         }
-        nelder_mead(cost, nfree, x0, lo, hi, step, tol, best);
-        for (size_t k = 0; k < nfree; k++) cur.v[free_idx[k]] = best[k];
     }
-    return cur;
+    if (!f->nm_const.p) {
+        f->nm_const.alloc(sizeof(NmConst));
+        f->nm_state.alloc(13 + 2 * NM_MAXL);
+        f->nm_flags.alloc(NM_FL_WORDS);
+        int dev = 0, cus = 0, per_cu = 0;
+        LSH_CHECK(hipGetDevice(&dev));
+        LSH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        LSH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_nm_search, NM_THREADS, 0));
+        LSH_REQUIRE(per_cu > 0, "the search kernel cannot be resident");
+        int want = 1;  // blocks per CU: more hide gather latency, fewer arrive faster
+        if (const char *e = std::getenv("LSH_SEARCH_BLOCKS_PER_CU")) want = std::max(1, std::atoi(e));
+        f->nm_grid = (unsigned)(std::min(per_cu, want) * cus);
+        f->nm_grid -= f->nm_grid % NM_GROUPS;
+        LSH_REQUIRE(f->nm_grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
+        f->nm_part.alloc((size_t)2 * 3 * f->nm_grid + 2);
+    }
+    uint8_t *pin = (uint8_t *)pinned_scratch(sizeof(NmConst) + 96 * sizeof(double));
+    std::memcpy(pin, &K, sizeof(NmConst));
+    double *st = (double *)(pin + sizeof(NmConst));
+    std::memcpy(st, init.v, 8 * sizeof(double));
+    LSH_CHECK(hipMemcpyAsync(f->nm_const.p, pin, sizeof(NmConst), hipMemcpyHostToDevice, stream()));
+    LSH_CHECK(hipMemcpyAsync(f->nm_state.p, st, 8 * sizeof(double), hipMemcpyHostToDevice, stream()));
+    unsigned *ctr = f->nm_flags.p, *err = f->nm_flags.p + NM_FL_ERR;
+    launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p);
+    const NmConst *kp = (const NmConst *)f->nm_const.p;
+    double *state = f->nm_state.p, *part = f->nm_part.p;
+    void *args[] = {(void *)&kp, (void *)&state, (void *)&part, (void *)&ctr, (void *)&err};
+    launch_coop("nm_search", 0.0, (const void *)k_nm_search, dim3(f->nm_grid), dim3(NM_THREADS), args, 0);
+    double *out = st + 16;
+    LSH_CHECK(hipMemcpyAsync(out, f->nm_state.p, (13 + 2 * NM_MAXL) * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    unsigned *flags = (unsigned *)(out + 13 + 2 * NM_MAXL);
+    LSH_CHECK(hipMemcpyAsync(flags, err, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
+    sync();
+    LSH_REQUIRE(flags[0] == 0, "search: a cost broadcast timed out");
+    double bytes = 0.0;
+    for (size_t l = 0; l < n; l++) bytes += out[9 + l] * 8.0 * (double)(f->lw[l] * f->lh[l]);
+    prof_set_last_bytes(bytes);
+    Sim s;
+    std::memcpy(s.v, out, 8 * sizeof(double));
+    f->evals = (uint64_t)out[8];
+    for (size_t l = 0; l < NM_MAXL; l++) {
+        f->lev_evals[l] = (uint64_t)out[9 + l];
+        f->lev_ticks[l] = (uint64_t)out[9 + NM_MAXL + l];
+    }
+    for (int q = 0; q < 4; q++) f->phase_ticks[q] = (uint64_t)out[9 + 2 * NM_MAXL + q];
+    return s;
 }
 
 static const uint16_t *frame_source(lsh_frames *f, const uint16_t *rgb, lsh_mem mem) {
@@ -765,5 +1083,18 @@ void lsh_frames_free(lsh_frames *f) {
 }
 
 uint64_t lsh_frames_last_evals(lsh_frames *f) { return f ? f->evals : 0; }
+
+char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, uint32_t cap) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(f, "lsh_frames_search_stats: null handle");
+    for (uint32_t l = 0; l < cap && l < NM_MAXL; l++) {
+        if (evals) evals[l] = f->lev_evals[l];
+        if (ticks) ticks[l] = f->lev_ticks[l];
+    }
+    // past the levels: block 0's ticks per phase (matrix, pixels, combine, steering)
+    for (uint32_t q = 0; q < 4 && NM_MAXL + q < cap; q++)
+        if (ticks) ticks[NM_MAXL + q] = f->phase_ticks[q];
+    LSH_API_END
+}
 
 }  // extern "C"

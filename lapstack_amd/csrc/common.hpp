@@ -88,6 +88,31 @@ inline void launch(const char *name, K kernel, dim3 grid, dim3 block, size_t shm
     launch_b(name, 0.0, kernel, grid, block, shmem, args...);
 }
 
+// A cooperative launch (every block resident at once, as grid barriers need).
+inline void launch_coop(const char *name, double bytes, const void *kernel, dim3 grid, dim3 block, void **args,
+                        size_t shmem) {
+    Prof &p = prof();
+    hipStream_t s = stream();
+    hipEvent_t a = nullptr, b = nullptr;
+    if (p.on) {
+        a = p.get();
+        b = p.get();
+        LSH_CHECK(hipEventRecord(a, s));
+    }
+    LSH_CHECK(hipLaunchCooperativeKernel(kernel, grid, block, args, (unsigned)shmem, s));
+    LSH_CHECK(hipGetLastError());
+    if (p.on) {
+        LSH_CHECK(hipEventRecord(b, s));
+        p.pending.push_back({name, bytes, a, b});
+    }
+}
+
+// Set the algorithmic bytes of the last timed launch (known only once it has run).
+inline void prof_set_last_bytes(double bytes) {
+    Prof &p = prof();
+    if (p.on && !p.pending.empty()) p.pending.back().bytes = bytes;
+}
+
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // --------------------------------------------------------------- buffers

@@ -28,6 +28,13 @@ struct lsh_frames {
     uint32_t ntaps = 0, taps_key = 0xffffffffu;
     size_t processed = 0;
     uint64_t evals = 0;           // cost evaluations of the last search
+    lsh::DBuf<uint8_t> nm_const;  // the search's level table
+    lsh::DBuf<double> nm_state;   // guess in / Sim out, evaluation counts
+    lsh::DBuf<unsigned> nm_flags; // grid-barrier counter, error flag
+    lsh::DBuf<double> nm_part;    // per-block partial sums, two slots
+    unsigned nm_grid = 0;
+    uint64_t lev_evals[16] = {0}, lev_ticks[16] = {0};  // the last search, per registration level
+    uint64_t phase_ticks[4] = {0};
 };
 
 struct lsh_fuser {

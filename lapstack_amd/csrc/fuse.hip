@@ -25,26 +25,40 @@ __constant__ float KR[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.
 #define EV2 (2.0f * (1.0f / 16.0f))
 #define ODDK (2.0f * (4.0f / 16.0f))
 
-// pyramid.rs:86-119: one REDUCE output, horizontal 5-tap sums on five rows then the vertical sum
+// pyramid.rs:86-119: REDUCE of one plane (blockIdx.z selects the plane) on a
+// 64x4 output tile: the 131x11 input window (reflect-101 at the borders) is
+// staged in LDS, the horizontal 5-tap sums of its 11 rows at the 64 even
+// columns go to LDS, and each thread sums its five rows.
 __global__ __launch_bounds__(256) void k_reduce3(const float *__restrict__ src, int w, int h,
                                                  float *__restrict__ dst, int ow, int oh) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
-    if (j >= ow || i >= oh) return;
+    constexpr int TX = 64, TY = 4, IW = 2 * TX + 3, IH = 2 * TY + 3;
+    __shared__ float T[IH][IW + 1];
+    __shared__ float Hs[IH][TX + 1];
+    const int ox0 = blockIdx.x * TX, oy0 = blockIdx.y * TY, tid = threadIdx.y * TX + threadIdx.x;
     const float *s = src + (size_t)blockIdx.z * w * h;
-    int xs[5];
-#pragma unroll
-    for (int t = 0; t < 5; t++) xs[t] = rf101(2 * j + t - 2, w);
-    float r[5];
-#pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const float *row = s + (size_t)rf101(2 * i + t - 2, h) * w;
-        float a = 0.0f;
-#pragma unroll
-        for (int u = 0; u < 5; u++) a += KR[u] * row[xs[u]];
-        r[t] = a;
+    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+    const int nx = min(IW, 2 * (ow - ox0) + 3), ny = min(IH, 2 * (oh - oy0) + 3);
+    for (int k = tid; k < IH * IW; k += TX * TY) {
+        int r = k / IW, c = k - r * IW;
+        if (r < ny && c < nx) T[r][c] = s[(size_t)rf101(iy0 + r, h) * w + rf101(ix0 + c, w)];
     }
+    __syncthreads();
+    for (int k = tid; k < IH * TX; k += TX * TY) {
+        int r = k / TX, jj = k - r * TX;
+        if (r < ny && ox0 + jj < ow) {
+            float a = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 5; u++) a += KR[u] * T[r][2 * jj + u];
+            Hs[r][jj] = a;
+        }
+    }
+    __syncthreads();
+    int j = ox0 + threadIdx.x, i = oy0 + threadIdx.y;
+    if (j >= ow || i >= oh) return;
+    int r0 = 2 * threadIdx.y;
     dst[(size_t)blockIdx.z * ow * oh + (size_t)i * ow + j] =
-        KR[0] * r[0] + KR[1] * r[1] + KR[2] * r[2] + KR[3] * r[3] + KR[4] * r[4];
+        KR[0] * Hs[r0][threadIdx.x] + KR[1] * Hs[r0 + 1][threadIdx.x] + KR[2] * Hs[r0 + 2][threadIdx.x] +
+        KR[3] * Hs[r0 + 3][threadIdx.x] + KR[4] * Hs[r0 + 4][threadIdx.x];
 }
 
 // pyramid.rs:128-140: the horizontal EXPAND of coarse row cr at fine column x
@@ -75,63 +89,105 @@ struct Wt9 {
     float k[9];
 };
 
-// One band-pass level of the fold (fuse.rs:401-451): L, region energy, select.
+__device__ __forceinline__ int floordiv2(int a) { return a >= 0 ? a / 2 : -((1 - a) / 2); }
+
+// One band-pass level of the fold (fuse.rs:401-451) on a 64x16 tile of the
+// level: L = G - EXPAND(G_next), the node energy, the separable binomial
+// window and the strict-'>' select, with nothing but the accumulator, the best
+// energies and the winner map written.  The coarse window under the tile and
+// its horizontal EXPAND at every tile column are staged in LDS first, so each
+// fine sample costs three global loads.  LDS rows and columns are indexed by
+// unreflected coordinates and hold the reflect-101 sample of each; every index
+// a fine sample the window can reach needs lies inside the staged window.
 template <int R>
 __global__ __launch_bounds__(256) void k_fold(const float *__restrict__ G, const float *__restrict__ Cn, int w, int h,
                                               int cw, int ch, float *__restrict__ best, float *__restrict__ acc,
                                               uint16_t *__restrict__ winner, int first, unsigned idx, Wt9 wt,
                                               int chroma, float *__restrict__ en_out) {
-    constexpr int TX = 64, TY = 4, EW = TX + 2 * R, EH = TY + 2 * R;
-    __shared__ float E[EH * EW];
-    __shared__ float Tm[EH * TX];
-    __shared__ float Lc[3][TY * TX];
-    const int bx = blockIdx.x * TX, by = blockIdx.y * TY, tid = threadIdx.y * TX + threadIdx.x;
+    constexpr int TX = 64, TY = 16, EW = TX + 2 * R, EH = TY + 2 * R;
+    constexpr int CWM = TX / 2 + R + 4, CHM = TY / 2 + R + 4;
+    __shared__ float Cs[3][CHM][CWM];
+    __shared__ float Hs[3][CHM][EW];
+    __shared__ float Es[EH][EW];
+    __shared__ float Ts[EH][TX];
+    __shared__ float Ls[3][TY][TX];
+    const int bx = blockIdx.x * TX, by = blockIdx.y * TY, tid = threadIdx.x;
     const size_t n = (size_t)w * h, cn = (size_t)cw * ch;
-    for (int k = tid; k < EH * EW; k += TX * TY) {
+    const int cx0 = floordiv2(bx - R) - 1, cy0 = floordiv2(by - R) - 1;
+    const int CW = floordiv2(bx + TX + R - 1) + 1 - cx0 + 1, CH = floordiv2(by + TY + R - 1) + 1 - cy0 + 1;
+    // the last fine column / row any output of this tile reads (window reach)
+    const int xlim = min(bx + TX, w) - 1 + R, ylim = min(by + TY, h) - 1 + R;
+    for (int k = tid; k < 3 * CH * CW; k += 256) {
+        int c = k / (CH * CW), r = k - c * CH * CW, j = r / CW, i = r - j * CW;
+        Cs[c][j][i] = Cn[c * cn + (size_t)rf101(cy0 + j, ch) * cw + rf101(cx0 + i, cw)];
+    }
+    __syncthreads();
+    // pyramid.rs:128-140: horizontal EXPAND of each staged coarse row at each tile column
+    for (int k = tid; k < 3 * CH * EW; k += 256) {
+        int c = k / (CH * EW), r = k - c * CH * EW, j = r / EW, lx = r - j * EW;
+        if (bx + lx - R > xlim) continue;
+        int gx = rf101(bx + lx - R, w), i0 = (gx >> 1) - cx0;
+        const float *row = Cs[c][j];
+        Hs[c][j][lx] = (gx & 1) == 0 ? EV0 * row[i0 - 1] + EV1 * row[i0] + EV2 * row[i0 + 1]
+                                     : ODDK * (row[i0] + row[i0 + 1]);
+    }
+    __syncthreads();
+    // pyramid.rs:151-166 vertical EXPAND, L = G - EXPAND (pyramid.rs:208-211), node energy
+    for (int k = tid; k < EH * EW; k += 256) {
         int ly = k / EW, lx = k - ly * EW;
-        int gy = rf101(by + ly - R, h), gx = rf101(bx + lx - R, w);
+        if (by + ly - R > ylim || bx + lx - R > xlim) continue;
+        int gy = rf101(by + ly - R, h), gx = rf101(bx + lx - R, w), j0 = (gy >> 1) - cy0;
         size_t o = (size_t)gy * w + gx;
-        float l0 = G[o] - expand_at(Cn, cw, ch, gy, gx);
-        float l1 = G[n + o] - expand_at(Cn + cn, cw, ch, gy, gx);
-        float l2 = G[2 * n + o] - expand_at(Cn + 2 * cn, cw, ch, gy, gx);
-        E[k] = node_energy(l0, l1, l2, chroma);
+        float l[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float e = (gy & 1) == 0 ? EV0 * Hs[c][j0 - 1][lx] + EV1 * Hs[c][j0][lx] + EV2 * Hs[c][j0 + 1][lx]
+                                    : ODDK * (Hs[c][j0][lx] + Hs[c][j0 + 1][lx]);
+            l[c] = G[c * n + o] - e;
+        }
+        Es[ly][lx] = node_energy(l[0], l[1], l[2], chroma);
         int cy = ly - R, cx = lx - R;
         if (cy >= 0 && cy < TY && cx >= 0 && cx < TX) {
-            Lc[0][cy * TX + cx] = l0;
-            Lc[1][cy * TX + cx] = l1;
-            Lc[2][cy * TX + cx] = l2;
+            Ls[0][cy][cx] = l[0];
+            Ls[1][cy][cx] = l[1];
+            Ls[2][cy][cx] = l[2];
         }
     }
     __syncthreads();
     // fuse.rs:174-183: horizontal window on every tile row
-    for (int k = tid; k < EH * TX; k += TX * TY) {
+    for (int k = tid; k < EH * TX; k += 256) {
         int ly = k / TX, lx = k - ly * TX;
+        if (by + ly - R > ylim || bx + lx >= w) continue;
         float a = 0.0f;
 #pragma unroll
-        for (int t = 0; t <= 2 * R; t++) a += wt.k[t] * E[ly * EW + lx + t];
-        Tm[k] = a;
+        for (int t = 0; t <= 2 * R; t++) a += wt.k[t] * Es[ly][lx + t];
+        Ts[ly][lx] = a;
     }
     __syncthreads();
-    int x = bx + threadIdx.x, y = by + threadIdx.y;
-    if (x >= w || y >= h) return;
-    // fuse.rs:186-195: vertical window
-    float en = 0.0f;
+    const int tx = tid & 63, x = bx + tx;
+    if (x >= w) return;
 #pragma unroll
-    for (int t = 0; t <= 2 * R; t++) en += wt.k[t] * Tm[(threadIdx.y + t) * TX + threadIdx.x];
-    size_t o = (size_t)y * w + x;
-    int c = threadIdx.y * TX + threadIdx.x;
-    if (en_out) en_out[o] = en;
-    if (first) {
-        best[o] = en;
-        acc[o] = Lc[0][c];
-        acc[n + o] = Lc[1][c];
-        acc[2 * n + o] = Lc[2][c];
-    } else if (en > best[o]) {  // strict: ties keep the earlier frame (fuse.rs:429, 442)
-        best[o] = en;
-        acc[o] = Lc[0][c];
-        acc[n + o] = Lc[1][c];
-        acc[2 * n + o] = Lc[2][c];
-        if (winner) winner[o] = (uint16_t)idx;
+    for (int q = 0; q < TY / 4; q++) {
+        int ty = (tid >> 6) + 4 * q, y = by + ty;
+        if (y >= h) break;
+        // fuse.rs:186-195: vertical window
+        float en = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; t++) en += wt.k[t] * Ts[ty + t][tx];
+        size_t o = (size_t)y * w + x;
+        if (en_out) en_out[o] = en;
+        if (first) {
+            best[o] = en;
+            acc[o] = Ls[0][ty][tx];
+            acc[n + o] = Ls[1][ty][tx];
+            acc[2 * n + o] = Ls[2][ty][tx];
+        } else if (en > best[o]) {  // strict: ties keep the earlier frame (fuse.rs:429, 442)
+            best[o] = en;
+            acc[o] = Ls[0][ty][tx];
+            acc[n + o] = Ls[1][ty][tx];
+            acc[2 * n + o] = Ls[2][ty][tx];
+            if (winner) winner[o] = (uint16_t)idx;
+        }
     }
 }
 
@@ -315,7 +371,7 @@ __global__ void k_share_count(const uint16_t *__restrict__ win, const float *__r
 static void fold_dispatch(int r, dim3 g, const float *G, const float *Cn, int w, int h, int cw, int ch, float *best,
                           float *acc, uint16_t *winner, int first, unsigned idx, const Wt9 &wt, int chroma,
                           float *en_out) {
-    dim3 b(64, 4);
+    dim3 b(256);
     // G_l and G_{l+1} read, best read and written, the accumulator written (fuse.rs:414-451)
     double by = 32.0 * w * h + 12.0 * cw * ch;
     switch (r) {
@@ -373,7 +429,7 @@ static void fuser_push(lsh_fuser *f, const float *g0) {
     Wt9 wt{};
     binomial(f->p.energy_radius, wt.k);
     for (size_t l = 0; l < nsel; l++) {
-        dim3 g(cdiv(f->lw[l], 64), cdiv(f->lh[l], 4));
+        dim3 g(cdiv(f->lw[l], 64), cdiv(f->lh[l], 16));
         bool is_depth = l == f->depth_level;
         float *en_out = (f->halo_on && l == f->guide) ? f->en_guide.p : nullptr;
         fold_dispatch((int)f->p.energy_radius, g, lvl(l), lvl(l + 1), (int)f->lw[l], (int)f->lh[l], (int)f->lw[l + 1],
