@@ -134,6 +134,34 @@ def cpu_baseline(frames, fp, apm, dp, k):
     }
 
 
+def wls_precision(job, n, dp, tol=1e-6):
+    """The WLS system of the last step's depth pass (the 45 MP stack's working
+    grid), solved to a 1e-6 relative residual by the reference FP32 MG-PCG, the
+    FP64-refined FP32 MG-PCG and the FP64 solver; each result's error against an
+    FP64 solve to 1e-12.  Off the timed region."""
+    import torch
+
+    import lapstack_amd as la
+
+    g, d, c, ginfo = la.depth_grid(job.slices, n, job.image, dp, out_device=True)
+    dh, dw = g.shape
+    u = torch.empty((dh, dw), dtype=torch.float32, device="cuda")
+    x = torch.empty((dh, dw), dtype=torch.float64, device="cuda")
+    _, orep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "fp64", tol=1e-12, max_iters=2000, u_out=u, u64_out=x)
+    oracle = x.clone()
+    out = {"grid": [dw, dh], "tol": tol, "oracle": {"tol": 1e-12, "iters": orep["iters"], "ms": round(orep["ms"], 2),
+                                                     "rel": orep["rel"]}}
+    for mode in ("reference", "refined", "fp64"):
+        iters = dp.cg_iters if mode == "reference" else 2000
+        _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=tol, max_iters=iters, u_out=u,
+                                 u64_out=None if mode == "reference" else x)
+        err = float((u.double() - oracle).abs().max())
+        out[mode] = {"iters": rep["iters"], "outer": rep["outer"], "ms": round(rep["ms"], 2), "rel": rep["rel"],
+                     "true_rel": rep["true_rel"], "max_err_vs_fp64": err}
+        log(f"WLS {mode}: {out[mode]}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -178,6 +206,8 @@ def main():
     total = time.perf_counter() - t
     ms_per_step = 1000.0 * total / max(args.steps, 1)
     value = n * args.steps / total
+
+    wls = wls_precision(job, n, dp)
 
     # per-kernel timing pass (HIP events on the library's stream), off the timed region
     _lib.prof_reset()
@@ -226,6 +256,7 @@ def main():
         "stages": stages,
         "align_evals_per_frame": round(last["evals"] / max(n - 1, 1), 1),
         "wls": last["info"],
+        "wls_precision": wls,
         "reference_published": {"note": "RTX 3060 + 128-thread host, 100x8280x5520 TIFF stack incl. decode "
                                         "(docs/engine.md:621-622)", "cuda_frames_per_s": round(100 / 26.0, 3),
                                 "cpu_frames_per_s": round(100 / 85.0, 3)},

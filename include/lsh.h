@@ -183,6 +183,44 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
                             lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
                             float *conf, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info);
 
+/* depth.rs:1125-1159, the depth pass up to the WLS solve: the working grid's
+ * guide luma, the peak depth (after the median when p->median) and the
+ * p90-normalised confidence, each dw*dh into out_mem; info gets dw, dh, p90
+ * and floor.  These are the WLS system's inputs (lsh_wls_solve). */
+char *lsh_depth_grid(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3, lsh_mem fused_mem,
+                     uint32_t w, uint32_t h, const lsh_depth_params *p, float *guide, float *depth, float *conf,
+                     lsh_mem out_mem, lsh_depth_info *info);
+
+/* ---- the WLS solve alone, in three precisions ------------------------- */
+
+typedef enum lsh_wls_mode {
+    LSH_WLS_REFERENCE = 0, /* depth.rs:1006-1077: FP32 MG-PCG, the CG recurrence's residual */
+    LSH_WLS_REFINED = 1,   /* FP32 MG-PCG inside FP64 iterative refinement (FP64 iterate and residual);
+                              exact FP64 coarsest solve */
+    LSH_WLS_FP64 = 2       /* the FP64 oracle: FP64 vectors and V-cycle over the same operator */
+} lsh_wls_mode;
+
+typedef struct lsh_wls_report {
+    uint32_t iters;  /* CG iterations (summed over the refinement's inner solves) */
+    uint32_t outer;  /* refinement steps */
+    double rel;      /* the solver's final relative residual: the CG recurrence's (REFERENCE), the
+                        true one of the FP64 iterate (REFINED), the FP64 recurrence's (FP64) */
+    double true_rel; /* ||W d - A u|| / ||W d|| of the returned FP32 u, in FP64 */
+    double err_fp64; /* max |u - u*| against an FP64 solve to 1e-12 (want_err), else -1 */
+    double ms;       /* device time of the solve: the FGS guess (or warm copy) and the PCG */
+} lsh_wls_report;
+
+/* depth.rs:1081 wls_solve (the one-call WlsSolver: new, set_weights, solve),
+ * with the tolerance, the precision mode and an optional warm start (w*h, in
+ * in_mem; NULL for the reference's FGS guess).  guide, conf, d in in_mem; u
+ * (w*h f32) and u64 (w*h f64, nullable; the FP64 iterate of the REFINED and
+ * FP64 modes) in out_mem.  lambda
+ * must be positive (lambda = 0 is the identity the caller skips, depth.rs:1161). */
+char *lsh_wls_solve(const float *guide, const float *conf, const float *d, const float *warm, lsh_mem in_mem,
+                    uint32_t w, uint32_t h, float lambda, float sigma_c, uint32_t mode, double tol,
+                    uint32_t max_iters, uint32_t want_err, float *u, double *u64, lsh_mem out_mem,
+                    lsh_wls_report *rep);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,9 @@ lapstack_amd/liblsh.so (declared in include/lsh.h):
 * ``HipFrames``            -- gpu.rs:636-900 GpuFrames: process (register, warp,
                               brightness, focus slice), ingest (registered stacks).
 * ``depth_from_slices``    -- depth.rs:1119-1200.
+* ``depth_grid``           -- depth.rs:1125-1159: the WLS system's inputs.
+* ``wls_solve``            -- depth.rs:1081 wls_solve, in three precisions
+                              (reference FP32, FP64-refined FP32, FP64).
 * ``fuse_and_depth``       -- stack.rs:740-853: the per-frame loop over a stack.
 
 Buffers are numpy arrays (host memory) or torch tensors on the HIP device
@@ -29,7 +32,7 @@ from . import _lib
 from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
-           "depth_from_slices", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "depth_from_slices", "depth_grid", "wls_solve", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -162,7 +165,7 @@ def _buf(a, dtype):
     if torch is not None and isinstance(a, torch.Tensor):
         if not a.is_cuda:
             return _buf(a.numpy(), dtype)
-        tdt = {np.float32: torch.float32, np.uint16: torch.uint16}[dtype]
+        tdt = {np.float32: torch.float32, np.float64: torch.float64, np.uint16: torch.uint16}[dtype]
         if a.dtype != tdt:
             raise TypeError(f"device tensor must be {tdt}, got {a.dtype}")
         if not a.is_contiguous():
@@ -335,6 +338,79 @@ def depth_from_slices(slices, n: int, fused, params: Optional[DepthParams] = Non
     del sk, fk, dk, ck
     d = {k: getattr(info, k) for k, _ in info._fields_}
     return DepthMap(depth_out, conf_out, w, h, dw, dh, floor_out, d)
+
+
+def depth_grid(slices, n: int, fused, params: Optional[DepthParams] = None, out_device: bool = False):
+    """depth.rs:1125-1159 on the device: the depth pass up to the WLS solve.
+    Returns (guide, depth, conf, info), each plane (dh, dw) -- the WLS system's
+    inputs -- as numpy arrays, or as device tensors with ``out_device``."""
+    lib = _lib.load()
+    p = params or DepthParams()
+    if n <= 0:
+        raise HipError("depth: no slices")
+    _, h, w = tuple(fused.shape)
+    dw, dh = p.grid(w, h)
+    if tuple(slices.shape) != (n, dh, dw):
+        raise HipError(f"depth: slices have shape {tuple(slices.shape)}, the working grid is {n}x{dh}x{dw}")
+    sp, smem, sk = _buf(slices, np.float32)
+    fp, fmem, fk = _buf(fused, np.float32)
+    if out_device:
+        import torch
+
+        outs = [torch.zeros((dh, dw), dtype=torch.float32, device="cuda") for _ in range(3)]
+    else:
+        outs = [np.zeros((dh, dw), np.float32) for _ in range(3)]
+    ptrs = [_buf(o, np.float32) for o in outs]
+    info = _lib.DepthInfoC()
+    _lib.check(lib.lsh_depth_grid(sp, smem, n, fp, fmem, w, h, C.byref(p._c()), ptrs[0][0], ptrs[1][0], ptrs[2][0],
+                                  ptrs[0][1], C.byref(info)))
+    del sk, fk
+    return outs[0], outs[1], outs[2], {k: getattr(info, k) for k, _ in info._fields_}
+
+
+WLS_MODES = {"reference": 0, "refined": 1, "fp64": 2}
+
+
+def wls_solve(guide, conf, d, lam: float, sigma_c: float, mode: str = "reference", tol: float = 1e-5,
+              max_iters: int = 200, warm=None, want_err: bool = False, u_out=None, u64_out=None):
+    """depth.rs:1081 wls_solve on the device: solve (W + lam L) u = W d with W =
+    max(conf, 0) + 1e-4 and L the guide's edge-weighted Laplacian.  ``mode``:
+    "reference" (depth.rs:1006-1077, FP32), "refined" (FP32 MG-PCG inside FP64
+    iterative refinement) or "fp64" (the FP64 oracle).  Planes are (h, w),
+    numpy or device tensors (all in one memory).  Returns (u, report, u64);
+    u64 is the FP64 iterate ("refined" and "fp64", else None)."""
+    lib = _lib.load()
+    if mode not in WLS_MODES:
+        raise ValueError(f"unknown WLS mode {mode!r}")
+    h, w = tuple(guide.shape)
+    for name, a in (("conf", conf), ("d", d), ("warm", warm)):
+        if a is not None and tuple(a.shape) != (h, w):
+            raise HipError(f"wls: {name} has shape {tuple(a.shape)}, the grid is {h}x{w}")
+    gp, gmem, gk = _buf(guide, np.float32)
+    cp, cmem, ck = _buf(conf, np.float32)
+    dp, dmem, dk = _buf(d, np.float32)
+    wp, wk = None, None
+    if warm is not None:
+        wp, wmem, wk = _buf(warm, np.float32)
+        if wmem != gmem:
+            raise HipError("wls: inputs must share one memory")
+    if not (gmem == cmem == dmem):
+        raise HipError("wls: inputs must share one memory")
+    if u_out is None:
+        u_out = np.zeros((h, w), np.float32)
+        if mode != "reference" and u64_out is None:
+            u64_out = np.zeros((h, w), np.float64)
+    up, umem, uk = _buf(u_out, np.float32)
+    u64p = None
+    if u64_out is not None:
+        u64p, u64mem, u64k = _buf(u64_out, np.float64)
+        if u64mem != umem:
+            raise HipError("wls: outputs must share one memory")
+    rep = _lib.WlsReportC()
+    _lib.check(lib.lsh_wls_solve(gp, cp, dp, wp, gmem, w, h, lam, sigma_c, WLS_MODES[mode], tol, max_iters,
+                                 int(want_err), up, u64p, umem, C.byref(rep)))
+    del gk, ck, dk, wk, uk
+    return u_out, {k: getattr(rep, k) for k, _ in rep._fields_}, u64_out
 
 
 def fuse_and_depth(frames: Sequence, fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,

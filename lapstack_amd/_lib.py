@@ -35,6 +35,11 @@ class DepthInfoC(C.Structure):
                 ("rel2", C.c_float), ("p90", C.c_float), ("floor", C.c_float)]
 
 
+class WlsReportC(C.Structure):
+    _fields_ = [("iters", _u32), ("outer", _u32), ("rel", C.c_double), ("true_rel", C.c_double),
+                ("err_fp64", C.c_double), ("ms", C.c_double)]
+
+
 # (name, restype, argtypes) of every entry point include/lsh.h declares
 SIGNATURES = [
     ("lsh_abi", _u32, []),
@@ -62,10 +67,30 @@ SIGNATURES = [
     ("lsh_fuser_free", None, [_p]),
     ("lsh_depth_from_slices", _p, [_p, C.c_int, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p,
                                    C.c_int, _p, C.POINTER(DepthInfoC)]),
+    ("lsh_depth_grid", _p, [_p, C.c_int, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p, _p, C.c_int,
+                            C.POINTER(DepthInfoC)]),
+    ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
+                           _p, _p, C.c_int, C.POINTER(WlsReportC)]),
 ]
 
 _lock = threading.Lock()
 _lib = None
+
+
+def _start_torch_first():
+    """torch ships a HIP runtime of its own beside the system one this library
+    links.  Both can run in one process (device pointers pass between them),
+    but torch finds no device when the library's runtime started first, so
+    torch's starts here, before the library loads, when torch is present."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except RuntimeError:
+        pass
 
 
 def load(path: str = LIB_PATH):
@@ -74,6 +99,7 @@ def load(path: str = LIB_PATH):
     with _lock:
         if _lib is not None:
             return _lib
+        _start_torch_first()
         if not os.path.exists(path):
             raise OSError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
         lib = C.CDLL(path)

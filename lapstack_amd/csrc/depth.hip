@@ -13,6 +13,7 @@
 #include "api.hpp"
 #include "common.hpp"
 #include "engine.hpp"
+#include "wls.hpp"
 
 namespace lsh {
 
@@ -451,459 +452,9 @@ __global__ void k_conf_norm(float *__restrict__ conf, size_t n, float p90) {
     if (i < n) conf[i] = fminf(conf[i] / p90, 1.0f);
 }
 
-// ---------------------------------------------------------------- WLS
-#define MG_OMEGA 0.8f
-
-// depth.rs:653-672 (scaled by lambda into ax/ay as well, depth.rs:980-981)
-__global__ void k_edges(const float *__restrict__ g, int w, int h, float inv, float lambda, float *__restrict__ ex,
-                        float *__restrict__ ey, float *__restrict__ ax, float *__restrict__ ay) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= w || y >= h) return;
-    size_t o = (size_t)y * w + x;
-    float e1 = x + 1 < w ? expf(fabsf(g[o] - g[o + 1]) * inv) : 0.0f;
-    float e2 = y + 1 < h ? expf(fabsf(g[o] - g[o + w]) * inv) : 0.0f;
-    ex[o] = e1;
-    ey[o] = e2;
-    ax[o] = lambda * e1;
-    ay[o] = lambda * e2;
-}
-
-// depth.rs:852-898: 2x2 Galerkin aggregation.  what: 1 data weights, 2 edges, 3 both.
-__global__ void k_coarsen(const float *__restrict__ wd, const float *__restrict__ ax, const float *__restrict__ ay,
-                          int w, int h, float *__restrict__ cwd, float *__restrict__ cax, float *__restrict__ cay,
-                          int cw, int ch, int what) {
-    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
-    if (xx >= cw || yy >= ch) return;
-    size_t co = (size_t)yy * cw + xx;
-    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
-    if (what & 1) {
-        float o = 0.0f;
-        for (int y = 2 * yy; y < ye; y++) {
-            float s = 0.0f;
-            for (int x = 2 * xx; x < xe; x++) s += wd[(size_t)y * w + x];
-            o += s;
-        }
-        cwd[co] = o;
-    }
-    if (what & 2) {
-        float o = 0.0f;
-        if (xx + 1 < cw)
-            for (int y = 2 * yy; y < ye; y++) o += ax[(size_t)y * w + 2 * xx + 1];
-        cax[co] = o;
-        float s = 0.0f;
-        if (yy + 1 < ch) {
-            int y = 2 * yy + 1;
-            for (int x = 2 * xx; x < xe; x++) s += ay[(size_t)y * w + x];
-        }
-        cay[co] = s;
-    }
-}
-
-// depth.rs:812-834
-__global__ void k_dinv(const float *__restrict__ wd, const float *__restrict__ ax, const float *__restrict__ ay, int w,
-                       int h, float *__restrict__ dinv) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= w || y >= h) return;
-    size_t k = (size_t)y * w + i;
-    float v = wd[k];
-    if (i > 0) v += ax[k - 1];
-    if (i + 1 < w) v += ax[k];
-    if (y > 0) v += ay[k - w];
-    if (y + 1 < h) v += ay[k];
-    dinv[k] = 1.0f / fmaxf(v, 1e-12f);
-}
-
-// depth.rs:994: W = max(conf, 0) + 1e-4; with robust: conf * min(tau / |d - u|, 1) first (depth.rs:1168)
-__global__ void k_data_w(const float *__restrict__ conf, const float *__restrict__ d, const float *__restrict__ u,
-                         float tau, size_t n, float *__restrict__ wd) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float c = conf[i];
-    if (u) c = c * fminf(tau / fabsf(d[i] - u[i]), 1.0f);
-    wd[i] = fmaxf(c, 0.0f) + 1e-4f;
-}
-
-__device__ __forceinline__ float apply_at(const float *__restrict__ wd, const float *__restrict__ ax,
-                                          const float *__restrict__ ay, const float *__restrict__ x, int w, int h,
-                                          int i, int y, size_t k) {
-    float xk = x[k];
-    float v = wd[k] * xk;
-    if (i > 0) v += ax[k - 1] * (xk - x[k - 1]);
-    if (i + 1 < w) v += ax[k] * (xk - x[k + 1]);
-    if (y > 0) v += ay[k - w] * (xk - x[k - w]);
-    if (y + 1 < h) v += ay[k] * (xk - x[k + w]);
-    return v;
-}
-
-// depth.rs:786-810 apply; mode 0: A x; 1: b - A x; 2: x + w D^-1 (b - A x) (depth.rs:837-847)
-__global__ __launch_bounds__(256) void k_stencil(const float *__restrict__ wd, const float *__restrict__ ax,
-                                                 const float *__restrict__ ay, const float *__restrict__ dinv,
-                                                 const float *__restrict__ x, const float *__restrict__ b, int w,
-                                                 int h, int mode, float *__restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= w || y >= h) return;
-    size_t k = (size_t)y * w + i;
-    float v = apply_at(wd, ax, ay, x, w, h, i, y, k);
-    if (mode == 0)
-        out[k] = v;
-    else if (mode == 1)
-        out[k] = b[k] - v;
-    else
-        out[k] = x[k] + MG_OMEGA * dinv[k] * (b[k] - v);
-}
-
-__global__ void k_jacobi0(const float *__restrict__ dinv, const float *__restrict__ b, size_t n, float *__restrict__ x) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = MG_OMEGA * dinv[i] * b[i];
-}
-
-// depth.rs:911-921
-__global__ void k_restrict(const float *__restrict__ r, int w, int h, float *__restrict__ bc, int cw, int ch) {
-    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
-    if (xx >= cw || yy >= ch) return;
-    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
-    float o = 0.0f;
-    for (int y = 2 * yy; y < ye; y++) {
-        float s = 0.0f;
-        for (int x = 2 * xx; x < xe; x++) s += r[(size_t)y * w + x];
-        o += s;
-    }
-    bc[(size_t)yy * cw + xx] = o;
-}
-
-// depth.rs:901-908
-__global__ void k_prolong(float *__restrict__ x, int w, int h, const float *__restrict__ xc, int cw) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= w || y >= h) return;
-    x[(size_t)y * w + i] += xc[(size_t)(y / 2) * cw + i / 2];
-}
-
-// depth.rs:677-702: Thomas solve of every row, one row per thread
-__global__ void k_solve_rows(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
-                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
-                             float *__restrict__ dp) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= h) return;
-    size_t o = (size_t)y * w;
-    float prev_a = 0.0f, cprev = 0.0f, dprev = 0.0f;
-    for (int i = 0; i < w; i++) {
-        float ai = i + 1 < w ? a[o + i] : 0.0f;
-        float wdi = wd ? wd[o + i] : 1.0f;
-        float diag = wdi + lam * (prev_a + ai);
-        float lower = -lam * prev_a;
-        float upper = -lam * ai;
-        float m = diag - lower * cprev;
-        if (fabsf(m) < 1e-12f) m = 1e-12f;
-        cprev = upper / m;
-        dprev = (wdi * f[o + i] - lower * dprev) / m;
-        cp[o + i] = cprev;
-        dp[o + i] = dprev;
-        prev_a = ai;
-    }
-    float nx = dp[o + w - 1];
-    u[o + w - 1] = nx;
-    for (int i = w - 2; i >= 0; i--) {
-        nx = dp[o + i] - cp[o + i] * nx;
-        u[o + i] = nx;
-    }
-}
-
-// depth.rs:707-746: the same down every column, one column per thread
-__global__ void k_solve_cols(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
-                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
-                             float *__restrict__ dp) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= w) return;
-    float cprev = 0.0f, dprev = 0.0f;
-    for (int y = 0; y < h; y++) {
-        size_t i = (size_t)y * w + x;
-        float prev_a = y > 0 ? a[i - w] : 0.0f;
-        float ai = y + 1 < h ? a[i] : 0.0f;
-        float wdi = wd ? wd[i] : 1.0f;
-        float diag = wdi + lam * (prev_a + ai);
-        float lower = -lam * prev_a;
-        float upper = -lam * ai;
-        float m = diag - lower * cprev;
-        if (fabsf(m) < 1e-12f) m = 1e-12f;
-        cprev = upper / m;
-        dprev = (wdi * f[i] - lower * dprev) / m;
-        cp[i] = cprev;
-        dp[i] = dprev;
-    }
-    float nx = dp[(size_t)(h - 1) * w + x];
-    u[(size_t)(h - 1) * w + x] = nx;
-    for (int y = h - 2; y >= 0; y--) {
-        size_t i = (size_t)y * w + x;
-        nx = dp[i] - cp[i] * nx;
-        u[i] = nx;
-    }
-}
-
-__device__ __forceinline__ double wsum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// fixed-order grid reduction of one double per thread (per-block partials, last block folds)
-__device__ void grid_sum1(double v, double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ out) {
-    __shared__ double sh[4];
-    __shared__ bool last;
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double s = wsum(v);
-    if (lane == 0) sh[wv] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += part[b];
-    __syncthreads();
-    s = wsum(acc);
-    if (lane == 0) sh[wv] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *out = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-        *ticket = 0;
-    }
-}
-
-// sum a*b in f64 (depth.rs:1037)
-__global__ __launch_bounds__(256) void k_dot(const float *__restrict__ a, const float *__restrict__ b, size_t n,
-                                             double *__restrict__ part, unsigned *__restrict__ ticket,
-                                             double *__restrict__ out) {
-    double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        s += (double)a[i] * (double)b[i];
-    grid_sum1(s, part, ticket, out);
-}
-
-// ap = A p and sum p*ap
-__global__ __launch_bounds__(256) void k_spmv_dot(const float *__restrict__ wd, const float *__restrict__ ax,
-                                                  const float *__restrict__ ay, const float *__restrict__ p, int w,
-                                                  int h, float *__restrict__ ap, double *__restrict__ part,
-                                                  unsigned *__restrict__ ticket, double *__restrict__ out) {
-    double s = 0.0;
-    size_t n = (size_t)w * h;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
-        int y = (int)(k / w), i = (int)(k - (size_t)y * w);
-        float v = apply_at(wd, ax, ay, p, w, h, i, y, k);
-        ap[k] = v;
-        s += (double)p[k] * (double)v;
-    }
-    grid_sum1(s, part, ticket, out);
-}
-
-// u += alpha p; r -= alpha ap; sum r*r (depth.rs:1064-1066)
-__global__ __launch_bounds__(256) void k_cg_update(float *__restrict__ u, float *__restrict__ r,
-                                                   const float *__restrict__ p, const float *__restrict__ ap,
-                                                   float alpha, size_t n, double *__restrict__ part,
-                                                   unsigned *__restrict__ ticket, double *__restrict__ out) {
-    double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        u[i] += alpha * p[i];
-        float ri = r[i] - alpha * ap[i];
-        r[i] = ri;
-        s += (double)ri * (double)ri;
-    }
-    grid_sum1(s, part, ticket, out);
-}
-
-// b = wd * d (depth.rs:1039)
-__global__ void k_mul(const float *__restrict__ a, const float *__restrict__ b, size_t n, float *__restrict__ out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = a[i] * b[i];
-}
-
-// p = z + beta p (depth.rs:1074)
-__global__ void k_pupdate(float *__restrict__ p, const float *__restrict__ z, float beta, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
-}
-
-struct MgLev {
-    int w, h;
-    DBuf<float> wd, ax, ay, dinv, x, b, r;
-};
-
-struct Wls {
-    int w, h;
-    float lambda;
-    DBuf<float> ex, ey, r, z, p, ap, f, cp, dp;
-    std::vector<MgLev> L;
-    DBuf<double> part, out;
-    DBuf<unsigned> ticket;
-};
-
 static dim3 g2(int w, int h) { return dim3(cdiv(w, 64), cdiv(h, 4)); }
 static const dim3 B2(64, 4);
 static unsigned g1(size_t n) { return cdiv(n, 256); }
-static unsigned gred(size_t n) { return std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024)); }
-
-static double read_scalar(Wls &s) {
-    double *pin = (double *)pinned_scratch(64);
-    LSH_CHECK(hipMemcpyAsync(pin, s.out.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
-    sync();
-    return pin[0];
-}
-
-// depth.rs:976-988
-static void wls_new(Wls &s, const float *guide, int w, int h, float lambda, float sigma_c) {
-    s.w = w;
-    s.h = h;
-    s.lambda = lambda;
-    size_t n = (size_t)w * h;
-    for (auto *b : {&s.ex, &s.ey, &s.r, &s.z, &s.p, &s.ap, &s.f, &s.cp, &s.dp}) b->alloc(n);
-    s.part.alloc(1024);
-    s.out.alloc(1);
-    s.ticket.alloc(1);
-    s.ticket.zero();
-    s.L.clear();
-    s.L.emplace_back();
-    MgLev &t = s.L[0];
-    t.w = w;
-    t.h = h;
-    for (auto *b : {&t.wd, &t.ax, &t.ay, &t.dinv, &t.x, &t.b, &t.r}) b->alloc(n);
-    float inv = -1.0f / fmaxf(sigma_c, 1e-6f);
-    launch("wls_edges", k_edges, g2(w, h), B2, 0, guide, w, h, inv, lambda, s.ex.p, s.ey.p, t.ax.p, t.ay.p);
-    while (std::max(s.L.back().w, s.L.back().h) > 16) {  // MG_MIN (depth.rs:777)
-        const MgLev &fl = s.L.back();
-        MgLev c;
-        c.w = (fl.w + 1) / 2;
-        c.h = (fl.h + 1) / 2;
-        size_t cn = (size_t)c.w * c.h;
-        for (auto *b : {&c.wd, &c.ax, &c.ay, &c.dinv, &c.x, &c.b, &c.r}) b->alloc(cn);
-        launch("wls_coarsen", k_coarsen, g2(c.w, c.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
-               (const float *)fl.ay.p, fl.w, fl.h, c.wd.p, c.ax.p, c.ay.p, c.w, c.h, 2);
-        s.L.push_back(std::move(c));
-    }
-}
-
-// depth.rs:992-1002 (with the optional robust reweighting of depth.rs:1168)
-static void wls_set_weights(Wls &s, const float *conf, const float *d, const float *u, float tau) {
-    size_t n = (size_t)s.w * s.h;
-    launch("wls_weights", k_data_w, dim3(g1(n)), dim3(256), 0, conf, d, u, tau, n, s.L[0].wd.p);
-    for (size_t l = 0; l < s.L.size(); l++) {
-        MgLev &lv = s.L[l];
-        if (l > 0) {
-            MgLev &fl = s.L[l - 1];
-            launch("wls_coarsen", k_coarsen, g2(lv.w, lv.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
-                   (const float *)fl.ay.p, fl.w, fl.h, lv.wd.p, lv.ax.p, lv.ay.p, lv.w, lv.h, 1);
-        }
-        launch("wls_dinv", k_dinv, g2(lv.w, lv.h), B2, 0, (const float *)lv.wd.p, (const float *)lv.ax.p,
-               (const float *)lv.ay.p, lv.w, lv.h, lv.dinv.p);
-    }
-}
-
-static void jacobi(MgLev &t, bool zero) {
-    size_t n = (size_t)t.w * t.h;
-    if (zero) {
-        launch_b("mg_jacobi", 12.0 * n, k_jacobi0, dim3(g1(n)), dim3(256), 0, (const float *)t.dinv.p,
-                 (const float *)t.b.p, n, t.x.p);
-        return;
-    }
-    launch_b("mg_jacobi", 28.0 * n, k_stencil, g2(t.w, t.h), B2, 0, (const float *)t.wd.p, (const float *)t.ax.p,
-             (const float *)t.ay.p, (const float *)t.dinv.p, (const float *)t.x.p, (const float *)t.b.p, t.w, t.h, 2,
-             t.r.p);
-    std::swap(t.x, t.r);
-}
-
-// depth.rs:924-947
-static void vcycle(Wls &s, size_t l) {
-    MgLev &t = s.L[l];
-    if (l + 1 == s.L.size()) {
-        for (int k = 0; k < 32; k++) jacobi(t, k == 0);  // MG_COARSE
-        return;
-    }
-    jacobi(t, true);
-    double n = (double)t.w * t.h;
-    launch_b("mg_residual", 24.0 * n, k_stencil, g2(t.w, t.h), B2, 0, (const float *)t.wd.p, (const float *)t.ax.p,
-             (const float *)t.ay.p, (const float *)t.dinv.p, (const float *)t.x.p, (const float *)t.b.p, t.w, t.h, 1,
-             t.r.p);
-    MgLev &c = s.L[l + 1];
-    launch_b("mg_restrict", 4.0 * n + 4.0 * c.w * c.h, k_restrict, g2(c.w, c.h), B2, 0, (const float *)t.r.p, t.w, t.h,
-             c.b.p, c.w, c.h);
-    vcycle(s, l + 1);
-    launch_b("mg_prolong", 8.0 * n + 4.0 * c.w * c.h, k_prolong, g2(t.w, t.h), B2, 0, t.x.p, t.w, t.h,
-             (const float *)c.x.p, c.w);
-    jacobi(t, false);
-}
-
-static void precond(Wls &s, const float *r, float *z) {
-    size_t n = (size_t)s.w * s.h;
-    LSH_CHECK(hipMemcpyAsync(s.L[0].b.p, r, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-    vcycle(s, 0);
-    LSH_CHECK(hipMemcpyAsync(z, s.L[0].x.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-}
-
-static double dot(Wls &s, const float *a, const float *b) {
-    size_t n = (size_t)s.w * s.h;
-    launch_b("cg_dot", 8.0 * n, k_dot, dim3(gred(n)), dim3(256), 0, a, b, n, s.part.p, s.ticket.p, s.out.p);
-    return read_scalar(s);
-}
-
-// depth.rs:1006-1077
-static uint32_t wls_solve(Wls &s, const float *d, uint32_t max_iters, float tol, float *u, float *rel_out) {
-    int w = s.w, h = s.h;
-    size_t n = (size_t)w * h;
-    const float *wd = s.L[0].wd.p;
-    if (w == 1 || h == 1) {
-        if (h == 1)
-            launch("fgs_rows", k_solve_rows, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ex.p, w, h, s.lambda, s.cp.p, s.dp.p);
-        else
-            launch("fgs_cols", k_solve_cols, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ey.p, w, h, s.lambda, s.cp.p, s.dp.p);
-        if (rel_out) *rel_out = 0.0f;
-        return 0;
-    }
-    const int T = 3;
-    LSH_CHECK(hipMemcpyAsync(s.f.p, d, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-    for (int t = 1; t <= T; t++) {
-        float lam_t = 1.5f * s.lambda * std::pow(4.0f, (float)(T - t)) / (std::pow(4.0f, (float)T) - 1.0f);
-        launch("fgs_rows", k_solve_rows, dim3(cdiv(h, 64)), dim3(64), 0, u, (const float *)s.f.p, t == 1 ? wd : nullptr,
-               (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
-        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-        launch("fgs_cols", k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, u, (const float *)s.f.p, nullptr,
-               (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
-        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-    }
-    float *b = s.f.p;
-    launch("wls_rhs", k_mul, dim3(g1(n)), dim3(256), 0, wd, d, n, b);
-    double bnorm = std::sqrt(dot(s, b, b));
-    if (bnorm < 1e-30) bnorm = 1e-30;
-    MgLev &t0 = s.L[0];
-    launch("cg_residual", k_stencil, g2(w, h), B2, 0, (const float *)t0.wd.p, (const float *)t0.ax.p,
-           (const float *)t0.ay.p, (const float *)t0.dinv.p, (const float *)u, (const float *)b, w, h, 1, s.r.p);
-    precond(s, s.r.p, s.z.p);
-    LSH_CHECK(hipMemcpyAsync(s.p.p, s.z.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-    double rz = dot(s, s.r.p, s.z.p);
-    float rel = (float)(std::sqrt(dot(s, s.r.p, s.r.p)) / bnorm);
-    uint32_t iters = 0;
-    for (uint32_t it = 0; it < max_iters; it++) {
-        if (rel < tol) break;
-        iters++;
-        launch_b("cg_spmv", 20.0 * n, k_spmv_dot, dim3(gred(n)), dim3(256), 0, (const float *)t0.wd.p,
-                 (const float *)t0.ax.p, (const float *)t0.ay.p, (const float *)s.p.p, w, h, s.ap.p, s.part.p,
-                 s.ticket.p, s.out.p);
-        double pap = read_scalar(s);
-        if (pap <= 0.0) break;
-        float alpha = (float)(rz / pap);
-        launch_b("cg_update", 24.0 * n, k_cg_update, dim3(gred(n)), dim3(256), 0, u, s.r.p, (const float *)s.p.p,
-                 (const float *)s.ap.p, alpha, n, s.part.p, s.ticket.p, s.out.p);
-        rel = (float)(std::sqrt(read_scalar(s)) / bnorm);
-        if (rel < tol) break;
-        precond(s, s.r.p, s.z.p);
-        double rz_new = dot(s, s.r.p, s.z.p);
-        float beta = (float)(rz_new / rz);
-        rz = rz_new;
-        launch("cg_pupdate", k_pupdate, dim3(g1(n)), dim3(256), 0, s.p.p, (const float *)s.z.p, beta, n);
-    }
-    if (rel_out) *rel_out = rel;
-    return iters;
-}
 
 // Guided-filter plumbing on the working grid (depth.rs:349-412)
 struct GF {
@@ -960,19 +511,27 @@ struct PeakBufs {
 
 using namespace lsh;
 
-extern "C" {
+// The depth pass up to the WLS solve (depth.rs:1125-1159): the guide, the
+// aggregated focus volume's peaks, the noise floor, the median and the p90
+// confidence normalisation, all on the working grid.
+struct DepthGrid {
+    int k = 1, dw = 0, dh = 0;
+    DBuf<float> yfull, guide, depth_w, conf, mins;
+    float p90 = 0.0f, floor_ = 0.0f;
+};
 
-char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
-                            lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
-                            float *conf_out, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info) {
-    LSH_API_BEGIN
-    LSH_REQUIRE(slices && fused3 && p && depth && conf_out, "lsh_depth_from_slices: null argument");
+static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
+                       lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p) {
     LSH_REQUIRE(n > 0, "depth: no slices");
     LSH_REQUIRE(w > 0 && h > 0, "depth: empty frame");
     LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
     const int k = 1 << p->scale;
     const int dw = (int)((w + k - 1) / k), dh = (int)((h + k - 1) / k);
+    G.k = k;
+    G.dw = dw;
+    G.dh = dh;
     const size_t N = (size_t)w * h, M = (size_t)dw * dh;
+    (void)N;
     // the guide: the fused image's luma, block-averaged (depth.rs:1129-1131)
     DBuf<float> fused_d;
     const float *fz = fused3;
@@ -981,7 +540,9 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         copy_mem(fused_d.p, LSH_DEVICE, fused3, LSH_HOST, 3 * N * sizeof(float));
         fz = fused_d.p;
     }
-    DBuf<float> yfull(N), guide(M);
+    DBuf<float> &yfull = G.yfull, &guide = G.guide;
+    yfull.alloc(N);
+    guide.alloc(M);
     launch("luma", k_luma3, dim3(g1(N)), dim3(256), 0, fz, N, yfull.p);
     launch("block_mean", k_block_mean, g2(dw, dh), B2, 0, (const float *)yfull.p, (int)w, (int)h, k, guide.p, dw, dh);
     // aggregation and the streamed peak search (depth.rs:1137-1151)
@@ -1011,8 +572,11 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
     launch("peak_close", k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
     DBuf<unsigned> hist;
     size_t ns = (M + 6) / 7;
-    float floor_ = select_kth(PB.cmin.p, M, 7, ns / 2, hist);
-    DBuf<float> floor_d(1), depth_w(M), conf(M), mins(M);
+    float floor_ = G.floor_ = select_kth(PB.cmin.p, M, 7, ns / 2, hist);
+    DBuf<float> floor_d(1), &depth_w = G.depth_w, &conf = G.conf, &mins = G.mins;
+    depth_w.alloc(M);
+    conf.alloc(M);
+    mins.alloc(M);
     copy_mem(floor_d.p, LSH_DEVICE, &floor_, LSH_HOST, sizeof(float));
     launch("peak_out", k_peak_out, dim3(g1(M)), dim3(256), 0, P, M, n, p->gate, (const float *)floor_d.p, depth_w.p,
            conf.p, mins.p);
@@ -1023,25 +587,44 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
     }
     // p90 normalisation (depth.rs:615-623)
     size_t kk = std::min(ns * 9 / 10, ns - 1);
-    float p90 = select_kth(conf.p, M, 7, kk, hist);
+    float p90 = G.p90 = select_kth(conf.p, M, 7, kk, hist);
     if (p90 > 1e-6f) launch("conf_norm", k_conf_norm, dim3(g1(M)), dim3(256), 0, conf.p, M, p90);
+}
+
+extern "C" {
+
+char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
+                            lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
+                            float *conf_out, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(slices && fused3 && p && depth && conf_out, "lsh_depth_from_slices: null argument");
+    LSH_REQUIRE(n > 0, "depth: no slices");
+    LSH_REQUIRE(w > 0 && h > 0, "depth: empty frame");
+    LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
+    DepthGrid G;
+    depth_grid(G, slices, slices_mem, n, fused3, fused_mem, w, h, p);
+    const int k = G.k, dw = G.dw, dh = G.dh;
+    const size_t N = (size_t)w * h, M = (size_t)dw * dh;
+    DBuf<float> &yfull = G.yfull, &guide = G.guide, &depth_w = G.depth_w, &conf = G.conf, &mins = G.mins;
+    const float p90 = G.p90, floor_ = G.floor_;
     // WLS (depth.rs:1161-1178)
     uint32_t it1 = 0, it2 = 0;
     float rel1 = 0.0f, rel2 = 0.0f;
     DBuf<float> u;
     const float *dres = depth_w.p;
     if (p->lambda > 0.0f) {
-        Wls s;
-        wls_new(s, guide.p, dw, dh, p->lambda, p->sigma_c);
-        wls_set_weights(s, conf.p, nullptr, nullptr, 0.0f);
+        WlsPtr s = wls_create(guide.p, dw, dh, p->lambda, p->sigma_c);
+        wls_set_weights(s.get(), conf.p, nullptr, nullptr, 0.0f);
         u.alloc(M);
         u.zero();
-        it1 = wls_solve(s, depth_w.p, p->cg_iters, p->tol, u.p, &rel1);
-        rel2 = rel1;
-        it2 = it1;
+        WlsResult r1 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, WLS_REFERENCE, nullptr, u.p, nullptr);
+        it1 = it2 = r1.iters;
+        rel1 = rel2 = (float)r1.rel;
         if (p->robust > 0.0f) {
-            wls_set_weights(s, conf.p, depth_w.p, u.p, p->robust);
-            it2 = wls_solve(s, depth_w.p, p->cg_iters, p->tol, u.p, &rel2);
+            wls_set_weights(s.get(), conf.p, depth_w.p, u.p, p->robust);
+            WlsResult r2 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, WLS_REFERENCE, nullptr, u.p, nullptr);
+            it2 = r2.iters;
+            rel2 = (float)r2.rel;
         }
         dres = u.p;
     }
@@ -1091,6 +674,28 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         info->rel2 = rel2;
         info->p90 = p90;
         info->floor = floor_;
+    }
+    LSH_API_END
+}
+
+char *lsh_depth_grid(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3, lsh_mem fused_mem,
+                     uint32_t w, uint32_t h, const lsh_depth_params *p, float *guide, float *depth, float *conf,
+                     lsh_mem out_mem, lsh_depth_info *info) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(slices && fused3 && p && guide && depth && conf, "lsh_depth_grid: null argument");
+    DepthGrid G;
+    depth_grid(G, slices, slices_mem, n, fused3, fused_mem, w, h, p);
+    const size_t M = (size_t)G.dw * G.dh;
+    copy_mem(guide, out_mem, G.guide.p, LSH_DEVICE, M * sizeof(float));
+    copy_mem(depth, out_mem, G.depth_w.p, LSH_DEVICE, M * sizeof(float));
+    copy_mem(conf, out_mem, G.conf.p, LSH_DEVICE, M * sizeof(float));
+    sync();
+    if (info) {
+        *info = lsh_depth_info{};
+        info->dw = (uint32_t)G.dw;
+        info->dh = (uint32_t)G.dh;
+        info->p90 = G.p90;
+        info->floor = G.floor_;
     }
     LSH_API_END
 }

@@ -1,0 +1,866 @@
+// wls.hip -- the confidence-weighted edge-aware WLS solve (depth.rs:653-1087)
+// on gfx950, in three precisions (wls.hpp): the reference's FP32 MG-PCG, FP32
+// MG-PCG with FP64 residual refinement and an exact FP64 coarsest solve, and an
+// all-FP64 oracle.
+//
+// The operator is the 5-point stencil of (W + lambda L) held as per-cell
+// coefficients (data weight, edge to the right, edge below) on a 2x2
+// aggregation hierarchy.  Every kernel is a streaming pass over cell planes;
+// dot products sum in f64 through per-block partials folded in a fixed order.
+#include <algorithm>
+
+#include "api.hpp"
+#include "common.hpp"
+#include "wls.hpp"
+
+namespace lsh {
+
+#define MG_OMEGA 0.8f  // depth.rs:770
+#define MG_COARSE 32   // depth.rs:775
+#define MG_MIN 16      // depth.rs:777
+
+static dim3 g2(int w, int h) { return dim3(cdiv(w, 64), cdiv(h, 4)); }
+static const dim3 B2(64, 4);
+static unsigned g1(size_t n) { return cdiv(n, 256); }
+static unsigned gred(size_t n) { return std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024)); }
+
+// depth.rs:653-672, and the lambda-scaled coefficients of depth.rs:980-981
+__global__ void k_edges(const float *__restrict__ g, int w, int h, float inv, float lambda, float *__restrict__ ex,
+                        float *__restrict__ ey, float *__restrict__ ax, float *__restrict__ ay) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    size_t o = (size_t)y * w + x;
+    float e1 = x + 1 < w ? expf(fabsf(g[o] - g[o + 1]) * inv) : 0.0f;
+    float e2 = y + 1 < h ? expf(fabsf(g[o] - g[o + w]) * inv) : 0.0f;
+    ex[o] = e1;
+    ey[o] = e2;
+    ax[o] = lambda * e1;
+    ay[o] = lambda * e2;
+}
+
+// depth.rs:852-898: 2x2 Galerkin aggregation.  what: 1 data weights, 2 edges.
+__global__ void k_coarsen(const float *__restrict__ wd, const float *__restrict__ ax, const float *__restrict__ ay,
+                          int w, int h, float *__restrict__ cwd, float *__restrict__ cax, float *__restrict__ cay,
+                          int cw, int ch, int what) {
+    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (xx >= cw || yy >= ch) return;
+    size_t co = (size_t)yy * cw + xx;
+    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
+    if (what & 1) {
+        float o = 0.0f;
+        for (int y = 2 * yy; y < ye; y++) {
+            float s = 0.0f;
+            for (int x = 2 * xx; x < xe; x++) s += wd[(size_t)y * w + x];
+            o += s;
+        }
+        cwd[co] = o;
+    }
+    if (what & 2) {
+        float o = 0.0f;
+        if (xx + 1 < cw)
+            for (int y = 2 * yy; y < ye; y++) o += ax[(size_t)y * w + 2 * xx + 1];
+        cax[co] = o;
+        float s = 0.0f;
+        if (yy + 1 < ch) {
+            int y = 2 * yy + 1;
+            for (int x = 2 * xx; x < xe; x++) s += ay[(size_t)y * w + x];
+        }
+        cay[co] = s;
+    }
+}
+
+// depth.rs:812-834 (f32, as the reference); the FP64 hierarchy computes its own in f64
+template <typename T>
+__global__ void k_dinv(const T *__restrict__ wd, const T *__restrict__ ax, const T *__restrict__ ay, int w, int h,
+                       T *__restrict__ dinv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h) return;
+    size_t k = (size_t)y * w + i;
+    T v = wd[k];
+    if (i > 0) v += ax[k - 1];
+    if (i + 1 < w) v += ax[k];
+    if (y > 0) v += ay[k - w];
+    if (y + 1 < h) v += ay[k];
+    dinv[k] = (T)1 / (v > (T)1e-12 ? v : (T)1e-12);
+}
+
+// depth.rs:994 and :1168
+__global__ void k_data_w(const float *__restrict__ conf, const float *__restrict__ d, const float *__restrict__ u,
+                         float tau, size_t n, float *__restrict__ wd) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float c = conf[i];
+    if (u) c = c * fminf(tau / fabsf(d[i] - u[i]), 1.0f);
+    wd[i] = fmaxf(c, 0.0f) + 1e-4f;
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_at(const T *__restrict__ wd, const T *__restrict__ ax, const T *__restrict__ ay,
+                                      const T *__restrict__ x, int w, int h, int i, int y, size_t k) {
+    T xk = x[k];
+    T v = wd[k] * xk;
+    if (i > 0) v += ax[k - 1] * (xk - x[k - 1]);
+    if (i + 1 < w) v += ax[k] * (xk - x[k + 1]);
+    if (y > 0) v += ay[k - w] * (xk - x[k - w]);
+    if (y + 1 < h) v += ay[k] * (xk - x[k + w]);
+    return v;
+}
+
+// depth.rs:786-810 apply; mode 0: A x; 1: b - A x; 2: x + w D^-1 (b - A x) (depth.rs:837-847)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ wd, const T *__restrict__ ax,
+                                                 const T *__restrict__ ay, const T *__restrict__ dinv,
+                                                 const T *__restrict__ x, const T *__restrict__ b, int w, int h,
+                                                 int mode, T *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h) return;
+    size_t k = (size_t)y * w + i;
+    T v = apply_at(wd, ax, ay, x, w, h, i, y, k);
+    if (mode == 0)
+        out[k] = v;
+    else if (mode == 1)
+        out[k] = b[k] - v;
+    else
+        out[k] = x[k] + (T)MG_OMEGA * dinv[k] * (b[k] - v);
+}
+
+template <typename T>
+__global__ void k_jacobi0(const T *__restrict__ dinv, const T *__restrict__ b, size_t n, T *__restrict__ x) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (T)MG_OMEGA * dinv[i] * b[i];
+}
+
+// depth.rs:911-921
+template <typename T>
+__global__ void k_restrict(const T *__restrict__ r, int w, int h, T *__restrict__ bc, int cw, int ch) {
+    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (xx >= cw || yy >= ch) return;
+    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
+    T o = 0;
+    for (int y = 2 * yy; y < ye; y++) {
+        T s = 0;
+        for (int x = 2 * xx; x < xe; x++) s += r[(size_t)y * w + x];
+        o += s;
+    }
+    bc[(size_t)yy * cw + xx] = o;
+}
+
+// depth.rs:901-908
+template <typename T>
+__global__ void k_prolong(T *__restrict__ x, int w, int h, const T *__restrict__ xc, int cw) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h) return;
+    x[(size_t)y * w + i] += xc[(size_t)(y / 2) * cw + i / 2];
+}
+
+// depth.rs:677-702: Thomas solve of every row, one row per thread (wd null: unit weights)
+__global__ void k_solve_rows(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
+                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
+                             float *__restrict__ dp) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h) return;
+    size_t o = (size_t)y * w;
+    float prev_a = 0.0f, cprev = 0.0f, dprev = 0.0f;
+    for (int i = 0; i < w; i++) {
+        float ai = i + 1 < w ? a[o + i] : 0.0f;
+        float wdi = wd ? wd[o + i] : 1.0f;
+        float diag = wdi + lam * (prev_a + ai);
+        float lower = -lam * prev_a;
+        float upper = -lam * ai;
+        float m = diag - lower * cprev;
+        if (fabsf(m) < 1e-12f) m = 1e-12f;
+        cprev = upper / m;
+        dprev = (wdi * f[o + i] - lower * dprev) / m;
+        cp[o + i] = cprev;
+        dp[o + i] = dprev;
+        prev_a = ai;
+    }
+    float nx = dp[o + w - 1];
+    u[o + w - 1] = nx;
+    for (int i = w - 2; i >= 0; i--) {
+        nx = dp[o + i] - cp[o + i] * nx;
+        u[o + i] = nx;
+    }
+}
+
+// depth.rs:707-746: the same down every column, one column per thread
+__global__ void k_solve_cols(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
+                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
+                             float *__restrict__ dp) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w) return;
+    float cprev = 0.0f, dprev = 0.0f;
+    for (int y = 0; y < h; y++) {
+        size_t i = (size_t)y * w + x;
+        float prev_a = y > 0 ? a[i - w] : 0.0f;
+        float ai = y + 1 < h ? a[i] : 0.0f;
+        float wdi = wd ? wd[i] : 1.0f;
+        float diag = wdi + lam * (prev_a + ai);
+        float lower = -lam * prev_a;
+        float upper = -lam * ai;
+        float m = diag - lower * cprev;
+        if (fabsf(m) < 1e-12f) m = 1e-12f;
+        cprev = upper / m;
+        dprev = (wdi * f[i] - lower * dprev) / m;
+        cp[i] = cprev;
+        dp[i] = dprev;
+    }
+    float nx = dp[(size_t)(h - 1) * w + x];
+    u[(size_t)(h - 1) * w + x] = nx;
+    for (int y = h - 2; y >= 0; y--) {
+        size_t i = (size_t)y * w + x;
+        nx = dp[i] - cp[i] * nx;
+        u[i] = nx;
+    }
+}
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// fixed-order grid reduction of one double per thread (per-block partials, last block folds)
+__device__ void grid_sum1(double v, double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ out) {
+    __shared__ double sh[4];
+    __shared__ bool last;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double s = wsum(v);
+    if (lane == 0) sh[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += part[b];
+    __syncthreads();
+    s = wsum(acc);
+    if (lane == 0) sh[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *out = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        *ticket = 0;
+    }
+}
+
+// sum a*b in f64 (depth.rs:1037)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n,
+                                             double *__restrict__ part, unsigned *__restrict__ ticket,
+                                             double *__restrict__ out) {
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    grid_sum1(s, part, ticket, out);
+}
+
+// ap = A p and sum p*ap
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_dot(const T *__restrict__ wd, const T *__restrict__ ax,
+                                                  const T *__restrict__ ay, const T *__restrict__ p, int w, int h,
+                                                  T *__restrict__ ap, double *__restrict__ part,
+                                                  unsigned *__restrict__ ticket, double *__restrict__ out) {
+    double s = 0.0;
+    size_t n = (size_t)w * h;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        int y = (int)(k / w), i = (int)(k - (size_t)y * w);
+        T v = apply_at(wd, ax, ay, p, w, h, i, y, k);
+        ap[k] = v;
+        s += (double)p[k] * (double)v;
+    }
+    grid_sum1(s, part, ticket, out);
+}
+
+// u += alpha p; r -= alpha ap; sum r*r (depth.rs:1064-1066)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_update(T *__restrict__ u, T *__restrict__ r, const T *__restrict__ p,
+                                                   const T *__restrict__ ap, T alpha, size_t n,
+                                                   double *__restrict__ part, unsigned *__restrict__ ticket,
+                                                   double *__restrict__ out) {
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        u[i] += alpha * p[i];
+        T ri = r[i] - alpha * ap[i];
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    grid_sum1(s, part, ticket, out);
+}
+
+// p = z + beta p (depth.rs:1074)
+template <typename T>
+__global__ void k_pupdate(T *__restrict__ p, const T *__restrict__ z, T beta, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// b = wd * d (depth.rs:1039), in T
+template <typename T>
+__global__ void k_rhs(const float *__restrict__ wd, const float *__restrict__ d, size_t n, T *__restrict__ b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (T)wd[i] * (T)d[i];
+}
+
+// The true residual in FP64 of the FP32 iterate: r = b - A u with the f32
+// coefficients and u promoted; scaled copy for the next inner solve later.
+template <typename U>
+__global__ __launch_bounds__(256) void k_resid64(const float *__restrict__ wd, const float *__restrict__ ax,
+                                                 const float *__restrict__ ay, const U *__restrict__ u,
+                                                 const double *__restrict__ b, int w, int h, double *__restrict__ r,
+                                                 double *__restrict__ part, unsigned *__restrict__ ticket,
+                                                 double *__restrict__ out) {
+    double s = 0.0;
+    size_t n = (size_t)w * h;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        int y = (int)(k / w), i = (int)(k - (size_t)y * w);
+        double uk = u[k];
+        double v = (double)wd[k] * uk;
+        if (i > 0) v += (double)ax[k - 1] * (uk - (double)u[k - 1]);
+        if (i + 1 < w) v += (double)ax[k] * (uk - (double)u[k + 1]);
+        if (y > 0) v += (double)ay[k - w] * (uk - (double)u[k - w]);
+        if (y + 1 < h) v += (double)ay[k] * (uk - (double)u[k + w]);
+        double rk = b[k] - v;
+        r[k] = rk;
+        s += rk * rk;
+    }
+    grid_sum1(s, part, ticket, out);
+}
+
+__global__ void k_scale_to_f32(const double *__restrict__ a, double s, size_t n, float *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)(a[i] * s);
+}
+
+// u += e / s (the refinement's correction)
+__global__ void k_correct(double *__restrict__ x, const float *__restrict__ e, double inv_s, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)e[i] * inv_s;
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ a, size_t n, B *__restrict__ b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (B)a[i];
+}
+
+// max |u - u64| per block (the host folds the blocks)
+__global__ void k_absdiff_max(const float *__restrict__ u, const double *__restrict__ v, size_t n,
+                              double *__restrict__ part) {
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs((double)u[i] - v[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// The exact solve of the coarsest grid (at most 16x16 cells): a banded
+// Cholesky factor in f64 of the 5-point operator, half bandwidth p = cw in the
+// row-major cell order, L(i,j) for i-p <= j <= i at Lb[i*(p+1) + i-j].  The
+// factor is right-looking in LDS: per column, the pivot, the p entries below
+// it, then the p(p+1)/2 trailing updates in parallel (one thread per pair).
+#define CHOL_MAXN 256
+#define CHOL_STRIDE 17
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_factor(const T *__restrict__ wd, const T *__restrict__ ax,
+                                                     const T *__restrict__ ay, int cw, int ch,
+                                                     double *__restrict__ Lb) {
+    __shared__ double S[CHOL_MAXN * CHOL_STRIDE];
+    const int n = cw * ch, p = cw, st = p + 1, tid = threadIdx.x;
+    for (int idx = tid; idx < n * st; idx += blockDim.x) {
+        int i = idx / st, o = idx - i * st, j = i - o;
+        double v = 0.0;
+        if (o == 0) {
+            int yi = i / cw, xi = i - yi * cw;
+            v = (double)wd[i];
+            if (xi > 0) v += (double)ax[i - 1];
+            if (xi + 1 < cw) v += (double)ax[i];
+            if (yi > 0) v += (double)ay[i - cw];
+            if (yi + 1 < ch) v += (double)ay[i];
+        } else if (o == 1 && j >= 0 && (i % cw) > 0) {
+            v = -(double)ax[j];
+        } else if (o == p && j >= 0) {
+            v = -(double)ay[j];
+        }
+        S[idx] = v;
+    }
+    __syncthreads();
+    const int a = tid / 16 + 1, b = tid % 16 + 1;  // trailing pair (row j+a, column j+b), b <= a <= p
+    for (int j = 0; j < n; j++) {
+        if (tid == 0) {
+            double d = S[j * st];
+            S[j * st] = sqrt(d > 1e-300 ? d : 1e-300);
+        }
+        __syncthreads();
+        if (tid < p && j + 1 + tid < n) {
+            int i = j + 1 + tid;
+            S[i * st + i - j] /= S[j * st];
+        }
+        __syncthreads();
+        if (b <= a && a <= p && j + a < n) {
+            int i = j + a, k = j + b;
+            S[i * st + i - k] -= S[i * st + i - j] * S[k * st + k - j];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < n * st; idx += blockDim.x) Lb[idx] = S[idx];
+}
+
+// x = (L L^T)^-1 b by column-oriented substitutions in LDS (one wavefront)
+template <typename T>
+__global__ __launch_bounds__(64) void k_chol_solve(const double *__restrict__ Lb, int cw, int ch,
+                                                   const T *__restrict__ b, T *__restrict__ x) {
+    __shared__ double S[CHOL_MAXN * CHOL_STRIDE];
+    __shared__ double y[CHOL_MAXN];
+    const int n = cw * ch, p = cw, st = p + 1, tid = threadIdx.x;
+    for (int idx = tid; idx < n * st; idx += blockDim.x) S[idx] = Lb[idx];
+    for (int i = tid; i < n; i += blockDim.x) y[i] = (double)b[i];
+    __syncthreads();
+    for (int j = 0; j < n; j++) {  // L y = b
+        if (tid == 0) y[j] /= S[j * st];
+        __syncthreads();
+        if (tid < p && j + 1 + tid < n) {
+            int i = j + 1 + tid;
+            y[i] -= S[i * st + i - j] * y[j];
+        }
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; j--) {  // L^T x = y
+        if (tid == 0) y[j] /= S[j * st];
+        __syncthreads();
+        if (tid < p && j - 1 - tid >= 0) {
+            int i = j - 1 - tid;
+            y[i] -= S[j * st + j - i] * y[j];
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += blockDim.x) x[i] = (T)y[i];
+}
+
+template <typename T>
+struct Lev {
+    int w = 0, h = 0;
+    DBuf<T> wd, ax, ay, dinv, x, b, r;
+    void alloc(int w_, int h_) {
+        w = w_;
+        h = h_;
+        size_t n = (size_t)w * h;
+        for (auto *bf : {&wd, &ax, &ay, &dinv, &x, &b, &r}) bf->alloc(n);
+    }
+};
+
+struct WlsSolver {
+    int w = 0, h = 0;
+    float lambda = 0.f;
+    DBuf<float> ex, ey, cp, dp, f;
+    std::vector<Lev<float>> L;
+    std::vector<Lev<double>> L64;
+    bool l64_ok = false, chol_ok = false;
+    DBuf<double> chol;
+    DBuf<float> r, z, p, ap, e;
+    DBuf<double> b64, r64, z64, p64, ap64, x64;
+    DBuf<double> part, out;
+    DBuf<unsigned> ticket;
+};
+
+void WlsDeleter::operator()(WlsSolver *s) const { delete s; }
+
+static double read_out(WlsSolver &s) {
+    double *pin = (double *)pinned_scratch(64);
+    LSH_CHECK(hipMemcpyAsync(pin, s.out.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    return pin[0];
+}
+
+WlsPtr wls_create(const float *guide, int w, int h, float lambda, float sigma_c) {
+    WlsPtr sp(new WlsSolver());
+    WlsSolver &s = *sp;
+    s.w = w;
+    s.h = h;
+    s.lambda = lambda;
+    size_t n = (size_t)w * h;
+    for (auto *b : {&s.ex, &s.ey, &s.cp, &s.dp, &s.f, &s.r, &s.z, &s.p, &s.ap}) b->alloc(n);
+    s.part.alloc(1024);
+    s.out.alloc(1);
+    s.ticket.alloc(1);
+    s.ticket.zero();
+    s.L.emplace_back();
+    s.L[0].alloc(w, h);
+    float inv = -1.0f / fmaxf(sigma_c, 1e-6f);
+    launch_b("wls_edges", 20.0 * n, k_edges, g2(w, h), B2, 0, guide, w, h, inv, lambda, s.ex.p, s.ey.p, s.L[0].ax.p,
+             s.L[0].ay.p);
+    while (std::max(s.L.back().w, s.L.back().h) > MG_MIN) {
+        int cw = (s.L.back().w + 1) / 2, chh = (s.L.back().h + 1) / 2;
+        s.L.emplace_back();
+        s.L.back().alloc(cw, chh);
+        Lev<float> &fl = s.L[s.L.size() - 2], &c = s.L.back();
+        launch("wls_coarsen", k_coarsen, g2(c.w, c.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
+               (const float *)fl.ay.p, fl.w, fl.h, c.wd.p, c.ax.p, c.ay.p, c.w, c.h, 2);
+    }
+    return sp;
+}
+
+void wls_set_weights(WlsSolver *sp, const float *conf, const float *d, const float *u, float tau) {
+    WlsSolver &s = *sp;
+    size_t n = (size_t)s.w * s.h;
+    launch_b("wls_weights", 8.0 * n, k_data_w, dim3(g1(n)), dim3(256), 0, conf, d, u, tau, n, s.L[0].wd.p);
+    for (size_t l = 0; l < s.L.size(); l++) {
+        Lev<float> &lv = s.L[l];
+        if (l > 0) {
+            Lev<float> &fl = s.L[l - 1];
+            launch("wls_coarsen", k_coarsen, g2(lv.w, lv.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
+                   (const float *)fl.ay.p, fl.w, fl.h, lv.wd.p, lv.ax.p, lv.ay.p, lv.w, lv.h, 1);
+        }
+        launch("wls_dinv", k_dinv<float>, g2(lv.w, lv.h), B2, 0, (const float *)lv.wd.p, (const float *)lv.ax.p,
+               (const float *)lv.ay.p, lv.w, lv.h, lv.dinv.p);
+    }
+    s.l64_ok = false;
+    s.chol_ok = false;
+}
+
+// the coarsest grid's factor (from the f32 coefficients, promoted)
+static void ensure_chol(WlsSolver &s) {
+    if (s.chol_ok) return;
+    Lev<float> &c = s.L.back();
+    LSH_REQUIRE(c.w <= 16 && c.h <= 16, "coarsest grid larger than 16x16");
+    s.chol.ensure((size_t)c.w * c.h * (c.w + 1));
+    launch("wls_coarse_factor", k_chol_factor<float>, dim3(1), dim3(256), 0, (const float *)c.wd.p,
+           (const float *)c.ax.p, (const float *)c.ay.p, c.w, c.h, s.chol.p);
+    s.chol_ok = true;
+}
+
+// the FP64 hierarchy: the f32 coefficients promoted, dinv in f64
+static void ensure_l64(WlsSolver &s) {
+    if (s.l64_ok) return;
+    if (s.L64.size() != s.L.size()) {
+        s.L64.clear();
+        for (auto &lv : s.L) {
+            s.L64.emplace_back();
+            s.L64.back().alloc(lv.w, lv.h);
+        }
+    }
+    for (size_t l = 0; l < s.L.size(); l++) {
+        Lev<float> &a = s.L[l];
+        Lev<double> &b = s.L64[l];
+        size_t n = (size_t)a.w * a.h;
+        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.wd.p, n, b.wd.p);
+        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.ax.p, n, b.ax.p);
+        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.ay.p, n, b.ay.p);
+        launch("wls_dinv", k_dinv<double>, g2(b.w, b.h), B2, 0, (const double *)b.wd.p, (const double *)b.ax.p,
+               (const double *)b.ay.p, b.w, b.h, b.dinv.p);
+    }
+    s.l64_ok = true;
+}
+
+template <typename T>
+static void jacobi(Lev<T> &t, bool zero) {
+    size_t n = (size_t)t.w * t.h;
+    double sz = sizeof(T);
+    if (zero) {
+        launch_b("mg_jacobi", 3.0 * sz * n, k_jacobi0<T>, dim3(g1(n)), dim3(256), 0, (const T *)t.dinv.p,
+                 (const T *)t.b.p, n, t.x.p);
+        return;
+    }
+    launch_b("mg_jacobi", 7.0 * sz * n, k_stencil<T>, g2(t.w, t.h), B2, 0, (const T *)t.wd.p, (const T *)t.ax.p,
+             (const T *)t.ay.p, (const T *)t.dinv.p, (const T *)t.x.p, (const T *)t.b.p, t.w, t.h, 2, t.r.p);
+    std::swap(t.x, t.r);
+}
+
+// depth.rs:924-947; exact: the coarsest grid solved by the FP64 factor instead of 32 sweeps
+template <typename T>
+static void vcycle(WlsSolver &s, std::vector<Lev<T>> &H, size_t l, bool exact) {
+    Lev<T> &t = H[l];
+    if (l + 1 == H.size()) {
+        if (exact)
+            launch("mg_coarse_solve", k_chol_solve<T>, dim3(1), dim3(64), 0, (const double *)s.chol.p, t.w, t.h,
+                   (const T *)t.b.p, t.x.p);
+        else
+            for (int k = 0; k < MG_COARSE; k++) jacobi(t, k == 0);
+        return;
+    }
+    jacobi(t, true);
+    double n = (double)t.w * t.h, sz = sizeof(T);
+    launch_b("mg_residual", 6.0 * sz * n, k_stencil<T>, g2(t.w, t.h), B2, 0, (const T *)t.wd.p, (const T *)t.ax.p,
+             (const T *)t.ay.p, (const T *)t.dinv.p, (const T *)t.x.p, (const T *)t.b.p, t.w, t.h, 1, t.r.p);
+    Lev<T> &c = H[l + 1];
+    launch_b("mg_restrict", sz * (n + (double)c.w * c.h), k_restrict<T>, g2(c.w, c.h), B2, 0, (const T *)t.r.p, t.w,
+             t.h, c.b.p, c.w, c.h);
+    vcycle(s, H, l + 1, exact);
+    launch_b("mg_prolong", sz * (2.0 * n + (double)c.w * c.h), k_prolong<T>, g2(t.w, t.h), B2, 0, t.x.p, t.w, t.h,
+             (const T *)c.x.p, c.w);
+    jacobi(t, false);
+}
+
+template <typename T>
+static double dot(WlsSolver &s, const T *a, const T *b, size_t n) {
+    launch_b("cg_dot", 2.0 * sizeof(T) * n, k_dot<T>, dim3(gred(n)), dim3(256), 0, a, b, n, s.part.p, s.ticket.p, s.out.p);
+    return read_out(s);
+}
+
+template <typename T>
+static void precond(WlsSolver &s, std::vector<Lev<T>> &H, const T *r, T *z, bool exact) {
+    size_t n = (size_t)H[0].w * H[0].h;
+    LSH_CHECK(hipMemcpyAsync(H[0].b.p, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+    vcycle(s, H, 0, exact);
+    LSH_CHECK(hipMemcpyAsync(z, H[0].x.p, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+}
+
+// MG-PCG (depth.rs:1036-1077) on H from the iterate u for the right-hand side b:
+// the reference's sequence of operations; rel is the recurrence's residual over bnorm.
+template <typename T>
+static uint32_t pcg(WlsSolver &s, std::vector<Lev<T>> &H, T *u, const T *b, double bnorm, double tol, uint32_t max_iters,
+                    bool exact, double &rel_out, T *r, T *z, T *p, T *ap) {
+    const int w = H[0].w, h = H[0].h;
+    const size_t n = (size_t)w * h;
+    Lev<T> &t0 = H[0];
+    const double sz = sizeof(T);
+    launch_b("cg_residual", 6.0 * sz * n, k_stencil<T>, g2(w, h), B2, 0, (const T *)t0.wd.p, (const T *)t0.ax.p,
+             (const T *)t0.ay.p, (const T *)t0.dinv.p, (const T *)u, b, w, h, 1, r);
+    precond(s, H, (const T *)r, z, exact);
+    LSH_CHECK(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+    double rz = dot<T>(s, r, z, n);
+    // the reference keeps the residual in f32 (depth.rs:1051); so does the FP32 solver here
+    auto relv = [&](double rr) { return sizeof(T) == 4 ? (double)(float)(std::sqrt(rr) / bnorm) : std::sqrt(rr) / bnorm; };
+    auto below = [&](double rel) { return sizeof(T) == 4 ? (float)rel < (float)tol : rel < tol; };
+    double rel = relv(dot<T>(s, r, r, n));
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < max_iters; it++) {
+        if (below(rel)) break;
+        iters++;
+        launch_b("cg_spmv", 5.0 * sz * n, k_spmv_dot<T>, dim3(gred(n)), dim3(256), 0, (const T *)t0.wd.p,
+                 (const T *)t0.ax.p, (const T *)t0.ay.p, (const T *)p, w, h, ap, s.part.p, s.ticket.p, s.out.p);
+        double pap = read_out(s);
+        if (pap <= 0.0) break;
+        T alpha = (T)(rz / pap);
+        launch_b("cg_update", 6.0 * sz * n, k_cg_update<T>, dim3(gred(n)), dim3(256), 0, u, r, (const T *)p,
+                 (const T *)ap, alpha, n, s.part.p, s.ticket.p, s.out.p);
+        rel = relv(read_out(s));
+        if (below(rel)) break;
+        precond(s, H, (const T *)r, z, exact);
+        double rz_new = dot<T>(s, r, z, n);
+        T beta = (T)(rz_new / rz);
+        rz = rz_new;
+        launch_b("cg_pupdate", 3.0 * sz * n, k_pupdate<T>, dim3(g1(n)), dim3(256), 0, p, (const T *)z, beta, n);
+    }
+    rel_out = rel;
+    return iters;
+}
+
+// depth.rs:1019-1034: three rounds of row and column sweeps with the lambda_t schedule
+static void fgs_guess(WlsSolver &s, const float *d, float *u) {
+    const int w = s.w, h = s.h, T = 3;
+    const size_t n = (size_t)w * h;
+    const float *wd = s.L[0].wd.p;
+    LSH_CHECK(hipMemcpyAsync(s.f.p, d, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    for (int t = 1; t <= T; t++) {
+        float lam_t = 1.5f * s.lambda * std::pow(4.0f, (float)(T - t)) / (std::pow(4.0f, (float)T) - 1.0f);
+        launch("fgs_rows", k_solve_rows, dim3(cdiv(h, 64)), dim3(64), 0, u, (const float *)s.f.p, t == 1 ? wd : nullptr,
+               (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
+        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+        launch("fgs_cols", k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, u, (const float *)s.f.p, nullptr,
+               (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
+        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    }
+}
+
+WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double tol, int mode, const float *warm,
+                    float *u, double *u64) {
+    WlsSolver &s = *sp;
+    const int w = s.w, h = s.h;
+    const size_t n = (size_t)w * h;
+    const float *wd = s.L[0].wd.p;
+    WlsResult res;
+    if (w == 1 || h == 1) {  // depth.rs:1009-1017: the exact 1-D solve
+        if (h == 1)
+            launch("fgs_rows", k_solve_rows, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ex.p, w, h, s.lambda, s.cp.p, s.dp.p);
+        else
+            launch("fgs_cols", k_solve_cols, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ey.p, w, h, s.lambda, s.cp.p, s.dp.p);
+        if (u64) launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, u64);
+        return res;
+    }
+    if (warm)
+        LSH_CHECK(hipMemcpyAsync(u, warm, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    else
+        fgs_guess(s, d, u);
+    if (mode == WLS_REFERENCE) {
+        float *b = s.f.p;
+        launch_b("wls_rhs", 12.0 * n, k_rhs<float>, dim3(g1(n)), dim3(256), 0, wd, d, n, b);
+        double bnorm = std::sqrt(dot<float>(s, b, b, n));
+        if (bnorm < 1e-30) bnorm = 1e-30;
+        res.iters = pcg<float>(s, s.L, u, b, bnorm, tol, max_iters, false, res.rel, s.r.p, s.z.p, s.p.p, s.ap.p);
+        return res;
+    }
+    for (auto *bf : {&s.b64, &s.r64}) bf->ensure(n);
+    launch_b("wls_rhs", 16.0 * n, k_rhs<double>, dim3(g1(n)), dim3(256), 0, wd, d, n, s.b64.p);
+    double bnorm = std::sqrt(dot<double>(s, s.b64.p, s.b64.p, n));
+    if (bnorm < 1e-30) bnorm = 1e-30;
+    ensure_chol(s);
+    if (mode == WLS_REFINED) {
+        // the iterate x (FP64) starts at the guess; each step: r = b - A x in
+        // FP64, the FP32 MG-PCG solves A e = r / |r|, x += |r| e in FP64
+        s.e.ensure(n);
+        s.x64.ensure(n);
+        double *x = u64 ? u64 : s.x64.p;
+        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, x);
+        const Lev<float> &t0 = s.L[0];
+        for (uint32_t outer = 0; outer < 64; outer++) {
+            launch_b("wls_resid64", 36.0 * n, k_resid64<double>, dim3(gred(n)), dim3(256), 0, (const float *)t0.wd.p,
+                     (const float *)t0.ax.p, (const float *)t0.ay.p, (const double *)x, (const double *)s.b64.p, w, h,
+                     s.r64.p, s.part.p, s.ticket.p, s.out.p);
+            double rr = read_out(s);
+            res.rel = std::sqrt(rr) / bnorm;
+            if (res.rel < tol || res.iters >= max_iters || rr == 0.0) break;
+            res.outer++;
+            double scale = 1.0 / std::sqrt(rr);
+            launch_b("wls_demote", 12.0 * n, k_scale_to_f32, dim3(g1(n)), dim3(256), 0, (const double *)s.r64.p, scale, n,
+                     s.f.p);
+            s.e.zero();
+            // the inner FP32 solve is asked for what is still missing, with a
+            // margin, but never below what FP32 vectors hold
+            double inner_tol = std::min(0.5, std::max(1e-5, 0.5 * tol / res.rel));
+            double inner_rel = 0.0;
+            uint32_t it = pcg<float>(s, s.L, s.e.p, s.f.p, 1.0, inner_tol, max_iters - res.iters, true, inner_rel, s.r.p,
+                                     s.z.p, s.p.p, s.ap.p);
+            res.iters += it;
+            launch_b("wls_correct", 20.0 * n, k_correct, dim3(g1(n)), dim3(256), 0, x, (const float *)s.e.p, 1.0 / scale, n);
+            if (it == 0) break;
+        }
+        launch("wls_demote", k_convert<double, float>, dim3(g1(n)), dim3(256), 0, (const double *)x, n, u);
+        return res;
+    }
+    // FP64 oracle
+    ensure_l64(s);
+    for (auto *bf : {&s.z64, &s.p64, &s.ap64, &s.x64}) bf->ensure(n);
+    double *x = u64 ? u64 : s.x64.p;
+    launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, x);
+    res.iters = pcg<double>(s, s.L64, x, (const double *)s.b64.p, bnorm, tol, max_iters, true, res.rel, s.r64.p, s.z64.p,
+                            s.p64.p, s.ap64.p);
+    launch("wls_demote", k_convert<double, float>, dim3(g1(n)), dim3(256), 0, (const double *)x, n, u);
+    return res;
+}
+
+double wls_true_rel(WlsSolver *sp, const float *d, const float *u) {
+    WlsSolver &s = *sp;
+    const int w = s.w, h = s.h;
+    const size_t n = (size_t)w * h;
+    for (auto *bf : {&s.b64, &s.r64}) bf->ensure(n);
+    const Lev<float> &t0 = s.L[0];
+    launch_b("wls_rhs", 16.0 * n, k_rhs<double>, dim3(g1(n)), dim3(256), 0, (const float *)t0.wd.p, d, n, s.b64.p);
+    double bnorm = std::sqrt(dot<double>(s, s.b64.p, s.b64.p, n));
+    if (bnorm < 1e-30) bnorm = 1e-30;
+    launch_b("wls_resid64", 28.0 * n, k_resid64<float>, dim3(gred(n)), dim3(256), 0, (const float *)t0.wd.p,
+             (const float *)t0.ax.p, (const float *)t0.ay.p, u, (const double *)s.b64.p, w, h, s.r64.p, s.part.p,
+             s.ticket.p, s.out.p);
+    return std::sqrt(read_out(s)) / bnorm;
+}
+
+double wls_max_abs_diff(WlsSolver *sp, const float *u, const double *u64) {
+    WlsSolver &s = *sp;
+    size_t n = (size_t)s.w * s.h;
+    unsigned g = gred(n);
+    launch("wls_absdiff", k_absdiff_max, dim3(g), dim3(256), 0, u, u64, n, s.part.p);
+    std::vector<double> h(g);
+    copy_mem(h.data(), LSH_HOST, s.part.p, LSH_DEVICE, g * sizeof(double));
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    return m;
+}
+
+}  // namespace lsh
+
+extern "C" {
+
+char *lsh_wls_solve(const float *guide, const float *conf, const float *d, const float *warm, lsh_mem in_mem,
+                    uint32_t w, uint32_t h, float lambda, float sigma_c, uint32_t mode, double tol,
+                    uint32_t max_iters, uint32_t want_err, float *u, double *u64, lsh_mem out_mem,
+                    lsh_wls_report *rep) {
+    using namespace lsh;
+    LSH_API_BEGIN
+    LSH_REQUIRE(guide && conf && d && u, "lsh_wls_solve: null argument");
+    LSH_REQUIRE(w > 0 && h > 0, "wls: empty grid");
+    LSH_REQUIRE(mode <= LSH_WLS_FP64, "wls: unknown mode");
+    LSH_REQUIRE(lambda > 0.0f, "wls: lambda must be positive");
+    LSH_REQUIRE(!u64 || mode != LSH_WLS_REFERENCE, "wls: u64 is an output of the refined and FP64 modes");
+    const size_t n = (size_t)w * h;
+    DBuf<float> in_d;
+    const float *g = guide, *c = conf, *dd = d, *wm = warm;
+    if (in_mem == LSH_HOST) {
+        in_d.alloc((warm ? 4 : 3) * n);
+        copy_mem(in_d.p, LSH_DEVICE, guide, LSH_HOST, n * sizeof(float));
+        copy_mem(in_d.p + n, LSH_DEVICE, conf, LSH_HOST, n * sizeof(float));
+        copy_mem(in_d.p + 2 * n, LSH_DEVICE, d, LSH_HOST, n * sizeof(float));
+        if (warm) copy_mem(in_d.p + 3 * n, LSH_DEVICE, warm, LSH_HOST, n * sizeof(float));
+        g = in_d.p;
+        c = in_d.p + n;
+        dd = in_d.p + 2 * n;
+        wm = warm ? in_d.p + 3 * n : nullptr;
+    }
+    DBuf<float> u_d;
+    DBuf<double> u64_d, ref64;
+    float *uu = u;
+    if (out_mem == LSH_HOST) {
+        u_d.alloc(n);
+        uu = u_d.p;
+    }
+    double *uu64 = u64;
+    if (mode != LSH_WLS_REFERENCE && (!u64 || out_mem == LSH_HOST)) {
+        u64_d.alloc(n);
+        uu64 = u64_d.p;
+    }
+    WlsPtr s = wls_create(g, (int)w, (int)h, lambda, sigma_c);
+    wls_set_weights(s.get(), c, nullptr, nullptr, 0.0f);
+    hipEvent_t e0, e1;
+    LSH_CHECK(hipEventCreate(&e0));
+    LSH_CHECK(hipEventCreate(&e1));
+    WlsResult r;
+    float ms = 0.0f;
+    try {
+        LSH_CHECK(hipEventRecord(e0, stream()));
+        r = wls_solve(s.get(), dd, max_iters, tol, (int)mode, wm, uu, uu64);
+        LSH_CHECK(hipEventRecord(e1, stream()));
+        LSH_CHECK(hipEventSynchronize(e1));
+        LSH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    double err = -1.0;
+    if (want_err && w > 1 && h > 1) {
+        if (mode == LSH_WLS_FP64) {
+            err = wls_max_abs_diff(s.get(), uu, uu64);
+        } else {
+            ref64.alloc(n);
+            DBuf<float> tmp(n);
+            wls_solve(s.get(), dd, 2000, 1e-12, WLS_FP64, nullptr, tmp.p, ref64.p);
+            err = wls_max_abs_diff(s.get(), uu, ref64.p);
+        }
+    }
+    double true_rel = (w > 1 && h > 1) ? wls_true_rel(s.get(), dd, uu) : 0.0;
+    if (out_mem == LSH_HOST) {
+        copy_mem(u, LSH_HOST, uu, LSH_DEVICE, n * sizeof(float));
+        if (u64) copy_mem(u64, LSH_HOST, uu64, LSH_DEVICE, n * sizeof(double));
+    } else if (u64 && uu64 != u64) {
+        copy_mem(u64, LSH_DEVICE, uu64, LSH_DEVICE, n * sizeof(double));
+    }
+    sync();
+    if (rep) {
+        rep->iters = r.iters;
+        rep->outer = r.outer;
+        rep->rel = r.rel;
+        rep->true_rel = true_rel;
+        rep->err_fp64 = err;
+        rep->ms = ms;
+    }
+    LSH_API_END
+}
+
+}  // extern "C"

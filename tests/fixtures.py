@@ -1,8 +1,10 @@
 """Procedural fixtures restated from the reference's own unit tests (they hold no
 data files): xorshift noise (pyramid.rs:235-250), the LCG (depth.rs:1206-1209),
 checkerboards (fuse.rs:547-558, depth.rs:1343-1354), repeated 5-tap blurs
-(fuse.rs:561-569, depth.rs:1356-1364)."""
+(fuse.rs:561-569, depth.rs:1356-1364); and the WLS system (W + lambda L) as a
+scipy sparse matrix in f64, the textbook form of depth.rs:653-672/786-810."""
 import numpy as np
+import scipy.sparse
 
 
 def noise_img(w, h, seed):
@@ -45,3 +47,28 @@ def blur(img3, passes):
 def rms(a, b):
     d = (a.astype(np.float32) - b.astype(np.float32)).astype(np.float64)
     return float(np.sqrt((d * d).mean()))
+
+
+def wls_matrix(guide, conf, lam, sigma):
+    h, w = guide.shape
+    g = guide.astype(np.float32)
+    inv = np.float32(-1.0) / np.float32(max(sigma, 1e-6))
+    ex = np.exp(np.abs(g[:, :-1] - g[:, 1:]) * inv).astype(np.float64) * lam
+    ey = np.exp(np.abs(g[:-1] - g[1:]) * inv).astype(np.float64) * lam
+    n = w * h
+    idx = np.arange(n).reshape(h, w)
+    rows, cols, vals = [], [], []
+    diag = np.maximum(conf.astype(np.float64), 0) + 1e-4
+    diag = diag.ravel().copy()
+    for (a, b, wgt) in [(idx[:, :-1], idx[:, 1:], ex), (idx[:-1], idx[1:], ey)]:
+        a, b, wgt = a.ravel(), b.ravel(), wgt.ravel()
+        rows += [a, b]
+        cols += [b, a]
+        vals += [-wgt, -wgt]
+        np.add.at(diag, a, wgt)
+        np.add.at(diag, b, wgt)
+    rows.append(np.arange(n))
+    cols.append(np.arange(n))
+    vals.append(diag)
+    A = scipy.sparse.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    return A

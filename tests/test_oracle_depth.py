@@ -9,7 +9,7 @@ import scipy.sparse
 import scipy.sparse.linalg
 
 from oracle import ref
-from tests.fixtures import blur, checker, lcg_seq
+from tests.fixtures import blur, checker, lcg_seq, wls_matrix
 
 
 def test_box_filter_matches_naive():  # depth.rs:1212-1230
@@ -141,31 +141,6 @@ def test_thomas_sweeps_match_solve_banded():
     assert np.allclose(uc.T, u, rtol=1e-5, atol=1e-6)
 
 
-def _wls_matrix(guide, conf, lam, sigma):
-    h, w = guide.shape
-    g = guide.astype(np.float32)
-    inv = np.float32(-1.0) / np.float32(max(sigma, 1e-6))
-    ex = np.exp(np.abs(g[:, :-1] - g[:, 1:]) * inv).astype(np.float64) * lam
-    ey = np.exp(np.abs(g[:-1] - g[1:]) * inv).astype(np.float64) * lam
-    n = w * h
-    idx = np.arange(n).reshape(h, w)
-    rows, cols, vals = [], [], []
-    diag = np.maximum(conf.astype(np.float64), 0) + 1e-4
-    diag = diag.ravel().copy()
-    for (a, b, wgt) in [(idx[:, :-1], idx[:, 1:], ex), (idx[:-1], idx[1:], ey)]:
-        a, b, wgt = a.ravel(), b.ravel(), wgt.ravel()
-        rows += [a, b]
-        cols += [b, a]
-        vals += [-wgt, -wgt]
-        np.add.at(diag, a, wgt)
-        np.add.at(diag, b, wgt)
-    rows.append(np.arange(n))
-    cols.append(np.arange(n))
-    vals.append(diag)
-    A = scipy.sparse.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
-    return A
-
-
 @pytest.mark.parametrize("w,h,lam", [(32, 24, 5.0), (64, 40, 3.0), (45, 33, 50.0)])
 def test_wls_matches_a_direct_f64_solve(w, h, lam):
     guide = lcg_seq(21 + w, w * h).reshape(h, w) * np.float32(0.3)
@@ -174,7 +149,7 @@ def test_wls_matches_a_direct_f64_solve(w, h, lam):
     d = (lcg_seq(23 + w, w * h).reshape(h, w) * np.float32(7.0)).astype(np.float32)
     u, rel, it = ref.wls_solve(d, conf, guide, lam, 0.04, 300, tol=1e-6)
     assert rel < 1e-6 and it > 0
-    A = _wls_matrix(guide, conf, lam, 0.04)
+    A = wls_matrix(guide, conf, lam, 0.04)
     b = (np.maximum(conf, 0) + np.float32(1e-4)).astype(np.float64).ravel() * d.astype(np.float64).ravel()
     want = scipy.sparse.linalg.spsolve(A.tocsc(), b).reshape(h, w)
     # the oracle's f32 operator and the f64 matrix agree: residual of the oracle's answer in f64
@@ -195,7 +170,7 @@ def test_vcycle_is_symmetric_and_apply_is_the_matrix():
     a, b = float((mx.astype(np.float64) * y).sum()), float((x.astype(np.float64) * my).sum())
     assert abs(a - b) < 1e-4 * max(abs(a), abs(b))
     assert float((mx.astype(np.float64) * x).sum()) > 0  # positive definite on x
-    A = _wls_matrix(guide, conf, 3.0, 0.04)
+    A = wls_matrix(guide, conf, 3.0, 0.04)
     assert np.allclose(s.apply(x).ravel(), A @ x.astype(np.float64).ravel(), atol=1e-5)
 
 
