@@ -88,6 +88,7 @@ typedef struct lsh_depth_info {
 
 typedef struct lsh_frames lsh_frames;
 typedef struct lsh_fuser lsh_fuser;
+typedef struct lsh_warm lsh_warm;
 
 /* ---- library ----------------------------------------------------------- */
 
@@ -220,6 +221,40 @@ char *lsh_wls_solve(const float *guide, const float *conf, const float *d, const
                     uint32_t w, uint32_t h, float lambda, float sigma_c, uint32_t mode, double tol,
                     uint32_t max_iters, uint32_t want_err, float *u, double *u64, lsh_mem out_mem,
                     lsh_wls_report *rep);
+
+/* ---- the learned warm start of the WLS solve ---------------------------- */
+
+/* No counterpart in the reference, whose PCG starts from its FGS sweeps
+ * (depth.rs:1019-1034): a bf16 convolutional encoder-decoder (channels
+ * 16/32/64/64, 3x3 convolutions, ReLU, 254,481 parameters) mapping the depth
+ * pass's grid planes (peak depth, p90-normalised confidence, guide luma;
+ * lsh_depth_grid) to a starting depth for lsh_wls_solve.  Inference and
+ * training run on the device (bf16 MFMA, f32 accumulation, f32 master weights,
+ * Adam).  Depth enters the network divided by dscale (the frame-index range,
+ * n - 1) and leaves multiplied by it; the WLS solution scales the same way.
+ *
+ * Parameters travel as one flat f32 array, layer by layer in network order
+ * (e0a, e0b, e1a, e1b, e2a, e2b, e3a, e3b, d2, d1, d0, out): the weights
+ * W[out][in][ky][kx], then the biases. */
+char *lsh_warm_new(uint64_t seed, lsh_warm **out);
+void lsh_warm_free(lsh_warm *net);
+char *lsh_warm_nparams(lsh_warm *net, uint32_t *count);
+char *lsh_warm_get(lsh_warm *net, float *params, uint32_t count);
+/* Load parameters (resets the optimiser state). */
+char *lsh_warm_set(lsh_warm *net, const float *params, uint32_t count);
+/* nimg images of w*h planes each (image-major) in in_mem; u (nimg*w*h) in out_mem. */
+char *lsh_warm_predict(lsh_warm *net, const float *depth, const float *conf, const float *guide, lsh_mem in_mem,
+                       uint32_t nimg, uint32_t w, uint32_t h, float dscale, float *u, lsh_mem out_mem);
+/* The batch's mean squared error of the normalised prediction against target
+ * (the WLS solution) and its gradient (host, count floats, nullable), from the
+ * device's backward pass. */
+char *lsh_warm_grads(lsh_warm *net, const float *depth, const float *conf, const float *guide, const float *target,
+                     lsh_mem in_mem, uint32_t nimg, uint32_t w, uint32_t h, float dscale, double *loss, float *grads,
+                     uint32_t count);
+/* One Adam step (beta 0.9 / 0.999) on the batch; loss before the step. */
+char *lsh_warm_train_step(lsh_warm *net, const float *depth, const float *conf, const float *guide,
+                          const float *target, lsh_mem in_mem, uint32_t nimg, uint32_t w, uint32_t h, float dscale,
+                          float lr, double *loss);
 
 #ifdef __cplusplus
 }

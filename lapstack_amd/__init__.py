@@ -11,6 +11,8 @@ lapstack_amd/liblsh.so (declared in include/lsh.h):
                               brightness, focus slice), ingest (registered stacks).
 * ``depth_from_slices``    -- depth.rs:1119-1200.
 * ``depth_grid``           -- depth.rs:1125-1159: the WLS system's inputs.
+* ``WarmNet``              -- the learned warm start of the WLS solve (no
+                              reference counterpart), trained on the device.
 * ``wls_solve``            -- depth.rs:1081 wls_solve, in three precisions
                               (reference FP32, FP64-refined FP32, FP64).
 * ``fuse_and_depth``       -- stack.rs:740-853: the per-frame loop over a stack.
@@ -32,7 +34,7 @@ from . import _lib
 from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
-           "depth_from_slices", "depth_grid", "wls_solve", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "depth_from_slices", "depth_grid", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -366,6 +368,83 @@ def depth_grid(slices, n: int, fused, params: Optional[DepthParams] = None, out_
                                   ptrs[0][1], C.byref(info)))
     del sk, fk
     return outs[0], outs[1], outs[2], {k: getattr(info, k) for k, _ in info._fields_}
+
+
+class WarmNet:
+    """The learned warm start of the WLS solve (include/lsh.h lsh_warm_*): a
+    bf16 convolutional encoder-decoder from the grid planes (peak depth,
+    confidence, guide) to a starting depth, run and trained on the device.
+    Planes are (h, w) or (nimg, h, w), numpy or device tensors."""
+
+    def __init__(self, seed: int = 0):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.lsh_warm_new(seed, C.byref(h)))
+        self._h = h
+        n = C.c_uint32()
+        _lib.check(self._lib.lsh_warm_nparams(h, C.byref(n)))
+        self.nparams = n.value
+
+    def get(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        _lib.check(self._lib.lsh_warm_get(self._h, p.ctypes.data_as(C.c_void_p), self.nparams))
+        return p
+
+    def set(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.nparams:
+            raise HipError(f"warm: {p.size} parameters, the network has {self.nparams}")
+        _lib.check(self._lib.lsh_warm_set(self._h, p.ctypes.data_as(C.c_void_p), self.nparams))
+
+    def save(self, path: str):
+        self.get().tofile(path)
+
+    def load(self, path: str):
+        self.set(np.fromfile(path, np.float32))
+
+    @staticmethod
+    def _shape(a):
+        s = tuple(a.shape)
+        return (1,) + s if len(s) == 2 else s
+
+    def _in(self, *planes):
+        shape = self._shape(planes[0])
+        bufs = [_buf(a, np.float32) for a in planes]
+        if any(self._shape(a) != shape for a in planes) or len({b[1] for b in bufs}) != 1:
+            raise HipError("warm: planes must share one shape and one memory")
+        return shape, bufs
+
+    def predict(self, d, conf, guide, dscale: float, out=None):
+        (nimg, h, w), bufs = self._in(d, conf, guide)
+        if out is None:
+            out = np.zeros(tuple(d.shape), np.float32)
+        op, omem, ok = _buf(out, np.float32)
+        _lib.check(self._lib.lsh_warm_predict(self._h, bufs[0][0], bufs[1][0], bufs[2][0], bufs[0][1], nimg, w, h,
+                                              float(dscale), op, omem))
+        del bufs, ok
+        return out
+
+    def grads(self, d, conf, guide, target, dscale: float):
+        (nimg, h, w), bufs = self._in(d, conf, guide, target)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_double()
+        _lib.check(self._lib.lsh_warm_grads(self._h, *[b[0] for b in bufs], bufs[0][1], nimg, w, h, float(dscale),
+                                            C.byref(loss), g.ctypes.data_as(C.c_void_p), self.nparams))
+        return loss.value, g
+
+    def train_step(self, d, conf, guide, target, dscale: float, lr: float = 1e-3) -> float:
+        (nimg, h, w), bufs = self._in(d, conf, guide, target)
+        loss = C.c_double()
+        _lib.check(self._lib.lsh_warm_train_step(self._h, *[b[0] for b in bufs], bufs[0][1], nimg, w, h,
+                                                 float(dscale), float(lr), C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lsh_warm_free(self._h)
+            self._h = None
+
+    __del__ = close
 
 
 WLS_MODES = {"reference": 0, "refined": 1, "fp64": 2}

@@ -69,6 +69,16 @@ SIGNATURES = [
                                    C.c_int, _p, C.POINTER(DepthInfoC)]),
     ("lsh_depth_grid", _p, [_p, C.c_int, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p, _p, C.c_int,
                             C.POINTER(DepthInfoC)]),
+    ("lsh_warm_new", _p, [C.c_uint64, C.POINTER(_p)]),
+    ("lsh_warm_free", None, [_p]),
+    ("lsh_warm_nparams", _p, [_p, C.POINTER(_u32)]),
+    ("lsh_warm_get", _p, [_p, _p, _u32]),
+    ("lsh_warm_set", _p, [_p, _p, _u32]),
+    ("lsh_warm_predict", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, _p, C.c_int]),
+    ("lsh_warm_grads", _p, [_p, _p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, C.POINTER(C.c_double), _p,
+                            _u32]),
+    ("lsh_warm_train_step", _p, [_p, _p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, C.c_float,
+                                 C.POINTER(C.c_double)]),
     ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
                            _p, _p, C.c_int, C.POINTER(WlsReportC)]),
 ]

@@ -1,0 +1,839 @@
+// warm.hip -- the learned warm start of the WLS solve: a small convolutional
+// encoder-decoder that maps the depth pass's grid planes (peak depth,
+// confidence, guide luma) to a starting depth for MG-PCG.  Trained and run on
+// gfx950 by the kernels here: implicit-GEMM 3x3 convolutions on bf16 MFMA with
+// f32 accumulation for the forward pass and both halves of the backward pass
+// (input gradient, weight gradient), 2x2 average pooling, nearest upsampling,
+// the MSE loss and Adam on f32 master weights.  No reference counterpart: the
+// reference starts PCG from its FGS sweeps (depth.rs:1019-1034).
+//
+// Layout.  Activations are bf16, channel-last ([image][y][x][channel]) with a
+// channel stride that may exceed the channel count, so the decoder's
+// concatenations are views of one buffer (the upsampled half written at
+// channel 0, the encoder's skip tensor at its offset).  Weights are bf16
+// [tap][out channel][in channel] (tap = 3*(dy+1) + dx+1), channel counts padded
+// to multiples of 16; the input-gradient pass runs the same kernel over the
+// flipped, transposed copy [8-tap][in][out].
+//
+// Network (channels): 3 -> e0 16,16 -> pool -> e1 32,32 -> pool -> e2 64,64 ->
+// pool -> e3 64,64 -> up ++ e2 -> d2 64 -> up ++ e1 -> d1 32 -> up ++ e0 -> d0
+// 16 -> out 1; ReLU after every conv but the last; the prediction is the
+// normalised peak depth plus the output.  254,481 parameters.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "api.hpp"
+#include "common.hpp"
+
+namespace lsh {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static uint16_t f2bf_host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// a channel-last bf16 tensor view: image b, pixel (y, x), channel c at
+// p[((b*H + y)*W + x)*cs + c]
+struct View {
+    uint16_t *p;
+    int H, W, C, cs;
+};
+
+// ------------------------------------------------------------------ conv
+// One workgroup = 4 waves = 64 consecutive pixels of one row; each wave owns
+// 16 pixels and all NT*16 output channels.  Per tap and per KC-channel slice
+// of the input: A (16 pixels x KC channels) from the input, B (KC x 16) from
+// the weights, one MFMA per output tile.  KC = 32 uses the 16x16x32 bf16 MFMA,
+// KC = 16 the 16x16x16 one.  Zero padding at the borders.
+template <int KC, int NT>
+__global__ __launch_bounds__(256) void k_conv(View in, const uint16_t *__restrict__ wt, const float *__restrict__ bias,
+                                              int cop, View out, float *__restrict__ out32, View mask, int relu) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * 64 + wv * 16;
+    const int H = in.H, W = in.W, cin = in.C;
+    const int m = lane & 15, kg = lane >> 4;
+    constexpr int PER = KC / 4;  // bf16 per lane per MFMA
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int px = x0 + m;
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = y + tap / 3 - 1, xx = px + tap % 3 - 1;
+        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const uint16_t *src = in.p + (((size_t)b * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * in.cs + PER * kg;
+        const uint16_t *wsrc = wt + ((size_t)tap * cop + m) * cin + PER * kg;
+        for (int ci0 = 0; ci0 < cin; ci0 += KC) {
+            if constexpr (KC == 16) {
+                s16x4 a = {0, 0, 0, 0};
+                if (ok) a = *(const s16x4 *)(src + ci0);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    s16x4 bw = *(const s16x4 *)(wsrc + (size_t)t * 16 * cin + ci0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[t], 0, 0, 0);
+                }
+            } else {
+                bf16x8 a;
+                if (ok) {
+                    a = *(const bf16x8 *)(src + ci0);
+                } else {
+                    uint4 z = {0u, 0u, 0u, 0u};
+                    a = __builtin_bit_cast(bf16x8, z);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    bf16x8 bw = *(const bf16x8 *)(wsrc + (size_t)t * 16 * cin + ci0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // lane holds D[pixel 4*kg + j][channel t*16 + m]
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int co = t * 16 + m;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int x = x0 + 4 * kg + j;
+            if (x >= W) continue;
+            float v = acc[t][j] + (bias ? bias[co] : 0.0f);
+            if (relu) v = fmaxf(v, 0.0f);
+            const size_t pix = ((size_t)b * H + y) * W + x;
+            if (out32) {
+                if (co == 0) out32[pix] = v;
+                continue;
+            }
+            if (co >= out.C) continue;
+            if (mask.p && !(bf2f(mask.p[pix * mask.cs + co]) > 0.0f)) v = 0.0f;
+            out.p[pix * out.cs + co] = f2bf(v);
+        }
+    }
+}
+
+// Weight gradient dW[tap][co][ci] = sum over pixels dz[p][co] * in[p + tap][ci]:
+// a GEMM with the pixels as the reduction.  A workgroup stages a 64-pixel row
+// segment of dz and the 3 x 66 input window in LDS and accumulates up to 16
+// output tiles (4 per wave) over its share of the segments; each workgroup
+// writes its partial tiles, a second kernel sums them in a fixed order.
+__global__ __launch_bounds__(256) void k_conv_dw(View in, View dz, int nimg, int ntiles, int ncit, float *__restrict__ part) {
+    extern __shared__ uint16_t sm[];
+    const int cin = in.C, cop = dz.C, H = in.H, W = in.W;
+    uint16_t *Zs = sm, *Xs = sm + 64 * cop;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, kg = lane >> 4;
+    const int nxc = (W + 63) / 64;
+    const long nch = (long)nimg * H * nxc;
+    const int ncot = cop / 16;
+    f32x4 acc[4];
+    int tl[4], tdy[4], tdx[4], tco[4], tci[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+        int t = blockIdx.y * 16 + wv * 4 + i;
+        tl[i] = t;
+        int tap = t / (ncot * ncit), r = t - tap * ncot * ncit;
+        tdy[i] = tap / 3 - 1;
+        tdx[i] = tap % 3 - 1;
+        tco[i] = r / ncit;
+        tci[i] = r - (r / ncit) * ncit;
+    }
+    for (long c = blockIdx.x; c < nch; c += gridDim.x) {
+        const int b = (int)(c / ((long)H * nxc)), rr = (int)(c - (long)b * H * nxc), y = rr / nxc,
+                  x0 = (rr - y * nxc) * 64;
+        __syncthreads();
+        for (int i = threadIdx.x; i < 64 * cop / 4; i += 256) {
+            int px = i / (cop / 4), c4 = i - px * (cop / 4), x = x0 + px;
+            uint2 v = {0u, 0u};
+            if (x < W) v = *(const uint2 *)(dz.p + (((size_t)b * H + y) * W + x) * dz.cs + 4 * c4);
+            *(uint2 *)(Zs + px * cop + 4 * c4) = v;
+        }
+        for (int i = threadIdx.x; i < 3 * 66 * cin / 4; i += 256) {
+            int r = i / (66 * cin / 4), q = i - r * (66 * cin / 4), px = q / (cin / 4), c4 = q - px * (cin / 4);
+            int yy = y + r - 1, xx = x0 + px - 1;
+            uint2 v = {0u, 0u};
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *(const uint2 *)(in.p + (((size_t)b * H + yy) * W + xx) * in.cs + 4 * c4);
+            *(uint2 *)(Xs + (r * 66 + px) * cin + 4 * c4) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (tl[i] >= ntiles) continue;
+            for (int kk = 0; kk < 64; kk += 16) {
+                s16x4 a, bb;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    int px = kk + 4 * kg + j;
+                    a[j] = (short)Zs[px * cop + tco[i] * 16 + m];
+                    bb[j] = (short)Xs[((tdy[i] + 1) * 66 + px + tdx[i] + 1) * cin + tci[i] * 16 + m];
+                }
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[i], 0, 0, 0);
+            }
+        }
+    }
+    // lane holds D[co 4*kg + j][ci m] of each tile
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (tl[i] >= ntiles) continue;
+        float *o = part + ((size_t)blockIdx.x * gridDim.y * 16 + tl[i]) * 256;
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[(4 * kg + j) * 16 + m] = acc[i][j];
+    }
+}
+
+// sum the partial tiles over the workgroups (in order) into g[tap][co][ci]
+__global__ void k_dw_reduce(const float *__restrict__ part, int nparts, int tpad, int ntiles, int ncot, int ncit,
+                            int cop, int cip, float *__restrict__ g) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntiles * 256) return;
+    int t = i >> 8, e = i & 255;
+    float s = 0.0f;
+    for (int p = 0; p < nparts; p++) s += part[((size_t)p * tpad + t) * 256 + e];
+    int tap = t / (ncot * ncit), r = t - tap * ncot * ncit, cot = r / ncit, cit = r - cot * ncit;
+    g[((size_t)tap * cop + cot * 16 + (e >> 4)) * cip + cit * 16 + (e & 15)] = s;
+}
+
+// bias gradient: per-workgroup channel sums of dz, then the ordered sum
+__global__ __launch_bounds__(256) void k_bias_part(View dz, int nimg, float *__restrict__ part) {
+    __shared__ float sh[256];
+    const int cop = dz.C, per = 256 / cop, c = threadIdx.x % cop, lane = threadIdx.x / cop;
+    const size_t npx = (size_t)nimg * dz.H * dz.W;
+    float s = 0.0f;
+    if (lane < per)
+        for (size_t p = (size_t)blockIdx.x * per + lane; p < npx; p += (size_t)gridDim.x * per)
+            s += bf2f(dz.p[p * dz.cs + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cop) {
+        float t = 0.0f;
+        for (int l = 0; l < per; l++) t += sh[l * cop + threadIdx.x];
+        part[(size_t)blockIdx.x * cop + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_bias_reduce(const float *__restrict__ part, int nparts, int cop, float *__restrict__ g) {
+    int c = threadIdx.x;
+    if (c >= cop) return;
+    float s = 0.0f;
+    for (int p = 0; p < nparts; p++) s += part[(size_t)p * cop + c];
+    g[c] = s;
+}
+
+// ------------------------------------------------------- pool, upsample, pack
+// 2x2 average over the cells inside the finer grid (ceil sizes)
+__global__ void k_pool(View in, View out, int nimg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = out.C;
+    size_t n = (size_t)nimg * out.H * out.W * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    size_t q = i / C;
+    int x = (int)(q % out.W), y = (int)((q / out.W) % out.H), b = (int)(q / ((size_t)out.W * out.H));
+    float s = 0.0f;
+    int cnt = 0;
+    for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+            int yy = 2 * y + dy, xx = 2 * x + dx;
+            if (yy < in.H && xx < in.W) {
+                s += bf2f(in.p[(((size_t)b * in.H + yy) * in.W + xx) * in.cs + c]);
+                cnt++;
+            }
+        }
+    out.p[(((size_t)b * out.H + y) * out.W + x) * out.cs + c] = f2bf(s / (float)cnt);
+}
+
+// d(fine) += d(pooled) / count, kept where the fine activation is positive
+__global__ void k_pool_bwd(View dp, View dfine, View fine, int nimg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = dfine.C;
+    size_t n = (size_t)nimg * dfine.H * dfine.W * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    size_t q = i / C;
+    int x = (int)(q % dfine.W), y = (int)((q / dfine.W) % dfine.H), b = (int)(q / ((size_t)dfine.W * dfine.H));
+    int Y = y / 2, X = x / 2;
+    int cnt = (min(2 * Y + 2, dfine.H) - 2 * Y) * (min(2 * X + 2, dfine.W) - 2 * X);
+    size_t fo = (((size_t)b * dfine.H + y) * dfine.W + x);
+    float v = bf2f(dfine.p[fo * dfine.cs + c]);
+    if (bf2f(fine.p[fo * fine.cs + c]) > 0.0f)
+        v += bf2f(dp.p[(((size_t)b * dp.H + Y) * dp.W + X) * dp.cs + c]) / (float)cnt;
+    dfine.p[fo * dfine.cs + c] = f2bf(v);
+}
+
+// nearest upsampling into the finer grid
+__global__ void k_up(View in, View out, int nimg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = in.C;
+    size_t n = (size_t)nimg * out.H * out.W * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    size_t q = i / C;
+    int x = (int)(q % out.W), y = (int)((q / out.W) % out.H), b = (int)(q / ((size_t)out.W * out.H));
+    out.p[(((size_t)b * out.H + y) * out.W + x) * out.cs + c] =
+        in.p[(((size_t)b * in.H + y / 2) * in.W + x / 2) * in.cs + c];
+}
+
+// its adjoint: the sum of the finer cells of each coarse cell
+__global__ void k_up_bwd(View dfine, View dcoarse, int nimg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = dcoarse.C;
+    size_t n = (size_t)nimg * dcoarse.H * dcoarse.W * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    size_t q = i / C;
+    int x = (int)(q % dcoarse.W), y = (int)((q / dcoarse.W) % dcoarse.H), b = (int)(q / ((size_t)dcoarse.W * dcoarse.H));
+    float s = 0.0f;
+    for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+            int yy = 2 * y + dy, xx = 2 * x + dx;
+            if (yy < dfine.H && xx < dfine.W) s += bf2f(dfine.p[(((size_t)b * dfine.H + yy) * dfine.W + xx) * dfine.cs + c]);
+        }
+    dcoarse.p[(((size_t)b * dcoarse.H + y) * dcoarse.W + x) * dcoarse.cs + c] = f2bf(s);
+}
+
+// the network's input: depth / dscale, confidence, guide; channels 3..15 zero
+__global__ void k_pack(const float *__restrict__ d, const float *__restrict__ conf, const float *__restrict__ g,
+                       size_t n, float inv_scale, View x) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint16_t *o = x.p + i * x.cs;
+    o[0] = f2bf(d[i] * inv_scale);
+    o[1] = f2bf(conf[i]);
+    o[2] = f2bf(g[i]);
+#pragma unroll
+    for (int c = 3; c < 16; c++) o[c] = 0;
+}
+
+// prediction (depth units): (d / dscale + out) * dscale
+__global__ void k_predict(const float *__restrict__ out32, const float *__restrict__ d, size_t n, float inv_scale,
+                          float scale, float *__restrict__ u) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = (d[i] * inv_scale + out32[i]) * scale;
+}
+
+// MSE loss over the pixels and its gradient into channel 0 of dout (bf16);
+// per-block f64 partial sums of the squared error
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, const float *__restrict__ d,
+                                              const float *__restrict__ t, size_t n, float inv_scale, float gscale,
+                                              View dout, double *__restrict__ part) {
+    __shared__ double sh[256];
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        float e = (d[i] * inv_scale + out32[i]) - t[i] * inv_scale;
+        s = (double)e * (double)e;
+        uint16_t *o = dout.p + i * dout.cs;
+        o[0] = f2bf(gscale * e);
+#pragma unroll
+        for (int c = 1; c < 16; c++) o[c] = 0;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Adam on the f32 master weights (bias-corrected step sizes from the host)
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       size_t n, float lr_t, float b1, float b2, float eps_t) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr_t * mi / (sqrtf(vi) + eps_t);
+}
+
+// bf16 copies of one layer's weights: forward [tap][co][ci], and the flipped
+// transpose [8 - tap][ci][co] for the input gradient
+__global__ void k_cast_w(const float *__restrict__ w, int cop, int cip, uint16_t *__restrict__ wf,
+                         uint16_t *__restrict__ wt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9 * cop * cip) return;
+    int tap = i / (cop * cip), r = i - tap * cop * cip, co = r / cip, ci = r - co * cip;
+    uint16_t v = f2bf(w[i]);
+    wf[i] = v;
+    wt[((size_t)(8 - tap) * cip + ci) * cop + co] = v;
+}
+
+// ------------------------------------------------------------------ host
+struct LayerSpec {
+    int cin, cout;
+    bool relu;
+};
+static const LayerSpec SPECS[12] = {{3, 16, true},   {16, 16, true},  {16, 32, true}, {32, 32, true},
+                                    {32, 64, true},  {64, 64, true},  {64, 64, true}, {64, 64, true},
+                                    {128, 64, true}, {96, 32, true},  {48, 16, true}, {16, 1, false}};
+enum { E0A, E0B, E1A, E1B, E2A, E2B, E3A, E3B, D2, D1, D0, OUT, NLAYER };
+
+static int pad16(int c) { return (c + 15) / 16 * 16; }
+
+struct Layer {
+    int cin, cout, cip, cop;
+    bool relu;
+    size_t w_off, b_off;  // into the padded master arrays
+    size_t n_nat;         // natural parameters (W[co][ci][ky][kx], then b[co])
+    DBuf<uint16_t> wf, wt;
+};
+
+struct Acts {
+    int nimg = 0, H[4] = {0, 0, 0, 0}, W[4] = {0, 0, 0, 0};
+    bool bwd = false;
+    DBuf<uint16_t> X0, A0a, C0, P0, A1a, C1, P1, A2a, C2, P2, A3a, B3, D2b, D1b, D0b;
+    DBuf<float> out32;
+    DBuf<uint16_t> dOUT, dD0, dC0, dD1, dC1, dD2, dC2, dB3, dA3a, dP2, dA2a, dP1, dA1a, dP0, dA0a;
+};
+
+}  // namespace lsh
+
+struct lsh_warm {
+    std::vector<lsh::Layer> L;
+    size_t npad = 0, nnat = 0;
+    lsh::DBuf<float> w32, g32, m, v;
+    uint64_t step = 0;
+    lsh::Acts A;
+    lsh::DBuf<float> part;
+    lsh::DBuf<double> lpart;
+    lsh::DBuf<float> dbuf, cbuf, gbuf, tbuf, ubuf;  // host-memory inputs staged on the device
+};
+
+namespace lsh {
+
+static View view(DBuf<uint16_t> &b, int H, int W, int C, int cs, int off = 0) { return View{b.p + off, H, W, C, cs}; }
+static View noview() { return View{nullptr, 0, 0, 0, 0}; }
+
+static double splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static void recast(lsh_warm &n) {
+    for (auto &l : n.L)
+        launch("warm_cast", k_cast_w, dim3(cdiv(9 * l.cop * l.cip, 256)), dim3(256), 0, (const float *)(n.w32.p + l.w_off),
+               l.cop, l.cip, l.wf.p, l.wt.p);
+}
+
+// natural <-> padded parameter layouts (host)
+static void nat_to_pad(const lsh_warm &n, const float *nat, std::vector<float> &pad) {
+    pad.assign(n.npad, 0.0f);
+    size_t o = 0;
+    for (auto &l : n.L) {
+        for (int co = 0; co < l.cout; co++)
+            for (int ci = 0; ci < l.cin; ci++)
+                for (int t = 0; t < 9; t++) pad[l.w_off + ((size_t)t * l.cop + co) * l.cip + ci] = nat[o++];
+        for (int co = 0; co < l.cout; co++) pad[l.b_off + co] = nat[o++];
+    }
+}
+static void pad_to_nat(const lsh_warm &n, const std::vector<float> &pad, float *nat) {
+    size_t o = 0;
+    for (auto &l : n.L) {
+        for (int co = 0; co < l.cout; co++)
+            for (int ci = 0; ci < l.cin; ci++)
+                for (int t = 0; t < 9; t++) nat[o++] = pad[l.w_off + ((size_t)t * l.cop + co) * l.cip + ci];
+        for (int co = 0; co < l.cout; co++) nat[o++] = pad[l.b_off + co];
+    }
+}
+
+static void ensure_acts(lsh_warm &n, int nimg, int H, int W, bool bwd) {
+    Acts &a = n.A;
+    if (a.nimg == nimg && a.H[0] == H && a.W[0] == W && (a.bwd || !bwd)) return;
+    a = Acts();
+    a.nimg = nimg;
+    a.bwd = bwd;
+    for (int l = 0; l < 4; l++) {
+        a.H[l] = l ? (a.H[l - 1] + 1) / 2 : H;
+        a.W[l] = l ? (a.W[l - 1] + 1) / 2 : W;
+    }
+    auto px = [&](int l) { return (size_t)nimg * a.H[l] * a.W[l]; };
+    a.X0.alloc(px(0) * 16);
+    a.A0a.alloc(px(0) * 16);
+    a.C0.alloc(px(0) * 48);
+    a.P0.alloc(px(1) * 16);
+    a.A1a.alloc(px(1) * 32);
+    a.C1.alloc(px(1) * 96);
+    a.P1.alloc(px(2) * 32);
+    a.A2a.alloc(px(2) * 64);
+    a.C2.alloc(px(2) * 128);
+    a.P2.alloc(px(3) * 64);
+    a.A3a.alloc(px(3) * 64);
+    a.B3.alloc(px(3) * 64);
+    a.D2b.alloc(px(2) * 64);
+    a.D1b.alloc(px(1) * 32);
+    a.D0b.alloc(px(0) * 16);
+    a.out32.alloc(px(0));
+    if (bwd) {
+        a.dOUT.alloc(px(0) * 16);
+        a.dD0.alloc(px(0) * 16);
+        a.dC0.alloc(px(0) * 48);
+        a.dD1.alloc(px(1) * 32);
+        a.dC1.alloc(px(1) * 96);
+        a.dD2.alloc(px(2) * 64);
+        a.dC2.alloc(px(2) * 128);
+        a.dB3.alloc(px(3) * 64);
+        a.dA3a.alloc(px(3) * 64);
+        a.dP2.alloc(px(3) * 64);
+        a.dA2a.alloc(px(2) * 64);
+        a.dP1.alloc(px(2) * 32);
+        a.dA1a.alloc(px(1) * 32);
+        a.dP0.alloc(px(1) * 16);
+        a.dA0a.alloc(px(0) * 16);
+    }
+}
+
+template <int KC>
+static void conv_kc(const View &in, const uint16_t *w, const float *bias, int cop, const View &out, float *out32,
+                    const View &mask, bool relu, int nimg) {
+    dim3 grid(cdiv(in.W, 64), in.H, nimg), block(256);
+    double bytes = 2.0 * nimg * in.H * in.W * (in.C + (out32 ? 2 : out.C)) + 2.0 * 9 * cop * in.C;
+    switch (cop / 16) {
+    case 1: launch_b("warm_conv", bytes, k_conv<KC, 1>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
+    case 2: launch_b("warm_conv", bytes, k_conv<KC, 2>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
+    case 3: launch_b("warm_conv", bytes, k_conv<KC, 3>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
+    case 4: launch_b("warm_conv", bytes, k_conv<KC, 4>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
+    case 6: launch_b("warm_conv", bytes, k_conv<KC, 6>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
+    case 8: launch_b("warm_conv", bytes, k_conv<KC, 8>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
+    default: throw Error("warm: unsupported output width");
+    }
+}
+
+static void conv(const View &in, const uint16_t *w, const float *bias, int cop, const View &out, float *out32,
+                 const View &mask, bool relu, int nimg) {
+    LSH_REQUIRE(in.C % 16 == 0 && in.cs % 16 == 0, "warm: input channels not a multiple of 16");
+    if (in.C % 32 == 0 && in.cs % 8 == 0)
+        conv_kc<32>(in, w, bias, cop, out, out32, mask, relu, nimg);
+    else
+        conv_kc<16>(in, w, bias, cop, out, out32, mask, relu, nimg);
+}
+
+static void ew(const char *name, size_t n, void (*k)(View, View, int), View a, View b, int nimg) {
+    launch(name, k, dim3(cdiv(n, 256)), dim3(256), 0, a, b, nimg);
+}
+
+static void forward(lsh_warm &n, const float *d, const float *conf, const float *g, int nimg, float dscale) {
+    Acts &a = n.A;
+    const int *H = a.H, *W = a.W;
+    auto &L = n.L;
+    size_t n0 = (size_t)nimg * H[0] * W[0];
+    View X0 = view(a.X0, H[0], W[0], 16, 16);
+    launch("warm_pack", k_pack, dim3(cdiv(n0, 256)), dim3(256), 0, d, conf, g, n0, 1.0f / dscale, X0);
+    const float *b32 = n.w32.p;
+    auto B = [&](int l) { return b32 + L[l].b_off; };
+    View A0a = view(a.A0a, H[0], W[0], 16, 16), S0 = view(a.C0, H[0], W[0], 16, 48, 32);
+    View C0 = view(a.C0, H[0], W[0], 48, 48), U0 = view(a.C0, H[0], W[0], 32, 48, 0);
+    View P0 = view(a.P0, H[1], W[1], 16, 16), A1a = view(a.A1a, H[1], W[1], 32, 32);
+    View S1 = view(a.C1, H[1], W[1], 32, 96, 64), C1 = view(a.C1, H[1], W[1], 96, 96), U1 = view(a.C1, H[1], W[1], 64, 96, 0);
+    View P1 = view(a.P1, H[2], W[2], 32, 32), A2a = view(a.A2a, H[2], W[2], 64, 64);
+    View S2 = view(a.C2, H[2], W[2], 64, 128, 64), C2 = view(a.C2, H[2], W[2], 128, 128), U2 = view(a.C2, H[2], W[2], 64, 128, 0);
+    View P2 = view(a.P2, H[3], W[3], 64, 64), A3a = view(a.A3a, H[3], W[3], 64, 64), B3 = view(a.B3, H[3], W[3], 64, 64);
+    View D2v = view(a.D2b, H[2], W[2], 64, 64), D1v = view(a.D1b, H[1], W[1], 32, 32), D0v = view(a.D0b, H[0], W[0], 16, 16);
+    View none = noview();
+    auto px = [&](int l) { return (size_t)nimg * H[l] * W[l]; };
+    conv(X0, L[E0A].wf.p, B(E0A), L[E0A].cop, A0a, nullptr, none, true, nimg);
+    conv(A0a, L[E0B].wf.p, B(E0B), L[E0B].cop, S0, nullptr, none, true, nimg);
+    ew("warm_pool", px(1) * 16, k_pool, S0, P0, nimg);
+    conv(P0, L[E1A].wf.p, B(E1A), L[E1A].cop, A1a, nullptr, none, true, nimg);
+    conv(A1a, L[E1B].wf.p, B(E1B), L[E1B].cop, S1, nullptr, none, true, nimg);
+    ew("warm_pool", px(2) * 32, k_pool, S1, P1, nimg);
+    conv(P1, L[E2A].wf.p, B(E2A), L[E2A].cop, A2a, nullptr, none, true, nimg);
+    conv(A2a, L[E2B].wf.p, B(E2B), L[E2B].cop, S2, nullptr, none, true, nimg);
+    ew("warm_pool", px(3) * 64, k_pool, S2, P2, nimg);
+    conv(P2, L[E3A].wf.p, B(E3A), L[E3A].cop, A3a, nullptr, none, true, nimg);
+    conv(A3a, L[E3B].wf.p, B(E3B), L[E3B].cop, B3, nullptr, none, true, nimg);
+    ew("warm_up", px(2) * 64, k_up, B3, U2, nimg);
+    conv(C2, L[D2].wf.p, B(D2), L[D2].cop, D2v, nullptr, none, true, nimg);
+    ew("warm_up", px(1) * 64, k_up, D2v, U1, nimg);
+    conv(C1, L[D1].wf.p, B(D1), L[D1].cop, D1v, nullptr, none, true, nimg);
+    ew("warm_up", px(0) * 32, k_up, D1v, U0, nimg);
+    conv(C0, L[D0].wf.p, B(D0), L[D0].cop, D0v, nullptr, none, true, nimg);
+    View dummy = View{nullptr, H[0], W[0], 1, 1};
+    conv(D0v, L[OUT].wf.p, B(OUT), L[OUT].cop, dummy, a.out32.p, none, false, nimg);
+}
+
+static void weight_grad(lsh_warm &n, int l, const View &in, const View &dz, int nimg) {
+    Layer &ly = n.L[l];
+    const int ncot = ly.cop / 16, ncit = ly.cip / 16, ntiles = 9 * ncot * ncit, groups = cdiv(ntiles, 16);
+    LSH_REQUIRE(in.C == ly.cip && dz.C == ly.cop, "warm: weight-gradient shapes");
+    const long nch = (long)nimg * in.H * cdiv(in.W, 64);
+    const int parts = (int)std::min<long>(nch, 128);
+    const size_t npx = (size_t)nimg * dz.H * dz.W;
+    const int bparts = (int)std::min<size_t>(cdiv(npx, 256), 256);
+    n.part.ensure(std::max((size_t)parts * groups * 16 * 256, (size_t)bparts * ly.cop));
+    size_t shm = (size_t)(64 * ly.cop + 3 * 66 * ly.cip) * sizeof(uint16_t);
+    launch_b("warm_conv_dw", 2.0 * nimg * in.H * in.W * (in.C + dz.C), k_conv_dw, dim3(parts, groups), dim3(256), shm,
+             in, dz, nimg, ntiles, ncit, n.part.p);
+    launch("warm_dw_reduce", k_dw_reduce, dim3(cdiv(ntiles * 256, 256)), dim3(256), 0, (const float *)n.part.p, parts,
+           groups * 16, ntiles, ncot, ncit, ly.cop, ly.cip, n.g32.p + ly.w_off);
+    launch("warm_bias_grad", k_bias_part, dim3(bparts), dim3(256), 0, dz, nimg, n.part.p);
+    launch("warm_bias_grad", k_bias_reduce, dim3(1), dim3(128), 0, (const float *)n.part.p, bparts, ly.cop,
+           n.g32.p + ly.b_off);
+}
+
+// loss and gradients of the batch (forward first); returns the mean squared error
+static double backward(lsh_warm &n, const float *d, const float *t, int nimg, float dscale) {
+    Acts &a = n.A;
+    const int *H = a.H, *W = a.W;
+    auto &L = n.L;
+    const size_t n0 = (size_t)nimg * H[0] * W[0];
+    auto px = [&](int l) { return (size_t)nimg * H[l] * W[l]; };
+    View dOUT = view(a.dOUT, H[0], W[0], 16, 16);
+    const int lb = (int)cdiv(n0, 256);
+    n.lpart.ensure(lb);
+    launch("warm_loss", k_loss, dim3(lb), dim3(256), 0, (const float *)a.out32.p, d, t, n0, 1.0f / dscale,
+           2.0f / (float)n0, dOUT, n.lpart.p);
+    View X0 = view(a.X0, H[0], W[0], 16, 16);
+    View A0a = view(a.A0a, H[0], W[0], 16, 16), S0 = view(a.C0, H[0], W[0], 16, 48, 32), C0 = view(a.C0, H[0], W[0], 48, 48);
+    View P0 = view(a.P0, H[1], W[1], 16, 16), A1a = view(a.A1a, H[1], W[1], 32, 32);
+    View S1 = view(a.C1, H[1], W[1], 32, 96, 64), C1 = view(a.C1, H[1], W[1], 96, 96);
+    View P1 = view(a.P1, H[2], W[2], 32, 32), A2a = view(a.A2a, H[2], W[2], 64, 64);
+    View S2 = view(a.C2, H[2], W[2], 64, 128, 64), C2 = view(a.C2, H[2], W[2], 128, 128);
+    View P2 = view(a.P2, H[3], W[3], 64, 64), A3a = view(a.A3a, H[3], W[3], 64, 64);
+    // the decoder's own outputs need no mask here: their ReLU masks act through
+    // the concatenations they were upsampled into (C0, C1) and D0 directly
+    View D0v = view(a.D0b, H[0], W[0], 16, 16);
+    View dD0 = view(a.dD0, H[0], W[0], 16, 16), dC0 = view(a.dC0, H[0], W[0], 48, 48);
+    View dU0 = view(a.dC0, H[0], W[0], 32, 48, 0), dS0 = view(a.dC0, H[0], W[0], 16, 48, 32);
+    View dD1 = view(a.dD1, H[1], W[1], 32, 32), dC1 = view(a.dC1, H[1], W[1], 96, 96);
+    View dU1 = view(a.dC1, H[1], W[1], 64, 96, 0), dS1 = view(a.dC1, H[1], W[1], 32, 96, 64);
+    View dD2 = view(a.dD2, H[2], W[2], 64, 64), dC2 = view(a.dC2, H[2], W[2], 128, 128);
+    View dU2 = view(a.dC2, H[2], W[2], 64, 128, 0), dS2 = view(a.dC2, H[2], W[2], 64, 128, 64);
+    View dB3 = view(a.dB3, H[3], W[3], 64, 64), dA3a = view(a.dA3a, H[3], W[3], 64, 64), dP2 = view(a.dP2, H[3], W[3], 64, 64);
+    View dA2a = view(a.dA2a, H[2], W[2], 64, 64), dP1 = view(a.dP1, H[2], W[2], 32, 32);
+    View dA1a = view(a.dA1a, H[1], W[1], 32, 32), dP0 = view(a.dP0, H[1], W[1], 16, 16), dA0a = view(a.dA0a, H[0], W[0], 16, 16);
+    View none = noview();
+    // out layer (no ReLU): dz = dOUT
+    weight_grad(n, OUT, D0v, dOUT, nimg);
+    conv(dOUT, L[OUT].wt.p, nullptr, L[OUT].cip, dD0, nullptr, D0v, false, nimg);
+    weight_grad(n, D0, C0, dD0, nimg);
+    conv(dD0, L[D0].wt.p, nullptr, L[D0].cip, dC0, nullptr, C0, false, nimg);
+    ew("warm_up_bwd", px(1) * 32, k_up_bwd, dU0, dD1, nimg);
+    weight_grad(n, D1, C1, dD1, nimg);
+    conv(dD1, L[D1].wt.p, nullptr, L[D1].cip, dC1, nullptr, C1, false, nimg);
+    ew("warm_up_bwd", px(2) * 64, k_up_bwd, dU1, dD2, nimg);
+    weight_grad(n, D2, C2, dD2, nimg);
+    conv(dD2, L[D2].wt.p, nullptr, L[D2].cip, dC2, nullptr, C2, false, nimg);
+    ew("warm_up_bwd", px(3) * 64, k_up_bwd, dU2, dB3, nimg);
+    weight_grad(n, E3B, A3a, dB3, nimg);
+    conv(dB3, L[E3B].wt.p, nullptr, L[E3B].cip, dA3a, nullptr, A3a, false, nimg);
+    weight_grad(n, E3A, P2, dA3a, nimg);
+    conv(dA3a, L[E3A].wt.p, nullptr, L[E3A].cip, dP2, nullptr, none, false, nimg);
+    launch("warm_pool_bwd", k_pool_bwd, dim3(cdiv(px(2) * 64, 256)), dim3(256), 0, dP2, dS2, S2, nimg);
+    weight_grad(n, E2B, A2a, dS2, nimg);
+    conv(dS2, L[E2B].wt.p, nullptr, L[E2B].cip, dA2a, nullptr, A2a, false, nimg);
+    weight_grad(n, E2A, P1, dA2a, nimg);
+    conv(dA2a, L[E2A].wt.p, nullptr, L[E2A].cip, dP1, nullptr, none, false, nimg);
+    launch("warm_pool_bwd", k_pool_bwd, dim3(cdiv(px(1) * 32, 256)), dim3(256), 0, dP1, dS1, S1, nimg);
+    weight_grad(n, E1B, A1a, dS1, nimg);
+    conv(dS1, L[E1B].wt.p, nullptr, L[E1B].cip, dA1a, nullptr, A1a, false, nimg);
+    weight_grad(n, E1A, P0, dA1a, nimg);
+    conv(dA1a, L[E1A].wt.p, nullptr, L[E1A].cip, dP0, nullptr, none, false, nimg);
+    launch("warm_pool_bwd", k_pool_bwd, dim3(cdiv(px(0) * 16, 256)), dim3(256), 0, dP0, dS0, S0, nimg);
+    weight_grad(n, E0B, A0a, dS0, nimg);
+    conv(dS0, L[E0B].wt.p, nullptr, L[E0B].cip, dA0a, nullptr, A0a, false, nimg);
+    weight_grad(n, E0A, X0, dA0a, nimg);
+    std::vector<double> lp(lb);
+    copy_mem(lp.data(), LSH_HOST, n.lpart.p, LSH_DEVICE, lb * sizeof(double));
+    double s = 0.0;
+    for (double v : lp) s += v;
+    return s / (double)n0;
+}
+
+struct Inputs {
+    const float *d, *conf, *g, *t;
+};
+
+// device pointers of the caller's planes (staged when in host memory)
+static Inputs stage(lsh_warm &n, const float *d, const float *conf, const float *g, const float *t, lsh_mem mem,
+                    size_t count) {
+    if (mem == LSH_DEVICE) return Inputs{d, conf, g, t};
+    n.dbuf.ensure(count);
+    n.cbuf.ensure(count);
+    n.gbuf.ensure(count);
+    copy_mem(n.dbuf.p, LSH_DEVICE, d, LSH_HOST, count * sizeof(float));
+    copy_mem(n.cbuf.p, LSH_DEVICE, conf, LSH_HOST, count * sizeof(float));
+    copy_mem(n.gbuf.p, LSH_DEVICE, g, LSH_HOST, count * sizeof(float));
+    if (t) {
+        n.tbuf.ensure(count);
+        copy_mem(n.tbuf.p, LSH_DEVICE, t, LSH_HOST, count * sizeof(float));
+    }
+    return Inputs{n.dbuf.p, n.cbuf.p, n.gbuf.p, t ? n.tbuf.p : nullptr};
+}
+
+}  // namespace lsh
+
+using namespace lsh;
+
+extern "C" {
+
+char *lsh_warm_new(uint64_t seed, lsh_warm **out) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(out, "lsh_warm_new: null argument");
+    std::unique_ptr<lsh_warm> n(new lsh_warm());
+    size_t off = 0, nat = 0;
+    for (int i = 0; i < NLAYER; i++) {
+        Layer l;
+        l.cin = SPECS[i].cin;
+        l.cout = SPECS[i].cout;
+        l.cip = pad16(l.cin);
+        l.cop = pad16(l.cout);
+        l.relu = SPECS[i].relu;
+        l.w_off = off;
+        off += (size_t)9 * l.cop * l.cip;
+        l.b_off = off;
+        off += l.cop;
+        l.n_nat = (size_t)9 * l.cin * l.cout + l.cout;
+        nat += l.n_nat;
+        l.wf.alloc((size_t)9 * l.cop * l.cip);
+        l.wt.alloc((size_t)9 * l.cop * l.cip);
+        n->L.push_back(std::move(l));
+    }
+    n->npad = off;
+    n->nnat = nat;
+    for (auto *b : {&n->w32, &n->g32, &n->m, &n->v}) {
+        b->alloc(off);
+        b->zero();
+    }
+    // He-normal weights (std sqrt(2 / (9 cin))), zero biases; the last layer
+    // starts at zero so the first prediction is the peak depth itself
+    std::vector<float> natw(nat, 0.0f);
+    uint64_t s = seed;
+    size_t o = 0;
+    for (auto &l : n->L) {
+        double sd = (&l == &n->L.back()) ? 0.0 : std::sqrt(2.0 / (9.0 * l.cin));
+        for (size_t i = 0; i < (size_t)9 * l.cin * l.cout; i++) {
+            double u1 = splitmix(s), u2 = splitmix(s);
+            natw[o++] = (float)(sd * std::sqrt(-2.0 * std::log(u1 + 1e-300)) * std::cos(6.283185307179586 * u2));
+        }
+        o += l.cout;
+    }
+    std::vector<float> pad;
+    nat_to_pad(*n, natw.data(), pad);
+    copy_mem(n->w32.p, LSH_DEVICE, pad.data(), LSH_HOST, pad.size() * sizeof(float));
+    recast(*n);
+    sync();
+    *out = n.release();
+    LSH_API_END
+}
+
+void lsh_warm_free(lsh_warm *n) { delete n; }
+
+char *lsh_warm_nparams(lsh_warm *n, uint32_t *count) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n && count, "lsh_warm_nparams: null argument");
+    *count = (uint32_t)n->nnat;
+    LSH_API_END
+}
+
+char *lsh_warm_get(lsh_warm *n, float *params, uint32_t count) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n && params, "lsh_warm_get: null argument");
+    LSH_REQUIRE(count == n->nnat, "warm: parameter count mismatch");
+    std::vector<float> pad(n->npad);
+    copy_mem(pad.data(), LSH_HOST, n->w32.p, LSH_DEVICE, n->npad * sizeof(float));
+    pad_to_nat(*n, pad, params);
+    LSH_API_END
+}
+
+char *lsh_warm_set(lsh_warm *n, const float *params, uint32_t count) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n && params, "lsh_warm_set: null argument");
+    LSH_REQUIRE(count == n->nnat, "warm: parameter count mismatch");
+    std::vector<float> pad;
+    nat_to_pad(*n, params, pad);
+    copy_mem(n->w32.p, LSH_DEVICE, pad.data(), LSH_HOST, n->npad * sizeof(float));
+    n->m.zero();
+    n->v.zero();
+    n->step = 0;
+    recast(*n);
+    sync();
+    LSH_API_END
+}
+
+char *lsh_warm_predict(lsh_warm *n, const float *d, const float *conf, const float *guide, lsh_mem in_mem,
+                       uint32_t nimg, uint32_t w, uint32_t h, float dscale, float *u, lsh_mem out_mem) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n && d && conf && guide && u, "lsh_warm_predict: null argument");
+    LSH_REQUIRE(nimg > 0 && w > 0 && h > 0, "warm: empty batch");
+    LSH_REQUIRE(dscale > 0.0f, "warm: dscale must be positive");
+    const size_t cnt = (size_t)nimg * w * h;
+    Inputs in = stage(*n, d, conf, guide, nullptr, in_mem, cnt);
+    ensure_acts(*n, (int)nimg, (int)h, (int)w, false);
+    forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
+    float *uo = u;
+    if (out_mem == LSH_HOST) {
+        n->ubuf.ensure(cnt);
+        uo = n->ubuf.p;
+    }
+    launch("warm_predict", k_predict, dim3(cdiv(cnt, 256)), dim3(256), 0, (const float *)n->A.out32.p, in.d, cnt,
+           1.0f / dscale, dscale, uo);
+    if (out_mem == LSH_HOST) copy_mem(u, LSH_HOST, uo, LSH_DEVICE, cnt * sizeof(float));
+    sync();
+    LSH_API_END
+}
+
+char *lsh_warm_grads(lsh_warm *n, const float *d, const float *conf, const float *guide, const float *target,
+                     lsh_mem in_mem, uint32_t nimg, uint32_t w, uint32_t h, float dscale, double *loss, float *grads,
+                     uint32_t count) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n && d && conf && guide && target, "lsh_warm_grads: null argument");
+    LSH_REQUIRE(nimg > 0 && w > 0 && h > 0, "warm: empty batch");
+    LSH_REQUIRE(dscale > 0.0f, "warm: dscale must be positive");
+    LSH_REQUIRE(!grads || count == n->nnat, "warm: parameter count mismatch");
+    const size_t cnt = (size_t)nimg * w * h;
+    Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
+    ensure_acts(*n, (int)nimg, (int)h, (int)w, true);
+    forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
+    double l = backward(*n, in.d, in.t, (int)nimg, dscale);
+    if (loss) *loss = l;
+    if (grads) {
+        std::vector<float> pad(n->npad);
+        copy_mem(pad.data(), LSH_HOST, n->g32.p, LSH_DEVICE, n->npad * sizeof(float));
+        pad_to_nat(*n, pad, grads);
+    }
+    LSH_API_END
+}
+
+char *lsh_warm_train_step(lsh_warm *n, const float *d, const float *conf, const float *guide, const float *target,
+                          lsh_mem in_mem, uint32_t nimg, uint32_t w, uint32_t h, float dscale, float lr, double *loss) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n && d && conf && guide && target, "lsh_warm_train_step: null argument");
+    LSH_REQUIRE(nimg > 0 && w > 0 && h > 0, "warm: empty batch");
+    LSH_REQUIRE(dscale > 0.0f, "warm: dscale must be positive");
+    const size_t cnt = (size_t)nimg * w * h;
+    Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
+    ensure_acts(*n, (int)nimg, (int)h, (int)w, true);
+    forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
+    double l = backward(*n, in.d, in.t, (int)nimg, dscale);
+    n->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    double c1 = 1.0 - std::pow((double)b1, (double)n->step), c2 = 1.0 - std::pow((double)b2, (double)n->step);
+    float lr_t = (float)(lr * std::sqrt(c2) / c1), eps_t = (float)(1e-8 * std::sqrt(c2));
+    launch("warm_adam", k_adam, dim3(cdiv(n->npad, 256)), dim3(256), 0, n->w32.p, (const float *)n->g32.p, n->m.p,
+           n->v.p, n->npad, lr_t, b1, b2, eps_t);
+    recast(*n);
+    if (loss) *loss = l;
+    LSH_API_END
+}
+
+}  // extern "C"

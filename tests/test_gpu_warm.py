@@ -1,0 +1,114 @@
+"""The learned warm start's device kernels (lsh_warm_*) against the CPU oracle
+(oracle/warm_net.py), which rounds to bf16 where the device stores bf16.
+
+Bars: the prediction within 1e-2 of the normalised range at every pixel (and
+1e-3 in the mean: the device and the oracle differ only where f32 sums in
+another order round an activation to the neighbouring bf16 value); the loss
+within 1e-3 relative; every layer's weight gradient within 3e-2 relative
+(L2) and the whole gradient at cosine > 0.999; training lowers the loss."""
+import numpy as np
+import pytest
+
+import lapstack_amd as la
+from oracle import warm_net as wn
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    why = la.available()
+    if why is not None:
+        pytest.fail(f"HIP backend unavailable on a GPU run: {why}")
+
+
+def _batch(B, H, W, seed, dscale=7.0):
+    """Peak-depth-like planes: a smooth field plus noise, confidence with a
+    hole, a guide; the target is the field itself (what the smoothing should
+    recover)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:H, 0:W]
+    out = []
+    for b in range(B):
+        f = dscale * (0.5 + 0.3 * np.sin(x / (5.0 + b)) * np.cos(y / 7.0) + 0.1 * x / max(W, 1))
+        d = f + rng.normal(0, 0.4, (H, W))
+        conf = rng.uniform(0, 1.2, (H, W))
+        conf[H // 3: H // 2, W // 4: W // 2] = 0.0
+        g = 0.5 + 0.2 * np.sin(x / 3.0) + rng.normal(0, 0.02, (H, W))
+        out.append((d, conf, g, f))
+    return [np.stack([o[i] for o in out]).astype(np.float32) for i in range(4)]
+
+
+@pytest.mark.parametrize("B,H,W", [(2, 37, 70), (1, 64, 64), (3, 9, 130)])
+def test_forward_matches_oracle(B, H, W):
+    print(f"warm forward {B}x{H}x{W}", flush=True)
+    net = la.WarmNet()
+    p = wn.init(7)
+    net.set(p)
+    d, conf, g, _ = _batch(B, H, W, 1)
+    u = net.predict(d, conf, g, 7.0)
+    want, _ = wn.forward(p, d, conf, g, 7.0, emulate=True)
+    err = np.abs(u / 7.0 - want)
+    rng = float(np.abs(want).max())
+    assert err.max() <= 1e-2 * rng, err.max()
+    assert err.mean() <= 1e-3 * rng, err.mean()
+    net.close()
+
+
+@pytest.mark.parametrize("B,H,W", [(2, 37, 70), (1, 40, 33)])
+def test_gradients_match_oracle(B, H, W):
+    print(f"warm gradients {B}x{H}x{W}", flush=True)
+    net = la.WarmNet()
+    p = wn.init(11)
+    net.set(p)
+    d, conf, g, t = _batch(B, H, W, 2)
+    loss, grad = net.grads(d, conf, g, t, 7.0)
+    wloss, wgrad = wn.loss_and_grads(p, d, conf, g, t, 7.0, emulate=True)
+    assert loss == pytest.approx(wloss, rel=1e-3)
+    offs = np.cumsum([0] + [ci * co * 9 + co for ci, co, _ in wn.SPECS])
+    for li, name in enumerate(wn.NAMES):
+        a, b = grad[offs[li]:offs[li + 1]].astype(np.float64), wgrad[offs[li]:offs[li + 1]]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel < 3e-2, f"layer {name}: relative gradient error {rel:.3e}"
+    cos = float(grad @ wgrad / (np.linalg.norm(grad) * np.linalg.norm(wgrad)))
+    assert cos > 0.999, cos
+    net.close()
+
+
+def test_training_lowers_the_loss():
+    print("warm training", flush=True)
+    net = la.WarmNet(seed=3)
+    d, conf, g, t = _batch(4, 64, 96, 3)
+    first = net.train_step(d, conf, g, t, 7.0, lr=2e-3)
+    for _ in range(150):
+        last = net.train_step(d, conf, g, t, 7.0, lr=2e-3)
+    after, _ = net.grads(d, conf, g, t, 7.0)
+    assert after < 0.5 * first, (first, last, after)
+    # the parameters round-trip through get/set, and set resets to the same function
+    p = net.get()
+    u1 = net.predict(d, conf, g, 7.0)
+    net2 = la.WarmNet(seed=99)
+    net2.set(p)
+    assert np.array_equal(net2.predict(d, conf, g, 7.0), u1)
+    net.close()
+    net2.close()
+
+
+def test_device_planes_and_errors():
+    import torch
+
+    net = la.WarmNet()
+    net.set(wn.init(5))
+    d, conf, g, _ = _batch(1, 33, 47, 4)
+    host = net.predict(d[0], conf[0], g[0], 7.0)
+    dev = [torch.from_numpy(a[0]).cuda() for a in (d, conf, g)]
+    out = torch.zeros((33, 47), dtype=torch.float32, device="cuda")
+    net.predict(*dev, 7.0, out=out)
+    assert np.array_equal(out.cpu().numpy(), host)
+    with pytest.raises(la.HipError):
+        net.predict(d, conf[:, :5], g, 7.0)
+    with pytest.raises(la.HipError):
+        net.set(np.zeros(10, np.float32))
+    with pytest.raises(la.HipError):
+        net.predict(d, conf, g, 0.0)
+    net.close()
