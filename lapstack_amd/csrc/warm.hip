@@ -56,73 +56,96 @@ struct View {
 };
 
 // ------------------------------------------------------------------ conv
-// One workgroup = 4 waves = 64 consecutive pixels of one row; each wave owns
-// 16 pixels and all NT*16 output channels.  Per tap and per KC-channel slice
-// of the input: A (16 pixels x KC channels) from the input, B (KC x 16) from
-// the weights, one MFMA per output tile.  KC = 32 uses the 16x16x32 bf16 MFMA,
-// KC = 16 the 16x16x16 one.  Zero padding at the borders.
+// Implicit GEMM on a 4-row x 64-pixel tile per workgroup: wave w owns row w of
+// the tile (4 groups of 16 pixels) and NT*16 output channels (the output
+// channel group blockIdx.z % groups).  The input channels go through in slices
+// of KC: the (4+2) x 66 window of the slice is staged in LDS once for the 9
+// taps and 4 waves; per tap, each B fragment (KC x 16 weights, read from
+// global/L1) serves the wave's 4 pixel groups and each A fragment (16 pixels x
+// KC channels, from LDS) serves the NT channel tiles.  KC = 32 runs the
+// 16x16x32 bf16 MFMA, KC = 16 the 16x16x16 one.  Zero padding at the borders.
+#define CONV_TR 4
+#define CONV_PW 66
 template <int KC, int NT>
 __global__ __launch_bounds__(256) void k_conv(View in, const uint16_t *__restrict__ wt, const float *__restrict__ bias,
-                                              int cop, View out, float *__restrict__ out32, View mask, int relu) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int b = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * 64 + wv * 16;
+                                              int cop, int groups, View out, float *__restrict__ out32, View mask,
+                                              int relu) {
+    constexpr int PER = KC / 4, PS = KC + 8;  // bf16 per lane per MFMA; LDS pixel stride (padded)
+    __shared__ __attribute__((aligned(16))) uint16_t Xs[(CONV_TR + 2) * CONV_PW * PS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, kg = lane >> 4;
+    const int cgi = blockIdx.z % groups, b = blockIdx.z / groups;
+    const int y0 = blockIdx.y * CONV_TR, x0 = blockIdx.x * 64;
     const int H = in.H, W = in.W, cin = in.C;
-    const int m = lane & 15, kg = lane >> 4;
-    constexpr int PER = KC / 4;  // bf16 per lane per MFMA
-    f32x4 acc[NT];
+    f32x4 acc[4][NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int px = x0 + m;
-    for (int tap = 0; tap < 9; tap++) {
-        const int yy = y + tap / 3 - 1, xx = px + tap % 3 - 1;
-        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const uint16_t *src = in.p + (((size_t)b * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * in.cs + PER * kg;
-        const uint16_t *wsrc = wt + ((size_t)tap * cop + m) * cin + PER * kg;
-        for (int ci0 = 0; ci0 < cin; ci0 += KC) {
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[g][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *wbase = wt + ((size_t)cgi * NT * 16 + m) * cin + PER * kg;
+    for (int ci0 = 0; ci0 < cin; ci0 += KC) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < (CONV_TR + 2) * CONV_PW * (KC / 4); i += 256) {
+            int r = i / (CONV_PW * (KC / 4)), q = i - r * (CONV_PW * (KC / 4)), px = q / (KC / 4), c4 = q - px * (KC / 4);
+            int yy = y0 + r - 1, xx = x0 + px - 1;
+            uint2 v = {0u, 0u};
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *(const uint2 *)(in.p + (((size_t)b * H + yy) * W + xx) * in.cs + ci0 + 4 * c4);
+            *(uint2 *)(Xs + (r * CONV_PW + px) * PS + 4 * c4) = v;
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;  // window offsets (0..2)
+            const uint16_t *wsrc = wbase + (size_t)tap * cop * cin + ci0;
+            const uint16_t *xrow = Xs + ((wv + dy) * CONV_PW + m + dx) * PS + PER * kg;
             if constexpr (KC == 16) {
-                s16x4 a = {0, 0, 0, 0};
-                if (ok) a = *(const s16x4 *)(src + ci0);
+                s16x4 bw[NT];
 #pragma unroll
-                for (int t = 0; t < NT; t++) {
-                    s16x4 bw = *(const s16x4 *)(wsrc + (size_t)t * 16 * cin + ci0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[t], 0, 0, 0);
+                for (int t = 0; t < NT; t++) bw[t] = *(const s16x4 *)(wsrc + (size_t)t * 16 * cin);
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    s16x4 a = *(const s16x4 *)(xrow + g * 16 * PS);
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[g][t], 0, 0, 0);
                 }
             } else {
-                bf16x8 a;
-                if (ok) {
-                    a = *(const bf16x8 *)(src + ci0);
-                } else {
-                    uint4 z = {0u, 0u, 0u, 0u};
-                    a = __builtin_bit_cast(bf16x8, z);
-                }
+                bf16x8 bw[NT];
 #pragma unroll
-                for (int t = 0; t < NT; t++) {
-                    bf16x8 bw = *(const bf16x8 *)(wsrc + (size_t)t * 16 * cin + ci0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+                for (int t = 0; t < NT; t++) bw[t] = *(const bf16x8 *)(wsrc + (size_t)t * 16 * cin);
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    bf16x8 a = *(const bf16x8 *)(xrow + g * 16 * PS);
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[t], acc[g][t], 0, 0, 0);
                 }
             }
         }
     }
-    // lane holds D[pixel 4*kg + j][channel t*16 + m]
+    // lane holds D[pixel g*16 + 4*kg + j][channel t*16 + m]
+    const int y = y0 + wv;
+    if (y >= H) return;
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int co = t * 16 + m;
+    for (int g = 0; g < 4; g++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int x = x0 + 4 * kg + j;
-            if (x >= W) continue;
-            float v = acc[t][j] + (bias ? bias[co] : 0.0f);
-            if (relu) v = fmaxf(v, 0.0f);
-            const size_t pix = ((size_t)b * H + y) * W + x;
-            if (out32) {
-                if (co == 0) out32[pix] = v;
-                continue;
+        for (int t = 0; t < NT; t++) {
+            const int co = (cgi * NT + t) * 16 + m;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int x = x0 + g * 16 + 4 * kg + j;
+                if (x >= W) continue;
+                float v = acc[g][t][j] + (bias ? bias[co] : 0.0f);
+                if (relu) v = fmaxf(v, 0.0f);
+                const size_t pix = ((size_t)b * H + y) * W + x;
+                if (out32) {
+                    if (co == 0) out32[pix] = v;
+                    continue;
+                }
+                if (co >= out.C) continue;
+                if (mask.p && !(bf2f(mask.p[pix * mask.cs + co]) > 0.0f)) v = 0.0f;
+                out.p[pix * out.cs + co] = f2bf(v);
             }
-            if (co >= out.C) continue;
-            if (mask.p && !(bf2f(mask.p[pix * mask.cs + co]) > 0.0f)) v = 0.0f;
-            out.p[pix * out.cs + co] = f2bf(v);
         }
-    }
 }
 
 // Weight gradient dW[tap][co][ci] = sum over pixels dz[p][co] * in[p + tap][ci]:
@@ -505,16 +528,17 @@ static void ensure_acts(lsh_warm &n, int nimg, int H, int W, bool bwd) {
 template <int KC>
 static void conv_kc(const View &in, const uint16_t *w, const float *bias, int cop, const View &out, float *out32,
                     const View &mask, bool relu, int nimg) {
-    dim3 grid(cdiv(in.W, 64), in.H, nimg), block(256);
+    // output channels in groups of at most 64 (NT <= 4 tiles per wave)
+    const int tiles = cop / 16, nt = tiles % 4 == 0 ? 4 : (tiles % 3 == 0 ? 3 : (tiles <= 4 ? tiles : 2));
+    const int groups = tiles / nt;
+    LSH_REQUIRE(groups * nt == tiles, "warm: unsupported output width");
+    dim3 grid(cdiv(in.W, 64), cdiv(in.H, CONV_TR), nimg * groups), block(256);
     double bytes = 2.0 * nimg * in.H * in.W * (in.C + (out32 ? 2 : out.C)) + 2.0 * 9 * cop * in.C;
-    switch (cop / 16) {
-    case 1: launch_b("warm_conv", bytes, k_conv<KC, 1>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
-    case 2: launch_b("warm_conv", bytes, k_conv<KC, 2>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
-    case 3: launch_b("warm_conv", bytes, k_conv<KC, 3>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
-    case 4: launch_b("warm_conv", bytes, k_conv<KC, 4>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
-    case 6: launch_b("warm_conv", bytes, k_conv<KC, 6>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
-    case 8: launch_b("warm_conv", bytes, k_conv<KC, 8>, grid, block, 0, in, w, bias, cop, out, out32, mask, (int)relu); break;
-    default: throw Error("warm: unsupported output width");
+    switch (nt) {
+    case 1: launch_b("warm_conv", bytes, k_conv<KC, 1>, grid, block, 0, in, w, bias, cop, groups, out, out32, mask, (int)relu); break;
+    case 2: launch_b("warm_conv", bytes, k_conv<KC, 2>, grid, block, 0, in, w, bias, cop, groups, out, out32, mask, (int)relu); break;
+    case 3: launch_b("warm_conv", bytes, k_conv<KC, 3>, grid, block, 0, in, w, bias, cop, groups, out, out32, mask, (int)relu); break;
+    default: launch_b("warm_conv", bytes, k_conv<KC, 4>, grid, block, 0, in, w, bias, cop, groups, out, out32, mask, (int)relu); break;
     }
 }
 
