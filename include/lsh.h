@@ -151,7 +151,9 @@ char *lsh_frames_download(lsh_frames *f, float *img3, lsh_mem mem);
 uint64_t lsh_frames_last_evals(lsh_frames *f);
 /* The last search per registration level (level 0 = full resolution): cost
  * evaluations, and the device wall-clock ticks (hipDeviceAttributeWallClockRate)
- * the search spent there; up to cap levels. */
+ * the search spent there; up to cap levels.  With cap above 16: evals[16 + l]
+ * the collective passes at level l (each pass evaluates up to 9 points), and
+ * ticks[16..20) the ticks per phase (matrix, pixels, combine, steering). */
 char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, uint32_t cap);
 void lsh_frames_free(lsh_frames *f);
 

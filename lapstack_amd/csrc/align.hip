@@ -160,6 +160,7 @@ struct NmConst {
     int lw[NM_MAXL], lh[NM_MAXL];
     int nlev, finest, nfree;
     int solo_max;  // levels of at most this many pixels are searched by block 0 alone
+    int spec_max;  // grid levels of at most this many pixels evaluate r, e and c in one pass
     int free_idx[8];
     double step[NM_MAXL][8], tol[NM_MAXL][8];
     double span[8];
@@ -200,34 +201,22 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 
 #define NM_THREADS 1024
 #define NM_WAVES (NM_THREADS / 64)
+#define NM_MAXP 9     // cost evaluations per collective pass
+#define NM_TILE 64    // a pass walks 64-pixel-wide destination tiles of 16, 32 or 64 rows
+#define NM_BOX 8192   // floats of a tile's staged source window
+// the state words: the Sim, the evaluation total, per level the evaluations and
+// block 0's ticks, the four phase tick totals, per level the passes
+#define NM_STATE (13 + 3 * NM_MAXL)
 
 struct NmShared {
-    double S[9][8], FV[9], XR[8], XT[8], C[8], lo[8], hi[8], cur[8], inv[9], cost, red[3][NM_WAVES];
-    unsigned lev_evals[NM_MAXL];
+    double S[9][8], FV[9], XR[8], XE[8], XC[8], C[8], lo[8], hi[8], cur[8], FR;
+    double PT[NM_MAXP][8], PC[NM_MAXP], inv[NM_MAXP][9];
+    double red[3 * NM_MAXP][NM_WAVES], sums[3 * NM_MAXP];
+    unsigned lev_evals[NM_MAXL], lev_passes[NM_MAXL];
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
+    int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
     int stop, conv, folds;
 };
-
-// the block's three sums, folded over its waves in wave order
-__device__ __forceinline__ void block_sums(NmShared &sh, double v0, double v1, double v2, double *out) {
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    v2 = wave_sum(v2);
-    __syncthreads();
-    if (lane == 0) {
-        sh.red[0][wv] = v0;
-        sh.red[1][wv] = v1;
-        sh.red[2][wv] = v2;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) {
-            double t = 0.0;
-            for (int q = 0; q < NM_WAVES; q++) t += sh.red[k][q];
-            out[k] = t;
-        }
-}
 
 // align.rs:630-634
 __device__ __forceinline__ double rms_of(double sd, double sd2, double cnt) {
@@ -236,108 +225,226 @@ __device__ __forceinline__ double rms_of(double sd, double sd2, double cnt) {
     return sqrt(vv > 0.0 ? vv : 0.0);
 }
 
-// One collective cost evaluation (align.rs:581-635) of the point x (nfree
-// doubles, in LDS) at level lvl.  solo: block 0 alone, over all the level's
-// pixels; otherwise every block over its grid-strided share, the partial sums
-// folded by the block completing the arrival tree and the cost broadcast.
-__device__ double nm_cost(const NmConst &K, NmShared &sh, int lvl, const double *x, bool solo, double *part,
-                          unsigned *ctr, unsigned *err, unsigned &gen) {
+// sh.sums[k] (k < nv) = sh.red[k][0] + ... + sh.red[k][NM_WAVES-1] (the waves'
+// accumulators, in wave order)
+__device__ __forceinline__ void fold_waves(NmShared &sh, int nv) {
+    __syncthreads();
+    if ((int)threadIdx.x < nv) {
+        double t = 0.0;
+        for (int q = 0; q < NM_WAVES; q++) t += sh.red[threadIdx.x][q];
+        sh.sums[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void zero_waves(NmShared &sh) {
+    for (int i = threadIdx.x; i < 3 * NM_MAXP * NM_WAVES; i += NM_THREADS) (&sh.red[0][0])[i] = 0.0;
+}
+
+// One collective pass: the costs (align.rs:581-635) of the np points
+// sh.PT[0..np) at level lvl into sh.PC[0..np).  The level is walked in 64 x 64
+// destination tiles; the tile's source window (the union over the points of
+// the bounding box of the tile corners' source points, plus the 4 x 4 support)
+// is staged in LDS when it fits, and every tap inside it is read from there,
+// any other from global memory.  solo: block 0 alone over all tiles; otherwise
+// every block over its grid-strided share, the partial sums folded by the
+// block completing the arrival tree and the costs broadcast.  Batching the
+// points does not change any cost: each point's sums are the ones a pass of
+// its own would form.
+__device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, int np, bool solo, double *part,
+                        unsigned *ctr, unsigned *err, unsigned &gen) {
     const int tw = K.lw[lvl], th = K.lh[lvl];
     uint64_t tq = wall_clock64();
-    if (threadIdx.x == 0) {
-        sh.ph[3] += tq - sh.tmark;
+    if (threadIdx.x == 0) sh.ph[3] += tq - sh.tmark;
+    if ((int)threadIdx.x < np) {
         double v[8];
         for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
-        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = x[k];
-        sim_inverse_dev(v, tw, th, sh.inv);
+        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = sh.PT[threadIdx.x][k];
+        sim_inverse_dev(v, tw, th, sh.inv[threadIdx.x]);
     }
     __syncthreads();
     uint64_t tq1 = wall_clock64();
     if (threadIdx.x == 0) sh.ph[0] += tq1 - tq;
-    Inv9 inv;
-    for (int k = 0; k < 9; k++) inv.m[k] = sh.inv[k];
-    int affine = inv.m[6] == 0.0 && inv.m[7] == 0.0 && inv.m[8] == 1.0;
     const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
-    size_t n = (size_t)tw * th;
-    double xmax = (double)(tw - 1), ymax = (double)(th - 1);
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-    size_t start = solo ? threadIdx.x : (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = solo ? blockDim.x : (size_t)gridDim.x * blockDim.x;
-    for (size_t i = start; i < n; i += stride) {
-        int y = (int)(i / tw), x0p = (int)(i - (size_t)y * tw);
-        double sx, sy;
-        src_point(inv, affine, x0p, y, sx, sy);
-        if (!(sx >= 0.0 && sx <= xmax && sy >= 0.0 && sy <= ymax)) continue;
-        double fx = floor(sx), fy = floor(sy);
-        int x0 = (int)fx, y0 = (int)fy;
-        float wx[4], wy[4];
-        spline4f((float)(sx - fx), wx);
-        spline4f((float)(sy - fy), wy);
-        float acc = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float *row = tg + (size_t)min(max(y0 + j - 1, 0), th - 1) * tw;
-            float r = 0.0f;
-#pragma unroll
-            for (int q = 0; q < 4; q++) r += wx[q] * row[min(max(x0 + q - 1, 0), tw - 1)];
-            acc += wy[j] * r;
+    const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
+    zero_waves(sh);  // ordered before its first use by the tile loop's barriers
+    // The displacement (source - destination) of an affine map is affine in the
+    // pixel, so over the level it is bounded by its values at the 4 corners: one
+    // bound for the pass, from which every tile's window follows.  A projective
+    // point (or a wild one) turns the staging off for the pass.
+    if (threadIdx.x == 0) {
+        double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
+        int ok = 1;
+        for (int p = 0; p < np; p++) {
+            Inv9 iv;
+            for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
+            if (!(iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0)) ok = 0;
+            for (int c = 0; c < 4; c++) {
+                const int cx = (c & 1) ? tw - 1 : 0, cy = (c & 2) ? th - 1 : 0;
+                double sx, sy;
+                src_point(iv, 1, cx, cy, sx, sy);
+                mnx = fmin(mnx, sx - cx);
+                mxx = fmax(mxx, sx - cx);
+                mny = fmin(mny, sy - cy);
+                mxy = fmax(mxy, sy - cy);
+            }
         }
-        double d = (double)(rf[i] - acc);
-        v0 += d;
-        v1 += d * d;
-        v2 += 1.0;
+        if (!(mnx > -256.0 && mxx < 256.0 && mny > -256.0 && mxy < 256.0)) ok = 0;
+        sh.box[0] = ok ? (int)floor(mnx) - 1 : 0;  // window = tile + [lo, hi] on each axis
+        sh.box[1] = ok ? (int)floor(mxx) + 2 : 0;
+        sh.box[2] = ok ? (int)floor(mny) - 1 : 0;
+        sh.box[3] = ok ? (int)floor(mxy) + 2 : 0;
+        sh.box[4] = ok;
     }
-    double bs[3];
-    block_sums(sh, v0, v1, v2, bs);
+    // rows per thread: 4 where the level has a 64 x 64 tile for every block
+    // (or one block does all), fewer on small levels so more blocks share them
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE;
+    int R = 4;
+    if (!solo)
+        while (R > 1 && ntx * ((th + 16 * R - 1) / (16 * R)) < (int)gridDim.x) R >>= 1;
+    const int TH = 16 * R, nty = (th + TH - 1) / TH, ntiles = ntx * nty;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : gridDim.x) {
+        const int X0 = (t % ntx) * NM_TILE, Y0 = (t / ntx) * TH;
+        const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
+        __syncthreads();  // the bound is written; the previous tile's window is no longer read
+        const int bx0 = max(0, X0 + sh.box[0]), bx1 = min(tw - 1, X1 + sh.box[1]);
+        const int by0 = max(0, Y0 + sh.box[2]), by1 = min(th - 1, Y1 + sh.box[3]);
+        const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+        const int staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= NM_BOX;
+
+        if (staged)
+            for (int i = threadIdx.x; i < bw * bh; i += NM_THREADS) {
+                int r = i / bw;
+                boxs[i] = tg[(size_t)(by0 + r) * tw + bx0 + (i - r * bw)];
+            }
+        __syncthreads();
+        // per point: the thread's 4 pixels, summed over the wave, added to the
+        // wave's accumulators (one lane per wave writes them: no race)
+        float rv[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int x = X0 + tx, y = Y0 + ty + 16 * r;
+            rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
+        }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (int p = 0; p < np; p++) {
+            Inv9 iv;
+            for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
+            const int aff = iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll 1
+            for (int r = 0; r < R; r++) {
+                const int x = X0 + tx, y = Y0 + ty + 16 * r;
+                if (x >= tw || y >= th) continue;
+                double sx, sy;
+                src_point(iv, aff, x, y, sx, sy);
+                if (!(sx >= 0.0 && sx <= xmax && sy >= 0.0 && sy <= ymax)) continue;
+                double fx = floor(sx), fy = floor(sy);
+                int x0 = (int)fx, y0 = (int)fy;
+                float wx[4], wy[4];
+                spline4f((float)(sx - fx), wx);
+                spline4f((float)(sy - fy), wy);
+                // the 4 x 4 support (clamped to the level) in the staged window, or global reads
+                const int cx0 = min(max(x0 - 1, 0), tw - 1), cx3 = min(max(x0 + 2, 0), tw - 1);
+                const int cy0 = min(max(y0 - 1, 0), th - 1), cy3 = min(max(y0 + 2, 0), th - 1);
+                float a = 0.0f;
+                if (staged && cx0 >= bx0 && cx3 < bx0 + bw && cy0 >= by0 && cy3 < by0 + bh) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float *row = boxs + (min(max(y0 + j - 1, 0), th - 1) - by0) * bw - bx0;
+                        float rr = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) rr += wx[q] * row[min(max(x0 + q - 1, 0), tw - 1)];
+                        a += wy[j] * rr;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float *row = tg + (size_t)min(max(y0 + j - 1, 0), th - 1) * tw;
+                        float rr = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) rr += wx[q] * row[min(max(x0 + q - 1, 0), tw - 1)];
+                        a += wy[j] * rr;
+                    }
+                }
+                const double d = (double)(rv[r] - a);
+                s0 += d;
+                s1 += d * d;
+                s2 += 1.0;
+            }
+            s0 = wave_sum(s0);
+            s1 = wave_sum(s1);
+            s2 = wave_sum(s2);
+            if (lane == 0) {
+                sh.red[3 * p][wv] += s0;
+                sh.red[3 * p + 1][wv] += s1;
+                sh.red[3 * p + 2][wv] += s2;
+            }
+        }
+    }
+    fold_waves(sh, 3 * np);
     uint64_t tq2 = wall_clock64();
-    if (threadIdx.x == 0) sh.ph[1] += tq2 - tq1;
+    if (threadIdx.x == 0) {
+        sh.ph[1] += tq2 - tq1;
+        sh.lev_passes[lvl]++;
+    }
     if (solo) {
+        if ((int)threadIdx.x < np)
+            sh.PC[threadIdx.x] = rms_of(sh.sums[3 * threadIdx.x], sh.sums[3 * threadIdx.x + 1], sh.sums[3 * threadIdx.x + 2]);
         if (threadIdx.x == 0) {
-            sh.cost = rms_of(bs[0], bs[1], bs[2]);
-            sh.lev_evals[lvl]++;
             sh.tmark = wall_clock64();
             sh.ph[2] += sh.tmark - tq2;
         }
         __syncthreads();
-        return sh.cost;
+        return;
     }
-    unsigned slot = gen & 1;
-    double *pp = part + (size_t)slot * gridDim.x * 3;
+    const unsigned slot = gen & 1;
+    const int nv = 3 * np;
+    double *pp = part + (size_t)slot * gridDim.x * 3 * NM_MAXP;
     if (threadIdx.x == 0) {
-        for (int k = 0; k < 3; k++) pp[(size_t)blockIdx.x * 3 + k] = bs[k];
+        for (int k = 0; k < nv; k++) pp[(size_t)blockIdx.x * 3 * NM_MAXP + k] = sh.sums[k];
         __threadfence();
         // two-level arrival: NM_GROUPS counters (one cache line each), then one
         // counter of completed groups; the block completing the last group folds
         const unsigned gs = gridDim.x / NM_GROUPS;
-        unsigned t = atomicAdd(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u);
+        unsigned a = atomicAdd(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u);
         int folds = 0;
-        if (t == (gen + 1) * gs - 1) {
+        if (a == (gen + 1) * gs - 1) {
             __threadfence();
-            unsigned t2 = atomicAdd(&ctr[NM_FL_TOP], 1u);
-            folds = t2 == (gen + 1) * NM_GROUPS - 1;
+            unsigned a2 = atomicAdd(&ctr[NM_FL_TOP], 1u);
+            folds = a2 == (gen + 1) * NM_GROUPS - 1;
         }
         sh.folds = folds;
     }
     __syncthreads();
     unsigned *bflag = ctr + NM_FL_BCAST;
-    double *bcost = part + (size_t)2 * gridDim.x * 3;
+    double *bcost = part + (size_t)2 * gridDim.x * 3 * NM_MAXP + (size_t)slot * NM_MAXP;
     if (sh.folds) {
         // the fold order depends on thread indices only, not on which block folds
         __threadfence();
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            a0 += __hip_atomic_load(&pp[(size_t)b * 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a1 += __hip_atomic_load(&pp[(size_t)b * 3 + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a2 += __hip_atomic_load(&pp[(size_t)b * 3 + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        zero_waves(sh);
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (unsigned b0 = 0; b0 < gridDim.x; b0 += blockDim.x) {
+            const unsigned b = b0 + threadIdx.x;
+            for (int k = 0; k < nv; k++) {
+                double v = b < gridDim.x
+                               ? __hip_atomic_load(&pp[(size_t)b * 3 * NM_MAXP + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0.0;
+                v = wave_sum(v);
+                if (lane == 0) sh.red[k][wv] += v;
+            }
         }
-        double tot[3];
-        block_sums(sh, a0, a1, a2, tot);
+        fold_waves(sh, nv);
         if (threadIdx.x == 0) {
-            double c = rms_of(tot[0], tot[1], tot[2]);
-            __hip_atomic_store(&bcost[slot], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int p = 0; p < np; p++) {
+                double c = rms_of(sh.sums[3 * p], sh.sums[3 * p + 1], sh.sums[3 * p + 2]);
+                __hip_atomic_store(&bcost[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                sh.PC[p] = c;
+            }
             __threadfence();
             __hip_atomic_store(bflag, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            sh.cost = c;
         }
     } else if (threadIdx.x == 0) {
         // wait for the broadcast; bounded, and left at once when any block has raised the error flag
@@ -351,17 +458,16 @@ __device__ double nm_cost(const NmConst &K, NmShared &sh, int lvl, const double 
             }
         }
         __threadfence();
-        sh.cost = __hip_atomic_load(&bcost[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int p = 0; p < np; p++)
+            sh.PC[p] = __hip_atomic_load(&bcost[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     gen++;
     if (threadIdx.x == 0) {
-        sh.lev_evals[lvl]++;
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) sh.stop = 1;
         sh.tmark = wall_clock64();
         sh.ph[2] += sh.tmark - tq2;
     }
     __syncthreads();
-    return sh.cost;
 }
 
 __device__ __forceinline__ double clampd_dev(double v, double lo, double hi) {
@@ -372,13 +478,23 @@ __device__ __forceinline__ double clampd_dev(double v, double lo, double hi) {
 
 // One level's bounded Nelder-Mead (align.rs:796-883), collective over the
 // block (solo) or the grid; the simplex lives in LDS and thread 0 steers it.
-__device__ void nm_level(const NmConst &K, NmShared &sh, int lvl, bool solo, double *part, unsigned *ctr,
+// The loop is a small state machine around one pass call site.  On the grid
+// levels where a pass costs mostly synchronisation (spec_max), an iteration
+// evaluates the reflection, the expansion and the contraction in one pass
+// (the latter two are fixed by the centroid and the worst vertex before the
+// reflection's cost is known) and then takes the reference's branch;
+// elsewhere it evaluates them one by one as the reference does.  A shrink
+// evaluates its n new vertices in one pass, as does the initial simplex.  The
+// iterates and the counted evaluations are the reference's: a speculative
+// cost the branch does not use is not counted.
+__device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, int lvl, bool solo, double *part, unsigned *ctr,
                          unsigned *err, unsigned &gen) {
+    enum { INIT, ITER, EXPAND, CONTRACT, SHRINK };
     const int n = K.nfree;
     const bool t0 = threadIdx.x == 0;
+    const bool spec = !solo && (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.spec_max;
     const double *step = K.step[lvl], *tol = K.tol[lvl];
 #define CLAMPK(v, k) clampd_dev((v), sh.lo[k], sh.hi[k])
-#define COST(x) nm_cost(K, sh, lvl, (x), solo, part, ctr, err, gen)
     if (t0) {
         for (int k = 0; k < n; k++) sh.S[0][k] = sh.cur[K.free_idx[k]];
         for (int i = 0; i < n; i++) {
@@ -386,14 +502,88 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, int lvl, bool solo, dou
             sh.S[i + 1][i] += step[i];
             for (int k = 0; k < n; k++) sh.S[i + 1][k] = CLAMPK(sh.S[i + 1][k], k);
         }
+        for (int i = 0; i <= n; i++)
+            for (int k = 0; k < n; k++) sh.PT[i][k] = sh.S[i][k];
     }
     __syncthreads();
-    for (int i = 0; i <= n && !sh.stop; i++) {
-        double c = COST(sh.S[i]);
-        if (t0) sh.FV[i] = c;
-    }
-    __syncthreads();
-    for (int it = 0; it < 200 && !sh.stop; it++) {
+    int state = INIT, np = n + 1, it = 0;
+    for (;;) {
+        nm_pass(K, sh, boxs, lvl, np, solo, part, ctr, err, gen);
+        int next = -1;  // a follow-up pass within the iteration, or none
+        if (state == INIT) {
+            if (t0) {
+                for (int i = 0; i <= n; i++) sh.FV[i] = sh.PC[i];
+                sh.lev_evals[lvl] += n + 1;
+            }
+        } else if (state == ITER) {
+            // every thread reads the branch's inputs before thread 0 changes any of them
+            const double fr = sh.PC[0], fe = sh.PC[1], fc = sh.PC[2];
+            const int br = fr < sh.FV[0] ? 0 : (fr < sh.FV[n - 1] ? 1 : 2);
+            const int bc = spec && br == 2 ? (fc < sh.FV[n] ? 2 : 3) : br;
+            __syncthreads();
+            if (t0) {
+                sh.FR = fr;
+                sh.lev_evals[lvl] += 1;
+                if (br == 1) {  // reflect
+                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XR[k];
+                    sh.FV[n] = fr;
+                } else if (!spec) {  // the expansion or the contraction is the next pass
+                    for (int k = 0; k < n; k++) sh.PT[0][k] = br == 0 ? sh.XE[k] : sh.XC[k];
+                } else if (br == 0) {  // expand (align.rs:838-849)
+                    const double *src = fe < fr ? sh.XE : sh.XR;
+                    for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
+                    sh.FV[n] = fe < fr ? fe : fr;
+                    sh.lev_evals[lvl] += 1;
+                } else if (bc == 2) {  // contract (align.rs:853-862)
+                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XC[k];
+                    sh.FV[n] = fc;
+                    sh.lev_evals[lvl] += 1;
+                } else {
+                    sh.lev_evals[lvl] += 1;
+                }
+            }
+            if (!spec && br != 1) next = br == 0 ? EXPAND : CONTRACT;
+            if (spec && bc == 3) next = SHRINK;
+        } else if (state == EXPAND) {
+            if (t0) {
+                const double fe = sh.PC[0], fr = sh.FR;
+                const double *src = fe < fr ? sh.XE : sh.XR;
+                for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
+                sh.FV[n] = fe < fr ? fe : fr;
+                sh.lev_evals[lvl] += 1;
+            }
+        } else if (state == CONTRACT) {
+            const double fc = sh.PC[0];
+            const bool take = fc < sh.FV[n];
+            __syncthreads();
+            if (t0) {
+                if (take) {
+                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XC[k];
+                    sh.FV[n] = fc;
+                }
+                sh.lev_evals[lvl] += 1;
+            }
+            if (!take) next = SHRINK;
+        } else {  // SHRINK
+            if (t0) {
+                for (int i = 1; i <= n; i++) sh.FV[i] = sh.PC[i - 1];
+                sh.lev_evals[lvl] += n;
+            }
+        }
+        if (next == SHRINK && t0)  // shrink towards the best (align.rs:863-870)
+            for (int i = 1; i <= n; i++)
+                for (int k = 0; k < n; k++) {
+                    sh.S[i][k] = CLAMPK(sh.S[0][k] + 0.5 * (sh.S[i][k] - sh.S[0][k]), k);
+                    sh.PT[i - 1][k] = sh.S[i][k];
+                }
+        __syncthreads();
+        if (next >= 0 && !sh.stop) {
+            state = next;
+            np = next == SHRINK ? n : 1;
+            continue;
+        }
+        if (sh.stop || it == 200) break;
+        it++;
         if (t0) {
             // stable insertion sort of the vertices by value
             for (int i = 1; i <= n; i++) {
@@ -414,54 +604,25 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, int lvl, bool solo, dou
                 for (int k = 0; k < n; k++)
                     if (!(fabs(sh.S[i][k] - sh.S[0][k]) <= tol[k])) conv = 0;
             sh.conv = conv;
-            // centroid of the best n and the reflection (align.rs:823-836)
+            // centroid of the best n, the reflection, expansion and contraction (align.rs:823-862)
             for (int k = 0; k < n; k++) sh.C[k] = 0.0;
             for (int i = 0; i < n; i++)
                 for (int k = 0; k < n; k++) sh.C[k] += sh.S[i][k];
             for (int k = 0; k < n; k++) sh.C[k] /= (double)n;
             for (int k = 0; k < n; k++) sh.XR[k] = CLAMPK(sh.C[k] + 1.0 * (sh.C[k] - sh.S[n][k]), k);
+            for (int k = 0; k < n; k++) sh.XE[k] = CLAMPK(sh.C[k] + 2.0 * (sh.XR[k] - sh.C[k]), k);
+            for (int k = 0; k < n; k++) sh.XC[k] = CLAMPK(sh.C[k] + 0.5 * (sh.S[n][k] - sh.C[k]), k);
+            for (int k = 0; k < n; k++) {
+                sh.PT[0][k] = sh.XR[k];
+                sh.PT[1][k] = sh.XE[k];
+                sh.PT[2][k] = sh.XC[k];
+            }
         }
         __syncthreads();
         if (sh.conv) break;
-        double fr = COST(sh.XR);
-        if (fr < sh.FV[0]) {
-            if (t0)
-                for (int k = 0; k < n; k++) sh.XT[k] = CLAMPK(sh.C[k] + 2.0 * (sh.XR[k] - sh.C[k]), k);
-            __syncthreads();
-            double fe = COST(sh.XT);
-            if (t0) {
-                const double *src = fe < fr ? sh.XT : sh.XR;
-                for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
-                sh.FV[n] = fe < fr ? fe : fr;
-            }
-        } else if (fr < sh.FV[n - 1]) {
-            if (t0) {
-                for (int k = 0; k < n; k++) sh.S[n][k] = sh.XR[k];
-                sh.FV[n] = fr;
-            }
-        } else {
-            if (t0)
-                for (int k = 0; k < n; k++) sh.XT[k] = CLAMPK(sh.C[k] + 0.5 * (sh.S[n][k] - sh.C[k]), k);
-            __syncthreads();
-            double fc = COST(sh.XT);
-            if (fc < sh.FV[n]) {
-                if (t0) {
-                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XT[k];
-                    sh.FV[n] = fc;
-                }
-            } else {
-                for (int i = 1; i <= n && !sh.stop; i++) {
-                    if (t0)
-                        for (int k = 0; k < n; k++) sh.S[i][k] = CLAMPK(sh.S[0][k] + 0.5 * (sh.S[i][k] - sh.S[0][k]), k);
-                    __syncthreads();
-                    double fi = COST(sh.S[i]);
-                    if (t0) sh.FV[i] = fi;
-                }
-            }
-        }
-        __syncthreads();
+        state = ITER;
+        np = spec ? 3 : 1;
     }
-#undef COST
 #undef CLAMPK
     if (t0) {
         int best = 0;
@@ -482,6 +643,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
                                                           double *__restrict__ part, unsigned *__restrict__ ctr,
                                                           unsigned *__restrict__ err) {
     __shared__ NmShared sh;
+    __shared__ float boxs[NM_BOX];
     const NmConst &K = *Kp;
     const int n = K.nfree;
     unsigned gen = 0;
@@ -494,25 +656,20 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
             sh.hi[k] = iv + K.span[K.free_idx[k]];
         }
         sh.stop = 0;
-        for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = 0;
+        for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = sh.lev_passes[l] = 0;
         for (int q = 0; q < 4; q++) sh.ph[q] = 0;
         sh.tmark = wall_clock64();
         if (blockIdx.x == 0)
             for (int l = 0; l < NM_MAXL; l++) state[9 + NM_MAXL + l] = 0.0;
     }
     __syncthreads();
-    int lvl = K.nlev - 1;
-    if (n > 0) {
-        // the coarse levels, block 0 alone
-        if (blockIdx.x == 0)
-            for (int l = lvl; l >= K.finest && (size_t)K.lw[l] * K.lh[l] <= (size_t)K.solo_max; l--) {
-                uint64_t t = wall_clock64();
-                nm_level(K, sh, l, true, part, ctr, err, gen);
-                if (t0) state[9 + NM_MAXL + l] = (double)(wall_clock64() - t);
-            }
-        while (lvl >= K.finest && (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.solo_max) lvl--;
-        if (lvl >= K.finest) {
-            // hand-off: block 0 publishes the Sim, the others take it
+    // the levels, coarse to fine; those of at most solo_max pixels (the
+    // coarsest) by block 0 alone, which then hands the Sim to the grid
+    bool handed = false;
+    for (int lvl = K.nlev - 1; n > 0 && lvl >= K.finest && !sh.stop; lvl--) {
+        const bool solo = (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.solo_max;
+        if (!solo && !handed) {
+            handed = true;
             unsigned *hflag = ctr + NM_FL_HAND;
             if (t0) {
                 if (blockIdx.x == 0) {
@@ -535,12 +692,12 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
                 }
             }
             __syncthreads();
-            for (; lvl >= K.finest && !sh.stop; lvl--) {
-                uint64_t t = wall_clock64();
-                nm_level(K, sh, lvl, false, part, ctr, err, gen);
-                if (t0 && blockIdx.x == 0) state[9 + NM_MAXL + lvl] = (double)(wall_clock64() - t);
-            }
+            if (sh.stop) break;
         }
+        if (solo && blockIdx.x != 0) continue;
+        uint64_t t = wall_clock64();
+        nm_level(K, sh, boxs, lvl, solo, part, ctr, err, gen);
+        if (t0 && blockIdx.x == 0) state[9 + NM_MAXL + lvl] = (double)(wall_clock64() - t);
     }
     if (blockIdx.x == 0 && t0) {
         double tot = 0.0;
@@ -551,6 +708,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
         for (int k = 0; k < 8; k++) state[k] = sh.cur[k];
         state[8] = tot;
         for (int q = 0; q < 4; q++) state[9 + 2 * NM_MAXL + q] = (double)sh.ph[q];
+        for (int l = 0; l < NM_MAXL; l++) state[13 + 2 * NM_MAXL + l] = (double)sh.lev_passes[l];
     }
 }
 
@@ -825,6 +983,9 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     K.finest = (int)std::min(coarsen, n - 1);
     K.solo_max = 4096;
     if (const char *e = std::getenv("LSH_SEARCH_SOLO_MAX")) K.solo_max = std::atoi(e);
+    // speculation pays where a pass is dominated by the grid's synchronisation
+    K.spec_max = 400000;
+    if (const char *e = std::getenv("LSH_SEARCH_SPEC_MAX")) K.spec_max = std::atoi(e);
     for (size_t l = 0; l < n; l++) {
         K.ref[l] = f->yr.p + f->loff[l];
         K.tgt[l] = f->yt.p + f->loff[l];
@@ -846,7 +1007,7 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     }
     if (!f->nm_const.p) {
         f->nm_const.alloc(sizeof(NmConst));
-        f->nm_state.alloc(13 + 2 * NM_MAXL);
+        f->nm_state.alloc(NM_STATE);
         f->nm_flags.alloc(NM_FL_WORDS);
         int dev = 0, cus = 0, per_cu = 0;
         LSH_CHECK(hipGetDevice(&dev));
@@ -858,9 +1019,9 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
         f->nm_grid = (unsigned)(std::min(per_cu, want) * cus);
         f->nm_grid -= f->nm_grid % NM_GROUPS;
         LSH_REQUIRE(f->nm_grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
-        f->nm_part.alloc((size_t)2 * 3 * f->nm_grid + 2);
+        f->nm_part.alloc((size_t)2 * 3 * NM_MAXP * f->nm_grid + 2 * NM_MAXP);
     }
-    uint8_t *pin = (uint8_t *)pinned_scratch(sizeof(NmConst) + 96 * sizeof(double));
+    uint8_t *pin = (uint8_t *)pinned_scratch(sizeof(NmConst) + (24 + NM_STATE) * sizeof(double));
     std::memcpy(pin, &K, sizeof(NmConst));
     double *st = (double *)(pin + sizeof(NmConst));
     std::memcpy(st, init.v, 8 * sizeof(double));
@@ -873,13 +1034,14 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     void *args[] = {(void *)&kp, (void *)&state, (void *)&part, (void *)&ctr, (void *)&err};
     launch_coop("nm_search", 0.0, (const void *)k_nm_search, dim3(f->nm_grid), dim3(NM_THREADS), args, 0);
     double *out = st + 16;
-    LSH_CHECK(hipMemcpyAsync(out, f->nm_state.p, (13 + 2 * NM_MAXL) * sizeof(double), hipMemcpyDeviceToHost, stream()));
-    unsigned *flags = (unsigned *)(out + 13 + 2 * NM_MAXL);
+    LSH_CHECK(hipMemcpyAsync(out, f->nm_state.p, NM_STATE * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    unsigned *flags = (unsigned *)(out + NM_STATE);
     LSH_CHECK(hipMemcpyAsync(flags, err, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
     sync();
     LSH_REQUIRE(flags[0] == 0, "search: a cost broadcast timed out");
     double bytes = 0.0;
-    for (size_t l = 0; l < n; l++) bytes += out[9 + l] * 8.0 * (double)(f->lw[l] * f->lh[l]);
+    for (size_t l = 0; l < n; l++)  // a pass reads the level's reference and target once
+        bytes += out[13 + 2 * NM_MAXL + l] * 8.0 * (double)(f->lw[l] * f->lh[l]);
     prof_set_last_bytes(bytes);
     Sim s;
     std::memcpy(s.v, out, 8 * sizeof(double));
@@ -889,6 +1051,7 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
         f->lev_ticks[l] = (uint64_t)out[9 + NM_MAXL + l];
     }
     for (int q = 0; q < 4; q++) f->phase_ticks[q] = (uint64_t)out[9 + 2 * NM_MAXL + q];
+    for (size_t l = 0; l < NM_MAXL; l++) f->lev_passes[l] = (uint64_t)out[13 + 2 * NM_MAXL + l];
     return s;
 }
 
@@ -1091,9 +1254,12 @@ char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, u
         if (evals) evals[l] = f->lev_evals[l];
         if (ticks) ticks[l] = f->lev_ticks[l];
     }
-    // past the levels: block 0's ticks per phase (matrix, pixels, combine, steering)
+    // past the levels: block 0's ticks per phase (matrix, pixels, combine,
+    // steering), and the collective passes per level
     for (uint32_t q = 0; q < 4 && NM_MAXL + q < cap; q++)
         if (ticks) ticks[NM_MAXL + q] = f->phase_ticks[q];
+    for (uint32_t l = 0; l < NM_MAXL && NM_MAXL + l < cap; l++)
+        if (evals) evals[NM_MAXL + l] = f->lev_passes[l];
     LSH_API_END
 }
 

@@ -34,6 +34,7 @@ struct lsh_frames {
     lsh::DBuf<double> nm_part;    // per-block partial sums, two slots
     unsigned nm_grid = 0;
     uint64_t lev_evals[16] = {0}, lev_ticks[16] = {0};  // the last search, per registration level
+    uint64_t lev_passes[16] = {0};                       // its collective passes (several evaluations each)
     uint64_t phase_ticks[4] = {0};
 };
 
