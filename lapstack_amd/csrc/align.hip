@@ -209,13 +209,13 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_STATE (13 + 3 * NM_MAXL)
 
 struct NmShared {
-    double S[9][8], FV[9], XR[8], XE[8], XC[8], C[8], lo[8], hi[8], cur[8], FR;
+    double S[9][8], FV[9], S2[9][8], FV2[9], XR[8], XE[8], XC[8], C[8], lo[8], hi[8], cur[8], FR;
     double PT[NM_MAXP][8], PC[NM_MAXP], inv[NM_MAXP][9];
     double red[3 * NM_MAXP][NM_WAVES], sums[3 * NM_MAXP];
     unsigned lev_evals[NM_MAXL], lev_passes[NM_MAXL];
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
-    int stop, conv, folds;
+    int stop, folds;
 };
 
 // align.rs:630-634
@@ -584,42 +584,49 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, int lvl, b
         }
         if (sh.stop || it == 200) break;
         it++;
-        if (t0) {
-            // stable insertion sort of the vertices by value
-            for (int i = 1; i <= n; i++) {
-                double f = sh.FV[i], sv[8];
-                for (int k = 0; k < n; k++) sv[k] = sh.S[i][k];
-                int j = i;
-                while (j > 0 && sh.FV[j - 1] > f) {
-                    sh.FV[j] = sh.FV[j - 1];
-                    for (int k = 0; k < n; k++) sh.S[j][k] = sh.S[j - 1][k];
-                    j--;
-                }
-                sh.FV[j] = f;
-                for (int k = 0; k < n; k++) sh.S[j][k] = sv[k];
-            }
-            // align.rs:888-894
-            int conv = !(fabs(sh.FV[n] - sh.FV[0]) > 1e-4 * (1.0 + fabs(sh.FV[0])));
-            for (int i = 1; i <= n && conv; i++)
-                for (int k = 0; k < n; k++)
-                    if (!(fabs(sh.S[i][k] - sh.S[0][k]) <= tol[k])) conv = 0;
-            sh.conv = conv;
-            // centroid of the best n, the reflection, expansion and contraction (align.rs:823-862)
-            for (int k = 0; k < n; k++) sh.C[k] = 0.0;
-            for (int i = 0; i < n; i++)
-                for (int k = 0; k < n; k++) sh.C[k] += sh.S[i][k];
-            for (int k = 0; k < n; k++) sh.C[k] /= (double)n;
-            for (int k = 0; k < n; k++) sh.XR[k] = CLAMPK(sh.C[k] + 1.0 * (sh.C[k] - sh.S[n][k]), k);
-            for (int k = 0; k < n; k++) sh.XE[k] = CLAMPK(sh.C[k] + 2.0 * (sh.XR[k] - sh.C[k]), k);
-            for (int k = 0; k < n; k++) sh.XC[k] = CLAMPK(sh.C[k] + 0.5 * (sh.S[n][k] - sh.C[k]), k);
-            for (int k = 0; k < n; k++) {
-                sh.PT[0][k] = sh.XR[k];
-                sh.PT[1][k] = sh.XE[k];
-                sh.PT[2][k] = sh.XC[k];
-            }
+        // the iteration's set-up, spread over threads (every step keeps the
+        // reference's arithmetic and order per coordinate):
+        // 1. the stable sort of the vertices by value (align.rs:816-818): vertex
+        //    i goes to its rank (values below it, then equal values before it)
+        const int tid = threadIdx.x;
+        if (tid <= n) {
+            const double f = sh.FV[tid];
+            int r = 0;
+            for (int j = 0; j <= n; j++) r += sh.FV[j] < f || (sh.FV[j] == f && j < tid);
+            for (int k = 0; k < n; k++) sh.S2[r][k] = sh.S[tid][k];
+            sh.FV2[r] = f;
         }
         __syncthreads();
-        if (sh.conv) break;
+        if (tid < (n + 1) * 8) {
+            const int i = tid >> 3, k = tid & 7;
+            if (k < n) sh.S[i][k] = sh.S2[i][k];
+            if (k == 0) sh.FV[i] = sh.FV2[i];
+        }
+        __syncthreads();
+        // 2. convergence (align.rs:888-894): the value spread and every vertex
+        //    within tol of the best, one (vertex, coordinate) per thread
+        int ok = 1;
+        if (tid == 0) ok = !(fabs(sh.FV[n] - sh.FV[0]) > 1e-4 * (1.0 + fabs(sh.FV[0])));
+        if (tid >= 8 && tid < (n + 1) * 8) {
+            const int i = tid >> 3, k = tid & 7;
+            if (k < n && !(fabs(sh.S[i][k] - sh.S[0][k]) <= tol[k])) ok = 0;
+        }
+        // 3. per coordinate: the centroid of the best n, the reflection, the
+        //    expansion and the contraction (align.rs:823-862)
+        if (tid < n) {
+            const int k = tid;
+            double c = 0.0;
+            for (int i = 0; i < n; i++) c += sh.S[i][k];
+            c /= (double)n;
+            sh.C[k] = c;
+            const double xr = CLAMPK(c + 1.0 * (c - sh.S[n][k]), k);
+            const double xe = CLAMPK(c + 2.0 * (xr - c), k);
+            const double xc = CLAMPK(c + 0.5 * (sh.S[n][k] - c), k);
+            sh.XR[k] = sh.PT[0][k] = xr;
+            sh.XE[k] = sh.PT[1][k] = xe;
+            sh.XC[k] = sh.PT[2][k] = xc;
+        }
+        if (__syncthreads_and(ok)) break;
         state = ITER;
         np = spec ? 3 : 1;
     }
