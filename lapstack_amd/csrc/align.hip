@@ -421,45 +421,70 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
     unsigned *bflag = ctr + NM_FL_BCAST;
     double *bcost = part + (size_t)2 * gridDim.x * 3 * NM_MAXP + (size_t)slot * NM_MAXP;
     if (sh.folds) {
-        // the fold order depends on thread indices only, not on which block folds
+        // The fold order depends on thread indices only, not on which block
+        // folds.  All partials are loaded at once (up to 4 independent loads
+        // per thread) into LDS (the window buffer: the pixels are done), then
+        // wave w sums value k = w, w + 16, ... over the blocks, lanes in block order.
         __threadfence();
-        zero_waves(sh);
-        __syncthreads();
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        for (unsigned b0 = 0; b0 < gridDim.x; b0 += blockDim.x) {
-            const unsigned b = b0 + threadIdx.x;
-            for (int k = 0; k < nv; k++) {
-                double v = b < gridDim.x
-                               ? __hip_atomic_load(&pp[(size_t)b * 3 * NM_MAXP + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : 0.0;
-                v = wave_sum(v);
-                if (lane == 0) sh.red[k][wv] += v;
+        double *stg = (double *)boxs;
+        const int G = (int)gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int kchunk = min(nv, (NM_BOX / 2) / G);
+        for (int k0 = 0; k0 < nv; k0 += kchunk) {
+            const int kn = min(kchunk, nv - k0), total = G * kn;
+            for (int i0 = 0; i0 < total; i0 += 4 * NM_THREADS) {
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int i = i0 + u * NM_THREADS + threadIdx.x;
+                    v[u] = i < total ? __hip_atomic_load(&pp[(size_t)(i / kn) * 3 * NM_MAXP + k0 + i % kn],
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int i = i0 + u * NM_THREADS + threadIdx.x;
+                    if (i < total) stg[(i % kn) * G + i / kn] = v[u];
+                }
             }
+            __syncthreads();
+            for (int k = wv; k < kn; k += NM_WAVES) {
+                double acc = 0.0;
+                for (int b = lane; b < G; b += 64) acc += stg[k * G + b];
+                acc = wave_sum(acc);
+                if (lane == 0) sh.sums[k0 + k] = acc;
+            }
+            __syncthreads();
         }
-        fold_waves(sh, nv);
+        if ((int)threadIdx.x < np) {
+            const int q = threadIdx.x;
+            const double c = rms_of(sh.sums[3 * q], sh.sums[3 * q + 1], sh.sums[3 * q + 2]);
+            __hip_atomic_store(&bcost[q], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sh.PC[q] = c;
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
-            for (int p = 0; p < np; p++) {
-                double c = rms_of(sh.sums[3 * p], sh.sums[3 * p + 1], sh.sums[3 * p + 2]);
-                __hip_atomic_store(&bcost[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                sh.PC[p] = c;
-            }
             __threadfence();
             __hip_atomic_store(bflag, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-    } else if (threadIdx.x == 0) {
+    } else {
         // wait for the broadcast; bounded, and left at once when any block has raised the error flag
-        unsigned spins = 0;
-        while (__hip_atomic_load(bflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen + 1) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1u << 24)) {
-                atomicOr(err, 1u);
-                break;
+        if (threadIdx.x == 0) {
+            unsigned spins = 0;
+            while (__hip_atomic_load(bflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen + 1) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1u << 24)) {
+                    atomicOr(err, 1u);
+                    break;
+                }
             }
+            __threadfence();
         }
-        __threadfence();
-        for (int p = 0; p < np; p++)
-            sh.PC[p] = __hip_atomic_load(&bcost[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if ((int)threadIdx.x < np) {  // the costs share one line: one request
+            __threadfence();
+            sh.PC[threadIdx.x] = __hip_atomic_load(&bcost[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     gen++;
     if (threadIdx.x == 0) {
@@ -650,7 +675,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
                                                           double *__restrict__ part, unsigned *__restrict__ ctr,
                                                           unsigned *__restrict__ err) {
     __shared__ NmShared sh;
-    __shared__ float boxs[NM_BOX];
+    __shared__ __attribute__((aligned(16))) float boxs[NM_BOX];  // a tile's source window; the fold's staging
     const NmConst &K = *Kp;
     const int n = K.nfree;
     unsigned gen = 0;
@@ -1026,6 +1051,7 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
         f->nm_grid = (unsigned)(std::min(per_cu, want) * cus);
         f->nm_grid -= f->nm_grid % NM_GROUPS;
         LSH_REQUIRE(f->nm_grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
+        if (f->nm_grid > NM_BOX / 2) f->nm_grid = NM_BOX / 2;  // the fold stages one value per block in LDS
         f->nm_part.alloc((size_t)2 * 3 * NM_MAXP * f->nm_grid + 2 * NM_MAXP);
     }
     uint8_t *pin = (uint8_t *)pinned_scratch(sizeof(NmConst) + (24 + NM_STATE) * sizeof(double));
