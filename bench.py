@@ -44,7 +44,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-frames", type=int, default=3, help="oracle sample size (frames of the same stack)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--prof-frames", type=int, default=6, help="frames of the per-kernel timing pass")
+    ap.add_argument("--prof-frames", type=int, default=0,
+                    help="frames of the per-kernel timing pass (0: the whole stack, so the totals are the job's)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (twice) and exit: the calibration run of the "
+                         "HBM traffic counters")
     return ap.parse_args()
 
 
@@ -162,8 +166,39 @@ def wls_precision(job, n, dp, tol=1e-6):
     return out
 
 
+def measured_traffic(kernel, w, h):
+    """HBM bytes per launch of ``kernel`` at this frame size, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in profiles/traffic.json
+    (each counter scaled by its reading for a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rows = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rows:
+        if r.get("kernel") == kernel and r.get("width") == w and r.get("height") == h:
+            return r.get("hbm_bytes_per_launch")
+    return None
+
+
+def copy_probe(mib):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "copies": 2}), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported (use --gpus 1)",
               file=sys.stderr)
@@ -212,7 +247,7 @@ def main():
     # per-kernel timing pass (HIP events on the library's stream), off the timed region
     _lib.prof_reset()
     _lib.prof_enable(True)
-    job.run(fp, apm, dp, nframes=args.prof_frames)
+    job.run(fp, apm, dp, nframes=args.prof_frames or None)
     sync()
     _lib.prof_enable(False)
     prof = _lib.prof_read()
@@ -225,7 +260,7 @@ def main():
     ms, by, cnt = prof[dom]
     achieved = by / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, w, h),
                 "bytes_per_launch": round(by / cnt), "avg_launch_us": round(1000 * ms / cnt, 2)}
 
     cpu = None
