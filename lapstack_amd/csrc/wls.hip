@@ -153,64 +153,138 @@ __global__ void k_prolong(T *__restrict__ x, int w, int h, const T *__restrict__
     x[(size_t)y * w + i] += xc[(size_t)(y / 2) * cw + i / 2];
 }
 
-// depth.rs:677-702: Thomas solve of every row, one row per thread (wd null: unit weights)
-__global__ void k_solve_rows(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
-                             const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
-                             float *__restrict__ dp) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= h) return;
-    size_t o = (size_t)y * w;
+// depth.rs:677-702: the Thomas solve of every row (wd null: unit weights).
+// A block owns 16 rows and walks them in 64-column tiles staged through LDS
+// (coalesced loads and stores); one thread per row runs the recurrence over
+// the tile, in place, with the reference's arithmetic in the reference's order.
+#define FGS_R 16
+#define FGS_T 64
+__global__ __launch_bounds__(256) void k_solve_rows(float *__restrict__ u, const float *__restrict__ f,
+                                                    const float *__restrict__ wd, const float *__restrict__ a, int w,
+                                                    int h, float lam, float *__restrict__ cp, float *__restrict__ dp) {
+    __shared__ float TA[FGS_R][FGS_T + 1], TF[FGS_R][FGS_T + 1], TW[FGS_R][FGS_T + 1];
+    const int y0 = blockIdx.x * FGS_R, r = threadIdx.x;
+    const bool live = r < FGS_R && y0 + r < h;
     float prev_a = 0.0f, cprev = 0.0f, dprev = 0.0f;
-    for (int i = 0; i < w; i++) {
-        float ai = i + 1 < w ? a[o + i] : 0.0f;
-        float wdi = wd ? wd[o + i] : 1.0f;
-        float diag = wdi + lam * (prev_a + ai);
-        float lower = -lam * prev_a;
-        float upper = -lam * ai;
-        float m = diag - lower * cprev;
-        if (fabsf(m) < 1e-12f) m = 1e-12f;
-        cprev = upper / m;
-        dprev = (wdi * f[o + i] - lower * dprev) / m;
-        cp[o + i] = cprev;
-        dp[o + i] = dprev;
-        prev_a = ai;
+    for (int x0 = 0; x0 < w; x0 += FGS_T) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < FGS_R * FGS_T; i += 256) {
+            const int rr = i / FGS_T, c = i % FGS_T, y = y0 + rr, x = x0 + c;
+            if (y < h && x < w) {
+                const size_t o = (size_t)y * w + x;
+                TA[rr][c] = a[o];
+                TF[rr][c] = f[o];
+                TW[rr][c] = wd ? wd[o] : 1.0f;
+            }
+        }
+        __syncthreads();
+        if (live) {
+            const int cn = min(FGS_T, w - x0);
+            for (int c = 0; c < cn; c++) {
+                const float ai = x0 + c + 1 < w ? TA[r][c] : 0.0f;
+                const float wdi = TW[r][c];
+                const float diag = wdi + lam * (prev_a + ai);
+                const float lower = -lam * prev_a;
+                const float upper = -lam * ai;
+                float m = diag - lower * cprev;
+                if (fabsf(m) < 1e-12f) m = 1e-12f;
+                cprev = upper / m;
+                dprev = (wdi * TF[r][c] - lower * dprev) / m;
+                TA[r][c] = cprev;
+                TF[r][c] = dprev;
+                prev_a = ai;
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < FGS_R * FGS_T; i += 256) {
+            const int rr = i / FGS_T, c = i % FGS_T, y = y0 + rr, x = x0 + c;
+            if (y < h && x < w) {
+                const size_t o = (size_t)y * w + x;
+                cp[o] = TA[rr][c];
+                dp[o] = TF[rr][c];
+            }
+        }
     }
-    float nx = dp[o + w - 1];
-    u[o + w - 1] = nx;
-    for (int i = w - 2; i >= 0; i--) {
-        nx = dp[o + i] - cp[o + i] * nx;
-        u[o + i] = nx;
+    float nx = 0.0f;
+    for (int x0 = (w - 1) / FGS_T * FGS_T; x0 >= 0; x0 -= FGS_T) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < FGS_R * FGS_T; i += 256) {
+            const int rr = i / FGS_T, c = i % FGS_T, y = y0 + rr, x = x0 + c;
+            if (y < h && x < w) {
+                const size_t o = (size_t)y * w + x;
+                TA[rr][c] = cp[o];
+                TF[rr][c] = dp[o];
+            }
+        }
+        __syncthreads();
+        if (live) {
+            for (int c = min(FGS_T, w - x0) - 1; c >= 0; c--) {
+                nx = x0 + c == w - 1 ? TF[r][c] : TF[r][c] - TA[r][c] * nx;
+                TA[r][c] = nx;
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < FGS_R * FGS_T; i += 256) {
+            const int rr = i / FGS_T, c = i % FGS_T, y = y0 + rr, x = x0 + c;
+            if (y < h && x < w) u[(size_t)y * w + x] = TA[rr][c];
+        }
     }
 }
 
 // depth.rs:707-746: the same down every column, one column per thread
+// (coalesced across threads); the inputs of 16 rows are loaded ahead of the
+// 16 recurrence steps that use them
+#define FGS_C 16
 __global__ void k_solve_cols(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
                              const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
                              float *__restrict__ dp) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= w) return;
-    float cprev = 0.0f, dprev = 0.0f;
-    for (int y = 0; y < h; y++) {
-        size_t i = (size_t)y * w + x;
-        float prev_a = y > 0 ? a[i - w] : 0.0f;
-        float ai = y + 1 < h ? a[i] : 0.0f;
-        float wdi = wd ? wd[i] : 1.0f;
-        float diag = wdi + lam * (prev_a + ai);
-        float lower = -lam * prev_a;
-        float upper = -lam * ai;
-        float m = diag - lower * cprev;
-        if (fabsf(m) < 1e-12f) m = 1e-12f;
-        cprev = upper / m;
-        dprev = (wdi * f[i] - lower * dprev) / m;
-        cp[i] = cprev;
-        dp[i] = dprev;
+    float cprev = 0.0f, dprev = 0.0f, prev_a = 0.0f;
+    for (int y0 = 0; y0 < h; y0 += FGS_C) {
+        float A[FGS_C], F[FGS_C], WD[FGS_C];
+#pragma unroll
+        for (int k = 0; k < FGS_C; k++) {
+            const int y = y0 + k;
+            const size_t i = (size_t)min(y, h - 1) * w + x;
+            A[k] = y + 1 < h ? a[i] : 0.0f;
+            F[k] = f[i];
+            WD[k] = wd ? wd[i] : 1.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < FGS_C; k++) {
+            const int y = y0 + k;
+            if (y >= h) continue;
+            const size_t i = (size_t)y * w + x;
+            const float ai = A[k], wdi = WD[k];
+            const float diag = wdi + lam * (prev_a + ai);
+            const float lower = -lam * prev_a;
+            const float upper = -lam * ai;
+            float m = diag - lower * cprev;
+            if (fabsf(m) < 1e-12f) m = 1e-12f;
+            cprev = upper / m;
+            dprev = (wdi * F[k] - lower * dprev) / m;
+            cp[i] = cprev;
+            dp[i] = dprev;
+            prev_a = ai;
+        }
     }
-    float nx = dp[(size_t)(h - 1) * w + x];
-    u[(size_t)(h - 1) * w + x] = nx;
-    for (int y = h - 2; y >= 0; y--) {
-        size_t i = (size_t)y * w + x;
-        nx = dp[i] - cp[i] * nx;
-        u[i] = nx;
+    float nx = 0.0f;
+    for (int y1 = h - 1; y1 >= 0; y1 -= FGS_C) {
+        float C[FGS_C], D[FGS_C];
+#pragma unroll
+        for (int k = 0; k < FGS_C; k++) {
+            const size_t i = (size_t)max(y1 - k, 0) * w + x;
+            C[k] = cp[i];
+            D[k] = dp[i];
+        }
+#pragma unroll
+        for (int k = 0; k < FGS_C; k++) {
+            const int y = y1 - k;
+            if (y < 0) continue;
+            nx = y == h - 1 ? D[k] : D[k] - C[k] * nx;
+            u[(size_t)y * w + x] = nx;
+        }
     }
 }
 
@@ -652,21 +726,24 @@ static uint32_t pcg(WlsSolver &s, std::vector<Lev<T>> &H, T *u, const T *b, doub
     return iters;
 }
 
-// depth.rs:1019-1034: three rounds of row and column sweeps with the lambda_t schedule
+// depth.rs:1019-1034: three rounds of row and column sweeps with the lambda_t
+// schedule.  Each sweep reads the previous one's output directly (rows: d or
+// f -> u, columns: u -> f), which is what the reference's copies between them
+// amount to; the last result moves into u once.
 static void fgs_guess(WlsSolver &s, const float *d, float *u) {
     const int w = s.w, h = s.h, T = 3;
     const size_t n = (size_t)w * h;
     const float *wd = s.L[0].wd.p;
-    LSH_CHECK(hipMemcpyAsync(s.f.p, d, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    const float *src = d;
     for (int t = 1; t <= T; t++) {
         float lam_t = 1.5f * s.lambda * std::pow(4.0f, (float)(T - t)) / (std::pow(4.0f, (float)T) - 1.0f);
-        launch("fgs_rows", k_solve_rows, dim3(cdiv(h, 64)), dim3(64), 0, u, (const float *)s.f.p, t == 1 ? wd : nullptr,
-               (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
-        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-        launch("fgs_cols", k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, u, (const float *)s.f.p, nullptr,
-               (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
-        LSH_CHECK(hipMemcpyAsync(s.f.p, u, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+        launch_b("fgs_rows", 16.0 * n, k_solve_rows, dim3(cdiv(h, FGS_R)), dim3(256), 0, u, src, t == 1 ? wd : nullptr,
+                 (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
+        launch_b("fgs_cols", 16.0 * n, k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, s.f.p, (const float *)u, nullptr,
+                 (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
+        src = s.f.p;
     }
+    LSH_CHECK(hipMemcpyAsync(u, s.f.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
 }
 
 WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double tol, int mode, const float *warm,
@@ -678,7 +755,7 @@ WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double to
     WlsResult res;
     if (w == 1 || h == 1) {  // depth.rs:1009-1017: the exact 1-D solve
         if (h == 1)
-            launch("fgs_rows", k_solve_rows, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ex.p, w, h, s.lambda, s.cp.p, s.dp.p);
+            launch("fgs_rows", k_solve_rows, dim3(1), dim3(256), 0, u, d, wd, (const float *)s.ex.p, w, h, s.lambda, s.cp.p, s.dp.p);
         else
             launch("fgs_cols", k_solve_cols, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ey.p, w, h, s.lambda, s.cp.p, s.dp.p);
         if (u64) launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, u64);
