@@ -163,6 +163,33 @@ def wls_precision(job, n, dp, tol=1e-6):
         out[mode] = {"iters": rep["iters"], "outer": rep["outer"], "ms": round(rep["ms"], 2), "rel": rep["rel"],
                      "true_rel": rep["true_rel"], "max_err_vs_fp64": err}
         log(f"WLS {mode}: {out[mode]}")
+    # the learned warm start (trained weights shipped with the package), then
+    # the same reference MG-PCG to the same tolerance
+    from lapstack_amd import learn
+
+    if os.path.exists(learn.WEIGHTS):
+        net = la.WarmNet()
+        net.load(learn.WEIGHTS)
+        warm = torch.empty_like(d)
+        dscale = float(max(n - 1, 1))
+        net.predict(d, c, g, dscale, out=warm)  # sizes the activations
+        from lapstack_amd import _lib
+
+        _lib.sync()
+        t0 = time.perf_counter()
+        net.predict(d, c, g, dscale, out=warm)
+        infer_ms = 1000.0 * (time.perf_counter() - t0)
+        _, g0, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=0, warm=warm, u_out=u)
+        _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=dp.cg_iters,
+                                 warm=warm, u_out=u)
+        _, p0, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=0, u_out=u)
+        out["learned"] = {"infer_ms": round(infer_ms, 2), "iters": rep["iters"], "solve_ms": round(rep["ms"], 2),
+                          "ms": round(infer_ms + rep["ms"], 2), "rel": rep["rel"], "true_rel": rep["true_rel"],
+                          "guess_true_rel": g0["true_rel"], "fgs_guess_true_rel": p0["true_rel"]}
+        la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=dp.cg_iters, warm=warm, u_out=u)
+        out["learned"]["max_err_vs_fp64"] = float((u.double() - oracle).abs().max())
+        log(f"WLS learned: {out['learned']}")
+        net.close()
     return out
 
 

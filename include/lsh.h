@@ -244,6 +244,11 @@ char *lsh_warm_nparams(lsh_warm *net, uint32_t *count);
 char *lsh_warm_get(lsh_warm *net, float *params, uint32_t count);
 /* Load parameters (resets the optimiser state). */
 char *lsh_warm_set(lsh_warm *net, const float *params, uint32_t count);
+/* The training loss: 0 (the default) the mean squared error of the normalised
+ * prediction; 1 the energy norm of that error under each image's own WLS
+ * operator (data weights max(conf, 0) + 1e-4, edges lambda exp(-|dg| /
+ * sigma_c) of the guide), the norm MG-PCG reduces, per cell. */
+char *lsh_warm_set_loss(lsh_warm *net, uint32_t mode, float lambda, float sigma_c);
 /* nimg images of w*h planes each (image-major) in in_mem; u (nimg*w*h) in out_mem. */
 char *lsh_warm_predict(lsh_warm *net, const float *depth, const float *conf, const float *guide, lsh_mem in_mem,
                        uint32_t nimg, uint32_t w, uint32_t h, float dscale, float *u, lsh_mem out_mem);

@@ -396,6 +396,12 @@ class WarmNet:
             raise HipError(f"warm: {p.size} parameters, the network has {self.nparams}")
         _lib.check(self._lib.lsh_warm_set(self._h, p.ctypes.data_as(C.c_void_p), self.nparams))
 
+    def set_loss(self, mode: str = "mse", lam: float = 0.0, sigma_c: float = 1.0):
+        """Training loss: "mse", or "energy" (the error's energy norm under the
+        WLS operator with lambda ``lam`` and guide scale ``sigma_c``)."""
+        m = {"mse": 0, "energy": 1}[mode]
+        _lib.check(self._lib.lsh_warm_set_loss(self._h, m, float(lam), float(sigma_c)))
+
     def save(self, path: str):
         self.get().tofile(path)
 

@@ -74,6 +74,7 @@ SIGNATURES = [
     ("lsh_warm_nparams", _p, [_p, C.POINTER(_u32)]),
     ("lsh_warm_get", _p, [_p, _p, _u32]),
     ("lsh_warm_set", _p, [_p, _p, _u32]),
+    ("lsh_warm_set_loss", _p, [_p, _u32, C.c_float, C.c_float]),
     ("lsh_warm_predict", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, _p, C.c_int]),
     ("lsh_warm_grads", _p, [_p, _p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, C.POINTER(C.c_double), _p,
                             _u32]),

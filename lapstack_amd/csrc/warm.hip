@@ -373,6 +373,58 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, c
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// The error of the normalised prediction, e = d / dscale + out - t / dscale.
+__global__ void k_err(const float *__restrict__ out32, const float *__restrict__ d, const float *__restrict__ t,
+                      size_t n, float inv_scale, float *__restrict__ e) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) e[i] = (d[i] * inv_scale + out32[i]) - t[i] * inv_scale;
+}
+
+// The energy norm of the error under the WLS operator of each image (the
+// norm MG-PCG reduces): sum over cells of w e^2 + lambda a (e - e_right)^2 +
+// lambda a (e - e_below)^2, with w = max(conf, 0) + 1e-4 and a =
+// exp(-|g - g'| / sigma) as the solver builds them (depth.rs:653-672, 994),
+// over each image with free borders.  Its gradient 2 (A e) / N goes into
+// channel 0 of dout; per-block f64 partial sums of the loss.
+__global__ __launch_bounds__(256) void k_energy(const float *__restrict__ e, const float *__restrict__ conf,
+                                                const float *__restrict__ g, int H, int W, size_t n, float lambda,
+                                                float inv_sigma, float gscale, View dout,
+                                                double *__restrict__ part) {
+    __shared__ double sh[256];
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        const int x = (int)(i % W), y = (int)((i / W) % H);
+        const float ei = e[i], gi = g[i];
+        const float wi = fmaxf(conf[i], 0.0f) + 1e-4f;
+        float ae = wi * ei;
+        s = (double)wi * ei * ei;
+        if (x + 1 < W) {
+            const float a = lambda * expf(fabsf(gi - g[i + 1]) * inv_sigma), de = ei - e[i + 1];
+            ae += a * de;
+            s += (double)a * de * de;
+        }
+        if (y + 1 < H) {
+            const float a = lambda * expf(fabsf(gi - g[i + W]) * inv_sigma), de = ei - e[i + W];
+            ae += a * de;
+            s += (double)a * de * de;
+        }
+        if (x > 0) ae += lambda * expf(fabsf(g[i - 1] - gi) * inv_sigma) * (ei - e[i - 1]);
+        if (y > 0) ae += lambda * expf(fabsf(g[i - W] - gi) * inv_sigma) * (ei - e[i - W]);
+        uint16_t *o = dout.p + i * dout.cs;
+        o[0] = f2bf(gscale * ae);
+#pragma unroll
+        for (int c = 1; c < 16; c++) o[c] = 0;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // Adam on the f32 master weights (bias-corrected step sizes from the host)
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
                        size_t n, float lr_t, float b1, float b2, float eps_t) {
@@ -430,6 +482,9 @@ struct Acts {
 
 struct lsh_warm {
     std::vector<lsh::Layer> L;
+    int loss_mode = 0;  // 0: mean squared error, 1: the WLS operator's energy norm
+    float loss_lambda = 0.0f, loss_sigma = 1.0f;
+    lsh::DBuf<float> ebuf;
     size_t npad = 0, nnat = 0;
     lsh::DBuf<float> w32, g32, m, v;
     uint64_t step = 0;
@@ -615,7 +670,8 @@ static void weight_grad(lsh_warm &n, int l, const View &in, const View &dz, int 
 }
 
 // loss and gradients of the batch (forward first); returns the mean squared error
-static double backward(lsh_warm &n, const float *d, const float *t, int nimg, float dscale) {
+static double backward(lsh_warm &n, const float *d, const float *conf, const float *g, const float *t, int nimg,
+                       float dscale) {
     Acts &a = n.A;
     const int *H = a.H, *W = a.W;
     auto &L = n.L;
@@ -624,8 +680,15 @@ static double backward(lsh_warm &n, const float *d, const float *t, int nimg, fl
     View dOUT = view(a.dOUT, H[0], W[0], 16, 16);
     const int lb = (int)cdiv(n0, 256);
     n.lpart.ensure(lb);
-    launch("warm_loss", k_loss, dim3(lb), dim3(256), 0, (const float *)a.out32.p, d, t, n0, 1.0f / dscale,
-           2.0f / (float)n0, dOUT, n.lpart.p);
+    if (n.loss_mode == 0) {
+        launch("warm_loss", k_loss, dim3(lb), dim3(256), 0, (const float *)a.out32.p, d, t, n0, 1.0f / dscale,
+               2.0f / (float)n0, dOUT, n.lpart.p);
+    } else {
+        n.ebuf.ensure(n0);
+        launch("warm_loss", k_err, dim3(lb), dim3(256), 0, (const float *)a.out32.p, d, t, n0, 1.0f / dscale, n.ebuf.p);
+        launch("warm_loss", k_energy, dim3(lb), dim3(256), 0, (const float *)n.ebuf.p, conf, g, H[0], W[0], n0,
+               n.loss_lambda, -1.0f / fmaxf(n.loss_sigma, 1e-6f), 2.0f / (float)n0, dOUT, n.lpart.p);
+    }
     View X0 = view(a.X0, H[0], W[0], 16, 16);
     View A0a = view(a.A0a, H[0], W[0], 16, 16), S0 = view(a.C0, H[0], W[0], 16, 48, 32), C0 = view(a.C0, H[0], W[0], 48, 48);
     View P0 = view(a.P0, H[1], W[1], 16, 16), A1a = view(a.A1a, H[1], W[1], 32, 32);
@@ -794,6 +857,17 @@ char *lsh_warm_set(lsh_warm *n, const float *params, uint32_t count) {
     LSH_API_END
 }
 
+char *lsh_warm_set_loss(lsh_warm *n, uint32_t mode, float lambda, float sigma_c) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(n, "lsh_warm_set_loss: null handle");
+    LSH_REQUIRE(mode <= 1, "warm: unknown loss");
+    LSH_REQUIRE(mode == 0 || (lambda >= 0.0f && sigma_c > 0.0f), "warm: the energy loss needs lambda >= 0, sigma_c > 0");
+    n->loss_mode = (int)mode;
+    n->loss_lambda = lambda;
+    n->loss_sigma = sigma_c;
+    LSH_API_END
+}
+
 char *lsh_warm_predict(lsh_warm *n, const float *d, const float *conf, const float *guide, lsh_mem in_mem,
                        uint32_t nimg, uint32_t w, uint32_t h, float dscale, float *u, lsh_mem out_mem) {
     LSH_API_BEGIN
@@ -828,7 +902,7 @@ char *lsh_warm_grads(lsh_warm *n, const float *d, const float *conf, const float
     Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
     ensure_acts(*n, (int)nimg, (int)h, (int)w, true);
     forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
-    double l = backward(*n, in.d, in.t, (int)nimg, dscale);
+    double l = backward(*n, in.d, in.conf, in.g, in.t, (int)nimg, dscale);
     if (loss) *loss = l;
     if (grads) {
         std::vector<float> pad(n->npad);
@@ -848,7 +922,7 @@ char *lsh_warm_train_step(lsh_warm *n, const float *d, const float *conf, const 
     Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
     ensure_acts(*n, (int)nimg, (int)h, (int)w, true);
     forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
-    double l = backward(*n, in.d, in.t, (int)nimg, dscale);
+    double l = backward(*n, in.d, in.conf, in.g, in.t, (int)nimg, dscale);
     n->step++;
     const float b1 = 0.9f, b2 = 0.999f;
     double c1 = 1.0 - std::pow((double)b1, (double)n->step), c2 = 1.0 - std::pow((double)b2, (double)n->step);

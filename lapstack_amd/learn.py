@@ -85,11 +85,11 @@ def solve_pair(net, g, d, c, dscale, dp, mode="reference"):
     la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "fp64", tol=1e-12, max_iters=2000, u_out=u, u64_out=x)
     oracle = x.clone()
     res = {}
-    la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=0, u_out=u)  # the FGS guess alone
+    _, rep0, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=0, u_out=u)  # the FGS guess
     fgs_err = float((u.double() - oracle).abs().max())
     _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=1000, u_out=u)
     res["plain"] = dict(iters=rep["iters"], ms=rep["ms"], true_rel=rep["true_rel"],
-                        err=float((u.double() - oracle).abs().max()), guess_err=fgs_err)
+                        err=float((u.double() - oracle).abs().max()), guess_err=fgs_err, guess_rel=rep0["true_rel"])
     warm = torch.empty_like(d)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -98,8 +98,9 @@ def solve_pair(net, g, d, c, dscale, dp, mode="reference"):
     e1.record()
     torch.cuda.synchronize()
     infer_ms = e0.elapsed_time(e1)
+    _, repw, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=0, warm=warm, u_out=u)
     _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=1000, warm=warm, u_out=u)
-    res["learned"] = dict(iters=rep["iters"], ms=rep["ms"] + infer_ms, solve_ms=rep["ms"], infer_ms=infer_ms,
+    res["learned"] = dict(guess_rel=repw["true_rel"], iters=rep["iters"], ms=rep["ms"] + infer_ms, solve_ms=rep["ms"], infer_ms=infer_ms,
                           true_rel=rep["true_rel"], err=float((u.double() - oracle).abs().max()),
                           guess_err=float((warm.double() - oracle).abs().max()))
     return res
@@ -147,6 +148,8 @@ def train(args):
     N, H, W = D.shape
     dscale = 7.0  # 8 frames: frame index range n - 1
     net = la.WarmNet(seed=args.seed)
+    if args.loss == "energy":
+        net.set_loss("energy", dp.lam, dp.sigma_c)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(args.seed)
     cs, B = args.crop, args.batch
@@ -180,7 +183,7 @@ def train(args):
     infer = infer_timing(net, args.infer_grid) if args.infer_grid else None
     summary = {
         "infer": infer,
-        "train": {"stacks": args.stacks, "grid": [W, H], "steps": args.steps, "batch": B, "crop": cs,
+        "train": {"loss": args.loss, "stacks": args.stacks, "grid": [W, H], "steps": args.steps, "batch": B, "crop": cs,
                   "final_loss": float(np.mean(losses[-250:])), "train_s": round(time.perf_counter() - t1, 1)},
         "eval_stacks": args.eval,
         "plain": {k: float(np.mean([r["plain"][k] for r in rows])) for k in rows[0]["plain"]},
@@ -200,6 +203,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20_000)
     ap.add_argument("--eval", type=int, default=8)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--loss", choices=["mse", "energy"], default="energy",
+                    help="energy: the error's energy norm under the WLS operator (what MG-PCG reduces)")
     ap.add_argument("--infer-grid", default="4128x2752", help="time one prediction on this grid ('' to skip)")
     args = ap.parse_args()
     sys.exit(0 if train(args) else 1)

@@ -7,12 +7,15 @@ The network has no counterpart in the reference (whose PCG starts from the FGS
 sweeps, crates/lapstack-core/src/depth.rs:1019-1034); this file states it in
 numpy f64: zero-padded 3x3 convolutions, ReLU, 2x2 average pooling over the
 cells inside the finer grid, nearest upsampling, channel concatenation, the
-MSE loss of the normalised prediction, and the backward pass by the chain
-rule.  With ``emulate=True`` it rounds to bf16 exactly where the device stores
+loss of the normalised prediction (mean squared error, or the error's energy
+norm under the WLS operator of depth.rs:653-672/994), and the backward pass by
+the chain rule.  With ``emulate=True`` it rounds to bf16 exactly where the device stores
 bf16 (weights, the packed input, every activation and every activation
 gradient), so the two differ only by the order of f32 accumulation.
 
-Pinned by: central finite differences of the f64 loss (tests/test_oracle_warm.py),
+Pinned by: central finite differences of the f64 loss, both losses, and the
+energy norm against e^T A e with A assembled as a scipy sparse matrix
+(tests/test_oracle_warm.py),
 the convolution against a direct 7-loop sum, and pooling/upsampling as
 adjoints (<P x, y> = <x, P* y>).
 """
@@ -179,9 +182,31 @@ def forward(params, d, conf, g, dscale, emulate=False):
     return dn + out, c
 
 
-def loss_and_grads(params, d, conf, g, target, dscale, emulate=False):
-    """Mean squared error of the normalised prediction against target / dscale,
-    and its gradient with respect to the flat parameters."""
+def energy(e, conf, g, lam, sigma):
+    """The energy norm per cell of the error planes e (B, H, W) under each
+    image's WLS operator (data weights max(conf, 0) + 1e-4, edges
+    lam * exp(-|dg| / sigma)), and its gradient 2 (A e) / N."""
+    e = np.asarray(e, np.float64)
+    conf = np.asarray(conf, np.float64).reshape(e.shape)
+    g = np.asarray(g, np.float64).reshape(e.shape)
+    w = np.maximum(conf, 0.0) + 1e-4
+    ax = lam * np.exp(-np.abs(g[:, :, 1:] - g[:, :, :-1]) / sigma)
+    ay = lam * np.exp(-np.abs(g[:, 1:] - g[:, :-1]) / sigma)
+    dx, dy = e[:, :, :-1] - e[:, :, 1:], e[:, :-1] - e[:, 1:]
+    N = e.size
+    loss = (np.sum(w * e * e) + np.sum(ax * dx * dx) + np.sum(ay * dy * dy)) / N
+    Ae = w * e
+    Ae[:, :, :-1] += ax * dx
+    Ae[:, :, 1:] -= ax * dx
+    Ae[:, :-1] += ay * dy
+    Ae[:, 1:] -= ay * dy
+    return loss, 2.0 * Ae / N
+
+
+def loss_and_grads(params, d, conf, g, target, dscale, emulate=False, loss="mse", lam=0.0, sigma=1.0):
+    """The loss of the normalised prediction against target / dscale (mean
+    squared error, or the energy norm under the WLS operator) and its gradient
+    with respect to the flat parameters."""
     R = bf16 if emulate else (lambda a: a)
     pred, c = forward(params, d, conf, g, dscale, emulate)
     L = c["L"]
@@ -193,14 +218,21 @@ def loss_and_grads(params, d, conf, g, target, dscale, emulate=False):
         pred = pred.astype(np.float32).astype(np.float64)
     e = pred - tn
     N = e.size
-    loss = float(np.mean(e * e))
+    if emulate:
+        e = e.astype(np.float32).astype(np.float64)
+    if loss == "energy":
+        cf = np.asarray(conf, np.float32)
+        gg = np.asarray(g, np.float32)
+        loss_v, de = energy(e, cf.reshape(e.shape), gg.reshape(e.shape), lam, sigma)
+    else:
+        loss_v, de = float(np.mean(e * e)), (2.0 / N) * e
     grads = [None] * len(L)
     pos = lambda a: (a > 0).astype(np.float64)
 
     def layer(i, x, dz):
         grads[i] = conv_grads(x, L[i][0], dz)
 
-    dOUT = R((2.0 / N) * e)[..., None]
+    dOUT = R(de)[..., None]
     layer(11, c["D0"], dOUT)
     dD0 = R(conv_dx(dOUT, L[11][0]) * pos(c["D0"]))
     layer(10, c["C0"], dD0)
@@ -230,4 +262,4 @@ def loss_and_grads(params, d, conf, g, target, dscale, emulate=False):
     layer(1, c["A0a"], dS0)
     dA0a = R(conv_dx(dS0, L[1][0]) * pos(c["A0a"]))
     layer(0, c["X0"], dA0a)
-    return loss, flatten(grads)
+    return float(loss_v), flatten(grads)

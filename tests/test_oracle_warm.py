@@ -52,7 +52,8 @@ def test_adjoints(H, W):
     assert np.allclose(wn.pool(np.ones((1, H, W, 2))), 1.0)
 
 
-def test_backward_matches_finite_differences():
+@pytest.mark.parametrize("loss", ["mse", "energy"])
+def test_backward_matches_finite_differences(loss):
     rng = np.random.default_rng(5)
     B, H, W = 1, 10, 12
     d = rng.uniform(0, 7, (B, H, W)).astype(np.float32)
@@ -60,8 +61,9 @@ def test_backward_matches_finite_differences():
     g = rng.uniform(0, 1, (B, H, W)).astype(np.float32)
     t = (d + rng.normal(0, 0.5, (B, H, W))).astype(np.float32)
     p = wn.init(3)
-    loss, grad = wn.loss_and_grads(p, d, conf, g, t, 7.0)
-    f = lambda q: wn.loss_and_grads(q, d, conf, g, t, 7.0)[0]
+    kw = dict(loss=loss, lam=3.0, sigma=0.2)
+    value, grad = wn.loss_and_grads(p, d, conf, g, t, 7.0, **kw)
+    f = lambda q: wn.loss_and_grads(q, d, conf, g, t, 7.0, **kw)[0]
     offs = np.cumsum([0] + [ci * co * 9 + co for ci, co, _ in wn.SPECS])
     idx = []
     for li in range(len(wn.SPECS)):  # two weights and a bias of every layer
@@ -74,7 +76,27 @@ def test_backward_matches_finite_differences():
         e[i] = eps
         num = (f(p + e) - f(p - e)) / (2 * eps)
         assert abs(num - grad[i]) <= 1e-6 * max(1.0, abs(num)) + 1e-9, (i, num, grad[i])
-    assert loss > 0
+    assert value > 0
+
+
+def test_energy_is_the_quadratic_form_of_the_wls_matrix():
+    from tests.fixtures import wls_matrix
+
+    rng = np.random.default_rng(8)
+    B, H, W = 2, 7, 9
+    e = rng.standard_normal((B, H, W))
+    conf = rng.uniform(-0.2, 1.0, (B, H, W)).astype(np.float32)
+    g = rng.uniform(0, 1, (B, H, W)).astype(np.float32)
+    loss, grad = wn.energy(e, conf, g, 3.0, 0.2)
+    want, wgrad = 0.0, []
+    for b in range(B):
+        A = wls_matrix(g[b], conf[b], 3.0, 0.2)
+        v = e[b].ravel()
+        want += v @ (A @ v)
+        wgrad.append((2.0 * (A @ v) / e.size).reshape(H, W))
+    assert loss == pytest.approx(want / e.size, rel=1e-6)
+    # the fixture builds the edge weights in f32 (as the solver does), the oracle in f64
+    assert np.allclose(grad, np.stack(wgrad), rtol=1e-6, atol=1e-7)
 
 
 def test_bf16_rounding():
