@@ -9,7 +9,10 @@ frame is registered against the previous aligned one (Nelder-Mead on the
 frame 0's brightness, folded into the 7-level region-energy pyramid and
 measured (RDF focus slice); after the last frame the residuals are fused, the
 pyramid collapsed, and the depth pass runs (guided filter r=8 eps=1e-3, peak
-search, MG-PCG WLS to a 1e-6 relative residual, guided upsampling).
+search, MG-PCG WLS to a 1e-6 relative residual, guided upsampling).  Every
+frame after the first runs as one replayed hipGraph with no host round trip
+(lsh_stack_stream); the same step launched per frame, and the host-steered
+per-frame path, are timed once each beside it.
 
 Prints one JSON line (the last line of stdout).  --gpus above 1 exits: this
 backend runs on one GPU.
@@ -76,11 +79,19 @@ class Job:
         self.conf = torch.empty((h, w), dtype=torch.float32, device="cuda")
         self.floor = torch.empty((dh, dw), dtype=torch.float32, device="cuda")
 
-    def run(self, fp, apm, dp, nframes=None):
-        """stack.rs fuse_and_depth over the resident frames; returns the per-frame report."""
+    def run(self, fp, apm, dp, nframes=None, path="graph"):
+        """stack.rs fuse_and_depth over the resident frames; returns the per-frame report.
+        path: "graph" (lsh_stack_stream, the per-frame step replayed as one hipGraph),
+        "launches" (the same device-only step launched per frame) or "host"
+        (lsh_frames_process per frame, the map and gains derived on the host)."""
         import lapstack_amd as la
 
         n = nframes or len(self.frames)
+        if path != "host":
+            out = la.stack_stream(self.frames[:n], fp, apm, dp, graph=path == "graph", slices_out=self.slices[:n],
+                                  img_out=self.image, winner_out=self.winner)
+            dm = la.depth_from_slices(self.slices[:n], n, self.image, dp, self.depth, self.conf, self.floor)
+            return dict(evals=int(out["evals"].sum()), sims=out["sims"], info=dm.info)
         fr = la.HipFrames(self.w, self.h, apm.interp)
         fu = la.Fuser(self.w, self.h, fp)
         last, evals, sims = la.SIM_ID, 0, []
@@ -271,10 +282,20 @@ def main():
 
     wls = wls_precision(job, n, dp)
 
+    # the same step through the other two per-frame paths, once each (untimed by the contract)
+    paths = {"graph": round(ms_per_step, 2)}
+    for path in ("launches", "host"):
+        sync()
+        tp = time.perf_counter()
+        job.run(fp, apm, dp, path=path)
+        sync()
+        paths[path] = round(1000.0 * (time.perf_counter() - tp), 2)
+        log(f"{path} path: {paths[path]} ms per step")
+
     # per-kernel timing pass (HIP events on the library's stream), off the timed region
     _lib.prof_reset()
     _lib.prof_enable(True)
-    job.run(fp, apm, dp, nframes=args.prof_frames or None)
+    job.run(fp, apm, dp, nframes=args.prof_frames or None, path="launches")
     sync()
     _lib.prof_enable(False)
     prof = _lib.prof_read()
@@ -311,13 +332,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{n} frames {w}x{h} RGB u16 focus bracket with similarity jitter: align (NM on "
                                f"1/4-res luma, coarsen 2, bilinear resample) + fuse (7 levels, 3x3 energy) + depth "
-                               f"(RDF, guided filter r=8 eps=1e-3, MG-PCG WLS to 1e-6)",
-                   "frames": n, "width": w, "height": h, "inputs": "resident in HBM"},
+                               f"(RDF, guided filter r=8 eps=1e-3, MG-PCG WLS to 1e-6); frames 1..{n - 1} each one "
+                               f"replay of a captured hipGraph",
+                   "frames": n, "width": w, "height": h, "inputs": "resident in HBM", "per_frame_path": "graph"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,
         "align_evals_per_frame": round(last["evals"] / max(n - 1, 1), 1),
         "wls": last["info"],
+        "per_frame_paths_ms_per_step": paths,
         "wls_precision": wls,
         "reference_published": {"note": "RTX 3060 + 128-thread host, 100x8280x5520 TIFF stack incl. decode "
                                         "(docs/engine.md:621-622)", "cuda_frames_per_s": round(100 / 26.0, 3),

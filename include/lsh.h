@@ -177,6 +177,24 @@ char *lsh_fuser_finish(lsh_fuser *f, float *img3, lsh_mem img_mem, float *winner
                        uint16_t *winner_level);
 void lsh_fuser_free(lsh_fuser *f);
 
+/* ---- a whole stack, streamed ---------------------------------------------- */
+
+/* stack.rs:740-826 fuse_range over n frames (frames[i]: interleaved RGB u16 in
+ * frames_mem) into a fresh frames handle and a fresh fuser, with no host round
+ * trip per frame after the first: frame 0 takes lsh_frames_process (identity,
+ * brightness anchor), every later frame registers against the previous aligned
+ * frame from its Sim (the reference's guess), and its map and brightness gains
+ * are derived on the device, then warp, focus slice (measure non-NULL: n slices
+ * of dw*dh into slices, slices_mem) and the fold.  graph != 0: the per-frame step
+ * is captured once as a hipGraph and replayed for every frame (the frame staged
+ * into a fixed buffer first); graph == 0: the same kernels launched per frame.
+ * Per frame (nullable): the Sim (n*8), the gains (n*3), the search's
+ * evaluations (n; 0 for frame 0).  Finish the fuser with lsh_fuser_finish. */
+char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *frames, uint32_t n, lsh_mem frames_mem,
+                       const uint8_t *free8, uint32_t coarsen, uint32_t brightness, const lsh_depth_params *measure,
+                       float *slices, lsh_mem slices_mem, uint32_t graph, double *sims_out, float *gains_out,
+                       uint64_t *evals_out);
+
 /* ---- depth ------------------------------------------------------------ */
 
 /* depth.rs:1119-1200 depth_from_slices: n slices of dw*dh (contiguous, frame

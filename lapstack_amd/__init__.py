@@ -16,6 +16,8 @@ lapstack_amd/liblsh.so (declared in include/lsh.h):
 * ``wls_solve``            -- depth.rs:1081 wls_solve, in three precisions
                               (reference FP32, FP64-refined FP32, FP64).
 * ``fuse_and_depth``       -- stack.rs:740-853: the per-frame loop over a stack.
+* ``stack_stream``         -- the same loop with no host round trip per frame,
+                              the per-frame step replayed as a hipGraph.
 
 Buffers are numpy arrays (host memory) or torch tensors on the HIP device
 (device memory, passed by pointer, no copy).  There is no CPU fallback: every
@@ -34,7 +36,7 @@ from . import _lib
 from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
-           "depth_from_slices", "depth_grid", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "depth_from_slices", "depth_grid", "stack_stream", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -496,6 +498,50 @@ def wls_solve(guide, conf, d, lam: float, sigma_c: float, mode: str = "reference
                                  int(want_err), up, u64p, umem, C.byref(rep)))
     del gk, ck, dk, wk, uk
     return u_out, {k: getattr(rep, k) for k, _ in rep._fields_}, u64_out
+
+
+def stack_stream(frames: Sequence, fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,
+                 depth: Optional[DepthParams] = None, brightness: bool = True, graph: bool = True, slices_out=None,
+                 img_out=None, winner_out=None):
+    """stack.rs:740-826 over a whole stack with no host round trip per frame
+    (include/lsh.h lsh_stack_stream): frame 0 through the ordinary path, every
+    later frame through the device-only step, captured once as a hipGraph and
+    replayed per frame (``graph``), then the collapse.  ``frames``: (h, w, 3)
+    uint16 frames, all numpy or all device tensors.  Returns a dict with image,
+    winner, sims, gains, evals and slices (when ``depth`` is given)."""
+    lib = _lib.load()
+    fuse = fuse or FuseParams()
+    align = align or AlignParams()
+    n = len(frames)
+    if n == 0:
+        raise HipError("no input images")
+    h, w, _ = tuple(frames[0].shape)
+    fr = HipFrames(w, h, align.interp)
+    fu = Fuser(w, h, fuse)
+    bufs = [_buf(f, np.uint16) for f in frames]
+    if len({b[1] for b in bufs}) != 1:
+        raise HipError("stream: frames must share one memory")
+    ptrs = (C.c_void_p * n)(*[b[0].value if isinstance(b[0], C.c_void_p) else b[0] for b in bufs])
+    free = (C.c_uint8 * 8)(*align.free())
+    dp = depth._c() if depth is not None else None
+    sp, smem, sk = None, LSH_HOST, None
+    if depth is not None:
+        dw, dh = depth.grid(w, h)
+        if slices_out is None:
+            slices_out = np.zeros((n, dh, dw), np.float32)
+        sp, smem, sk = _buf(slices_out, np.float32)
+    sims = np.zeros((n, 8), np.float64)
+    gains = np.zeros((n, 3), np.float32)
+    evals = np.zeros(n, np.uint64)
+    _lib.check(lib.lsh_stack_stream(fr._h, fu._h, ptrs, n, bufs[0][1], free, align.coarsen, int(brightness),
+                                    C.byref(dp) if dp is not None else None, sp, smem, int(graph),
+                                    sims.ctypes.data_as(C.c_void_p), gains.ctypes.data_as(C.c_void_p),
+                                    evals.ctypes.data_as(C.c_void_p)))
+    image, winner = fu.finish(img_out, winner_out)
+    fr.close()
+    fu.close()
+    del bufs, sk
+    return dict(image=image, winner=winner, sims=sims, gains=gains, evals=evals, slices=slices_out)
 
 
 def fuse_and_depth(frames: Sequence, fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,

@@ -80,6 +80,8 @@ SIGNATURES = [
                             _u32]),
     ("lsh_warm_train_step", _p, [_p, _p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, C.c_float,
                                  C.POINTER(C.c_double)]),
+    ("lsh_stack_stream", _p, [_p, _p, _p, _u32, C.c_int, _p, _u32, _u32, C.POINTER(DepthParamsC), _p, C.c_int, _u32,
+                              _p, _p, _p]),
     ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
                            _p, _p, C.c_int, C.POINTER(WlsReportC)]),
 ]

@@ -834,15 +834,62 @@ struct Gains {
     int on[3];
 };
 
+// A frame's map and gains where the device derives them itself (the streaming
+// path, no host round trip): the inverse of the searched Sim and the
+// brightness gains from the device sums.
+struct FrameDev {
+    Inv9 inv;
+    int affine, is_id;
+    Gains g;
+    float gains[3];
+};
+
+// align.rs:241-286: the Sim the search left in state[0..8), inverted (device sin/cos)
+__global__ void k_frame_inv(const double *__restrict__ state, int w, int h, FrameDev *__restrict__ fp) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[8];
+    for (int k = 0; k < 8; k++) s[k] = state[k];
+    sim_inverse_dev(s, w, h, fp->inv.m);
+    fp->affine = fp->inv.m[6] == 0.0 && fp->inv.m[7] == 0.0 && fp->inv.m[8] == 1.0;
+    fp->is_id = s[0] == 0.0 && s[1] == 0.0 && s[2] == 1.0 && s[3] == 0.0 && s[4] == 1.0 && s[5] == 0.0 &&
+                s[6] == 0.0 && s[7] == 0.0;
+}
+
+// brightness.rs:93-98 and :119-133 on the device: the gains from the sums the
+// brightness kernel left (frame means in [0..3), frame 0's in [3..6), count
+// in [6]); sums null: unit gains
+__global__ void k_frame_gains(const double *__restrict__ sums, int enabled, FrameDev *__restrict__ fp) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float g[3] = {1.0f, 1.0f, 1.0f};
+    if (sums && enabled && sums[6] >= 64.0)
+        for (int c = 0; c < 3; c++) {
+            float v = sums[c] > 1e-9 ? (float)(sums[3 + c] / sums[c]) : 1.0f;
+            g[c] = v < 0.25f ? 0.25f : (v > 4.0f ? 4.0f : v);
+        }
+    bool unity = true;
+    for (int c = 0; c < 3; c++)
+        if (!(fabsf(g[c] - 1.0f) < 1e-6f)) unity = false;
+    for (int c = 0; c < 3; c++) {
+        fp->g.g[c] = g[c];
+        fp->g.on[c] = enabled && !unity && fabsf(g[c] - 1.0f) > 1e-6f;
+        fp->gains[c] = g[c];
+    }
+}
+
 // The final resample of a frame (align.rs:554-559) with the as-shot fallback
 // outside the frame (stack.rs:669-678), the brightness gains (brightness.rs:
 // 124-133), and optionally the warped registration luma (stack.rs:527).
 template <int N>
 __global__ __launch_bounds__(256) void k_warp(const uint16_t *__restrict__ rgb, const float *__restrict__ yin, int w,
                                               int h, Inv9 inv, int affine, int ik, Gains g, float *__restrict__ planes,
-                                              float *__restrict__ yout) {
+                                              float *__restrict__ yout, const FrameDev *__restrict__ fp) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
+    if (fp) {
+        inv = fp->inv;
+        affine = fp->affine;
+        g = fp->g;
+    }
     size_t n = (size_t)w * h, o = (size_t)y * w + x;
     double sx, sy;
     src_point(inv, affine, x, y, sx, sy);
@@ -889,7 +936,13 @@ template <int N>
 __global__ __launch_bounds__(256) void k_bright_sums(const uint16_t *__restrict__ rgb, int w, int h, Inv9 inv,
                                                      int affine, int ik, int mode, const float *__restrict__ grid,
                                                      int gw, int gh, double *__restrict__ part,
-                                                     unsigned *__restrict__ ticket, double *__restrict__ out) {
+                                                     unsigned *__restrict__ ticket, double *__restrict__ out,
+                                                     const FrameDev *__restrict__ fp) {
+    if (fp) {
+        inv = fp->inv;
+        affine = fp->affine;
+        mode = fp->is_id;
+    }
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     size_t ng = (size_t)gw * gh;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += (size_t)gridDim.x * blockDim.x) {
@@ -929,30 +982,33 @@ __global__ __launch_bounds__(256) void k_bright_sums(const uint16_t *__restrict_
 static int taps_n(uint32_t ik) { return ik <= 1 ? 2 : (ik <= 3 ? 4 : 6); }
 
 static void warp_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int affine, const Gains &g,
-                        float *yout) {
+                        float *yout, const FrameDev *fp = nullptr) {
     dim3 gr(cdiv(f->w, 64), cdiv(f->h, 4)), b(64, 4);
     const float *yin = f->yt.p;
     int ik = (int)f->interp, w = (int)f->w, h = (int)f->h;
     // u16 frame read, 3 f32 planes written; with the luma: luma read, warped luma written
     double by = (double)f->w * f->h * (yout ? 26.0 : 18.0);
     switch (taps_n(f->interp)) {
-    case 2: launch_b("warp", by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout); break;
-    case 4: launch_b("warp", by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout); break;
-    default: launch_b("warp", by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout); break;
+    case 2: launch_b("warp", by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout, fp); break;
+    case 4: launch_b("warp", by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout, fp); break;
+    default: launch_b("warp", by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout, fp); break;
     }
 }
 
-static void sums_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int affine, int mode, double *host7) {
+// host7 null: the sums stay in red_out for a device consumer (no round trip)
+static void sums_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int affine, int mode, double *host7,
+                        const FrameDev *fp = nullptr) {
     size_t ng = f->gw * f->gh;
     unsigned grid = std::max(1u, std::min<unsigned>(cdiv(ng, 256), 1024));
     f->red.ensure((size_t)grid * 7);
     const float *gp = f->have_grid ? f->grid.p : nullptr;
     int ik = (int)f->interp, w = (int)f->w, h = (int)f->h, gw = (int)f->gw, gh = (int)f->gh;
     switch (taps_n(f->interp)) {
-    case 2: launch("bright_sums", k_bright_sums<2>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p); break;
-    case 4: launch("bright_sums", k_bright_sums<4>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p); break;
-    default: launch("bright_sums", k_bright_sums<6>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p); break;
+    case 2: launch("bright_sums", k_bright_sums<2>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
+    case 4: launch("bright_sums", k_bright_sums<4>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
+    default: launch("bright_sums", k_bright_sums<6>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
     }
+    if (!host7) return;
     double *pin = (double *)pinned_scratch(64);
     LSH_CHECK(hipMemcpyAsync(pin, f->red_out.p, 7 * sizeof(double), hipMemcpyDeviceToHost, stream()));
     sync();
@@ -1007,10 +1063,13 @@ static const double SPAN[8] = {0.10, 0.10, 0.10, 0.087266462599716474, 0.05, 0.0
 
 // multiscale_align (align.rs:896-942) through the cooperative search kernel:
 // the target pyramid in yt, the reference in yr, the guess in, the Sim out.
-static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8, size_t coarsen) {
+// The search's constants (levels, steps, tolerances, free parameters) on the
+// device, and its buffers; the search box follows the guess in the kernel.
+static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
     size_t n = f->lw.size();  // levels 0..n-1, level 0 the full-resolution plane
     LSH_REQUIRE(n <= NM_MAXL, "registration pyramid deeper than the search supports");
-    NmConst K{};
+    NmConst K;
+    std::memset(&K, 0, sizeof(K));
     K.nlev = (int)n;
     K.finest = (int)std::min(coarsen, n - 1);
     K.solo_max = 4096;
@@ -1054,18 +1113,42 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
         if (f->nm_grid > NM_BOX / 2) f->nm_grid = NM_BOX / 2;  // the fold stages one value per block in LDS
         f->nm_part.alloc((size_t)2 * 3 * NM_MAXP * f->nm_grid + 2 * NM_MAXP);
     }
-    uint8_t *pin = (uint8_t *)pinned_scratch(sizeof(NmConst) + (24 + NM_STATE) * sizeof(double));
+    // upload only what changed (a stack's frames share one set)
+    if (f->nm_host.size() == sizeof(NmConst) && std::memcmp(f->nm_host.data(), &K, sizeof(NmConst)) == 0) return;
+    f->nm_host.assign((const uint8_t *)&K, (const uint8_t *)&K + sizeof(NmConst));
+    uint8_t *pin = (uint8_t *)pinned_scratch(sizeof(NmConst));
     std::memcpy(pin, &K, sizeof(NmConst));
-    double *st = (double *)(pin + sizeof(NmConst));
-    std::memcpy(st, init.v, 8 * sizeof(double));
     LSH_CHECK(hipMemcpyAsync(f->nm_const.p, pin, sizeof(NmConst), hipMemcpyHostToDevice, stream()));
-    LSH_CHECK(hipMemcpyAsync(f->nm_state.p, st, 8 * sizeof(double), hipMemcpyHostToDevice, stream()));
+    sync();  // the pinned block is reused
+}
+
+// One search from the guess in state[0..8) (left there by the caller or by the
+// previous search), its result written back there.  plain: an ordinary launch
+// of the same grid (inside a captured graph), which every block of fits at
+// once on an otherwise idle device; the kernel's bounded waits end it with an
+// error rather than a hang if that ever fails.
+static void search_launch(lsh_frames *f, bool plain = false) {
     unsigned *ctr = f->nm_flags.p, *err = f->nm_flags.p + NM_FL_ERR;
     launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p);
     const NmConst *kp = (const NmConst *)f->nm_const.p;
     double *state = f->nm_state.p, *part = f->nm_part.p;
+    if (plain) {
+        launch("nm_search", k_nm_search, dim3(f->nm_grid), dim3(NM_THREADS), 0, kp, state, part, ctr, err);
+        return;
+    }
     void *args[] = {(void *)&kp, (void *)&state, (void *)&part, (void *)&ctr, (void *)&err};
     launch_coop("nm_search", 0.0, (const void *)k_nm_search, dim3(f->nm_grid), dim3(NM_THREADS), args, 0);
+}
+
+static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8, size_t coarsen) {
+    const size_t n = f->lw.size();
+    search_const(f, free8, coarsen);
+    uint8_t *pin = (uint8_t *)pinned_scratch((24 + NM_STATE) * sizeof(double));
+    double *st = (double *)pin;
+    std::memcpy(st, init.v, 8 * sizeof(double));
+    LSH_CHECK(hipMemcpyAsync(f->nm_state.p, st, 8 * sizeof(double), hipMemcpyHostToDevice, stream()));
+    search_launch(f);
+    unsigned *err = f->nm_flags.p + NM_FL_ERR;
     double *out = st + 16;
     LSH_CHECK(hipMemcpyAsync(out, f->nm_state.p, NM_STATE * sizeof(double), hipMemcpyDeviceToHost, stream()));
     unsigned *flags = (unsigned *)(out + NM_STATE);
@@ -1086,6 +1169,89 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     for (int q = 0; q < 4; q++) f->phase_ticks[q] = (uint64_t)out[9 + 2 * NM_MAXL + q];
     for (size_t l = 0; l < NM_MAXL; l++) f->lev_passes[l] = (uint64_t)out[13 + 2 * NM_MAXL + l];
     return s;
+}
+
+// the per-frame record of a stream: Sim, gains, evaluations at the frame
+// index, a sticky copy of the search's error flag, then the index advances
+__global__ void k_frame_record(const double *__restrict__ state, const FrameDev *__restrict__ fp,
+                               const unsigned *__restrict__ err, unsigned *__restrict__ cursor,
+                               double *__restrict__ sims, float *__restrict__ gains, double *__restrict__ evals) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const unsigned i = cursor[0];
+    for (int k = 0; k < 8; k++) sims[(size_t)i * 8 + k] = state[k];
+    for (int c = 0; c < 3; c++) gains[(size_t)i * 3 + c] = fp->gains[c];
+    evals[i] = state[8];
+    cursor[1] |= *err;
+    cursor[0] = i + 1;
+}
+
+__global__ void k_cursor_set(unsigned *cursor, unsigned v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        cursor[0] = v;
+        cursor[1] = 0;
+    }
+}
+
+__global__ void k_state_set(double *state, Sim s) {
+    if (threadIdx.x < 8 && blockIdx.x == 0) state[threadIdx.x] = s.v[threadIdx.x];
+}
+
+void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, size_t nframes) {
+    LSH_REQUIRE(f->have_ref && f->processed > 0, "stream: the first frame goes through lsh_frames_process");
+    search_const(f, free8, coarsen);
+    f->fdev.ensure(sizeof(FrameDev));
+    f->cursor.ensure(2);
+    f->rec_sims.ensure(nframes * 8);
+    f->rec_gains.ensure(nframes * 3);
+    f->rec_evals.ensure(nframes);
+    f->red.ensure((size_t)std::max(1u, std::min<unsigned>(cdiv(f->gw * f->gh, 256), 1024)) * 7);
+    launch("stream_cursor", k_cursor_set, dim3(1), dim3(64), 0, f->cursor.p, (unsigned)f->processed);
+    launch("stream_state", k_state_set, dim3(1), dim3(64), 0, f->nm_state.p, Sim());  // frame 0 sits at the identity
+}
+
+// stack.rs:513-530 and :638-706 for one frame after the first, on the device
+// alone: luma and pyramid, the search from the previous Sim, the map and the
+// gains derived on the device, the warp (planes and the next reference), the
+// focus slice into slice_dev.
+void frames_device_step(lsh_frames *f, const uint16_t *src, bool brightness, const lsh_depth_params *measure,
+                        float *slice_dev, bool plain_search) {
+    const size_t w = f->w, h = f->h, n = w * h;
+    FrameDev *fp = (FrameDev *)f->fdev.p;
+    launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, f->yt.p);
+    build_reg_pyramid(f, f->yt.p);
+    search_launch(f, plain_search);
+    launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p, (int)w, (int)h, fp);
+    const Inv9 unused{};
+    const Gains ug{};
+    if (brightness && f->have_grid) {
+        sums_launch(f, src, unused, 1, 0, nullptr, fp);
+        launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)f->red_out.p, 1, fp);
+    } else {
+        launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)nullptr, 0, fp);
+    }
+    warp_launch(f, src, unused, 1, ug, f->yr.p, fp);
+    build_reg_pyramid(f, f->yr.p);
+    if (measure) focus_slice(f, f->planes.p, *measure, slice_dev);
+    f->processed++;
+}
+
+void frames_record(lsh_frames *f) {
+    launch("frame_record", k_frame_record, dim3(1), dim3(64), 0, (const double *)f->nm_state.p,
+           (const FrameDev *)f->fdev.p, (const unsigned *)(f->nm_flags.p + NM_FL_ERR), f->cursor.p, f->rec_sims.p,
+           f->rec_gains.p, f->rec_evals.p);
+}
+
+void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals) {
+    std::vector<unsigned> cur(2);
+    copy_mem(cur.data(), LSH_HOST, f->cursor.p, LSH_DEVICE, 2 * sizeof(unsigned));
+    LSH_REQUIRE(cur[1] == 0, "search: a cost broadcast timed out");
+    if (sims) copy_mem(sims, LSH_HOST, f->rec_sims.p, LSH_DEVICE, n * 8 * sizeof(double));
+    if (gains) copy_mem(gains, LSH_HOST, f->rec_gains.p, LSH_DEVICE, n * 3 * sizeof(float));
+    if (evals) {
+        std::vector<double> e(n);
+        copy_mem(e.data(), LSH_HOST, f->rec_evals.p, LSH_DEVICE, n * sizeof(double));
+        for (size_t i = 0; i < n; i++) evals[i] = (uint64_t)e[i];
+    }
 }
 
 static const uint16_t *frame_source(lsh_frames *f, const uint16_t *rgb, lsh_mem mem) {

@@ -44,6 +44,7 @@ void *pinned_scratch(size_t);  // a per-device pinned host block (reused; at lea
 // ------------------------------------------------------------- profiling
 struct Prof {
     bool on = false;
+    bool capturing = false;  // a graph is being captured on the stream: no timing events
     struct Rec {
         const char *name;
         double bytes;
@@ -69,7 +70,7 @@ inline void launch_b(const char *name, double bytes, K kernel, dim3 grid, dim3 b
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
     Prof &p = prof();
     hipStream_t s = stream();
-    if (p.on) {
+    if (p.on && !p.capturing) {
         hipEvent_t a = p.get(), b = p.get();
         LSH_CHECK(hipEventRecord(a, s));
         hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);

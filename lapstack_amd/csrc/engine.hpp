@@ -35,6 +35,11 @@ struct lsh_frames {
     unsigned nm_grid = 0;
     uint64_t lev_evals[16] = {0}, lev_ticks[16] = {0};  // the last search, per registration level
     uint64_t lev_passes[16] = {0};                       // its collective passes (several evaluations each)
+    std::vector<uint8_t> nm_host;                        // the search constants last uploaded
+    lsh::DBuf<uint8_t> fdev;                             // the streaming path's per-frame map and gains
+    lsh::DBuf<unsigned> cursor;                          // [0] the frame index, [1] a sticky search error
+    lsh::DBuf<double> rec_sims, rec_evals;               // per-frame Sim and evaluations of a stream
+    lsh::DBuf<float> rec_gains;                          // per-frame gains of a stream
     uint64_t phase_ticks[4] = {0};
 };
 
@@ -56,3 +61,17 @@ struct lsh_fuser {
     lsh::DBuf<float> ent_tab;  // p*ln(p) per window count (residual rule)
     lsh::DBuf<unsigned> cnt;   // shares
 };
+
+namespace lsh {
+// The device-only per-frame path of a stack stream (align.hip): the search
+// constants for the stack, then per frame the registration, map, gains, warp
+// and measurement without a host round trip, and the per-frame record.
+void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, size_t nframes);
+void frames_device_step(lsh_frames *f, const uint16_t *src, bool brightness, const lsh_depth_params *measure,
+                        float *slice_dev, bool plain_search);
+void frames_record(lsh_frames *f);
+void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals);
+// fuse.hip: a push whose frame index comes from device memory (a replayed graph)
+void fuser_reserve(lsh_fuser *f, size_t nframes);
+void fuser_push_dev(lsh_fuser *f, const float *g0, const unsigned *idx_dev);
+}  // namespace lsh

@@ -103,7 +103,8 @@ template <int R>
 __global__ __launch_bounds__(256) void k_fold(const float *__restrict__ G, const float *__restrict__ Cn, int w, int h,
                                               int cw, int ch, float *__restrict__ best, float *__restrict__ acc,
                                               uint16_t *__restrict__ winner, int first, unsigned idx, Wt9 wt,
-                                              int chroma, float *__restrict__ en_out) {
+                                              int chroma, float *__restrict__ en_out,
+                                              const unsigned *__restrict__ idx_dev) {
     constexpr int TX = 64, TY = 16, EW = TX + 2 * R, EH = TY + 2 * R;
     constexpr int CWM = TX / 2 + R + 4, CHM = TY / 2 + R + 4;
     __shared__ float Cs[3][CHM][CWM];
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) void k_fold(const float *__restrict__ G, const
             acc[o] = Ls[0][ty][tx];
             acc[n + o] = Ls[1][ty][tx];
             acc[2 * n + o] = Ls[2][ty][tx];
-            if (winner) winner[o] = (uint16_t)idx;
+            if (winner) winner[o] = (uint16_t)(idx_dev ? *idx_dev : idx);
         }
     }
 }
@@ -370,16 +371,16 @@ __global__ void k_share_count(const uint16_t *__restrict__ win, const float *__r
 
 static void fold_dispatch(int r, dim3 g, const float *G, const float *Cn, int w, int h, int cw, int ch, float *best,
                           float *acc, uint16_t *winner, int first, unsigned idx, const Wt9 &wt, int chroma,
-                          float *en_out) {
+                          float *en_out, const unsigned *idx_dev) {
     dim3 b(256);
     // G_l and G_{l+1} read, best read and written, the accumulator written (fuse.rs:414-451)
     double by = 32.0 * w * h + 12.0 * cw * ch;
     switch (r) {
-    case 0: launch_b("fold", by, k_fold<0>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out); break;
-    case 1: launch_b("fold", by, k_fold<1>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out); break;
-    case 2: launch_b("fold", by, k_fold<2>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out); break;
-    case 3: launch_b("fold", by, k_fold<3>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out); break;
-    case 4: launch_b("fold", by, k_fold<4>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out); break;
+    case 0: launch_b("fold", by, k_fold<0>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out, idx_dev); break;
+    case 1: launch_b("fold", by, k_fold<1>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out, idx_dev); break;
+    case 2: launch_b("fold", by, k_fold<2>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out, idx_dev); break;
+    case 3: launch_b("fold", by, k_fold<3>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out, idx_dev); break;
+    case 4: launch_b("fold", by, k_fold<4>, g, b, 0, G, Cn, w, h, cw, ch, best, acc, winner, first, idx, wt, chroma, en_out, idx_dev); break;
     default: throw Error("energy_radius above 4 is not supported");
     }
 }
@@ -410,7 +411,14 @@ static size_t auto_levels(size_t w, size_t h, size_t min_top) {
 }
 
 // fuse.rs:401-484 Fuser::push with level 0 at g0 (3 planes on the device)
-static void fuser_push(lsh_fuser *f, const float *g0) {
+// the residual level of the frame into tops at the frame index in device memory
+__global__ void k_tops_put(const float *__restrict__ top, size_t m, const unsigned *__restrict__ idx_dev,
+                           float *__restrict__ tops) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) tops[(size_t)(*idx_dev) * m + i] = top[i];
+}
+
+static void fuser_push(lsh_fuser *f, const float *g0, const unsigned *idx_dev = nullptr) {
     LSH_REQUIRE(!f->finished, "fuser: push after finish");
     LSH_REQUIRE(f->count < 65535, "fuser: more than 65535 frames");
     const size_t L = f->levels;
@@ -434,7 +442,8 @@ static void fuser_push(lsh_fuser *f, const float *g0) {
         float *en_out = (f->halo_on && l == f->guide) ? f->en_guide.p : nullptr;
         fold_dispatch((int)f->p.energy_radius, g, lvl(l), lvl(l + 1), (int)f->lw[l], (int)f->lh[l], (int)f->lw[l + 1],
                       (int)f->lh[l + 1], f->best.p + f->off[l], f->acc.p + 3 * f->off[l],
-                      is_depth ? f->winner.p : nullptr, first ? 1 : 0, idx, wt, f->p.use_chroma ? 1 : 0, en_out);
+                      is_depth ? f->winner.p : nullptr, first ? 1 : 0, idx, wt, f->p.use_chroma ? 1 : 0, en_out,
+                      idx_dev);
     }
     if (f->halo_on) {
         size_t g = f->guide, n = f->lw[g] * f->lh[g];
@@ -459,6 +468,12 @@ static void fuser_push(lsh_fuser *f, const float *g0) {
         }
     } else {
         size_t tsz = f->lw[L] * f->lh[L];
+        if (idx_dev) {
+            LSH_REQUIRE(f->count + 1 <= f->tops_cap, "fuser: residual store not reserved for the stream");
+            launch("tops_put", k_tops_put, dim3(cdiv(3 * tsz, 256)), dim3(256), 0, lvl(L), 3 * tsz, idx_dev, f->tops.p);
+            f->count++;
+            return;
+        }
         if (f->count + 1 > f->tops_cap) {
             size_t cap = std::max<size_t>(16, 2 * f->tops_cap);
             DBuf<float> nt(cap * 3 * tsz);
@@ -477,6 +492,19 @@ static void fuser_push(lsh_fuser *f, const float *g0) {
 using namespace lsh;
 
 void lsh_fuser_push_device(lsh_fuser *f, const float *planes) { fuser_push(f, planes); }
+
+namespace lsh {
+void fuser_reserve(lsh_fuser *f, size_t nframes) {
+    if (f->halo_on || nframes <= f->tops_cap) return;
+    size_t tsz = f->lw[f->levels] * f->lh[f->levels];
+    DBuf<float> nt(nframes * 3 * tsz);
+    if (f->count) copy_mem(nt.p, LSH_DEVICE, f->tops.p, LSH_DEVICE, f->count * 3 * tsz * sizeof(float));
+    sync();
+    f->tops = std::move(nt);
+    f->tops_cap = nframes;
+}
+void fuser_push_dev(lsh_fuser *f, const float *g0, const unsigned *idx_dev) { fuser_push(f, g0, idx_dev); }
+}  // namespace lsh
 
 extern "C" {
 
