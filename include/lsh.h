@@ -195,6 +195,52 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                        float *slices, lsh_mem slices_mem, uint32_t graph, double *sims_out, float *gains_out,
                        uint64_t *evals_out);
 
+/* ---- many stacks in one launch -------------------------------------------- */
+
+/* One stack of a batch: n frames of w*h (interleaved RGB u16, device memory). */
+typedef struct lsh_batch_stack {
+    uint32_t w, h, n;
+    const uint16_t *const *frames; /* host array of n device pointers */
+} lsh_batch_stack;
+
+/* How a batch is scheduled.  tune: 0 the configuration given here (0 fields:
+ * defaults 16 rows, as many blocks per CU as fit, 2 frames per round); 1 the
+ * autotuner, its choice cached per shape class (depth, energy window, log2 of
+ * the pixel-frames, log2 of the stack count); 2 the autotuner again. */
+typedef struct lsh_batch_options {
+    uint32_t tune;
+    uint32_t tile_h;           /* fold tile rows: 8 or 16 (0: default) */
+    uint32_t blocks_per_cu;    /* resident blocks per CU (0: as many as fit) */
+    uint32_t frames_per_round; /* frames of each stack per round, 1..4 (0: default) */
+} lsh_batch_options;
+
+typedef struct lsh_batch_report {
+    double ms;                 /* device time of the batch launch */
+    double mpix_frames;        /* sum over stacks of w*h*n / 1e6 */
+    uint32_t rounds;           /* ceil(largest frame count / frames_per_round) */
+    uint32_t max_levels;
+    uint32_t tile_h;           /* the configuration run */
+    uint32_t blocks_per_cu;    /* launched per CU (the grid is blocks_per_cu * CUs) */
+    uint32_t resident_blocks;  /* of the grid, the blocks resident at the start, which do the work */
+    uint32_t frames_per_round;
+    uint32_t from_cache;       /* the configuration came from the shape-class cache */
+    uint32_t ntried;           /* autotuner candidates timed (round 0 each) */
+    uint32_t tried_cfg[8];     /* tile_h | blocks_per_cu << 8 | frames_per_round << 16 */
+    float tried_us_per_mpf[8]; /* their device time per megapixel-frame */
+} lsh_batch_report;
+
+/* No counterpart in the reference, whose engine runs one stack at a time
+ * (stack.rs:740-826 fuse_range): count stacks of uneven sizes and frame
+ * counts through the per-frame fuse path (the frame as decoded, without
+ * brightness matching; the REDUCE chain; the fold of every band-pass level; the
+ * residual kept) in one persistent launch.  fusers[i] (fresh, from
+ * lsh_fuser_new with the stack's size and levels; one energy window and chroma
+ * setting for all, halo control off) receive stacks[i]; each is then finished
+ * with lsh_fuser_finish as if its frames had been pushed one by one, with the
+ * same result.  opt: nullable (autotuner with its cache); rep: nullable. */
+char *lsh_batch_fuse(lsh_fuser **fusers, const lsh_batch_stack *stacks, uint32_t count, const lsh_batch_options *opt,
+                     lsh_batch_report *rep);
+
 /* ---- depth ------------------------------------------------------------ */
 
 /* depth.rs:1119-1200 depth_from_slices: n slices of dw*dh (contiguous, frame

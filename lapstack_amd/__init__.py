@@ -18,6 +18,8 @@ lapstack_amd/liblsh.so (declared in include/lsh.h):
 * ``fuse_and_depth``       -- stack.rs:740-853: the per-frame loop over a stack.
 * ``stack_stream``         -- the same loop with no host round trip per frame,
                               the per-frame step replayed as a hipGraph.
+* ``batch_fuse``           -- many uneven stacks through the fuse path in one
+                              persistent launch, tile shape autotuned.
 
 Buffers are numpy arrays (host memory) or torch tensors on the HIP device
 (device memory, passed by pointer, no copy).  There is no CPU fallback: every
@@ -36,7 +38,7 @@ from . import _lib
 from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
-           "depth_from_slices", "depth_grid", "stack_stream", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -542,6 +544,55 @@ def stack_stream(frames: Sequence, fuse: Optional[FuseParams] = None, align: Opt
     fu.close()
     del bufs, sk
     return dict(image=image, winner=winner, sims=sims, gains=gains, evals=evals, slices=slices_out)
+
+
+def batch_fuse(stacks: Sequence[Sequence], levels: Optional[Sequence[int]] = None, fuse: Optional[FuseParams] = None,
+               tune: int = 1, tile_h: int = 0, blocks_per_cu: int = 0, frames_per_round: int = 0):
+    """Many stacks of uneven sizes and frame counts through the fuse path in
+    one persistent launch (include/lsh.h lsh_batch_fuse).  ``stacks``: lists
+    of (h, w, 3) uint16 device tensors; ``levels``: pyramid levels per stack
+    (None: ``fuse.levels`` for all).  ``tune``: 0 the configuration given by
+    ``tile_h``, ``blocks_per_cu`` and ``frames_per_round`` (0: defaults), 1 the
+    autotuner (cached per shape class), 2 the autotuner again.  Returns
+    ([(image, winner)] per stack as numpy arrays, report dict)."""
+    import torch
+
+    lib = _lib.load()
+    fuse = fuse or FuseParams()
+    k = len(stacks)
+    if k == 0:
+        raise HipError("batch: no stacks")
+    fusers, descs, keep = [], (_lib.BatchStackC * k)(), []
+    try:
+        for i, frames in enumerate(stacks):
+            if len(frames) == 0:
+                raise HipError("batch: a stack without frames")
+            h, w, _ = tuple(frames[0].shape)
+            for f in frames:
+                if not (isinstance(f, torch.Tensor) and f.is_cuda and f.dtype == torch.uint16 and f.is_contiguous()):
+                    raise HipError("batch: frames must be contiguous uint16 device tensors")
+                if tuple(f.shape) != (h, w, 3):
+                    raise HipError("batch: a stack's frames must share one size")
+            fp = dataclasses.replace(fuse, levels=levels[i]) if levels is not None else fuse
+            fusers.append(Fuser(w, h, fp))
+            ptrs = (C.c_void_p * len(frames))(*[f.data_ptr() for f in frames])
+            keep.append((frames, ptrs))
+            descs[i].w, descs[i].h, descs[i].n = w, h, len(frames)
+            descs[i].frames = C.cast(ptrs, C.c_void_p)
+        handles = (C.c_void_p * k)(*[fu._h.value for fu in fusers])
+        opt = _lib.BatchOptionsC(int(tune), int(tile_h), int(blocks_per_cu), int(frames_per_round))
+        rep = _lib.BatchReportC()
+        _lib.check(lib.lsh_batch_fuse(handles, descs, k, C.byref(opt), C.byref(rep)))
+        out = [fu.finish() for fu in fusers]
+    finally:
+        for fu in fusers:
+            fu.close()
+        del keep
+    r = {name: getattr(rep, name) for name, _ in rep._fields_ if name not in ("tried_cfg", "tried_us_per_mpf")}
+    r["tried"] = [{"tile_h": c & 0xFF, "blocks_per_cu": (c >> 8) & 0xFF, "frames_per_round": c >> 16,
+                   "us_per_mpix_frame": round(float(u), 4)}
+                  for c, u in zip(list(rep.tried_cfg)[:rep.ntried], list(rep.tried_us_per_mpf)[:rep.ntried])]
+    return out, r
 
 
 def fuse_and_depth(frames: Sequence, fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,

@@ -40,6 +40,21 @@ class WlsReportC(C.Structure):
                 ("err_fp64", C.c_double), ("ms", C.c_double)]
 
 
+class BatchStackC(C.Structure):
+    _fields_ = [("w", _u32), ("h", _u32), ("n", _u32), ("frames", _p)]
+
+
+class BatchOptionsC(C.Structure):
+    _fields_ = [("tune", _u32), ("tile_h", _u32), ("blocks_per_cu", _u32), ("frames_per_round", _u32)]
+
+
+class BatchReportC(C.Structure):
+    _fields_ = [("ms", C.c_double), ("mpix_frames", C.c_double), ("rounds", _u32), ("max_levels", _u32),
+                ("tile_h", _u32), ("blocks_per_cu", _u32), ("resident_blocks", _u32), ("frames_per_round", _u32),
+                ("from_cache", _u32),
+                ("ntried", _u32), ("tried_cfg", _u32 * 8), ("tried_us_per_mpf", C.c_float * 8)]
+
+
 # (name, restype, argtypes) of every entry point include/lsh.h declares
 SIGNATURES = [
     ("lsh_abi", _u32, []),
@@ -82,6 +97,7 @@ SIGNATURES = [
                                  C.POINTER(C.c_double)]),
     ("lsh_stack_stream", _p, [_p, _p, _p, _u32, C.c_int, _p, _u32, _u32, C.POINTER(DepthParamsC), _p, C.c_int, _u32,
                               _p, _p, _p]),
+    ("lsh_batch_fuse", _p, [_p, _p, _u32, C.POINTER(BatchOptionsC), C.POINTER(BatchReportC)]),
     ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
                            _p, _p, C.c_int, C.POINTER(WlsReportC)]),
 ]

@@ -15,181 +15,28 @@
 #include "api.hpp"
 #include "common.hpp"
 #include "engine.hpp"
+#include "fuse_tiles.hpp"
 
 namespace lsh {
 
-// pyramid.rs:46, :124-125
-__constant__ float KR[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
-#define EV0 (2.0f * (1.0f / 16.0f))
-#define EV1 (2.0f * (6.0f / 16.0f))
-#define EV2 (2.0f * (1.0f / 16.0f))
-#define ODDK (2.0f * (4.0f / 16.0f))
-
-// pyramid.rs:86-119: REDUCE of one plane (blockIdx.z selects the plane) on a
-// 64x4 output tile: the 131x11 input window (reflect-101 at the borders) is
-// staged in LDS, the horizontal 5-tap sums of its 11 rows at the 64 even
-// columns go to LDS, and each thread sums its five rows.
+// REDUCE of one plane (blockIdx.z selects the plane) on 64x4 output tiles
 __global__ __launch_bounds__(256) void k_reduce3(const float *__restrict__ src, int w, int h,
                                                  float *__restrict__ dst, int ow, int oh) {
-    constexpr int TX = 64, TY = 4, IW = 2 * TX + 3, IH = 2 * TY + 3;
-    __shared__ float T[IH][IW + 1];
-    __shared__ float Hs[IH][TX + 1];
-    const int ox0 = blockIdx.x * TX, oy0 = blockIdx.y * TY, tid = threadIdx.y * TX + threadIdx.x;
-    const float *s = src + (size_t)blockIdx.z * w * h;
-    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
-    const int nx = min(IW, 2 * (ow - ox0) + 3), ny = min(IH, 2 * (oh - oy0) + 3);
-    for (int k = tid; k < IH * IW; k += TX * TY) {
-        int r = k / IW, c = k - r * IW;
-        if (r < ny && c < nx) T[r][c] = s[(size_t)rf101(iy0 + r, h) * w + rf101(ix0 + c, w)];
-    }
-    __syncthreads();
-    for (int k = tid; k < IH * TX; k += TX * TY) {
-        int r = k / TX, jj = k - r * TX;
-        if (r < ny && ox0 + jj < ow) {
-            float a = 0.0f;
-#pragma unroll
-            for (int u = 0; u < 5; u++) a += KR[u] * T[r][2 * jj + u];
-            Hs[r][jj] = a;
-        }
-    }
-    __syncthreads();
-    int j = ox0 + threadIdx.x, i = oy0 + threadIdx.y;
-    if (j >= ow || i >= oh) return;
-    int r0 = 2 * threadIdx.y;
-    dst[(size_t)blockIdx.z * ow * oh + (size_t)i * ow + j] =
-        KR[0] * Hs[r0][threadIdx.x] + KR[1] * Hs[r0 + 1][threadIdx.x] + KR[2] * Hs[r0 + 2][threadIdx.x] +
-        KR[3] * Hs[r0 + 3][threadIdx.x] + KR[4] * Hs[r0 + 4][threadIdx.x];
+    __shared__ ReduceLds S;
+    reduce_tile(PlaneSrc{src, (size_t)w * h}, (int)blockIdx.z, w, h, dst + (size_t)blockIdx.z * ow * oh, ow, oh,
+                blockIdx.x * RED_TX, blockIdx.y * RED_TY, S);
 }
 
-// pyramid.rs:128-140: the horizontal EXPAND of coarse row cr at fine column x
-__device__ __forceinline__ float expand_h(const float *__restrict__ C, int cw, int cr, int x) {
-    const float *r = C + (size_t)cr * cw;
-    int c = x >> 1;
-    if ((x & 1) == 0) return EV0 * r[rf101(c - 1, cw)] + EV1 * r[rf101(c, cw)] + EV2 * r[rf101(c + 1, cw)];
-    return ODDK * (r[rf101(c, cw)] + r[rf101(c + 1, cw)]);
-}
-
-// pyramid.rs:143-168: EXPAND of the coarse plane at fine pixel (y, x)
-__device__ __forceinline__ float expand_at(const float *__restrict__ C, int cw, int ch, int y, int x) {
-    int c = y >> 1;
-    if ((y & 1) == 0)
-        return EV0 * expand_h(C, cw, rf101(c - 1, ch), x) + EV1 * expand_h(C, cw, rf101(c, ch), x) +
-               EV2 * expand_h(C, cw, rf101(c + 1, ch), x);
-    return ODDK * (expand_h(C, cw, rf101(c, ch), x) + expand_h(C, cw, rf101(c + 1, ch), x));
-}
-
-// fuse.rs:152-165
-__device__ __forceinline__ float node_energy(float r, float g, float b, int chroma) {
-    if (chroma) return r * r + g * g + b * b;
-    float y = 0.299f * r + 0.587f * g + 0.114f * b;
-    return y * y;
-}
-
-struct Wt9 {
-    float k[9];
-};
-
-__device__ __forceinline__ int floordiv2(int a) { return a >= 0 ? a / 2 : -((1 - a) / 2); }
-
-// One band-pass level of the fold (fuse.rs:401-451) on a 64x16 tile of the
-// level: L = G - EXPAND(G_next), the node energy, the separable binomial
-// window and the strict-'>' select, with nothing but the accumulator, the best
-// energies and the winner map written.  The coarse window under the tile and
-// its horizontal EXPAND at every tile column are staged in LDS first, so each
-// fine sample costs three global loads.  LDS rows and columns are indexed by
-// unreflected coordinates and hold the reflect-101 sample of each; every index
-// a fine sample the window can reach needs lies inside the staged window.
+// one band-pass level of the fold on 64x16 tiles
 template <int R>
 __global__ __launch_bounds__(256) void k_fold(const float *__restrict__ G, const float *__restrict__ Cn, int w, int h,
                                               int cw, int ch, float *__restrict__ best, float *__restrict__ acc,
                                               uint16_t *__restrict__ winner, int first, unsigned idx, Wt9 wt,
                                               int chroma, float *__restrict__ en_out,
                                               const unsigned *__restrict__ idx_dev) {
-    constexpr int TX = 64, TY = 16, EW = TX + 2 * R, EH = TY + 2 * R;
-    constexpr int CWM = TX / 2 + R + 4, CHM = TY / 2 + R + 4;
-    __shared__ float Cs[3][CHM][CWM];
-    __shared__ float Hs[3][CHM][EW];
-    __shared__ float Es[EH][EW];
-    __shared__ float Ts[EH][TX];
-    __shared__ float Ls[3][TY][TX];
-    const int bx = blockIdx.x * TX, by = blockIdx.y * TY, tid = threadIdx.x;
-    const size_t n = (size_t)w * h, cn = (size_t)cw * ch;
-    const int cx0 = floordiv2(bx - R) - 1, cy0 = floordiv2(by - R) - 1;
-    const int CW = floordiv2(bx + TX + R - 1) + 1 - cx0 + 1, CH = floordiv2(by + TY + R - 1) + 1 - cy0 + 1;
-    // the last fine column / row any output of this tile reads (window reach)
-    const int xlim = min(bx + TX, w) - 1 + R, ylim = min(by + TY, h) - 1 + R;
-    for (int k = tid; k < 3 * CH * CW; k += 256) {
-        int c = k / (CH * CW), r = k - c * CH * CW, j = r / CW, i = r - j * CW;
-        Cs[c][j][i] = Cn[c * cn + (size_t)rf101(cy0 + j, ch) * cw + rf101(cx0 + i, cw)];
-    }
-    __syncthreads();
-    // pyramid.rs:128-140: horizontal EXPAND of each staged coarse row at each tile column
-    for (int k = tid; k < 3 * CH * EW; k += 256) {
-        int c = k / (CH * EW), r = k - c * CH * EW, j = r / EW, lx = r - j * EW;
-        if (bx + lx - R > xlim) continue;
-        int gx = rf101(bx + lx - R, w), i0 = (gx >> 1) - cx0;
-        const float *row = Cs[c][j];
-        Hs[c][j][lx] = (gx & 1) == 0 ? EV0 * row[i0 - 1] + EV1 * row[i0] + EV2 * row[i0 + 1]
-                                     : ODDK * (row[i0] + row[i0 + 1]);
-    }
-    __syncthreads();
-    // pyramid.rs:151-166 vertical EXPAND, L = G - EXPAND (pyramid.rs:208-211), node energy
-    for (int k = tid; k < EH * EW; k += 256) {
-        int ly = k / EW, lx = k - ly * EW;
-        if (by + ly - R > ylim || bx + lx - R > xlim) continue;
-        int gy = rf101(by + ly - R, h), gx = rf101(bx + lx - R, w), j0 = (gy >> 1) - cy0;
-        size_t o = (size_t)gy * w + gx;
-        float l[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            float e = (gy & 1) == 0 ? EV0 * Hs[c][j0 - 1][lx] + EV1 * Hs[c][j0][lx] + EV2 * Hs[c][j0 + 1][lx]
-                                    : ODDK * (Hs[c][j0][lx] + Hs[c][j0 + 1][lx]);
-            l[c] = G[c * n + o] - e;
-        }
-        Es[ly][lx] = node_energy(l[0], l[1], l[2], chroma);
-        int cy = ly - R, cx = lx - R;
-        if (cy >= 0 && cy < TY && cx >= 0 && cx < TX) {
-            Ls[0][cy][cx] = l[0];
-            Ls[1][cy][cx] = l[1];
-            Ls[2][cy][cx] = l[2];
-        }
-    }
-    __syncthreads();
-    // fuse.rs:174-183: horizontal window on every tile row
-    for (int k = tid; k < EH * TX; k += 256) {
-        int ly = k / TX, lx = k - ly * TX;
-        if (by + ly - R > ylim || bx + lx >= w) continue;
-        float a = 0.0f;
-#pragma unroll
-        for (int t = 0; t <= 2 * R; t++) a += wt.k[t] * Es[ly][lx + t];
-        Ts[ly][lx] = a;
-    }
-    __syncthreads();
-    const int tx = tid & 63, x = bx + tx;
-    if (x >= w) return;
-#pragma unroll
-    for (int q = 0; q < TY / 4; q++) {
-        int ty = (tid >> 6) + 4 * q, y = by + ty;
-        if (y >= h) break;
-        // fuse.rs:186-195: vertical window
-        float en = 0.0f;
-#pragma unroll
-        for (int t = 0; t <= 2 * R; t++) en += wt.k[t] * Ts[ty + t][tx];
-        size_t o = (size_t)y * w + x;
-        if (en_out) en_out[o] = en;
-        if (first) {
-            best[o] = en;
-            acc[o] = Ls[0][ty][tx];
-            acc[n + o] = Ls[1][ty][tx];
-            acc[2 * n + o] = Ls[2][ty][tx];
-        } else if (en > best[o]) {  // strict: ties keep the earlier frame (fuse.rs:429, 442)
-            best[o] = en;
-            acc[o] = Ls[0][ty][tx];
-            acc[n + o] = Ls[1][ty][tx];
-            acc[2 * n + o] = Ls[2][ty][tx];
-            if (winner) winner[o] = (uint16_t)(idx_dev ? *idx_dev : idx);
-        }
-    }
+    __shared__ FoldLds<R, 16> S;
+    fold_tile<R, 16>(PlaneSrc{G, (size_t)w * h}, Cn, w, h, cw, ch, best, acc, winner, first, idx_dev ? *idx_dev : idx,
+                     wt, chroma, en_out, blockIdx.x * 64, blockIdx.y * 16, S);
 }
 
 // L = G - EXPAND(next) (pyramid.rs:208-211), or the collapse step out = L + EXPAND(next)
@@ -430,7 +277,7 @@ static void fuser_push(lsh_fuser *f, const float *g0, const unsigned *idx_dev = 
     for (size_t l = 0; l < L; l++) {
         dim3 g(cdiv(f->lw[l + 1], 64), cdiv(f->lh[l + 1], 4), 3);
         double bytes = 12.0 * (f->lw[l] * f->lh[l] + f->lw[l + 1] * f->lh[l + 1]);
-        launch_b("reduce", bytes, k_reduce3, g, dim3(64, 4), 0, lvl(l), (int)f->lw[l], (int)f->lh[l],
+        launch_b("reduce", bytes, k_reduce3, g, dim3(256), 0, lvl(l), (int)f->lw[l], (int)f->lh[l],
                  f->G.p + 3 * f->off[l + 1], (int)f->lw[l + 1], (int)f->lh[l + 1]);
     }
     if (first) f->winner.zero();
@@ -452,7 +299,7 @@ static void fuser_push(lsh_fuser *f, const float *g0, const unsigned *idx_dev = 
         float *wa = f->wgt_a.p, *wb = f->wgt_b.p;
         for (size_t li = g + 1; li <= L; li++) {
             dim3 gr(cdiv(f->lw[li], 64), cdiv(f->lh[li], 4), 1);
-            launch("reduce", k_reduce3, gr, dim3(64, 4), 0, (const float *)wa, (int)f->lw[li - 1], (int)f->lh[li - 1], wb,
+            launch("reduce", k_reduce3, gr, dim3(256), 0, (const float *)wa, (int)f->lw[li - 1], (int)f->lh[li - 1], wb,
                    (int)f->lw[li], (int)f->lh[li]);
             std::swap(wa, wb);
             size_t nl = f->lw[li] * f->lh[li];
