@@ -5,10 +5,13 @@ uniform in [2, 12] MP with aspect 3:2 or 4:3, interleaved RGB u16 resident in
 HBM; pyramid depth per stack floor(log2(min side)) - 4.  The fuse path only
 (the frame as decoded, REDUCE chain, region-energy fold, residual store).
 
-Each stack is a two-level focus bracket: the seeded scene of synth.scene and
-its Gaussian blur at sigma = 3 px, mixed per pixel by min(1, 2 |depth - f_i|)
-for frame i's focus f_i = i / (n - 1), plus the same noise and quantisation
-as synth.Bracket (one FFT blur per stack instead of fourteen).
+Each stack is a two-level focus bracket cut from one seeded master: the
+scene of synth.scene at 4352 x 3072 (it holds the largest stack at either
+aspect) and its Gaussian blur at sigma = 3 px, made once (one FFT size, so the
+FFT library builds its kernels once); a stack takes a w x h window at a seeded
+offset and its own depth map (synth.depth_map), and frame i mixes the two by
+min(1, 2 |depth - f_i|) for the focus f_i = i / (n - 1), plus the noise and
+quantisation of synth.Bracket.
 
 Timed, after a warm-up of both paths:
   loop   -- per stack, per frame: lsh_frames_ingest + lsh_fuser_push_frames
@@ -49,15 +52,28 @@ def plan(n_stacks, seed):
     return out
 
 
-def make_stack(w, h, n, seed):
+MASTER_W, MASTER_H = 4352, 3072
+
+
+def make_master(seed):
+    """(scene, blurred scene), each (3, MASTER_H, MASTER_W) float32 on cuda."""
+    from lapstack_amd import synth
+
+    sc = synth.scene(MASTER_H, MASTER_W, seed, "cuda")
+    return sc, synth._Blurrer(sc).blur(3.0)
+
+
+def make_stack(master, w, h, n, seed):
     """n frames (h, w, 3) torch.uint16 on cuda: the two-level bracket above."""
     import torch
 
     from lapstack_amd import synth
 
-    sc = synth.scene(h, w, seed, "cuda")
+    rng = np.random.default_rng(seed)
+    ox, oy = int(rng.integers(0, MASTER_W - w + 1)), int(rng.integers(0, MASTER_H - h + 1))
+    sc = master[0][:, oy:oy + h, ox:ox + w]
+    bl = master[1][:, oy:oy + h, ox:ox + w]
     depth = synth.depth_map(h, w, seed + 1, "cuda")
-    bl = synth._Blurrer(sc).blur(3.0)
     frames = []
     for i in range(n):
         fi = i / (n - 1) if n > 1 else 0.0
@@ -82,20 +98,34 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--stacks", type=int, default=64)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--only-batch", metavar="TILE_H,BLOCKS_PER_CU,FRAMES_PER_ROUND", default=None,
+                    help="one batched launch with this configuration, no loop and no tuning (for profiling)")
     args = ap.parse_args()
     shapes = plan(args.stacks, args.seed)
+    assert all(w <= MASTER_W and h <= MASTER_H for w, h, _, _ in shapes)
     t0 = time.perf_counter()
+    master = make_master(args.seed)
     stacks = []
     for k, (w, h, n, _) in enumerate(shapes):
-        stacks.append(make_stack(w, h, n, args.seed * 1000 + k))
+        stacks.append(make_stack(master, w, h, n, args.seed * 1000 + k))
         if (k + 1) % 16 == 0:
             print(f"generated {k + 1}/{len(shapes)} stacks ({time.perf_counter() - t0:.1f}s)", flush=True)
+    del master
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     mpf = sum(w * h * n for w, h, n, _ in shapes) / 1e6
     levels = [lv for _, _, _, lv in shapes]
     fp = la.FuseParams()
     lib = _lib.load()
+
+    if args.only_batch:
+        ty, pc, nf = (int(v) for v in args.only_batch.split(","))
+        out, rep = la.batch_fuse(stacks, levels, fp, tune=0, tile_h=ty, blocks_per_cu=pc, frames_per_round=nf)
+        print(json.dumps({"mpix_frames": round(mpf, 1), "batch_device_ms": round(rep["ms"], 2),
+                          "batch_device_mpix_frames_per_s": round(mpf / (rep["ms"] / 1e3), 1),
+                          "config": {k: rep[k] for k in ("tile_h", "blocks_per_cu", "resident_blocks",
+                                                         "frames_per_round", "rounds")}}), flush=True)
+        return
 
     # warm-up: both paths on two small stacks
     small = sorted(range(len(shapes)), key=lambda k: shapes[k][0] * shapes[k][1])[:2]
@@ -167,7 +197,8 @@ def main():
         "speedup_vs_loop": round(t_loop / t_batch, 3),
         "batch_s": round(t_batch, 3), "batch_device_ms": round(rep.ms, 2), "loop_s": round(t_loop, 3),
         "batch_with_tuning_s": round(t_tune, 3),
-        "config": {"tile_h": rep.tile_h, "blocks_per_cu": rep.blocks_per_cu, "frames_per_round": rep.frames_per_round,
+        "config": {"tile_h": rep.tile_h, "blocks_per_cu": rep.blocks_per_cu, "resident_blocks": rep.resident_blocks,
+                   "frames_per_round": rep.frames_per_round,
                    "rounds": rep.rounds, "from_cache": rep.from_cache},
         "autotuner_tried": [{"tile_h": a, "blocks_per_cu": b, "frames_per_round": c, "us_per_mpix_frame_round0": u}
                             for a, b, c, u in tried],
