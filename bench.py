@@ -292,10 +292,12 @@ def main():
         paths[path] = round(1000.0 * (time.perf_counter() - tp), 2)
         log(f"{path} path: {paths[path]} ms per step")
 
-    # per-kernel timing pass (HIP events on the library's stream), off the timed region
+    # per-kernel timing pass (HIP events on the library's stream), off the timed region; the host-steered
+    # path runs the same kernels and reads back each search's passes per level, which gives the search's
+    # algorithmic bytes (a pass reads the level's reference and target once)
     _lib.prof_reset()
     _lib.prof_enable(True)
-    job.run(fp, apm, dp, nframes=args.prof_frames or None, path="launches")
+    job.run(fp, apm, dp, nframes=args.prof_frames or None, path="host")
     sync()
     _lib.prof_enable(False)
     prof = _lib.prof_read()
