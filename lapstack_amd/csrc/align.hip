@@ -982,16 +982,18 @@ __global__ __launch_bounds__(256) void k_bright_sums(const uint16_t *__restrict_
 static int taps_n(uint32_t ik) { return ik <= 1 ? 2 : (ik <= 3 ? 4 : 6); }
 
 static void warp_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int affine, const Gains &g,
-                        float *yout, const FrameDev *fp = nullptr) {
+                        float *yout, const FrameDev *fp = nullptr, const float *yin_ = nullptr,
+                        float *planes_out = nullptr) {
     dim3 gr(cdiv(f->w, 64), cdiv(f->h, 4)), b(64, 4);
-    const float *yin = f->yt.p;
+    const float *yin = yin_ ? yin_ : f->yt.p;
+    float *planes = planes_out ? planes_out : f->planes.p;
     int ik = (int)f->interp, w = (int)f->w, h = (int)f->h;
     // u16 frame read, 3 f32 planes written; with the luma: luma read, warped luma written
     double by = (double)f->w * f->h * (yout ? 26.0 : 18.0);
     switch (taps_n(f->interp)) {
-    case 2: launch_b("warp", by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout, fp); break;
-    case 4: launch_b("warp", by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout, fp); break;
-    default: launch_b("warp", by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, f->planes.p, yout, fp); break;
+    case 2: launch_b("warp", by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
+    case 4: launch_b("warp", by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
+    default: launch_b("warp", by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
     }
 }
 
@@ -1199,6 +1201,7 @@ __global__ void k_state_set(double *state, Sim s) {
 void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, size_t nframes) {
     LSH_REQUIRE(f->have_ref && f->processed > 0, "stream: the first frame goes through lsh_frames_process");
     search_const(f, free8, coarsen);
+    f->coarsen = coarsen;
     f->fdev.ensure(sizeof(FrameDev));
     f->cursor.ensure(2);
     f->rec_sims.ensure(nframes * 8);
@@ -1232,6 +1235,42 @@ void frames_device_step(lsh_frames *f, const uint16_t *src, bool brightness, con
     warp_launch(f, src, unused, 1, ug, f->yr.p, fp);
     build_reg_pyramid(f, f->yr.p);
     if (measure) focus_slice(f, f->planes.p, *measure, slice_dev);
+    f->processed++;
+}
+
+void frames_stream_slots(lsh_frames *f) {
+    const size_t tot = f->yt.n;
+    for (auto &b : f->yts) b.ensure(tot);
+    f->planes_b.ensure(3 * f->w * f->h);
+}
+
+void frames_target(lsh_frames *f, const uint16_t *src, int slot) {
+    const size_t n = f->w * f->h;
+    float *dst = f->yts[slot].p;
+    launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, dst);
+    build_reg_pyramid(f, dst);
+}
+
+void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search) {
+    const size_t w = f->w, h = f->h;
+    const float *tgt = f->yts[slot].p;
+    // the levels the search reads (the finest searched and coarser) into its target buffer
+    const size_t fin = f->loff[std::min<size_t>(f->coarsen, f->lw.size() - 1)];
+    copy_mem(f->yt.p + fin, LSH_DEVICE, tgt + fin, LSH_DEVICE, (f->yt.n - fin) * sizeof(float));
+    FrameDev *fp = (FrameDev *)f->fdev.p;
+    search_launch(f, plain_search);
+    launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p, (int)w, (int)h, fp);
+    const Inv9 unused{};
+    const Gains ug{};
+    if (brightness && f->have_grid) {
+        sums_launch(f, src, unused, 1, 0, nullptr, fp);
+        launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)f->red_out.p, 1, fp);
+    } else {
+        launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)nullptr, 0, fp);
+    }
+    // the warp reads the target's full-resolution luma from its slot
+    warp_launch(f, src, unused, 1, ug, f->yr.p, fp, tgt, planes_out);
+    build_reg_pyramid(f, f->yr.p);
     f->processed++;
 }
 

@@ -37,8 +37,17 @@ struct Error : std::runtime_error {
     } while (0)
 
 // ---------------------------------------------------------------- runtime
-hipStream_t stream();          // the library's stream on the current device
-void sync();                   // wait for it
+hipStream_t stream();          // the library's stream on the current device (or the OnStream one)
+hipStream_t side_stream();     // a second stream on the current device (default priority), made on first use
+void sync();                   // wait for both
+// While one lives, this thread's launches and copies go to stream s.
+struct OnStream {
+    hipStream_t prev;
+    explicit OnStream(hipStream_t s);
+    ~OnStream();
+    OnStream(const OnStream &) = delete;
+    OnStream &operator=(const OnStream &) = delete;
+};
 void *pinned_scratch(size_t);  // a per-device pinned host block (reused; at least the size asked)
 
 // ------------------------------------------------------------- profiling

@@ -13,6 +13,8 @@ struct lsh_frames {
     lsh::DBuf<float> planes;      // 3*w*h: the frame after warp and gains
     lsh::DBuf<float> yt;          // target luma pyramid: level 0 (full res) then the registration levels
     lsh::DBuf<float> yr;          // reference (previous aligned frame's luma) pyramid, same layout
+    lsh::DBuf<float> yts[2];      // a pipelined stream's target pyramids (frame k in slot k % 2)
+    lsh::DBuf<float> planes_b;    // a pipelined stream's second planes buffer
     std::vector<size_t> lw, lh, loff;  // registration level dims and offsets (level 0 = full res)
     bool have_ref = false;        // yr holds a reference
     bool have_grid = false;       // frame 0's brightness grid is set
@@ -41,6 +43,7 @@ struct lsh_frames {
     lsh::DBuf<double> rec_sims, rec_evals;               // per-frame Sim and evaluations of a stream
     lsh::DBuf<float> rec_gains;                          // per-frame gains of a stream
     uint64_t phase_ticks[4] = {0};
+    size_t coarsen = 0;                                  // a stream's finest searched level
 };
 
 struct lsh_fuser {
@@ -70,6 +73,16 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
 void frames_device_step(lsh_frames *f, const uint16_t *src, bool brightness, const lsh_depth_params *measure,
                         float *slice_dev, bool plain_search);
 void frames_record(lsh_frames *f);
+// The parts of a pipelined stream: a frame's target luma and pyramid into a
+// slot (yts[slot]); and its serial chain -- the searched levels of that slot
+// into the search's buffer, the search from the previous Sim, the map and
+// gains, the warp into planes_out (and the warped luma, the next reference),
+// the reference pyramid.  The focus slice and the fold of planes_out are left
+// to the caller, who may run them beside the next frame's chain.
+void frames_stream_slots(lsh_frames *f);
+void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out);  // depth.hip
+void frames_target(lsh_frames *f, const uint16_t *src, int slot);
+void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search);
 void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals);
 // fuse.hip: a push whose frame index comes from device memory (a replayed graph)
 void fuser_reserve(lsh_fuser *f, size_t nframes);

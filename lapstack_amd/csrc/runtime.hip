@@ -12,6 +12,7 @@ namespace {
 std::mutex g_mu;
 struct DevState {
     hipStream_t s = nullptr;
+    hipStream_t side = nullptr;  // a second stream, for work that runs beside the main chain
     void *pinned = nullptr;
     size_t pinned_n = 0;
     Prof prof;
@@ -28,9 +29,24 @@ DevState &dev_state() {
 }
 }  // namespace
 
-hipStream_t stream() { return dev_state().s; }
+thread_local hipStream_t t_override = nullptr;  // OnStream: this thread's launches go there
 
-void sync() { LSH_CHECK(hipStreamSynchronize(stream())); }
+hipStream_t stream() { return t_override ? t_override : dev_state().s; }
+
+hipStream_t side_stream() {
+    DevState &st = dev_state();
+    if (!st.side) LSH_CHECK(hipStreamCreateWithFlags(&st.side, hipStreamNonBlocking));
+    return st.side;
+}
+
+OnStream::OnStream(hipStream_t s) : prev(t_override) { t_override = s; }
+OnStream::~OnStream() { t_override = prev; }
+
+void sync() {
+    DevState &st = dev_state();
+    LSH_CHECK(hipStreamSynchronize(st.s));
+    if (st.side) LSH_CHECK(hipStreamSynchronize(st.side));
+}
 
 void *pinned_scratch(size_t n) {
     DevState &st = dev_state();

@@ -1,12 +1,27 @@
 // stream.hip -- a whole stack through the frames handle and the fuser with no
 // host round trip per frame (stack.rs:740-826 fuse_range, the per-frame
 // loop).  Frame 0 takes the ordinary path (it is the reference and the
-// brightness anchor); every later frame runs the device-only step
-// (registration, map, gains, warp, focus slice), the fold into the fuser and a
-// per-frame record.  With graph on, that step is captured once as a hipGraph
-// and replayed per frame: the frame is staged into a fixed buffer before the
-// replay, its focus slice leaves a fixed buffer after it, and the frame index
-// the fold and the record need lives in device memory.
+// brightness anchor); every later frame k runs on the device alone:
+//
+//   the chain (main stream): the searched levels of frame k's target pyramid
+//     into the search's buffer, the search from the previous Sim, the map and
+//     the gains derived on the device, the warp into planes[k % 2] (and the
+//     warped luma, frame k+1's reference), the reference pyramid, the record;
+//   the side work: the focus slice and the fold of planes[k % 2], and the
+//     target luma and pyramid of frame k+2 into slot k % 2.
+//
+// Only the chain is serial across frames (frame k+1 registers against frame
+// k's warped luma), so frame k's side work runs beside frame k+1's chain: the
+// search is bound by the latency of its grid-wide reductions and leaves most
+// of the chip to the side work.  Every value is the one the per-frame path
+// computes; only the overlap differs.
+//
+// graph == 0: two streams and events.  graph != 0: frame k >= 2 (with a frame
+// after it) is one replay of a captured hipGraph with two branches -- frame
+// k's chain beside frame k-1's side work -- one graph per parity of k (the
+// buffers alternate); the raw frames are staged into two fixed buffers, the
+// slices leave through two fixed buffers, and the fold takes its frame index
+// from a device counter only the side work advances.
 #include <vector>
 
 #include "api.hpp"
@@ -14,6 +29,41 @@
 #include "engine.hpp"
 
 using namespace lsh;
+
+namespace {
+
+__global__ void k_counter_set(unsigned *c, unsigned v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = v;
+}
+__global__ void k_counter_next(unsigned *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + 1;
+}
+
+struct Events {
+    std::vector<hipEvent_t> ev;
+    hipEvent_t make() {
+        hipEvent_t e = nullptr;
+        LSH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ev.push_back(e);
+        return e;
+    }
+    ~Events() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+struct Graphs {
+    hipGraph_t g[2] = {nullptr, nullptr};
+    hipGraphExec_t x[2] = {nullptr, nullptr};
+    ~Graphs() {
+        for (int p = 0; p < 2; p++) {
+            if (x[p]) (void)hipGraphExecDestroy(x[p]);
+            if (g[p]) (void)hipGraphDestroy(g[p]);
+        }
+    }
+};
+
+}  // namespace
 
 extern "C" {
 
@@ -27,6 +77,7 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
     LSH_REQUIRE(f->processed == 0 && fu->count == 0, "stream: the handles must be fresh");
     LSH_REQUIRE(f->w == fu->w && f->h == fu->h, "stream: frames and fuser sizes differ");
     LSH_REQUIRE(!measure || slices, "stream: measure needs slices");
+    LSH_REQUIRE(!fu->halo_on, "stream: halo control is not supported");
     const size_t w = f->w, h = f->h, npx = w * h;
     size_t M = 0;
     if (measure) {
@@ -51,54 +102,150 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
     if (n > 1) {
         frames_stream_setup(f, free8, coarsen, n);
         fuser_reserve(fu, n);
-        f->staging.ensure(3 * npx);
-        DBuf<float> sbuf(measure ? M : 0);
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        auto step = [&](bool plain) {
-            frames_device_step(f, f->staging.p, brightness != 0, measure, sbuf.p, plain);
-            fuser_push_dev(fu, f->planes.p, f->cursor.p);
+        frames_stream_slots(f);
+        const hipStream_t Ms = stream(), Ss = side_stream();
+        float *planes[2] = {f->planes.p, f->planes_b.p};
+        const size_t fbytes = 3 * npx * sizeof(uint16_t);
+        // raw frames: the caller's device pointers, or staged into two fixed buffers (host frames, graphs)
+        const bool staged = graph || frames_mem != LSH_DEVICE;
+        DBuf<uint16_t> stage[2];
+        if (staged) {
+            stage[0].alloc(3 * npx);
+            stage[1].alloc(3 * npx);
+        }
+        auto raw = [&](uint32_t k) -> const uint16_t * { return staged ? stage[k & 1].p : frames[k]; };
+        auto put = [&](uint32_t k) {
+            if (staged) copy_mem(stage[k & 1].p, LSH_DEVICE, frames[k], frames_mem, fbytes);
+        };
+        // slices: in place in the caller's device buffer (launches), else through device buffers copied out
+        const bool direct = measure && slices_mem == LSH_DEVICE && !graph;
+        DBuf<float> sall, sbuf[2];
+        if (measure && !direct) {
+            if (graph) {
+                sbuf[0].alloc(M);
+                sbuf[1].alloc(M);
+            } else {
+                sall.alloc((size_t)n * M);
+            }
+        }
+        auto slice_at = [&](uint32_t k) -> float * {
+            if (!measure) return nullptr;
+            if (direct) return slices + (size_t)k * M;
+            return graph ? sbuf[k & 1].p : sall.p + (size_t)k * M;
+        };
+        DBuf<unsigned> fidx(1);  // the fold's frame index (side work only)
+        launch("stream_fold_index", k_counter_set, dim3(1), dim3(64), 0, fidx.p, 1u);
+        Events E;
+        const hipEvent_t e_main = E.make();
+        hipEvent_t e_target[2] = {E.make(), E.make()}, e_warp[2] = {E.make(), E.make()},
+                   e_side[2] = {E.make(), E.make()};
+        const bool bright = brightness != 0;
+        // the pieces; each goes to the stream current when it is called
+        auto chain = [&](uint32_t k, bool plain) {
+            frames_chain(f, raw(k), (int)(k & 1), planes[k & 1], bright, plain);
             frames_record(f);
         };
-        try {
-            for (uint32_t i = 1; i < n; i++) {
-                copy_mem(f->staging.p, LSH_DEVICE, frames[i], frames_mem, 3 * npx * sizeof(uint16_t));
-                if (graph) {
-                    if (!ge) {
+        auto side = [&](uint32_t k) {  // the focus slice and the fold of frame k
+            if (measure) focus_slice(f, planes[k & 1], *measure, slice_at(k));
+            fuser_push_dev(fu, planes[k & 1], fidx.p);
+            launch("stream_fold_next", k_counter_next, dim3(1), dim3(64), 0, fidx.p);
+        };
+        auto target = [&](uint32_t k) {  // frame k's target pyramid into slot k % 2
+            put(k);
+            frames_target(f, raw(k), (int)(k & 1));
+        };
+        // the side stream starts after frame 0 (the fuser's state, the frames handle's)
+        LSH_CHECK(hipEventRecord(e_main, Ms));
+        LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
+        {
+            OnStream on(Ss);
+            target(1);
+            LSH_CHECK(hipEventRecord(e_target[1], Ss));
+            if (n > 2) {
+                target(2);
+                LSH_CHECK(hipEventRecord(e_target[0], Ss));
+            }
+        }
+        if (!graph) {
+            for (uint32_t k = 1; k < n; k++) {
+                LSH_CHECK(hipStreamWaitEvent(Ms, e_target[k & 1], 0));
+                if (k >= 3) LSH_CHECK(hipStreamWaitEvent(Ms, e_side[k & 1], 0));  // planes[k % 2] read by frame k-2
+                chain(k, true);
+                LSH_CHECK(hipEventRecord(e_warp[k & 1], Ms));
+                OnStream on(Ss);
+                LSH_CHECK(hipStreamWaitEvent(Ss, e_warp[k & 1], 0));
+                side(k);
+                LSH_CHECK(hipEventRecord(e_side[k & 1], Ss));
+                if (k + 2 < n) {
+                    target(k + 2);
+                    LSH_CHECK(hipEventRecord(e_target[k & 1], Ss));
+                }
+            }
+        } else {
+            // frame 1 and the last frame run eagerly (their cycles differ); frames 2 .. n-2 replay a graph
+            // holding frame k's chain beside frame k-1's side work and frame k+1's target pyramid
+            LSH_CHECK(hipStreamWaitEvent(Ms, e_target[1], 0));
+            chain(1, true);
+            Graphs G;
+            // the host-side counts the captured cycle advances once per replay
+            auto cycle = [&](uint32_t k) {
+                LSH_CHECK(hipEventRecord(e_main, Ms));  // fork
+                LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
+                chain(k, true);
+                {
+                    OnStream on(Ss);
+                    side(k - 1);
+                    if (k + 1 < n) frames_target(f, raw(k + 1), (int)((k + 1) & 1));
+                    LSH_CHECK(hipEventRecord(e_side[0], Ss));  // join
+                }
+                LSH_CHECK(hipStreamWaitEvent(Ms, e_side[0], 0));
+            };
+            LSH_CHECK(hipStreamWaitEvent(Ms, e_target[0], 0));  // the prologue's side work is done
+            for (uint32_t k = 2; k < n; k++) {
+                if (k + 1 < n) {
+                    put(k + 1);  // on the main stream, before the replay (its slot was read by frame k-1)
+                    const int p = (int)(k & 1);
+                    if (!G.x[p]) {
                         prof().capturing = true;
-                        LSH_CHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+                        LSH_CHECK(hipStreamBeginCapture(Ms, hipStreamCaptureModeThreadLocal));
                         try {
-                            step(true);
+                            cycle(k);
                         } catch (...) {
                             hipGraph_t junk = nullptr;
-                            (void)hipStreamEndCapture(stream(), &junk);
+                            (void)hipStreamEndCapture(Ms, &junk);
                             if (junk) (void)hipGraphDestroy(junk);
                             prof().capturing = false;
                             throw;
                         }
-                        LSH_CHECK(hipStreamEndCapture(stream(), &g));
+                        LSH_CHECK(hipStreamEndCapture(Ms, &G.g[p]));
                         prof().capturing = false;
-                        LSH_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                        LSH_CHECK(hipGraphInstantiate(&G.x[p], G.g[p], nullptr, nullptr, 0));
                     } else {
-                        // the host-side counts the captured step advanced once
                         f->processed++;
                         fu->count++;
                     }
-                    LSH_CHECK(hipGraphLaunch(ge, stream()));
+                    LSH_CHECK(hipGraphLaunch(G.x[p], Ms));
                 } else {
-                    step(false);
+                    cycle(k);  // the last frame: no target after it
                 }
-                if (measure)
-                    copy_mem(slices + (size_t)i * M, slices_mem, sbuf.p, LSH_DEVICE, M * sizeof(float));
+                if (measure) copy_mem(slices + (size_t)(k - 1) * M, slices_mem, sbuf[(k - 1) & 1].p, LSH_DEVICE,
+                                      M * sizeof(float));
             }
-        } catch (...) {
-            if (ge) (void)hipGraphExecDestroy(ge);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
+            // the last frame's side work
+            LSH_CHECK(hipEventRecord(e_main, Ms));
+            {
+                OnStream on(Ss);
+                LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
+                side(n - 1);
+                LSH_CHECK(hipEventRecord(e_side[1], Ss));
+            }
+            LSH_CHECK(hipStreamWaitEvent(Ms, e_side[1], 0));
+            if (measure) copy_mem(slices + (size_t)(n - 1) * M, slices_mem, sbuf[(n - 1) & 1].p, LSH_DEVICE,
+                                  M * sizeof(float));
         }
         sync();
-        if (ge) LSH_CHECK(hipGraphExecDestroy(ge));
-        if (g) LSH_CHECK(hipGraphDestroy(g));
+        if (measure && !direct && !graph)
+            copy_mem(slices + M, slices_mem, sall.p + M, LSH_DEVICE, (size_t)(n - 1) * M * sizeof(float));
         std::vector<double> ds((size_t)n * 8);
         std::vector<float> dg((size_t)n * 3);
         frames_stream_results(f, n, ds.data(), dg.data(), evals.data());

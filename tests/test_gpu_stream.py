@@ -1,5 +1,6 @@
-"""The streamed stack (lsh_stack_stream): no host round trip per frame, the
-per-frame step captured once as a hipGraph and replayed.
+"""The streamed stack (lsh_stack_stream): no host round trip per frame; each
+frame's serial chain (search, gains, warp) runs beside the previous frame's
+focus slice and fold, as two streams or as a replayed two-branch hipGraph.
 
 Bars: the graph replay and the per-frame launches of the same step agree
 bit for bit (the same kernels on the same data); against the ordinary
@@ -44,7 +45,9 @@ def _per_frame(frames, fp, ap, dp):
     return dict(image=image, winner=winner, sims=np.array(sims), evals=np.array(evals), slices=slices)
 
 
-@pytest.mark.parametrize("w,h,n,interp", [(384, 288, 5, "bilinear"), (320, 240, 4, "spline4x4")])
+# 5 and 4 frames: each parity's graph captured once; 9 frames: both replayed (frames 2..7 replay, 8 runs eagerly)
+@pytest.mark.parametrize("w,h,n,interp", [(384, 288, 5, "bilinear"), (320, 240, 4, "spline4x4"),
+                                          (256, 192, 9, "bilinear")])
 def test_stream_matches_the_per_frame_path(w, h, n, interp):
     print(f"stream {w}x{h}x{n} {interp}", flush=True)
     b = synth.Bracket(w, h, n, seed=17, jitter=True, device="cuda")
