@@ -1077,7 +1077,7 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
     K.solo_max = 4096;
     if (const char *e = std::getenv("LSH_SEARCH_SOLO_MAX")) K.solo_max = std::atoi(e);
     // speculation pays where a pass is dominated by the grid's synchronisation
-    K.spec_max = 400000;
+    K.spec_max = 1000000;
     if (const char *e = std::getenv("LSH_SEARCH_SPEC_MAX")) K.spec_max = std::atoi(e);
     for (size_t l = 0; l < n; l++) {
         K.ref[l] = f->yr.p + f->loff[l];
