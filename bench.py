@@ -91,20 +91,41 @@ class Job:
             out = la.stack_stream(self.frames[:n], fp, apm, dp, graph=path == "graph", slices_out=self.slices[:n],
                                   img_out=self.image, winner_out=self.winner)
             dm = la.depth_from_slices(self.slices[:n], n, self.image, dp, self.depth, self.conf, self.floor)
-            return dict(evals=int(out["evals"].sum()), sims=out["sims"], info=dm.info)
+            return dict(evals=int(out["evals"].sum()), sims=out["sims"], gains=out["gains"], info=dm.info)
         fr = la.HipFrames(self.w, self.h, apm.interp)
         fu = la.Fuser(self.w, self.h, fp)
-        last, evals, sims = la.SIM_ID, 0, []
+        last, evals, sims, gains = la.SIM_ID, 0, [], []
         for i in range(n):
-            last, _, _ = fr.process(self.frames[i], last, apm.free(), apm.coarsen, None, True, dp, self.slices[i])
+            last, g, _ = fr.process(self.frames[i], last, apm.free(), apm.coarsen, None, True, dp, self.slices[i])
             evals += fr.last_evals
             sims.append(last)
+            gains.append(g)
             fu.push_frames(fr)
         fu.finish(self.image, self.winner)
         dm = la.depth_from_slices(self.slices[:n], n, self.image, dp, self.depth, self.conf, self.floor)
         fr.close()
         fu.close()
-        return dict(evals=evals, sims=sims, info=dm.info)
+        return dict(evals=evals, sims=sims, gains=gains, info=dm.info)
+
+    def weighted_average(self, apm, dp, sims, gains):
+        """wav.rs:97 weighted_average over the resident frames after the depth
+        pass: each frame re-registered with the run's Sim and gains (a re-pass:
+        the warp, no search), its focus measure, its weights above the run's noise
+        floor smoothed by the guided filter on the fused luma, the weighted sums."""
+        import torch
+
+        import lapstack_amd as la
+
+        out = torch.empty((3, self.h, self.w), dtype=torch.float32, device="cuda")
+        fr = la.HipFrames(self.w, self.h, apm.interp)
+        acc = la.WeightedAverage(self.w, self.h, self.floor, self.image, dp, la.WavParams())
+        for i in range(len(self.frames)):
+            fr.process(self.frames[i], la.SIM_ID, (0,) * 8, 0, (sims[i], gains[i]), True, None)
+            acc.push_frames(fr)
+        acc.finish(out)
+        acc.close()
+        fr.close()
+        return out
 
 
 def sync():
@@ -292,6 +313,16 @@ def main():
         paths[path] = round(1000.0 * (time.perf_counter() - tp), 2)
         log(f"{path} path: {paths[path]} ms per step")
 
+    # the weighted average (wav.rs:97), a second pass over the stack after the depth pass, untimed by the contract
+    rep0 = reports[-1]
+    job.weighted_average(apm, dp, rep0["sims"], rep0["gains"])
+    sync()
+    tw = time.perf_counter()
+    job.weighted_average(apm, dp, rep0["sims"], rep0["gains"])
+    sync()
+    wav_ms = 1000.0 * (time.perf_counter() - tw)
+    log(f"weighted average: {wav_ms:.1f} ms for {len(frames)} frames")
+
     # per-kernel timing pass (HIP events on the library's stream), off the timed region; the host-steered
     # path runs the same kernels and reads back each search's passes per level, which gives the search's
     # algorithmic bytes (a pass reads the level's reference and target once)
@@ -343,6 +374,10 @@ def main():
         "align_evals_per_frame": round(last["evals"] / max(n - 1, 1), 1),
         "wls": last["info"],
         "per_frame_paths_ms_per_step": paths,
+        "weighted_average": {"ms_per_stack": round(wav_ms, 2), "ms_per_frame": round(wav_ms / len(frames), 3),
+                             "what": "wav.rs:97 after the depth pass: per frame the re-pass warp with the run's Sim "
+                                     "and gains, focus measure, weights above the noise floor smoothed by the guided "
+                                     "filter (r=3), bilinear weighted sums"},
         "wls_precision": wls,
         "reference_published": {"note": "RTX 3060 + 128-thread host, 100x8280x5520 TIFF stack incl. decode "
                                         "(docs/engine.md:621-622)", "cuda_frames_per_s": round(100 / 26.0, 3),

@@ -258,6 +258,40 @@ char *lsh_depth_grid(const float *slices, lsh_mem slices_mem, uint32_t n, const 
                      uint32_t w, uint32_t h, const lsh_depth_params *p, float *guide, float *depth, float *conf,
                      lsh_mem out_mem, lsh_depth_info *info);
 
+/* ---- the weighted average, slabs -------------------------------------- */
+
+/* wav.rs:60-80 WavParams: the contrast above the floor raised to power; the
+ * box radius (grid cells) the contrast and then the weights are smoothed by,
+ * 0 none; the margin over the noise floor; the weights' smoothing follows the
+ * fused image's edges (the guided filter) when edge != 0 and a guide is given. */
+typedef struct lsh_wav_params {
+    float power;
+    uint32_t smooth;
+    float gate;
+    uint32_t edge;
+} lsh_wav_params;
+
+typedef struct lsh_wav lsh_wav;
+
+/* wav.rs:97-161 weighted_average as an accumulator over the aligned frames:
+ * w x h frames, the depth pass's measure and grid (dp), floor the depth pass's
+ * per-cell noise floor (dw*dh, lsh_depth_from_slices' floor_plane) in
+ * floor_mem, fused3 the fused image (3*w*h, the guide; nullable) in fused_mem.
+ * Push every frame (3*w*h planes, aligned and equalized) with its focus slice
+ * (dw*dh, as the depth pass took it; nullable: measured here), or the frame a
+ * frames handle holds; finish writes the average (3*w*h) into mem. */
+char *lsh_wav_new(uint32_t w, uint32_t h, const lsh_depth_params *dp, const lsh_wav_params *wp, const float *floor,
+                  lsh_mem floor_mem, const float *fused3, lsh_mem fused_mem, lsh_wav **out);
+char *lsh_wav_push(lsh_wav *a, const float *img3, lsh_mem img_mem, const float *slice, lsh_mem slice_mem);
+char *lsh_wav_push_frames(lsh_wav *a, lsh_frames *src);
+char *lsh_wav_finish(lsh_wav *a, float *img3, lsh_mem mem);
+void lsh_wav_free(lsh_wav *a);
+
+/* stack.rs:76-92 slab_ranges: the [lo, hi] frame ranges of slabs of size frames
+ * overlapping by overlap; returns their count, writes up to cap pairs into lohi
+ * (nullable).  Each slab is fused on its own (a fuser per slab, stack.rs:873-893). */
+uint32_t lsh_slab_ranges(uint32_t count, uint32_t size, uint32_t overlap, uint32_t *lohi, uint32_t cap);
+
 /* ---- the WLS solve alone, in three precisions ------------------------- */
 
 typedef enum lsh_wls_mode {

@@ -38,7 +38,8 @@ from . import _lib
 from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
-           "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
+           "weighted_average", "slab_ranges", "fuse_slabs", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -344,6 +345,132 @@ def depth_from_slices(slices, n: int, fused, params: Optional[DepthParams] = Non
     del sk, fk, dk, ck
     d = {k: getattr(info, k) for k, _ in info._fields_}
     return DepthMap(depth_out, conf_out, w, h, dw, dh, floor_out, d)
+
+
+@dataclasses.dataclass
+class WavParams:
+    """wav.rs:60-80 WavParams (defaults wav.rs:82-86)."""
+
+    power: float = 2.0
+    smooth: int = 3
+    gate: float = 0.5
+    edge: bool = True
+
+    def _c(self):
+        return _lib.WavParamsC(float(self.power), int(self.smooth), float(self.gate), int(bool(self.edge)))
+
+
+class WeightedAverage:
+    """wav.rs:97-161 weighted_average as a device accumulator (include/lsh.h
+    lsh_wav_*): made with the depth pass's floor and the fused image (the guide),
+    fed every aligned frame, finished into the average."""
+
+    def __init__(self, w: int, h: int, floor, fused=None, depth: Optional[DepthParams] = None,
+                 params: Optional[WavParams] = None):
+        self._lib = _lib.load()
+        self.w, self.h = w, h
+        self.depth = depth or DepthParams()
+        self.params = params or WavParams()
+        dw, dh = self.depth.grid(w, h)
+        fshape = tuple(floor.shape) if hasattr(floor, "shape") else tuple(np.shape(floor))
+        if fshape not in ((dh, dw), (dh * dw,)):
+            raise HipError(f"weighted average: the noise floor has shape {fshape}, the grid is {dh}x{dw}")
+        fp, fmem, fk = _buf(floor, np.float32)
+        gp, gmem, gk = _buf(fused, np.float32) if fused is not None else (None, LSH_HOST, None)
+        h_ = C.c_void_p()
+        _lib.check(self._lib.lsh_wav_new(w, h, C.byref(self.depth._c()), C.byref(self.params._c()), fp, fmem, gp,
+                                         gmem, C.byref(h_)))
+        del fk, gk
+        self._h = h_
+        self.count = 0
+
+    def push(self, img3, slice_=None):
+        """One aligned frame ((3, h, w) planes) and, optionally, its focus slice as the depth pass took it."""
+        ip, imem, ik = _buf(img3, np.float32)
+        sp, smem, sk = _buf(slice_, np.float32) if slice_ is not None else (None, LSH_HOST, None)
+        _lib.check(self._lib.lsh_wav_push(self._h, ip, imem, sp, smem))
+        del ik, sk
+        self.count += 1
+
+    def push_frames(self, frames: HipFrames):
+        """The frame a frames handle holds (after process/ingest), without a copy."""
+        _lib.check(self._lib.lsh_wav_push_frames(self._h, frames._h))
+        self.count += 1
+
+    def finish(self, out=None):
+        if out is None:
+            out = np.zeros((3, self.h, self.w), np.float32)
+        op, omem, ok = _buf(out, np.float32)
+        _lib.check(self._lib.lsh_wav_finish(self._h, op, omem))
+        del ok
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lsh_wav_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def weighted_average(frames, depth: DepthParams, params: WavParams, floor, guide=None, slices=None, out=None):
+    """wav.rs:97 weighted_average(src, dp, wp, floor, guide): the aligned
+    frames ((3, h, w) each, host or device) averaged by their contrast above
+    the depth pass's noise floor; ``slices`` (the depth pass's focus slices,
+    optional) spare measuring them again."""
+    frames = list(frames)
+    if not frames:
+        raise HipError("weighted average: no frames")
+    _, h, w = tuple(frames[0].shape)
+    acc = WeightedAverage(w, h, floor, guide, depth, params)
+    try:
+        for i, f in enumerate(frames):
+            acc.push(f, slices[i] if slices is not None else None)
+        return acc.finish(out)
+    finally:
+        acc.close()
+
+
+def slab_ranges(count: int, size: int, overlap: int):
+    """stack.rs:76-92 slab_ranges: the (lo, hi) frame ranges of the slabs."""
+    lib = _lib.load()
+    ns = lib.lsh_slab_ranges(count, size, overlap, None, 0)
+    buf = (C.c_uint32 * max(2 * ns, 2))()
+    lib.lsh_slab_ranges(count, size, overlap, buf, ns)
+    return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(ns)]
+
+
+def fuse_slabs(frames, size: int, overlap: int, fuse: Optional[FuseParams] = None, sims=None, gains=None,
+               interp="spline4x4", brightness: bool = True):
+    """stack.rs:873-893 fuse_slabs: every slab of slab_ranges fused on its own
+    over the same frames, returned as [(lo, hi, image, winner)].  Each frame is
+    taken once -- registered with its Sim and gains from a run over the whole
+    stack when ``sims``/``gains`` are given (a re-pass, no search), as decoded
+    otherwise -- and folded into every slab that holds it."""
+    frames = list(frames)
+    n = len(frames)
+    ranges = slab_ranges(n, size, overlap)
+    if not ranges:
+        return []
+    fuse = fuse or FuseParams()
+    h, w, _ = tuple(frames[0].shape)
+    fr = HipFrames(w, h, interp)
+    fusers = [Fuser(w, h, fuse) for _ in ranges]
+    try:
+        for i, f in enumerate(frames):
+            if sims is not None:
+                fr.process(f, SIM_ID, (0,) * 8, 0, (sims[i], gains[i]), brightness, None)
+            else:
+                fr.ingest(f, brightness, None)
+            for (lo, hi), fu in zip(ranges, fusers):
+                if lo <= i <= hi:
+                    fu.push_frames(fr)
+        return [(lo, hi) + tuple(fu.finish()) for (lo, hi), fu in zip(ranges, fusers)]
+    finally:
+        for fu in fusers:
+            fu.close()
+        fr.close()
 
 
 def depth_grid(slices, n: int, fused, params: Optional[DepthParams] = None, out_device: bool = False):

@@ -44,6 +44,10 @@ class BatchStackC(C.Structure):
     _fields_ = [("w", _u32), ("h", _u32), ("n", _u32), ("frames", _p)]
 
 
+class WavParamsC(C.Structure):
+    _fields_ = [("power", C.c_float), ("smooth", _u32), ("gate", C.c_float), ("edge", _u32)]
+
+
 class BatchOptionsC(C.Structure):
     _fields_ = [("tune", _u32), ("tile_h", _u32), ("blocks_per_cu", _u32), ("frames_per_round", _u32)]
 
@@ -98,6 +102,12 @@ SIGNATURES = [
     ("lsh_stack_stream", _p, [_p, _p, _p, _u32, C.c_int, _p, _u32, _u32, C.POINTER(DepthParamsC), _p, C.c_int, _u32,
                               _p, _p, _p]),
     ("lsh_batch_fuse", _p, [_p, _p, _u32, C.POINTER(BatchOptionsC), C.POINTER(BatchReportC)]),
+    ("lsh_wav_new", _p, [_u32, _u32, _p, C.POINTER(WavParamsC), _p, _u32, _p, _u32, C.POINTER(_p)]),
+    ("lsh_wav_push", _p, [_p, _p, _u32, _p, _u32]),
+    ("lsh_wav_push_frames", _p, [_p, _p]),
+    ("lsh_wav_finish", _p, [_p, _p, _u32]),
+    ("lsh_wav_free", None, [_p]),
+    ("lsh_slab_ranges", _u32, [_u32, _u32, _u32, _p, _u32]),
     ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
                            _p, _p, C.c_int, C.POINTER(WlsReportC)]),
 ]

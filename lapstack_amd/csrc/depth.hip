@@ -100,12 +100,24 @@ static void rdf_taps_ordered(uint32_t r_in, uint32_t r_out, std::vector<int32_t>
             }
 }
 
+// The focus slice of planes (w x h) with a holder of the measure's taps
+// (uploaded when the measure changes).
+static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
+                             uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
+                             float *out);
+
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out) {
+    focus_slice_with(f->taps_dy, f->taps_dx, f->taps_wt, f->ntaps, f->taps_key, f->w, f->h, planes, p, out);
+}
+
+static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
+                             uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
+                             float *out) {
     LSH_REQUIRE(p.scale <= 2, "focus slice: scale above 2 is not supported");
     LSH_REQUIRE(p.focus_kind <= 1, "focus slice: unknown focus measure");
     int k = 1 << p.scale;
     uint32_t key = p.focus_kind == 0 ? (p.r_in << 8 | p.r_out) : (0x10000u | p.sml_step);
-    if (key != f->taps_key) {
+    if (key != taps_key) {
         std::vector<int32_t> dy, dx;
         std::vector<float> wt;
         if (p.focus_kind == 0) {
@@ -117,26 +129,26 @@ void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, 
             dx = {0};
             wt = {0.f};
         }
-        f->taps_dy.alloc(dy.size());
-        f->taps_dx.alloc(dx.size());
-        f->taps_wt.alloc(wt.size());
-        copy_mem(f->taps_dy.p, LSH_DEVICE, dy.data(), LSH_HOST, dy.size() * sizeof(int32_t));
-        copy_mem(f->taps_dx.p, LSH_DEVICE, dx.data(), LSH_HOST, dx.size() * sizeof(int32_t));
-        copy_mem(f->taps_wt.p, LSH_DEVICE, wt.data(), LSH_HOST, wt.size() * sizeof(float));
-        f->ntaps = (uint32_t)wt.size();
-        f->taps_key = key;
+        taps_dy.alloc(dy.size());
+        taps_dx.alloc(dx.size());
+        taps_wt.alloc(wt.size());
+        copy_mem(taps_dy.p, LSH_DEVICE, dy.data(), LSH_HOST, dy.size() * sizeof(int32_t));
+        copy_mem(taps_dx.p, LSH_DEVICE, dx.data(), LSH_HOST, dx.size() * sizeof(int32_t));
+        copy_mem(taps_wt.p, LSH_DEVICE, wt.data(), LSH_HOST, wt.size() * sizeof(float));
+        ntaps = (uint32_t)wt.size();
+        taps_key = key;
     }
     Taps T;
     T.kind = (int)p.focus_kind;
-    T.n = (int)f->ntaps;
+    T.n = (int)ntaps;
     T.step = (int)std::max<uint32_t>(p.sml_step, 1);
     T.halo = p.focus_kind == 0 ? (int)p.r_out : T.step;
-    int dw = (int)((f->w + k - 1) / k), dh = (int)((f->h + k - 1) / k);
+    int dw = (int)((w + k - 1) / k), dh = (int)((h + k - 1) / k);
     size_t shm = (size_t)(64 * k + 2 * T.halo) * (4 * k + 2 * T.halo) * sizeof(float);
-    launch_b("focus_slice", 12.0 * f->w * f->h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4)),
+    launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4)),
              dim3(64, 4), shm, (const float *)planes,
-           (int)f->w, (int)f->h, k, (const int32_t *)f->taps_dy.p, (const int32_t *)f->taps_dx.p,
-           (const float *)f->taps_wt.p, T, out, dw, dh);
+           (int)w, (int)h, k, (const int32_t *)taps_dy.p, (const int32_t *)taps_dx.p,
+           (const float *)taps_wt.p, T, out, dw, dh);
 }
 
 // ----------------------------------------------------------------- helpers
@@ -698,6 +710,239 @@ char *lsh_depth_grid(const float *slices, lsh_mem slices_mem, uint32_t n, const 
         info->floor = G.floor_;
     }
     LSH_API_END
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------- the weighted average
+// wav.rs:97-161 as an accumulator: per aligned frame the focus slice on the
+// working grid (the depth pass's measure), its box mean, the weights above the
+// noise floor, their smoothing (the guided filter on the fused luma's grid
+// means, or the box mean), and the weighted sums at full resolution with the
+// grid weights taken there bilinearly on the fly; at the end the clamped
+// quotient.  Each step is the reference's operation in its order.
+namespace lsh {
+
+// wav.rs:88-91
+__global__ void k_wav_weight(const float *__restrict__ c, const float *__restrict__ floor_p, size_t n, float gate,
+                             float power, float *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float f = floor_p[i];
+    out[i] = powf(fmaxf(c[i] - (1.0f + gate) * f, 0.0f), power) + powf(1e-2f * f, power) + 1e-30f;
+}
+
+// wav.rs:142: the guided filter undershoots at an edge, a weight stays a weight
+__global__ void k_wav_atleast(float *__restrict__ v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = fmaxf(v[i], 1e-30f);
+}
+
+// wav.rs:146-155: the grid weights at full resolution (depth.rs:201-221) and the weighted sums
+__global__ void k_wav_accum(const float *__restrict__ g, int dw, int dh, int w, int h, float inv,
+                            const float *__restrict__ planes, float *__restrict__ acc, float *__restrict__ wsum) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const size_t n = (size_t)w * h, o = (size_t)y * w + x;
+    const float wv = bilinear_at(g, dw, dh, x, y, inv);
+    acc[o] += wv * planes[o];
+    acc[n + o] += wv * planes[n + o];
+    acc[2 * n + o] += wv * planes[2 * n + o];
+    wsum[o] += wv;
+}
+
+// wav.rs:158-160
+__global__ void k_wav_finish(const float *__restrict__ acc, const float *__restrict__ wsum, size_t n,
+                             float *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float s = wsum[i];
+    out[i] = rclampf(acc[i] / s, 0.0f, 1.0f);
+    out[n + i] = rclampf(acc[n + i] / s, 0.0f, 1.0f);
+    out[2 * n + i] = rclampf(acc[2 * n + i] / s, 0.0f, 1.0f);
+}
+
+// the box mean of one grid plane (depth.rs:263-347)
+static void box_mean_plane(const float *src, int w, int h, int r, float *out, float *t1, float *t2, float *spare) {
+    Peaks none{};
+    double M = (double)w * h;
+    launch_b("box_h", 12.0 * M, k_box_h2, g2(w, h), B2, 0, src, src, (const float *)nullptr, 0, w, h, r, t1, t2);
+    launch_b("box_v", 16.0 * M, k_box_v2, g2(w, h), B2, 0, (const float *)t1, (const float *)t2, w, h, r, 3,
+             (const float *)nullptr, (const float *)nullptr, (const float *)nullptr, 0.0f, out, spare, none, 0u);
+}
+
+}  // namespace lsh
+
+struct lsh_wav {
+    size_t w = 0, h = 0, k = 1, dw = 0, dh = 0, count = 0;
+    lsh_depth_params dp{};
+    float power = 2.0f, gate = 0.5f;
+    uint32_t smooth = 3;
+    bool guided = false;
+    lsh::GF gf;
+    lsh::DBuf<float> floor, gy, acc, wsum, g, t, b1, b2, b3, slice, stage;
+    lsh::DBuf<int32_t> taps_dy, taps_dx;
+    lsh::DBuf<float> taps_wt;
+    uint32_t ntaps = 0, taps_key = 0xffffffffu;
+};
+
+namespace lsh {
+static void wav_push(lsh_wav *a, const float *planes, const float *slice) {
+    const int dw = (int)a->dw, dh = (int)a->dh;
+    const size_t m = a->dw * a->dh;
+    const float *c = slice;
+    if (!c) {
+        focus_slice_with(a->taps_dy, a->taps_dx, a->taps_wt, a->ntaps, a->taps_key, a->w, a->h, planes, a->dp,
+                         a->slice.p);
+        c = a->slice.p;
+    }
+    if (a->smooth > 0) {
+        box_mean_plane(c, dw, dh, (int)a->smooth, a->g.p, a->b1.p, a->b2.p, a->b3.p);
+        c = a->g.p;
+    }
+    launch_b("wav_weight", 12.0 * m, k_wav_weight, dim3(g1(m)), dim3(256), 0, c, (const float *)a->floor.p, m, a->gate,
+             a->power, a->t.p);
+    const float *wg = a->t.p;
+    if (a->guided) {
+        Peaks none{};
+        gf_step1(a->gf, a->t.p);
+        gf_step2(a->gf, 4, a->g.p, nullptr, none, 0u);
+        launch("wav_atleast", k_wav_atleast, dim3(g1(m)), dim3(256), 0, a->g.p, m);
+        wg = a->g.p;
+    } else if (a->smooth > 0) {
+        box_mean_plane(a->t.p, dw, dh, (int)a->smooth, a->g.p, a->b1.p, a->b2.p, a->b3.p);
+        wg = a->g.p;
+    }
+    const double N = (double)a->w * a->h;
+    launch_b("wav_accum", 32.0 * N, k_wav_accum, g2((int)a->w, (int)a->h), B2, 0, wg, dw, dh, (int)a->w, (int)a->h,
+             1.0f / (float)a->k, planes, a->acc.p, a->wsum.p);
+    a->count++;
+}
+}  // namespace lsh
+
+extern "C" {
+
+char *lsh_wav_new(uint32_t w, uint32_t h, const lsh_depth_params *dp, const lsh_wav_params *wp, const float *floor_p,
+                  lsh_mem floor_mem, const float *fused3, lsh_mem fused_mem, lsh_wav **out) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(dp && wp && floor_p && out, "lsh_wav_new: null argument");
+    *out = nullptr;
+    LSH_REQUIRE(w > 0 && h > 0, "weighted average: empty frame");
+    LSH_REQUIRE(dp->scale <= 2, "weighted average: scale above 2 is not supported");
+    auto a = new lsh_wav();
+    try {
+        a->w = w;
+        a->h = h;
+        a->k = (size_t)1 << dp->scale;
+        a->dw = (w + a->k - 1) / a->k;
+        a->dh = (h + a->k - 1) / a->k;
+        a->dp = *dp;
+        a->power = std::max(wp->power, 0.0f);  // wav.rs:110-111
+        a->gate = std::max(wp->gate, 0.0f);
+        a->smooth = wp->smooth;
+        a->guided = wp->edge && wp->smooth > 0 && fused3 != nullptr;
+        const size_t m = a->dw * a->dh, N = (size_t)w * h;
+        a->floor.alloc(m);
+        copy_mem(a->floor.p, LSH_DEVICE, floor_p, floor_mem, m * sizeof(float));
+        for (auto *b : {&a->g, &a->t, &a->b1, &a->b2, &a->b3, &a->slice}) b->alloc(m);
+        a->acc.alloc(3 * N);
+        a->wsum.alloc(N);
+        a->acc.zero();
+        a->wsum.zero();
+        if (a->guided) {  // wav.rs:114-122: the fused luma's block means on the grid guide the weights
+            DBuf<float> f3, y(N);
+            const float *src = fused3;
+            if (fused_mem == LSH_HOST) {
+                f3.alloc(3 * N);
+                copy_mem(f3.p, LSH_DEVICE, fused3, LSH_HOST, 3 * N * sizeof(float));
+                src = f3.p;
+            }
+            launch("luma", k_luma3, dim3(g1(N)), dim3(256), 0, src, N, y.p);
+            a->gy.alloc(m);
+            launch("block_mean", k_block_mean, g2((int)a->dw, (int)a->dh), B2, 0, (const float *)y.p, (int)w, (int)h,
+                   (int)a->k, a->gy.p, (int)a->dw, (int)a->dh);
+            gf_init(a->gf, a->gy.p, (int)a->dw, (int)a->dh, (int)a->smooth, dp->agg_eps);
+            sync();
+        }
+    } catch (...) {
+        delete a;
+        throw;
+    }
+    *out = a;
+    LSH_API_END
+}
+
+char *lsh_wav_push(lsh_wav *a, const float *img3, lsh_mem img_mem, const float *slice, lsh_mem slice_mem) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(a && img3, "lsh_wav_push: null argument");
+    const size_t N = a->w * a->h, m = a->dw * a->dh;
+    const float *planes = img3;
+    if (img_mem == LSH_HOST) {
+        a->stage.ensure(3 * N);
+        copy_mem(a->stage.p, LSH_DEVICE, img3, LSH_HOST, 3 * N * sizeof(float));
+        planes = a->stage.p;
+    }
+    const float *sl = slice;
+    if (slice && slice_mem == LSH_HOST) {
+        copy_mem(a->slice.p, LSH_DEVICE, slice, LSH_HOST, m * sizeof(float));
+        sl = a->slice.p;
+    }
+    wav_push(a, planes, sl);
+    LSH_API_END
+}
+
+char *lsh_wav_push_frames(lsh_wav *a, lsh_frames *src) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(a && src, "lsh_wav_push_frames: null argument");
+    LSH_REQUIRE(src->w == a->w && src->h == a->h, "weighted average: the frames handle's size differs");
+    LSH_REQUIRE(src->processed > 0, "weighted average: the frames handle holds no frame");
+    wav_push(a, src->planes.p, nullptr);
+    LSH_API_END
+}
+
+char *lsh_wav_finish(lsh_wav *a, float *img3, lsh_mem mem) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(a && img3, "lsh_wav_finish: null argument");
+    LSH_REQUIRE(a->count > 0, "weighted average: no frames");
+    const size_t N = a->w * a->h;
+    DBuf<float> tmp;
+    float *dst = img3;
+    if (mem == LSH_HOST) {
+        tmp.alloc(3 * N);
+        dst = tmp.p;
+    }
+    launch_b("wav_finish", 28.0 * N, k_wav_finish, dim3(g1(N)), dim3(256), 0, (const float *)a->acc.p,
+             (const float *)a->wsum.p, N, dst);
+    if (mem == LSH_HOST) copy_mem(img3, LSH_HOST, tmp.p, LSH_DEVICE, 3 * N * sizeof(float));
+    sync();
+    LSH_API_END
+}
+
+void lsh_wav_free(lsh_wav *a) {
+    if (!a) return;
+    try {
+        sync();
+    } catch (...) {
+    }
+    delete a;
+}
+
+uint32_t lsh_slab_ranges(uint32_t count, uint32_t size, uint32_t overlap, uint32_t *lohi, uint32_t cap) {
+    // stack.rs:76-92
+    if (count == 0) return 0;
+    size = std::min(std::max(size, 1u), count);
+    overlap = std::min(overlap, size - 1);
+    uint32_t lo = 0, ns = 0;
+    for (;;) {
+        const uint32_t hi = std::min(lo + size - 1, count - 1);
+        if (lohi && ns < cap) {
+            lohi[2 * ns] = lo;
+            lohi[2 * ns + 1] = hi;
+        }
+        ns++;
+        if (hi + 1 >= count) return ns;
+        lo = hi + 1 - overlap;
+    }
 }
 
 }  // extern "C"

@@ -2262,6 +2262,97 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
     return 0;
 }
 
+/* ---- wav.rs: the weighted average ---- */
+
+/* wav.rs:88-91: a cell's weight, the contrast above (1 + gate) x the noise
+ * floor raised to power, plus the even weight of a cell nothing rises above */
+float ora_wav_weight(float c, float f, float gate, float power) {
+    return powf(rmaxf(c - (1.0f + gate) * f, 0.0f), power) + powf(1e-2f * f, power) + 1e-30f;
+}
+
+/* wav.rs:97-161 weighted_average over n aligned frames (n x 3*w*h planes):
+ * per frame the focus slice (luma, focus measure, block mean), its box mean
+ * (smooth > 0), the weights, their smoothing (the guided filter on the fused
+ * luma's grid means when edge and a guide are given, else the box mean),
+ * the bilinear upsample, and the weighted sums; then the clamped quotient.
+ * floor: dw*dh (the depth pass's noise floor).  guide3: the fused image or NULL. */
+void ora_weighted_average(const float *frames, size_t n, size_t w, size_t h, const ora_depth_params *dp, float power,
+                          size_t smooth, float gate, int edge, const float *floor, const float *guide3, float *out3) {
+    const size_t k = (size_t)1 << dp->scale, dw = blocks(w, k), dh = blocks(h, k), m = dw * dh, N = w * h;
+    power = rmaxf(power, 0.0f);
+    gate = rmaxf(gate, 0.0f);
+    const int have_bf = smooth > 0, have_gf = edge && smooth > 0 && guide3 != NULL;
+    boxf_t bf;
+    if (have_bf) boxf_init(&bf, dw, dh, smooth);
+    guided_t gf;
+    float *gy = NULL;
+    if (have_gf) {
+        float *y = fzeros(N);
+        ora_luma(guide3, N, y);
+        gy = fzeros(m);
+        ora_block_mean(y, w, h, k, gy);
+        free(y);
+        guided_init(&gf, gy, dw, dh, smooth, dp->agg_eps);
+    }
+    float *acc = fzeros(3 * N), *wsum = fzeros(N), *g = fzeros(m), *t = fzeros(m), *wf = fzeros(N);
+    for (size_t q = 0; q < n; q++) {
+        const float *f = frames + q * 3 * N;
+        ora_focus_slice(f, w, h, dp, g);
+        if (have_bf) {
+            boxf_mean(&bf, g, t);
+            memcpy(g, t, m * sizeof(float));
+        }
+        for (size_t i = 0; i < m; i++) t[i] = ora_wav_weight(g[i], floor[i], gate, power);
+        if (have_gf) {
+            guided_filter_into(&gf, t, g);
+            for (size_t i = 0; i < m; i++) g[i] = rmaxf(g[i], 1e-30f);  /* a weight stays a weight */
+        } else if (have_bf) {
+            boxf_mean(&bf, t, g);
+        } else {
+            memcpy(g, t, m * sizeof(float));
+        }
+        ora_upsample_bilinear(g, dw, dh, w, h, k, wf);
+        for (size_t c = 0; c < 3; c++) {
+            float *a = acc + c * N;
+            const float *s = f + c * N;
+#pragma omp parallel for schedule(static) if (N >= PAR_MIN)
+            for (size_t i = 0; i < N; i++) a[i] += wf[i] * s[i];
+        }
+        for (size_t i = 0; i < N; i++) wsum[i] += wf[i];
+    }
+    for (size_t c = 0; c < 3; c++)
+        for (size_t i = 0; i < N; i++) out3[c * N + i] = rclampf(acc[c * N + i] / wsum[i], 0.0f, 1.0f);
+    free(acc);
+    free(wsum);
+    free(g);
+    free(t);
+    free(wf);
+    if (have_bf) boxf_free(&bf);
+    if (have_gf) {
+        guided_free(&gf);
+        free(gy);
+    }
+}
+
+/* stack.rs:76-92: the frame ranges [lo, hi] of the slabs; returns their count
+ * (at most cap written, two entries each) */
+size_t ora_slab_ranges(size_t count, size_t size, size_t overlap, size_t *out2, size_t cap) {
+    if (count == 0) return 0;
+    size = size < 1 ? 1 : (size > count ? count : size);
+    overlap = overlap < size - 1 ? overlap : size - 1;
+    size_t lo = 0, ns = 0;
+    for (;;) {
+        size_t hi = lo + size - 1 < count - 1 ? lo + size - 1 : count - 1;
+        if (ns < cap) {
+            out2[2 * ns] = lo;
+            out2[2 * ns + 1] = hi;
+        }
+        ns++;
+        if (hi + 1 >= count) return ns;
+        lo = hi + 1 - overlap;
+    }
+}
+
 /* ======================================================================
  * io.rs and stack.rs
  * ==================================================================== */

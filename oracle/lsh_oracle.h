@@ -159,6 +159,14 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
                           const ora_depth_params *p, float *depth, float *conf, float *floor_plane,
                           ora_depth_out *info);
 
+/* ---- wav.rs ---- */
+float ora_wav_weight(float c, float f, float gate, float power);
+/* frames: n x 3*w*h; floor: the depth pass's grid floor; guide3: the fused image or NULL */
+void ora_weighted_average(const float *frames, size_t n, size_t w, size_t h, const ora_depth_params *dp, float power,
+                          size_t smooth, float gate, int edge, const float *floor, const float *guide3, float *out3);
+/* stack.rs:76-92; returns the number of slabs, writes up to cap (lo, hi) pairs */
+size_t ora_slab_ranges(size_t count, size_t size, size_t overlap, size_t *out2, size_t cap);
+
 /* ---- stack.rs: the per-frame loop (fuse_range + AlignedFrames + fuse_and_depth) ---- */
 typedef struct ora_stack_params {
     uint32_t align;         /* 0: frames are registered already (LazyFrames path) */

@@ -180,6 +180,10 @@ _sig("ora_wls_solve", C.c_uint32, _p, _p, C.c_uint32, C.c_float, _p, C.POINTER(C
 _sig("ora_wls_vcycle", None, _p, _p, _p)
 _sig("ora_wls_apply", None, _p, _p, _p)
 _sig("ora_wls_free", None, _p)
+_sig("ora_wav_weight", C.c_float, C.c_float, C.c_float, C.c_float, C.c_float)
+_sig("ora_weighted_average", None, _p, _sz, _sz, _sz, C.POINTER(DepthParams), C.c_float, _sz, C.c_float, C.c_int, _p, _p,
+     _p)
+_sig("ora_slab_ranges", _sz, _sz, _sz, _sz, _p, _sz)
 _sig("ora_depth_from_slices", C.c_int, _p, _sz, _p, _sz, _sz, C.POINTER(DepthParams), _p, _p, _p,
      C.POINTER(DepthOut))
 _sig("ora_stack_run", C.c_int, _p, _sz, _sz, _sz, C.POINTER(FuseParams), C.POINTER(StackParams),
@@ -672,6 +676,36 @@ def depth_from_slices(slices, fused3, params=None, **kw):
     if rc != 0:
         raise ValueError("depth: no slices")
     return depth, conf, floor, info
+
+
+# -------------------------------------------------------------- wav.rs, slabs
+def wav_weight(c, f, gate, power):
+    """wav.rs:88-91 weight(), one cell."""
+    return float(_L.ora_wav_weight(float(c), float(f), float(gate), float(power)))
+
+
+def weighted_average(frames, floor, guide=None, power=2.0, smooth=3, gate=0.5, edge=True, params=None, **kw):
+    """wav.rs:97-161 weighted_average over aligned frames ((3, h, w) each), the
+    depth pass's grid floor, and the fused image as guide (or None)."""
+    frames = _f32(np.stack(frames))
+    n, _, h, w = frames.shape
+    p = params or DepthParams.make(**kw)
+    k = 1 << p.scale
+    floor = _f32(floor)
+    assert floor.size == (-(-h // k)) * (-(-w // k)), "floor: not the working grid"
+    g = _f32(guide) if guide is not None else None
+    out = np.zeros((3, h, w), np.float32)
+    _L.ora_weighted_average(_ptr(frames), n, w, h, C.byref(p), float(power), int(smooth), float(gate), int(bool(edge)),
+                            _ptr(floor), _ptr(g) if g is not None else None, _ptr(out))
+    return out
+
+
+def slab_ranges(count, size, overlap):
+    """stack.rs:76-92: the (lo, hi) frame ranges of the slabs."""
+    cap = max(1, count)
+    out = np.zeros(2 * cap, np.uintp)
+    ns = _L.ora_slab_ranges(count, size, overlap, out.ctypes.data_as(C.c_void_p), cap)
+    return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(min(ns, cap))]
 
 
 # -------------------------------------------------------------- io / stack
