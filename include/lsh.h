@@ -292,6 +292,24 @@ void lsh_wav_free(lsh_wav *a);
  * (nullable).  Each slab is fused on its own (a fuser per slab, stack.rs:873-893). */
 uint32_t lsh_slab_ranges(uint32_t count, uint32_t size, uint32_t overlap, uint32_t *lohi, uint32_t cap);
 
+/* ---- the output side --------------------------------------------------- */
+
+/* align.rs:344-403 common_area: the largest pixel-aligned rectangle (x, y, w,
+ * h into rect4) that every frame covers with its kernel's support inside the
+ * frame, from the n Sims (n*8) of a w x h stack; the whole frame when nothing
+ * is cut or nothing is common.  Host arithmetic only. */
+char *lsh_common_area(const double *sims, uint32_t n, uint32_t w, uint32_t h, uint32_t interp, uint32_t *rect4);
+/* Img3::crop: the rect4 (x, y, w, h) of each of planes w x h planes, packed. */
+char *lsh_crop(const float *src, lsh_mem src_mem, uint32_t planes, uint32_t w, uint32_t h, const uint32_t *rect4,
+               float *dst, lsh_mem dst_mem);
+/* prep.rs:46-71 resize / resize_plane (the restretch of stack.rs:951-974):
+ * planes pw x ph planes resampled to w x h by the warp with the Sim that maps
+ * pixel centers onto pixel centers, in interpolation interp. */
+char *lsh_resize(const float *src, lsh_mem src_mem, uint32_t planes, uint32_t pw, uint32_t ph, uint32_t w, uint32_t h,
+                 uint32_t interp, float *dst, lsh_mem dst_mem);
+/* io.rs:283-288: 3 planes w x h to interleaved RGB u16, (clamp(v, 0, 1) * 65535 + 0.5) truncated. */
+char *lsh_to_u16(const float *img3, lsh_mem img_mem, uint32_t w, uint32_t h, uint16_t *rgb, lsh_mem rgb_mem);
+
 /* ---- the WLS solve alone, in three precisions ------------------------- */
 
 typedef enum lsh_wls_mode {

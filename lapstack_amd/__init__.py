@@ -39,7 +39,8 @@ from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
            "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
-           "weighted_average", "slab_ranges", "fuse_slabs", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "weighted_average", "slab_ranges", "fuse_slabs",
+           "common_area", "crop", "resize", "restretch", "to_u16", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -471,6 +472,77 @@ def fuse_slabs(frames, size: int, overlap: int, fuse: Optional[FuseParams] = Non
         for fu in fusers:
             fu.close()
         fr.close()
+
+
+def common_area(sims, w: int, h: int, interp="spline4x4"):
+    """align.rs:344-403 common_area: (x, y, w, h) of the largest rectangle every frame covers."""
+    lib = _lib.load()
+    s = np.ascontiguousarray(np.asarray(sims, np.float64).reshape(-1, 8))
+    rect = (C.c_uint32 * 4)()
+    _lib.check(lib.lsh_common_area(s.ctypes.data_as(C.c_void_p) if len(s) else None, len(s), w, h,
+                                   Interp.parse(interp), rect))
+    return tuple(int(v) for v in rect)
+
+
+def _planes_of(img):
+    shape = tuple(img.shape)
+    return (1,) + shape if len(shape) == 2 else shape
+
+
+def _like(img, shape):
+    import torch
+
+    if isinstance(img, torch.Tensor) and img.is_cuda:
+        return torch.empty(shape, dtype=torch.float32, device=img.device)
+    return np.zeros(shape, np.float32)
+
+
+def crop(img, rect, out=None):
+    """Img3::crop: the rect (x, y, w, h) of an image ((c, h, w) or (h, w)), host or device."""
+    c, h, w = _planes_of(img)
+    rx, ry, rw, rh = (int(v) for v in rect)
+    shape = (c, rh, rw) if len(tuple(img.shape)) == 3 else (rh, rw)
+    if out is None:
+        out = _like(img, shape)
+    sp, smem, sk = _buf(img, np.float32)
+    dp, dmem, dk = _buf(out, np.float32)
+    r4 = (C.c_uint32 * 4)(rx, ry, rw, rh)
+    _lib.check(_lib.load().lsh_crop(sp, smem, c, w, h, r4, dp, dmem))
+    del sk, dk
+    return out
+
+
+def resize(img, w: int, h: int, interp="spline4x4", out=None):
+    """prep.rs:46-71 resize / resize_plane on the device: an image ((c, ph, pw) or (ph, pw)) resampled to w x h."""
+    c, ph, pw = _planes_of(img)
+    shape = (c, h, w) if len(tuple(img.shape)) == 3 else (h, w)
+    if out is None:
+        out = _like(img, shape)
+    sp, smem, sk = _buf(img, np.float32)
+    dp, dmem, dk = _buf(out, np.float32)
+    _lib.check(_lib.load().lsh_resize(sp, smem, c, pw, ph, w, h, Interp.parse(interp), dp, dmem))
+    del sk, dk
+    return out
+
+
+def restretch(image, rect, w: int, h: int, interp="spline4x4"):
+    """stack.rs:951-974 restretched, for one image cropped to rect: back to the frame's w x h (each axis on its
+    own), or unchanged when the crop kept the whole frame."""
+    if rect is None or (int(rect[2]), int(rect[3])) == (w, h):
+        return image
+    return resize(image, w, h, interp)
+
+
+def to_u16(img3, out=None):
+    """io.rs:283-288: (3, h, w) planes to interleaved RGB u16 (h, w, 3)."""
+    _, h, w = tuple(img3.shape)
+    if out is None:
+        out = np.zeros((h, w, 3), np.uint16)
+    sp, smem, sk = _buf(img3, np.float32)
+    dp, dmem, dk = _buf(out, np.uint16)
+    _lib.check(_lib.load().lsh_to_u16(sp, smem, w, h, dp, dmem))
+    del sk, dk
+    return out
 
 
 def depth_grid(slices, n: int, fused, params: Optional[DepthParams] = None, out_device: bool = False):

@@ -108,6 +108,10 @@ SIGNATURES = [
     ("lsh_wav_finish", _p, [_p, _p, _u32]),
     ("lsh_wav_free", None, [_p]),
     ("lsh_slab_ranges", _u32, [_u32, _u32, _u32, _p, _u32]),
+    ("lsh_common_area", _p, [_p, _u32, _u32, _u32, _u32, _p]),
+    ("lsh_crop", _p, [_p, _u32, _u32, _u32, _u32, _p, _p, _u32]),
+    ("lsh_resize", _p, [_p, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _p, _u32]),
+    ("lsh_to_u16", _p, [_p, _u32, _u32, _u32, _p, _u32]),
     ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
                            _p, _p, C.c_int, C.POINTER(WlsReportC)]),
 ]

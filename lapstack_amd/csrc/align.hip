@@ -13,7 +13,10 @@
 // wavefront shuffles and across blocks, after one grid barrier, in a fixed
 // order -- no host round trip per evaluation.
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstdlib>
+#include <vector>
 
 #include "api.hpp"
 #include "common.hpp"
@@ -1498,6 +1501,248 @@ char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, u
         if (ticks) ticks[NM_MAXL + q] = f->phase_ticks[q];
     for (uint32_t l = 0; l < NM_MAXL && NM_MAXL + l < cap; l++)
         if (evals) evals[NM_MAXL + l] = f->lev_passes[l];
+    LSH_API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- the output side
+// What happens to the result after the path: the crop to the area every frame
+// covers (align.rs:344-403 common_area, Img3::crop), the restretch back to
+// the frame's size (stack.rs:951-974 through prep.rs:46-71 resize), and the
+// 16-bit quantisation of the writers (io.rs:283-288).
+namespace lsh {
+
+// prep.rs:63-71 resize_plane: the warp of align.rs:489-559 (edge-clamped taps)
+// onto the ow x oh grid with the Sim that scales the source over the output
+// about its center, pixel centers onto pixel centers; np planes
+template <int N>
+__global__ __launch_bounds__(256) void k_resize(const float *__restrict__ src, int pw, int ph, Inv9 inv, int affine,
+                                                int ik, int np, float *__restrict__ dst, int ow, int oh) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= ow || y >= oh) return;
+    double sx, sy;
+    src_point(inv, affine, x, y, sx, sy);
+    const double fx = floor(sx), fy = floor(sy);
+    const int x0 = (int)fmax(fmin(fx, 1e9), -1e9), y0 = (int)fmax(fmin(fy, 1e9), -1e9);
+    float wx[6], wy[6];
+    interp_w(ik, sx - fx, wx);
+    interp_w(ik, sy - fy, wy);
+    const size_t pn = (size_t)pw * ph, on = (size_t)ow * oh, o = (size_t)y * ow + x;
+    for (int c = 0; c < np; c++) dst[c * on + o] = warp_sample<N>(nullptr, src + c * pn, -1, pw, ph, x0, y0, wx, wy);
+}
+
+// Img3::crop of np planes: the rect (x, y, w, h) of each
+__global__ void k_crop(const float *__restrict__ src, int w, int h, int np, int rx, int ry, int rw, int rh,
+                       float *__restrict__ dst) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= rw || y >= rh) return;
+    const size_t n = (size_t)w * h, rn = (size_t)rw * rh;
+    for (int c = 0; c < np; c++) dst[c * rn + (size_t)y * rw + x] = src[c * n + (size_t)(ry + y) * w + rx + x];
+}
+
+// io.rs:283-288: (v.clamp(0, 1) * 65535 + 0.5) as u16, interleaved RGB
+__global__ void k_to_u16(const float *__restrict__ img3, size_t n, uint16_t *__restrict__ rgb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 3; c++) {
+        const float v = rclampf(img3[c * n + i], 0.0f, 1.0f) * 65535.0f + 0.5f;
+        rgb[3 * i + c] = (uint16_t)fminf(v, 65535.0f);
+    }
+}
+
+// align.rs:178-192 Interp::taps / margin
+static double interp_margin(uint32_t ik) {
+    static const int taps[6] = {1, 2, 4, 4, 6, 6};
+    return (double)(taps[ik] / 2);
+}
+
+// align.rs:344-403 common_area, on the host: per row the columns every
+// (non-identity) frame's source point stays `margin` inside the frame for
+// (each bound of an affine or projective row is linear in the column), then
+// the largest rectangle over consecutive rows; the whole frame when nothing
+// is cut or nothing is common.
+static void common_area_host(const double *sims, size_t n, size_t w, size_t h, uint32_t ik, uint32_t rect[4]) {
+    rect[0] = 0;
+    rect[1] = 0;
+    rect[2] = (uint32_t)w;
+    rect[3] = (uint32_t)h;
+    if (w == 0 || h == 0) return;
+    std::vector<std::array<double, 9>> invs;
+    for (size_t i = 0; i < n; i++) {
+        Sim s;
+        std::copy(sims + 8 * i, sims + 8 * i + 8, s.v);
+        if (s.is_id()) continue;
+        double m[9];
+        std::array<double, 9> iv;
+        sim_matrix(s, w, h, m);
+        inverse9(m, iv.data());
+        invs.push_back(iv);
+    }
+    if (invs.empty()) return;
+    const double mg = interp_margin(ik), xmax = (double)(w - 1) - mg, ymax = (double)(h - 1) - mg;
+    std::vector<std::pair<int64_t, int64_t>> rows(h);
+    for (size_t y = 0; y < h; y++) {
+        double lo = 0.0, hi = (double)(w - 1);
+        const double yf = (double)y;
+        for (const auto &v : invs) {
+            const double g = v[6], d = v[7] * yf + v[8];
+            if (d <= 0.0 || g * (double)(w - 1) + d <= 0.0) {
+                lo = 1.0;
+                hi = 0.0;
+                break;
+            }
+            const double cf[2][4] = {{v[0], v[1] * yf + v[2], mg, xmax}, {v[3], v[4] * yf + v[5], mg, ymax}};
+            for (const auto &r : cf) {
+                const double a = r[0], k = r[1], smin = r[2], smax = r[3];
+                const double pq[2][2] = {{a - smin * g, k - smin * d}, {smax * g - a, smax * d - k}};
+                for (const auto &e : pq) {
+                    const double p = e[0], q = e[1];
+                    if (std::fabs(p) < 1e-12) {
+                        if (q < 0.0) {
+                            lo = 1.0;
+                            hi = 0.0;
+                        }
+                    } else if (p > 0.0) {
+                        lo = std::max(lo, -q / p);
+                    } else {
+                        hi = std::min(hi, -q / p);
+                    }
+                }
+            }
+        }
+        rows[y] = lo <= hi ? std::make_pair((int64_t)std::ceil(lo), (int64_t)std::floor(hi)) : std::make_pair((int64_t)1, (int64_t)0);
+    }
+    size_t bx = 0, by = 0, bw = 0, bh = 0;
+    for (size_t y0 = 0; y0 < h; y0++) {
+        int64_t lo = rows[y0].first, hi = rows[y0].second;
+        for (size_t y1 = y0; y1 < h; y1++) {
+            lo = std::max(lo, rows[y1].first);
+            hi = std::min(hi, rows[y1].second);
+            if (lo > hi) break;
+            const size_t rw = (size_t)(hi - lo + 1), rh = y1 - y0 + 1;
+            if (rw * rh > bw * bh) {
+                bx = (size_t)lo;
+                by = y0;
+                bw = rw;
+                bh = rh;
+            }
+        }
+    }
+    if (bw * bh == 0) return;
+    rect[0] = (uint32_t)bx;
+    rect[1] = (uint32_t)by;
+    rect[2] = (uint32_t)bw;
+    rect[3] = (uint32_t)bh;
+}
+
+}  // namespace lsh
+
+extern "C" {
+
+char *lsh_common_area(const double *sims, uint32_t n, uint32_t w, uint32_t h, uint32_t interp, uint32_t *rect4) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(rect4 && (sims || n == 0), "lsh_common_area: null argument");
+    LSH_REQUIRE(interp <= 5, "common area: unknown interpolation");
+    common_area_host(sims, n, w, h, interp, rect4);
+    LSH_API_END
+}
+
+char *lsh_crop(const float *src, lsh_mem src_mem, uint32_t planes, uint32_t w, uint32_t h, const uint32_t *rect4,
+               float *dst, lsh_mem dst_mem) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(src && dst && rect4, "lsh_crop: null argument");
+    const uint32_t rx = rect4[0], ry = rect4[1], rw = rect4[2], rh = rect4[3];
+    LSH_REQUIRE(rw > 0 && rh > 0 && (uint64_t)rx + rw <= w && (uint64_t)ry + rh <= h, "crop: the rectangle leaves the image");
+    const size_t n = (size_t)w * h, rn = (size_t)rw * rh;
+    DBuf<float> s, d;
+    const float *sp = src;
+    if (src_mem == LSH_HOST) {
+        s.alloc(planes * n);
+        copy_mem(s.p, LSH_DEVICE, src, LSH_HOST, planes * n * sizeof(float));
+        sp = s.p;
+    }
+    float *dp = dst;
+    if (dst_mem == LSH_HOST) {
+        d.alloc(planes * rn);
+        dp = d.p;
+    }
+    launch_b("crop", 8.0 * planes * rn, k_crop, dim3(cdiv(rw, 64), cdiv(rh, 4)), dim3(64, 4), 0, sp, (int)w, (int)h,
+             (int)planes, (int)rx, (int)ry, (int)rw, (int)rh, dp);
+    if (dst_mem == LSH_HOST) copy_mem(dst, LSH_HOST, d.p, LSH_DEVICE, planes * rn * sizeof(float));
+    sync();
+    LSH_API_END
+}
+
+char *lsh_resize(const float *src, lsh_mem src_mem, uint32_t planes, uint32_t pw, uint32_t ph, uint32_t w, uint32_t h,
+                 uint32_t interp, float *dst, lsh_mem dst_mem) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(src && dst, "lsh_resize: null argument");
+    LSH_REQUIRE(pw > 0 && ph > 0 && w > 0 && h > 0, "resize: empty image");
+    LSH_REQUIRE(interp <= 5, "resize: unknown interpolation");
+    const size_t pn = (size_t)pw * ph, on = (size_t)w * h;
+    if (pw == w && ph == h) {  // prep.rs:47-49, 64-66: the same size is a copy
+        copy_mem(dst, dst_mem, src, src_mem, planes * pn * sizeof(float));
+        sync();
+        return nullptr;
+    }
+    // prep.rs:67-69: scale kx, ky and the offset that puts output pixel o on source (o + 1/2) / k - 1/2
+    const double sw = pw, sh = ph, kx = (double)w / sw, ky = (double)h / sh;
+    Sim s;
+    s.v[0] = (kx - 1.0) * (sw + 1.0) / (2.0 * sw);
+    s.v[1] = (ky - 1.0) * (sh + 1.0) / (2.0 * sh);
+    s.v[2] = kx;
+    s.v[4] = ky / kx;
+    double m[9];
+    Inv9 inv;
+    sim_matrix(s, pw, ph, m);
+    inverse9(m, inv.m);
+    const int affine = inv.m[6] == 0.0 && inv.m[7] == 0.0 && inv.m[8] == 1.0;
+    DBuf<float> sb, db;
+    const float *sp = src;
+    if (src_mem == LSH_HOST) {
+        sb.alloc(planes * pn);
+        copy_mem(sb.p, LSH_DEVICE, src, LSH_HOST, planes * pn * sizeof(float));
+        sp = sb.p;
+    }
+    float *dp = dst;
+    if (dst_mem == LSH_HOST) {
+        db.alloc(planes * on);
+        dp = db.p;
+    }
+    const dim3 g(cdiv(w, 64), cdiv(h, 4)), b(64, 4);
+    const double by = 4.0 * planes * ((double)on + (double)pn);
+    const int ik = (int)interp;
+    switch (taps_n(interp)) {
+    case 2: launch_b("resize", by, k_resize<2>, g, b, 0, sp, (int)pw, (int)ph, inv, affine, ik, (int)planes, dp, (int)w, (int)h); break;
+    case 4: launch_b("resize", by, k_resize<4>, g, b, 0, sp, (int)pw, (int)ph, inv, affine, ik, (int)planes, dp, (int)w, (int)h); break;
+    default: launch_b("resize", by, k_resize<6>, g, b, 0, sp, (int)pw, (int)ph, inv, affine, ik, (int)planes, dp, (int)w, (int)h); break;
+    }
+    if (dst_mem == LSH_HOST) copy_mem(dst, LSH_HOST, db.p, LSH_DEVICE, planes * on * sizeof(float));
+    sync();
+    LSH_API_END
+}
+
+char *lsh_to_u16(const float *img3, lsh_mem img_mem, uint32_t w, uint32_t h, uint16_t *rgb, lsh_mem rgb_mem) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(img3 && rgb, "lsh_to_u16: null argument");
+    const size_t n = (size_t)w * h;
+    DBuf<float> sb;
+    DBuf<uint16_t> db;
+    const float *sp = img3;
+    if (img_mem == LSH_HOST) {
+        sb.alloc(3 * n);
+        copy_mem(sb.p, LSH_DEVICE, img3, LSH_HOST, 3 * n * sizeof(float));
+        sp = sb.p;
+    }
+    uint16_t *dp = rgb;
+    if (rgb_mem == LSH_HOST) {
+        db.alloc(3 * n);
+        dp = db.p;
+    }
+    launch_b("to_u16", 18.0 * n, k_to_u16, dim3(cdiv(n, 256)), dim3(256), 0, sp, n, dp);
+    if (rgb_mem == LSH_HOST) copy_mem(rgb, LSH_HOST, db.p, LSH_DEVICE, 3 * n * sizeof(uint16_t));
+    sync();
     LSH_API_END
 }
 

@@ -389,6 +389,19 @@ def warp_plane(src, sim, interp="spline4x4", ow=None, oh=None):
     return out, valid
 
 
+def resize_plane(p, w, h, interp="spline4x4"):
+    """prep.rs:63-71 resize_plane: the plane warped onto w x h with the Sim that
+    scales it over the output about its center, pixel centers onto pixel centers."""
+    p = _f32(p)
+    ph, pw = p.shape
+    if (pw, ph) == (w, h):
+        return p.copy()
+    sw, sh = float(pw), float(ph)
+    kx, ky = w / sw, h / sh
+    sim = [(kx - 1.0) * (sw + 1.0) / (2.0 * sw), (ky - 1.0) * (sh + 1.0) / (2.0 * sh), kx, 0.0, ky / kx, 0.0, 0.0, 0.0]
+    return warp_plane(p, sim, interp, w, h)[0]
+
+
 def warp_cost(rf, tgt, sim):
     rf, tgt, sim = _f32(rf), _f32(tgt), _f64(sim)
     th, tw = tgt.shape
