@@ -142,18 +142,18 @@ __device__ void grid_sum(double (&v)[NV], double *__restrict__ part, unsigned *_
 // ------------------------------------------------------- the search on device
 // multiscale_align (align.rs:896-942) as one cooperative kernel: every block
 // runs the same bounded Nelder-Mead (align.rs:796-883) on the same numbers.
-// Each cost evaluation is a grid-strided warp-and-reduce; the last block to
-// arrive folds all blocks' partial sums in a fixed order and broadcasts the
-// cost, so all blocks hold the identical value and take the identical next
-// step, with no host round trip.
+// Each cost evaluation is a grid-strided warp-and-reduce; once the arrival
+// tree says every block's partial sums are written, every block folds them in
+// the same fixed order, so all blocks hold the identical value and take the
+// identical next step, with no host round trip and no broadcast.
 #define NM_MAXL 16
 // the search's flag words, each on its own 128-byte line: arrivals of groups,
-// the error flag, the broadcast generation, then one arrival counter per group
+// the error flag, (a spare line), the hand-off generation, then one arrival
+// counter per group
 #define NM_GROUPS 16
 #define NM_LINE 32
 #define NM_FL_TOP 0
 #define NM_FL_ERR (1 * NM_LINE)
-#define NM_FL_BCAST (2 * NM_LINE)
 #define NM_FL_HAND (3 * NM_LINE)
 #define NM_FL_GROUP (4 * NM_LINE)
 #define NM_FL_WORDS ((4 + NM_GROUPS) * NM_LINE)
@@ -218,7 +218,7 @@ struct NmShared {
     unsigned lev_evals[NM_MAXL], lev_passes[NM_MAXL];
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
-    int stop, folds;
+    int stop;
 };
 
 // align.rs:630-634
@@ -409,26 +409,35 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
         for (int k = 0; k < nv; k++) pp[(size_t)blockIdx.x * 3 * NM_MAXP + k] = sh.sums[k];
         __threadfence();
         // two-level arrival: NM_GROUPS counters (one cache line each), then one
-        // counter of completed groups; the block completing the last group folds
+        // counter of completed groups, which every block waits on (bounded, and
+        // left at once when any block has raised the error flag)
         const unsigned gs = gridDim.x / NM_GROUPS;
         unsigned a = atomicAdd(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u);
-        int folds = 0;
         if (a == (gen + 1) * gs - 1) {
             __threadfence();
-            unsigned a2 = atomicAdd(&ctr[NM_FL_TOP], 1u);
-            folds = a2 == (gen + 1) * NM_GROUPS - 1;
+            atomicAdd(&ctr[NM_FL_TOP], 1u);
         }
-        sh.folds = folds;
+        const unsigned target = (gen + 1) * NM_GROUPS;
+        unsigned spins = 0;
+        while (__hip_atomic_load(&ctr[NM_FL_TOP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 24)) {
+                atomicOr(err, 1u);
+                break;
+            }
+        }
+        __threadfence();
     }
     __syncthreads();
-    unsigned *bflag = ctr + NM_FL_BCAST;
-    double *bcost = part + (size_t)2 * gridDim.x * 3 * NM_MAXP + (size_t)slot * NM_MAXP;
-    if (sh.folds) {
-        // The fold order depends on thread indices only, not on which block
-        // folds.  All partials are loaded at once (up to 4 independent loads
-        // per thread) into LDS (the window buffer: the pixels are done), then
-        // wave w sums value k = w, w + 16, ... over the blocks, lanes in block order.
-        __threadfence();
+    {
+        // Every block folds all blocks' partial sums itself, in an order that
+        // depends on thread indices only, so all hold the identical costs with
+        // no broadcast.  All partials are loaded at once (up to 4 independent
+        // loads per thread) into LDS (the window buffer: the pixels are done),
+        // then wave w sums value k = w, w + 16, ... over the blocks, lanes in
+        // block order.  A block a pass ahead writes the other slot; none can be
+        // two ahead before every block has arrived at the pass between.
         double *stg = (double *)boxs;
         const int G = (int)gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         const int kchunk = min(nv, (NM_BOX / 2) / G);
@@ -460,33 +469,7 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
         }
         if ((int)threadIdx.x < np) {
             const int q = threadIdx.x;
-            const double c = rms_of(sh.sums[3 * q], sh.sums[3 * q + 1], sh.sums[3 * q + 2]);
-            __hip_atomic_store(&bcost[q], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            sh.PC[q] = c;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __threadfence();
-            __hip_atomic_store(bflag, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    } else {
-        // wait for the broadcast; bounded, and left at once when any block has raised the error flag
-        if (threadIdx.x == 0) {
-            unsigned spins = 0;
-            while (__hip_atomic_load(bflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen + 1) {
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1u << 24)) {
-                    atomicOr(err, 1u);
-                    break;
-                }
-            }
-            __threadfence();
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < np) {  // the costs share one line: one request
-            __threadfence();
-            sh.PC[threadIdx.x] = __hip_atomic_load(&bcost[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sh.PC[q] = rms_of(sh.sums[3 * q], sh.sums[3 * q + 1], sh.sums[3 * q + 2]);
         }
     }
     gen++;
