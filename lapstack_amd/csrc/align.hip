@@ -407,16 +407,17 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
     double *pp = part + (size_t)slot * gridDim.x * 3 * NM_MAXP;
     if (threadIdx.x == 0) {
         for (int k = 0; k < nv; k++) pp[(size_t)blockIdx.x * 3 * NM_MAXP + k] = sh.sums[k];
-        __threadfence();
         // two-level arrival: NM_GROUPS counters (one cache line each), then one
         // counter of completed groups, which every block waits on (bounded, and
-        // left at once when any block has raised the error flag)
+        // left at once when any block has raised the error flag).  The partials
+        // are released by the group increment; the group's last arriver
+        // acquires its group's and releases them on to the top counter; every
+        // block acquires them all after its wait.
         const unsigned gs = gridDim.x / NM_GROUPS;
-        unsigned a = atomicAdd(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u);
-        if (a == (gen + 1) * gs - 1) {
-            __threadfence();
-            atomicAdd(&ctr[NM_FL_TOP], 1u);
-        }
+        unsigned a = __hip_atomic_fetch_add(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u,
+                                            __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (a == (gen + 1) * gs - 1)
+            __hip_atomic_fetch_add(&ctr[NM_FL_TOP], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned target = (gen + 1) * NM_GROUPS;
         unsigned spins = 0;
         while (__hip_atomic_load(&ctr[NM_FL_TOP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -427,7 +428,7 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     {

@@ -117,13 +117,14 @@ __device__ void b_join(unsigned *ctl, unsigned *sh) {
     __syncthreads();
 }
 
-// a grid barrier over the nb participants: one monotonic counter, bounded waits that raise the error
+// a grid barrier over the nb participants: one monotonic counter (the block's
+// writes released by its increment, everyone's acquired after the wait),
+// bounded waits that raise the error
 __device__ void b_barrier(unsigned *ctl, unsigned nb, unsigned &gen) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
         const unsigned target = (gen + 1) * nb;
-        atomicAdd(&ctl[B_BAR], 1u);
+        __hip_atomic_fetch_add(&ctl[B_BAR], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
         while (__hip_atomic_load(&ctl[B_BAR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(&ctl[B_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
@@ -133,7 +134,7 @@ __device__ void b_barrier(unsigned *ctl, unsigned nb, unsigned &gen) {
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     gen++;
     __syncthreads();
