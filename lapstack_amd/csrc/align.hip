@@ -449,9 +449,9 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int i = i0 + u * NM_THREADS + threadIdx.x;
-                    v[u] = i < total ? __hip_atomic_load(&pp[(size_t)(i / kn) * 3 * NM_MAXP + k0 + i % kn],
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : 0.0;
+                    // plain loads: the acquire after the wait ordered every block's partials before
+                    // them (and invalidated stale lines), so a block's L2 serves its XCD's other blocks
+                    v[u] = i < total ? pp[(size_t)(i / kn) * 3 * NM_MAXP + k0 + i % kn] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
