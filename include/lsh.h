@@ -292,6 +292,15 @@ void lsh_wav_free(lsh_wav *a);
  * (nullable).  Each slab is fused on its own (a fuser per slab, stack.rs:873-893). */
 uint32_t lsh_slab_ranges(uint32_t count, uint32_t size, uint32_t overlap, uint32_t *lohi, uint32_t cap);
 
+/* align.rs:741-754 gauss_levels: the registration pyramid's levels above the
+ * plane y (w x h) -- each half the size (rounded up) of the one below while
+ * h > 64 and w > 8, by the border-renormalised Burt REDUCE (align.rs:650-734)
+ * -- packed level after level into packed (out_mem); *nlevels gets their
+ * count (packed NULL: the count only).  The frames handle builds the same
+ * pyramid for every frame's search. */
+char *lsh_gauss_levels(const float *y, lsh_mem y_mem, uint32_t w, uint32_t h, float *packed, lsh_mem out_mem,
+                       uint32_t *nlevels);
+
 /* ---- the output side --------------------------------------------------- */
 
 /* align.rs:344-403 common_area: the largest pixel-aligned rectangle (x, y, w,

@@ -40,7 +40,7 @@ from ._lib import LSH_DEVICE, LSH_HOST, HipError
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
            "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
            "weighted_average", "slab_ranges", "fuse_slabs",
-           "common_area", "crop", "resize", "restretch", "to_u16", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "common_area", "crop", "resize", "restretch", "to_u16", "gauss_levels", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -472,6 +472,29 @@ def fuse_slabs(frames, size: int, overlap: int, fuse: Optional[FuseParams] = Non
         for fu in fusers:
             fu.close()
         fr.close()
+
+
+def gauss_levels(y):
+    """align.rs:741-754 gauss_levels on the device: the registration pyramid's
+    levels above the plane y ((h, w) float32, host), as a list of numpy arrays."""
+    lib = _lib.load()
+    y = np.ascontiguousarray(y, np.float32)
+    h, w = y.shape
+    n = C.c_uint32()
+    _lib.check(lib.lsh_gauss_levels(y.ctypes.data_as(C.c_void_p), LSH_HOST, w, h, None, LSH_HOST, C.byref(n)))
+    dims, cw, ch = [], w, h
+    for _ in range(n.value):
+        cw, ch = (cw + 1) // 2, (ch + 1) // 2
+        dims.append((ch, cw))
+    packed = np.zeros(max(1, sum(a * b for a, b in dims)), np.float32)
+    if dims:
+        _lib.check(lib.lsh_gauss_levels(y.ctypes.data_as(C.c_void_p), LSH_HOST, w, h,
+                                        packed.ctypes.data_as(C.c_void_p), LSH_HOST, C.byref(n)))
+    out, off = [], 0
+    for a, b in dims:
+        out.append(packed[off:off + a * b].reshape(a, b))
+        off += a * b
+    return out
 
 
 def common_area(sims, w: int, h: int, interp="spline4x4"):

@@ -108,6 +108,7 @@ SIGNATURES = [
     ("lsh_wav_finish", _p, [_p, _p, _u32]),
     ("lsh_wav_free", None, [_p]),
     ("lsh_slab_ranges", _u32, [_u32, _u32, _u32, _p, _u32]),
+    ("lsh_gauss_levels", _p, [_p, _u32, _u32, _u32, _p, _u32, C.POINTER(_u32)]),
     ("lsh_common_area", _p, [_p, _u32, _u32, _u32, _u32, _p]),
     ("lsh_crop", _p, [_p, _u32, _u32, _u32, _u32, _p, _p, _u32]),
     ("lsh_resize", _p, [_p, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _p, _u32]),

@@ -46,33 +46,73 @@ __device__ __forceinline__ void burt_k(float *k) {
     k[4] = c;
 }
 
-__global__ __launch_bounds__(256) void k_burt(const float *__restrict__ src, int w, int h, float *__restrict__ dst,
-                                              int ow, int oh) {
-    int oj = blockIdx.x * blockDim.x + threadIdx.x, oi = blockIdx.y * blockDim.y + threadIdx.y;
-    if (oj >= ow || oi >= oh) return;
+// The 64 x 4 output tile at (ox0, oy0): the 131 x 11 input window staged in
+// LDS (in-range samples only), the horizontal 5-tap sums of its 11 rows at the
+// 64 even columns into LDS, then each of the 256 threads sums its five rows
+// and divides by the in-range weight sums -- per output the operations of the
+// per-output loop of align.rs:683-734, in its order (taps outside the plane are
+// skipped, not zero-filled).  Interior tiles skip the range tests.
+#define BT_TX 64
+#define BT_TY 4
+struct BurtLds {
+    float T[2 * BT_TY + 3][2 * BT_TX + 3 + 1];
+    float Hs[2 * BT_TY + 3][BT_TX + 1];
+};
+
+template <bool EDGE>
+__device__ __forceinline__ void burt_body(const float *__restrict__ src, int w, int h, float *__restrict__ dst, int ow,
+                                          int oh, int ox0, int oy0, BurtLds &S) {
+    constexpr int IW = 2 * BT_TX + 3, IH = 2 * BT_TY + 3;
     float k[5];
     burt_k(k);
-    int cx = 2 * oj, cy = 2 * oi;
+    const int tid = threadIdx.x, ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+    for (int q = tid; q < IH * IW; q += 256) {
+        const int r = q / IW, c = q - r * IW, gy = iy0 + r, gx = ix0 + c;
+        if (!EDGE || (gy >= 0 && gy < h && gx >= 0 && gx < w)) S.T[r][c] = src[(size_t)gy * w + gx];
+    }
+    __syncthreads();
+    for (int q = tid; q < IH * BT_TX; q += 256) {
+        const int r = q / BT_TX, jj = q - r * BT_TX, gy = iy0 + r;
+        if (EDGE && (gy < 0 || gy >= h || ox0 + jj >= ow)) continue;
+        const int cx = 2 * (ox0 + jj);
+        float a = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 5; u++) {
+            const int i = cx + u - 2;
+            if (!EDGE || (i >= 0 && i < w)) a += k[u] * S.T[r][2 * jj + u];
+        }
+        S.Hs[r][jj] = a;
+    }
+    __syncthreads();
+    const int tx = tid & (BT_TX - 1), ty = tid / BT_TX, oj = ox0 + tx, oi = oy0 + ty;
+    if (EDGE && (oj >= ow || oi >= oh)) return;
+    const int cx = 2 * oj, cy = 2 * oi;
     float rn = 0.0f, cn = 0.0f;
+#pragma unroll
     for (int t = 0; t < 5; t++) {
-        int yy = cy + t - 2, xx = cx + t - 2;
-        if (yy >= 0 && yy < h) rn += k[t];
-        if (xx >= 0 && xx < w) cn += k[t];
+        const int yy = cy + t - 2, xx = cx + t - 2;
+        if (!EDGE || (yy >= 0 && yy < h)) rn += k[t];
+        if (!EDGE || (xx >= 0 && xx < w)) cn += k[t];
     }
     float acc = 0.0f;
+#pragma unroll
     for (int t = 0; t < 5; t++) {
-        int s = cy + t - 2;
-        if (s >= 0 && s < h) {
-            const float *row = src + (size_t)s * w;
-            float a = 0.0f;
-            for (int u = 0; u < 5; u++) {
-                int i = cx + u - 2;
-                if (i >= 0 && i < w) a += k[u] * row[i];
-            }
-            acc += k[t] * a;
-        }
+        const int yy = cy + t - 2;
+        if (!EDGE || (yy >= 0 && yy < h)) acc += k[t] * S.Hs[2 * ty + t][tx];
     }
     dst[(size_t)oi * ow + oj] = acc / (rn * cn);
+}
+
+__global__ __launch_bounds__(256) void k_burt(const float *__restrict__ src, int w, int h, float *__restrict__ dst,
+                                              int ow, int oh) {
+    __shared__ BurtLds S;
+    const int ox0 = blockIdx.x * BT_TX, oy0 = blockIdx.y * BT_TY;
+    const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * BT_TX <= w - 1 && 2 * oy0 + 2 * BT_TY <= h - 1 &&
+                       ox0 + BT_TX <= ow && oy0 + BT_TY <= oh;
+    if (inner)
+        burt_body<false>(src, w, h, dst, ow, oh, ox0, oy0, S);
+    else
+        burt_body<true>(src, w, h, dst, ow, oh, ox0, oy0, S);
 }
 
 // align.rs:639-646
@@ -1038,13 +1078,14 @@ static Inv9 inv_for(const Sim &s, size_t w, size_t h, int &affine) {
 }
 
 // the registration pyramid above level 0 (align.rs:741-754)
+static void burt_level(const float *src, size_t w, size_t h, float *dst, size_t ow, size_t oh) {
+    launch_b("burt_reduce", 4.0 * (w * h + ow * oh), k_burt, dim3(cdiv(ow, BT_TX), cdiv(oh, BT_TY)), dim3(256), 0, src,
+             (int)w, (int)h, dst, (int)ow, (int)oh);
+}
+
 static void build_reg_pyramid(lsh_frames *f, float *base) {
-    for (size_t l = 1; l < f->lw.size(); l++) {
-        dim3 g(cdiv(f->lw[l], 64), cdiv(f->lh[l], 4));
-        launch_b("burt_reduce", 4.0 * (f->lw[l - 1] * f->lh[l - 1] + f->lw[l] * f->lh[l]), k_burt, g, dim3(64, 4), 0,
-                 (const float *)(base + f->loff[l - 1]), (int)f->lw[l - 1], (int)f->lh[l - 1], base + f->loff[l],
-                 (int)f->lw[l], (int)f->lh[l]);
-    }
+    for (size_t l = 1; l < f->lw.size(); l++)
+        burt_level(base + f->loff[l - 1], f->lw[l - 1], f->lh[l - 1], base + f->loff[l], f->lw[l], f->lh[l]);
 }
 
 // align.rs:220: the search box's half-width around the guess, per parameter
@@ -1623,6 +1664,45 @@ static void common_area_host(const double *sims, size_t n, size_t w, size_t h, u
 }  // namespace lsh
 
 extern "C" {
+
+char *lsh_gauss_levels(const float *y, lsh_mem y_mem, uint32_t w, uint32_t h, float *packed, lsh_mem out_mem,
+                       uint32_t *nlevels) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(y && nlevels, "lsh_gauss_levels: null argument");
+    LSH_REQUIRE(w > 0 && h > 0, "registration pyramid: empty plane");
+    // align.rs:741-754: half size (rounded up) while h > 64 and w > 8
+    std::vector<size_t> lw{w}, lh{h}, off{0};
+    size_t tot = 0, cw = w, ch = h;
+    while (ch > 64 && cw > 8) {
+        cw = (cw + 1) / 2;
+        ch = (ch + 1) / 2;
+        lw.push_back(cw);
+        lh.push_back(ch);
+        off.push_back(tot);
+        tot += cw * ch;
+    }
+    *nlevels = (uint32_t)(lw.size() - 1);
+    if (!packed || tot == 0) return nullptr;
+    DBuf<float> ys, outs;
+    const float *prev = y;
+    if (y_mem == LSH_HOST) {
+        ys.alloc((size_t)w * h);
+        copy_mem(ys.p, LSH_DEVICE, y, LSH_HOST, (size_t)w * h * sizeof(float));
+        prev = ys.p;
+    }
+    float *out = packed;
+    if (out_mem == LSH_HOST) {
+        outs.alloc(tot);
+        out = outs.p;
+    }
+    for (size_t l = 1; l < lw.size(); l++) {
+        burt_level(prev, lw[l - 1], lh[l - 1], out + off[l], lw[l], lh[l]);
+        prev = out + off[l];
+    }
+    if (out_mem == LSH_HOST) copy_mem(packed, LSH_HOST, outs.p, LSH_DEVICE, tot * sizeof(float));
+    sync();
+    LSH_API_END
+}
 
 char *lsh_common_area(const double *sims, uint32_t n, uint32_t w, uint32_t h, uint32_t interp, uint32_t *rect4) {
     LSH_API_BEGIN

@@ -86,3 +86,17 @@ def test_output_chain_of_a_registered_stack():
     want = np.stack([ref.resize_plane(cropped[c], w, h, "spline4x4") for c in range(3)])
     assert back.shape == (3, h, w) and np.array_equal(back, want)
     assert np.array_equal(la.to_u16(back), ref.planes_to_u16(want))
+
+
+@pytest.mark.parametrize("w,h", [(1000, 700), (8256, 5504), (130, 129), (257, 66), (17, 200), (9, 65), (3, 300)])
+def test_registration_pyramid_matches_the_oracle(w, h):
+    """align.rs:741-754 on the device (the pyramid every frame's search runs on), bit for bit."""
+    rng = np.random.default_rng(w * 7 + h)
+    y = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    if w > 100:
+        yy, xx = np.mgrid[0:h, 0:w]
+        y = (y * 0.2 + 0.8 * (0.5 + 0.5 * np.sin(xx / 13.0) * np.cos(yy / 7.0))).astype(np.float32)
+    got, want = la.gauss_levels(y), ref.gauss_levels(y)
+    assert len(got) == len(want)
+    for l, (a, b) in enumerate(zip(got, want)):
+        assert a.shape == b.shape and np.array_equal(a, b), (w, h, l)
