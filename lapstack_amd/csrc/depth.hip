@@ -21,50 +21,67 @@ namespace lsh {
 struct Taps {
     int n, halo, kind, step;
 };
+// The ring-difference taps as the kernel reads them: offsets into the staged
+// luma window (dy * pitch + dx) and weights, in the reference's order
+// (r_out <= 7: at most 15 x 15 taps); passed by value, so every lane reads
+// them through the scalar unit.
+#define MAX_TAPS 225
+struct TapTable {
+    int off[MAX_TAPS];
+    float wt[MAX_TAPS];
+};
 
 // depth.rs:417-495 + :174-197: luma, ring difference filter (or SML), |.|, k x k
 // block mean, one output cell per thread, the luma tile staged in LDS.
-__global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k,
-                                                     const int32_t *__restrict__ tdy, const int32_t *__restrict__ tdx,
-                                                     const float *__restrict__ twt, Taps T, float *__restrict__ out,
-                                                     int dw, int dh) {
+// depth.rs:1097-1100: the focus measure of the luma (depth.rs:160-165) block-
+// averaged on the working grid.  A block covers 64 x (4 CY) grid cells: the
+// luma of its pixels and their halo is staged in LDS once (reflect-101 at the
+// borders), then each thread averages the measure over CY cells of its column.
+__global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k, int CY,
+                                                     const TapTable tab, Taps T, float *__restrict__ out, int dw,
+                                                     int dh) {
     extern __shared__ float Y[];
-    const int TX = 64, TY = 4, R = T.halo;
+    const int TX = 64, TY = 4 * CY, R = T.halo;
     const int PW = TX * k + 2 * R, PH = TY * k + 2 * R;
     const int px0 = blockIdx.x * TX * k - R, py0 = blockIdx.y * TY * k - R;
     const size_t n = (size_t)w * h;
     const int tid = threadIdx.y * TX + threadIdx.x;
-    for (int q = tid; q < PW * PH; q += TX * TY) {
+    for (int q = tid; q < PW * PH; q += 256) {
         int ly = q / PW, lx = q - ly * PW;
         size_t o = (size_t)rf101(py0 + ly, h) * w + rf101(px0 + lx, w);
         Y[q] = 0.299f * planes[o] + 0.587f * planes[n + o] + 0.114f * planes[2 * n + o];  // depth.rs:160-165
     }
     __syncthreads();
-    int ox = blockIdx.x * TX + threadIdx.x, oy = blockIdx.y * TY + threadIdx.y;
-    if (ox >= dw || oy >= dh) return;
-    int x0 = ox * k, y0 = oy * k, x1 = min(x0 + k, w), y1 = min(y0 + k, h);
-    float o = 0.0f;
-    for (int y = y0; y < y1; y++) {
-        float rs = 0.0f;
-        int ly = y - py0;
-        for (int x = x0; x < x1; x++) {
-            int lx = x - px0;
-            float f;
-            if (T.kind == 0) {
-                float a = 0.0f;
-                for (int t = 0; t < T.n; t++) a += twt[t] * Y[(ly + tdy[t]) * PW + lx + tdx[t]];
-                f = fabsf(a);
-            } else {
-                int s = T.step;
-                float c = Y[ly * PW + lx];
-                f = fabsf(2.0f * c - Y[ly * PW + lx - s] - Y[ly * PW + lx + s]) +
-                    fabsf(2.0f * c - Y[(ly - s) * PW + lx] - Y[(ly + s) * PW + lx]);
+    const int ox = blockIdx.x * TX + threadIdx.x;
+    if (ox >= dw) return;
+    for (int c = 0; c < CY; c++) {
+        const int oy = blockIdx.y * TY + threadIdx.y + 4 * c;
+        if (oy >= dh) return;
+        int x0 = ox * k, y0 = oy * k, x1 = min(x0 + k, w), y1 = min(y0 + k, h);
+        float o = 0.0f;
+        for (int y = y0; y < y1; y++) {
+            float rs = 0.0f;
+            int ly = y - py0;
+            for (int x = x0; x < x1; x++) {
+                int lx = x - px0;
+                float f;
+                if (T.kind == 0) {
+                    const float *c0 = Y + ly * PW + lx;
+                    float a = 0.0f;
+                    for (int t = 0; t < T.n; t++) a += tab.wt[t] * c0[tab.off[t]];
+                    f = fabsf(a);
+                } else {
+                    int s = T.step;
+                    float cc = Y[ly * PW + lx];
+                    f = fabsf(2.0f * cc - Y[ly * PW + lx - s] - Y[ly * PW + lx + s]) +
+                        fabsf(2.0f * cc - Y[(ly - s) * PW + lx] - Y[(ly + s) * PW + lx]);
+                }
+                rs += f;
             }
-            rs += f;
+            o += rs;
         }
-        o += rs;
+        out[(size_t)oy * dw + ox] = o / (float)((y1 - y0) * (x1 - x0));
     }
-    out[(size_t)oy * dw + ox] = o / (float)((y1 - y0) * (x1 - x0));
 }
 
 // depth.rs:454-471 with the sparse convolution's order (depth.rs:417-451): rows
@@ -144,11 +161,24 @@ static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBu
     T.step = (int)std::max<uint32_t>(p.sml_step, 1);
     T.halo = p.focus_kind == 0 ? (int)p.r_out : T.step;
     int dw = (int)((w + k - 1) / k), dh = (int)((h + k - 1) / k);
-    size_t shm = (size_t)(64 * k + 2 * T.halo) * (4 * k + 2 * T.halo) * sizeof(float);
-    launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4)),
-             dim3(64, 4), shm, (const float *)planes,
-           (int)w, (int)h, k, (const int32_t *)taps_dy.p, (const int32_t *)taps_dx.p,
-           (const float *)taps_wt.p, T, out, dw, dh);
+    // cells per thread: as many (up to 4) as keep the staged window within 32 KB
+    const int PW = 64 * k + 2 * T.halo;
+    int CY = 4;
+    while (CY > 1 && (size_t)PW * (4 * CY * k + 2 * T.halo) * sizeof(float) > 32768) CY--;
+    size_t shm = (size_t)PW * (4 * CY * k + 2 * T.halo) * sizeof(float);
+    TapTable tab{};
+    if (p.focus_kind == 0) {  // the same taps as uploaded, as offsets into the window of pitch PW
+        std::vector<int32_t> dy, dx;
+        std::vector<float> wt;
+        rdf_taps_ordered(p.r_in, p.r_out, dy, dx, wt);
+        LSH_REQUIRE(wt.size() <= MAX_TAPS && wt.size() == ntaps, "focus slice: tap table");
+        for (size_t t = 0; t < wt.size(); t++) {
+            tab.off[t] = dy[t] * PW + dx[t];
+            tab.wt[t] = wt[t];
+        }
+    }
+    launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)),
+             dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY, tab, T, out, dw, dh);
 }
 
 // ----------------------------------------------------------------- helpers
