@@ -46,30 +46,39 @@ __device__ __forceinline__ void burt_k(float *k) {
     k[4] = c;
 }
 
-// The 64 x 4 output tile at (ox0, oy0): the 131 x 11 input window staged in
-// LDS (in-range samples only), the horizontal 5-tap sums of its 11 rows at the
-// 64 even columns into LDS, then each of the 256 threads sums its five rows
-// and divides by the in-range weight sums -- per output the operations of the
-// per-output loop of align.rs:683-734, in its order (taps outside the plane are
-// skipped, not zero-filled).  Interior tiles skip the range tests.
+// The 64 x TY output tile at (ox0, oy0): the (2 TY + 3) x 131 input window
+// staged in LDS (in-range samples only), the horizontal 5-tap sums of its rows
+// at the 64 even columns into LDS, then each of the 256 threads sums five rows
+// for each of its TY / 4 outputs and divides by the in-range weight sums -- per
+// output the operations of the per-output loop of align.rs:683-734, in its
+// order (taps outside the plane are skipped, not zero-filled).  Interior tiles
+// skip the range tests.
 #define BT_TX 64
-#define BT_TY 4
+#define BT_TY 4  // taller tiles measured no faster on a 45 MP plane
+template <int TY>
 struct BurtLds {
-    float T[2 * BT_TY + 3][2 * BT_TX + 3 + 1];
-    float Hs[2 * BT_TY + 3][BT_TX + 1];
+    float T[2 * TY + 3][2 * BT_TX + 3 + 1];
+    float Hs[2 * TY + 3][BT_TX + 1];
 };
 
-template <bool EDGE>
+template <int TY, bool EDGE>
 __device__ __forceinline__ void burt_body(const float *__restrict__ src, int w, int h, float *__restrict__ dst, int ow,
-                                          int oh, int ox0, int oy0, BurtLds &S) {
-    constexpr int IW = 2 * BT_TX + 3, IH = 2 * BT_TY + 3;
+                                          int oh, int ox0, int oy0, BurtLds<TY> &S) {
+    constexpr int IW = 2 * BT_TX + 3, IH = 2 * TY + 3;
     float k[5];
     burt_k(k);
     const int tid = threadIdx.x, ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
-    for (int q = tid; q < IH * IW; q += 256) {
-        const int r = q / IW, c = q - r * IW, gy = iy0 + r, gx = ix0 + c;
-        if (!EDGE || (gy >= 0 && gy < h && gx >= 0 && gx < w)) S.T[r][c] = src[(size_t)gy * w + gx];
-    }
+    lds_stage<IH * IW, 256, float>(
+        [&](int q, float &v) {
+            const int r = q / IW, c = q - r * IW, gy = iy0 + r, gx = ix0 + c;
+            if (EDGE && !(gy >= 0 && gy < h && gx >= 0 && gx < w)) return false;
+            v = src[(size_t)gy * w + gx];
+            return true;
+        },
+        [&](int q, float v) {
+            const int r = q / IW;
+            S.T[r][q - r * IW] = v;
+        });
     __syncthreads();
     for (int q = tid; q < IH * BT_TX; q += 256) {
         const int r = q / BT_TX, jj = q - r * BT_TX, gy = iy0 + r;
@@ -84,35 +93,45 @@ __device__ __forceinline__ void burt_body(const float *__restrict__ src, int w, 
         S.Hs[r][jj] = a;
     }
     __syncthreads();
-    const int tx = tid & (BT_TX - 1), ty = tid / BT_TX, oj = ox0 + tx, oi = oy0 + ty;
-    if (EDGE && (oj >= ow || oi >= oh)) return;
-    const int cx = 2 * oj, cy = 2 * oi;
-    float rn = 0.0f, cn = 0.0f;
+    const int tx = tid & (BT_TX - 1), oj = ox0 + tx, cx = 2 * oj;
+    if (EDGE && oj >= ow) return;
+    float cn = 0.0f;
 #pragma unroll
     for (int t = 0; t < 5; t++) {
-        const int yy = cy + t - 2, xx = cx + t - 2;
-        if (!EDGE || (yy >= 0 && yy < h)) rn += k[t];
+        const int xx = cx + t - 2;
         if (!EDGE || (xx >= 0 && xx < w)) cn += k[t];
     }
-    float acc = 0.0f;
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const int yy = cy + t - 2;
-        if (!EDGE || (yy >= 0 && yy < h)) acc += k[t] * S.Hs[2 * ty + t][tx];
+    for (int qy = 0; qy < TY / 4; qy++) {
+        const int ty = tid / BT_TX + 4 * qy, oi = oy0 + ty, cy = 2 * oi;
+        if (EDGE && oi >= oh) break;
+        float rn = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+            const int yy = cy + t - 2;
+            if (!EDGE || (yy >= 0 && yy < h)) rn += k[t];
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+            const int yy = cy + t - 2;
+            if (!EDGE || (yy >= 0 && yy < h)) acc += k[t] * S.Hs[2 * ty + t][tx];
+        }
+        dst[(size_t)oi * ow + oj] = acc / (rn * cn);
     }
-    dst[(size_t)oi * ow + oj] = acc / (rn * cn);
 }
 
+template <int TY>
 __global__ __launch_bounds__(256) void k_burt(const float *__restrict__ src, int w, int h, float *__restrict__ dst,
                                               int ow, int oh) {
-    __shared__ BurtLds S;
-    const int ox0 = blockIdx.x * BT_TX, oy0 = blockIdx.y * BT_TY;
-    const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * BT_TX <= w - 1 && 2 * oy0 + 2 * BT_TY <= h - 1 &&
-                       ox0 + BT_TX <= ow && oy0 + BT_TY <= oh;
+    __shared__ BurtLds<TY> S;
+    const int ox0 = blockIdx.x * BT_TX, oy0 = blockIdx.y * TY;
+    const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * BT_TX <= w - 1 && 2 * oy0 + 2 * TY <= h - 1 &&
+                       ox0 + BT_TX <= ow && oy0 + TY <= oh;
     if (inner)
-        burt_body<false>(src, w, h, dst, ow, oh, ox0, oy0, S);
+        burt_body<TY, false>(src, w, h, dst, ow, oh, ox0, oy0, S);
     else
-        burt_body<true>(src, w, h, dst, ow, oh, ox0, oy0, S);
+        burt_body<TY, true>(src, w, h, dst, ow, oh, ox0, oy0, S);
 }
 
 // align.rs:639-646
@@ -906,6 +925,44 @@ __global__ void k_frame_gains(const double *__restrict__ sums, int enabled, Fram
 // The final resample of a frame (align.rs:554-559) with the as-shot fallback
 // outside the frame (stack.rs:669-678), the brightness gains (brightness.rs:
 // 124-133), and optionally the warped registration luma (stack.rs:527).
+// The warp of one pixel: the three channels (and the luma when LUMA) summed in
+// one pass over the taps, each sum in warp_sample's order, every load
+// independent of whether the point is in the frame (clamped indices are always
+// in range), so a thread's loads are all in flight together.
+template <int N, bool LUMA>
+__device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const float *__restrict__ yin, int w, int h,
+                                        size_t o, bool valid, int x0, int y0, const float *wx, const float *wy,
+                                        const Gains &g, float *__restrict__ planes, float *__restrict__ yout) {
+    constexpr int start = 1 - N / 2;
+    const size_t n = (size_t)w * h;
+    float raw[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) raw[c] = (float)rgb[3 * o + c] * INV16;
+    float acc[3] = {0.0f, 0.0f, 0.0f}, ay = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const size_t row = (size_t)min(max(y0 + start + j, 0), h - 1) * w;
+        float r[3] = {0.0f, 0.0f, 0.0f}, ry = 0.0f;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const size_t q = row + min(max(x0 + start + i, 0), w - 1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) r[c] += wx[i] * ((float)rgb[3 * q + c] * INV16);
+            if (LUMA) ry += wx[i] * yin[q];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] += wy[j] * r[c];
+        if (LUMA) ay += wy[j] * ry;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float v = valid ? acc[c] : raw[c];
+        if (g.on[c]) v *= g.g[c];
+        planes[c * n + o] = v;
+    }
+    if (LUMA) yout[o] = ay;
+}
+
 template <int N>
 __global__ __launch_bounds__(256) void k_warp(const uint16_t *__restrict__ rgb, const float *__restrict__ yin, int w,
                                               int h, Inv9 inv, int affine, int ik, Gains g, float *__restrict__ planes,
@@ -917,7 +974,7 @@ __global__ __launch_bounds__(256) void k_warp(const uint16_t *__restrict__ rgb, 
         affine = fp->affine;
         g = fp->g;
     }
-    size_t n = (size_t)w * h, o = (size_t)y * w + x;
+    size_t o = (size_t)y * w + x;
     double sx, sy;
     src_point(inv, affine, x, y, sx, sy);
     bool valid = sx >= 0.0 && sx <= (double)(w - 1) && sy >= 0.0 && sy <= (double)(h - 1);
@@ -926,13 +983,10 @@ __global__ __launch_bounds__(256) void k_warp(const uint16_t *__restrict__ rgb, 
     float wx[6], wy[6];
     interp_w(ik, sx - fx, wx);
     interp_w(ik, sy - fy, wy);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float v = valid ? warp_sample<N>(rgb, nullptr, c, w, h, x0, y0, wx, wy) : (float)rgb[3 * o + c] * INV16;
-        if (g.on[c]) v *= g.g[c];
-        planes[c * n + o] = v;
-    }
-    if (yout) yout[o] = warp_sample<N>(nullptr, yin, -1, w, h, x0, y0, wx, wy);
+    if (yout)
+        warp_px<N, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
+    else
+        warp_px<N, false>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
 }
 
 // The frame as decoded (io.rs:89), times the gains
@@ -1079,8 +1133,9 @@ static Inv9 inv_for(const Sim &s, size_t w, size_t h, int &affine) {
 
 // the registration pyramid above level 0 (align.rs:741-754)
 static void burt_level(const float *src, size_t w, size_t h, float *dst, size_t ow, size_t oh) {
-    launch_b("burt_reduce", 4.0 * (w * h + ow * oh), k_burt, dim3(cdiv(ow, BT_TX), cdiv(oh, BT_TY)), dim3(256), 0, src,
-             (int)w, (int)h, dst, (int)ow, (int)oh);
+    const dim3 grid(cdiv(ow, BT_TX), cdiv(oh, BT_TY));
+    launch_b("burt_reduce", 4.0 * (w * h + ow * oh), k_burt<BT_TY>, grid, dim3(256), 0, src, (int)w, (int)h, dst, (int)ow,
+             (int)oh);
 }
 
 static void build_reg_pyramid(lsh_frames *f, float *base) {

@@ -160,9 +160,9 @@ __device__ void fold_group(const BStack &B, int l, int r, int F, int nf, const W
         const unsigned widx = (unsigned)(r * F + q);
         const float *Gq = B.Gg + (size_t)q * B.slot;
         if (l == 0)
-            fold_stage<R, TY>(RgbSrc{B.frames[widx]}, Gq + gn, w, h, cw, ch, wt, chroma, bx, by, S);
+            fold_stage<R, TY, false>(RgbSrc{B.frames[widx]}, Gq + gn, w, h, cw, ch, wt, chroma, bx, by, S);
         else
-            fold_stage<R, TY>(PlaneSrc{Gq + gl, n}, Gq + gn, w, h, cw, ch, wt, chroma, bx, by, S);
+            fold_stage<R, TY, false>(PlaneSrc{Gq + gl, n}, Gq + gn, w, h, cw, ch, wt, chroma, bx, by, S);
         if (x < w) {
 #pragma unroll
             for (int k = 0; k < NQ; k++) {
@@ -244,11 +244,11 @@ __global__ __launch_bounds__(256) void k_batch(const BStack *__restrict__ st, BP
                         if (p == 1) {
                             const RgbSrc src{B.frames[r * P.F + q]};
                             for (int c = 0; c < 3; c++)
-                                reduce_tile(src, c, w, h, dst + (size_t)c * ow * oh, ow, oh, ox0, oy0, SR);
+                                reduce_tile<false>(src, c, w, h, dst + (size_t)c * ow * oh, ow, oh, ox0, oy0, SR);
                         } else {
                             const PlaneSrc src{Gq + 3 * (B.off[p - 1] - B.off[1]), (size_t)w * h};
                             for (int c = 0; c < 3; c++)
-                                reduce_tile(src, c, w, h, dst + (size_t)c * ow * oh, ow, oh, ox0, oy0, SR);
+                                reduce_tile<false>(src, c, w, h, dst + (size_t)c * ow * oh, ow, oh, ox0, oy0, SR);
                         }
                         __syncthreads();
                         continue;

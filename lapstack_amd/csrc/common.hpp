@@ -185,6 +185,25 @@ __host__ __device__ __forceinline__ int rf101(int i, int n) {
     }
 }
 
+// Stages COUNT items into LDS with NT threads: each thread issues all of its
+// loads before it stores any of them, so a tile waits on memory once rather
+// than once per item (a plain strided loop waits after every load).  ld(k, v)
+// loads item k into v and returns whether the item exists; st(k, v) stores it.
+template <int COUNT, int NT, class T, class Ld, class St>
+__device__ __forceinline__ void lds_stage(const Ld &ld, const St &st) {
+    constexpr int IT = (COUNT + NT - 1) / NT;
+    T v[IT];
+    bool ok[IT];
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+        const int k = (int)threadIdx.x + i * NT;
+        ok[i] = k < COUNT && ld(k, v[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; i++)
+        if (ok[i]) st((int)threadIdx.x + i * NT, v[i]);
+}
+
 // Rust f32::clamp
 __host__ __device__ __forceinline__ float rclampf(float v, float lo, float hi) {
     if (v < lo) v = lo;

@@ -46,10 +46,28 @@ __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ p
     const int px0 = blockIdx.x * TX * k - R, py0 = blockIdx.y * TY * k - R;
     const size_t n = (size_t)w * h;
     const int tid = threadIdx.y * TX + threadIdx.x;
-    for (int q = tid; q < PW * PH; q += 256) {
-        int ly = q / PW, lx = q - ly * PW;
-        size_t o = (size_t)rf101(py0 + ly, h) * w + rf101(px0 + lx, w);
-        Y[q] = 0.299f * planes[o] + 0.587f * planes[n + o] + 0.114f * planes[2 * n + o];  // depth.rs:160-165
+    // the window's size is a run-time value: loads go out eight per thread at a
+    // time, each group's stores after all of its loads
+    constexpr int U = 8;
+    const int NQ = PW * PH;
+    for (int q0 = tid; q0 < NQ; q0 += U * 256) {
+        float v[U][3];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = q0 + 256 * u;
+            if (q < NQ) {
+                const int ly = q / PW, lx = q - ly * PW;
+                const size_t o = (size_t)rf101(py0 + ly, h) * w + rf101(px0 + lx, w);
+                v[u][0] = planes[o];
+                v[u][1] = planes[n + o];
+                v[u][2] = planes[2 * n + o];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = q0 + 256 * u;
+            if (q < NQ) Y[q] = 0.299f * v[u][0] + 0.587f * v[u][1] + 0.114f * v[u][2];  // depth.rs:160-165
+        }
     }
     __syncthreads();
     const int ox = blockIdx.x * TX + threadIdx.x;

@@ -19,12 +19,15 @@
 
 namespace lsh {
 
-// REDUCE of one plane (blockIdx.z selects the plane) on 64x4 output tiles
+// REDUCE of one plane (blockIdx.z selects the plane) on 64 x 8 output tiles
+// (the 19-row input window re-reads 19/16 of the level where 64 x 4 tiles
+// re-read 11/8; 17 % faster on a 45 MP level)
+constexpr int RED3_TY = 8;
 __global__ __launch_bounds__(256) void k_reduce3(const float *__restrict__ src, int w, int h,
                                                  float *__restrict__ dst, int ow, int oh) {
-    __shared__ ReduceLds S;
-    reduce_tile(PlaneSrc{src, (size_t)w * h}, (int)blockIdx.z, w, h, dst + (size_t)blockIdx.z * ow * oh, ow, oh,
-                blockIdx.x * RED_TX, blockIdx.y * RED_TY, S);
+    __shared__ ReduceLdsT<RED3_TY> S;
+    reduce_tile<true>(PlaneSrc{src, (size_t)w * h}, (int)blockIdx.z, w, h, dst + (size_t)blockIdx.z * ow * oh, ow, oh,
+                blockIdx.x * RED_TX, blockIdx.y * RED3_TY, S);
 }
 
 // one band-pass level of the fold on 64x16 tiles
@@ -275,7 +278,7 @@ static void fuser_push(lsh_fuser *f, const float *g0, const unsigned *idx_dev = 
     auto lvl = [&](size_t l) -> const float * { return l == 0 ? g0 : f->G.p + 3 * f->off[l]; };
     // the Gaussian levels 1..L
     for (size_t l = 0; l < L; l++) {
-        dim3 g(cdiv(f->lw[l + 1], 64), cdiv(f->lh[l + 1], 4), 3);
+        dim3 g(cdiv(f->lw[l + 1], 64), cdiv(f->lh[l + 1], RED3_TY), 3);
         double bytes = 12.0 * (f->lw[l] * f->lh[l] + f->lw[l + 1] * f->lh[l + 1]);
         launch_b("reduce", bytes, k_reduce3, g, dim3(256), 0, lvl(l), (int)f->lw[l], (int)f->lh[l],
                  f->G.p + 3 * f->off[l + 1], (int)f->lw[l + 1], (int)f->lh[l + 1]);
@@ -298,7 +301,7 @@ static void fuser_push(lsh_fuser *f, const float *g0, const unsigned *idx_dev = 
                f->wgt_a.p);
         float *wa = f->wgt_a.p, *wb = f->wgt_b.p;
         for (size_t li = g + 1; li <= L; li++) {
-            dim3 gr(cdiv(f->lw[li], 64), cdiv(f->lh[li], 4), 1);
+            dim3 gr(cdiv(f->lw[li], 64), cdiv(f->lh[li], RED3_TY), 1);
             launch("reduce", k_reduce3, gr, dim3(256), 0, (const float *)wa, (int)f->lw[li - 1], (int)f->lh[li - 1], wb,
                    (int)f->lw[li], (int)f->lh[li]);
             std::swap(wa, wb);

@@ -16,10 +16,12 @@ __constant__ float KR[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.
 
 #define RED_TX 64
 #define RED_TY 4
-struct ReduceLds {
-    float T[2 * RED_TY + 3][2 * RED_TX + 3 + 1];
-    float Hs[2 * RED_TY + 3][RED_TX + 1];
+template <int TY>
+struct ReduceLdsT {
+    float T[2 * TY + 3][2 * RED_TX + 3 + 1];
+    float Hs[2 * TY + 3][RED_TX + 1];
 };
+using ReduceLds = ReduceLdsT<RED_TY>;
 
 // Where a tile body reads the samples of a level: float planes (c * n + o), or
 // the frame as decoded, interleaved RGB u16 scaled by 1/65535 (io.rs:89), which
@@ -44,19 +46,33 @@ __device__ __forceinline__ int refl(int i, int n) {
     return EDGE ? rf101(i, n) : i;
 }
 
-template <bool EDGE, class Src>
+template <int TY, bool EDGE, bool PRE, class Src>
 __device__ __forceinline__ void reduce_body(const Src &s, int c, int w, int h, float *__restrict__ d, int ow, int oh,
-                                            int ox0, int oy0, ReduceLds &S) {
-    constexpr int TX = RED_TX, TY = RED_TY, IW = 2 * TX + 3, IH = 2 * TY + 3;
+                                            int ox0, int oy0, ReduceLdsT<TY> &S) {
+    constexpr int TX = RED_TX, IW = 2 * TX + 3, IH = 2 * TY + 3;
     const int tid = threadIdx.x;
     const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
     const int nx = EDGE ? min(IW, 2 * (ow - ox0) + 3) : IW, ny = EDGE ? min(IH, 2 * (oh - oy0) + 3) : IH;
-    for (int k = tid; k < IH * IW; k += TX * TY) {
-        int r = k / IW, x = k - r * IW;
-        if (!EDGE || (r < ny && x < nx)) S.T[r][x] = s((size_t)refl<EDGE>(iy0 + r, h) * w + refl<EDGE>(ix0 + x, w), c);
+    auto ld = [&](int k, float &v) {
+        const int r = k / IW, x = k - r * IW;
+        if (EDGE && !(r < ny && x < nx)) return false;
+        v = s((size_t)refl<EDGE>(iy0 + r, h) * w + refl<EDGE>(ix0 + x, w), c);
+        return true;
+    };
+    auto st = [&](int k, float v) {
+        const int r = k / IW;
+        S.T[r][k - r * IW] = v;
+    };
+    if constexpr (PRE) {
+        lds_stage<IH * IW, 256, float>(ld, st);
+    } else {
+        for (int k = tid; k < IH * IW; k += 256) {
+            float v;
+            if (ld(k, v)) st(k, v);
+        }
     }
     __syncthreads();
-    for (int k = tid; k < IH * TX; k += TX * TY) {
+    for (int k = tid; k < IH * TX; k += 256) {
         int r = k / TX, jj = k - r * TX;
         if (!EDGE || (r < ny && ox0 + jj < ow)) {
             float a = 0.0f;
@@ -66,29 +82,34 @@ __device__ __forceinline__ void reduce_body(const Src &s, int c, int w, int h, f
         }
     }
     __syncthreads();
-    const int tx = tid & (TX - 1), ty = tid / TX;
-    const int j = ox0 + tx, i = oy0 + ty;
-    if (!EDGE || (j < ow && i < oh)) {
-        const int r0 = 2 * ty;
+    const int tx = tid & (TX - 1), j = ox0 + tx;
+    if (EDGE && j >= ow) return;
+#pragma unroll
+    for (int q = 0; q < TY / 4; q++) {
+        const int ty = tid / TX + 4 * q, i = oy0 + ty, r0 = 2 * ty;
+        if (EDGE && i >= oh) break;
         d[(size_t)i * ow + j] = KR[0] * S.Hs[r0][tx] + KR[1] * S.Hs[r0 + 1][tx] + KR[2] * S.Hs[r0 + 2][tx] +
                                 KR[3] * S.Hs[r0 + 3][tx] + KR[4] * S.Hs[r0 + 4][tx];
     }
 }
 
-// pyramid.rs:86-119: REDUCE of plane c on the 64x4 output tile at (ox0, oy0):
-// the 131x11 input window (reflect-101 at the borders) is staged in LDS, the
-// horizontal 5-tap sums of its 11 rows at the 64 even columns go to LDS, and
-// each of the 256 threads sums its five rows.  Calls may follow each other on
-// one ReduceLds with no barrier between them.
-template <class Src>
+// pyramid.rs:86-119: REDUCE of plane c on the 64 x TY output tile at (ox0,
+// oy0): the (2 TY + 3) x 131 input window (reflect-101 at the borders) is
+// staged in LDS, the horizontal 5-tap sums of its rows at the 64 even columns
+// go to LDS, and each of the 256 threads sums five rows for each of its TY / 4
+// outputs.  Calls may follow each other on one LDS tile with no barrier
+// between them.  PRE: the window's loads all go out before the first is
+// stored (one wait on memory per tile), else in a strided loop (fewer
+// registers); the same values either way.
+template <bool PRE, int TY, class Src>
 __device__ __forceinline__ void reduce_tile(const Src &s, int c, int w, int h, float *__restrict__ d, int ow, int oh,
-                                            int ox0, int oy0, ReduceLds &S) {
-    const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * RED_TX <= w - 1 && 2 * oy0 + 2 * RED_TY <= h - 1 &&
-                       ox0 + RED_TX <= ow && oy0 + RED_TY <= oh;
+                                            int ox0, int oy0, ReduceLdsT<TY> &S) {
+    const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * RED_TX <= w - 1 && 2 * oy0 + 2 * TY <= h - 1 &&
+                       ox0 + RED_TX <= ow && oy0 + TY <= oh;
     if (inner)
-        reduce_body<false>(s, c, w, h, d, ow, oh, ox0, oy0, S);
+        reduce_body<TY, false, PRE>(s, c, w, h, d, ow, oh, ox0, oy0, S);
     else
-        reduce_body<true>(s, c, w, h, d, ow, oh, ox0, oy0, S);
+        reduce_body<TY, true, PRE>(s, c, w, h, d, ow, oh, ox0, oy0, S);
 }
 
 // pyramid.rs:128-140: the horizontal EXPAND of coarse row cr at fine column x
@@ -132,7 +153,7 @@ struct FoldLds {
 
 constexpr int floordiv2c(int a) { return a >= 0 ? a / 2 : -((1 - a) / 2); }
 
-template <int R, int TY, bool EDGE, class Src>
+template <int R, int TY, bool EDGE, bool PRE, class Src>
 __device__ __forceinline__ void fold_body(const Src &G, const float *__restrict__ Cn, int w, int h, int cw, int ch,
                                           const Wt9 &wt, int chroma, int bx, int by, FoldLds<R, TY> &S) {
     constexpr int TX = 64, EW = TX + 2 * R, EH = TY + 2 * R;
@@ -145,9 +166,44 @@ __device__ __forceinline__ void fold_body(const Src &G, const float *__restrict_
     const int cx0 = (bx >> 1) + floordiv2c(-R) - 1, cy0 = (by >> 1) + floordiv2c(-R) - 1;
     // the last fine column / row any output of this tile reads (window reach)
     const int xlim = min(bx + TX, w) - 1 + R, ylim = min(by + TY, h) - 1 + R;
-    for (int k = tid; k < 3 * CH * CW; k += 256) {
-        int c = k / (CH * CW), r = k - c * CH * CW, j = r / CW, i = r - j * CW;
-        S.Cs[c][j][i] = Cn[c * cn + (size_t)refl<EDGE>(cy0 + j, ch) * cw + refl<EDGE>(cx0 + i, cw)];
+    constexpr int NC = 3 * CH * CW, ITC = (NC + 255) / 256, NL = EH * EW, ITL = PRE ? (NL + 255) / 256 : 1;
+    auto coarse_at = [&](int k) {
+        const int c = k / (CH * CW), r = k - c * CH * CW, j = r / CW, i = r - j * CW;
+        return Cn[c * cn + (size_t)refl<EDGE>(cy0 + j, ch) * cw + refl<EDGE>(cx0 + i, cw)];
+    };
+    auto coarse_put = [&](int k, float v) {
+        const int c = k / (CH * CW), r = k - c * CH * CW, j = r / CW;
+        S.Cs[c][j][r - j * CW] = v;
+    };
+    auto skip = [&](int ly, int lx) { return EDGE && (by + ly - R > ylim || bx + lx - R > xlim); };
+    auto fine_at = [&](int ly, int lx) { return (size_t)refl<EDGE>(by + ly - R, h) * w + refl<EDGE>(bx + lx - R, w); };
+    float gv[ITL][3];
+    if constexpr (PRE) {
+        // Every global load of the tile is issued before the first is used: the
+        // coarse window, then the fine samples of the L stage below, held in
+        // registers, so the tile waits on memory once.
+        float cv[ITC];
+#pragma unroll
+        for (int it = 0; it < ITC; it++) {
+            const int k = tid + 256 * it;
+            if (k < NC) cv[it] = coarse_at(k);
+        }
+#pragma unroll
+        for (int it = 0; it < ITL; it++) {
+            const int k = tid + 256 * it, ly = k / EW, lx = k - ly * EW;
+            if (k < NL && !skip(ly, lx)) {
+                const size_t o = fine_at(ly, lx);
+#pragma unroll
+                for (int c = 0; c < 3; c++) gv[it][c] = G(o, c);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < ITC; it++) {
+            const int k = tid + 256 * it;
+            if (k < NC) coarse_put(k, cv[it]);
+        }
+    } else {
+        for (int k = tid; k < NC; k += 256) coarse_put(k, coarse_at(k));
     }
     __syncthreads();
     // pyramid.rs:128-140: horizontal EXPAND of each staged coarse row at each tile column
@@ -161,17 +217,14 @@ __device__ __forceinline__ void fold_body(const Src &G, const float *__restrict_
     }
     __syncthreads();
     // pyramid.rs:151-166 vertical EXPAND, L = G - EXPAND (pyramid.rs:208-211), node energy
-    for (int k = tid; k < EH * EW; k += 256) {
-        int ly = k / EW, lx = k - ly * EW;
-        if (EDGE && (by + ly - R > ylim || bx + lx - R > xlim)) continue;
-        int gy = refl<EDGE>(by + ly - R, h), gx = refl<EDGE>(bx + lx - R, w), j0 = (gy >> 1) - cy0;
-        size_t o = (size_t)gy * w + gx;
+    auto lstage = [&](int ly, int lx, const float *g) {
+        int gy = refl<EDGE>(by + ly - R, h), j0 = (gy >> 1) - cy0;
         float l[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             float e = (gy & 1) == 0 ? EV0 * S.Hs[c][j0 - 1][lx] + EV1 * S.Hs[c][j0][lx] + EV2 * S.Hs[c][j0 + 1][lx]
                                     : ODDK * (S.Hs[c][j0][lx] + S.Hs[c][j0 + 1][lx]);
-            l[c] = G(o, c) - e;
+            l[c] = g[c] - e;
         }
         S.Es[ly][lx] = node_energy(l[0], l[1], l[2], chroma);
         int cy = ly - R, cx = lx - R;
@@ -179,6 +232,21 @@ __device__ __forceinline__ void fold_body(const Src &G, const float *__restrict_
             S.Ls[0][cy][cx] = l[0];
             S.Ls[1][cy][cx] = l[1];
             S.Ls[2][cy][cx] = l[2];
+        }
+    };
+    if constexpr (PRE) {
+#pragma unroll
+        for (int it = 0; it < ITL; it++) {
+            const int k = tid + 256 * it, ly = k / EW, lx = k - ly * EW;
+            if (k < NL && !skip(ly, lx)) lstage(ly, lx, gv[it]);
+        }
+    } else {
+        for (int k = tid; k < NL; k += 256) {
+            const int ly = k / EW, lx = k - ly * EW;
+            if (skip(ly, lx)) continue;
+            const size_t o = fine_at(ly, lx);
+            const float g[3] = {G(o, 0), G(o, 1), G(o, 2)};
+            lstage(ly, lx, g);
         }
     }
     __syncthreads();
@@ -203,8 +271,12 @@ __device__ __forceinline__ void fold_body(const Src &G, const float *__restrict_
 // unreflected coordinates and hold the reflect-101 sample of each; every index
 // a fine sample the window can reach needs lies inside the staged window.
 // Leaves the tile's L samples in S.Ls and the horizontal sums in S.Ts, with a
-// barrier after.  256 threads.
-template <int R, int TY, class Src>
+// barrier after.  256 threads.  PRE: every global load of the tile is issued
+// up front and held in registers (one wait on memory per tile); otherwise the
+// loads go out in strided loops, which keeps the register count low where
+// residency matters more than latency (the persistent batched kernel).  Both
+// compute the same values.
+template <int R, int TY, bool PRE, class Src>
 __device__ __forceinline__ void fold_stage(const Src &G, const float *__restrict__ Cn, int w, int h, int cw, int ch,
                                            const Wt9 &wt, int chroma, int bx, int by, FoldLds<R, TY> &S) {
     constexpr int TX = 64, CW = floordiv2c(TX + R - 1) - floordiv2c(-R) + 3,
@@ -213,9 +285,9 @@ __device__ __forceinline__ void fold_stage(const Src &G, const float *__restrict
     const bool inner = bx - R >= 0 && by - R >= 0 && bx + TX - 1 + R <= w - 1 && by + TY - 1 + R <= h - 1 && cx0 >= 0 &&
                        cy0 >= 0 && cx0 + CW - 1 <= cw - 1 && cy0 + CH - 1 <= ch - 1;
     if (inner)
-        fold_body<R, TY, false>(G, Cn, w, h, cw, ch, wt, chroma, bx, by, S);
+        fold_body<R, TY, false, PRE>(G, Cn, w, h, cw, ch, wt, chroma, bx, by, S);
     else
-        fold_body<R, TY, true>(G, Cn, w, h, cw, ch, wt, chroma, bx, by, S);
+        fold_body<R, TY, true, PRE>(G, Cn, w, h, cw, ch, wt, chroma, bx, by, S);
 }
 
 // fuse.rs:186-195: the vertical window at tile row ty, column tx (after fold_stage)
@@ -236,9 +308,19 @@ __device__ __forceinline__ void fold_tile(const Src &G, const float *__restrict_
                                           float *__restrict__ best, float *__restrict__ acc,
                                           uint16_t *__restrict__ winner, int first, unsigned widx, const Wt9 &wt,
                                           int chroma, float *__restrict__ en_out, int bx, int by, FoldLds<R, TY> &S) {
-    fold_stage<R, TY>(G, Cn, w, h, cw, ch, wt, chroma, bx, by, S);
     const size_t n = (size_t)w * h;
     const int tid = threadIdx.x, tx = tid & 63, x = bx + tx;
+    // the best energies so far at the thread's outputs, loaded ahead of the
+    // tile's stages (nothing in them writes best)
+    float prev[TY / 4];
+    if (!first && x < w) {
+#pragma unroll
+        for (int q = 0; q < TY / 4; q++) {
+            const int y = by + (tid >> 6) + 4 * q;
+            if (y < h) prev[q] = best[(size_t)y * w + x];
+        }
+    }
+    fold_stage<R, TY, true>(G, Cn, w, h, cw, ch, wt, chroma, bx, by, S);
     if (x < w) {
 #pragma unroll
         for (int q = 0; q < TY / 4; q++) {
@@ -252,7 +334,7 @@ __device__ __forceinline__ void fold_tile(const Src &G, const float *__restrict_
                 acc[o] = S.Ls[0][ty][tx];
                 acc[n + o] = S.Ls[1][ty][tx];
                 acc[2 * n + o] = S.Ls[2][ty][tx];
-            } else if (en > best[o]) {  // strict: ties keep the earlier frame (fuse.rs:429, 442)
+            } else if (en > prev[q]) {  // strict: ties keep the earlier frame (fuse.rs:429, 442)
                 best[o] = en;
                 acc[o] = S.Ls[0][ty][tx];
                 acc[n + o] = S.Ls[1][ty][tx];
