@@ -375,12 +375,20 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
         const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
         const int staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= NM_BOX;
 
-        if (staged)
-            for (int i = threadIdx.x; i < bw * bh; i += NM_THREADS) {
-                int r = i / bw;
-                boxs[i] = tg[(size_t)(by0 + r) * tw + bx0 + (i - r * bw)];
+        // the window's samples (at most NM_BOX / NM_THREADS per thread) and the
+        // tile's reference samples all go out before the first is used, so the
+        // tile waits on memory once
+        constexpr int SU = NM_BOX / NM_THREADS;
+        const int nbox = staged ? bw * bh : 0;
+        float sv[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int i = threadIdx.x + u * NM_THREADS;
+            if (i < nbox) {
+                const int r = i / bw;
+                sv[u] = tg[(size_t)(by0 + r) * tw + bx0 + (i - r * bw)];
             }
-        __syncthreads();
+        }
         // per point: the thread's 4 pixels, summed over the wave, added to the
         // wave's accumulators (one lane per wave writes them: no race)
         float rv[4];
@@ -389,6 +397,12 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
             const int x = X0 + tx, y = Y0 + ty + 16 * r;
             rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
         }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int i = threadIdx.x + u * NM_THREADS;
+            if (i < nbox) boxs[i] = sv[u];
+        }
+        __syncthreads();
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         for (int p = 0; p < np; p++) {
             Inv9 iv;
@@ -1192,7 +1206,8 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
         LSH_REQUIRE(per_cu > 0, "the search kernel cannot be resident");
         int want = 1;  // blocks per CU: more hide gather latency, fewer arrive faster
         if (const char *e = std::getenv("LSH_SEARCH_BLOCKS_PER_CU")) want = std::max(1, std::atoi(e));
-        f->nm_grid = (unsigned)(std::min(per_cu, want) * cus);
+        f->nm_bpc = (unsigned)std::min(per_cu, want);
+        f->nm_grid = f->nm_bpc * (unsigned)cus;
         f->nm_grid -= f->nm_grid % NM_GROUPS;
         LSH_REQUIRE(f->nm_grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
         if (f->nm_grid > NM_BOX / 2) f->nm_grid = NM_BOX / 2;  // the fold stages one value per block in LDS
@@ -1217,12 +1232,19 @@ static void search_launch(lsh_frames *f, bool plain = false) {
     launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p);
     const NmConst *kp = (const NmConst *)f->nm_const.p;
     double *state = f->nm_state.p, *part = f->nm_part.p;
+    unsigned grid = f->nm_grid;
+    if (const unsigned c = stream_cus()) {
+        // a stream restricted to c CUs holds only that many blocks at once
+        grid = std::min(grid, f->nm_bpc * c);
+        grid -= grid % NM_GROUPS;
+        LSH_REQUIRE(grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
+    }
     if (plain) {
-        launch("nm_search", k_nm_search, dim3(f->nm_grid), dim3(NM_THREADS), 0, kp, state, part, ctr, err);
+        launch("nm_search", k_nm_search, dim3(grid), dim3(NM_THREADS), 0, kp, state, part, ctr, err);
         return;
     }
     void *args[] = {(void *)&kp, (void *)&state, (void *)&part, (void *)&ctr, (void *)&err};
-    launch_coop("nm_search", 0.0, (const void *)k_nm_search, dim3(f->nm_grid), dim3(NM_THREADS), args, 0);
+    launch_coop("nm_search", 0.0, (const void *)k_nm_search, dim3(grid), dim3(NM_THREADS), args, 0);
 }
 
 static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8, size_t coarsen) {

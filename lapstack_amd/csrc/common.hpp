@@ -39,15 +39,18 @@ struct Error : std::runtime_error {
 // ---------------------------------------------------------------- runtime
 hipStream_t stream();          // the library's stream on the current device (or the OnStream one)
 hipStream_t side_stream();     // a second stream on the current device (default priority), made on first use
-void sync();                   // wait for both
-// While one lives, this thread's launches and copies go to stream s.
+void sync();                   // wait for both (and for the OnStream stream, if another)
+// While one lives, this thread's launches and copies go to stream s; cus: the
+// number of CUs s may use when it is restricted to a subset (0: all of them).
 struct OnStream {
     hipStream_t prev;
-    explicit OnStream(hipStream_t s);
+    unsigned prev_cus;
+    explicit OnStream(hipStream_t s, unsigned cus = 0);
     ~OnStream();
     OnStream(const OnStream &) = delete;
     OnStream &operator=(const OnStream &) = delete;
 };
+unsigned stream_cus();  // the CU budget of the current stream (0: the whole device)
 void *pinned_scratch(size_t);  // a per-device pinned host block (reused; at least the size asked)
 
 // ------------------------------------------------------------- profiling

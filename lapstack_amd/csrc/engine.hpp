@@ -35,6 +35,7 @@ struct lsh_frames {
     lsh::DBuf<unsigned> nm_flags; // grid-barrier counter, error flag
     lsh::DBuf<double> nm_part;    // per-block partial sums, two slots
     unsigned nm_grid = 0;
+    unsigned nm_bpc = 1;  // the search's blocks per CU
     uint64_t lev_evals[16] = {0}, lev_ticks[16] = {0};  // the last search, per registration level
     uint64_t lev_passes[16] = {0};                       // its collective passes (several evaluations each)
     std::vector<uint8_t> nm_host;                        // the search constants last uploaded

@@ -30,8 +30,11 @@ DevState &dev_state() {
 }  // namespace
 
 thread_local hipStream_t t_override = nullptr;  // OnStream: this thread's launches go there
+thread_local unsigned t_cus = 0;                 // and the CUs that stream may use (0: all)
 
 hipStream_t stream() { return t_override ? t_override : dev_state().s; }
+
+unsigned stream_cus() { return t_override ? t_cus : 0; }
 
 hipStream_t side_stream() {
     DevState &st = dev_state();
@@ -39,11 +42,18 @@ hipStream_t side_stream() {
     return st.side;
 }
 
-OnStream::OnStream(hipStream_t s) : prev(t_override) { t_override = s; }
-OnStream::~OnStream() { t_override = prev; }
+OnStream::OnStream(hipStream_t s, unsigned cus) : prev(t_override), prev_cus(t_cus) {
+    t_override = s;
+    t_cus = cus;
+}
+OnStream::~OnStream() {
+    t_override = prev;
+    t_cus = prev_cus;
+}
 
 void sync() {
     DevState &st = dev_state();
+    if (t_override && t_override != st.s && t_override != st.side) LSH_CHECK(hipStreamSynchronize(t_override));
     LSH_CHECK(hipStreamSynchronize(st.s));
     if (st.side) LSH_CHECK(hipStreamSynchronize(st.side));
 }

@@ -12,9 +12,10 @@
 //
 // Only the chain is serial across frames (frame k+1 registers against frame
 // k's warped luma), so frame k's side work runs beside frame k+1's chain: the
-// search is bound by the latency of its grid-wide reductions and leaves most
-// of the chip to the side work.  Every value is the one the per-frame path
-// computes; only the overlap differs.
+// search is bound by the latency of its grid-wide reductions, and its grid
+// here covers three quarters of the CUs, which leaves the rest to the side
+// work.  Every value is the one the per-frame path computes; only the overlap
+// differs.
 //
 // graph == 0: two streams and events.  graph != 0: frame k >= 2 (with a frame
 // after it) is one replay of a captured hipGraph with two branches -- frame
@@ -22,6 +23,7 @@
 // buffers alternate); the raw frames are staged into two fixed buffers, the
 // slices leave through two fixed buffers, and the fold takes its frame index
 // from a device counter only the side work advances.
+#include <cstdlib>
 #include <vector>
 
 #include "api.hpp"
@@ -103,7 +105,22 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         frames_stream_setup(f, free8, coarsen, n);
         fuser_reserve(fu, n);
         frames_stream_slots(f);
+        // The search's grid covers three quarters of the CUs here, so the side
+        // work has the rest while it runs (a search block fills its CU, so a
+        // full grid leaves the side work nothing but the gaps between
+        // searches).  On the 100 x 45 MP stack this took the launched step
+        // from 953 to 906 ms: the smallest grid keeping the finest level's
+        // tile rounds (4) that frees eight CUs per XCD; 176 and 208 CUs were
+        // both slower.  LSH_STREAM_SEARCH_CUS overrides it.
+        int dev = 0, ncu = 0;
+        LSH_CHECK(hipGetDevice(&dev));
+        LSH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        unsigned search_cus = (unsigned)ncu * 3 / 4;
+        if (const char *e = std::getenv("LSH_STREAM_SEARCH_CUS"))
+            if (std::atoi(e) > 0) search_cus = (unsigned)std::atoi(e);
+        OnStream on_chain(stream(), search_cus);
         const hipStream_t Ms = stream(), Ss = side_stream();
+        Events E;
         float *planes[2] = {f->planes.p, f->planes_b.p};
         const size_t fbytes = 3 * npx * sizeof(uint16_t);
         // raw frames: the caller's device pointers, or staged into two fixed buffers (host frames, graphs)
@@ -135,7 +152,6 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         };
         DBuf<unsigned> fidx(1);  // the fold's frame index (side work only)
         launch("stream_fold_index", k_counter_set, dim3(1), dim3(64), 0, fidx.p, 1u);
-        Events E;
         const hipEvent_t e_main = E.make();
         hipEvent_t e_target[2] = {E.make(), E.make()}, e_warp[2] = {E.make(), E.make()},
                    e_side[2] = {E.make(), E.make()};
