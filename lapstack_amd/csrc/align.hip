@@ -312,9 +312,12 @@ __device__ __forceinline__ void zero_waves(NmShared &sh) {
 // every block over its grid-strided share, the partial sums folded by the
 // block completing the arrival tree and the costs broadcast.  Batching the
 // points does not change any cost: each point's sums are the ones a pass of
-// its own would form.
-__device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, int np, bool solo, double *part,
-                        unsigned *ctr, unsigned *err, unsigned &gen) {
+// its own would form.  resident (solo levels small enough): the whole target
+// level is boxs[0, n) and the reference level boxs[n, 2n), staged once for
+// the level, so a pass reads no global memory; the tiles, the taps and the
+// order of every sum are the same as when each tile stages its own window.
+__device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, int np, bool solo, bool resident,
+                        double *part, unsigned *ctr, unsigned *err, unsigned &gen) {
     const int tw = K.lw[lvl], th = K.lh[lvl];
     uint64_t tq = wall_clock64();
     if (threadIdx.x == 0) sh.ph[3] += tq - sh.tmark;
@@ -334,7 +337,7 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
     // pixel, so over the level it is bounded by its values at the 4 corners: one
     // bound for the pass, from which every tile's window follows.  A projective
     // point (or a wild one) turns the staging off for the pass.
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && !resident) {
         double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
         int ok = 1;
         for (int p = 0; p < np; p++) {
@@ -366,43 +369,41 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
         while (R > 1 && ntx * ((th + 16 * R - 1) / (16 * R)) < (int)gridDim.x) R >>= 1;
     const int TH = 16 * R, nty = (th + TH - 1) / TH, ntiles = ntx * nty;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (resident) __syncthreads();  // the cleared accumulators (the tile loop has no barrier then)
     for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : gridDim.x) {
         const int X0 = (t % ntx) * NM_TILE, Y0 = (t / ntx) * TH;
         const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
-        __syncthreads();  // the bound is written; the previous tile's window is no longer read
-        const int bx0 = max(0, X0 + sh.box[0]), bx1 = min(tw - 1, X1 + sh.box[1]);
-        const int by0 = max(0, Y0 + sh.box[2]), by1 = min(th - 1, Y1 + sh.box[3]);
-        const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
-        const int staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= NM_BOX;
-
-        // the window's samples (at most NM_BOX / NM_THREADS per thread) and the
-        // tile's reference samples all go out before the first is used, so the
-        // tile waits on memory once
-        constexpr int SU = NM_BOX / NM_THREADS;
-        const int nbox = staged ? bw * bh : 0;
-        float sv[SU];
+        int bx0 = 0, by0 = 0, bw = tw, bh = th, staged = 1;
+        float rv[4];
+        if (resident) {
 #pragma unroll
-        for (int u = 0; u < SU; u++) {
-            const int i = threadIdx.x + u * NM_THREADS;
-            if (i < nbox) {
-                const int r = i / bw;
-                sv[u] = tg[(size_t)(by0 + r) * tw + bx0 + (i - r * bw)];
+            for (int r = 0; r < 4; r++) {
+                const int x = X0 + tx, y = Y0 + ty + 16 * r;
+                rv[r] = (r < R && x < tw && y < th) ? boxs[tw * th + y * tw + x] : 0.0f;
+            }
+        } else {
+            __syncthreads();  // the bound is written; the previous tile's window is no longer read
+            bx0 = max(0, X0 + sh.box[0]);
+            by0 = max(0, Y0 + sh.box[2]);
+            bw = min(tw - 1, X1 + sh.box[1]) - bx0 + 1;
+            bh = min(th - 1, Y1 + sh.box[3]) - by0 + 1;
+            staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= NM_BOX;
+            // (issuing all of a thread's window loads before storing any measured
+            // no faster here and spilled registers)
+            if (staged)
+                for (int i = threadIdx.x; i < bw * bh; i += NM_THREADS) {
+                    int r = i / bw;
+                    boxs[i] = tg[(size_t)(by0 + r) * tw + bx0 + (i - r * bw)];
+                }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int x = X0 + tx, y = Y0 + ty + 16 * r;
+                rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
             }
         }
         // per point: the thread's 4 pixels, summed over the wave, added to the
         // wave's accumulators (one lane per wave writes them: no race)
-        float rv[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int x = X0 + tx, y = Y0 + ty + 16 * r;
-            rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
-            const int i = threadIdx.x + u * NM_THREADS;
-            if (i < nbox) boxs[i] = sv[u];
-        }
-        __syncthreads();
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         for (int p = 0; p < np; p++) {
             Inv9 iv;
@@ -479,18 +480,28 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
     const int nv = 3 * np;
     double *pp = part + (size_t)slot * gridDim.x * 3 * NM_MAXP;
     if (threadIdx.x == 0) {
-        for (int k = 0; k < nv; k++) pp[(size_t)blockIdx.x * 3 * NM_MAXP + k] = sh.sums[k];
+        // The partials are agent-scope atomic stores, written through to the
+        // level every XCD reads coherently; the workgroup release fence waits
+        // for them to complete (s_waitcnt) before the arrival count is raised,
+        // and every reader loads them with agent-scope atomic loads once it has
+        // seen the count.  So no XCD's L2 has to be written back or invalidated
+        // for them: an agent-scope release / acquire does both to the whole L2
+        // on every pass, which here (the side work's dirty lines in it, the
+        // level's samples lost from it) cost more than a small level's pass.
+        for (int k = 0; k < nv; k++)
+            __hip_atomic_store(&pp[(size_t)blockIdx.x * 3 * NM_MAXP + k], sh.sums[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         // two-level arrival: NM_GROUPS counters (one cache line each), then one
         // counter of completed groups, which every block waits on (bounded, and
-        // left at once when any block has raised the error flag).  The partials
-        // are released by the group increment; the group's last arriver
-        // acquires its group's and releases them on to the top counter; every
-        // block acquires them all after its wait.
+        // left at once when any block has raised the error flag).  A group's
+        // last arriver has seen every increment of its group (each made after
+        // that block's partials were complete) when it raises the top count.
         const unsigned gs = gridDim.x / NM_GROUPS;
         unsigned a = __hip_atomic_fetch_add(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u,
-                                            __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (a == (gen + 1) * gs - 1)
-            __hip_atomic_fetch_add(&ctr[NM_FL_TOP], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ctr[NM_FL_TOP], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned target = (gen + 1) * NM_GROUPS;
         unsigned spins = 0;
         while (__hip_atomic_load(&ctr[NM_FL_TOP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -501,7 +512,6 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
                 break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     {
@@ -522,9 +532,10 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int i = i0 + u * NM_THREADS + threadIdx.x;
-                    // plain loads: the acquire after the wait ordered every block's partials before
-                    // them (and invalidated stale lines), so a block's L2 serves its XCD's other blocks
-                    v[u] = i < total ? pp[(size_t)(i / kn) * 3 * NM_MAXP + k0 + i % kn] : 0.0;
+                    // agent-scope atomic loads, issued after the wait has seen every arrival
+                    v[u] = i < total ? __hip_atomic_load(&pp[(size_t)(i / kn) * 3 * NM_MAXP + k0 + i % kn],
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -590,10 +601,29 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, int lvl, b
         for (int i = 0; i <= n; i++)
             for (int k = 0; k < n; k++) sh.PT[i][k] = sh.S[i][k];
     }
+    // a solo level whose target and reference fit the window buffer together
+    // is staged once (target, then reference) and read from there by every pass
+    const int npx = K.lw[lvl] * K.lh[lvl];
+    const bool resident = solo && 2 * npx <= NM_BOX;
+    if (resident) {
+        const float *tg = K.tgt[lvl], *rf = K.ref[lvl];
+        constexpr int SU = NM_BOX / NM_THREADS;
+        float v[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int i = threadIdx.x + u * NM_THREADS;
+            if (i < 2 * npx) v[u] = i < npx ? tg[i] : rf[i - npx];
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int i = threadIdx.x + u * NM_THREADS;
+            if (i < 2 * npx) boxs[i] = v[u];
+        }
+    }
     __syncthreads();
     int state = INIT, np = n + 1, it = 0;
     for (;;) {
-        nm_pass(K, sh, boxs, lvl, np, solo, part, ctr, err, gen);
+        nm_pass(K, sh, boxs, lvl, np, solo, resident, part, ctr, err, gen);
         int next = -1;  // a follow-up pass within the iteration, or none
         if (state == INIT) {
             if (t0) {
@@ -1227,9 +1257,10 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
 // of the same grid (inside a captured graph), which every block of fits at
 // once on an otherwise idle device; the kernel's bounded waits end it with an
 // error rather than a hang if that ever fails.
-static void search_launch(lsh_frames *f, bool plain = false) {
+static void search_launch(lsh_frames *f, bool plain = false, hipEvent_t started = nullptr) {
     unsigned *ctr = f->nm_flags.p, *err = f->nm_flags.p + NM_FL_ERR;
     launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p);
+    if (started) LSH_CHECK(hipEventRecord(started, stream()));
     const NmConst *kp = (const NmConst *)f->nm_const.p;
     double *state = f->nm_state.p, *part = f->nm_part.p;
     unsigned grid = f->nm_grid;
@@ -1356,14 +1387,15 @@ void frames_target(lsh_frames *f, const uint16_t *src, int slot) {
     build_reg_pyramid(f, dst);
 }
 
-void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search) {
+void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search,
+                  hipEvent_t started) {
     const size_t w = f->w, h = f->h;
     const float *tgt = f->yts[slot].p;
     // the levels the search reads (the finest searched and coarser) into its target buffer
     const size_t fin = f->loff[std::min<size_t>(f->coarsen, f->lw.size() - 1)];
     copy_mem(f->yt.p + fin, LSH_DEVICE, tgt + fin, LSH_DEVICE, (f->yt.n - fin) * sizeof(float));
     FrameDev *fp = (FrameDev *)f->fdev.p;
-    search_launch(f, plain_search);
+    search_launch(f, plain_search, started);
     launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p, (int)w, (int)h, fp);
     const Inv9 unused{};
     const Gains ug{};

@@ -79,11 +79,15 @@ void frames_record(lsh_frames *f);
 // into the search's buffer, the search from the previous Sim, the map and
 // gains, the warp into planes_out (and the warped luma, the next reference),
 // the reference pyramid.  The focus slice and the fold of planes_out are left
-// to the caller, who may run them beside the next frame's chain.
+// to the caller, who may run them beside the next frame's chain.  started (if
+// given) is recorded between the search's reset and the search itself: work
+// another stream orders after it is dispatched after the search, so it cannot
+// take the CUs the search's grid needs at its start.
 void frames_stream_slots(lsh_frames *f);
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out);  // depth.hip
 void frames_target(lsh_frames *f, const uint16_t *src, int slot);
-void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search);
+void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search,
+                  hipEvent_t started = nullptr);
 void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals);
 // fuse.hip: a push whose frame index comes from device memory (a replayed graph)
 void fuser_reserve(lsh_fuser *f, size_t nframes);

@@ -157,8 +157,8 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                    e_side[2] = {E.make(), E.make()};
         const bool bright = brightness != 0;
         // the pieces; each goes to the stream current when it is called
-        auto chain = [&](uint32_t k, bool plain) {
-            frames_chain(f, raw(k), (int)(k & 1), planes[k & 1], bright, plain);
+        auto chain = [&](uint32_t k, bool plain, hipEvent_t started) {
+            frames_chain(f, raw(k), (int)(k & 1), planes[k & 1], bright, plain, started);
             frames_record(f);
         };
         auto side = [&](uint32_t k) {  // the focus slice and the fold of frame k
@@ -183,33 +183,40 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             }
         }
         if (!graph) {
+            // frame k-1's side work and frame k+1's target follow frame k's
+            // search reset on the side stream (dispatched behind the search, and
+            // after frame k-1's chain, the last reader of slot (k+1) % 2)
+            const hipEvent_t e_started = E.make();
             for (uint32_t k = 1; k < n; k++) {
                 LSH_CHECK(hipStreamWaitEvent(Ms, e_target[k & 1], 0));
                 if (k >= 3) LSH_CHECK(hipStreamWaitEvent(Ms, e_side[k & 1], 0));  // planes[k % 2] read by frame k-2
-                chain(k, true);
-                LSH_CHECK(hipEventRecord(e_warp[k & 1], Ms));
+                chain(k, true, e_started);
+                if (k < 2) continue;
                 OnStream on(Ss);
-                LSH_CHECK(hipStreamWaitEvent(Ss, e_warp[k & 1], 0));
-                side(k);
-                LSH_CHECK(hipEventRecord(e_side[k & 1], Ss));
-                if (k + 2 < n) {
-                    target(k + 2);
-                    LSH_CHECK(hipEventRecord(e_target[k & 1], Ss));
+                LSH_CHECK(hipStreamWaitEvent(Ss, e_started, 0));
+                side(k - 1);
+                LSH_CHECK(hipEventRecord(e_side[(k - 1) & 1], Ss));
+                if (k + 1 < n) {
+                    target(k + 1);
+                    LSH_CHECK(hipEventRecord(e_target[(k + 1) & 1], Ss));
                 }
             }
+            LSH_CHECK(hipEventRecord(e_warp[0], Ms));  // the last frame's side work
+            OnStream on(Ss);
+            LSH_CHECK(hipStreamWaitEvent(Ss, e_warp[0], 0));
+            side(n - 1);
         } else {
             // frame 1 and the last frame run eagerly (their cycles differ); frames 2 .. n-2 replay a graph
             // holding frame k's chain beside frame k-1's side work and frame k+1's target pyramid
             LSH_CHECK(hipStreamWaitEvent(Ms, e_target[1], 0));
-            chain(1, true);
+            chain(1, true, nullptr);
             Graphs G;
             // the host-side counts the captured cycle advances once per replay
             auto cycle = [&](uint32_t k) {
-                LSH_CHECK(hipEventRecord(e_main, Ms));  // fork
-                LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
-                chain(k, true);
+                chain(k, true, e_main);  // fork after search k's reset
                 {
                     OnStream on(Ss);
+                    LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
                     side(k - 1);
                     if (k + 1 < n) frames_target(f, raw(k + 1), (int)((k + 1) & 1));
                     LSH_CHECK(hipEventRecord(e_side[0], Ss));  // join
