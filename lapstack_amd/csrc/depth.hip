@@ -264,7 +264,7 @@ __global__ void k_up_guided(const float *__restrict__ a, const float *__restrict
 // Horizontal window sums of two planes in f64, rounded to f32.  src1 is a
 // product plane when mul is set: src1[i] = g[i] * src0[i] (or g[i]*g[i] when
 // src0 is the guide itself).
-__global__ __launch_bounds__(256) void k_box_h2(const float *__restrict__ a, const float *__restrict__ b,
+__global__ __launch_bounds__(256) void k_box_h2_direct(const float *__restrict__ a, const float *__restrict__ b,
                                                 const float *__restrict__ g, int mode, int w, int h, int r,
                                                 float *__restrict__ oa, float *__restrict__ ob) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -333,19 +333,11 @@ __device__ __forceinline__ void peak_push(const Peaks &P, size_t i, float v, uin
 //  2  guided filter step 2 fused with the peak push: q = mean(a) I + mean(b), max(q, 0), push
 //  3  guided coefficients: oa = mean(a), ob = mean(b)
 //  4  guided filter output: oa = mean(a) I + mean(b)
-__global__ __launch_bounds__(256) void k_box_v2(const float *__restrict__ ta, const float *__restrict__ tb, int w,
-                                                int h, int r, int mode, const float *__restrict__ guide,
-                                                const float *__restrict__ mean_i, const float *__restrict__ var_i,
-                                                float eps, float *__restrict__ oa, float *__restrict__ ob, Peaks P,
-                                                uint32_t m) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= w || y >= h) return;
-    int lo = max(y - r, 0), hi = min(y + r, h - 1);
-    double sa = 0.0, sb = 0.0;
-    for (int i = lo; i <= hi; i++) {
-        sa += (double)ta[(size_t)i * w + x];
-        sb += (double)tb[(size_t)i * w + x];
-    }
+// The per-output tail of the vertical pass (sa, sb the window sums at (x, y)).
+__device__ __forceinline__ void box_v_tail(int x, int y, int w, int h, int r, int mode, double sa, double sb,
+                                           const float *__restrict__ guide, const float *__restrict__ mean_i,
+                                           const float *__restrict__ var_i, float eps, float *__restrict__ oa,
+                                           float *__restrict__ ob, const Peaks &P, uint32_t m) {
     float ic = inv_count(x, y, w, h, r);
     float ma = (float)sa * ic, mb = (float)sb * ic;
     size_t o = (size_t)y * w + x;
@@ -365,6 +357,119 @@ __global__ __launch_bounds__(256) void k_box_v2(const float *__restrict__ ta, co
         ob[o] = mb;
     } else {
         oa[o] = ma * guide[o] + mb;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_box_v2_direct(const float *__restrict__ ta, const float *__restrict__ tb,
+                                                       int w, int h, int r, int mode, const float *__restrict__ guide,
+                                                       const float *__restrict__ mean_i,
+                                                       const float *__restrict__ var_i, float eps,
+                                                       float *__restrict__ oa, float *__restrict__ ob, Peaks P,
+                                                       uint32_t m) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    int lo = max(y - r, 0), hi = min(y + r, h - 1);
+    double sa = 0.0, sb = 0.0;
+    for (int i = lo; i <= hi; i++) {
+        sa += (double)ta[(size_t)i * w + x];
+        sb += (double)tb[(size_t)i * w + x];
+    }
+    box_v_tail(x, y, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
+}
+
+// The same two passes on tiles staged in LDS: a row segment of BH_TX outputs
+// and its radius-r halo (horizontal), or BV_TX columns of BV_TY outputs and
+// their 2r halo rows (vertical), loaded once with every load of a thread's
+// chunk issued before it is stored; each output then sums its window from LDS
+// with the same f64 additions in the same order as the direct kernels.
+#define BH_TX 256
+#define BV_TX 64
+#define BV_TY 32
+#define BOX_TILE_MAX_R 32
+__global__ __launch_bounds__(256) void k_box_h2_tile(const float *__restrict__ a, const float *__restrict__ b,
+                                                     const float *__restrict__ g, int mode, int w, int h, int r,
+                                                     float *__restrict__ oa, float *__restrict__ ob) {
+    extern __shared__ float S[];  // [2][BH_TX + 2 r]
+    float *Sa = S, *Sb = S + BH_TX + 2 * r;
+    const int x0 = blockIdx.x * BH_TX, y = blockIdx.y;
+    const size_t row = (size_t)y * w;
+    const int c0 = max(x0 - r, 0), nc = min(x0 + BH_TX - 1 + r, w - 1) - c0 + 1;
+    for (int q0 = threadIdx.x; q0 < nc; q0 += 4 * BH_TX) {
+        float va[4], vb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = q0 + u * BH_TX;
+            if (q < nc) {
+                const size_t i = row + c0 + q;
+                va[u] = a[i];
+                vb[u] = mode == 0 ? b[i] : (mode == 1 ? g[i] * va[u] : va[u] * va[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = q0 + u * BH_TX;
+            if (q < nc) {
+                Sa[q] = va[u];
+                Sb[q] = vb[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= w) return;
+    const int lo = max(x - r, 0), hi = min(x + r, w - 1);
+    double sa = 0.0, sb = 0.0;
+    for (int i = lo; i <= hi; i++) {
+        sa += (double)Sa[i - c0];
+        sb += (double)Sb[i - c0];
+    }
+    oa[row + x] = (float)sa;
+    ob[row + x] = (float)sb;
+}
+
+__global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ ta, const float *__restrict__ tb,
+                                                     int w, int h, int r, int mode, const float *__restrict__ guide,
+                                                     const float *__restrict__ mean_i, const float *__restrict__ var_i,
+                                                     float eps, float *__restrict__ oa, float *__restrict__ ob, Peaks P,
+                                                     uint32_t m) {
+    extern __shared__ float S[];  // [2][BV_TY + 2 r][BV_TX]
+    float *Sa = S, *Sb = S + (BV_TY + 2 * r) * BV_TX;
+    const int x0 = blockIdx.x * BV_TX, y0 = blockIdx.y * BV_TY;
+    const int tx = threadIdx.x & (BV_TX - 1), ty = threadIdx.x / BV_TX, x = x0 + tx;
+    const int r0 = max(y0 - r, 0), nr = min(y0 + BV_TY - 1 + r, h - 1) - r0 + 1;
+    if (x < w)
+        for (int q0 = ty; q0 < nr; q0 += 4 * 4) {
+            float va[4], vb[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + 4 * u;
+                if (q < nr) {
+                    const size_t o = (size_t)(r0 + q) * w + x;
+                    va[u] = ta[o];
+                    vb[u] = tb[o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + 4 * u;
+                if (q < nr) {
+                    Sa[q * BV_TX + tx] = va[u];
+                    Sb[q * BV_TX + tx] = vb[u];
+                }
+            }
+        }
+    __syncthreads();
+    if (x >= w) return;
+    for (int k = 0; k < BV_TY / 4; k++) {
+        const int y = y0 + ty + 4 * k;
+        if (y >= h) break;
+        const int lo = max(y - r, 0), hi = min(y + r, h - 1);
+        double sa = 0.0, sb = 0.0;
+        for (int i = lo; i <= hi; i++) {
+            sa += (double)Sa[(i - r0) * BV_TX + tx];
+            sb += (double)Sb[(i - r0) * BV_TX + tx];
+        }
+        box_v_tail(x, y, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
     }
 }
 
@@ -516,6 +621,32 @@ static dim3 g2(int w, int h) { return dim3(cdiv(w, 64), cdiv(h, 4)); }
 static const dim3 B2(64, 4);
 static unsigned g1(size_t n) { return cdiv(n, 256); }
 
+// The two box passes: the LDS-tiled kernels when the radius fits their tiles,
+// else the direct ones (the same sums either way).
+static void box_h2(const char *name, double bytes, const float *a, const float *b, const float *g, int mode, int w,
+                   int h, int r, float *oa, float *ob) {
+    if (r <= BOX_TILE_MAX_R && h <= 65535) {
+        const size_t shm = 2 * (size_t)(BH_TX + 2 * r) * sizeof(float);
+        launch_b(name, bytes, k_box_h2_tile, dim3(cdiv(w, BH_TX), h), dim3(BH_TX), shm, a, b, g, mode, w, h, r, oa, ob);
+    } else {
+        launch_b(name, bytes, k_box_h2_direct, g2(w, h), B2, 0, a, b, g, mode, w, h, r, oa, ob);
+    }
+}
+
+static void box_v2(const char *name, double bytes, const float *ta, const float *tb, int w, int h, int r, int mode,
+                   const float *guide, const float *mean_i, const float *var_i, float eps, float *oa, float *ob,
+                   const Peaks &P, uint32_t m) {
+    if (r <= BOX_TILE_MAX_R) {
+        const size_t shm = 2 * (size_t)(BV_TY + 2 * r) * BV_TX * sizeof(float);
+        const dim3 grid(cdiv(w, BV_TX), cdiv(h, BV_TY));
+        launch_b(name, bytes, k_box_v2_tile, grid, dim3(256), shm, ta, tb, w, h, r, mode, guide, mean_i, var_i, eps, oa,
+                 ob, P, m);
+    } else {
+        launch_b(name, bytes, k_box_v2_direct, g2(w, h), B2, 0, ta, tb, w, h, r, mode, guide, mean_i, var_i, eps, oa,
+                 ob, P, m);
+    }
+}
+
 // Guided-filter plumbing on the working grid (depth.rs:349-412)
 struct GF {
     int w, h, r;
@@ -533,32 +664,27 @@ static void gf_init(GF &g, const float *guide, int w, int h, int r, float eps) {
     size_t n = (size_t)w * h;
     for (auto *b : {&g.mean_i, &g.var_i, &g.t1, &g.t2, &g.s1, &g.s2}) b->alloc(n);
     Peaks none{};
-    launch("box_h", k_box_h2, g2(w, h), B2, 0, guide, (const float *)nullptr, (const float *)nullptr, 2, w, h, r, g.t1.p,
-           g.t2.p);
-    launch("box_v", k_box_v2, g2(w, h), B2, 0, (const float *)g.t1.p, (const float *)g.t2.p, w, h, r, 0,
-           (const float *)nullptr, (const float *)nullptr, (const float *)nullptr, 0.0f, g.mean_i.p, g.var_i.p, none, 0u);
+    box_h2("box_h", 0.0, guide, nullptr, nullptr, 2, w, h, r, g.t1.p, g.t2.p);
+    box_v2("box_v", 0.0, g.t1.p, g.t2.p, w, h, r, 0, nullptr, nullptr, nullptr, 0.0f, g.mean_i.p, g.var_i.p, none, 0u);
 }
 
 // step 1 of a filtering of p: s1 = a, s2 = b
 static void gf_step1(GF &g, const float *p) {
     Peaks none{};
     double M = (double)g.w * g.h;
-    launch_b("box_h", 16.0 * M, k_box_h2, g2(g.w, g.h), B2, 0, p, (const float *)nullptr, g.guide, 1, g.w, g.h, g.r,
-             g.t1.p, g.t2.p);
-    launch_b("box_v", 24.0 * M, k_box_v2, g2(g.w, g.h), B2, 0, (const float *)g.t1.p, (const float *)g.t2.p, g.w, g.h,
-             g.r, 1, g.guide, (const float *)g.mean_i.p, (const float *)g.var_i.p, g.eps, g.s1.p, g.s2.p, none, 0u);
+    box_h2("box_h", 16.0 * M, p, nullptr, g.guide, 1, g.w, g.h, g.r, g.t1.p, g.t2.p);
+    box_v2("box_v", 24.0 * M, g.t1.p, g.t2.p, g.w, g.h, g.r, 1, g.guide, g.mean_i.p, g.var_i.p, g.eps, g.s1.p, g.s2.p,
+           none, 0u);
 }
 
 // step 2: the box means of a and b, then per mode (2 peak push, 3 coefficients into oa/ob, 4 filtered into oa)
 static void gf_step2(GF &g, int mode, float *oa, float *ob, const Peaks &P, uint32_t m) {
     double M = (double)g.w * g.h;
-    launch_b("box_h", 16.0 * M, k_box_h2, g2(g.w, g.h), B2, 0, (const float *)g.s1.p, (const float *)g.s2.p,
-             (const float *)nullptr, 0, g.w, g.h, g.r, g.t1.p, g.t2.p);
+    box_h2("box_h", 16.0 * M, g.s1.p, g.s2.p, nullptr, 0, g.w, g.h, g.r, g.t1.p, g.t2.p);
     // mode 2: t1, t2 and the guide read, the nine-plane peak state read and written
     double by = mode == 2 ? (12.0 + 72.0) * M : 16.0 * M;
-    launch_b(mode == 2 ? "box_v_peaks" : "box_v", by, k_box_v2, g2(g.w, g.h), B2, 0, (const float *)g.t1.p,
-             (const float *)g.t2.p, g.w, g.h, g.r, mode, g.guide, (const float *)nullptr, (const float *)nullptr, 0.0f,
-             oa, ob, P, m);
+    box_v2(mode == 2 ? "box_v_peaks" : "box_v", by, g.t1.p, g.t2.p, g.w, g.h, g.r, mode, g.guide, nullptr, nullptr,
+           0.0f, oa, ob, P, m);
 }
 
 struct PeakBufs {
@@ -814,9 +940,8 @@ __global__ void k_wav_finish(const float *__restrict__ acc, const float *__restr
 static void box_mean_plane(const float *src, int w, int h, int r, float *out, float *t1, float *t2, float *spare) {
     Peaks none{};
     double M = (double)w * h;
-    launch_b("box_h", 12.0 * M, k_box_h2, g2(w, h), B2, 0, src, src, (const float *)nullptr, 0, w, h, r, t1, t2);
-    launch_b("box_v", 16.0 * M, k_box_v2, g2(w, h), B2, 0, (const float *)t1, (const float *)t2, w, h, r, 3,
-             (const float *)nullptr, (const float *)nullptr, (const float *)nullptr, 0.0f, out, spare, none, 0u);
+    box_h2("box_h", 12.0 * M, src, src, nullptr, 0, w, h, r, t1, t2);
+    box_v2("box_v", 16.0 * M, t1, t2, w, h, r, 3, nullptr, nullptr, nullptr, 0.0f, out, spare, none, 0u);
 }
 
 }  // namespace lsh
