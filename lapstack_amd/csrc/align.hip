@@ -24,14 +24,14 @@
 
 namespace lsh {
 
-void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out);  // depth.hip
 
 #define INV16 (1.0f / 65535.0f)
 
 // align.rs:562-571 on the decoded frame
-__global__ void k_luma_u16(const uint16_t *__restrict__ rgb, size_t n, float *__restrict__ y) {
+__global__ void k_luma_u16(const FrameRef src, size_t n, float *__restrict__ y) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint16_t *__restrict__ rgb = src.get();
     float r = (float)rgb[3 * i] * INV16, g = (float)rgb[3 * i + 1] * INV16, b = (float)rgb[3 * i + 2] * INV16;
     y[i] = 0.299f * r + 0.587f * g + 0.114f * b;
 }
@@ -1008,11 +1008,12 @@ __device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const 
 }
 
 template <int N>
-__global__ __launch_bounds__(256) void k_warp(const uint16_t *__restrict__ rgb, const float *__restrict__ yin, int w,
-                                              int h, Inv9 inv, int affine, int ik, Gains g, float *__restrict__ planes,
+__global__ __launch_bounds__(256) void k_warp(const FrameRef src, const float *__restrict__ yin, int w, int h, Inv9 inv,
+                                              int affine, int ik, Gains g, float *__restrict__ planes,
                                               float *__restrict__ yout, const FrameDev *__restrict__ fp) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
+    const uint16_t *__restrict__ rgb = src.get();
     if (fp) {
         inv = fp->inv;
         affine = fp->affine;
@@ -1058,7 +1059,7 @@ __global__ void k_grid(const uint16_t *__restrict__ rgb, int w, int h, int gw, i
 // is warped by inv (the full warp's value, fallback included, is recomputed at
 // each grid pixel); mode 1: the frame as decoded, every grid pixel counts.
 template <int N>
-__global__ __launch_bounds__(256) void k_bright_sums(const uint16_t *__restrict__ rgb, int w, int h, Inv9 inv,
+__global__ __launch_bounds__(256) void k_bright_sums(const FrameRef src, int w, int h, Inv9 inv,
                                                      int affine, int ik, int mode, const float *__restrict__ grid,
                                                      int gw, int gh, double *__restrict__ part,
                                                      unsigned *__restrict__ ticket, double *__restrict__ out,
@@ -1068,6 +1069,7 @@ __global__ __launch_bounds__(256) void k_bright_sums(const uint16_t *__restrict_
         affine = fp->affine;
         mode = fp->is_id;
     }
+    const uint16_t *__restrict__ rgb = src.get();
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     size_t ng = (size_t)gw * gh;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += (size_t)gridDim.x * blockDim.x) {
@@ -1106,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_bright_sums(const uint16_t *__restrict_
 
 static int taps_n(uint32_t ik) { return ik <= 1 ? 2 : (ik <= 3 ? 4 : 6); }
 
-static void warp_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int affine, const Gains &g,
+static void warp_launch(lsh_frames *f, const FrameRef &rgb, const Inv9 &inv, int affine, const Gains &g,
                         float *yout, const FrameDev *fp = nullptr, const float *yin_ = nullptr,
                         float *planes_out = nullptr) {
     dim3 gr(cdiv(f->w, 64), cdiv(f->h, 4)), b(64, 4);
@@ -1123,7 +1125,7 @@ static void warp_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int
 }
 
 // host7 null: the sums stay in red_out for a device consumer (no round trip)
-static void sums_launch(lsh_frames *f, const uint16_t *rgb, const Inv9 &inv, int affine, int mode, double *host7,
+static void sums_launch(lsh_frames *f, const FrameRef &rgb, const Inv9 &inv, int affine, int mode, double *host7,
                         const FrameDev *fp = nullptr) {
     size_t ng = f->gw * f->gh;
     unsigned grid = std::max(1u, std::min<unsigned>(cdiv(ng, 256), 1024));
@@ -1380,14 +1382,14 @@ void frames_stream_slots(lsh_frames *f) {
     f->planes_b.ensure(3 * f->w * f->h);
 }
 
-void frames_target(lsh_frames *f, const uint16_t *src, int slot) {
+void frames_target(lsh_frames *f, const FrameRef &src, int slot) {
     const size_t n = f->w * f->h;
     float *dst = f->yts[slot].p;
     launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, dst);
     build_reg_pyramid(f, dst);
 }
 
-void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search,
+void frames_chain(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness, bool plain_search,
                   hipEvent_t started) {
     const size_t w = f->w, h = f->h;
     const float *tgt = f->yts[slot].p;

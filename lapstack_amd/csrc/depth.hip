@@ -39,7 +39,8 @@ struct TapTable {
 // borders), then each thread averages the measure over CY cells of its column.
 __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k, int CY,
                                                      const TapTable tab, Taps T, float *__restrict__ out, int dw,
-                                                     int dh) {
+                                                     int dh, const unsigned *__restrict__ out_idx) {
+    if (out_idx) out += (size_t)out_idx[0] * dw * dh;
     extern __shared__ float Y[];
     const int TX = 64, TY = 4 * CY, R = T.halo;
     const int PW = TX * k + 2 * R, PH = TY * k + 2 * R;
@@ -139,15 +140,15 @@ static void rdf_taps_ordered(uint32_t r_in, uint32_t r_out, std::vector<int32_t>
 // (uploaded when the measure changes).
 static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
                              uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
-                             float *out);
+                             float *out, const unsigned *out_idx = nullptr);
 
-void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out) {
-    focus_slice_with(f->taps_dy, f->taps_dx, f->taps_wt, f->ntaps, f->taps_key, f->w, f->h, planes, p, out);
+void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out, const unsigned *out_idx) {
+    focus_slice_with(f->taps_dy, f->taps_dx, f->taps_wt, f->ntaps, f->taps_key, f->w, f->h, planes, p, out, out_idx);
 }
 
 static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
                              uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
-                             float *out) {
+                             float *out, const unsigned *out_idx) {
     LSH_REQUIRE(p.scale <= 2, "focus slice: scale above 2 is not supported");
     LSH_REQUIRE(p.focus_kind <= 1, "focus slice: unknown focus measure");
     int k = 1 << p.scale;
@@ -196,7 +197,7 @@ static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBu
         }
     }
     launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)),
-             dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY, tab, T, out, dw, dh);
+             dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY, tab, T, out, dw, dh, out_idx);
 }
 
 // ----------------------------------------------------------------- helpers

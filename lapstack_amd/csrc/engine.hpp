@@ -2,6 +2,19 @@
 #pragma once
 #include "common.hpp"
 
+// A raw frame (interleaved RGB u16) as a kernel finds it: the pointer itself,
+// or, inside a replayed graph, entry cur[0] + off of a device table of frame
+// pointers (the table and the counter stay put while the frame changes).
+struct FrameRef {
+    const uint16_t *p = nullptr;
+    const uint16_t *const *tab = nullptr;
+    const unsigned *cur = nullptr;
+    int off = 0;
+    FrameRef(const uint16_t *q = nullptr) : p(q) {}
+    FrameRef(const uint16_t *const *t, const unsigned *c, int o) : tab(t), cur(c), off(o) {}
+    __device__ __forceinline__ const uint16_t *get() const { return tab ? tab[*cur + off] : p; }
+};
+
 // The frames handle (the reference's GpuFrames, gpu.rs:636-900): the current
 // frame's planes, its registration luma and pyramid, the previous aligned
 // frame's luma pyramid (the next frame's search reference), and frame 0's
@@ -84,9 +97,11 @@ void frames_record(lsh_frames *f);
 // another stream orders after it is dispatched after the search, so it cannot
 // take the CUs the search's grid needs at its start.
 void frames_stream_slots(lsh_frames *f);
-void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out);  // depth.hip
-void frames_target(lsh_frames *f, const uint16_t *src, int slot);
-void frames_chain(lsh_frames *f, const uint16_t *src, int slot, float *planes_out, bool brightness, bool plain_search,
+// out_idx (if given): the slice goes to out + out_idx[0] * (its cell count)
+void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out,
+                 const unsigned *out_idx = nullptr);  // depth.hip
+void frames_target(lsh_frames *f, const FrameRef &src, int slot);
+void frames_chain(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness, bool plain_search,
                   hipEvent_t started = nullptr);
 void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals);
 // fuse.hip: a push whose frame index comes from device memory (a replayed graph)

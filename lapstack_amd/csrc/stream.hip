@@ -20,9 +20,12 @@
 // graph == 0: two streams and events.  graph != 0: frame k >= 2 (with a frame
 // after it) is one replay of a captured hipGraph with two branches -- frame
 // k's chain beside frame k-1's side work -- one graph per parity of k (the
-// buffers alternate); the raw frames are staged into two fixed buffers, the
-// slices leave through two fixed buffers, and the fold takes its frame index
-// from a device counter only the side work advances.
+// buffers alternate).  A replay finds its frames in a device table of the
+// caller's frame pointers at a device counter the graph advances once per
+// replay (host frames are staged into two fixed buffers instead), the fold
+// takes its frame index from a device counter only the side work advances,
+// and the focus slice writes the caller's device slices at that index (host
+// slices leave through two fixed buffers).
 #include <cstdlib>
 #include <vector>
 
@@ -123,19 +126,33 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         Events E;
         float *planes[2] = {f->planes.p, f->planes_b.p};
         const size_t fbytes = 3 * npx * sizeof(uint16_t);
-        // raw frames: the caller's device pointers, or staged into two fixed buffers (host frames, graphs)
-        const bool staged = graph || frames_mem != LSH_DEVICE;
+        // raw frames: the caller's device pointers (read through a device table
+        // of them inside a replayed graph), or host frames staged into two
+        // fixed device buffers
+        const bool staged = frames_mem != LSH_DEVICE;
         DBuf<uint16_t> stage[2];
         if (staged) {
             stage[0].alloc(3 * npx);
             stage[1].alloc(3 * npx);
         }
+        DBuf<const uint16_t *> ftab;
+        DBuf<unsigned> fcur(1);  // a replay's frame index (the graph's chain frame)
+        if (graph && !staged) {
+            ftab.alloc(n);
+            copy_mem(ftab.p, LSH_DEVICE, frames, LSH_HOST, n * sizeof(const uint16_t *));
+        }
         auto raw = [&](uint32_t k) -> const uint16_t * { return staged ? stage[k & 1].p : frames[k]; };
+        // frame k as a replayed cycle reads it: off = k - the cycle's frame index
+        auto cyc = [&](uint32_t k, int off) -> FrameRef {
+            return staged ? FrameRef(raw(k)) : FrameRef(ftab.p, fcur.p, off);
+        };
         auto put = [&](uint32_t k) {
             if (staged) copy_mem(stage[k & 1].p, LSH_DEVICE, frames[k], frames_mem, fbytes);
         };
-        // slices: in place in the caller's device buffer (launches), else through device buffers copied out
-        const bool direct = measure && slices_mem == LSH_DEVICE && !graph;
+        // slices: in place in the caller's device buffer (launches: at frame k's
+        // offset; a graph: at the fold index's), else through device buffers
+        // copied out
+        const bool direct = measure && slices_mem == LSH_DEVICE;
         DBuf<float> sall, sbuf[2];
         if (measure && !direct) {
             if (graph) {
@@ -147,7 +164,7 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         }
         auto slice_at = [&](uint32_t k) -> float * {
             if (!measure) return nullptr;
-            if (direct) return slices + (size_t)k * M;
+            if (direct) return graph ? slices : slices + (size_t)k * M;
             return graph ? sbuf[k & 1].p : sall.p + (size_t)k * M;
         };
         DBuf<unsigned> fidx(1);  // the fold's frame index (side work only)
@@ -157,12 +174,12 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                    e_side[2] = {E.make(), E.make()};
         const bool bright = brightness != 0;
         // the pieces; each goes to the stream current when it is called
-        auto chain = [&](uint32_t k, bool plain, hipEvent_t started) {
-            frames_chain(f, raw(k), (int)(k & 1), planes[k & 1], bright, plain, started);
+        auto chain = [&](uint32_t k, bool plain, hipEvent_t started, const FrameRef &src) {
+            frames_chain(f, src, (int)(k & 1), planes[k & 1], bright, plain, started);
             frames_record(f);
         };
-        auto side = [&](uint32_t k) {  // the focus slice and the fold of frame k
-            if (measure) focus_slice(f, planes[k & 1], *measure, slice_at(k));
+        auto side = [&](uint32_t k) {  // the focus slice and the fold of frame k (the fold index is k)
+            if (measure) focus_slice(f, planes[k & 1], *measure, slice_at(k), graph && direct ? fidx.p : nullptr);
             fuser_push_dev(fu, planes[k & 1], fidx.p);
             launch("stream_fold_next", k_counter_next, dim3(1), dim3(64), 0, fidx.p);
         };
@@ -190,7 +207,7 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             for (uint32_t k = 1; k < n; k++) {
                 LSH_CHECK(hipStreamWaitEvent(Ms, e_target[k & 1], 0));
                 if (k >= 3) LSH_CHECK(hipStreamWaitEvent(Ms, e_side[k & 1], 0));  // planes[k % 2] read by frame k-2
-                chain(k, true, e_started);
+                chain(k, true, e_started, raw(k));
                 if (k < 2) continue;
                 OnStream on(Ss);
                 LSH_CHECK(hipStreamWaitEvent(Ss, e_started, 0));
@@ -209,24 +226,27 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             // frame 1 and the last frame run eagerly (their cycles differ); frames 2 .. n-2 replay a graph
             // holding frame k's chain beside frame k-1's side work and frame k+1's target pyramid
             LSH_CHECK(hipStreamWaitEvent(Ms, e_target[1], 0));
-            chain(1, true, nullptr);
+            chain(1, true, nullptr, raw(1));
             Graphs G;
+            // device frames: a replay reads frame k + off as entry fcur + off of the table
+            launch("stream_frame_index", k_counter_set, dim3(1), dim3(64), 0, fcur.p, 2u);
             // the host-side counts the captured cycle advances once per replay
             auto cycle = [&](uint32_t k) {
-                chain(k, true, e_main);  // fork after search k's reset
+                chain(k, true, e_main, cyc(k, 0));  // fork after search k's reset
                 {
                     OnStream on(Ss);
                     LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
                     side(k - 1);
-                    if (k + 1 < n) frames_target(f, raw(k + 1), (int)((k + 1) & 1));
+                    if (k + 1 < n) frames_target(f, cyc(k + 1, 1), (int)((k + 1) & 1));
                     LSH_CHECK(hipEventRecord(e_side[0], Ss));  // join
                 }
                 LSH_CHECK(hipStreamWaitEvent(Ms, e_side[0], 0));
+                if (!staged) launch("stream_frame_next", k_counter_next, dim3(1), dim3(64), 0, fcur.p);
             };
             LSH_CHECK(hipStreamWaitEvent(Ms, e_target[0], 0));  // the prologue's side work is done
             for (uint32_t k = 2; k < n; k++) {
                 if (k + 1 < n) {
-                    put(k + 1);  // on the main stream, before the replay (its slot was read by frame k-1)
+                    put(k + 1);  // host frames: on the main stream, before the replay (slot read by frame k-1)
                     const int p = (int)(k & 1);
                     if (!G.x[p]) {
                         prof().capturing = true;
@@ -251,8 +271,9 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                 } else {
                     cycle(k);  // the last frame: no target after it
                 }
-                if (measure) copy_mem(slices + (size_t)(k - 1) * M, slices_mem, sbuf[(k - 1) & 1].p, LSH_DEVICE,
-                                      M * sizeof(float));
+                if (measure && !direct)
+                    copy_mem(slices + (size_t)(k - 1) * M, slices_mem, sbuf[(k - 1) & 1].p, LSH_DEVICE,
+                             M * sizeof(float));
             }
             // the last frame's side work
             LSH_CHECK(hipEventRecord(e_main, Ms));
@@ -263,8 +284,9 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                 LSH_CHECK(hipEventRecord(e_side[1], Ss));
             }
             LSH_CHECK(hipStreamWaitEvent(Ms, e_side[1], 0));
-            if (measure) copy_mem(slices + (size_t)(n - 1) * M, slices_mem, sbuf[(n - 1) & 1].p, LSH_DEVICE,
-                                  M * sizeof(float));
+            if (measure && !direct)
+                copy_mem(slices + (size_t)(n - 1) * M, slices_mem, sbuf[(n - 1) & 1].p, LSH_DEVICE,
+                         M * sizeof(float));
         }
         sync();
         if (measure && !direct && !graph)

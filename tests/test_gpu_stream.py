@@ -81,3 +81,27 @@ def test_stream_device_buffers_and_single_frame():
     s = la.stack_stream(frames, la.FuseParams(levels=4), la.AlignParams(coarsen=0, interp="bilinear"), None,
                         img_out=img, winner_out=win)
     assert float(img.max()) > 0.0 and s["evals"][1] > 0
+
+
+@pytest.mark.parametrize("n", [3, 8])
+def test_graph_reads_frames_through_the_table_and_writes_device_slices(n):
+    """A replayed graph takes device frames through its device table of frame
+    pointers and writes device slices at the fold's index; host frames go
+    through the staging buffers.  All equal the launched path bit for bit."""
+    import torch
+
+    print(f"graph table {n}", flush=True)
+    w, h = 288, 208
+    b = synth.Bracket(w, h, n, seed=23, jitter=True, device="cuda")
+    frames = [b.frame(i) for i in range(n)]
+    fp, ap, dp = la.FuseParams(levels=5), la.AlignParams(coarsen=1, interp="bilinear"), la.DepthParams()
+    dw, dh = dp.grid(w, h)
+    ref = la.stack_stream(frames, fp, ap, dp, graph=False)
+    sd = torch.full((n, dh, dw), -1.0, dtype=torch.float32, device="cuda")
+    g = la.stack_stream(frames, fp, ap, dp, graph=True, slices_out=sd)
+    hf = la.stack_stream([f.cpu().numpy() for f in frames], fp, ap, dp, graph=True)
+    for k in ("image", "winner", "sims", "gains", "evals"):
+        assert np.array_equal(np.asarray(g[k]), np.asarray(ref[k])), k
+        assert np.array_equal(np.asarray(hf[k]), np.asarray(ref[k])), k
+    assert np.array_equal(sd.cpu().numpy(), np.asarray(ref["slices"]))
+    assert np.array_equal(np.asarray(hf["slices"]), np.asarray(ref["slices"]))
