@@ -932,13 +932,20 @@ struct FrameDev {
     int affine, is_id;
     Gains g;
     float gains[3];
+    double sim[8], evals;  // the searched Sim and its evaluation count, for the record
+    unsigned err;          // the search's error flag as it ended
 };
 
-// align.rs:241-286: the Sim the search left in state[0..8), inverted (device sin/cos)
-__global__ void k_frame_inv(const double *__restrict__ state, int w, int h, FrameDev *__restrict__ fp) {
+// align.rs:241-286: the Sim the search left in state[0..8), inverted (device
+// sin/cos); the Sim, its evaluation count and the search's error flag are kept
+// with the map for the frame's record, which may run after the next search
+__global__ void k_frame_inv(const double *__restrict__ state, const unsigned *__restrict__ err, int w, int h,
+                            FrameDev *__restrict__ fp) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double s[8];
-    for (int k = 0; k < 8; k++) s[k] = state[k];
+    for (int k = 0; k < 8; k++) s[k] = fp->sim[k] = state[k];
+    fp->evals = state[8];
+    fp->err = *err;
     sim_inverse_dev(s, w, h, fp->inv.m);
     fp->affine = fp->inv.m[6] == 0.0 && fp->inv.m[7] == 0.0 && fp->inv.m[8] == 1.0;
     fp->is_id = s[0] == 0.0 && s[1] == 0.0 && s[2] == 1.0 && s[3] == 0.0 && s[4] == 1.0 && s[5] == 0.0 &&
@@ -973,7 +980,7 @@ __global__ void k_frame_gains(const double *__restrict__ sums, int enabled, Fram
 // one pass over the taps, each sum in warp_sample's order, every load
 // independent of whether the point is in the frame (clamped indices are always
 // in range), so a thread's loads are all in flight together.
-template <int N, bool LUMA>
+template <int N, bool LUMA, bool RGB>
 __device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const float *__restrict__ yin, int w, int h,
                                         size_t o, bool valid, int x0, int y0, const float *wx, const float *wy,
                                         const Gains &g, float *__restrict__ planes, float *__restrict__ yout) {
@@ -981,7 +988,7 @@ __device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const 
     const size_t n = (size_t)w * h;
     float raw[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) raw[c] = (float)rgb[3 * o + c] * INV16;
+    for (int c = 0; c < 3; c++) raw[c] = RGB ? (float)rgb[3 * o + c] * INV16 : 0.0f;
     float acc[3] = {0.0f, 0.0f, 0.0f}, ay = 0.0f;
 #pragma unroll
     for (int j = 0; j < N; j++) {
@@ -990,20 +997,22 @@ __device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const 
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const size_t q = row + min(max(x0 + start + i, 0), w - 1);
+            if (RGB)
 #pragma unroll
-            for (int c = 0; c < 3; c++) r[c] += wx[i] * ((float)rgb[3 * q + c] * INV16);
+                for (int c = 0; c < 3; c++) r[c] += wx[i] * ((float)rgb[3 * q + c] * INV16);
             if (LUMA) ry += wx[i] * yin[q];
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) acc[c] += wy[j] * r[c];
         if (LUMA) ay += wy[j] * ry;
     }
+    if (RGB)
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float v = valid ? acc[c] : raw[c];
-        if (g.on[c]) v *= g.g[c];
-        planes[c * n + o] = v;
-    }
+        for (int c = 0; c < 3; c++) {
+            float v = valid ? acc[c] : raw[c];
+            if (g.on[c]) v *= g.g[c];
+            planes[c * n + o] = v;
+        }
     if (LUMA) yout[o] = ay;
 }
 
@@ -1028,10 +1037,12 @@ __global__ __launch_bounds__(256) void k_warp(const FrameRef src, const float *_
     float wx[6], wy[6];
     interp_w(ik, sx - fx, wx);
     interp_w(ik, sy - fy, wy);
-    if (yout)
-        warp_px<N, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
+    if (yout && planes)
+        warp_px<N, true, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
+    else if (yout)
+        warp_px<N, true, false>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
     else
-        warp_px<N, false>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
+        warp_px<N, false, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
 }
 
 // The frame as decoded (io.rs:89), times the gains
@@ -1108,19 +1119,21 @@ __global__ __launch_bounds__(256) void k_bright_sums(const FrameRef src, int w, 
 
 static int taps_n(uint32_t ik) { return ik <= 1 ? 2 : (ik <= 3 ? 4 : 6); }
 
+// luma_only: the warped registration luma alone (no frame read, no planes)
 static void warp_launch(lsh_frames *f, const FrameRef &rgb, const Inv9 &inv, int affine, const Gains &g,
                         float *yout, const FrameDev *fp = nullptr, const float *yin_ = nullptr,
-                        float *planes_out = nullptr) {
+                        float *planes_out = nullptr, bool luma_only = false) {
     dim3 gr(cdiv(f->w, 64), cdiv(f->h, 4)), b(64, 4);
     const float *yin = yin_ ? yin_ : f->yt.p;
-    float *planes = planes_out ? planes_out : f->planes.p;
+    float *planes = luma_only ? nullptr : (planes_out ? planes_out : f->planes.p);
     int ik = (int)f->interp, w = (int)f->w, h = (int)f->h;
-    // u16 frame read, 3 f32 planes written; with the luma: luma read, warped luma written
-    double by = (double)f->w * f->h * (yout ? 26.0 : 18.0);
+    // u16 frame read, 3 f32 planes written; the luma: luma read, warped luma written
+    const double by = (double)f->w * f->h * ((planes ? 18.0 : 0.0) + (yout ? 8.0 : 0.0));
+    const char *name = planes ? "warp" : "warp_luma";
     switch (taps_n(f->interp)) {
-    case 2: launch_b("warp", by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
-    case 4: launch_b("warp", by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
-    default: launch_b("warp", by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
+    case 2: launch_b(name, by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
+    case 4: launch_b(name, by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
+    default: launch_b(name, by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
     }
 }
 
@@ -1313,15 +1326,14 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
 
 // the per-frame record of a stream: Sim, gains, evaluations at the frame
 // index, a sticky copy of the search's error flag, then the index advances
-__global__ void k_frame_record(const double *__restrict__ state, const FrameDev *__restrict__ fp,
-                               const unsigned *__restrict__ err, unsigned *__restrict__ cursor,
+__global__ void k_frame_record(const FrameDev *__restrict__ fp, unsigned *__restrict__ cursor,
                                double *__restrict__ sims, float *__restrict__ gains, double *__restrict__ evals) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const unsigned i = cursor[0];
-    for (int k = 0; k < 8; k++) sims[(size_t)i * 8 + k] = state[k];
+    for (int k = 0; k < 8; k++) sims[(size_t)i * 8 + k] = fp->sim[k];
     for (int c = 0; c < 3; c++) gains[(size_t)i * 3 + c] = fp->gains[c];
-    evals[i] = state[8];
-    cursor[1] |= *err;
+    evals[i] = fp->evals;
+    cursor[1] |= fp->err;
     cursor[0] = i + 1;
 }
 
@@ -1340,7 +1352,7 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
     LSH_REQUIRE(f->have_ref && f->processed > 0, "stream: the first frame goes through lsh_frames_process");
     search_const(f, free8, coarsen);
     f->coarsen = coarsen;
-    f->fdev.ensure(sizeof(FrameDev));
+    f->fdev.ensure(2 * sizeof(FrameDev));  // one per parity of the frame
     f->cursor.ensure(2);
     f->rec_sims.ensure(nframes * 8);
     f->rec_gains.ensure(nframes * 3);
@@ -1348,32 +1360,6 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
     f->red.ensure((size_t)std::max(1u, std::min<unsigned>(cdiv(f->gw * f->gh, 256), 1024)) * 7);
     launch("stream_cursor", k_cursor_set, dim3(1), dim3(64), 0, f->cursor.p, (unsigned)f->processed);
     launch("stream_state", k_state_set, dim3(1), dim3(64), 0, f->nm_state.p, Sim());  // frame 0 sits at the identity
-}
-
-// stack.rs:513-530 and :638-706 for one frame after the first, on the device
-// alone: luma and pyramid, the search from the previous Sim, the map and the
-// gains derived on the device, the warp (planes and the next reference), the
-// focus slice into slice_dev.
-void frames_device_step(lsh_frames *f, const uint16_t *src, bool brightness, const lsh_depth_params *measure,
-                        float *slice_dev, bool plain_search) {
-    const size_t w = f->w, h = f->h, n = w * h;
-    FrameDev *fp = (FrameDev *)f->fdev.p;
-    launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, f->yt.p);
-    build_reg_pyramid(f, f->yt.p);
-    search_launch(f, plain_search);
-    launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p, (int)w, (int)h, fp);
-    const Inv9 unused{};
-    const Gains ug{};
-    if (brightness && f->have_grid) {
-        sums_launch(f, src, unused, 1, 0, nullptr, fp);
-        launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)f->red_out.p, 1, fp);
-    } else {
-        launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)nullptr, 0, fp);
-    }
-    warp_launch(f, src, unused, 1, ug, f->yr.p, fp);
-    build_reg_pyramid(f, f->yr.p);
-    if (measure) focus_slice(f, f->planes.p, *measure, slice_dev);
-    f->processed++;
 }
 
 void frames_stream_slots(lsh_frames *f) {
@@ -1389,16 +1375,26 @@ void frames_target(lsh_frames *f, const FrameRef &src, int slot) {
     build_reg_pyramid(f, dst);
 }
 
-void frames_chain(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness, bool plain_search,
-                  hipEvent_t started) {
+void frames_chain(lsh_frames *f, int slot, bool plain_search, hipEvent_t started) {
     const size_t w = f->w, h = f->h;
     const float *tgt = f->yts[slot].p;
     // the levels the search reads (the finest searched and coarser) into its target buffer
     const size_t fin = f->loff[std::min<size_t>(f->coarsen, f->lw.size() - 1)];
     copy_mem(f->yt.p + fin, LSH_DEVICE, tgt + fin, LSH_DEVICE, (f->yt.n - fin) * sizeof(float));
-    FrameDev *fp = (FrameDev *)f->fdev.p;
+    FrameDev *fp = (FrameDev *)f->fdev.p + slot;
     search_launch(f, plain_search, started);
-    launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p, (int)w, (int)h, fp);
+    launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p,
+           (const unsigned *)(f->nm_flags.p + NM_FL_ERR), (int)w, (int)h, fp);
+    // the warped registration luma from the target's full-resolution luma in its slot
+    const Inv9 unused{};
+    const Gains ug{};
+    warp_launch(f, FrameRef(), unused, 1, ug, f->yr.p, fp, tgt, nullptr, true);
+    build_reg_pyramid(f, f->yr.p);
+    f->processed++;
+}
+
+void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness) {
+    FrameDev *fp = (FrameDev *)f->fdev.p + slot;
     const Inv9 unused{};
     const Gains ug{};
     if (brightness && f->have_grid) {
@@ -1407,15 +1403,8 @@ void frames_chain(lsh_frames *f, const FrameRef &src, int slot, float *planes_ou
     } else {
         launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)nullptr, 0, fp);
     }
-    // the warp reads the target's full-resolution luma from its slot
-    warp_launch(f, src, unused, 1, ug, f->yr.p, fp, tgt, planes_out);
-    build_reg_pyramid(f, f->yr.p);
-    f->processed++;
-}
-
-void frames_record(lsh_frames *f) {
-    launch("frame_record", k_frame_record, dim3(1), dim3(64), 0, (const double *)f->nm_state.p,
-           (const FrameDev *)f->fdev.p, (const unsigned *)(f->nm_flags.p + NM_FL_ERR), f->cursor.p, f->rec_sims.p,
+    warp_launch(f, src, unused, 1, ug, nullptr, fp, nullptr, planes_out);
+    launch("frame_record", k_frame_record, dim3(1), dim3(64), 0, (const FrameDev *)fp, f->cursor.p, f->rec_sims.p,
            f->rec_gains.p, f->rec_evals.p);
 }
 

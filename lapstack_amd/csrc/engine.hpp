@@ -11,8 +11,13 @@ struct FrameRef {
     const unsigned *cur = nullptr;
     int off = 0;
     FrameRef(const uint16_t *q = nullptr) : p(q) {}
-    FrameRef(const uint16_t *const *t, const unsigned *c, int o) : tab(t), cur(c), off(o) {}
-    __device__ __forceinline__ const uint16_t *get() const { return tab ? tab[*cur + off] : p; }
+    int mod = 0;  // > 0: the entry is (cur[0] + off) % mod (a ring of staging buffers)
+    FrameRef(const uint16_t *const *t, const unsigned *c, int o, int m = 0) : tab(t), cur(c), off(o), mod(m) {}
+    __device__ __forceinline__ const uint16_t *get() const {
+        if (!tab) return p;
+        const unsigned i = *cur + off;
+        return tab[mod ? i % mod : i];
+    }
 };
 
 // The frames handle (the reference's GpuFrames, gpu.rs:636-900): the current
@@ -84,15 +89,13 @@ namespace lsh {
 // constants for the stack, then per frame the registration, map, gains, warp
 // and measurement without a host round trip, and the per-frame record.
 void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, size_t nframes);
-void frames_device_step(lsh_frames *f, const uint16_t *src, bool brightness, const lsh_depth_params *measure,
-                        float *slice_dev, bool plain_search);
-void frames_record(lsh_frames *f);
-// The parts of a pipelined stream: a frame's target luma and pyramid into a
-// slot (yts[slot]); and its serial chain -- the searched levels of that slot
-// into the search's buffer, the search from the previous Sim, the map and
-// gains, the warp into planes_out (and the warped luma, the next reference),
-// the reference pyramid.  The focus slice and the fold of planes_out are left
-// to the caller, who may run them beside the next frame's chain.  started (if
+// The parts of a pipelined stream, slot = the frame's parity: its target luma
+// and pyramid into yts[slot]; its serial chain -- the searched levels of that
+// slot into the search's buffer, the search from the previous Sim, the map
+// (kept with the Sim in the slot's FrameDev), the warped registration luma
+// (the next frame's reference) and its pyramid; and its side warp (below).
+// The focus slice and the fold of the planes are left to the caller, who may
+// run them and the side warp beside the next frame's chain.  started (if
 // given) is recorded between the search's reset and the search itself: work
 // another stream orders after it is dispatched after the search, so it cannot
 // take the CUs the search's grid needs at its start.
@@ -101,8 +104,10 @@ void frames_stream_slots(lsh_frames *f);
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out,
                  const unsigned *out_idx = nullptr);  // depth.hip
 void frames_target(lsh_frames *f, const FrameRef &src, int slot);
-void frames_chain(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness, bool plain_search,
-                  hipEvent_t started = nullptr);
+void frames_chain(lsh_frames *f, int slot, bool plain_search, hipEvent_t started = nullptr);
+// The rest of frame k's step, off the chain: the brightness sums and gains,
+// the RGB warp into planes_out (the map of the FrameDev at slot), the record.
+void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness);
 void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals);
 // fuse.hip: a push whose frame index comes from device memory (a replayed graph)
 void fuser_reserve(lsh_fuser *f, size_t nframes);

@@ -4,11 +4,12 @@
 // brightness anchor); every later frame k runs on the device alone:
 //
 //   the chain (main stream): the searched levels of frame k's target pyramid
-//     into the search's buffer, the search from the previous Sim, the map and
-//     the gains derived on the device, the warp into planes[k % 2] (and the
-//     warped luma, frame k+1's reference), the reference pyramid, the record;
-//   the side work: the focus slice and the fold of planes[k % 2], and the
-//     target luma and pyramid of frame k+2 into slot k % 2.
+//     into the search's buffer, the search from the previous Sim, the map
+//     derived on the device, the warped registration luma (frame k+1's
+//     reference) and its pyramid -- nothing else the next search waits for;
+//   the side work: frame k's brightness gains, its RGB warp into planes[k % 2]
+//     and its record, its focus slice and fold, and the target luma and
+//     pyramid of frame k+1 into slot (k+1) % 2.
 //
 // Only the chain is serial across frames (frame k+1 registers against frame
 // k's warped luma), so frame k's side work runs beside frame k+1's chain: the
@@ -129,25 +130,24 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         // raw frames: the caller's device pointers (read through a device table
         // of them inside a replayed graph), or host frames staged into two
         // fixed device buffers
+        // host frames: a ring of three staging buffers (frame k in stage[k % 3]:
+        // its side work, its target and the next frame's are in flight at once)
         const bool staged = frames_mem != LSH_DEVICE;
-        DBuf<uint16_t> stage[2];
-        if (staged) {
-            stage[0].alloc(3 * npx);
-            stage[1].alloc(3 * npx);
-        }
+        DBuf<uint16_t> stage[3];
+        if (staged)
+            for (auto &b : stage) b.alloc(3 * npx);
         DBuf<const uint16_t *> ftab;
         DBuf<unsigned> fcur(1);  // a replay's frame index (the graph's chain frame)
-        if (graph && !staged) {
-            ftab.alloc(n);
-            copy_mem(ftab.p, LSH_DEVICE, frames, LSH_HOST, n * sizeof(const uint16_t *));
+        if (graph) {
+            const uint16_t *ring[3] = {stage[0].p, stage[1].p, stage[2].p};
+            ftab.alloc(staged ? 3 : n);
+            copy_mem(ftab.p, LSH_DEVICE, staged ? ring : frames, LSH_HOST, ftab.n * sizeof(const uint16_t *));
         }
-        auto raw = [&](uint32_t k) -> const uint16_t * { return staged ? stage[k & 1].p : frames[k]; };
+        auto raw = [&](uint32_t k) -> const uint16_t * { return staged ? stage[k % 3].p : frames[k]; };
         // frame k as a replayed cycle reads it: off = k - the cycle's frame index
-        auto cyc = [&](uint32_t k, int off) -> FrameRef {
-            return staged ? FrameRef(raw(k)) : FrameRef(ftab.p, fcur.p, off);
-        };
+        auto cyc = [&](int off) -> FrameRef { return FrameRef(ftab.p, fcur.p, off, staged ? 3 : 0); };
         auto put = [&](uint32_t k) {
-            if (staged) copy_mem(stage[k & 1].p, LSH_DEVICE, frames[k], frames_mem, fbytes);
+            if (staged) copy_mem(stage[k % 3].p, LSH_DEVICE, frames[k], frames_mem, fbytes);
         };
         // slices: in place in the caller's device buffer (launches: at frame k's
         // offset; a graph: at the fold index's), else through device buffers
@@ -174,11 +174,11 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                    e_side[2] = {E.make(), E.make()};
         const bool bright = brightness != 0;
         // the pieces; each goes to the stream current when it is called
-        auto chain = [&](uint32_t k, bool plain, hipEvent_t started, const FrameRef &src) {
-            frames_chain(f, src, (int)(k & 1), planes[k & 1], bright, plain, started);
-            frames_record(f);
-        };
-        auto side = [&](uint32_t k) {  // the focus slice and the fold of frame k (the fold index is k)
+        auto chain = [&](uint32_t k, bool plain, hipEvent_t started) { frames_chain(f, (int)(k & 1), plain, started); };
+        // frame k's side work: its gains, RGB warp and record, its focus slice
+        // and its fold (the fold index is k)
+        auto side = [&](uint32_t k, const FrameRef &src) {
+            frames_side_warp(f, src, (int)(k & 1), planes[k & 1], bright);
             if (measure) focus_slice(f, planes[k & 1], *measure, slice_at(k), graph && direct ? fidx.p : nullptr);
             fuser_push_dev(fu, planes[k & 1], fidx.p);
             launch("stream_fold_next", k_counter_next, dim3(1), dim3(64), 0, fidx.p);
@@ -206,12 +206,13 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             const hipEvent_t e_started = E.make();
             for (uint32_t k = 1; k < n; k++) {
                 LSH_CHECK(hipStreamWaitEvent(Ms, e_target[k & 1], 0));
-                if (k >= 3) LSH_CHECK(hipStreamWaitEvent(Ms, e_side[k & 1], 0));  // planes[k % 2] read by frame k-2
-                chain(k, true, e_started, raw(k));
+                // FrameDev k % 2: frame k-2's side work reads it
+                if (k >= 3) LSH_CHECK(hipStreamWaitEvent(Ms, e_side[k & 1], 0));
+                chain(k, true, e_started);
                 if (k < 2) continue;
                 OnStream on(Ss);
                 LSH_CHECK(hipStreamWaitEvent(Ss, e_started, 0));
-                side(k - 1);
+                side(k - 1, raw(k - 1));
                 LSH_CHECK(hipEventRecord(e_side[(k - 1) & 1], Ss));
                 if (k + 1 < n) {
                     target(k + 1);
@@ -221,32 +222,32 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             LSH_CHECK(hipEventRecord(e_warp[0], Ms));  // the last frame's side work
             OnStream on(Ss);
             LSH_CHECK(hipStreamWaitEvent(Ss, e_warp[0], 0));
-            side(n - 1);
+            side(n - 1, raw(n - 1));
         } else {
             // frame 1 and the last frame run eagerly (their cycles differ); frames 2 .. n-2 replay a graph
             // holding frame k's chain beside frame k-1's side work and frame k+1's target pyramid
             LSH_CHECK(hipStreamWaitEvent(Ms, e_target[1], 0));
-            chain(1, true, nullptr, raw(1));
+            chain(1, true, nullptr);
             Graphs G;
             // device frames: a replay reads frame k + off as entry fcur + off of the table
             launch("stream_frame_index", k_counter_set, dim3(1), dim3(64), 0, fcur.p, 2u);
             // the host-side counts the captured cycle advances once per replay
             auto cycle = [&](uint32_t k) {
-                chain(k, true, e_main, cyc(k, 0));  // fork after search k's reset
+                chain(k, true, e_main);  // fork after search k's reset
                 {
                     OnStream on(Ss);
                     LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
-                    side(k - 1);
-                    if (k + 1 < n) frames_target(f, cyc(k + 1, 1), (int)((k + 1) & 1));
+                    side(k - 1, cyc(-1));
+                    if (k + 1 < n) frames_target(f, cyc(1), (int)((k + 1) & 1));
                     LSH_CHECK(hipEventRecord(e_side[0], Ss));  // join
                 }
                 LSH_CHECK(hipStreamWaitEvent(Ms, e_side[0], 0));
-                if (!staged) launch("stream_frame_next", k_counter_next, dim3(1), dim3(64), 0, fcur.p);
+                launch("stream_frame_next", k_counter_next, dim3(1), dim3(64), 0, fcur.p);
             };
             LSH_CHECK(hipStreamWaitEvent(Ms, e_target[0], 0));  // the prologue's side work is done
             for (uint32_t k = 2; k < n; k++) {
                 if (k + 1 < n) {
-                    put(k + 1);  // host frames: on the main stream, before the replay (slot read by frame k-1)
+                    put(k + 1);  // host frames: before the replay (its ring slot was last read by frame k-2)
                     const int p = (int)(k & 1);
                     if (!G.x[p]) {
                         prof().capturing = true;
@@ -280,7 +281,7 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             {
                 OnStream on(Ss);
                 LSH_CHECK(hipStreamWaitEvent(Ss, e_main, 0));
-                side(n - 1);
+                side(n - 1, raw(n - 1));
                 LSH_CHECK(hipEventRecord(e_side[1], Ss));
             }
             LSH_CHECK(hipStreamWaitEvent(Ms, e_side[1], 0));
