@@ -266,6 +266,9 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_MAXP 9     // cost evaluations per collective pass
 #define NM_TILE 64    // a pass walks 64-pixel-wide destination tiles of 16, 32 or 64 rows
 #define NM_BOX 8192   // floats of a tile's staged source window
+#define NM_HELD 2     // tiles a block holds for a whole level (levels of 16- or 32-row tiles)
+#define NM_HSLOT 8192 // floats per held tile: its reference pixels, then its source window
+#define NM_HMARGIN 6  // pixels a held window extends past the first pass's bound
 // the state words: the Sim, the evaluation total, per level the evaluations and
 // block 0's ticks, the four phase tick totals, per level the passes
 #define NM_STATE (13 + 3 * NM_MAXL)
@@ -277,6 +280,7 @@ struct NmShared {
     unsigned lev_evals[NM_MAXL], lev_passes[NM_MAXL];
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
+    int hwin[NM_HELD][5];   // the held tiles' windows (x0, y0, w, h, staged)
     int stop;
 };
 
@@ -303,105 +307,111 @@ __device__ __forceinline__ void zero_waves(NmShared &sh) {
     for (int i = threadIdx.x; i < 3 * NM_MAXP * NM_WAVES; i += NM_THREADS) (&sh.red[0][0])[i] = 0.0;
 }
 
-// One collective pass: the costs (align.rs:581-635) of the np points
-// sh.PT[0..np) at level lvl into sh.PC[0..np).  The level is walked in 64 x 64
-// destination tiles; the tile's source window (the union over the points of
-// the bounding box of the tile corners' source points, plus the 4 x 4 support)
-// is staged in LDS when it fits, and every tap inside it is read from there,
-// any other from global memory.  solo: block 0 alone over all tiles; otherwise
-// every block over its grid-strided share, the partial sums folded by the
-// block completing the arrival tree and the costs broadcast.  Batching the
-// points does not change any cost: each point's sums are the ones a pass of
-// its own would form.  resident (solo levels small enough): the whole target
-// level is boxs[0, n) and the reference level boxs[n, 2n), staged once for
-// the level, so a pass reads no global memory; the tiles, the taps and the
-// order of every sum are the same as when each tile stages its own window.
-__device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, int np, bool solo, bool resident,
-                        double *part, unsigned *ctr, unsigned *err, unsigned &gen) {
-    const int tw = K.lw[lvl], th = K.lh[lvl];
-    uint64_t tq = wall_clock64();
-    if (threadIdx.x == 0) sh.ph[3] += tq - sh.tmark;
-    if ((int)threadIdx.x < np) {
-        double v[8];
-        for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
-        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = sh.PT[threadIdx.x][k];
-        sim_inverse_dev(v, tw, th, sh.inv[threadIdx.x]);
-    }
-    __syncthreads();
-    uint64_t tq1 = wall_clock64();
-    if (threadIdx.x == 0) sh.ph[0] += tq1 - tq;
-    const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
-    const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
-    zero_waves(sh);  // ordered before its first use by the tile loop's barriers
-    // The displacement (source - destination) of an affine map is affine in the
-    // pixel, so over the level it is bounded by its values at the 4 corners: one
-    // bound for the pass, from which every tile's window follows.  A projective
-    // point (or a wild one) turns the staging off for the pass.
-    if (threadIdx.x == 0 && !resident) {
-        double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
-        int ok = 1;
-        for (int p = 0; p < np; p++) {
-            Inv9 iv;
-            for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
-            if (!(iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0)) ok = 0;
-            for (int c = 0; c < 4; c++) {
-                const int cx = (c & 1) ? tw - 1 : 0, cy = (c & 2) ? th - 1 : 0;
-                double sx, sy;
-                src_point(iv, 1, cx, cy, sx, sy);
-                mnx = fmin(mnx, sx - cx);
-                mxx = fmax(mxx, sx - cx);
-                mny = fmin(mny, sy - cy);
-                mxy = fmax(mxy, sy - cy);
-            }
-        }
-        if (!(mnx > -256.0 && mxx < 256.0 && mny > -256.0 && mxy < 256.0)) ok = 0;
-        sh.box[0] = ok ? (int)floor(mnx) - 1 : 0;  // window = tile + [lo, hi] on each axis
-        sh.box[1] = ok ? (int)floor(mxx) + 2 : 0;
-        sh.box[2] = ok ? (int)floor(mny) - 1 : 0;
-        sh.box[3] = ok ? (int)floor(mxy) + 2 : 0;
-        sh.box[4] = ok;
-    }
-    // rows per thread: 4 where the level has a 64 x 64 tile for every block
-    // (or one block does all), fewer on small levels so more blocks share them
+// rows per thread of a pass's tiles: 4 where the level has a 64 x 64 tile for
+// every block (or one block does all), fewer on small levels so more blocks
+// share them
+__device__ __forceinline__ int nm_rows(int tw, int th, bool solo) {
     const int ntx = (tw + NM_TILE - 1) / NM_TILE;
     int R = 4;
     if (!solo)
         while (R > 1 && ntx * ((th + 16 * R - 1) / (16 * R)) < (int)gridDim.x) R >>= 1;
+    return R;
+}
+
+// Whether a grid level's tiles can be held (nm_tiles): they are 16 or 32 rows
+// and no block has more than NM_HELD of them.
+__device__ __forceinline__ bool nm_holdable(int tw, int th) {
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE, R = nm_rows(tw, th, false);
+    return R <= 2 && ntx * ((th + 16 * R - 1) / (16 * R)) <= NM_HELD * (int)gridDim.x;
+}
+
+// The pass's tiles, the sums into the waves' accumulators.  MODE 1: a
+// resident solo level.  MODE 0: each tile's source window (the pass's bound)
+// staged per pass, or, on a held level, staged by the level's first pass
+// (hold) with the bound widened by NM_HMARGIN on each side for the moves to
+// come, into the block's slot of hb together with the tile's reference
+// pixels, and read from there by every later pass.
+template <int MODE>
+__device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np,
+                                         bool solo, bool held, bool hold) {
+    const int tw = K.lw[lvl], th = K.lh[lvl];
+    const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
+    const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE, R = nm_rows(tw, th, solo);
     const int TH = 16 * R, nty = (th + TH - 1) / TH, ntiles = ntx * nty;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    if (resident) __syncthreads();  // the cleared accumulators (the tile loop has no barrier then)
+    if (MODE != 0 || (held && !hold)) __syncthreads();  // the cleared accumulators (no barrier in the loop then)
     for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : gridDim.x) {
         const int X0 = (t % ntx) * NM_TILE, Y0 = (t / ntx) * TH;
         const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
         int bx0 = 0, by0 = 0, bw = tw, bh = th, staged = 1;
+        const float *win = boxs;  // the staged target samples (window origin at bx0, by0)
         float rv[4];
-        if (resident) {
+        if constexpr (MODE == 1) {
+            bx0 = -1;
+            by0 = -1;
+            bw = tw + 3;
+            bh = th + 3;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int x = X0 + tx, y = Y0 + ty + 16 * r;
-                rv[r] = (r < R && x < tw && y < th) ? boxs[tw * th + y * tw + x] : 0.0f;
+                rv[r] = (r < R && x < tw && y < th) ? boxs[bw * bh + y * tw + x] : 0.0f;
             }
         } else {
-            __syncthreads();  // the bound is written; the previous tile's window is no longer read
-            bx0 = max(0, X0 + sh.box[0]);
-            by0 = max(0, Y0 + sh.box[2]);
-            bw = min(tw - 1, X1 + sh.box[1]) - bx0 + 1;
-            bh = min(th - 1, Y1 + sh.box[3]) - by0 + 1;
-            staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= NM_BOX;
-            // (issuing all of a thread's window loads before storing any measured
-            // no faster here and spilled registers)
-            if (staged)
-                for (int i = threadIdx.x; i < bw * bh; i += NM_THREADS) {
-                    int r = i / bw;
-                    boxs[i] = tg[(size_t)(by0 + r) * tw + bx0 + (i - r * bw)];
+            const int q = held ? (t - (int)blockIdx.x) / (int)gridDim.x : 0;  // a held tile's slot
+            float *hs = hb + q * NM_HSLOT;
+            int *hw = sh.hwin[q];
+            if (!held || hold) {
+                __syncthreads();  // the bound is written; the previous tile's window is no longer read
+                // the window may reach one sample before and two past the level
+                // on each axis (a support's reach), where it holds the edge samples
+                const int m = held ? NM_HMARGIN : 0;
+                bx0 = max(-1, X0 + sh.box[0] - m);
+                by0 = max(-1, Y0 + sh.box[2] - m);
+                bw = min(tw + 1, X1 + sh.box[1] + m) - bx0 + 1;
+                bh = min(th + 1, Y1 + sh.box[3] + m) - by0 + 1;
+                float *wb = held ? hs + NM_TILE * TH : boxs;
+                staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= (held ? NM_HSLOT - NM_TILE * TH : NM_BOX);
+                // (issuing all of a thread's window loads before storing any measured
+                // no faster here and spilled registers)
+                if (staged)
+                    for (int i = threadIdx.x; i < bw * bh; i += NM_THREADS) {
+                        const int r = i / bw, c = i - r * bw;
+                        wb[i] = tg[(size_t)min(max(by0 + r, 0), th - 1) * tw + min(max(bx0 + c, 0), tw - 1)];
+                    }
+                if (held && threadIdx.x == 0) {
+                    hw[0] = bx0;
+                    hw[1] = by0;
+                    hw[2] = bw;
+                    hw[3] = bh;
+                    hw[4] = staged;
                 }
-            __syncthreads();
+                win = wb;
+                __syncthreads();
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int x = X0 + tx, y = Y0 + ty + 16 * r;
-                rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
+                for (int r = 0; r < 4; r++) {
+                    const int x = X0 + tx, y = Y0 + ty + 16 * r;
+                    rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
+                    // kept for the later passes (each thread reads back its own)
+                    if (held && r < R) hs[(ty + 16 * r) * NM_TILE + tx] = rv[r];
+                }
+            } else {
+                bx0 = hw[0];
+                by0 = hw[1];
+                bw = hw[2];
+                bh = hw[3];
+                staged = hw[4];
+                win = hs + NM_TILE * TH;
+#pragma unroll
+                for (int r = 0; r < 4; r++) rv[r] = r < R ? hs[(ty + 16 * r) * NM_TILE + tx] : 0.0f;
             }
         }
+        // the window is the block's (scalar registers)
+        bx0 = __builtin_amdgcn_readfirstlane(bx0);
+        by0 = __builtin_amdgcn_readfirstlane(by0);
+        bw = __builtin_amdgcn_readfirstlane(bw);
+        bh = __builtin_amdgcn_readfirstlane(bh);
+        staged = __builtin_amdgcn_readfirstlane(staged);
         // per point: the thread's 4 pixels, summed over the wave, added to the
         // wave's accumulators (one lane per wave writes them: no race)
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -422,17 +432,16 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
                 float wx[4], wy[4];
                 spline4f((float)(sx - fx), wx);
                 spline4f((float)(sy - fy), wy);
-                // the 4 x 4 support (clamped to the level) in the staged window, or global reads
-                const int cx0 = min(max(x0 - 1, 0), tw - 1), cx3 = min(max(x0 + 2, 0), tw - 1);
-                const int cy0 = min(max(y0 - 1, 0), th - 1), cy3 = min(max(y0 + 2, 0), th - 1);
+                // the 4 x 4 support (clamped to the level) in the staged window
+                // (whose edge samples stand for the clamped ones), or global reads
                 float a = 0.0f;
-                if (staged && cx0 >= bx0 && cx3 < bx0 + bw && cy0 >= by0 && cy3 < by0 + bh) {
+                if (staged && x0 - 1 >= bx0 && x0 + 2 < bx0 + bw && y0 - 1 >= by0 && y0 + 2 < by0 + bh) {
+                    const float *sup = win + (y0 - 1 - by0) * bw + (x0 - 1 - bx0);
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        const float *row = boxs + (min(max(y0 + j - 1, 0), th - 1) - by0) * bw - bx0;
                         float rr = 0.0f;
 #pragma unroll
-                        for (int q = 0; q < 4; q++) rr += wx[q] * row[min(max(x0 + q - 1, 0), tw - 1)];
+                        for (int q = 0; q < 4; q++) rr += wx[q] * sup[j * bw + q];
                         a += wy[j] * rr;
                     }
                 } else {
@@ -460,6 +469,70 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, int lvl, in
             }
         }
     }
+}
+
+// One collective pass: the costs (align.rs:581-635) of the np points
+// sh.PT[0..np) at level lvl into sh.PC[0..np).  The level is walked in 64-wide
+// destination tiles (nm_tiles); each tile's source window (the union over the
+// points of the bounding box of the tile corners' source points, plus the
+// 4 x 4 support) is staged in LDS when it fits, and every tap inside it is
+// read from there, any other from global memory.  solo: block 0 alone over
+// all tiles; otherwise every block over its grid-strided share, the blocks'
+// partial sums combined by every block in the same order.  Batching the
+// points does not change any cost: each point's sums are the ones a pass of
+// its own would form.  resident (solo levels small enough): the whole target
+// level and the reference level are staged in boxs once for the level, so a
+// pass reads no global memory.  held: a grid level whose tiles the block holds
+// in hb (nm_tiles); hold: the level's first pass, which stages them.  The
+// tiles, the taps and the order of every sum are the same in every case.
+__device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np, bool solo,
+                        bool resident, bool held, bool hold, double *part, unsigned *ctr, unsigned *err,
+                        unsigned &gen) {
+    const int tw = K.lw[lvl], th = K.lh[lvl];
+    uint64_t tq = wall_clock64();
+    if (threadIdx.x == 0) sh.ph[3] += tq - sh.tmark;
+    if ((int)threadIdx.x < np) {
+        double v[8];
+        for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
+        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = sh.PT[threadIdx.x][k];
+        sim_inverse_dev(v, tw, th, sh.inv[threadIdx.x]);
+    }
+    __syncthreads();
+    uint64_t tq1 = wall_clock64();
+    if (threadIdx.x == 0) sh.ph[0] += tq1 - tq;
+    zero_waves(sh);  // ordered before its first use by the tile loop's barriers
+    // The displacement (source - destination) of an affine map is affine in the
+    // pixel, so over the level it is bounded by its values at the 4 corners: one
+    // bound for the pass, from which every tile's window follows.  A projective
+    // point (or a wild one) turns the staging off for the pass.
+    if (threadIdx.x == 0 && !resident && (!held || hold)) {
+        double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
+        int ok = 1;
+        for (int p = 0; p < np; p++) {
+            Inv9 iv;
+            for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
+            if (!(iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0)) ok = 0;
+            for (int c = 0; c < 4; c++) {
+                const int cx = (c & 1) ? tw - 1 : 0, cy = (c & 2) ? th - 1 : 0;
+                double sx, sy;
+                src_point(iv, 1, cx, cy, sx, sy);
+                mnx = fmin(mnx, sx - cx);
+                mxx = fmax(mxx, sx - cx);
+                mny = fmin(mny, sy - cy);
+                mxy = fmax(mxy, sy - cy);
+            }
+        }
+        if (!(mnx > -256.0 && mxx < 256.0 && mny > -256.0 && mxy < 256.0)) ok = 0;
+        sh.box[0] = ok ? (int)floor(mnx) - 1 : 0;  // window = tile + [lo, hi] on each axis
+        sh.box[1] = ok ? (int)floor(mxx) + 2 : 0;
+        sh.box[2] = ok ? (int)floor(mny) - 1 : 0;
+        sh.box[3] = ok ? (int)floor(mxy) + 2 : 0;
+        sh.box[4] = ok;
+    }
+    if (resident)
+        nm_tiles<1>(K, sh, boxs, hb, lvl, np, solo, false, false);
+    else
+        nm_tiles<0>(K, sh, boxs, hb, lvl, np, solo, held, hold);
     fold_waves(sh, 3 * np);
     uint64_t tq2 = wall_clock64();
     if (threadIdx.x == 0) {
@@ -583,8 +656,8 @@ __device__ __forceinline__ double clampd_dev(double v, double lo, double hi) {
 // evaluates its n new vertices in one pass, as does the initial simplex.  The
 // iterates and the counted evaluations are the reference's: a speculative
 // cost the branch does not use is not counted.
-__device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, int lvl, bool solo, double *part, unsigned *ctr,
-                         unsigned *err, unsigned &gen) {
+__device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, bool solo, double *part,
+                         unsigned *ctr, unsigned *err, unsigned &gen) {
     enum { INIT, ITER, EXPAND, CONTRACT, SHRINK };
     const int n = K.nfree;
     const bool t0 = threadIdx.x == 0;
@@ -602,9 +675,11 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, int lvl, b
             for (int k = 0; k < n; k++) sh.PT[i][k] = sh.S[i][k];
     }
     // a solo level whose target and reference fit the window buffer together
-    // is staged once (target, then reference) and read from there by every pass
-    const int npx = K.lw[lvl] * K.lh[lvl];
-    const bool resident = solo && 2 * npx <= NM_BOX;
+    // is staged once and read from there by every pass: the target with one
+    // edge sample before and two past it on each axis ((w + 3) x (h + 3), as
+    // a tile's window holds them), then the reference
+    const int lw = K.lw[lvl], lh = K.lh[lvl], npx = lw * lh, pw = lw + 3, np3 = pw * (lh + 3);
+    const bool resident = solo && np3 + npx <= NM_BOX;
     if (resident) {
         const float *tg = K.tgt[lvl], *rf = K.ref[lvl];
         constexpr int SU = NM_BOX / NM_THREADS;
@@ -612,18 +687,24 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, int lvl, b
 #pragma unroll
         for (int u = 0; u < SU; u++) {
             const int i = threadIdx.x + u * NM_THREADS;
-            if (i < 2 * npx) v[u] = i < npx ? tg[i] : rf[i - npx];
+            if (i < np3) {
+                const int r = i / pw, c = i - r * pw;
+                v[u] = tg[min(max(r - 1, 0), lh - 1) * lw + min(max(c - 1, 0), lw - 1)];
+            } else if (i < np3 + npx) {
+                v[u] = rf[i - np3];
+            }
         }
 #pragma unroll
         for (int u = 0; u < SU; u++) {
             const int i = threadIdx.x + u * NM_THREADS;
-            if (i < 2 * npx) boxs[i] = v[u];
+            if (i < np3 + npx) boxs[i] = v[u];
         }
     }
     __syncthreads();
+    const bool held = !solo && nm_holdable(K.lw[lvl], K.lh[lvl]);
     int state = INIT, np = n + 1, it = 0;
     for (;;) {
-        nm_pass(K, sh, boxs, lvl, np, solo, resident, part, ctr, err, gen);
+        nm_pass(K, sh, boxs, hb, lvl, np, solo, resident, held, held && state == INIT, part, ctr, err, gen);
         int next = -1;  // a follow-up pass within the iteration, or none
         if (state == INIT) {
             if (t0) {
@@ -766,6 +847,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
                                                           unsigned *__restrict__ err) {
     __shared__ NmShared sh;
     __shared__ __attribute__((aligned(16))) float boxs[NM_BOX];  // a tile's source window; the fold's staging
+    __shared__ __attribute__((aligned(16))) float hb[NM_HELD * NM_HSLOT];  // a held level's tiles
     const NmConst &K = *Kp;
     const int n = K.nfree;
     unsigned gen = 0;
@@ -818,7 +900,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
         }
         if (solo && blockIdx.x != 0) continue;
         uint64_t t = wall_clock64();
-        nm_level(K, sh, boxs, lvl, solo, part, ctr, err, gen);
+        nm_level(K, sh, boxs, hb, lvl, solo, part, ctr, err, gen);
         if (t0 && blockIdx.x == 0) state[9 + NM_MAXL + lvl] = (double)(wall_clock64() - t);
     }
     if (blockIdx.x == 0 && t0) {
