@@ -307,22 +307,26 @@ __device__ __forceinline__ void zero_waves(NmShared &sh) {
     for (int i = threadIdx.x; i < 3 * NM_MAXP * NM_WAVES; i += NM_THREADS) (&sh.red[0][0])[i] = 0.0;
 }
 
-// rows per thread of a pass's tiles: 4 where the level has a 64 x 64 tile for
-// every block (or one block does all), fewer on small levels so more blocks
-// share them
-__device__ __forceinline__ int nm_rows(int tw, int th, bool solo) {
-    const int ntx = (tw + NM_TILE - 1) / NM_TILE;
-    int R = 4;
-    if (!solo)
-        while (R > 1 && ntx * ((th + 16 * R - 1) / (16 * R)) < (int)gridDim.x) R >>= 1;
-    return R;
+// The rows of a pass's tiles: 64 where the level has a 64 x 64 tile for every
+// block (or one block does all), fewer on smaller levels so more blocks share
+// them: down to 16 (a row per wave), then to 8 or 4 (waves left idle) while
+// every tile still has a block of its own, since a block's tile costs it
+// about the same with 16 rows as with 4 when its SIMDs take turns.
+__device__ __forceinline__ int nm_tile_rows(int tw, int th, bool solo) {
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE, G = (int)gridDim.x;
+    int TH = 64;
+    if (!solo) {
+        while (TH > 16 && ntx * ((th + TH - 1) / TH) < G) TH >>= 1;
+        while (TH > 4 && ntx * ((th + TH / 2 - 1) / (TH / 2)) <= G) TH >>= 1;
+    }
+    return TH;
 }
 
-// Whether a grid level's tiles can be held (nm_tiles): they are 16 or 32 rows
-// and no block has more than NM_HELD of them.
+// Whether a grid level's tiles can be held (nm_tiles): they are at most 32
+// rows and no block has more than NM_HELD of them.
 __device__ __forceinline__ bool nm_holdable(int tw, int th) {
-    const int ntx = (tw + NM_TILE - 1) / NM_TILE, R = nm_rows(tw, th, false);
-    return R <= 2 && ntx * ((th + 16 * R - 1) / (16 * R)) <= NM_HELD * (int)gridDim.x;
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, false);
+    return TH <= 32 && ntx * ((th + TH - 1) / TH) <= NM_HELD * (int)gridDim.x;
 }
 
 // The pass's tiles, the sums into the waves' accumulators.  MODE 1: a
@@ -337,12 +341,15 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     const int tw = K.lw[lvl], th = K.lh[lvl];
     const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
     const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
-    const int ntx = (tw + NM_TILE - 1) / NM_TILE, R = nm_rows(tw, th, solo);
-    const int TH = 16 * R, nty = (th + TH - 1) / TH, ntiles = ntx * nty;
+    // tiles column by column, so that the last (narrowest) column is what a
+    // block takes as its second tile
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, solo), R = max(1, TH / 16);
+    const int nty = (th + TH - 1) / TH, ntiles = ntx * nty;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const bool act = ty < TH;  // the wave has a row of the tile
     if (MODE != 0 || (held && !hold)) __syncthreads();  // the cleared accumulators (no barrier in the loop then)
     for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : gridDim.x) {
-        const int X0 = (t % ntx) * NM_TILE, Y0 = (t / ntx) * TH;
+        const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
         const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
         int bx0 = 0, by0 = 0, bw = tw, bh = th, staged = 1;
         const float *win = boxs;  // the staged target samples (window origin at bx0, by0)
@@ -355,7 +362,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int x = X0 + tx, y = Y0 + ty + 16 * r;
-                rv[r] = (r < R && x < tw && y < th) ? boxs[bw * bh + y * tw + x] : 0.0f;
+                rv[r] = (r < R && act && x < tw && y < th) ? boxs[bw * bh + y * tw + x] : 0.0f;
             }
         } else {
             const int q = held ? (t - (int)blockIdx.x) / (int)gridDim.x : 0;  // a held tile's slot
@@ -391,9 +398,9 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int x = X0 + tx, y = Y0 + ty + 16 * r;
-                    rv[r] = (r < R && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
+                    rv[r] = (r < R && act && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
                     // kept for the later passes (each thread reads back its own)
-                    if (held && r < R) hs[(ty + 16 * r) * NM_TILE + tx] = rv[r];
+                    if (held && r < R && act) hs[(ty + 16 * r) * NM_TILE + tx] = rv[r];
                 }
             } else {
                 bx0 = hw[0];
@@ -403,7 +410,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 staged = hw[4];
                 win = hs + NM_TILE * TH;
 #pragma unroll
-                for (int r = 0; r < 4; r++) rv[r] = r < R ? hs[(ty + 16 * r) * NM_TILE + tx] : 0.0f;
+                for (int r = 0; r < 4; r++) rv[r] = r < R && act ? hs[(ty + 16 * r) * NM_TILE + tx] : 0.0f;
             }
         }
         // the window is the block's (scalar registers)
@@ -415,7 +422,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
         // per point: the thread's 4 pixels, summed over the wave, added to the
         // wave's accumulators (one lane per wave writes them: no race)
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        for (int p = 0; p < np; p++) {
+        for (int p = 0; p < (act ? np : 0); p++) {
             Inv9 iv;
             for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
             const int aff = iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0;
