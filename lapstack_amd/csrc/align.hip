@@ -155,6 +155,16 @@ __device__ __forceinline__ void src_point(const Inv9 &v, int affine, int x, int 
     }
 }
 
+// threadIdx.x, read afresh where it is used: the addresses derived from it in
+// an outer loop are then recomputed there rather than kept live across the
+// whole search (where the register allocator spills them to scratch memory,
+// and each reload waits on the memory system).
+__device__ __forceinline__ int tid_here() {
+    int t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+    return t;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
@@ -343,10 +353,12 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
     // tiles column by column, so that the last (narrowest) column is what a
     // block takes as its second tile
-    const int ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, solo), R = max(1, TH / 16);
-    const int nty = (th + TH - 1) / TH, ntiles = ntx * nty;
+    // (a resident level is one tile: its pixels strided over the block)
+    const int npx = tw * th, ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, solo);
+    const int R = MODE == 1 ? (npx + NM_THREADS - 1) / NM_THREADS : max(1, TH / 16);
+    const int nty = (th + TH - 1) / TH, ntiles = MODE == 1 ? 1 : ntx * nty;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const bool act = ty < TH;  // the wave has a row of the tile
+    const bool act = MODE == 1 || ty < TH;  // the wave has a row of the tile
     if (MODE != 0 || (held && !hold)) __syncthreads();  // the cleared accumulators (no barrier in the loop then)
     for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : gridDim.x) {
         const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
@@ -361,8 +373,8 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
             bh = th + 3;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int x = X0 + tx, y = Y0 + ty + 16 * r;
-                rv[r] = (r < R && act && x < tw && y < th) ? boxs[bw * bh + y * tw + x] : 0.0f;
+                const int i = (int)threadIdx.x + r * NM_THREADS;
+                rv[r] = (r < R && i < npx) ? boxs[bw * bh + i] : 0.0f;
             }
         } else {
             const int q = held ? (t - (int)blockIdx.x) / (int)gridDim.x : 0;  // a held tile's slot
@@ -429,7 +441,13 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll 1
             for (int r = 0; r < R; r++) {
-                const int x = X0 + tx, y = Y0 + ty + 16 * r;
+                int x = X0 + tx, y = Y0 + ty + 16 * r;
+                if constexpr (MODE == 1) {
+                    const int i = (int)threadIdx.x + r * NM_THREADS;
+                    if (i >= npx) continue;
+                    y = i / tw;
+                    x = i - y * tw;
+                }
                 if (x >= tw || y >= th) continue;
                 double sx, sy;
                 src_point(iv, aff, x, y, sx, sy);
@@ -791,7 +809,7 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
         // reference's arithmetic and order per coordinate):
         // 1. the stable sort of the vertices by value (align.rs:816-818): vertex
         //    i goes to its rank (values below it, then equal values before it)
-        const int tid = threadIdx.x;
+        const int tid = tid_here();
         if (tid <= n) {
             const double f = sh.FV[tid];
             int r = 0;
