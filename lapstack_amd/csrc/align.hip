@@ -291,10 +291,15 @@ struct NmShared {
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
     int hwin[NM_HELD][5];   // the held tiles' windows (x0, y0, w, h, staged)
+    int grid;               // gridDim.x (read through nm_G: gridDim.x costs a load from the kernel's argument block)
+    double tolv[8];         // the level's convergence tolerances
     int stop;
 };
 
 // align.rs:630-634
+// the grid size, block-uniform (scalar registers)
+__device__ __forceinline__ int nm_G(const NmShared &sh) { return __builtin_amdgcn_readfirstlane(sh.grid); }
+
 __device__ __forceinline__ double rms_of(double sd, double sd2, double cnt) {
     if (cnt < 16.0) return 1e9;
     double vv = (sd2 - sd * sd / cnt) / cnt;
@@ -322,8 +327,8 @@ __device__ __forceinline__ void zero_waves(NmShared &sh) {
 // them: down to 16 (a row per wave), then to 8 or 4 (waves left idle) while
 // every tile still has a block of its own, since a block's tile costs it
 // about the same with 16 rows as with 4 when its SIMDs take turns.
-__device__ __forceinline__ int nm_tile_rows(int tw, int th, bool solo) {
-    const int ntx = (tw + NM_TILE - 1) / NM_TILE, G = (int)gridDim.x;
+__device__ __forceinline__ int nm_tile_rows(int tw, int th, bool solo, int G) {
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE;
     int TH = 64;
     if (!solo) {
         while (TH > 16 && ntx * ((th + TH - 1) / TH) < G) TH >>= 1;
@@ -334,9 +339,9 @@ __device__ __forceinline__ int nm_tile_rows(int tw, int th, bool solo) {
 
 // Whether a grid level's tiles can be held (nm_tiles): they are at most 32
 // rows and no block has more than NM_HELD of them.
-__device__ __forceinline__ bool nm_holdable(int tw, int th) {
-    const int ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, false);
-    return TH <= 32 && ntx * ((th + TH - 1) / TH) <= NM_HELD * (int)gridDim.x;
+__device__ __forceinline__ bool nm_holdable(int tw, int th, int G) {
+    const int ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, false, G);
+    return TH <= 32 && ntx * ((th + TH - 1) / TH) <= NM_HELD * G;
 }
 
 // The pass's tiles, the sums into the waves' accumulators.  MODE 1: a
@@ -351,16 +356,17 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     const int tw = K.lw[lvl], th = K.lh[lvl];
     const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
     const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
+    const int G = nm_G(sh);
     // tiles column by column, so that the last (narrowest) column is what a
     // block takes as its second tile
     // (a resident level is one tile: its pixels strided over the block)
-    const int npx = tw * th, ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, solo);
+    const int npx = tw * th, ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, solo, G);
     const int R = MODE == 1 ? (npx + NM_THREADS - 1) / NM_THREADS : max(1, TH / 16);
     const int nty = (th + TH - 1) / TH, ntiles = MODE == 1 ? 1 : ntx * nty;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const bool act = MODE == 1 || ty < TH;  // the wave has a row of the tile
     if (MODE != 0 || (held && !hold)) __syncthreads();  // the cleared accumulators (no barrier in the loop then)
-    for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : gridDim.x) {
+    for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : G) {
         const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
         const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
         int bx0 = 0, by0 = 0, bw = tw, bh = th, staged = 1;
@@ -377,7 +383,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 rv[r] = (r < R && i < npx) ? boxs[bw * bh + i] : 0.0f;
             }
         } else {
-            const int q = held ? (t - (int)blockIdx.x) / (int)gridDim.x : 0;  // a held tile's slot
+            const int q = held ? (t - (int)blockIdx.x) / G : 0;  // a held tile's slot
             float *hs = hb + q * NM_HSLOT;
             int *hw = sh.hwin[q];
             if (!held || hold) {
@@ -576,7 +582,8 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
     }
     const unsigned slot = gen & 1;
     const int nv = 3 * np;
-    double *pp = part + (size_t)slot * gridDim.x * 3 * NM_MAXP;
+    const int G = nm_G(sh);
+    double *pp = part + (size_t)slot * G * 3 * NM_MAXP;
     if (threadIdx.x == 0) {
         // The partials are agent-scope atomic stores, written through to the
         // level every XCD reads coherently; the workgroup release fence waits
@@ -595,7 +602,7 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         // left at once when any block has raised the error flag).  A group's
         // last arriver has seen every increment of its group (each made after
         // that block's partials were complete) when it raises the top count.
-        const unsigned gs = gridDim.x / NM_GROUPS;
+        const unsigned gs = (unsigned)G / NM_GROUPS;
         unsigned a = __hip_atomic_fetch_add(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u,
                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (a == (gen + 1) * gs - 1)
@@ -621,7 +628,7 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         // block order.  A block a pass ahead writes the other slot; none can be
         // two ahead before every block has arrived at the pass between.
         double *stg = (double *)boxs;
-        const int G = (int)gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         const int kchunk = min(nv, (NM_BOX / 2) / G);
         for (int k0 = 0; k0 < nv; k0 += kchunk) {
             const int kn = min(kchunk, nv - k0), total = G * kn;
@@ -687,8 +694,9 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
     const int n = K.nfree;
     const bool t0 = threadIdx.x == 0;
     const bool spec = !solo && (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.spec_max;
-    const double *step = K.step[lvl], *tol = K.tol[lvl];
+    const double *step = K.step[lvl];
 #define CLAMPK(v, k) clampd_dev((v), sh.lo[k], sh.hi[k])
+    if ((int)threadIdx.x < n) sh.tolv[threadIdx.x] = K.tol[lvl][threadIdx.x];  // read by the iterations' set-up
     if (t0) {
         for (int k = 0; k < n; k++) sh.S[0][k] = sh.cur[K.free_idx[k]];
         for (int i = 0; i < n; i++) {
@@ -726,7 +734,7 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
         }
     }
     __syncthreads();
-    const bool held = !solo && nm_holdable(K.lw[lvl], K.lh[lvl]);
+    const bool held = !solo && nm_holdable(K.lw[lvl], K.lh[lvl], nm_G(sh));
     int state = INIT, np = n + 1, it = 0;
     for (;;) {
         nm_pass(K, sh, boxs, hb, lvl, np, solo, resident, held, held && state == INIT, part, ctr, err, gen);
@@ -830,7 +838,7 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
         if (tid == 0) ok = !(fabs(sh.FV[n] - sh.FV[0]) > 1e-4 * (1.0 + fabs(sh.FV[0])));
         if (tid >= 8 && tid < (n + 1) * 8) {
             const int i = tid >> 3, k = tid & 7;
-            if (k < n && !(fabs(sh.S[i][k] - sh.S[0][k]) <= tol[k])) ok = 0;
+            if (k < n && !(fabs(sh.S[i][k] - sh.S[0][k]) <= sh.tolv[k])) ok = 0;
         }
         // 3. per coordinate: the centroid of the best n, the reflection, the
         //    expansion and the contraction (align.rs:823-862)
@@ -885,6 +893,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
             sh.hi[k] = iv + K.span[K.free_idx[k]];
         }
         sh.stop = 0;
+        sh.grid = (int)gridDim.x;
         for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = sh.lev_passes[l] = 0;
         for (int q = 0; q < 4; q++) sh.ph[q] = 0;
         sh.tmark = wall_clock64();
