@@ -378,40 +378,43 @@ __global__ __launch_bounds__(256) void k_box_v2_direct(const float *__restrict__
     box_v_tail(x, y, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
 }
 
-// The same two passes on tiles staged in LDS: a row segment of BH_TX outputs
-// and its radius-r halo (horizontal), or BV_TX columns of BV_TY outputs and
+// The same two passes on tiles staged in LDS: BH_RB row segments of BH_TX
+// outputs and their radius-r halo (horizontal), or BV_TX columns of BV_TY outputs and
 // their 2r halo rows (vertical), loaded once with every load of a thread's
 // chunk issued before it is stored; each output then sums its window from LDS
 // with the same f64 additions in the same order as the direct kernels.
 #define BH_TX 256
+#define BH_RB 4  // rows per horizontal tile
 #define BV_TX 64
 #define BV_TY 32
 #define BOX_TILE_MAX_R 32
 __global__ __launch_bounds__(256) void k_box_h2_tile(const float *__restrict__ a, const float *__restrict__ b,
                                                      const float *__restrict__ g, int mode, int w, int h, int r,
                                                      float *__restrict__ oa, float *__restrict__ ob) {
-    extern __shared__ float S[];  // [2][BH_TX + 2 r]
-    float *Sa = S, *Sb = S + BH_TX + 2 * r;
-    const int x0 = blockIdx.x * BH_TX, y = blockIdx.y;
-    const size_t row = (size_t)y * w;
-    const int c0 = max(x0 - r, 0), nc = min(x0 + BH_TX - 1 + r, w - 1) - c0 + 1;
-    for (int q0 = threadIdx.x; q0 < nc; q0 += 4 * BH_TX) {
+    extern __shared__ float S[];  // [2][BH_RB][BH_TX + 2 r]
+    const int L = BH_TX + 2 * r;
+    float *Sa = S, *Sb = S + BH_RB * L;
+    const int x0 = blockIdx.x * BH_TX, y0 = blockIdx.y * BH_RB, nr = min(BH_RB, h - y0);
+    const int c0 = max(x0 - r, 0), nc = min(x0 + BH_TX - 1 + r, w - 1) - c0 + 1, n = nr * nc;
+    for (int k0 = threadIdx.x; k0 < n; k0 += 4 * BH_TX) {
         float va[4], vb[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int q = q0 + u * BH_TX;
-            if (q < nc) {
-                const size_t i = row + c0 + q;
+            const int k = k0 + u * BH_TX;
+            if (k < n) {
+                const int j = k / nc;
+                const size_t i = (size_t)(y0 + j) * w + c0 + (k - j * nc);
                 va[u] = a[i];
                 vb[u] = mode == 0 ? b[i] : (mode == 1 ? g[i] * va[u] : va[u] * va[u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int q = q0 + u * BH_TX;
-            if (q < nc) {
-                Sa[q] = va[u];
-                Sb[q] = vb[u];
+            const int k = k0 + u * BH_TX;
+            if (k < n) {
+                const int j = k / nc, q = k - j * nc;
+                Sa[j * L + q] = va[u];
+                Sb[j * L + q] = vb[u];
             }
         }
     }
@@ -419,13 +422,23 @@ __global__ __launch_bounds__(256) void k_box_h2_tile(const float *__restrict__ a
     const int x = x0 + threadIdx.x;
     if (x >= w) return;
     const int lo = max(x - r, 0), hi = min(x + r, w - 1);
-    double sa = 0.0, sb = 0.0;
+    // the rows' sums are independent chains, interleaved
+    double sa[BH_RB], sb[BH_RB];
+#pragma unroll
+    for (int j = 0; j < BH_RB; j++) sa[j] = sb[j] = 0.0;
     for (int i = lo; i <= hi; i++) {
-        sa += (double)Sa[i - c0];
-        sb += (double)Sb[i - c0];
+#pragma unroll
+        for (int j = 0; j < BH_RB; j++) {
+            sa[j] += (double)Sa[j * L + i - c0];
+            sb[j] += (double)Sb[j * L + i - c0];
+        }
     }
-    oa[row + x] = (float)sa;
-    ob[row + x] = (float)sb;
+#pragma unroll
+    for (int j = 0; j < BH_RB; j++)
+        if (j < nr) {
+            oa[(size_t)(y0 + j) * w + x] = (float)sa[j];
+            ob[(size_t)(y0 + j) * w + x] = (float)sb[j];
+        }
 }
 
 __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ ta, const float *__restrict__ tb,
@@ -626,9 +639,10 @@ static unsigned g1(size_t n) { return cdiv(n, 256); }
 // else the direct ones (the same sums either way).
 static void box_h2(const char *name, double bytes, const float *a, const float *b, const float *g, int mode, int w,
                    int h, int r, float *oa, float *ob) {
-    if (r <= BOX_TILE_MAX_R && h <= 65535) {
-        const size_t shm = 2 * (size_t)(BH_TX + 2 * r) * sizeof(float);
-        launch_b(name, bytes, k_box_h2_tile, dim3(cdiv(w, BH_TX), h), dim3(BH_TX), shm, a, b, g, mode, w, h, r, oa, ob);
+    if (r <= BOX_TILE_MAX_R && cdiv(h, BH_RB) <= 65535) {
+        const size_t shm = 2 * (size_t)BH_RB * (BH_TX + 2 * r) * sizeof(float);
+        launch_b(name, bytes, k_box_h2_tile, dim3(cdiv(w, BH_TX), cdiv(h, BH_RB)), dim3(BH_TX), shm, a, b, g, mode, w,
+                 h, r, oa, ob);
     } else {
         launch_b(name, bytes, k_box_h2_direct, g2(w, h), B2, 0, a, b, g, mode, w, h, r, oa, ob);
     }
