@@ -276,7 +276,7 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_MAXP 9     // cost evaluations per collective pass
 #define NM_TILE 64    // a pass walks 64-pixel-wide destination tiles of 16, 32 or 64 rows
 #define NM_BOX 8192   // floats of a tile's staged source window
-#define NM_HELD 2     // tiles a block holds for a whole level (levels of 16- or 32-row tiles)
+#define NM_HELD 2     // tiles a block holds for a whole level (levels of tiles of at most 32 rows)
 #define NM_HSLOT 8192 // floats per held tile: its reference pixels, then its source window
 #define NM_HMARGIN 6  // pixels a held window extends past the first pass's bound
 // the state words: the Sim, the evaluation total, per level the evaluations and
@@ -515,7 +515,9 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
 // level and the reference level are staged in boxs once for the level, so a
 // pass reads no global memory.  held: a grid level whose tiles the block holds
 // in hb (nm_tiles); hold: the level's first pass, which stages them.  The
-// tiles, the taps and the order of every sum are the same in every case.
+// taps and the samples they read are the same in every case (a window holds
+// the level's own samples, and a tap outside it is read from memory); the
+// order of the sums follows the level's tiling.
 __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np, bool solo,
                         bool resident, bool held, bool hold, double *part, unsigned *ctr, unsigned *err,
                         unsigned &gen) {
