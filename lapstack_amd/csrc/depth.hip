@@ -474,16 +474,37 @@ __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ t
         }
     __syncthreads();
     if (x >= w) return;
-    for (int k = 0; k < BV_TY / 4; k++) {
-        const int y = y0 + ty + 4 * k;
-        if (y >= h) break;
-        const int lo = max(y - r, 0), hi = min(y + r, h - 1);
-        double sa = 0.0, sb = 0.0;
-        for (int i = lo; i <= hi; i++) {
-            sa += (double)Sa[(i - r0) * BV_TX + tx];
-            sb += (double)Sb[(i - r0) * BV_TX + tx];
+    // two rows at a time (y and y + 4): where their windows are as long, the
+    // two sums are interleaved, each still added from its window's top down
+    for (int k = 0; k < BV_TY / 4; k += 2) {
+        const int ya = y0 + ty + 4 * k, yb = ya + 4;
+        if (ya >= h) break;
+        const bool two = yb < h;
+        const int loa = max(ya - r, 0), hia = min(ya + r, h - 1);
+        const int lob = max(yb - r, 0), hib = min(yb + r, h - 1);
+        double sa = 0.0, sb = 0.0, ta = 0.0, tb = 0.0;
+        if (two && hia - loa == hib - lob) {
+            const float *pa = Sa + (loa - r0) * BV_TX + tx, *pb = Sb + (loa - r0) * BV_TX + tx;
+            const int d4 = (lob - loa) * BV_TX;
+            for (int i = 0; i <= hia - loa; i++) {
+                sa += (double)pa[i * BV_TX];
+                sb += (double)pb[i * BV_TX];
+                ta += (double)pa[i * BV_TX + d4];
+                tb += (double)pb[i * BV_TX + d4];
+            }
+        } else {
+            for (int i = loa; i <= hia; i++) {
+                sa += (double)Sa[(i - r0) * BV_TX + tx];
+                sb += (double)Sb[(i - r0) * BV_TX + tx];
+            }
+            if (two)
+                for (int i = lob; i <= hib; i++) {
+                    ta += (double)Sa[(i - r0) * BV_TX + tx];
+                    tb += (double)Sb[(i - r0) * BV_TX + tx];
+                }
         }
-        box_v_tail(x, y, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
+        box_v_tail(x, ya, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
+        if (two) box_v_tail(x, yb, w, h, r, mode, ta, tb, guide, mean_i, var_i, eps, oa, ob, P, m);
     }
 }
 
