@@ -155,6 +155,17 @@ uint64_t lsh_frames_last_evals(lsh_frames *f);
  * the collective passes at level l (each pass evaluates up to 9 points), and
  * ticks[16..20) the ticks per phase (matrix, pixels, combine, steering). */
 char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, uint32_t cap);
+/* No counterpart in the reference (whose search is a host loop it can log per
+ * evaluation, align.rs:796-883): trace the costs the following searches count
+ * -- every cost the reference's nelder_mead evaluates, in its order, level by
+ * level coarse to fine, up to cap of them (0 turns tracing off) -- and read the
+ * last search's (*n the count; up to cap into costs). */
+char *lsh_frames_search_trace(lsh_frames *f, uint32_t cap);
+char *lsh_frames_search_costs(lsh_frames *f, double *costs, uint32_t cap, uint32_t *n);
+/* The blocks that took part in the last lsh_frames_process search (*n) and,
+ * per block, a digest of every partial-sum total it folded: the blocks steer
+ * one search each from the costs they fold, so all digests are equal. */
+char *lsh_frames_search_blocks(lsh_frames *f, uint64_t *digests, uint32_t cap, uint32_t *n);
 void lsh_frames_free(lsh_frames *f);
 
 /* ---- fuser: Laplacian pyramid, region-energy fold, residual rule, collapse */

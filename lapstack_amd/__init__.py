@@ -237,6 +237,32 @@ class HipFrames:
     def last_evals(self) -> int:
         return int(self._lib.lsh_frames_last_evals(self._h))
 
+    def search_levels(self):
+        """The last search's cost evaluations per registration level (level 0 = full resolution)."""
+        ev = (C.c_uint64 * 16)()
+        _lib.check(self._lib.lsh_frames_search_stats(self._h, ev, None, 16))
+        return [int(v) for v in ev]
+
+    def trace_costs(self, cap: int):
+        """Record the costs of the following searches (up to cap per search; 0: off)."""
+        _lib.check(self._lib.lsh_frames_search_trace(self._h, int(cap)))
+
+    def search_costs(self) -> np.ndarray:
+        """The costs the last search counted, in the reference's order (tracing on)."""
+        n = C.c_uint32()
+        _lib.check(self._lib.lsh_frames_search_costs(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        _lib.check(self._lib.lsh_frames_search_costs(self._h, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+        return out
+
+    def search_blocks(self) -> np.ndarray:
+        """Per block that took part in the last search: its digest of every sum it folded."""
+        n = C.c_uint32()
+        _lib.check(self._lib.lsh_frames_search_blocks(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint64)
+        _lib.check(self._lib.lsh_frames_search_blocks(self._h, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+        return out
+
     def download(self) -> np.ndarray:
         out = np.zeros((3, self.h, self.w), np.float32)
         _lib.check(self._lib.lsh_frames_download(self._h, out.ctypes.data_as(C.c_void_p), LSH_HOST))

@@ -76,6 +76,9 @@ SIGNATURES = [
     ("lsh_frames_download", _p, [_p, _p, C.c_int]),
     ("lsh_frames_last_evals", C.c_uint64, [_p]),
     ("lsh_frames_search_stats", _p, [_p, _p, _p, _u32]),
+    ("lsh_frames_search_trace", _p, [_p, _u32]),
+    ("lsh_frames_search_costs", _p, [_p, _p, _u32, C.POINTER(_u32)]),
+    ("lsh_frames_search_blocks", _p, [_p, _p, _u32, C.POINTER(_u32)]),
     ("lsh_frames_free", None, [_p]),
     ("lsh_fuser_new", _p, [_u32, _u32, C.POINTER(FuseParamsC), C.POINTER(_p)]),
     ("lsh_fuser_levels", _p, [_p, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),

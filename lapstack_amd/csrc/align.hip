@@ -217,15 +217,17 @@ __device__ void grid_sum(double (&v)[NV], double *__restrict__ part, unsigned *_
 // identical next step, with no host round trip and no broadcast.
 #define NM_MAXL 16
 // the search's flag words, each on its own 128-byte line: arrivals of groups,
-// the error flag, (a spare line), the hand-off generation, then one arrival
-// counter per group
+// the error flag, the join count, the hand-off generation, one arrival counter
+// per group, then the closed participant count
 #define NM_GROUPS 16
 #define NM_LINE 32
 #define NM_FL_TOP 0
 #define NM_FL_ERR (1 * NM_LINE)
+#define NM_FL_JOIN (2 * NM_LINE)
 #define NM_FL_HAND (3 * NM_LINE)
 #define NM_FL_GROUP (4 * NM_LINE)
-#define NM_FL_WORDS ((4 + NM_GROUPS) * NM_LINE)
+#define NM_FL_CLOSED ((4 + NM_GROUPS) * NM_LINE)
+#define NM_FL_WORDS ((5 + NM_GROUPS) * NM_LINE)
 struct NmConst {
     const float *ref[NM_MAXL];
     const float *tgt[NM_MAXL];
@@ -236,12 +238,30 @@ struct NmConst {
     int free_idx[8];
     double step[NM_MAXL][8], tol[NM_MAXL][8];
     double span[8];
+    // every wait ends (with the error flag raised) after wait_ticks of the
+    // device wall clock; the first block closes the set of participants once
+    // the whole grid has joined, or when none has joined for join_ticks
+    uint64_t wait_ticks, join_ticks;
+    int max_part;                    // at most this many blocks take part (0: the grid)
+    double *trace;                   // non-null: the counted costs, in the reference's order
+    int trace_cap;
+    unsigned long long *digest;      // per participant: a digest of every cost it folded
 };
 
-// align.rs:241-252 and :258-286 on the device (sin and cos from the device library)
+// align.rs:241-252 and :258-286 on the device.  CR: the rotation's sin and cos
+// correctly rounded (sincos_cr, the host's bits: the maps the warps use);
+// otherwise the device library's (the search's cost evaluations, where a last
+// bit of the map moves a cost by about 1e-13 of itself and no decision).
+template <bool CR>
 __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
     double cx = (double)w / 2.0, cy = (double)h / 2.0;
-    double c = cos(s[3]), sn = sin(s[3]);
+    double c, sn;
+    if constexpr (CR) {
+        sincos_cr(s[3], &sn, &c);
+    } else {
+        c = cos(s[3]);
+        sn = sin(s[3]);
+    }
     double sc = s[2];
     double a = sc * c, b = sc * (c * s[5] - sn * s[4]), d = sc * sn, e = sc * (sn * s[5] + c * s[4]);
     double gx = s[6] / (double)w, gy = s[7] / (double)h;
@@ -280,8 +300,12 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_HSLOT 8192 // floats per held tile: its reference pixels, then its source window
 #define NM_HMARGIN 6  // pixels a held window extends past the first pass's bound
 // the state words: the Sim, the evaluation total, per level the evaluations and
-// block 0's ticks, the four phase tick totals, per level the passes
-#define NM_STATE (13 + 3 * NM_MAXL)
+// block 0's ticks, the four phase tick totals, per level the passes, the
+// participants and the traced costs
+#define NM_ST_PART (13 + 3 * NM_MAXL)
+#define NM_ST_TRACE (14 + 3 * NM_MAXL)
+#define NM_ST_HAND (15 + 3 * NM_MAXL)  // block 0's Sim after the solo levels, handed to the grid
+#define NM_STATE (23 + 3 * NM_MAXL)
 
 struct NmShared {
     double S[9][8], FV[9], S2[9][8], FV2[9], XR[8], XE[8], XC[8], C[8], lo[8], hi[8], cur[8], FR;
@@ -291,14 +315,24 @@ struct NmShared {
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
     int hwin[NM_HELD][5];   // the held tiles' windows (x0, y0, w, h, staged)
-    int grid;               // gridDim.x (read through nm_G: gridDim.x costs a load from the kernel's argument block)
+    int grid;               // the participating blocks (read through nm_G, as a scalar)
+    int rank;               // this block's place among them (nm_B)
     double tolv[8];         // the level's convergence tolerances
     int stop;
+    unsigned long long dig; // the digest of every cost this block has folded
+    int ntrace;             // the costs traced so far (rank 0)
 };
 
-// align.rs:630-634
-// the grid size, block-uniform (scalar registers)
+// The participating blocks and this block's rank among them, block-uniform
+// (scalar registers).  A search takes the blocks that are running when it
+// starts: normally the whole grid, but a plain launch has no promise that all
+// its blocks are resident at once (other work on the device, a profiler that
+// serialises dispatches), so each block takes a ticket as it starts and only
+// the first gridDim.x tickets issued before the set closes do the work.
 __device__ __forceinline__ int nm_G(const NmShared &sh) { return __builtin_amdgcn_readfirstlane(sh.grid); }
+__device__ __forceinline__ int nm_B(const NmShared &sh) { return __builtin_amdgcn_readfirstlane(sh.rank); }
+
+// align.rs:630-634
 
 __device__ __forceinline__ double rms_of(double sd, double sd2, double cnt) {
     if (cnt < 16.0) return 1e9;
@@ -366,7 +400,8 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const bool act = MODE == 1 || ty < TH;  // the wave has a row of the tile
     if (MODE != 0 || (held && !hold)) __syncthreads();  // the cleared accumulators (no barrier in the loop then)
-    for (int t = solo ? 0 : blockIdx.x; t < ntiles; t += solo ? 1 : G) {
+    const int B = nm_B(sh);
+    for (int t = solo ? 0 : B; t < ntiles; t += solo ? 1 : G) {
         const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
         const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
         int bx0 = 0, by0 = 0, bw = tw, bh = th, staged = 1;
@@ -383,7 +418,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 rv[r] = (r < R && i < npx) ? boxs[bw * bh + i] : 0.0f;
             }
         } else {
-            const int q = held ? (t - (int)blockIdx.x) / G : 0;  // a held tile's slot
+            const int q = held ? (t - B) / G : 0;  // a held tile's slot
             float *hs = hb + q * NM_HSLOT;
             int *hw = sh.hwin[q];
             if (!held || hold) {
@@ -528,7 +563,7 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         double v[8];
         for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
         for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = sh.PT[threadIdx.x][k];
-        sim_inverse_dev(v, tw, th, sh.inv[threadIdx.x]);
+        sim_inverse_dev<false>(v, tw, th, sh.inv[threadIdx.x]);
     }
     __syncthreads();
     uint64_t tq1 = wall_clock64();
@@ -584,37 +619,46 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
     }
     const unsigned slot = gen & 1;
     const int nv = 3 * np;
-    const int G = nm_G(sh);
+    const int G = nm_G(sh), B = nm_B(sh);
     double *pp = part + (size_t)slot * G * 3 * NM_MAXP;
     if (threadIdx.x == 0) {
-        // The partials are agent-scope atomic stores, written through to the
-        // level every XCD reads coherently; the workgroup release fence waits
-        // for them to complete (s_waitcnt) before the arrival count is raised,
-        // and every reader loads them with agent-scope atomic loads once it has
-        // seen the count.  So no XCD's L2 has to be written back or invalidated
-        // for them: an agent-scope release / acquire does both to the whole L2
-        // on every pass, which here (the side work's dirty lines in it, the
-        // level's samples lost from it) cost more than a small level's pass.
+        // The partials are agent-scope atomic stores: written through to the
+        // level every XCD reads coherently.  Their completion is waited for
+        // explicitly (s_waitcnt vmcnt(0): on gfx9 a store leaves the vector
+        // memory count only once the write is acknowledged at its scope)
+        // before the arrival count is raised; the workgroup release fence
+        // keeps the compiler from moving the stores past the count, but emits
+        // no wait of its own on this target.  Every reader loads them with
+        // agent-scope atomic loads once it has seen the count.  So no XCD's L2
+        // has to be written back or invalidated for them: an agent-scope
+        // release / acquire does both to the whole L2 on every pass, which
+        // here (the side work's dirty lines in it, the level's samples lost
+        // from it) cost more than a small level's pass.
         for (int k = 0; k < nv; k++)
-            __hip_atomic_store(&pp[(size_t)blockIdx.x * 3 * NM_MAXP + k], sh.sums[k], __ATOMIC_RELAXED,
+            __hip_atomic_store(&pp[(size_t)B * 3 * NM_MAXP + k], sh.sums[k], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        // two-level arrival: NM_GROUPS counters (one cache line each), then one
-        // counter of completed groups, which every block waits on (bounded, and
-        // left at once when any block has raised the error flag).  A group's
-        // last arriver has seen every increment of its group (each made after
-        // that block's partials were complete) when it raises the top count.
-        const unsigned gs = (unsigned)G / NM_GROUPS;
-        unsigned a = __hip_atomic_fetch_add(&ctr[NM_FL_GROUP + NM_LINE * (blockIdx.x % NM_GROUPS)], 1u,
-                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (a == (gen + 1) * gs - 1)
+        __builtin_amdgcn_s_waitcnt(0);
+        // two-level arrival: NM_GROUPS counters (one cache line each; group g
+        // holds the ranks congruent to g), then one counter of completed
+        // groups, which every block waits on (bounded, and left at once when
+        // any block has raised the error flag).  A group's last arriver has
+        // seen every increment of its group (each made after that block's
+        // partials were complete) when it raises the top count.
+        const unsigned g = (unsigned)B % NM_GROUPS;
+        const unsigned members = (unsigned)G / NM_GROUPS + (g < (unsigned)G % NM_GROUPS ? 1u : 0u);
+        const unsigned groups = (unsigned)min(G, NM_GROUPS);
+        unsigned a = __hip_atomic_fetch_add(&ctr[NM_FL_GROUP + NM_LINE * g], 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+        if (a == (gen + 1) * members - 1)
             __hip_atomic_fetch_add(&ctr[NM_FL_TOP], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned target = (gen + 1) * NM_GROUPS;
+        const unsigned target = (gen + 1) * groups;
+        const uint64_t t_in = wall_clock64();
         unsigned spins = 0;
         while (__hip_atomic_load(&ctr[NM_FL_TOP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1u << 24)) {
+            if ((++spins & 63u) == 0 && wall_clock64() - t_in > K.wait_ticks) {
                 atomicOr(err, 1u);
                 break;
             }
@@ -666,11 +710,25 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
     }
     gen++;
     if (threadIdx.x == 0) {
+        // the digest of the sums this block folded (FNV-1a over their bits;
+        // final since the fold's last barrier): every participant must end a
+        // search with the same one
+        unsigned long long d = sh.dig;
+        for (int k = 0; k < nv; k++) d = (d ^ (unsigned long long)__double_as_longlong(sh.sums[k])) * 1099511628211ull;
+        sh.dig = d;
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) sh.stop = 1;
         sh.tmark = wall_clock64();
         sh.ph[2] += sh.tmark - tq2;
     }
     __syncthreads();
+}
+
+// A counted cost into the trace (thread 0 of rank 0, when tracing), in the
+// order the reference evaluates them.
+__device__ __forceinline__ void nm_trace(const NmConst &K, NmShared &sh, double v) {
+    if (!K.trace || nm_B(sh) != 0) return;
+    if (sh.ntrace < K.trace_cap) K.trace[sh.ntrace] = v;
+    sh.ntrace++;
 }
 
 __device__ __forceinline__ double clampd_dev(double v, double lo, double hi) {
@@ -743,7 +801,10 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
         int next = -1;  // a follow-up pass within the iteration, or none
         if (state == INIT) {
             if (t0) {
-                for (int i = 0; i <= n; i++) sh.FV[i] = sh.PC[i];
+                for (int i = 0; i <= n; i++) {
+                    sh.FV[i] = sh.PC[i];
+                    nm_trace(K, sh, sh.PC[i]);
+                }
                 sh.lev_evals[lvl] += n + 1;
             }
         } else if (state == ITER) {
@@ -755,6 +816,7 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
             if (t0) {
                 sh.FR = fr;
                 sh.lev_evals[lvl] += 1;
+                nm_trace(K, sh, fr);
                 if (br == 1) {  // reflect
                     for (int k = 0; k < n; k++) sh.S[n][k] = sh.XR[k];
                     sh.FV[n] = fr;
@@ -765,12 +827,15 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
                     for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
                     sh.FV[n] = fe < fr ? fe : fr;
                     sh.lev_evals[lvl] += 1;
+                    nm_trace(K, sh, fe);
                 } else if (bc == 2) {  // contract (align.rs:853-862)
                     for (int k = 0; k < n; k++) sh.S[n][k] = sh.XC[k];
                     sh.FV[n] = fc;
                     sh.lev_evals[lvl] += 1;
+                    nm_trace(K, sh, fc);
                 } else {
                     sh.lev_evals[lvl] += 1;
+                    nm_trace(K, sh, fc);
                 }
             }
             if (!spec && br != 1) next = br == 0 ? EXPAND : CONTRACT;
@@ -782,6 +847,7 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
                 for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
                 sh.FV[n] = fe < fr ? fe : fr;
                 sh.lev_evals[lvl] += 1;
+                nm_trace(K, sh, fe);
             }
         } else if (state == CONTRACT) {
             const double fc = sh.PC[0];
@@ -793,11 +859,15 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
                     sh.FV[n] = fc;
                 }
                 sh.lev_evals[lvl] += 1;
+                nm_trace(K, sh, fc);
             }
             if (!take) next = SHRINK;
         } else {  // SHRINK
             if (t0) {
-                for (int i = 1; i <= n; i++) sh.FV[i] = sh.PC[i - 1];
+                for (int i = 1; i <= n; i++) {
+                    sh.FV[i] = sh.PC[i - 1];
+                    nm_trace(K, sh, sh.PC[i - 1]);
+                }
                 sh.lev_evals[lvl] += n;
             }
         }
@@ -872,11 +942,12 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
 }
 
 // state: [0..8) the initial Sim (the guess) in, the found Sim out; [8] the
-// evaluations; [9 + l] the evaluations at level l; [9 + NM_MAXL + l] block 0's
-// wall-clock ticks at level l.  The coarse levels (up to
-// NM_SOLO_MAX pixels) are searched by block 0 alone; it then hands the Sim to
-// the grid (state[0..8) and the hand-off generation), and all blocks search the
-// fine levels together.
+// evaluations; [9 + l] the evaluations at level l; [9 + NM_MAXL + l] rank 0's
+// wall-clock ticks at level l; then the phase ticks, the passes per level,
+// the participants, the traced costs and the hand-off Sim.  The coarse levels
+// (up to solo_max pixels) are searched by rank 0 alone; it then hands the Sim
+// to the grid (state[NM_ST_HAND..] and the hand-off flag), and all
+// participants search the fine levels together.
 __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restrict__ Kp, double *__restrict__ state,
                                                           double *__restrict__ part, unsigned *__restrict__ ctr,
                                                           unsigned *__restrict__ err) {
@@ -888,23 +959,76 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
     unsigned gen = 0;
     const bool t0 = threadIdx.x == 0;
     if (t0) {
-        for (int k = 0; k < 8; k++) sh.cur[k] = state[k];
-        for (int k = 0; k < n; k++) {  // the search box about the guess (align.rs:912-913)
-            double iv = state[K.free_idx[k]];
-            sh.lo[k] = iv - K.span[K.free_idx[k]];
-            sh.hi[k] = iv + K.span[K.free_idx[k]];
-        }
         sh.stop = 0;
-        sh.grid = (int)gridDim.x;
-        for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = sh.lev_passes[l] = 0;
-        for (int q = 0; q < 4; q++) sh.ph[q] = 0;
-        sh.tmark = wall_clock64();
-        if (blockIdx.x == 0)
-            for (int l = 0; l < NM_MAXL; l++) state[9 + NM_MAXL + l] = 0.0;
+        sh.rank = -1;
+        sh.grid = 0;
+        // an earlier search of a stream that failed leaves the flag raised:
+        // every later one leaves at once (the host reports the failure)
+        if (!__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            // the participants: a ticket per block as it starts; ticket 0
+            // closes the set once every block it may take has joined, or
+            // when none has joined for join_ticks, and publishes its size
+            const unsigned cap = K.max_part > 0 ? min(gridDim.x, (unsigned)K.max_part) : gridDim.x;
+            const unsigned tk = __hip_atomic_fetch_add(&ctr[NM_FL_JOIN], 1u, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            unsigned G = 0;
+            if (tk == 0) {
+                unsigned seen = 1;
+                uint64_t t_last = wall_clock64();
+                for (;;) {
+                    const unsigned j = __hip_atomic_load(&ctr[NM_FL_JOIN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (j >= cap) {
+                        seen = cap;
+                        break;
+                    }
+                    if (j != seen) {
+                        seen = j;
+                        t_last = wall_clock64();
+                    } else if (wall_clock64() - t_last > K.join_ticks) {
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                G = seen;
+                __hip_atomic_store(&ctr[NM_FL_CLOSED], G | 0x80000000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (tk < cap) {
+                const uint64_t t_in = wall_clock64();
+                unsigned v = 0, spins = 0;
+                while ((v = __hip_atomic_load(&ctr[NM_FL_CLOSED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 63u) == 0 && wall_clock64() - t_in > K.wait_ticks) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                }
+                G = v & 0x7fffffffu;
+            }
+            if (tk < G) {
+                sh.rank = (int)tk;
+                sh.grid = (int)G;
+            }
+        }
+        if (sh.rank >= 0) {
+            for (int k = 0; k < 8; k++) sh.cur[k] = state[k];
+            for (int k = 0; k < n; k++) {  // the search box about the guess (align.rs:912-913)
+                double iv = state[K.free_idx[k]];
+                sh.lo[k] = iv - K.span[K.free_idx[k]];
+                sh.hi[k] = iv + K.span[K.free_idx[k]];
+            }
+            for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = sh.lev_passes[l] = 0;
+            for (int q = 0; q < 4; q++) sh.ph[q] = 0;
+            sh.dig = 14695981039346656037ull;
+            sh.ntrace = 0;
+            sh.tmark = wall_clock64();
+            if (sh.rank == 0)
+                for (int l = 0; l < NM_MAXL; l++) state[9 + NM_MAXL + l] = 0.0;
+        }
     }
     __syncthreads();
+    if (sh.rank < 0) return;  // not a participant (late, beyond the cap, or a failed stream)
+    const bool r0 = nm_B(sh) == 0;
     // the levels, coarse to fine; those of at most solo_max pixels (the
-    // coarsest) by block 0 alone, which then hands the Sim to the grid
+    // coarsest) by rank 0 alone, which then hands the Sim to the grid
     bool handed = false;
     for (int lvl = K.nlev - 1; n > 0 && lvl >= K.finest && !sh.stop; lvl--) {
         const bool solo = (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.solo_max;
@@ -912,34 +1036,38 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
             handed = true;
             unsigned *hflag = ctr + NM_FL_HAND;
             if (t0) {
-                if (blockIdx.x == 0) {
-                    for (int k = 0; k < 8; k++) __hip_atomic_store(&state[k], sh.cur[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (r0) {
+                    for (int k = 0; k < 8; k++)
+                        __hip_atomic_store(&state[NM_ST_HAND + k], sh.cur[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __threadfence();
                     __hip_atomic_store(hflag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
+                    const uint64_t t_in = wall_clock64();
                     unsigned spins = 0;
                     while (__hip_atomic_load(hflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                         __builtin_amdgcn_s_sleep(1);
-                        if (++spins > (1u << 24)) {
+                        if ((++spins & 63u) == 0 && wall_clock64() - t_in > K.wait_ticks) {
                             atomicOr(err, 1u);
                             break;
                         }
                     }
                     __threadfence();
-                    for (int k = 0; k < 8; k++) sh.cur[k] = __hip_atomic_load(&state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (int k = 0; k < 8; k++)
+                        sh.cur[k] = __hip_atomic_load(&state[NM_ST_HAND + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) sh.stop = 1;
                 }
             }
             __syncthreads();
             if (sh.stop) break;
         }
-        if (solo && blockIdx.x != 0) continue;
+        if (solo && !r0) continue;
         uint64_t t = wall_clock64();
         nm_level(K, sh, boxs, hb, lvl, solo, part, ctr, err, gen);
-        if (t0 && blockIdx.x == 0) state[9 + NM_MAXL + lvl] = (double)(wall_clock64() - t);
+        if (t0 && r0) state[9 + NM_MAXL + lvl] = (double)(wall_clock64() - t);
     }
-    if (blockIdx.x == 0 && t0) {
+    if (t0 && K.digest) K.digest[nm_B(sh)] = sh.dig;
+    if (r0 && t0) {
         double tot = 0.0;
         for (int l = 0; l < NM_MAXL; l++) {
             state[9 + l] = (double)sh.lev_evals[l];
@@ -949,13 +1077,18 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
         state[8] = tot;
         for (int q = 0; q < 4; q++) state[9 + 2 * NM_MAXL + q] = (double)sh.ph[q];
         for (int l = 0; l < NM_MAXL; l++) state[13 + 2 * NM_MAXL + l] = (double)sh.lev_passes[l];
+        state[NM_ST_PART] = (double)nm_G(sh);
+        state[NM_ST_TRACE] = (double)sh.ntrace;
     }
 }
 
-// clears the arrival counter, the error flag and the broadcast generation
-// before a search (a kernel, so the sequence can be captured in a graph)
-__global__ void k_nm_reset(unsigned *flags) {
-    for (int i = threadIdx.x; i < NM_FL_WORDS; i += blockDim.x) flags[i] = 0;
+// clears the arrival counters, the join count and the hand-off flag before a
+// search (a kernel, so the sequence can be captured in a graph), and the error
+// flag unless keep_err (a stream's later frames: one failed search fails the
+// rest at once, and the host reports it after the last frame)
+__global__ void k_nm_reset(unsigned *flags, int keep_err) {
+    for (int i = threadIdx.x; i < NM_FL_WORDS; i += blockDim.x)
+        if (!(keep_err && i == NM_FL_ERR)) flags[i] = 0;
 }
 
 // align.rs:416-474 in f64: the interpolation weights of kernel `ik` at fraction t
@@ -1064,7 +1197,7 @@ __global__ void k_frame_inv(const double *__restrict__ state, const unsigned *__
     for (int k = 0; k < 8; k++) s[k] = fp->sim[k] = state[k];
     fp->evals = state[8];
     fp->err = *err;
-    sim_inverse_dev(s, w, h, fp->inv.m);
+    sim_inverse_dev<true>(s, w, h, fp->inv.m);
     fp->affine = fp->inv.m[6] == 0.0 && fp->inv.m[7] == 0.0 && fp->inv.m[8] == 1.0;
     fp->is_id = s[0] == 0.0 && s[1] == 0.0 && s[2] == 1.0 && s[3] == 0.0 && s[4] == 1.0 && s[5] == 0.0 &&
                 s[6] == 0.0 && s[7] == 0.0;
@@ -1349,6 +1482,20 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
         K.span[k] = SPAN[k];
         if (free8[k]) K.free_idx[K.nfree++] = k;
     }
+    {
+        // the wait bounds in ticks of the device wall clock: a wait for the
+        // other participants ends with an error after 4 s (a pass takes well
+        // under a millisecond), and the set of participants closes 20 ms after
+        // the last block joined (any other kernel's blocks have long left the
+        // CUs by then, so a normal launch always gets its whole grid)
+        int dev = 0, khz = 0;
+        LSH_CHECK(hipGetDevice(&dev));
+        LSH_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        LSH_REQUIRE(khz > 0, "the device reports no wall clock");
+        K.wait_ticks = (uint64_t)khz * 4000u;
+        K.join_ticks = (uint64_t)khz * 20u;
+    }
+    if (const char *e = std::getenv("LSH_SEARCH_MAX_BLOCKS")) K.max_part = std::max(0, std::atoi(e));
     double W = (double)f->w, H = (double)f->h, r = std::max(W, H) / 2.0;
     const double gain[8] = {W, H, r, r, H / 2.0, H / 2.0, W / 4.0, H / 4.0};  // align.rs:766-770
     for (size_t l = 0; l < n; l++) {
@@ -1375,6 +1522,13 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
         LSH_REQUIRE(f->nm_grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
         if (f->nm_grid > NM_BOX / 2) f->nm_grid = NM_BOX / 2;  // the fold stages one value per block in LDS
         f->nm_part.alloc((size_t)2 * 3 * NM_MAXP * f->nm_grid + 2 * NM_MAXP);
+        f->nm_digest.alloc(f->nm_grid);
+    }
+    K.digest = (unsigned long long *)f->nm_digest.p;
+    if (f->trace_cap) {
+        f->nm_trace.ensure(f->trace_cap);
+        K.trace = f->nm_trace.p;
+        K.trace_cap = (int)f->trace_cap;
     }
     // upload only what changed (a stack's frames share one set)
     if (f->nm_host.size() == sizeof(NmConst) && std::memcmp(f->nm_host.data(), &K, sizeof(NmConst)) == 0) return;
@@ -1392,7 +1546,9 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
 // error rather than a hang if that ever fails.
 static void search_launch(lsh_frames *f, bool plain = false, hipEvent_t started = nullptr) {
     unsigned *ctr = f->nm_flags.p, *err = f->nm_flags.p + NM_FL_ERR;
-    launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p);
+    // a stream's searches (plain) keep a raised error flag: the frames after a
+    // failed one leave at once
+    launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p, plain ? 1 : 0);
     if (started) LSH_CHECK(hipEventRecord(started, stream()));
     const NmConst *kp = (const NmConst *)f->nm_const.p;
     double *state = f->nm_state.p, *part = f->nm_part.p;
@@ -1425,7 +1581,9 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     unsigned *flags = (unsigned *)(out + NM_STATE);
     LSH_CHECK(hipMemcpyAsync(flags, err, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
     sync();
-    LSH_REQUIRE(flags[0] == 0, "search: a cost broadcast timed out");
+    LSH_REQUIRE(flags[0] == 0, "search: a wait for the other blocks timed out");
+    f->search_part = (unsigned)out[NM_ST_PART];
+    f->trace_n = (size_t)out[NM_ST_TRACE];
     double bytes = 0.0;
     for (size_t l = 0; l < n; l++)  // a pass reads the level's reference and target once
         bytes += out[13 + 2 * NM_MAXL + l] * 8.0 * (double)(f->lw[l] * f->lh[l]);
@@ -1476,6 +1634,7 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
     f->rec_gains.ensure(nframes * 3);
     f->rec_evals.ensure(nframes);
     f->red.ensure((size_t)std::max(1u, std::min<unsigned>(cdiv(f->gw * f->gh, 256), 1024)) * 7);
+    launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p, 0);  // a fresh error flag
     launch("stream_cursor", k_cursor_set, dim3(1), dim3(64), 0, f->cursor.p, (unsigned)f->processed);
     launch("stream_state", k_state_set, dim3(1), dim3(64), 0, f->nm_state.p, Sim());  // frame 0 sits at the identity
 }
@@ -1744,6 +1903,35 @@ char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, u
         if (ticks) ticks[NM_MAXL + q] = f->phase_ticks[q];
     for (uint32_t l = 0; l < NM_MAXL && NM_MAXL + l < cap; l++)
         if (evals) evals[NM_MAXL + l] = f->lev_passes[l];
+    LSH_API_END
+}
+
+char *lsh_frames_search_trace(lsh_frames *f, uint32_t cap) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(f, "lsh_frames_search_trace: null handle");
+    f->trace_cap = cap;
+    f->trace_n = 0;
+    if (!cap) f->nm_trace.release();
+    LSH_API_END
+}
+
+char *lsh_frames_search_costs(lsh_frames *f, double *costs, uint32_t cap, uint32_t *n) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(f && n, "lsh_frames_search_costs: null argument");
+    LSH_REQUIRE(f->trace_cap, "lsh_frames_search_costs: tracing is off (lsh_frames_search_trace)");
+    LSH_REQUIRE(f->trace_n <= f->trace_cap, "lsh_frames_search_costs: the search counted more costs than traced");
+    *n = (uint32_t)f->trace_n;
+    const size_t m = std::min<size_t>(cap, f->trace_n);
+    if (costs && m) copy_mem(costs, LSH_HOST, f->nm_trace.p, LSH_DEVICE, m * sizeof(double));
+    LSH_API_END
+}
+
+char *lsh_frames_search_blocks(lsh_frames *f, uint64_t *digests, uint32_t cap, uint32_t *n) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(f && n, "lsh_frames_search_blocks: null argument");
+    *n = f->search_part;
+    const size_t m = std::min<size_t>(cap, f->search_part);
+    if (digests && m) copy_mem(digests, LSH_HOST, f->nm_digest.p, LSH_DEVICE, m * sizeof(uint64_t));
     LSH_API_END
 }
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/lsh.h"
+#include "sincos.hpp"
 
 namespace lsh {
 

@@ -54,6 +54,10 @@ struct lsh_frames {
     lsh::DBuf<double> nm_part;    // per-block partial sums, two slots
     unsigned nm_grid = 0;
     unsigned nm_bpc = 1;  // the search's blocks per CU
+    lsh::DBuf<unsigned long long> nm_digest;  // per participant of the last search: its digest of the folded sums
+    lsh::DBuf<double> nm_trace;               // the counted costs of the last search (tracing on)
+    size_t trace_cap = 0, trace_n = 0;        // tracing's capacity (0: off); costs the last search counted
+    unsigned search_part = 0;                 // participants of the last host-steered search
     uint64_t lev_evals[16] = {0}, lev_ticks[16] = {0};  // the last search, per registration level
     uint64_t lev_passes[16] = {0};                       // its collective passes (several evaluations each)
     std::vector<uint8_t> nm_host;                        // the search constants last uploaded

@@ -112,7 +112,8 @@ void copy_mem(void *dst, lsh_mem dmem, const void *src, lsh_mem smem, size_t byt
 void sim_matrix(const Sim &sim, size_t w, size_t h, double *m) {
     const double *s = sim.v;
     double cx = (double)w / 2.0, cy = (double)h / 2.0;
-    double c = std::cos(s[3]), sn = std::sin(s[3]);
+    double c, sn;
+    sincos_cr(s[3], &sn, &c);  // correctly rounded: the device's maps take the same bits
     double sc = s[2];
     double a = sc * c, b = sc * (c * s[5] - sn * s[4]), d = sc * sn, e = sc * (sn * s[5] + c * s[4]);
     double gx = s[6] / (double)w, gy = s[7] / (double)h;

@@ -692,10 +692,53 @@ static int sim_is_id(const double *s) {
     return 1;
 }
 
+/* sin and cos of a rotation, correctly rounded (align.rs:243 takes f64::cos / f64::sin, the platform's
+ * libm, whose last bit varies between libraries: glibc 2.35 is one ulp from the correctly rounded value
+ * on about 0.3 % of arguments in [-0.3, 0.3]).  For |x| <= pi/4 the Taylor series is summed term by
+ * term in double-double arithmetic (Dekker's product through fma, Knuth's two-sum): t_k = -t_{k-1} x^2 /
+ * ((2k)(2k+1)) for the sine from t_0 = x, and t_k = -t_{k-1} x^2 / ((2k-1)(2k)) for the cosine from 1,
+ * until a term falls below 2^-110 of the sum, then rounded once; beyond pi/4 libm. */
+typedef struct { double h, l; } ora_dd;
+static inline ora_dd dd_fast(double a, double b) { ora_dd r; r.h = a + b; r.l = b - (r.h - a); return r; }
+static inline ora_dd dd_sum(ora_dd a, ora_dd b) {
+    double s = a.h + b.h, bb = s - a.h, e = (a.h - (s - bb)) + (b.h - bb);
+    return dd_fast(s, e + a.l + b.l);
+}
+static inline ora_dd dd_mul(ora_dd a, ora_dd b) {
+    double p = a.h * b.h, e = fma(a.h, b.h, -p);
+    return dd_fast(p, e + (a.h * b.l + a.l * b.h));
+}
+static inline ora_dd dd_div_d(ora_dd a, double b) {
+    double q1 = a.h / b, p = q1 * b, e = fma(q1, b, -p);
+    double r = ((a.h - p) - e) + a.l;
+    return dd_fast(q1, r / b);
+}
+static double dd_series(double x, int odd) {
+    ora_dd x2 = {x * x, fma(x, x, -(x * x))};
+    ora_dd t = {odd ? x : 1.0, 0.0}, s = t;
+    for (int k = 1; k < 40; k++) {
+        double den = odd ? (double)(2 * k) * (double)(2 * k + 1) : (double)(2 * k - 1) * (double)(2 * k);
+        t = dd_div_d(dd_mul(t, x2), -den);
+        s = dd_sum(s, t);
+        if (fabs(t.h) < 0x1p-110 * fabs(s.h)) break;
+    }
+    return s.h;
+}
+void ora_sincos(double x, double *s, double *c) {
+    if (!(fabs(x) <= 0.78539816339744828)) {
+        *s = sin(x);
+        *c = cos(x);
+        return;
+    }
+    *s = dd_series(x, 1);
+    *c = dd_series(x, 0);
+}
+
 /* align.rs:241-252 */
 void ora_sim_matrix(const double *s, size_t w, size_t h, double *m) {
     double cx = (double)w / 2.0, cy = (double)h / 2.0;
-    double c = cos(s[3]), sn = sin(s[3]);
+    double c, sn;
+    ora_sincos(s[3], &sn, &c);
     double sc = s[2];
     double a = sc * c, b = sc * (c * s[5] - sn * s[4]), d = sc * sn, e = sc * (sn * s[5] + c * s[4]);
     double gx = s[6] / (double)w, gy = s[7] / (double)h;
@@ -1144,6 +1187,8 @@ typedef struct {
     size_t tw, th;
     double cur[8];
     size_t free_idx[8], nfree;
+    double *trace; /* every evaluated cost in order (nullable) */
+    size_t trace_cap, *trace_n;
 } align_ctx;
 
 static double align_cost(const double *xf, void *vctx) {
@@ -1151,12 +1196,27 @@ static double align_cost(const double *xf, void *vctx) {
     double v[8];
     memcpy(v, c->cur, sizeof(v));
     for (size_t k = 0; k < c->nfree; k++) v[c->free_idx[k]] = xf[k];
-    return ora_warp_cost(c->a, c->aw, c->t, c->tw, c->th, v);
+    double cost = ora_warp_cost(c->a, c->aw, c->t, c->tw, c->th, v);
+    if (c->trace_n) {
+        if (*c->trace_n < c->trace_cap) c->trace[*c->trace_n] = cost;
+        (*c->trace_n)++;
+    }
+    return cost;
 }
 
-/* align.rs:896-942 */
 void ora_multiscale_align(const float *rf, const float *tg, size_t w, size_t h, const double *init,
                           const uint8_t *free8, size_t coarsen, double *out, uint32_t *evals) {
+    ora_multiscale_align_trace(rf, tg, w, h, init, free8, coarsen, out, evals, NULL, NULL, 0, NULL);
+}
+
+/* align.rs:896-942; lev_evals: the evaluations per level (level 0 = full resolution, MAXL entries);
+ * trace: the costs in the order nelder_mead evaluates them (up to trace_cap; *trace_n the count) */
+void ora_multiscale_align_trace(const float *rf, const float *tg, size_t w, size_t h, const double *init,
+                                const uint8_t *free8, size_t coarsen, double *out, uint32_t *evals,
+                                uint32_t *lev_evals, double *trace, size_t trace_cap, size_t *trace_n) {
+    if (trace_n) *trace_n = 0;
+    if (lev_evals)
+        for (int l = 0; l < ORA_MAXL; l++) lev_evals[l] = 0;
     size_t dims[2 * MAXL];
     size_t nl = ora_gauss_count(w, h, dims);
     size_t tot = 0;
@@ -1197,12 +1257,16 @@ void ora_multiscale_align(const float *rf, const float *tg, size_t w, size_t h, 
         memcpy(c.cur, cur, sizeof(cur));
         memcpy(c.free_idx, free_idx, sizeof(free_idx));
         c.nfree = nfree;
+        c.trace = trace;
+        c.trace_cap = trace_cap;
+        c.trace_n = trace_n;
         double x0[8], step[8], tol[8], best[8];
         for (size_t k = 0; k < nfree; k++) x0[k] = cur[free_idx[k]];
         ora_level_steps(free_idx, nfree, (size_t)lvl, w, h, step, tol);
         uint32_t ne = 0;
         ora_nelder_mead(align_cost, &c, nfree, x0, lo, hi, step, tol, best, &ne);
         tot_evals += ne;
+        if (lev_evals && lvl < ORA_MAXL) lev_evals[lvl] = ne;
         for (size_t k = 0; k < nfree; k++) cur[free_idx[k]] = best[k];
     }
     memcpy(out, cur, sizeof(cur));
@@ -2378,7 +2442,7 @@ void ora_planes_to_u16(const float *img3, size_t n, uint16_t *rgb) {
 int ora_stack_run(const uint16_t *frames, size_t n, size_t w, size_t h, const ora_fuse_params *fp,
                   const ora_stack_params *sp, const ora_depth_params *dp, float *img3_out, float *winner_full,
                   double *sims, float *gains, float *depth, float *conf, float *slices_out, ora_depth_out *dinfo,
-                  ora_stack_times *times) {
+                  ora_stack_times *times, uint32_t *evals) {
     if (n == 0) return -1;
     size_t N = w * h;
     double t_all = now_s();
@@ -2395,6 +2459,7 @@ int ora_stack_run(const uint16_t *frames, size_t n, size_t w, size_t h, const or
     double last_sim[8], ref_means[3];
     memcpy(last_sim, SIM_ID, sizeof(last_sim));
     int aligned = sp->align && n > 1;
+    if (evals) memset(evals, 0, n * (1 + ORA_MAXL) * sizeof(uint32_t));
     for (size_t i = 0; i < n; i++) {
         ora_u16_to_planes(frames + i * 3 * N, N, img);
         double sim[8];
@@ -2404,7 +2469,10 @@ int ora_stack_run(const uint16_t *frames, size_t n, size_t w, size_t h, const or
             double t0 = now_s();
             ora_luma(img, N, y);
             if (prev_ref) {
-                ora_multiscale_align(prev_ref, y, w, h, last_sim, sp->free8, sp->coarsen, sim, NULL);
+                ora_multiscale_align_trace(prev_ref, y, w, h, last_sim, sp->free8, sp->coarsen, sim, NULL,
+                                           evals ? evals + i * (1 + ORA_MAXL) + 1 : NULL, NULL, 0, NULL);
+                if (evals)
+                    for (int l = 0; l < ORA_MAXL; l++) evals[i * (1 + ORA_MAXL)] += evals[i * (1 + ORA_MAXL) + 1 + l];
             }
             if (i + 1 < n) {
                 if (!prev_ref) prev_ref = fzeros(N);

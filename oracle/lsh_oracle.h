@@ -88,6 +88,8 @@ void ora_fuser_finish(ora_fuser *f, float *img3, float *winner_full, uint16_t *w
 void ora_fuser_free(ora_fuser *f);
 
 /* ---- align.rs ---- */
+/* sin and cos, correctly rounded for |x| <= pi/4 (the rotation of every Sim matrix) */
+void ora_sincos(double x, double *s, double *c);
 void ora_sim_matrix(const double *sim, size_t w, size_t h, double *m9);
 void ora_inverse(const double *m9, double *inv9);
 void ora_map(const double *inv9, double x, double y, double *sx, double *sy);
@@ -103,6 +105,10 @@ void ora_level_steps(const size_t *free_idx, size_t nfree, size_t lvl, size_t w,
 typedef double (*ora_cost_fn)(const double *x, void *ctx);
 void ora_nelder_mead(ora_cost_fn f, void *ctx, size_t n, const double *x0, const double *lo, const double *hi,
                      const double *step, const double *tol, double *out, uint32_t *evals);
+#define ORA_MAXL 16
+void ora_multiscale_align_trace(const float *rf, const float *tg, size_t w, size_t h, const double *init,
+                                const uint8_t *free8, size_t coarsen, double *out, uint32_t *evals,
+                                uint32_t *lev_evals, double *trace, size_t trace_cap, size_t *trace_n);
 void ora_multiscale_align(const float *rf, const float *tg, size_t w, size_t h, const double *init,
                           const uint8_t *free8, size_t coarsen, double *out, uint32_t *evals);
 void ora_common_area(const double *sims, size_t n, size_t w, size_t h, uint32_t interp, size_t *rect4);
@@ -187,7 +193,8 @@ typedef struct ora_stack_times {
 int ora_stack_run(const uint16_t *frames, size_t n, size_t w, size_t h, const ora_fuse_params *fp,
                   const ora_stack_params *sp, const ora_depth_params *dp, float *img3, float *winner_full,
                   double *sims, float *gains, float *depth, float *conf, float *slices,
-                  ora_depth_out *dinfo, ora_stack_times *times);
+                  ora_depth_out *dinfo, ora_stack_times *times,
+                  uint32_t *evals /* nullable: per frame 1 + ORA_MAXL: the search's total, then per level */);
 
 /* io.rs:89 (u16 -> f32 planes) and io.rs:287 (f32 -> u16) */
 void ora_u16_to_planes(const uint16_t *rgb, size_t n, float *img3);

@@ -150,6 +150,9 @@ _sig("ora_gauss_levels", None, _p, _sz, _sz, _p)
 _sig("ora_level_steps", None, _p, _sz, _sz, _sz, _sz, _p, _p)
 _sig("ora_nelder_mead", None, COST_FN, _p, _sz, _p, _p, _p, _p, _p, _p, C.POINTER(C.c_uint32))
 _sig("ora_multiscale_align", None, _p, _p, _sz, _sz, _p, _p, _sz, _p, C.POINTER(C.c_uint32))
+_sig("ora_multiscale_align_trace", None, _p, _p, _sz, _sz, _p, _p, _sz, _p, C.POINTER(C.c_uint32), _p, _p, _sz,
+     C.POINTER(C.c_size_t))
+_sig("ora_sincos", None, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("ora_common_area", None, _p, _sz, _sz, _sz, C.c_uint32, _p)
 _sig("ora_bright_grid", None, _p, _sz, _sz, _p)
 _sig("ora_bright_gains", None, _p, _p, _sz, _sz, _p, _p)
@@ -187,7 +190,7 @@ _sig("ora_slab_ranges", _sz, _sz, _sz, _sz, _p, _sz)
 _sig("ora_depth_from_slices", C.c_int, _p, _sz, _p, _sz, _sz, C.POINTER(DepthParams), _p, _p, _p,
      C.POINTER(DepthOut))
 _sig("ora_stack_run", C.c_int, _p, _sz, _sz, _sz, C.POINTER(FuseParams), C.POINTER(StackParams),
-     C.POINTER(DepthParams), _p, _p, _p, _p, _p, _p, _p, C.POINTER(DepthOut), C.POINTER(StackTimes))
+     C.POINTER(DepthParams), _p, _p, _p, _p, _p, _p, _p, C.POINTER(DepthOut), C.POINTER(StackTimes), _p)
 _sig("ora_u16_to_planes", None, _p, _sz, _p)
 _sig("ora_planes_to_u16", None, _p, _sz, _p)
 
@@ -463,6 +466,33 @@ def multiscale_align(rf, tg, init=None, free=(1, 1, 1, 1, 0, 0, 0, 0), coarsen=0
     ev = C.c_uint32()
     _L.ora_multiscale_align(_ptr(rf), _ptr(tg), w, h, _ptr(init), _ptr(fr), coarsen, _ptr(out), C.byref(ev))
     return out, ev.value
+
+
+MAXL = 16
+
+
+def multiscale_align_trace(rf, tg, init=None, free=(1, 1, 1, 1, 0, 0, 0, 0), coarsen=0, cap=1 << 16):
+    """multiscale_align with its evaluations per level (level 0 = full resolution)
+    and every cost in the order nelder_mead evaluates them."""
+    rf, tg = _f32(rf), _f32(tg)
+    h, w = rf.shape
+    init = _f64(SIM_ID if init is None else init)
+    fr = np.ascontiguousarray(free, np.uint8)
+    out = np.zeros(8)
+    ev = C.c_uint32()
+    lev = np.zeros(MAXL, np.uint32)
+    costs = np.zeros(cap)
+    n = C.c_size_t()
+    _L.ora_multiscale_align_trace(_ptr(rf), _ptr(tg), w, h, _ptr(init), _ptr(fr), coarsen, _ptr(out), C.byref(ev),
+                                  _ptr(lev), _ptr(costs), cap, C.byref(n))
+    return out, ev.value, lev, costs[:min(n.value, cap)].copy()
+
+
+def sincos(x):
+    """The oracle's sin and cos of a rotation (correctly rounded for |x| <= pi/4)."""
+    s, c = C.c_double(), C.c_double()
+    _L.ora_sincos(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def common_area(sims, w, h, interp="spline4x4"):
@@ -754,10 +784,11 @@ def stack_run(frames, fuse=None, stack=None, depth=None, want_slices=False):
     k = 1 << dp.scale
     slices = np.zeros((n, -(-h // k), -(-w // k)), np.float32) if (want_slices and sp.depth) else None
     info, times = DepthOut(), StackTimes()
+    evals = np.zeros((n, 1 + MAXL), np.uint32)
     rc = _L.ora_stack_run(_ptr(frames), n, w, h, C.byref(fp), C.byref(sp), C.byref(dp), _ptr(img), _ptr(win),
                           _ptr(sims), _ptr(gains), _ptr(dmap), _ptr(conf), _ptr(slices), C.byref(info),
-                          C.byref(times))
+                          C.byref(times), _ptr(evals))
     if rc != 0:
         raise ValueError("stack run failed")
     return dict(image=img, winner=win, sims=sims, gains=gains, depth=dmap, conf=conf, slices=slices, info=info,
-                times=times)
+                times=times, evals=evals[:, 0].astype(np.int64), level_evals=evals[:, 1:].astype(np.int64))
