@@ -27,7 +27,7 @@ FLAGS = [
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+    "-Wall", "-Wshadow", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
 

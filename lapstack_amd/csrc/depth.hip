@@ -482,15 +482,15 @@ __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ t
         const bool two = yb < h;
         const int loa = max(ya - r, 0), hia = min(ya + r, h - 1);
         const int lob = max(yb - r, 0), hib = min(yb + r, h - 1);
-        double sa = 0.0, sb = 0.0, ta = 0.0, tb = 0.0;
+        double sa = 0.0, sb = 0.0, ua = 0.0, ub = 0.0;
         if (two && hia - loa == hib - lob) {
             const float *pa = Sa + (loa - r0) * BV_TX + tx, *pb = Sb + (loa - r0) * BV_TX + tx;
             const int d4 = (lob - loa) * BV_TX;
             for (int i = 0; i <= hia - loa; i++) {
                 sa += (double)pa[i * BV_TX];
                 sb += (double)pb[i * BV_TX];
-                ta += (double)pa[i * BV_TX + d4];
-                tb += (double)pb[i * BV_TX + d4];
+                ua += (double)pa[i * BV_TX + d4];
+                ub += (double)pb[i * BV_TX + d4];
             }
         } else {
             for (int i = loa; i <= hia; i++) {
@@ -499,12 +499,12 @@ __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ t
             }
             if (two)
                 for (int i = lob; i <= hib; i++) {
-                    ta += (double)Sa[(i - r0) * BV_TX + tx];
-                    tb += (double)Sb[(i - r0) * BV_TX + tx];
+                    ua += (double)Sa[(i - r0) * BV_TX + tx];
+                    ub += (double)Sb[(i - r0) * BV_TX + tx];
                 }
         }
         box_v_tail(x, ya, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
-        if (two) box_v_tail(x, yb, w, h, r, mode, ta, tb, guide, mean_i, var_i, eps, oa, ob, P, m);
+        if (two) box_v_tail(x, yb, w, h, r, mode, ua, ub, guide, mean_i, var_i, eps, oa, ob, P, m);
     }
 }
 
@@ -657,7 +657,11 @@ static const dim3 B2(64, 4);
 static unsigned g1(size_t n) { return cdiv(n, 256); }
 
 // The two box passes: the LDS-tiled kernels when the radius fits their tiles,
-// else the direct ones (the same sums either way).
+// else the direct ones (the same window sums either way).  Each window is
+// summed directly in f64, where the reference keeps an f64 running sum along
+// the row (depth.rs:293-331); the two agree whenever the f64 sums are exact,
+// which they are for f32 planes whose values span fewer than about 2^24 in
+// magnitude over a window's row (the rounding to f32 then sees the same value).
 static void box_h2(const char *name, double bytes, const float *a, const float *b, const float *g, int mode, int w,
                    int h, int r, float *oa, float *ob) {
     if (r <= BOX_TILE_MAX_R && cdiv(h, BH_RB) <= 65535) {

@@ -120,8 +120,8 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         LSH_CHECK(hipGetDevice(&dev));
         LSH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         unsigned search_cus = (unsigned)ncu * 3 / 4;
-        if (const char *e = std::getenv("LSH_STREAM_SEARCH_CUS"))
-            if (std::atoi(e) > 0) search_cus = (unsigned)std::atoi(e);
+        if (const char *env = std::getenv("LSH_STREAM_SEARCH_CUS"))
+            if (std::atoi(env) > 0) search_cus = (unsigned)std::atoi(env);
         OnStream on_chain(stream(), search_cus);
         const hipStream_t Ms = stream(), Ss = side_stream();
         Events E;

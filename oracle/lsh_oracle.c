@@ -2226,14 +2226,21 @@ void ora_wls_free(ora_wls *s) {
     free(s);
 }
 
-/* depth.rs:1119-1200 */
-int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, size_t w, size_t h,
-                          const ora_depth_params *p, float *depth, float *conf_out, float *floor_plane,
-                          ora_depth_out *info) {
-    if (n == 0) return -1;
+/* depth.rs:1168: the Huber IRLS weights, conf * min(tau / |d - u|, 1) */
+void ora_huber_weights(const float *conf, const float *d, const float *u, size_t n, float tau, float *out) {
+    for (size_t i = 0; i < n; i++) out[i] = conf[i] * rminf(tau / fabsf(d[i] - u[i]), 1.0f);
+}
+
+/* depth.rs:1125-1159: the aggregation and the streamed peak search over the n
+ * slices (guide = block-mean fused luma), the median, the p90 normalisation;
+ * guide, depth, conf (dw*dh each) are the WLS system's inputs; mins the per-cell
+ * minima, *floor the noise floor, *p90 the normalisation.  Returns the aggregation's
+ * and the peak search's wall time in t2 (nullable). */
+static void depth_grid_(const float *slices, size_t n, const float *fused3, size_t w, size_t h,
+                        const ora_depth_params *p, float *y_full, float *guide, float *depth_w, float *conf,
+                        float *mins, float *floor_, float *p90, double *t2) {
     size_t k = (size_t)1 << p->scale;
     size_t dw = blocks(w, k), dh = blocks(h, k), M = dw * dh;
-    float *y_full = fzeros(w * h), *guide = fzeros(M);
     ora_luma(fused3, w * h, y_full);
     ora_block_mean(y_full, w, h, k, guide);
     double t_agg = now_s();
@@ -2255,17 +2262,44 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
     free(agg);
     t_agg = now_s() - t_agg;
     double t_peaks = now_s();
-    float *depth_w = fzeros(M), *conf = fzeros(M), *mins = fzeros(M), floor_;
-    ora_peaks_finish(tr, p->gate, depth_w, conf, mins, &floor_);
+    ora_peaks_finish(tr, p->gate, depth_w, conf, mins, floor_);
     ora_peaks_free(tr);
     if (p->median) {
         float *m2 = fzeros(M);
         ora_median3(depth_w, dw, dh, m2);
-        free(depth_w);
-        depth_w = m2;
+        memcpy(depth_w, m2, M * sizeof(float));
+        free(m2);
     }
-    float p90 = ora_normalize_conf(conf, M);
-    t_peaks = now_s() - t_peaks;
+    *p90 = ora_normalize_conf(conf, M);
+    if (t2) {
+        t2[0] = t_agg;
+        t2[1] = now_s() - t_peaks;
+    }
+}
+
+int ora_depth_grid(const float *slices, size_t n, const float *fused3, size_t w, size_t h, const ora_depth_params *p,
+                   float *guide, float *depth, float *conf, float *p90) {
+    if (n == 0) return -1;
+    size_t k = (size_t)1 << p->scale, M = blocks(w, k) * blocks(h, k);
+    float *y_full = fzeros(w * h), *mins = fzeros(M), floor_;
+    depth_grid_(slices, n, fused3, w, h, p, y_full, guide, depth, conf, mins, &floor_, p90, NULL);
+    free(y_full);
+    free(mins);
+    return 0;
+}
+
+/* depth.rs:1119-1200 */
+int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, size_t w, size_t h,
+                          const ora_depth_params *p, float *depth, float *conf_out, float *floor_plane,
+                          ora_depth_out *info) {
+    if (n == 0) return -1;
+    size_t k = (size_t)1 << p->scale;
+    size_t dw = blocks(w, k), dh = blocks(h, k), M = dw * dh;
+    float *y_full = fzeros(w * h), *guide = fzeros(M);
+    float *depth_w = fzeros(M), *conf = fzeros(M), *mins = fzeros(M), floor_, p90;
+    double t2[2];
+    depth_grid_(slices, n, fused3, w, h, p, y_full, guide, depth_w, conf, mins, &floor_, &p90, t2);
+    double t_agg = t2[0], t_peaks = t2[1];
     double t_wls = now_s();
     uint32_t it1 = 0, it2 = 0;
     float rel1 = 0.0f, rel2 = 0.0f;
@@ -2278,7 +2312,7 @@ int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, si
         it2 = it1;
         if (p->robust > 0.0f) {
             float *w2 = fzeros(M);
-            for (size_t i = 0; i < M; i++) w2[i] = conf[i] * rminf(p->robust / fabsf(depth_w[i] - u[i]), 1.0f);
+            ora_huber_weights(conf, depth_w, u, M, p->robust, w2);
             ora_wls_set_weights(s, w2);
             free(w2);
             it2 = ora_wls_solve(s, depth_w, p->cg_iters, p->tol, u, &rel2);

@@ -161,6 +161,12 @@ typedef struct ora_depth_out {
     double agg_s, peaks_s, wls_s, up_s; /* wall time of the phases */
 } ora_depth_out;
 /* slices: n planes of dw*dh.  depth/conf: w*h.  floor_plane: dw*dh (nullable). */
+/* depth.rs:1168 Huber IRLS weights: out = conf * min(tau / |d - u|, 1) */
+void ora_huber_weights(const float *conf, const float *d, const float *u, size_t n, float tau, float *out);
+/* depth.rs:1125-1159, the depth pass up to the WLS solve: guide, peak depth (after the median) and
+ * p90-normalised confidence, dw*dh each; *p90 the normalisation.  Returns 0 on success. */
+int ora_depth_grid(const float *slices, size_t n, const float *fused3, size_t w, size_t h, const ora_depth_params *p,
+                   float *guide, float *depth, float *conf, float *p90);
 int ora_depth_from_slices(const float *slices, size_t n, const float *fused3, size_t w, size_t h,
                           const ora_depth_params *p, float *depth, float *conf, float *floor_plane,
                           ora_depth_out *info);

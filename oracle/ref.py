@@ -153,6 +153,8 @@ _sig("ora_multiscale_align", None, _p, _p, _sz, _sz, _p, _p, _sz, _p, C.POINTER(
 _sig("ora_multiscale_align_trace", None, _p, _p, _sz, _sz, _p, _p, _sz, _p, C.POINTER(C.c_uint32), _p, _p, _sz,
      C.POINTER(C.c_size_t))
 _sig("ora_sincos", None, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
+_sig("ora_huber_weights", None, _p, _p, _p, _sz, C.c_float, _p)
+_sig("ora_depth_grid", C.c_int, _p, _sz, _p, _sz, _sz, C.POINTER(DepthParams), _p, _p, _p, C.POINTER(C.c_float))
 _sig("ora_common_area", None, _p, _sz, _sz, _sz, C.c_uint32, _p)
 _sig("ora_bright_grid", None, _p, _sz, _sz, _p)
 _sig("ora_bright_gains", None, _p, _p, _sz, _sz, _p, _p)
@@ -555,6 +557,29 @@ def upsample_bilinear(g, w, h, k):
     out = np.zeros((h, w), np.float32)
     _L.ora_upsample_bilinear(_ptr(g), dw, dh, w, h, k, _ptr(out))
     return out
+
+
+def huber_weights(conf, d, u, tau):
+    """depth.rs:1168: conf * min(tau / |d - u|, 1)."""
+    conf, d, u = _f32(conf), _f32(d), _f32(u)
+    out = np.zeros_like(conf)
+    _L.ora_huber_weights(_ptr(conf), _ptr(d), _ptr(u), conf.size, tau, _ptr(out))
+    return out
+
+
+def depth_grid(slices, fused3, params=None, **kw):
+    """depth.rs:1125-1159: (guide, peak depth after the median, p90-normalised conf, p90) on the working grid."""
+    slices = _f32(np.stack(slices))
+    fused3 = _f32(fused3)
+    n, dh, dw = slices.shape
+    _, h, w = fused3.shape
+    p = params or DepthParams.make(**kw)
+    g, d, c = (np.zeros((dh, dw), np.float32) for _ in range(3))
+    p90 = C.c_float()
+    if _L.ora_depth_grid(_ptr(slices), n, _ptr(fused3), w, h, C.byref(p), _ptr(g), _ptr(d), _ptr(c),
+                         C.byref(p90)) != 0:
+        raise ValueError("depth: no slices")
+    return g, d, c, p90.value
 
 
 def box_mean(src, r):
