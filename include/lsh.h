@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LSH_ABI_VERSION 1u
+#define LSH_ABI_VERSION 2u
 
 typedef enum lsh_mem { LSH_HOST = 0, LSH_DEVICE = 1 } lsh_mem;
 
@@ -58,7 +58,10 @@ typedef struct lsh_fuse_params {
 
 /* depth.rs:68-109 DepthParams.  focus_kind: 0 RDF{r_in, r_out}, 1 SML{sml_step};
  * upsample_kind: 0 bilinear, 1 guided{up_radius, up_eps}.  tol is the PCG's
- * relative-residual stop, fixed at 1e-5 in the reference (depth.rs:1054). */
+ * relative-residual stop, fixed at 1e-5 in the reference (depth.rs:1054).
+ * precision (no counterpart in the reference): the WLS solves' mode, 0 the
+ * reference's FP32 MG-PCG (LSH_WLS_REFERENCE), 1 that MG-PCG inside FP64
+ * iterative refinement (LSH_WLS_REFINED), whose FP64 true residual meets tol. */
 typedef struct lsh_depth_params {
     uint32_t scale;
     uint32_t focus_kind;
@@ -75,15 +78,19 @@ typedef struct lsh_depth_params {
     float gate;
     float robust;
     float tol;
+    uint32_t precision;
 } lsh_depth_params;
 
-/* depth.rs:140-156 DepthMap's scalars plus the WLS report (depth.rs:1160, 1179). */
+/* depth.rs:140-156 DepthMap's scalars plus the WLS report (depth.rs:1160,
+ * 1179): per solve the iterations, the solver's relative residual, and the
+ * true relative residual ||W d - A u|| / ||W d|| of its FP32 result in FP64. */
 typedef struct lsh_depth_info {
     uint32_t dw, dh;
     uint32_t iters1, iters2;
     float rel1, rel2;
     float p90;
     float floor;
+    double true_rel1, true_rel2;
 } lsh_depth_info;
 
 typedef struct lsh_frames lsh_frames;

@@ -122,12 +122,15 @@ class DepthParams:
     gate: float = 1.0
     robust: float = 1.0
     tol: float = 1e-5
+    precision: str = "reference"  # the WLS mode: "reference" (FP32 MG-PCG) or "refined" (inside FP64 refinement)
 
     def _c(self):
+        if self.precision not in ("reference", "refined"):
+            raise ValueError(f"depth: unknown WLS precision {self.precision!r}")
         return _lib.DepthParamsC(self.scale, 0 if self.focus == "rdf" else 1, self.r_in, self.r_out, self.sml_step,
                                  self.agg_radius, self.agg_eps, self.lam, self.sigma_c, self.cg_iters,
                                  int(self.median), 1 if self.upsample == "guided" else 0, self.up_radius,
-                                 self.up_eps, self.gate, self.robust, self.tol)
+                                 self.up_eps, self.gate, self.robust, self.tol, WLS_MODES[self.precision])
 
     def grid(self, w, h):
         k = 1 << self.scale

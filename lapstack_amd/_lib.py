@@ -8,7 +8,7 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LSH_LIB", os.path.join(HERE, "liblsh.so"))
 LSH_HOST, LSH_DEVICE = 0, 1
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _p = C.c_void_p
 _u32 = C.c_uint32
@@ -27,12 +27,14 @@ class DepthParamsC(C.Structure):
     _fields_ = [("scale", _u32), ("focus_kind", _u32), ("r_in", _u32), ("r_out", _u32), ("sml_step", _u32),
                 ("agg_radius", _u32), ("agg_eps", C.c_float), ("lambda_", C.c_float), ("sigma_c", C.c_float),
                 ("cg_iters", _u32), ("median", _u32), ("upsample_kind", _u32), ("up_radius", _u32),
-                ("up_eps", C.c_float), ("gate", C.c_float), ("robust", C.c_float), ("tol", C.c_float)]
+                ("up_eps", C.c_float), ("gate", C.c_float), ("robust", C.c_float), ("tol", C.c_float),
+                ("precision", _u32)]
 
 
 class DepthInfoC(C.Structure):
     _fields_ = [("dw", _u32), ("dh", _u32), ("iters1", _u32), ("iters2", _u32), ("rel1", C.c_float),
-                ("rel2", C.c_float), ("p90", C.c_float), ("floor", C.c_float)]
+                ("rel2", C.c_float), ("p90", C.c_float), ("floor", C.c_float), ("true_rel1", C.c_double),
+                ("true_rel2", C.c_double)]
 
 
 class WlsReportC(C.Structure):

@@ -836,21 +836,27 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
     // WLS (depth.rs:1161-1178)
     uint32_t it1 = 0, it2 = 0;
     float rel1 = 0.0f, rel2 = 0.0f;
+    double trel1 = 0.0, trel2 = 0.0;
     DBuf<float> u;
     const float *dres = depth_w.p;
     if (p->lambda > 0.0f) {
+        LSH_REQUIRE(p->precision <= 1, "depth: precision is 0 (reference) or 1 (refined)");
+        // cg_iters caps the CG iterations summed over the refined mode's inner solves
+        const int mode = p->precision == 1 ? WLS_REFINED : WLS_REFERENCE;
         WlsPtr s = wls_create(guide.p, dw, dh, p->lambda, p->sigma_c);
         wls_set_weights(s.get(), conf.p, nullptr, nullptr, 0.0f);
         u.alloc(M);
         u.zero();
-        WlsResult r1 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, WLS_REFERENCE, nullptr, u.p, nullptr);
+        WlsResult r1 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, nullptr, u.p, nullptr);
         it1 = it2 = r1.iters;
         rel1 = rel2 = (float)r1.rel;
+        trel1 = trel2 = wls_true_rel(s.get(), depth_w.p, u.p);
         if (p->robust > 0.0f) {
             wls_set_weights(s.get(), conf.p, depth_w.p, u.p, p->robust);
-            WlsResult r2 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, WLS_REFERENCE, nullptr, u.p, nullptr);
+            WlsResult r2 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, nullptr, u.p, nullptr);
             it2 = r2.iters;
             rel2 = (float)r2.rel;
+            trel2 = wls_true_rel(s.get(), depth_w.p, u.p);
         }
         dres = u.p;
     }
@@ -900,6 +906,8 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         info->rel2 = rel2;
         info->p90 = p90;
         info->floor = floor_;
+        info->true_rel1 = trel1;
+        info->true_rel2 = trel2;
     }
     LSH_API_END
 }

@@ -13,6 +13,7 @@ std::mutex g_mu;
 struct DevState {
     hipStream_t s = nullptr;
     hipStream_t side = nullptr;  // a second stream, for work that runs beside the main chain
+    hipStream_t copy = nullptr;  // a third, for host-to-device uploads beside both
     void *pinned = nullptr;
     size_t pinned_n = 0;
     Prof prof;
@@ -42,6 +43,12 @@ hipStream_t side_stream() {
     return st.side;
 }
 
+hipStream_t copy_stream() {
+    DevState &st = dev_state();
+    if (!st.copy) LSH_CHECK(hipStreamCreateWithFlags(&st.copy, hipStreamNonBlocking));
+    return st.copy;
+}
+
 OnStream::OnStream(hipStream_t s, unsigned cus) : prev(t_override), prev_cus(t_cus) {
     t_override = s;
     t_cus = cus;
@@ -56,6 +63,7 @@ void sync() {
     if (t_override && t_override != st.s && t_override != st.side) LSH_CHECK(hipStreamSynchronize(t_override));
     LSH_CHECK(hipStreamSynchronize(st.s));
     if (st.side) LSH_CHECK(hipStreamSynchronize(st.side));
+    if (st.copy) LSH_CHECK(hipStreamSynchronize(st.copy));
 }
 
 void *pinned_scratch(size_t n) {

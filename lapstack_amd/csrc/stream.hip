@@ -128,26 +128,51 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
         float *planes[2] = {f->planes.p, f->planes_b.p};
         const size_t fbytes = 3 * npx * sizeof(uint16_t);
         // raw frames: the caller's device pointers (read through a device table
-        // of them inside a replayed graph), or host frames staged into two
-        // fixed device buffers
-        // host frames: a ring of three staging buffers (frame k in stage[k % 3]:
-        // its side work, its target and the next frame's are in flight at once)
+        // of them inside a replayed graph), or host frames uploaded into a ring
+        // of NRING staging buffers (frame k in stage[k % NRING]) on the copy
+        // stream, ahead of their first reader: frame k+1's upload waits only for
+        // the side work of frame k+1-NRING (the last reader of its slot), so
+        // with four slots it runs beside frame k-1's cycle.  Host frames in
+        // pinned memory upload at the link's rate without a host copy.
         const bool staged = frames_mem != LSH_DEVICE;
-        DBuf<uint16_t> stage[3];
+        constexpr int NRING = 4;
+        DBuf<uint16_t> stage[NRING];
+        hipEvent_t e_up[NRING] = {}, e_free[NRING] = {};
+        bool slot_used[NRING] = {};
         if (staged)
-            for (auto &b : stage) b.alloc(3 * npx);
+            for (int i = 0; i < NRING; i++) {
+                stage[i].alloc(3 * npx);
+                e_up[i] = E.make();
+                e_free[i] = E.make();
+            }
+        const hipStream_t Cs = staged ? copy_stream() : nullptr;
         DBuf<const uint16_t *> ftab;
         DBuf<unsigned> fcur(1);  // a replay's frame index (the graph's chain frame)
         if (graph) {
-            const uint16_t *ring[3] = {stage[0].p, stage[1].p, stage[2].p};
-            ftab.alloc(staged ? 3 : n);
+            const uint16_t *ring[NRING];
+            for (int i = 0; i < NRING; i++) ring[i] = stage[i].p;
+            ftab.alloc(staged ? NRING : n);
             copy_mem(ftab.p, LSH_DEVICE, staged ? ring : frames, LSH_HOST, ftab.n * sizeof(const uint16_t *));
         }
-        auto raw = [&](uint32_t k) -> const uint16_t * { return staged ? stage[k % 3].p : frames[k]; };
+        auto raw = [&](uint32_t k) -> const uint16_t * { return staged ? stage[k % NRING].p : frames[k]; };
         // frame k as a replayed cycle reads it: off = k - the cycle's frame index
-        auto cyc = [&](int off) -> FrameRef { return FrameRef(ftab.p, fcur.p, off, staged ? 3 : 0); };
+        auto cyc = [&](int off) -> FrameRef { return FrameRef(ftab.p, fcur.p, off, staged ? NRING : 0); };
+        // upload frame k into its slot once the slot's previous frame is done with
         auto put = [&](uint32_t k) {
-            if (staged) copy_mem(stage[k % 3].p, LSH_DEVICE, frames[k], frames_mem, fbytes);
+            if (!staged) return;
+            const int sl = (int)(k % NRING);
+            if (slot_used[sl]) LSH_CHECK(hipStreamWaitEvent(Cs, e_free[sl], 0));
+            slot_used[sl] = true;
+            LSH_CHECK(hipMemcpyAsync(stage[sl].p, frames[k], fbytes, hipMemcpyHostToDevice, Cs));
+            LSH_CHECK(hipEventRecord(e_up[sl], Cs));
+        };
+        // stream st reads frame k only after its upload; frame k's slot is free
+        // once what st has queued so far has run
+        auto ready = [&](uint32_t k, hipStream_t st) {
+            if (staged) LSH_CHECK(hipStreamWaitEvent(st, e_up[k % NRING], 0));
+        };
+        auto release = [&](uint32_t k, hipStream_t st) {
+            if (staged) LSH_CHECK(hipEventRecord(e_free[k % NRING], st));
         };
         // slices: in place in the caller's device buffer (launches: at frame k's
         // offset; a graph: at the fold index's), else through device buffers
@@ -183,8 +208,9 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             fuser_push_dev(fu, planes[k & 1], fidx.p);
             launch("stream_fold_next", k_counter_next, dim3(1), dim3(64), 0, fidx.p);
         };
-        auto target = [&](uint32_t k) {  // frame k's target pyramid into slot k % 2
+        auto target = [&](uint32_t k) {  // frame k's target pyramid into slot k % 2 (on the current stream)
             put(k);
+            ready(k, stream());
             frames_target(f, raw(k), (int)(k & 1));
         };
         // the side stream starts after frame 0 (the fuser's state, the frames handle's)
@@ -213,6 +239,7 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                 OnStream on(Ss);
                 LSH_CHECK(hipStreamWaitEvent(Ss, e_started, 0));
                 side(k - 1, raw(k - 1));
+                release(k - 1, Ss);
                 LSH_CHECK(hipEventRecord(e_side[(k - 1) & 1], Ss));
                 if (k + 1 < n) {
                     target(k + 1);
@@ -247,7 +274,9 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             LSH_CHECK(hipStreamWaitEvent(Ms, e_target[0], 0));  // the prologue's side work is done
             for (uint32_t k = 2; k < n; k++) {
                 if (k + 1 < n) {
-                    put(k + 1);  // host frames: before the replay (its ring slot was last read by frame k-2)
+                    // host frames: frame k+1 uploaded for the replay's target
+                    put(k + 1);
+                    ready(k + 1, Ms);
                     const int p = (int)(k & 1);
                     if (!G.x[p]) {
                         prof().capturing = true;
@@ -269,6 +298,7 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                         fu->count++;
                     }
                     LSH_CHECK(hipGraphLaunch(G.x[p], Ms));
+                    release(k - 1, Ms);  // the replay's side work was frame k-1's last reader
                 } else {
                     cycle(k);  // the last frame: no target after it
                 }

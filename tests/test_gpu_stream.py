@@ -83,7 +83,7 @@ def test_stream_device_buffers_and_single_frame():
     assert float(img.max()) > 0.0 and s["evals"][1] > 0
 
 
-@pytest.mark.parametrize("n", [3, 8])
+@pytest.mark.parametrize("n", [3, 8, 11])
 def test_graph_reads_frames_through_the_table_and_writes_device_slices(n):
     """A replayed graph takes device frames through its device table of frame
     pointers and writes device slices at the fold's index; host frames go
@@ -99,9 +99,15 @@ def test_graph_reads_frames_through_the_table_and_writes_device_slices(n):
     ref = la.stack_stream(frames, fp, ap, dp, graph=False)
     sd = torch.full((n, dh, dw), -1.0, dtype=torch.float32, device="cuda")
     g = la.stack_stream(frames, fp, ap, dp, graph=True, slices_out=sd)
+    # host frames: pageable numpy arrays and pinned tensors, uploaded through the ring on the copy stream
     hf = la.stack_stream([f.cpu().numpy() for f in frames], fp, ap, dp, graph=True)
+    pinned = [f.cpu().pin_memory() for f in frames]
+    hp = la.stack_stream(pinned, fp, ap, dp, graph=True)
+    hl = la.stack_stream(pinned, fp, ap, dp, graph=False)
     for k in ("image", "winner", "sims", "gains", "evals"):
         assert np.array_equal(np.asarray(g[k]), np.asarray(ref[k])), k
-        assert np.array_equal(np.asarray(hf[k]), np.asarray(ref[k])), k
+        for other in (hf, hp, hl):
+            assert np.array_equal(np.asarray(other[k]), np.asarray(ref[k])), k
     assert np.array_equal(sd.cpu().numpy(), np.asarray(ref["slices"]))
-    assert np.array_equal(np.asarray(hf["slices"]), np.asarray(ref["slices"]))
+    for other in (hf, hp, hl):
+        assert np.array_equal(np.asarray(other["slices"]), np.asarray(ref["slices"]))
