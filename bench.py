@@ -52,6 +52,9 @@ def parse():
                     help="skip the upload-inclusive run (frames handed over in pinned host memory)")
     ap.add_argument("--prof-frames", type=int, default=0,
                     help="frames of the per-kernel timing pass (0: the whole stack, so the totals are the job's)")
+    ap.add_argument("--host-only", action="store_true",
+                    help="only the host-steered path (one stream, cooperative search launches) over --frames frames, "
+                         "twice, then the per-kernel timings: the run the HBM traffic counters are read on")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (twice) and exit: the calibration run of the "
                          "HBM traffic counters")
@@ -361,6 +364,19 @@ def main():
     sync()
     log(f"stack resident in HBM: {n} x {w}x{h} RGB u16 ({n * w * h * 6 / 1e9:.1f} GB) in {time.perf_counter() - t0:.1f}s")
     job = Job(frames, w, h, dp)
+    if args.host_only:
+        job.run(fp, apm, dp, path="host")
+        sync()
+        _lib.prof_reset()
+        _lib.prof_enable(True)
+        r = job.run(fp, apm, dp, path="host")
+        sync()
+        _lib.prof_enable(False)
+        prof = _lib.prof_read()
+        print(json.dumps({"host_only": True, "frames": n, "evals": r["evals"],
+                          "kernels": {k: {"ms_total": round(v[0], 3), "bytes_total": v[1], "launches": v[2]}
+                                      for k, v in prof.items()}}), flush=True)
+        return
     for s in range(args.warmup):
         r = job.run(fp, apm, dp)
         sync()
