@@ -367,6 +367,22 @@ char *lsh_wls_solve(const float *guide, const float *conf, const float *d, const
                     uint32_t max_iters, uint32_t want_err, float *u, double *u64, lsh_mem out_mem,
                     lsh_wls_report *rep);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lsh_wls_solve in LSH_WLS_REFERENCE mode with every buffer
+ * of the solver (the multigrid hierarchy, the CG vectors, the reduction
+ * partials) in one device array the library holds per GPU, of
+ * LSH_WLS_NOALLOC_BYTES (20 MiB: a 512 x 384 grid and larger ones up to about
+ * 230 000 cells; a grid that needs more is refused with an error string).  It
+ * calls no hipMalloc, hipFree or other device or pinned allocation.  guide,
+ * conf, d, warm (nullable: the FGS guess) and u are device memory.  Every call
+ * on one GPU shares that GPU's array and overwrites it: calls that overlap on
+ * one GPU (other threads, other streams) race. */
+#define LSH_WLS_NOALLOC_BYTES (20u << 20)
+char *lsh_wls_solve_noalloc(const float *guide, const float *conf, const float *d, const float *warm, uint32_t w,
+                            uint32_t h, float lambda, float sigma_c, double tol, uint32_t max_iters, float *u,
+                            lsh_wls_report *rep);
+
 /* ---- the learned warm start of the WLS solve ---------------------------- */
 
 /* No counterpart in the reference, whose PCG starts from its FGS sweeps

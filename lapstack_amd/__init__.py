@@ -40,7 +40,7 @@ from ._lib import LSH_DEVICE, LSH_HOST, HipError
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
            "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
            "weighted_average", "slab_ranges", "fuse_slabs",
-           "common_area", "crop", "resize", "restretch", "to_u16", "gauss_levels", "wls_solve", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "common_area", "crop", "resize", "restretch", "to_u16", "gauss_levels", "wls_solve", "wls_solve_noalloc", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -751,6 +751,32 @@ def wls_solve(guide, conf, d, lam: float, sigma_c: float, mode: str = "reference
                                  int(want_err), up, u64p, umem, C.byref(rep)))
     del gk, ck, dk, wk, uk
     return u_out, {k: getattr(rep, k) for k, _ in rep._fields_}, u64_out
+
+
+def wls_solve_noalloc(guide, conf, d, lam: float, sigma_c: float, tol: float = 1e-5, max_iters: int = 200, warm=None,
+                      u_out=None):
+    """wls_solve in the "reference" mode with no device allocation
+    (include/lsh.h lsh_wls_solve_noalloc): every buffer of the solver in the
+    library's per-GPU scratch array (grids of up to about 230 000 cells).
+    Planes are (h, w) float32 device tensors; returns (u, report)."""
+    import torch
+
+    lib = _lib.load()
+    h, w = tuple(guide.shape)
+    planes = [("guide", guide), ("conf", conf), ("d", d)] + ([("warm", warm)] if warm is not None else [])
+    for name, a in planes:
+        if not (isinstance(a, torch.Tensor) and a.is_cuda and a.dtype == torch.float32 and a.is_contiguous()):
+            raise HipError(f"wls no-alloc: {name} must be a contiguous float32 device tensor")
+        if tuple(a.shape) != (h, w):
+            raise HipError(f"wls no-alloc: {name} has shape {tuple(a.shape)}, the grid is {h}x{w}")
+    if u_out is None:
+        u_out = torch.empty((h, w), dtype=torch.float32, device=guide.device)
+    rep = _lib.WlsReportC()
+    _lib.check(lib.lsh_wls_solve_noalloc(C.c_void_p(guide.data_ptr()), C.c_void_p(conf.data_ptr()),
+                                         C.c_void_p(d.data_ptr()),
+                                         C.c_void_p(warm.data_ptr()) if warm is not None else None, w, h, lam,
+                                         sigma_c, tol, max_iters, C.c_void_p(u_out.data_ptr()), C.byref(rep)))
+    return u_out, {k: getattr(rep, k) for k, _ in rep._fields_}
 
 
 def stack_stream(frames: Sequence, fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,

@@ -120,6 +120,8 @@ SIGNATURES = [
     ("lsh_to_u16", _p, [_p, _u32, _u32, _u32, _p, _u32]),
     ("lsh_wls_solve", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, C.c_float, C.c_float, _u32, C.c_double, _u32, _u32,
                            _p, _p, C.c_int, C.POINTER(WlsReportC)]),
+    ("lsh_wls_solve_noalloc", _p, [_p, _p, _p, _p, _u32, _u32, C.c_float, C.c_float, C.c_double, _u32, _p,
+                                   C.POINTER(WlsReportC)]),
 ]
 
 _lock = threading.Lock()

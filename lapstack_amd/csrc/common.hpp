@@ -131,25 +131,31 @@ inline void prof_set_last_bytes(double bytes) {
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // --------------------------------------------------------------- buffers
+// Device memory the buffer owns (alloc), or a view of memory it does not own
+// (view: released without a free).
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool owned = false;
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), owned(o.owned) {
         o.p = nullptr;
         o.n = 0;
+        o.owned = false;
     }
     DBuf &operator=(DBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            owned = o.owned;
             o.p = nullptr;
             o.n = 0;
+            o.owned = false;
         }
         return *this;
     }
@@ -158,14 +164,22 @@ struct DBuf {
         release();
         if (count) LSH_CHECK(hipMalloc((void **)&p, count * sizeof(T)));
         n = count;
+        owned = true;
+    }
+    void view(T *q, size_t count) {
+        release();
+        p = q;
+        n = count;
+        owned = false;
     }
     void ensure(size_t count) {
         if (count > n) alloc(count);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = false;
     }
     void zero() {
         if (n) LSH_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), stream()));

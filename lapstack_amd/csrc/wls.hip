@@ -8,6 +8,10 @@
 // aggregation hierarchy.  Every kernel is a streaming pass over cell planes;
 // dot products sum in f64 through per-block partials folded in a fixed order.
 #include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "api.hpp"
 #include "common.hpp"
@@ -517,15 +521,41 @@ __global__ __launch_bounds__(64) void k_chol_solve(const double *__restrict__ Lb
     for (int i = tid; i < n; i += blockDim.x) x[i] = (T)y[i];
 }
 
+// Where a solver's buffers come from: device allocations (base null, counting
+// off), or consecutive 256-byte-aligned pieces of a caller's block -- counted
+// only, with nothing placed, when base is null and counting is on.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    bool counting = false;
+    template <typename T>
+    void get(DBuf<T> &b, size_t n) {
+        if (!base && !counting) {
+            b.alloc(n);
+            return;
+        }
+        used = (used + 255) & ~(size_t)255;
+        const size_t at = used;
+        used += n * sizeof(T);
+        if (counting) return;
+        LSH_REQUIRE(used <= cap, "wls: the solver's buffers exceed the scratch block");
+        b.view((T *)(base + at), n);
+    }
+};
+
 template <typename T>
 struct Lev {
     int w = 0, h = 0;
     DBuf<T> wd, ax, ay, dinv, x, b, r;
-    void alloc(int w_, int h_) {
+    void alloc(int w_, int h_, Arena &a) {
         w = w_;
         h = h_;
         size_t n = (size_t)w * h;
-        for (auto *bf : {&wd, &ax, &ay, &dinv, &x, &b, &r}) bf->alloc(n);
+        for (auto *bf : {&wd, &ax, &ay, &dinv, &x, &b, &r}) a.get(*bf, n);
+    }
+    void alloc(int w_, int h_) {
+        Arena a;
+        alloc(w_, h_, a);
     }
 };
 
@@ -541,42 +571,92 @@ struct WlsSolver {
     DBuf<double> b64, r64, z64, p64, ap64, x64;
     DBuf<double> part, out;
     DBuf<unsigned> ticket;
+    bool noalloc = false;  // buffers in a caller's block: scalars come back through pageable host memory
 };
 
 void WlsDeleter::operator()(WlsSolver *s) const { delete s; }
 
 static double read_out(WlsSolver &s) {
+    if (s.noalloc) {
+        double v = 0.0;
+        LSH_CHECK(hipMemcpyAsync(&v, s.out.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        LSH_CHECK(hipStreamSynchronize(stream()));
+        return v;
+    }
     double *pin = (double *)pinned_scratch(64);
     LSH_CHECK(hipMemcpyAsync(pin, s.out.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
     sync();
     return pin[0];
 }
 
-WlsPtr wls_create(const float *guide, int w, int h, float lambda, float sigma_c) {
-    WlsPtr sp(new WlsSolver());
-    WlsSolver &s = *sp;
+__global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// The solver's buffers (depth.rs:976-988's vectors and the multigrid hierarchy
+// down to MG_MIN), from the arena; with64 adds the FP64 right-hand side and
+// residual the true residual needs.
+static void wls_layout(WlsSolver &s, int w, int h, Arena &a, bool with64) {
     s.w = w;
     s.h = h;
-    s.lambda = lambda;
     size_t n = (size_t)w * h;
-    for (auto *b : {&s.ex, &s.ey, &s.cp, &s.dp, &s.f, &s.r, &s.z, &s.p, &s.ap}) b->alloc(n);
-    s.part.alloc(1024);
-    s.out.alloc(1);
-    s.ticket.alloc(1);
-    s.ticket.zero();
+    for (auto *b : {&s.ex, &s.ey, &s.cp, &s.dp, &s.f, &s.r, &s.z, &s.p, &s.ap}) a.get(*b, n);
+    if (with64)
+        for (auto *b : {&s.b64, &s.r64}) a.get(*b, n);
+    a.get(s.part, 1024);
+    a.get(s.out, 1);
+    a.get(s.ticket, 1);
+    s.L.clear();
     s.L.emplace_back();
-    s.L[0].alloc(w, h);
-    float inv = -1.0f / fmaxf(sigma_c, 1e-6f);
-    launch_b("wls_edges", 20.0 * n, k_edges, g2(w, h), B2, 0, guide, w, h, inv, lambda, s.ex.p, s.ey.p, s.L[0].ax.p,
-             s.L[0].ay.p);
+    s.L[0].alloc(w, h, a);
     while (std::max(s.L.back().w, s.L.back().h) > MG_MIN) {
         int cw = (s.L.back().w + 1) / 2, chh = (s.L.back().h + 1) / 2;
         s.L.emplace_back();
-        s.L.back().alloc(cw, chh);
-        Lev<float> &fl = s.L[s.L.size() - 2], &c = s.L.back();
+        s.L.back().alloc(cw, chh, a);
+    }
+}
+
+// depth.rs:976-988: the guide's edges and the hierarchy's coarse operators
+static void wls_init(WlsSolver &s, const float *guide, float lambda, float sigma_c) {
+    const int w = s.w, h = s.h;
+    const size_t n = (size_t)w * h;
+    s.lambda = lambda;
+    launch("wls_ticket", k_fill_u32, dim3(1), dim3(64), 0, s.ticket.p, (size_t)1, 0u);
+    float inv = -1.0f / fmaxf(sigma_c, 1e-6f);
+    launch_b("wls_edges", 20.0 * n, k_edges, g2(w, h), B2, 0, guide, w, h, inv, lambda, s.ex.p, s.ey.p, s.L[0].ax.p,
+             s.L[0].ay.p);
+    for (size_t l = 1; l < s.L.size(); l++) {
+        Lev<float> &fl = s.L[l - 1], &c = s.L[l];
         launch("wls_coarsen", k_coarsen, g2(c.w, c.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
                (const float *)fl.ay.p, fl.w, fl.h, c.wd.p, c.ax.p, c.ay.p, c.w, c.h, 2);
     }
+}
+
+WlsPtr wls_create(const float *guide, int w, int h, float lambda, float sigma_c) {
+    WlsPtr sp(new WlsSolver());
+    Arena a;
+    wls_layout(*sp, w, h, a, false);
+    wls_init(*sp, guide, lambda, sigma_c);
+    return sp;
+}
+
+size_t wls_scratch_bytes(int w, int h) {
+    WlsSolver s;
+    Arena a;
+    a.counting = true;
+    wls_layout(s, w, h, a, true);
+    return a.used;
+}
+
+WlsPtr wls_create_in(void *block, size_t cap, const float *guide, int w, int h, float lambda, float sigma_c) {
+    WlsPtr sp(new WlsSolver());
+    Arena a;
+    a.base = (char *)block;
+    a.cap = cap;
+    sp->noalloc = true;
+    wls_layout(*sp, w, h, a, true);
+    wls_init(*sp, guide, lambda, sigma_c);
     return sp;
 }
 
@@ -935,6 +1015,70 @@ char *lsh_wls_solve(const float *guide, const float *conf, const float *d, const
         rep->rel = r.rel;
         rep->true_rel = true_rel;
         rep->err_fp64 = err;
+        rep->ms = ms;
+    }
+    LSH_API_END
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ the WLS solve with no allocation
+// A variant of lsh_wls_solve for callers that cannot allocate (code under HIP
+// graph capture, serving loops, harnesses that forbid hipMalloc): every buffer
+// of the solver lives in `scratch`, one file-scope device array per GPU, of
+// LSH_WLS_NOALLOC_BYTES; grids needing more are refused.  In host code the
+// array's name is no device address: the address each device holds it at is
+// taken once per device (hipGetSymbolAddress, with that device current) and
+// only that pointer is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[LSH_WLS_NOALLOC_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;
+
+void *scratch_block() {
+    int dev = 0;
+    LSH_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void *ptr = nullptr;
+    LSH_CHECK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    LSH_REQUIRE(ptr, "wls no-alloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+    return ptr;
+}
+}  // namespace
+
+extern "C" {
+
+char *lsh_wls_solve_noalloc(const float *guide, const float *conf, const float *d, const float *warm, uint32_t w,
+                            uint32_t h, float lambda, float sigma_c, double tol, uint32_t max_iters, float *u,
+                            lsh_wls_report *rep) {
+    using namespace lsh;
+    LSH_API_BEGIN
+    LSH_REQUIRE(guide && conf && d && u, "lsh_wls_solve_noalloc: null argument");
+    LSH_REQUIRE(w > 0 && h > 0, "wls: empty grid");
+    LSH_REQUIRE(lambda > 0.0f, "wls: lambda must be positive");
+    const size_t need = wls_scratch_bytes((int)w, (int)h);
+    if (need > (size_t)LSH_WLS_NOALLOC_BYTES)
+        throw Error("wls no-alloc: a " + std::to_string(w) + " x " + std::to_string(h) + " grid needs " +
+                    std::to_string(need) + " bytes of scratch, above the " + std::to_string(LSH_WLS_NOALLOC_BYTES) +
+                    " the library holds");
+    void *blk = scratch_block();
+    const auto t0 = std::chrono::steady_clock::now();
+    WlsPtr s = wls_create_in(blk, LSH_WLS_NOALLOC_BYTES, guide, (int)w, (int)h, lambda, sigma_c);
+    wls_set_weights(s.get(), conf, nullptr, nullptr, 0.0f);
+    WlsResult r = wls_solve(s.get(), d, max_iters, tol, WLS_REFERENCE, warm, u, nullptr);
+    const double true_rel = (w > 1 && h > 1) ? wls_true_rel(s.get(), d, u) : 0.0;
+    LSH_CHECK(hipStreamSynchronize(stream()));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rep) {
+        rep->iters = r.iters;
+        rep->outer = r.outer;
+        rep->rel = r.rel;
+        rep->true_rel = true_rel;
+        rep->err_fp64 = -1.0;
         rep->ms = ms;
     }
     LSH_API_END

@@ -30,6 +30,10 @@ using WlsPtr = std::unique_ptr<WlsSolver, WlsDeleter>;
 
 // depth.rs:976-988: edges of the guide, the multigrid hierarchy
 WlsPtr wls_create(const float *guide, int w, int h, float lambda, float sigma_c);
+// The same solver with every buffer in a caller's device block of cap bytes
+// (no allocation); wls_scratch_bytes is what a w x h grid needs there.
+size_t wls_scratch_bytes(int w, int h);
+WlsPtr wls_create_in(void *block, size_t cap, const float *guide, int w, int h, float lambda, float sigma_c);
 // depth.rs:992-1002: data weights max(c, 0) + 1e-4 from conf, or, with u set, the
 // Huber reweighting conf * min(tau / |d - u|, 1) first (depth.rs:1168)
 void wls_set_weights(WlsSolver *s, const float *conf, const float *d, const float *u, float tau);
