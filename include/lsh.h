@@ -213,6 +213,40 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
                        float *slices, lsh_mem slices_mem, uint32_t graph, double *sims_out, float *gains_out,
                        uint64_t *evals_out);
 
+/* stack.rs:277-360 with io.rs:62-97: lsh_stack_stream over n image files
+ * (paths[i], 16- or 8-bit PNG or TIFF, each the handles' size), decoded by
+ * `threads` host threads (0: 8) ahead of their use into a ring of pinned host
+ * buffers and uploaded from there on the library's copy stream, beside the
+ * device's work on earlier frames.  Same outputs as lsh_stack_stream; a file
+ * that cannot be decoded, or of another size, ends the call with its error. */
+char *lsh_stack_stream_files(lsh_frames *f, lsh_fuser *fu, const char *const *paths, uint32_t n, uint32_t threads,
+                             const uint8_t *free8, uint32_t coarsen, uint32_t brightness,
+                             const lsh_depth_params *measure, float *slices, lsh_mem slices_mem, uint32_t graph,
+                             double *sims_out, float *gains_out, uint64_t *evals_out);
+
+/* ---- image files (io.rs) -------------------------------------------------- */
+
+/* io.rs:62-97 load_rgb for PNG (colour types 0/2/4/6, 8 or 16 bits, not
+ * interlaced) and baseline TIFF (strips, chunky or planar, uncompressed, LZW or
+ * Deflate, predictor 1 or 2): the frame as interleaved RGB u16 (h*w*3, into rgb
+ * with room for cap samples; NULL: size only), 8-bit samples v given as v*257
+ * (the same fraction); *bits the file's bits per sample.  lsh_image_info: the
+ * size and bits only. */
+char *lsh_image_info(const char *path, uint32_t *w, uint32_t *h, uint32_t *bits);
+char *lsh_image_load(const char *path, uint16_t *rgb, size_t cap, uint32_t *w, uint32_t *h, uint32_t *bits);
+/* io.rs:273-315 save_rgb: three planes (3*w*h f32 in mem) as a 16- or 8-bit
+ * TIFF (uncompressed, strips of about 8 MB) or PNG, by the path's extension;
+ * samples (clamp(v, 0, 1) * 65535 + 0.5) or (... * 255 + 0.5), truncated.
+ * lsh_image_save_u16: interleaved RGB u16 samples (lsh_to_u16's) as they are. */
+char *lsh_image_save(const char *path, const float *img3, lsh_mem mem, uint32_t w, uint32_t h, uint32_t bits);
+char *lsh_image_save_u16(const char *path, const uint16_t *rgb, lsh_mem mem, uint32_t w, uint32_t h);
+/* io.rs:325-335 save_gray16: a plane in [0, scale] as a 16-bit gray PNG,
+ * (clamp(v, 0, scale) * 65535 / scale + 0.5) truncated (depth maps: scale =
+ * frames - 1; confidence: 1).  io.rs:337-346 save_gray: min-max normalised
+ * 8-bit gray PNG. */
+char *lsh_save_gray16(const char *path, const float *plane, lsh_mem mem, uint32_t w, uint32_t h, float scale);
+char *lsh_save_gray(const char *path, const float *plane, lsh_mem mem, uint32_t w, uint32_t h);
+
 /* ---- many stacks in one launch -------------------------------------------- */
 
 /* One stack of a batch: n frames of w*h (interleaved RGB u16, device memory). */

@@ -40,7 +40,7 @@ from ._lib import LSH_DEVICE, LSH_HOST, HipError
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
            "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
            "weighted_average", "slab_ranges", "fuse_slabs",
-           "common_area", "crop", "resize", "restretch", "to_u16", "gauss_levels", "wls_solve", "wls_solve_noalloc", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
+           "common_area", "crop", "resize", "restretch", "to_u16", "gauss_levels", "wls_solve", "wls_solve_noalloc", "image_info", "load_rgb", "save_rgb", "save_rgb_u16", "save_gray16", "save_gray", "stack_stream_files", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
 SIM_ID = (0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
 
@@ -820,6 +820,106 @@ def stack_stream(frames: Sequence, fuse: Optional[FuseParams] = None, align: Opt
     fr.close()
     fu.close()
     del bufs, sk
+    return dict(image=image, winner=winner, sims=sims, gains=gains, evals=evals, slices=slices_out)
+
+
+def image_info(path: str):
+    """(w, h, bits) of a PNG or TIFF file."""
+    lib = _lib.load()
+    w, h, b = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _lib.check(lib.lsh_image_info(path.encode(), C.byref(w), C.byref(h), C.byref(b)))
+    return w.value, h.value, b.value
+
+
+def load_rgb(path: str, out=None):
+    """io.rs:62-97 for PNG and TIFF: the frame as interleaved RGB u16 (h, w, 3)
+    (8-bit samples v as v*257) and the file's bits per sample.  ``out``: a
+    numpy uint16 array of that shape (or a pinned host tensor's numpy view)."""
+    lib = _lib.load()
+    w, h, b = image_info(path)
+    if out is None:
+        out = np.empty((h, w, 3), np.uint16)
+    if out.dtype != np.uint16 or out.size < w * h * 3 or not out.flags["C_CONTIGUOUS"]:
+        raise HipError(f"load_rgb: the output must be contiguous uint16 with room for {h}x{w}x3")
+    ww, hh, bb = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _lib.check(lib.lsh_image_load(path.encode(), out.ctypes.data_as(C.c_void_p), out.size, C.byref(ww), C.byref(hh),
+                                  C.byref(bb)))
+    return out.reshape(-1)[:w * h * 3].reshape(h, w, 3), bb.value
+
+
+def save_rgb(img3, path: str, bits: int = 16):
+    """io.rs:273-315 save_rgb: (3, h, w) planes, host or device, as a 16- or 8-bit TIFF or PNG."""
+    lib = _lib.load()
+    _, h, w = tuple(img3.shape)
+    p, mem, keep = _buf(img3, np.float32)
+    _lib.check(lib.lsh_image_save(path.encode(), p, mem, w, h, bits))
+    del keep
+
+
+def save_rgb_u16(rgb, path: str):
+    """Interleaved RGB u16 (h, w, 3), host or device, as a 16-bit TIFF or PNG."""
+    lib = _lib.load()
+    h, w, _ = tuple(rgb.shape)
+    p, mem, keep = _buf(rgb, np.uint16)
+    _lib.check(lib.lsh_image_save_u16(path.encode(), p, mem, w, h))
+    del keep
+
+
+def save_gray16(plane, path: str, scale: float):
+    """io.rs:325-335: a plane in [0, scale] as a 16-bit gray PNG with the fixed mapping scale -> 65535."""
+    lib = _lib.load()
+    h, w = tuple(plane.shape)
+    p, mem, keep = _buf(plane, np.float32)
+    _lib.check(lib.lsh_save_gray16(path.encode(), p, mem, w, h, scale))
+    del keep
+
+
+def save_gray(plane, path: str):
+    """io.rs:337-346: a plane min-max normalised to an 8-bit gray PNG."""
+    lib = _lib.load()
+    h, w = tuple(plane.shape)
+    p, mem, keep = _buf(plane, np.float32)
+    _lib.check(lib.lsh_save_gray(path.encode(), p, mem, w, h))
+    del keep
+
+
+def stack_stream_files(paths: Sequence[str], fuse: Optional[FuseParams] = None, align: Optional[AlignParams] = None,
+                       depth: Optional[DepthParams] = None, brightness: bool = True, graph: bool = True,
+                       threads: int = 8, slices_out=None, img_out=None, winner_out=None):
+    """stack_stream over image files: decoded by ``threads`` host threads ahead
+    of their use into pinned buffers and uploaded beside the device's work
+    (include/lsh.h lsh_stack_stream_files).  Returns what stack_stream returns."""
+    lib = _lib.load()
+    fuse = fuse or FuseParams()
+    align = align or AlignParams()
+    n = len(paths)
+    if n == 0:
+        raise HipError("no input images")
+    w, h, _ = image_info(paths[0])
+    fr = HipFrames(w, h, align.interp)
+    fu = Fuser(w, h, fuse)
+    cpaths = (C.c_char_p * n)(*[p.encode() for p in paths])
+    free = (C.c_uint8 * 8)(*align.free())
+    dp = depth._c() if depth is not None else None
+    sp, smem, sk = None, LSH_HOST, None
+    if depth is not None:
+        dw, dh = depth.grid(w, h)
+        if slices_out is None:
+            slices_out = np.zeros((n, dh, dw), np.float32)
+        sp, smem, sk = _buf(slices_out, np.float32)
+    sims = np.zeros((n, 8), np.float64)
+    gains = np.zeros((n, 3), np.float32)
+    evals = np.zeros(n, np.uint64)
+    try:
+        _lib.check(lib.lsh_stack_stream_files(fr._h, fu._h, cpaths, n, threads, free, align.coarsen, int(brightness),
+                                              C.byref(dp) if dp is not None else None, sp, smem, int(graph),
+                                              sims.ctypes.data_as(C.c_void_p), gains.ctypes.data_as(C.c_void_p),
+                                              evals.ctypes.data_as(C.c_void_p)))
+        image, winner = fu.finish(img_out, winner_out)
+    finally:
+        fr.close()
+        fu.close()
+    del sk
     return dict(image=image, winner=winner, sims=sims, gains=gains, evals=evals, slices=slices_out)
 
 

@@ -106,6 +106,14 @@ SIGNATURES = [
                                  C.POINTER(C.c_double)]),
     ("lsh_stack_stream", _p, [_p, _p, _p, _u32, C.c_int, _p, _u32, _u32, C.POINTER(DepthParamsC), _p, C.c_int, _u32,
                               _p, _p, _p]),
+    ("lsh_stack_stream_files", _p, [_p, _p, _p, _u32, _u32, _p, _u32, _u32, C.POINTER(DepthParamsC), _p, C.c_int,
+                                    _u32, _p, _p, _p]),
+    ("lsh_image_info", _p, [C.c_char_p, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    ("lsh_image_load", _p, [C.c_char_p, _p, C.c_size_t, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    ("lsh_image_save", _p, [C.c_char_p, _p, C.c_int, _u32, _u32, _u32]),
+    ("lsh_image_save_u16", _p, [C.c_char_p, _p, C.c_int, _u32, _u32]),
+    ("lsh_save_gray16", _p, [C.c_char_p, _p, C.c_int, _u32, _u32, C.c_float]),
+    ("lsh_save_gray", _p, [C.c_char_p, _p, C.c_int, _u32, _u32]),
     ("lsh_batch_fuse", _p, [_p, _p, _u32, C.POINTER(BatchOptionsC), C.POINTER(BatchReportC)]),
     ("lsh_wav_new", _p, [_u32, _u32, _p, C.POINTER(WavParamsC), _p, _u32, _p, _u32, C.POINTER(_p)]),
     ("lsh_wav_push", _p, [_p, _p, _u32, _p, _u32]),

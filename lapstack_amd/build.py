@@ -1,6 +1,6 @@
 """Builds lapstack_amd/liblsh.so, the gfx950 backend, with hipcc.
 
-Every source is compiled for gfx950 only, with -ffp-contract=off (the CPU path
+Every source (the kernels' .hip files and the host .cpp files) is compiled with hipcc for gfx950 only, with -ffp-contract=off (the CPU path
 adds taps left to right without fused multiply-adds, and parity depends on the
 same order) and with packed-FP32 instructions disabled.  Objects go to
 lapstack_amd/_build/ and the library is linked in place, next to this file.
@@ -45,11 +45,11 @@ def _compile(src: str) -> str:
 
 def build(jobs: int = 8) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

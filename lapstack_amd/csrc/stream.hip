@@ -27,12 +27,18 @@
 // takes its frame index from a device counter only the side work advances,
 // and the focus slice writes the caller's device slices at that index (host
 // slices leave through two fixed buffers).
+#include <atomic>
+#include <condition_variable>
 #include <cstdlib>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "api.hpp"
 #include "common.hpp"
 #include "engine.hpp"
+#include "imgio.hpp"
 
 using namespace lsh;
 
@@ -69,16 +75,129 @@ struct Graphs {
     }
 };
 
-}  // namespace
+// Frames decoded from files ahead of their use (stack.rs:277-360, the
+// reference's read-ahead): worker threads decode frame j into pinned slot
+// j % S once frame j - S has left it (its upload to the device, recorded as an
+// event on the copy stream, has completed), so decoding, the upload and the
+// device's work on earlier frames run at once.
+struct FileRing {
+    std::vector<std::string> paths;
+    size_t w, h, nvals;
+    int S = 0, dev = 0;
+    std::vector<uint16_t *> buf;
+    std::vector<hipEvent_t> ev;
+    std::vector<long> ready, free_from;
+    std::vector<char> pending;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::atomic<long> next{0};
+    std::vector<std::thread> th;
+    std::string err;
+    bool stop = false;
 
-extern "C" {
+    FileRing(const char *const *p, uint32_t n, size_t w_, size_t h_, unsigned threads) : w(w_), h(h_), nvals(w_ * h_ * 3) {
+        for (uint32_t i = 0; i < n; i++) paths.emplace_back(p[i]);
+        LSH_CHECK(hipGetDevice(&dev));
+        threads = std::max(1u, std::min(threads, n));
+        S = (int)std::min<uint32_t>(threads + 3, n);
+        buf.assign(S, nullptr);
+        ev.assign(S, nullptr);
+        ready.assign(S, -1);
+        pending.assign(S, 0);
+        free_from.resize(S);
+        for (int i = 0; i < S; i++) {
+            free_from[i] = i;
+            LSH_CHECK(hipHostMalloc((void **)&buf[i], nvals * sizeof(uint16_t), hipHostMallocDefault));
+            LSH_CHECK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+        }
+        for (unsigned t = 0; t < threads; t++) th.emplace_back([this] { work(); });
+    }
+    ~FileRing() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto &t : th) t.join();
+        for (int i = 0; i < S; i++) {
+            if (ev[i]) (void)hipEventSynchronize(ev[i]);
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+            if (buf[i]) (void)hipHostFree(buf[i]);
+        }
+    }
+    void work() {
+        (void)hipSetDevice(dev);
+        for (;;) {
+            const long j = next++;
+            if (j >= (long)paths.size()) return;
+            const int sl = (int)(j % S);
+            bool sync_ev = false;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !err.empty() || free_from[sl] >= j; });
+                if (stop || !err.empty()) return;
+                sync_ev = pending[sl];
+            }
+            std::string e;
+            try {
+                if (sync_ev) LSH_CHECK(hipEventSynchronize(ev[sl]));
+                io::Image im;
+                im.ext = buf[sl];
+                im.ext_cap = nvals;
+                io::decode(paths[j], im);
+                if (im.w != w || im.h != h)
+                    e = "stream: " + paths[j] + " is " + std::to_string(im.w) + "x" + std::to_string(im.h) +
+                        ", the stack is " + std::to_string(w) + "x" + std::to_string(h);
+            } catch (const std::exception &x) {
+                e = x.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!e.empty() && err.empty()) err = e;
+                if (e.empty()) ready[sl] = j;
+            }
+            cv.notify_all();
+        }
+    }
+    // frame k, decoded, in pinned memory
+    const uint16_t *get(uint32_t k) {
+        const int sl = (int)(k % S);
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return !err.empty() || ready[sl] == (long)k; });
+        if (!err.empty()) throw Error(err);
+        return buf[sl];
+    }
+    // frame k's upload is queued on st (or done, st null): its slot is the next frame's once that has run
+    void consumed(uint32_t k, hipStream_t st) {
+        const int sl = (int)(k % S);
+        if (st) LSH_CHECK(hipEventRecord(ev[sl], st));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            pending[sl] = st != nullptr;
+            ready[sl] = -1;
+            free_from[sl] = (long)k + S;
+        }
+        cv.notify_all();
+    }
+};
 
-char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *frames, uint32_t n, lsh_mem frames_mem,
-                       const uint8_t *free8, uint32_t coarsen, uint32_t brightness, const lsh_depth_params *measure,
-                       float *slices, lsh_mem slices_mem, uint32_t graph, double *sims_out, float *gains_out,
-                       uint64_t *evals_out) {
-    LSH_API_BEGIN
-    LSH_REQUIRE(f && fu && frames && free8, "lsh_stack_stream: null argument");
+// Where a stream's frames come from: device frames (a table of pointers), host
+// frames (pointers to host memory), or files decoded ahead (a FileRing).
+struct Source {
+    const uint16_t *const *frames = nullptr;
+    lsh_mem mem = LSH_DEVICE;
+    FileRing *ring = nullptr;
+    const uint16_t *host(uint32_t k) const { return ring ? ring->get(k) : frames[k]; }
+    void done(uint32_t k, hipStream_t st) const {
+        if (ring) ring->consumed(k, st);
+    }
+};
+
+char *stream_impl(lsh_frames *f, lsh_fuser *fu, const Source &src, uint32_t n, const uint8_t *free8, uint32_t coarsen,
+                  uint32_t brightness, const lsh_depth_params *measure, float *slices, lsh_mem slices_mem,
+                  uint32_t graph, double *sims_out, float *gains_out, uint64_t *evals_out) {
+    const lsh_mem frames_mem = src.mem;
+    const uint16_t *const *frames = src.frames;
     LSH_REQUIRE(n > 0, "stream: no frames");
     LSH_REQUIRE(f->processed == 0 && fu->count == 0, "stream: the handles must be fresh");
     LSH_REQUIRE(f->w == fu->w && f->h == fu->h, "stream: frames and fuser sizes differ");
@@ -95,9 +214,10 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
     double sim0[8];
     float g0[3];
     static const double ident[8] = {0, 0, 1, 0, 1, 0, 0, 0};
-    char *e = lsh_frames_process(f, frames[0], frames_mem, ident, free8, coarsen, nullptr, nullptr, brightness, measure,
-                                 sim0, g0, measure ? slices : nullptr, slices_mem);
+    char *e = lsh_frames_process(f, src.host(0), frames_mem, ident, free8, coarsen, nullptr, nullptr, brightness,
+                                 measure, sim0, g0, measure ? slices : nullptr, slices_mem);
     if (e) return e;
+    src.done(0, nullptr);  // the frame was copied before the call returned
     e = lsh_fuser_push_frames(fu, f);
     if (e) return e;
     std::vector<double> sims((size_t)n * 8);
@@ -163,7 +283,8 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
             const int sl = (int)(k % NRING);
             if (slot_used[sl]) LSH_CHECK(hipStreamWaitEvent(Cs, e_free[sl], 0));
             slot_used[sl] = true;
-            LSH_CHECK(hipMemcpyAsync(stage[sl].p, frames[k], fbytes, hipMemcpyHostToDevice, Cs));
+            LSH_CHECK(hipMemcpyAsync(stage[sl].p, src.host(k), fbytes, hipMemcpyHostToDevice, Cs));
+            src.done(k, Cs);
             LSH_CHECK(hipEventRecord(e_up[sl], Cs));
         };
         // stream st reads frame k only after its upload; frame k's slot is free
@@ -332,6 +453,43 @@ char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *fram
     if (sims_out) std::copy(sims.begin(), sims.end(), sims_out);
     if (gains_out) std::copy(gains.begin(), gains.end(), gains_out);
     if (evals_out) std::copy(evals.begin(), evals.end(), evals_out);
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+char *lsh_stack_stream(lsh_frames *f, lsh_fuser *fu, const uint16_t *const *frames, uint32_t n, lsh_mem frames_mem,
+                       const uint8_t *free8, uint32_t coarsen, uint32_t brightness, const lsh_depth_params *measure,
+                       float *slices, lsh_mem slices_mem, uint32_t graph, double *sims_out, float *gains_out,
+                       uint64_t *evals_out) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(f && fu && frames && free8, "lsh_stack_stream: null argument");
+    Source src;
+    src.frames = frames;
+    src.mem = frames_mem;
+    return stream_impl(f, fu, src, n, free8, coarsen, brightness, measure, slices, slices_mem, graph, sims_out,
+                       gains_out, evals_out);
+    LSH_API_END
+}
+
+char *lsh_stack_stream_files(lsh_frames *f, lsh_fuser *fu, const char *const *paths, uint32_t n, uint32_t threads,
+                             const uint8_t *free8, uint32_t coarsen, uint32_t brightness,
+                             const lsh_depth_params *measure, float *slices, lsh_mem slices_mem, uint32_t graph,
+                             double *sims_out, float *gains_out, uint64_t *evals_out) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(f && fu && paths && free8, "lsh_stack_stream_files: null argument");
+    LSH_REQUIRE(n > 0, "stream: no frames");
+    for (uint32_t i = 0; i < n; i++) LSH_REQUIRE(paths[i], "lsh_stack_stream_files: null path");
+    FileRing ring(paths, n, f->w, f->h, threads ? threads : 8);
+    Source src;
+    src.mem = LSH_HOST;
+    src.ring = &ring;
+    char *e = stream_impl(f, fu, src, n, free8, coarsen, brightness, measure, slices, slices_mem, graph, sims_out,
+                          gains_out, evals_out);
+    sync();  // every upload out of the ring has run before its buffers go
+    return e;
     LSH_API_END
 }
 

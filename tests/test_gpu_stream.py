@@ -111,3 +111,35 @@ def test_graph_reads_frames_through_the_table_and_writes_device_slices(n):
     assert np.array_equal(sd.cpu().numpy(), np.asarray(ref["slices"]))
     for other in (hf, hp, hl):
         assert np.array_equal(np.asarray(other["slices"]), np.asarray(ref["slices"]))
+
+
+@pytest.mark.parametrize("ext", ["tif", "png"])
+def test_stream_from_files_equals_the_device_frames(tmp_path, ext):
+    """lsh_stack_stream_files: frames written as 16-bit files, decoded by host
+    threads ahead of their use into pinned buffers and uploaded beside the
+    device's work -- the same result as the frames handed over in HBM."""
+    print(f"stream from {ext} files", flush=True)
+    w, h, n = 320, 224, 7
+    b = synth.Bracket(w, h, n, seed=29, jitter=True, device="cuda")
+    frames = [b.frame(i) for i in range(n)]
+    paths = []
+    for i, f in enumerate(frames):
+        p = str(tmp_path / f"frame{i:03d}.{ext}")
+        la.save_rgb_u16(f.cpu().numpy(), p)
+        paths.append(p)
+    fp, ap, dp = la.FuseParams(levels=5), la.AlignParams(coarsen=1, interp="bilinear"), la.DepthParams()
+    ref = la.stack_stream(frames, fp, ap, dp, graph=True)
+    for threads in (1, 4):
+        got = la.stack_stream_files(paths, fp, ap, dp, graph=True, threads=threads)
+        for k in ("image", "winner", "sims", "gains", "evals", "slices"):
+            assert np.array_equal(np.asarray(got[k]), np.asarray(ref[k])), (k, threads)
+    launched = la.stack_stream_files(paths, fp, ap, dp, graph=False, threads=2)
+    for k in ("image", "winner", "sims", "gains", "evals", "slices"):
+        assert np.array_equal(np.asarray(launched[k]), np.asarray(ref[k])), k
+    # a frame of another size ends the call with an error
+    la.save_rgb_u16(np.zeros((h, w - 2, 3), np.uint16), paths[4])
+    with pytest.raises(la.HipError, match="the stack is"):
+        la.stack_stream_files(paths, fp, ap, dp, graph=True, threads=3)
+    la.save_rgb_u16(np.zeros((h, w + 2, 3), np.uint16), paths[4])
+    with pytest.raises(la.HipError, match="larger than the frame buffer"):
+        la.stack_stream_files(paths, fp, ap, dp, graph=True, threads=3)
