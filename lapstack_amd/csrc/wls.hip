@@ -239,6 +239,233 @@ __global__ __launch_bounds__(256) void k_solve_rows(float *__restrict__ u, const
 // (coalesced across threads); the inputs of 16 rows are loaded ahead of the
 // 16 recurrence steps that use them
 #define FGS_C 16
+// ---- the FGS sweeps, partitioned
+// The row and column solves of depth.rs:674-746 (tridiagonal: lower -lam a_{i-1},
+// diagonal wd_i + lam (a_{i-1} + a_i), upper -lam a_i, right-hand side wd_i f_i)
+// by the partition method, so a row or column is solved by many lanes instead
+// of one: its cells are cut into segments [L, R] whose last cell R is an
+// interface unknown z.  Each segment's interior [L, R-1] is eliminated by its
+// own Thomas sweep with three right-hand sides -- the data y, and the
+// responses g, h to its left and right interface values -- so
+// x_i = y_i + z_left g_i + z_right h_i there; the equation of each interface
+// cell then couples z_{s-1}, z_s, z_{s+1} only, a tridiagonal system of one
+// unknown per segment, and a last backward sweep per segment writes x.  The
+// arithmetic per cell is the reference's (its pivot guard included); the
+// order of elimination differs, so the sweep's result matches the sequential
+// one to rounding (the guess PCG starts from, not the solution).
+#define FGS_SEG 64  // segments per row: one wave, one lane each
+
+struct Seg {
+    int L, R;  // the segment's interior is [L, R-1], its interface cell R
+};
+__device__ __forceinline__ Seg seg_of(int s, int nseg, int n) {
+    return Seg{(int)((long long)s * n / nseg), (int)((long long)(s + 1) * n / nseg) - 1};
+}
+
+// one row per block of one wave; the row's a, wd, f staged in LDS (3 * w
+// floats, dynamic), where the forward sweep leaves c', y', g' and the last
+// sweep the solution
+__global__ __launch_bounds__(64) void k_fgs_rows_seg(float *__restrict__ u, const float *__restrict__ f,
+                                                     const float *__restrict__ wd, const float *__restrict__ a, int w,
+                                                     float lam) {
+    extern __shared__ float lds[];
+    float *A = lds, *W = lds + w, *F = lds + 2 * w;
+    const int lane = threadIdx.x;
+    const size_t row = (size_t)blockIdx.x * w;
+    for (int i = lane; i < w; i += 64) {
+        A[i] = a[row + i];
+        W[i] = wd ? wd[row + i] : 1.0f;
+        F[i] = f[row + i];
+    }
+    __syncthreads();
+    const Seg sg = seg_of(lane, FGS_SEG, w);
+    // forward over the interior, c' y' g' in place
+    float prev_a = sg.L > 0 ? A[sg.L - 1] : 0.0f;
+    float cpv = 0.0f, ypv = 0.0f, gpv = 0.0f, den_last = 1.0f, up_last = 0.0f;
+    for (int i = sg.L; i < sg.R; i++) {
+        const float ai = A[i];  // i + 1 <= R < w: an interior cell always has a right neighbour
+        const float wdi = W[i];
+        const float diag = wdi + lam * (prev_a + ai);
+        const float lower = -lam * prev_a;
+        const float upper = -lam * ai;
+        float m = diag - lower * cpv;
+        if (fabsf(m) < 1e-12f) m = 1e-12f;
+        cpv = upper / m;
+        ypv = (wdi * F[i] - lower * ypv) / m;
+        gpv = i == sg.L ? -lower / m : (0.0f - lower * gpv) / m;
+        A[i] = cpv;
+        W[i] = ypv;
+        F[i] = gpv;
+        prev_a = ai;
+        den_last = m;
+        up_last = upper;
+    }
+    // the responses at the interior's two ends: backward sweeps of y, g, h
+    float ye = 0.0f, ge = 1.0f, he = 0.0f, ys = 0.0f, gs = 0.0f, hs = 1.0f;  // (an empty interior: x_{R-1} = z_{s-1})
+    if (sg.R > sg.L) {
+        float yb = W[sg.R - 1], gb = F[sg.R - 1], hb = -up_last / den_last;
+        ye = yb;
+        ge = gb;
+        he = hb;
+        for (int i = sg.R - 2; i >= sg.L; i--) {
+            const float c = A[i];
+            yb = W[i] - c * yb;
+            gb = F[i] - c * gb;
+            hb = -c * hb;
+        }
+        ys = yb;
+        gs = gb;
+        hs = hb;
+    }
+    // the interface cell's equation in z_{s-1}, z_s, z_{s+1}
+    const int R = sg.R;
+    const float ar = R + 1 < w ? A[R] : 0.0f, wr = W[R];
+    const float lo = -lam * prev_a, up = -lam * ar, di = wr + lam * (prev_a + ar), rhs = wr * F[R];
+    float ysn = __shfl_down(ys, 1, 64), gsn = __shfl_down(gs, 1, 64), hsn = __shfl_down(hs, 1, 64);
+    if (lane == FGS_SEG - 1) ysn = gsn = hsn = 0.0f;
+    float Ac = lo * ge, Bc = lo * he + di + up * gsn, Cc = up * hsn, Dc = rhs - lo * ye - up * ysn;
+    // parallel cyclic reduction over the wave
+    for (int d = 1; d < FGS_SEG; d <<= 1) {
+        float Am = __shfl_up(Ac, d, 64), Bm = __shfl_up(Bc, d, 64), Cm = __shfl_up(Cc, d, 64), Dm = __shfl_up(Dc, d, 64);
+        float Ap = __shfl_down(Ac, d, 64), Bp = __shfl_down(Bc, d, 64), Cp = __shfl_down(Cc, d, 64),
+              Dp = __shfl_down(Dc, d, 64);
+        if (lane < d) {
+            Am = Cm = Dm = 0.0f;
+            Bm = 1.0f;
+        }
+        if (lane + d >= FGS_SEG) {
+            Ap = Cp = Dp = 0.0f;
+            Bp = 1.0f;
+        }
+        const float al = -Ac / Bm, ga = -Cc / Bp;
+        const float nA = al * Am, nC = ga * Cp, nB = Bc + al * Cm + ga * Ap, nD = Dc + al * Dm + ga * Dp;
+        Ac = nA;
+        Bc = nB;
+        Cc = nC;
+        Dc = nD;
+    }
+    const float z = Dc / Bc;
+    float zl = __shfl_up(z, 1, 64);
+    if (lane == 0) zl = 0.0f;
+    // the segment's cells: x_i = (y'_i + z_{s-1} g'_i + z_s h'_i) - c'_i x_{i+1}
+    if (sg.R > sg.L) {
+        float x = W[sg.R - 1] + zl * F[sg.R - 1] + z * (-up_last / den_last);
+        A[sg.R - 1] = x;
+        for (int i = sg.R - 2; i >= sg.L; i--) {
+            x = (W[i] + zl * F[i]) - A[i] * x;
+            A[i] = x;
+        }
+    }
+    A[R] = z;
+    __syncthreads();
+    for (int i = lane; i < w; i += 64) u[row + i] = A[i];
+}
+
+// 64 columns per block, FGS_CSEG segments down each column (one wave per
+// segment, one lane per column): c', y', g' go through global scratch (cq, yq,
+// gq), the interface system of each column through LDS, solved by one lane
+#define FGS_CSEG 16
+__global__ __launch_bounds__(64 * FGS_CSEG) void k_fgs_cols_seg(float *__restrict__ u, const float *__restrict__ f,
+                                                              const float *__restrict__ wd, const float *__restrict__ a,
+                                                              int w, int h, float lam, float *__restrict__ cq,
+                                                              float *__restrict__ yq, float *__restrict__ gq) {
+    __shared__ float S0[FGS_CSEG][64], S1[FGS_CSEG][64], S2[FGS_CSEG][64];
+    __shared__ float EA[FGS_CSEG][64], EB[FGS_CSEG][64], EC[FGS_CSEG][64], ED[FGS_CSEG][64], Z[FGS_CSEG][64];
+    const int tx = threadIdx.x & 63, s = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + tx;
+    const bool live = x < w;
+    const Seg sg = seg_of(s, FGS_CSEG, h);
+    const int xc = live ? x : w - 1;
+    auto at = [&](int y) { return (size_t)y * w + xc; };
+    float prev_a = sg.L > 0 ? a[at(sg.L - 1)] : 0.0f;
+    float cpv = 0.0f, ypv = 0.0f, gpv = 0.0f, den_last = 1.0f, up_last = 0.0f;
+    for (int y = sg.L; y < sg.R; y++) {
+        const size_t o = at(y);
+        const float ai = a[o], wdi = wd ? wd[o] : 1.0f, fi = f[o];
+        const float diag = wdi + lam * (prev_a + ai);
+        const float lower = -lam * prev_a;
+        const float upper = -lam * ai;
+        float m = diag - lower * cpv;
+        if (fabsf(m) < 1e-12f) m = 1e-12f;
+        cpv = upper / m;
+        ypv = (wdi * fi - lower * ypv) / m;
+        gpv = y == sg.L ? -lower / m : (0.0f - lower * gpv) / m;
+        if (live) {
+            cq[o] = cpv;
+            yq[o] = ypv;
+            gq[o] = gpv;
+        }
+        prev_a = ai;
+        den_last = m;
+        up_last = upper;
+    }
+    float ye = 0.0f, ge = 1.0f, he = 0.0f, ys = 0.0f, gs = 0.0f, hs = 1.0f;
+    if (sg.R > sg.L) {
+        float yb = ypv, gb = gpv, hb = -up_last / den_last;
+        ye = yb;
+        ge = gb;
+        he = hb;
+        for (int y = sg.R - 2; y >= sg.L; y--) {
+            const size_t o = at(y);
+            const float c = cq[o];
+            yb = yq[o] - c * yb;
+            gb = gq[o] - c * gb;
+            hb = -c * hb;
+        }
+        ys = yb;
+        gs = gb;
+        hs = hb;
+    }
+    S0[s][tx] = ys;
+    S1[s][tx] = gs;
+    S2[s][tx] = hs;
+    __syncthreads();
+    {
+        const int R = sg.R;
+        const size_t o = at(R);
+        const float ar = R + 1 < h ? a[o] : 0.0f, wr = wd ? wd[o] : 1.0f;
+        const float lo = -lam * prev_a, up = -lam * ar, di = wr + lam * (prev_a + ar), rhs = wr * f[o];
+        const bool last = s == FGS_CSEG - 1;
+        const float ysn = last ? 0.0f : S0[s + 1][tx], gsn = last ? 0.0f : S1[s + 1][tx], hsn = last ? 0.0f : S2[s + 1][tx];
+        EA[s][tx] = lo * ge;
+        EB[s][tx] = lo * he + di + up * gsn;
+        EC[s][tx] = up * hsn;
+        ED[s][tx] = rhs - lo * ye - up * ysn;
+    }
+    __syncthreads();
+    if (s == 0) {  // the column's interface system, by Thomas
+        float cpz = 0.0f, dpz = 0.0f;
+        for (int k = 0; k < FGS_CSEG; k++) {
+            float m = EB[k][tx] - EA[k][tx] * cpz;
+            if (fabsf(m) < 1e-12f) m = 1e-12f;
+            cpz = EC[k][tx] / m;
+            dpz = (ED[k][tx] - EA[k][tx] * dpz) / m;
+            EC[k][tx] = cpz;
+            ED[k][tx] = dpz;
+        }
+        float zz = ED[FGS_CSEG - 1][tx];
+        Z[FGS_CSEG - 1][tx] = zz;
+        for (int k = FGS_CSEG - 2; k >= 0; k--) {
+            zz = ED[k][tx] - EC[k][tx] * zz;
+            Z[k][tx] = zz;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    const float z = Z[s][tx], zl = s > 0 ? Z[s - 1][tx] : 0.0f;
+    if (sg.R > sg.L) {
+        size_t o = at(sg.R - 1);
+        float xv = ypv + zl * gpv + z * (-up_last / den_last);
+        u[o] = xv;
+        for (int y = sg.R - 2; y >= sg.L; y--) {
+            o = at(y);
+            xv = (yq[o] + zl * gq[o]) - cq[o] * xv;
+            u[o] = xv;
+        }
+    }
+    u[at(sg.R)] = z;
+}
+
 __global__ void k_solve_cols(float *__restrict__ u, const float *__restrict__ f, const float *__restrict__ wd,
                              const float *__restrict__ a, int w, int h, float lam, float *__restrict__ cp,
                              float *__restrict__ dp) {
@@ -815,12 +1042,25 @@ static void fgs_guess(WlsSolver &s, const float *d, float *u) {
     const size_t n = (size_t)w * h;
     const float *wd = s.L[0].wd.p;
     const float *src = d;
+    // partitioned sweeps where every segment has an interior (rows of at least
+    // 2 * FGS_SEG cells, columns of 2 * FGS_CSEG) and a row fits in LDS; else
+    // the sequential ones
+    const bool rows_seg = w >= 2 * FGS_SEG && (size_t)3 * w * sizeof(float) <= 64 * 1024;
+    const bool cols_seg = h >= 2 * FGS_CSEG;
     for (int t = 1; t <= T; t++) {
         float lam_t = 1.5f * s.lambda * std::pow(4.0f, (float)(T - t)) / (std::pow(4.0f, (float)T) - 1.0f);
-        launch_b("fgs_rows", 16.0 * n, k_solve_rows, dim3(cdiv(h, FGS_R)), dim3(256), 0, u, src, t == 1 ? wd : nullptr,
-                 (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
-        launch_b("fgs_cols", 16.0 * n, k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, s.f.p, (const float *)u, nullptr,
-                 (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
+        if (rows_seg)
+            launch_b("fgs_rows", (t == 1 ? 16.0 : 12.0) * n, k_fgs_rows_seg, dim3(h), dim3(64),
+                     (size_t)3 * w * sizeof(float), u, src, t == 1 ? wd : nullptr, (const float *)s.ex.p, w, lam_t);
+        else
+            launch_b("fgs_rows", 16.0 * n, k_solve_rows, dim3(cdiv(h, FGS_R)), dim3(256), 0, u, src,
+                     t == 1 ? wd : nullptr, (const float *)s.ex.p, w, h, lam_t, s.cp.p, s.dp.p);
+        if (cols_seg)
+            launch_b("fgs_cols", 12.0 * n, k_fgs_cols_seg, dim3(cdiv(w, 64)), dim3(64 * FGS_CSEG), 0, s.f.p,
+                     (const float *)u, nullptr, (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p, s.z.p);
+        else
+            launch_b("fgs_cols", 16.0 * n, k_solve_cols, dim3(cdiv(w, 64)), dim3(64), 0, s.f.p, (const float *)u,
+                     nullptr, (const float *)s.ey.p, w, h, lam_t, s.cp.p, s.dp.p);
         src = s.f.p;
     }
     LSH_CHECK(hipMemcpyAsync(u, s.f.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));

@@ -57,6 +57,20 @@ def test_reference_mode_matches_oracle(w, h, lam, tol):
     assert np.max(np.abs(u - want)) < 1e-4 * 7.0
 
 
+@pytest.mark.parametrize("w,h,lam", [(256, 192, 0.5), (1000, 40, 3.0), (130, 700, 0.1), (97, 61, 3.0)])
+def test_fgs_guess_matches_the_oracle_sweeps(w, h, lam):
+    """The FGS guess (depth.rs:1019-1034; max_iters 0 stops before PCG): the
+    partitioned row and column sweeps (rows of >= 128 cells, columns of >= 32)
+    against the oracle's sequential Thomas sweeps, to rounding."""
+    guide, conf, d = _problem(w, h, 5 + w)
+    want, _, _ = ref.wls_solve(d, conf, guide, lam, 0.05, 0)
+    u, rep, _ = la.wls_solve(guide, conf, d, lam, 0.05, "reference", max_iters=0)
+    assert rep["iters"] == 0
+    err = np.max(np.abs(u - want))
+    print(f"FGS guess {w}x{h}: max |device - oracle| {err:.2e}", flush=True)
+    assert err < 2e-5 * 7.0
+
+
 @pytest.mark.parametrize("w,h,lam", [(96, 64, 0.1), (61, 47, 5.0), (17, 9, 1.0), (8, 8, 2.0)])
 def test_fp64_mode_matches_direct_solve(w, h, lam):
     guide, conf, d = _problem(w, h, 5 + h)
