@@ -1396,10 +1396,12 @@ static void sums_launch(lsh_frames *f, const FrameRef &rgb, const Inv9 &inv, int
     f->red.ensure((size_t)grid * 7);
     const float *gp = f->have_grid ? f->grid.p : nullptr;
     int ik = (int)f->interp, w = (int)f->w, h = (int)f->h, gw = (int)f->gw, gh = (int)f->gh;
+    // per step-4 grid point: the kernel's N x N taps of the three u16 channels, frame 0's three f32 samples
+    const double nt = (double)taps_n(f->interp), by = (double)ng * (6.0 * nt * nt + 12.0);
     switch (taps_n(f->interp)) {
-    case 2: launch("bright_sums", k_bright_sums<2>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
-    case 4: launch("bright_sums", k_bright_sums<4>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
-    default: launch("bright_sums", k_bright_sums<6>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
+    case 2: launch_b("bright_sums", by, k_bright_sums<2>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
+    case 4: launch_b("bright_sums", by, k_bright_sums<4>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
+    default: launch_b("bright_sums", by, k_bright_sums<6>, dim3(grid), dim3(256), 0, rgb, w, h, inv, affine, ik, mode, gp, gw, gh, f->red.p, f->ticket.p, f->red_out.p, fp); break;
     }
     if (!host7) return;
     double *pin = (double *)pinned_scratch(64);
@@ -1799,7 +1801,7 @@ char *lsh_frames_process(lsh_frames *f, const uint16_t *rgb, lsh_mem rgb_mem, co
     if (!known_sim && brightness) {
         if (first) {
             dim3 gg(cdiv(f->gw, 64), cdiv(f->gh, 4));
-            launch("bright_grid", k_grid, gg, dim3(64, 4), 0, src, (int)w, (int)h, (int)f->gw, (int)f->gh, f->grid.p);
+            launch_b("bright_grid", 18.0 * f->gw * f->gh, k_grid, gg, dim3(64, 4), 0, src, (int)w, (int)h, (int)f->gw, (int)f->gh, f->grid.p);
             f->have_grid = true;
         } else if (f->have_grid) {
             double s7[7];

@@ -604,7 +604,7 @@ static float select_kth(const float *d, size_t n, size_t stride, size_t k, DBuf<
     for (auto &pl : plan) {
         int shift = pl[0], bits = pl[1], nb = 1 << bits;
         LSH_CHECK(hipMemsetAsync(hist.p, 0, nb * sizeof(unsigned), stream()));
-        launch("radix_select", k_radix_hist, dim3(grid), dim3(256), 0, d, n, stride, prefix, pbits, shift, bits, hist.p);
+        launch_b("radix_select", 4.0 * ns, k_radix_hist, dim3(grid), dim3(256), 0, d, n, stride, prefix, pbits, shift, bits, hist.p);
         copy_mem(hh.data(), LSH_HOST, hist.p, LSH_DEVICE, nb * sizeof(unsigned));
         size_t acc = 0;
         int bin = 0;
@@ -769,14 +769,15 @@ static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, ui
     DBuf<float> &yfull = G.yfull, &guide = G.guide;
     yfull.alloc(N);
     guide.alloc(M);
-    launch("luma", k_luma3, dim3(g1(N)), dim3(256), 0, fz, N, yfull.p);
-    launch("block_mean", k_block_mean, g2(dw, dh), B2, 0, (const float *)yfull.p, (int)w, (int)h, k, guide.p, dw, dh);
+    launch_b("luma", 16.0 * N, k_luma3, dim3(g1(N)), dim3(256), 0, fz, N, yfull.p);
+    launch_b("block_mean", 4.0 * ((double)N + (double)dw * dh), k_block_mean, g2(dw, dh), B2, 0, (const float *)yfull.p,
+             (int)w, (int)h, k, guide.p, dw, dh);
     // aggregation and the streamed peak search (depth.rs:1137-1151)
     PeakBufs PB;
     for (auto *b : {&PB.c1, &PB.l1, &PB.r1, &PB.c2, &PB.prev, &PB.prev2, &PB.sum, &PB.cmin}) b->alloc(M);
     PB.i1.alloc(M);
     Peaks P = PB.view();
-    launch("peak_init", k_peak_init, dim3(g1(M)), dim3(256), 0, P, M);
+    launch_b("peak_init", 36.0 * M, k_peak_init, dim3(g1(M)), dim3(256), 0, P, M);
     GF gf;
     if (p->agg_radius > 0) gf_init(gf, guide.p, dw, dh, (int)p->agg_radius, p->agg_eps);
     DBuf<float> stage;
@@ -791,11 +792,11 @@ static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, ui
             gf_step1(gf, c);
             gf_step2(gf, 2, nullptr, nullptr, P, m);
         } else {
-            launch("peak_push", k_peak_push_raw, dim3(g1(M)), dim3(256), 0, c, M, P, m);
+            launch_b("peak_push", 76.0 * M, k_peak_push_raw, dim3(g1(M)), dim3(256), 0, c, M, P, m);
         }
     }
     // close the profiles, the noise floor (depth.rs:568-580)
-    launch("peak_close", k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
+    launch_b("peak_close", 72.0 * M, k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
     DBuf<unsigned> hist;
     size_t ns = (M + 6) / 7;
     float floor_ = G.floor_ = select_kth(PB.cmin.p, M, 7, ns / 2, hist);
@@ -804,17 +805,17 @@ static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, ui
     conf.alloc(M);
     mins.alloc(M);
     copy_mem(floor_d.p, LSH_DEVICE, &floor_, LSH_HOST, sizeof(float));
-    launch("peak_out", k_peak_out, dim3(g1(M)), dim3(256), 0, P, M, n, p->gate, (const float *)floor_d.p, depth_w.p,
+    launch_b("peak_out", 52.0 * M, k_peak_out, dim3(g1(M)), dim3(256), 0, P, M, n, p->gate, (const float *)floor_d.p, depth_w.p,
            conf.p, mins.p);
     if (p->median) {
         DBuf<float> med(M);
-        launch("median3", k_median3, g2(dw, dh), B2, 0, (const float *)depth_w.p, dw, dh, med.p);
+        launch_b("median3", 8.0 * M, k_median3, g2(dw, dh), B2, 0, (const float *)depth_w.p, dw, dh, med.p);
         depth_w = std::move(med);
     }
     // p90 normalisation (depth.rs:615-623)
     size_t kk = std::min(ns * 9 / 10, ns - 1);
     float p90 = G.p90 = select_kth(conf.p, M, 7, kk, hist);
-    if (p90 > 1e-6f) launch("conf_norm", k_conf_norm, dim3(g1(M)), dim3(256), 0, conf.p, M, p90);
+    if (p90 > 1e-6f) launch_b("conf_norm", 8.0 * M, k_conf_norm, dim3(g1(M)), dim3(256), 0, conf.p, M, p90);
 }
 
 extern "C" {
@@ -871,7 +872,8 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
     }
     float maxd = (float)(n - 1), inv = 1.0f / (float)k;
     if (p->upsample_kind == 0) {
-        launch("upsample", k_up_bilinear, g2(w, h), B2, 0, dres, dw, dh, (int)w, (int)h, inv, 0.0f, maxd, 1, dd);
+        launch_b("upsample", 4.0 * ((double)M + N), k_up_bilinear, g2(w, h), B2, 0, dres, dw, dh, (int)w, (int)h, inv,
+                 0.0f, maxd, 1, dd);
     } else {
         GF ug;
         gf_init(ug, guide.p, dw, dh, (int)p->up_radius, p->up_eps);
@@ -879,18 +881,18 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         Peaks none{};
         if (k == 1) {
             gf_step2(ug, 4, dd, nullptr, none, 0);
-            launch("clamp", k_clamp, dim3(g1(N)), dim3(256), 0, dd, N, 0.0f, maxd);
+            launch_b("clamp", 8.0 * N, k_clamp, dim3(g1(N)), dim3(256), 0, dd, N, 0.0f, maxd);
         } else {
             DBuf<float> ca(M), cb(M);
             gf_step2(ug, 3, ca.p, cb.p, none, 0);
-            launch("upsample", k_up_guided, g2(w, h), B2, 0, (const float *)ca.p, (const float *)cb.p, dw, dh,
+            launch_b("upsample", 8.0 * ((double)M + N), k_up_guided, g2(w, h), B2, 0, (const float *)ca.p, (const float *)cb.p, dw, dh,
                    (const float *)yfull.p, (int)w, (int)h, inv, maxd, dd);
         }
     }
     if (k == 1)
         LSH_CHECK(hipMemcpyAsync(cc, conf.p, M * sizeof(float), hipMemcpyDeviceToDevice, stream()));
     else
-        launch("upsample", k_up_bilinear, g2(w, h), B2, 0, (const float *)conf.p, dw, dh, (int)w, (int)h, inv, 0.0f, 0.0f, 0, cc);
+        launch_b("upsample", 4.0 * ((double)M + N), k_up_bilinear, g2(w, h), B2, 0, (const float *)conf.p, dw, dh, (int)w, (int)h, inv, 0.0f, 0.0f, 0, cc);
     if (out_mem == LSH_HOST) {
         copy_mem(depth, LSH_HOST, dd, LSH_DEVICE, N * sizeof(float));
         copy_mem(conf_out, LSH_HOST, cc, LSH_DEVICE, N * sizeof(float));

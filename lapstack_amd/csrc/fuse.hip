@@ -138,52 +138,57 @@ __global__ void k_devent(const float *__restrict__ tops, int nf, int tw, int th,
 }
 
 // fuse.rs:248-306: Pareto (or deviation-only, or average) fusion of the residuals
-__global__ void k_residual(const float *__restrict__ tops, const float *__restrict__ dev,
-                           const float *__restrict__ ent, int nf, size_t tsz, int rule, float *__restrict__ out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= tsz) return;
-    float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-    if (nf == 1 || rule == 2) {
-        float inv = 1.0f / (float)nf;
-        for (int t = 0; t < nf; t++) {
-            const float *tp = tops + (size_t)t * 3 * tsz;
-            o0 += tp[i] * inv;
-            o1 += tp[tsz + i] * inv;
-            o2 += tp[2 * tsz + i] * inv;
+// One block per cell: every frame's (deviation, entropy) of the cell staged in
+// LDS, thread a decides whether frame a is dominated (fuse.rs:283-296), then
+// three threads (one per channel) add the kept frames' coefficients in frame
+// order, as the reference does.
+#define RES_MAXF 4096
+__global__ __launch_bounds__(128) void k_residual(const float *__restrict__ tops, const float *__restrict__ dev,
+                                                  const float *__restrict__ ent, int nf, size_t tsz, int rule,
+                                                  float *__restrict__ out) {
+    __shared__ float D[RES_MAXF], E[RES_MAXF];
+    __shared__ unsigned char keep[RES_MAXF];
+    __shared__ int kept_n;
+    const size_t i = blockIdx.x;
+    const bool all = nf == 1 || rule == 2;
+    if (!all) {
+        for (int a = threadIdx.x; a < nf; a += blockDim.x) {
+            D[a] = dev[(size_t)a * tsz + i];
+            E[a] = ent[(size_t)a * tsz + i];
         }
-    } else {
-        bool use_entropy = rule == 0;
-        int kept = 0;
-        for (int a = 0; a < nf; a++) {
-            float da = dev[(size_t)a * tsz + i], ea = ent[(size_t)a * tsz + i];
+        __syncthreads();
+        const bool use_entropy = rule == 0;
+        for (int a = threadIdx.x; a < nf; a += blockDim.x) {
+            const float da = D[a], ea = E[a];
             bool dominated = false;
             for (int b = 0; b < nf && !dominated; b++) {
                 if (a == b) continue;
-                float db = dev[(size_t)b * tsz + i], eb = ent[(size_t)b * tsz + i];
+                const float db = D[b], eb = E[b];
                 dominated = use_entropy ? (db >= da && eb >= ea && (db > da || eb > ea)) : (db > da || (db == da && b < a));
             }
-            kept += !dominated;
+            keep[a] = !dominated;
         }
-        float inv = 1.0f / (float)(kept > 1 ? kept : 1);
-        for (int a = 0; a < nf; a++) {
-            float da = dev[(size_t)a * tsz + i], ea = ent[(size_t)a * tsz + i];
-            bool dominated = false;
-            for (int b = 0; b < nf && !dominated; b++) {
-                if (a == b) continue;
-                float db = dev[(size_t)b * tsz + i], eb = ent[(size_t)b * tsz + i];
-                dominated = use_entropy ? (db >= da && eb >= ea && (db > da || eb > ea)) : (db > da || (db == da && b < a));
-            }
-            if (!dominated) {
-                const float *tp = tops + (size_t)a * 3 * tsz;
-                o0 += tp[i] * inv;
-                o1 += tp[tsz + i] * inv;
-                o2 += tp[2 * tsz + i] * inv;
-            }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int k = 0;
+            for (int a = 0; a < nf; a++) k += keep[a];
+            kept_n = k;
         }
+        __syncthreads();
     }
-    out[i] = o0;
-    out[tsz + i] = o1;
-    out[2 * tsz + i] = o2;
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        float o = 0.0f;
+        if (all) {
+            const float inv = 1.0f / (float)nf;
+            for (int t = 0; t < nf; t++) o += tops[(size_t)t * 3 * tsz + (size_t)c * tsz + i] * inv;
+        } else {
+            const float inv = 1.0f / (float)(kept_n > 1 ? kept_n : 1);
+            for (int a = 0; a < nf; a++)
+                if (keep[a]) o += tops[(size_t)a * 3 * tsz + (size_t)c * tsz + i] * inv;
+        }
+        out[(size_t)c * tsz + i] = o;
+    }
 }
 
 // fuse.rs:336-346
@@ -495,12 +500,13 @@ char *lsh_fuser_finish(lsh_fuser *f, float *img3, lsh_mem img_mem, float *winner
             dev.alloc(tsz * nf);
             ent.alloc(tsz * nf);
             size_t bins = std::min<size_t>(std::max<size_t>(f->p.entropy_bins, 2), 65536);
-            launch("residual_devent", k_devent, dim3(cdiv(tsz * nf, 128)), dim3(128), 0, (const float *)f->tops.p, nf,
+            launch_b("residual_devent", 4.0 * (double)tsz * nf * 5.0, k_devent, dim3(cdiv(tsz * nf, 128)), dim3(128), 0, (const float *)f->tops.p, nf,
                    (int)f->lw[L], (int)f->lh[L], (int)f->p.top_radius, (float)(bins - 1), (const float *)f->ent_tab.p,
                    dev.p, ent.p);
         }
-        launch("residual_fuse", k_residual, dim3(cdiv(tsz, 128)), dim3(128), 0, (const float *)f->tops.p,
-               (const float *)dev.p, (const float *)ent.p, nf, tsz, (int)f->p.top_rule, top);
+        LSH_REQUIRE(nf <= RES_MAXF, "residual rule: more than 4096 frames");
+        launch_b("residual_fuse", 4.0 * (double)tsz * (5.0 * nf + 3.0), k_residual, dim3((unsigned)tsz), dim3(128), 0,
+                 (const float *)f->tops.p, (const float *)dev.p, (const float *)ent.p, nf, tsz, (int)f->p.top_rule, top);
         sync();
     }
     // collapse (pyramid.rs:219-229): G_l = acc_l + EXPAND(G_{l+1}); level 0 clamped straight into the output
@@ -528,7 +534,7 @@ char *lsh_fuser_finish(lsh_fuser *f, float *img3, lsh_mem img_mem, float *winner
             dst = wf.p;
         }
         dim3 g(cdiv(f->w, 64), cdiv(f->h, 4));
-        launch("winner_upsample", k_upsample_index, g, dim3(64, 4), 0, (const uint16_t *)f->winner.p, (int)f->lw[d],
+        launch_b("winner_upsample", 2.0 * f->lw[d] * f->lh[d] + 4.0 * N, k_upsample_index, g, dim3(64, 4), 0, (const uint16_t *)f->winner.p, (int)f->lw[d],
                (int)f->lh[d], (int)f->w, (int)f->h, (int)d, dst);
         if (win_mem == LSH_HOST) copy_mem(winner_full, LSH_HOST, wf.p, LSH_DEVICE, N * sizeof(float));
     }

@@ -855,8 +855,9 @@ static void wls_init(WlsSolver &s, const float *guide, float lambda, float sigma
              s.L[0].ay.p);
     for (size_t l = 1; l < s.L.size(); l++) {
         Lev<float> &fl = s.L[l - 1], &c = s.L[l];
-        launch("wls_coarsen", k_coarsen, g2(c.w, c.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
-               (const float *)fl.ay.p, fl.w, fl.h, c.wd.p, c.ax.p, c.ay.p, c.w, c.h, 2);
+        launch_b("wls_coarsen", 8.0 * ((double)fl.w * fl.h + (double)c.w * c.h), k_coarsen, g2(c.w, c.h), B2, 0,
+                 (const float *)fl.wd.p, (const float *)fl.ax.p, (const float *)fl.ay.p, fl.w, fl.h, c.wd.p, c.ax.p,
+                 c.ay.p, c.w, c.h, 2);
     }
 }
 
@@ -895,10 +896,12 @@ void wls_set_weights(WlsSolver *sp, const float *conf, const float *d, const flo
         Lev<float> &lv = s.L[l];
         if (l > 0) {
             Lev<float> &fl = s.L[l - 1];
-            launch("wls_coarsen", k_coarsen, g2(lv.w, lv.h), B2, 0, (const float *)fl.wd.p, (const float *)fl.ax.p,
-                   (const float *)fl.ay.p, fl.w, fl.h, lv.wd.p, lv.ax.p, lv.ay.p, lv.w, lv.h, 1);
+            launch_b("wls_coarsen", 4.0 * ((double)fl.w * fl.h + (double)lv.w * lv.h), k_coarsen, g2(lv.w, lv.h), B2,
+                     0, (const float *)fl.wd.p, (const float *)fl.ax.p, (const float *)fl.ay.p, fl.w, fl.h, lv.wd.p,
+                     lv.ax.p, lv.ay.p, lv.w, lv.h, 1);
         }
-        launch("wls_dinv", k_dinv<float>, g2(lv.w, lv.h), B2, 0, (const float *)lv.wd.p, (const float *)lv.ax.p,
+        launch_b("wls_dinv", 16.0 * lv.w * lv.h, k_dinv<float>, g2(lv.w, lv.h), B2, 0, (const float *)lv.wd.p,
+                 (const float *)lv.ax.p,
                (const float *)lv.ay.p, lv.w, lv.h, lv.dinv.p);
     }
     s.l64_ok = false;
