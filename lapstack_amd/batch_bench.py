@@ -5,13 +5,10 @@ uniform in [2, 12] MP with aspect 3:2 or 4:3, interleaved RGB u16 resident in
 HBM; pyramid depth per stack floor(log2(min side)) - 4.  The fuse path only
 (the frame as decoded, REDUCE chain, region-energy fold, residual store).
 
-Each stack is a two-level focus bracket cut from one seeded master: the
-scene of synth.scene at 4352 x 3072 (it holds the largest stack at either
-aspect) and its Gaussian blur at sigma = 3 px, made once (one FFT size, so the
-FFT library builds its kernels once); a stack takes a w x h window at a seeded
-offset and its own depth map (synth.depth_map), and frame i mixes the two by
-min(1, 2 |depth - f_i|) for the focus f_i = i / (n - 1), plus the noise and
-quantisation of synth.Bracket.
+Each stack is a seeded focus bracket of the benchmark's generator
+(synth.Bracket at the stack's own size, no jitter): its own 1/f scene and step
+edges, its own depth map, frame i blurred per pixel by 6 px |depth - f_i|,
+noise, u16 quantisation; stack k's seed is s0 + 1000000 k.
 
 Timed, after a warm-up of both paths:
   loop   -- per stack, per frame: lsh_frames_ingest + lsh_fuser_push_frames
@@ -52,36 +49,16 @@ def plan(n_stacks, seed):
     return out
 
 
-MASTER_W, MASTER_H = 4352, 3072
-
-
-def make_master(seed):
-    """(scene, blurred scene), each (3, MASTER_H, MASTER_W) float32 on cuda."""
-    from lapstack_amd import synth
-
-    sc = synth.scene(MASTER_H, MASTER_W, seed, "cuda")
-    return sc, synth._Blurrer(sc).blur(3.0)
-
-
-def make_stack(master, w, h, n, seed):
-    """n frames (h, w, 3) torch.uint16 on cuda: the two-level bracket above."""
+def make_stack(w, h, n, seed):
+    """n frames (h, w, 3) torch.uint16 on cuda: synth.Bracket(w, h, n, seed)."""
     import torch
 
     from lapstack_amd import synth
 
-    rng = np.random.default_rng(seed)
-    ox, oy = int(rng.integers(0, MASTER_W - w + 1)), int(rng.integers(0, MASTER_H - h + 1))
-    sc = master[0][:, oy:oy + h, ox:ox + w]
-    bl = master[1][:, oy:oy + h, ox:ox + w]
-    depth = synth.depth_map(h, w, seed + 1, "cuda")
-    frames = []
-    for i in range(n):
-        fi = i / (n - 1) if n > 1 else 0.0
-        t = (2.0 * (depth - fi).abs()).clamp(max=1.0)[None]
-        g = synth._gen(seed + 100 + i, "cuda")
-        img = sc + (bl - sc) * t + synth.NOISE_SIGMA * torch.randn(sc.shape, generator=g, device="cuda")
-        q = torch.round(img.clamp(0.0, 1.0) * 65535.0).to(torch.int32)
-        frames.append(q.permute(1, 2, 0).contiguous().to(torch.uint16))
+    b = synth.Bracket(w, h, n, seed=seed, jitter=False, device="cuda")
+    frames = [b.frame(i) for i in range(n)]
+    del b
+    torch.cuda.empty_cache()
     return frames
 
 
@@ -102,15 +79,12 @@ def main():
                     help="one batched launch with this configuration, no loop and no tuning (for profiling)")
     args = ap.parse_args()
     shapes = plan(args.stacks, args.seed)
-    assert all(w <= MASTER_W and h <= MASTER_H for w, h, _, _ in shapes)
     t0 = time.perf_counter()
-    master = make_master(args.seed)
     stacks = []
     for k, (w, h, n, _) in enumerate(shapes):
-        stacks.append(make_stack(master, w, h, n, args.seed * 1000 + k))
-        if (k + 1) % 16 == 0:
+        stacks.append(make_stack(w, h, n, args.seed + 1000000 * k))
+        if (k + 1) % 8 == 0:
             print(f"generated {k + 1}/{len(shapes)} stacks ({time.perf_counter() - t0:.1f}s)", flush=True)
-    del master
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     mpf = sum(w * h * n for w, h, n, _ in shapes) / 1e6
@@ -188,8 +162,8 @@ def main():
     same = sum(a == b == c for (_, a), b, c in zip(want, dig, dig_t))
     line = {
         "metric": "aggregate megapixel-frames/s, 64 uneven stacks, fuse path",
-        "workload": f"{len(shapes)} stacks, 4-40 frames, 2-12 MP (3:2 / 4:3), RGB u16 resident in HBM; "
-                    f"levels floor(log2(min side)) - 4",
+        "workload": f"{len(shapes)} stacks, 4-40 frames, 2-12 MP (3:2 / 4:3), RGB u16 resident in HBM, each a "
+                    f"seeded synth.Bracket; levels floor(log2(min side)) - 4",
         "mpix_frames": round(mpf, 1),
         "batch_mpix_frames_per_s": round(mpf / t_batch, 1),
         "batch_device_mpix_frames_per_s": round(mpf / (rep.ms / 1e3), 1),

@@ -2,16 +2,18 @@
 
 A stack is made from a scene and a depth map:
 
-* scene: 1/f band-limited noise (white Gaussian noise filtered by 1/|f| for
-  1/min(w,h) <= |f| <= 0.25 cycles/px, normalised to max |v| = 1, times 0.3,
-  plus 0.5), twelve random step edges (half-planes, +-0.1), RGB gains
+* scene: 1/f band-limited noise (white Gaussian noise on the power-of-two
+  canvas that holds w x h, filtered by 1/|f| for 1/min(w,h) <= |f| <= 0.25
+  cycles/px, cropped to w x h, normalised to max |v| = 1, times 0.3, plus
+  0.5), twelve random step edges (half-planes, +-0.1), RGB gains
   (1.0, 0.9, 0.8) and a +-0.02 smooth colour field per channel, clipped to [0, 1];
 * depth: a tilted plane plus six Gaussians (centres uniform, sigma 5-25 % of
   the short side, amplitudes +-[0.2, 0.6]), rescaled to [0, 1];
 * frame i of n focuses at f_i = i/(n-1): every pixel is blurred by a Gaussian
   of sigma = 6 px * |depth - f_i|.  The per-pixel blur is drawn from a stack of
-  the scene blurred at sigma = 0, 0.5, ..., 6.5 px, linearly between the two
-  nearest sigmas;
+  the scene blurred at sigma = 0, 0.5, ..., 6.5 px (by its spectrum: the scene
+  reflect-padded by 3 sigma_max, then edge-extended to power-of-two sizes),
+  linearly between the two nearest sigmas;
 * jitter (optional): frame i is resampled (bicubic) through the similarity
   A_i(p) = c + s_i R(theta_i) (p - c) + t_i about the centre c, with
   s_i = 1 + 0.002 i, t_i uniform in +-3 px per axis and theta_i uniform in
@@ -23,7 +25,9 @@ A stack is made from a scene and a depth map:
 
 Seeds: scene s0, depth s0+1, noise s0+100+i, jitter s0+1000+i.  The generator
 runs in torch on whatever device it is given; frames come back as device
-tensors of torch.uint16.
+tensors of torch.uint16.  Every FFT is of power-of-two sizes, so stacks of many
+different frame sizes share a few transform plans (on the GPU a plan of an
+unusual size is compiled at run time, seconds each).
 """
 from __future__ import annotations
 
@@ -43,16 +47,21 @@ def _gen(seed, device):
     return g
 
 
+def _pow2(n):
+    return 1 << max(0, (int(n) - 1).bit_length())
+
+
 def _one_over_f(h, w, seed, device):
     g = _gen(seed, device)
-    white = torch.randn(h, w, generator=g, device=device)
+    ch, cw = _pow2(h), _pow2(w)
+    white = torch.randn(ch, cw, generator=g, device=device)
     spec = torch.fft.rfft2(white)
-    fy = torch.fft.fftfreq(h, device=device)[:, None]
-    fx = torch.fft.rfftfreq(w, device=device)[None, :]
+    fy = torch.fft.fftfreq(ch, device=device)[:, None]
+    fx = torch.fft.rfftfreq(cw, device=device)[None, :]
     f = torch.sqrt(fy * fy + fx * fx)
     fmin = 1.0 / min(w, h)
     filt = torch.where((f >= fmin) & (f <= 0.25), 1.0 / f.clamp_min(1e-12), torch.zeros_like(f))
-    v = torch.fft.irfft2(spec * filt, s=(h, w))
+    v = torch.fft.irfft2(spec * filt, s=(ch, cw))[:h, :w]
     return v / v.abs().max().clamp_min(1e-12)
 
 
@@ -95,15 +104,19 @@ def depth_map(h, w, seed, device="cpu"):
 
 
 class _Blurrer:
-    """Gaussian blurs of one image by its spectrum (reflect-padded by 3 sigma_max)."""
+    """Gaussian blurs of one image by its spectrum: reflect-padded by 3 sigma_max,
+    then edge-extended on the bottom and right to power-of-two sizes (the
+    circular wrap of a blur of at most 3 sigma_max reaches only the padding)."""
 
     def __init__(self, img):
         self.pad = int(math.ceil(3 * SIGMA_MAX)) + 1
         c, h, w = img.shape
         self.h, self.w = h, w
         mode = "reflect" if self.pad < min(h, w) else "replicate"
-        x = F.pad(img[None], (self.pad,) * 4, mode=mode)[0]
-        self.ph, self.pw = x.shape[1], x.shape[2]
+        x = F.pad(img[None], (self.pad,) * 4, mode=mode)
+        ph, pw = _pow2(x.shape[2]), _pow2(x.shape[3])
+        x = F.pad(x, (0, pw - x.shape[3], 0, ph - x.shape[2]), mode="replicate")[0]
+        self.ph, self.pw = ph, pw
         self.spec = torch.fft.rfft2(x)
         fy = torch.fft.fftfreq(self.ph, device=img.device)[:, None]
         fx = torch.fft.rfftfreq(self.pw, device=img.device)[None, :]
