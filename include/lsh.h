@@ -302,6 +302,29 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
                             lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
                             float *conf, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info);
 
+/* depth.rs:1120 the slice iterator (Iterator<Item = Result<Vec<f32>, String>>)
+ * as a pull callback, called for m = 0, 1, ... n-1 in order.  It either sets
+ * *slice to slice m (*cells floats in *mem) and returns NULL; or leaves *slice
+ * NULL and returns NULL to say the source has ended (the pass then fails with
+ * "depth: slice m of n missing", depth.rs:1138); or returns an error message,
+ * which becomes the pass's error unchanged (the iterator's Err; the library
+ * copies it as the callback returns).  A slice has been read before the next
+ * call, so one buffer may serve every slice. */
+typedef const char *(*lsh_slice_fn)(void *user, uint32_t m, const float **slice, uint64_t *cells, lsh_mem *mem);
+
+/* depth.rs:1124 log: one progress line per call, NUL-terminated, valid during the call */
+typedef void (*lsh_log_fn)(void *user, const char *line);
+
+/* depth.rs:1119-1200 depth_from_slices with its slice iterator and log: as
+ * lsh_depth_from_slices, the slices pulled one at a time from next (a slice of
+ * the wrong cell count fails with "depth: slice m has c cells, the working grid
+ * dwxdh", depth.rs:1139-1141), and the reference's progress lines (guide and
+ * grid, slices aggregated, peaks and confidence, robust reweighting, WLS,
+ * upsampling) passed to log when it is not NULL. */
+char *lsh_depth_from_source(lsh_slice_fn next, void *next_user, uint32_t n, const float *fused3, lsh_mem fused_mem,
+                            uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth, float *conf,
+                            lsh_mem out_mem, float *floor_plane, lsh_depth_info *info, lsh_log_fn log, void *log_user);
+
 /* depth.rs:1125-1159, the depth pass up to the WLS solve: the working grid's
  * guide luma, the peak depth (after the median when p->median) and the
  * p90-normalised confidence, each dw*dh into out_mem; info gets dw, dh, p90

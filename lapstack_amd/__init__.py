@@ -10,6 +10,7 @@ lapstack_amd/liblsh.so (declared in include/lsh.h):
 * ``HipFrames``            -- gpu.rs:636-900 GpuFrames: process (register, warp,
                               brightness, focus slice), ingest (registered stacks).
 * ``depth_from_slices``    -- depth.rs:1119-1200.
+* ``depth_from_source``    -- the same with its slice iterator and log callback.
 * ``depth_grid``           -- depth.rs:1125-1159: the WLS system's inputs.
 * ``WarmNet``              -- the learned warm start of the WLS solve (no
                               reference counterpart), trained on the device.
@@ -38,7 +39,7 @@ from . import _lib
 from ._lib import LSH_DEVICE, LSH_HOST, HipError
 
 __all__ = ["available", "Backend", "FuseParams", "DepthParams", "AlignParams", "Fuser", "HipFrames", "DepthMap",
-           "depth_from_slices", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
+           "depth_from_slices", "depth_from_source", "depth_grid", "stack_stream", "batch_fuse", "WavParams", "WeightedAverage",
            "weighted_average", "slab_ranges", "fuse_slabs",
            "common_area", "crop", "resize", "restretch", "to_u16", "gauss_levels", "wls_solve", "wls_solve_noalloc", "image_info", "load_rgb", "save_rgb", "save_rgb_u16", "save_gray16", "save_gray", "stack_stream_files", "WarmNet", "WLS_MODES", "fuse_and_depth", "Interp", "TopRule", "HipError", "SIM_ID"]
 
@@ -373,6 +374,79 @@ def depth_from_slices(slices, n: int, fused, params: Optional[DepthParams] = Non
     _lib.check(lib.lsh_depth_from_slices(sp, smem, n, fp, fmem, w, h, C.byref(p._c()), dp, cp, dmem, flp,
                                          C.byref(info)))
     del sk, fk, dk, ck
+    d = {k: getattr(info, k) for k, _ in info._fields_}
+    return DepthMap(depth_out, conf_out, w, h, dw, dh, floor_out, d)
+
+
+def depth_from_source(slices, n: int, fused, params: Optional[DepthParams] = None, log=None, depth_out=None,
+                      conf_out=None, floor_out=None) -> DepthMap:
+    """depth.rs:1119-1200 depth_from_slices with its slice iterator and log
+    (include/lsh.h lsh_depth_from_source).  ``slices``: any iterable of the n
+    focus slices in frame order, each dw*dh float32 cells (numpy, host or
+    device tensor); an item that is an Exception is the iterator's Err and
+    fails the pass with its message, as does an exception the iterator raises.
+    ``log``: a callable taking each progress line, or None.  A device slice is
+    read before the next one is asked for, so one buffer may serve them all."""
+    lib = _lib.load()
+    p = params or DepthParams()
+    _, h, w = tuple(fused.shape)
+    dw, dh = p.grid(w, h)
+    it = iter(slices)
+    held = {}
+    failed = []
+
+    def next_slice(_user, m, slice_p, cells_p, mem_p):
+        try:
+            try:
+                s = next(it)
+            except StopIteration:
+                slice_p[0] = None
+                return None
+            if isinstance(s, BaseException):
+                raise s
+            ptr, mem, keep = _buf(s, np.float32)
+            if mem == LSH_DEVICE:
+                import torch
+
+                torch.cuda.current_stream().synchronize()  # the slice's producer has finished
+            held["slice"] = keep
+            slice_p[0] = ptr.value if isinstance(ptr, C.c_void_p) else ptr
+            cells_p[0] = int(np.prod(keep.shape)) if hasattr(keep, "shape") else 0
+            mem_p[0] = mem
+            return None
+        except BaseException as e:  # noqa: BLE001 -- handed to the library as the source's error
+            failed.append(e)
+            held["msg"] = C.create_string_buffer(str(e).encode() or type(e).__name__.encode())
+            return C.addressof(held["msg"])
+
+    logged = []
+
+    def log_line(_user, line):
+        try:
+            log(line.decode())
+        except BaseException as e:  # noqa: BLE001 -- raised once the pass has returned
+            logged.append(e)
+
+    nfn = _lib.SLICE_FN(next_slice)
+    lfn = _lib.LOG_FN(log_line) if log is not None else _lib.LOG_FN()
+    fp, fmem, fk = _buf(fused, np.float32)
+    if depth_out is None:
+        depth_out = np.zeros((h, w), np.float32)
+        conf_out = np.zeros((h, w), np.float32)
+        floor_out = np.zeros((dh, dw), np.float32)
+    dp, dmem, dk = _buf(depth_out, np.float32)
+    cp, cmem, ck = _buf(conf_out, np.float32)
+    flp = _buf(floor_out, np.float32)[0] if floor_out is not None else None
+    info = _lib.DepthInfoC()
+    err = lib.lsh_depth_from_source(nfn, None, n, fp, fmem, w, h, C.byref(p._c()), dp, cp, dmem, flp, C.byref(info),
+                                    lfn, None)
+    held.clear()
+    if err and failed:
+        raise HipError(_lib.take_err(err)) from failed[0]
+    _lib.check(err)
+    if logged:
+        raise logged[0]
+    del fk, dk, ck
     d = {k: getattr(info, k) for k, _ in info._fields_}
     return DepthMap(depth_out, conf_out, w, h, dw, dh, floor_out, d)
 

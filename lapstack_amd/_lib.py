@@ -37,6 +37,12 @@ class DepthInfoC(C.Structure):
                 ("true_rel2", C.c_double)]
 
 
+# include/lsh.h lsh_slice_fn / lsh_log_fn (the error return is a char pointer the
+# callback keeps alive; c_void_p, since ctypes cannot return c_char_p from a callback safely)
+SLICE_FN = C.CFUNCTYPE(_p, _p, _u32, C.POINTER(_p), C.POINTER(C.c_uint64), C.POINTER(C.c_int))
+LOG_FN = C.CFUNCTYPE(None, _p, C.c_char_p)
+
+
 class WlsReportC(C.Structure):
     _fields_ = [("iters", _u32), ("outer", _u32), ("rel", C.c_double), ("true_rel", C.c_double),
                 ("err_fp64", C.c_double), ("ms", C.c_double)]
@@ -91,6 +97,8 @@ SIGNATURES = [
     ("lsh_fuser_free", None, [_p]),
     ("lsh_depth_from_slices", _p, [_p, C.c_int, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p,
                                    C.c_int, _p, C.POINTER(DepthInfoC)]),
+    ("lsh_depth_from_source", _p, [SLICE_FN, _p, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p,
+                                   C.c_int, _p, C.POINTER(DepthInfoC), LOG_FN, _p]),
     ("lsh_depth_grid", _p, [_p, C.c_int, _u32, _p, C.c_int, _u32, _u32, C.POINTER(DepthParamsC), _p, _p, _p, C.c_int,
                             C.POINTER(DepthInfoC)]),
     ("lsh_warm_new", _p, [C.c_uint64, C.POINTER(_p)]),

@@ -9,6 +9,12 @@
 // are exact, which they are for these planes), then round to f32 and scale by
 // the f32 reciprocal count, as the CPU path does.
 #include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
 
 #include "api.hpp"
 #include "common.hpp"
@@ -733,6 +739,160 @@ struct PeakBufs {
     Peaks view() { return Peaks{c1.p, l1.p, r1.p, c2.p, prev.p, prev2.p, sum.p, cmin.p, i1.p}; }
 };
 
+// ------------------------------------------------------ slices in, lines out
+// The depth pass's slice iterator (depth.rs:1120, 1137-1142), pulled in frame
+// order: one contiguous array, or the caller's callback.
+struct SliceSource {
+    virtual ~SliceSource() = default;
+    // slice m, its cell count and memory; nullptr once the source has ended; throws the source's error
+    virtual const float *get(uint32_t m, uint64_t &cells, lsh_mem &mem) = 0;
+    // the pass has read slice m
+    virtual void done(uint32_t, lsh_mem) {}
+};
+
+struct ArraySource final : SliceSource {
+    const float *base;
+    lsh_mem mem;
+    size_t M;
+    ArraySource(const float *b, lsh_mem m, size_t cells) : base(b), mem(m), M(cells) {}
+    const float *get(uint32_t m, uint64_t &cells, lsh_mem &mm) override {
+        cells = M;
+        mm = mem;
+        return base + (size_t)m * M;
+    }
+};
+
+struct CallbackSource final : SliceSource {
+    lsh_slice_fn fn;
+    void *user;
+    CallbackSource(lsh_slice_fn f, void *u) : fn(f), user(u) {}
+    const float *get(uint32_t m, uint64_t &cells, lsh_mem &mm) override {
+        const float *s = nullptr;
+        cells = 0;
+        mm = LSH_HOST;
+        const char *e = fn(user, m, &s, &cells, &mm);
+        if (e) throw Error(e);  // the source's own message, as the reference's `??` passes it on
+        LSH_REQUIRE(mm == LSH_HOST || mm == LSH_DEVICE, "depth: a slice's memory is LSH_HOST or LSH_DEVICE");
+        return s;
+    }
+    // the caller may reuse a device buffer once its slice has been read (a
+    // host slice was copied, synchronously, before its kernels)
+    void done(uint32_t, lsh_mem mm) override {
+        if (mm == LSH_DEVICE) sync();
+    }
+};
+
+// Rust's Display of an f32 (the shortest decimal that reads back as the same
+// float, never in exponent form), for log lines worded as the reference's.
+static std::string rs_f32(float v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    if (v == 0.0f) return std::signbit(v) ? "-0" : "0";
+    char buf[64];
+    int p = 1;
+    for (; p < 9; p++) {
+        snprintf(buf, sizeof buf, "%.*e", p - 1, (double)v);
+        if (strtof(buf, nullptr) == v) break;
+    }
+    snprintf(buf, sizeof buf, "%.*e", p - 1, (double)v);
+    std::string s(buf);
+    const bool neg = s[0] == '-';
+    if (neg) s.erase(0, 1);
+    const size_t ep = s.find('e');
+    const int ex = atoi(s.c_str() + ep + 1);
+    std::string dig;
+    for (size_t i = 0; i < ep; i++)
+        if (s[i] >= '0' && s[i] <= '9') dig += s[i];
+    while (dig.size() > 1 && dig.back() == '0') dig.pop_back();
+    const int point = ex + 1;  // digits before the decimal point
+    std::string out;
+    if (point <= 0)
+        out = "0." + std::string((size_t)-point, '0') + dig;
+    else if ((size_t)point >= dig.size())
+        out = dig + std::string((size_t)point - dig.size(), '0');
+    else
+        out = dig.substr(0, (size_t)point) + "." + dig.substr((size_t)point);
+    return neg ? "-" + out : out;
+}
+
+// Rust's Debug of an f32 inside a struct: Display, with ".0" on whole numbers;
+// below 1e-4 or from 1e16 on, the shortest digits in exponent form ("1e-5")
+static std::string rs_f32_dbg(float v) {
+    const float a = std::fabs(v);
+    if (std::isfinite(v) && a != 0.0f && (a < 1e-4f || a >= 1e16f)) {
+        char buf[64];
+        int p = 1;
+        for (; p < 9; p++) {
+            snprintf(buf, sizeof buf, "%.*e", p - 1, (double)v);
+            if (strtof(buf, nullptr) == v) break;
+        }
+        std::string s(buf);
+        const size_t ep = s.find('e');
+        std::string m = s.substr(0, ep);
+        if (m.find('.') != std::string::npos) {
+            while (m.back() == '0') m.pop_back();
+            if (m.back() == '.') m.pop_back();
+        }
+        return m + "e" + std::to_string(atoi(s.c_str() + ep + 1));
+    }
+    std::string s = rs_f32(v);
+    if (std::isfinite(v) && s.find('.') == std::string::npos) s += ".0";
+    return s;
+}
+
+// Rust's {:.1e}: one decimal, the exponent unpadded ("1.2e-6")
+static std::string rs_e1(double v) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.1e", v);
+    std::string s(buf);
+    const size_t ep = s.find('e');
+    if (ep == std::string::npos) return s;
+    return s.substr(0, ep) + "e" + std::to_string(atoi(s.c_str() + ep + 1));
+}
+
+// the depth pass's progress lines (depth.rs:1124 log), timed from its start
+struct DepthLog {
+    lsh_log_fn fn = nullptr;
+    void *user = nullptr;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    bool on() const { return fn != nullptr; }
+    double secs() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    __attribute__((format(printf, 2, 3))) void line(const char *fmt, ...) const {
+        if (!fn) return;
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        fn(user, buf);
+    }
+};
+
+// per-block f64 sums of a plane, in a fixed order (the log's mean confidence)
+__global__ void k_sum_part(const float *__restrict__ v, size_t n, double *__restrict__ part) {
+    __shared__ double s[256];
+    double a = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) a += v[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+static double plane_mean(const float *v, size_t n) {
+    const unsigned nb = (unsigned)std::min<size_t>(512, std::max<size_t>(1, cdiv(n, 256)));
+    DBuf<double> part(nb);
+    launch_b("sum_part", 4.0 * n, k_sum_part, dim3(nb), dim3(256), 0, v, n, part.p);
+    std::vector<double> h(nb);
+    copy_mem(h.data(), LSH_HOST, part.p, LSH_DEVICE, nb * sizeof(double));
+    double s = 0.0;
+    for (double x : h) s += x;
+    return s / (double)n;
+}
+
 }  // namespace lsh
 
 using namespace lsh;
@@ -746,8 +906,8 @@ struct DepthGrid {
     float p90 = 0.0f, floor_ = 0.0f;
 };
 
-static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
-                       lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p) {
+static void depth_grid(DepthGrid &G, SliceSource &src, uint32_t n, const float *fused3, lsh_mem fused_mem, uint32_t w,
+                       uint32_t h, const lsh_depth_params *p, const DepthLog &log) {
     LSH_REQUIRE(n > 0, "depth: no slices");
     LSH_REQUIRE(w > 0 && h > 0, "depth: empty frame");
     LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
@@ -780,11 +940,27 @@ static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, ui
     launch_b("peak_init", 36.0 * M, k_peak_init, dim3(g1(M)), dim3(256), 0, P, M);
     GF gf;
     if (p->agg_radius > 0) gf_init(gf, guide.p, dw, dh, (int)p->agg_radius, p->agg_eps);
+    if (log.on()) {
+        char focus[64];
+        if (p->focus_kind == 0)
+            snprintf(focus, sizeof focus, "Rdf { r_in: %u, r_out: %u }", p->r_in, p->r_out);
+        else
+            snprintf(focus, sizeof focus, "Sml { step: %u }", p->sml_step);
+        sync();
+        log.line("depth: %u frames, focus %s, working grid %dx%d (1/%d), aggregation r=%u eps=%s (GPU, %.2fs to start)", n,
+                 focus, dw, dh, k, p->agg_radius, rs_f32(p->agg_eps).c_str(), log.secs());
+    }
     DBuf<float> stage;
-    if (slices_mem == LSH_HOST) stage.alloc(M);
     for (uint32_t m = 0; m < n; m++) {
-        const float *c = slices + (size_t)m * M;
-        if (slices_mem == LSH_HOST) {
+        uint64_t cells = 0;
+        lsh_mem mem = LSH_HOST;
+        const float *c = src.get(m, cells, mem);
+        if (!c) throw Error("depth: slice " + std::to_string(m) + " of " + std::to_string(n) + " missing");
+        if (cells != M)
+            throw Error("depth: slice " + std::to_string(m) + " has " + std::to_string(cells) + " cells, the working grid " +
+                        std::to_string(dw) + "x" + std::to_string(dh));
+        if (mem == LSH_HOST) {
+            if (!stage.p) stage.alloc(M);
             copy_mem(stage.p, LSH_DEVICE, c, LSH_HOST, M * sizeof(float));
             c = stage.p;
         }
@@ -794,6 +970,11 @@ static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, ui
         } else {
             launch_b("peak_push", 76.0 * M, k_peak_push_raw, dim3(g1(M)), dim3(256), 0, c, M, P, m);
         }
+        src.done(m, mem);
+    }
+    if (log.on()) {
+        sync();
+        log.line("depth: %u slices aggregated  (%.1fs)", n, log.secs());
     }
     // close the profiles, the noise floor (depth.rs:568-580)
     launch_b("peak_close", 72.0 * M, k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
@@ -816,20 +997,21 @@ static void depth_grid(DepthGrid &G, const float *slices, lsh_mem slices_mem, ui
     size_t kk = std::min(ns * 9 / 10, ns - 1);
     float p90 = G.p90 = select_kth(conf.p, M, 7, kk, hist);
     if (p90 > 1e-6f) launch_b("conf_norm", 8.0 * M, k_conf_norm, dim3(g1(M)), dim3(256), 0, conf.p, M, p90);
+    if (log.on())
+        log.line("depth: peaks found, confidence p90 %.3f, mean (normalized) %.3f  (%.1fs)", p90, plane_mean(conf.p, M),
+                 log.secs());
 }
 
-extern "C" {
-
-char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
-                            lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
-                            float *conf_out, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info) {
-    LSH_API_BEGIN
-    LSH_REQUIRE(slices && fused3 && p && depth && conf_out, "lsh_depth_from_slices: null argument");
+// depth.rs:1119-1200 over any slice source: the grid pass, the WLS solves, the
+// upsampling, with the reference's progress lines when log is on
+static void depth_run(SliceSource &src, uint32_t n, const float *fused3, lsh_mem fused_mem, uint32_t w, uint32_t h,
+                      const lsh_depth_params *p, float *depth, float *conf_out, lsh_mem out_mem, float *floor_plane,
+                      lsh_depth_info *info, const DepthLog &log) {
     LSH_REQUIRE(n > 0, "depth: no slices");
     LSH_REQUIRE(w > 0 && h > 0, "depth: empty frame");
     LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
     DepthGrid G;
-    depth_grid(G, slices, slices_mem, n, fused3, fused_mem, w, h, p);
+    depth_grid(G, src, n, fused3, fused_mem, w, h, p, log);
     const int k = G.k, dw = G.dw, dh = G.dh;
     const size_t N = (size_t)w * h, M = (size_t)dw * dh;
     DBuf<float> &yfull = G.yfull, &guide = G.guide, &depth_w = G.depth_w, &conf = G.conf, &mins = G.mins;
@@ -853,6 +1035,8 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         rel1 = rel2 = (float)r1.rel;
         trel1 = trel2 = wls_true_rel(s.get(), depth_w.p, u.p);
         if (p->robust > 0.0f) {
+            log.line("depth: robust reweighting (tau=%s frames), first solve %u iterations, residual %s",
+                     rs_f32(p->robust).c_str(), it1, rs_e1(rel1).c_str());
             wls_set_weights(s.get(), conf.p, depth_w.p, u.p, p->robust);
             WlsResult r2 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, nullptr, u.p, nullptr);
             it2 = r2.iters;
@@ -861,6 +1045,8 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         }
         dres = u.p;
     }
+    log.line("depth: WLS lambda=%s sigma_c=%s solved in %u iterations, residual %s  (%.1fs)", rs_f32(p->lambda).c_str(),
+             rs_f32(p->sigma_c).c_str(), it2, rs_e1(rel2).c_str(), log.secs());
     // upsampling (depth.rs:1180-1197)
     DBuf<float> dout, cout;
     float *dd = depth, *cc = conf_out;
@@ -899,6 +1085,12 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
     }
     if (floor_plane) copy_mem(floor_plane, out_mem, mins.p, LSH_DEVICE, M * sizeof(float));
     sync();
+    if (log.on()) {
+        std::string up = "Bilinear";
+        if (p->upsample_kind != 0)
+            up = "Guided { radius: " + std::to_string(p->up_radius) + ", eps: " + rs_f32_dbg(p->up_eps) + " }";
+        log.line("depth: upsampled to %ux%u (%s)  (%.1fs)", w, h, up.c_str(), log.secs());
+    }
     if (info) {
         info->dw = (uint32_t)dw;
         info->dh = (uint32_t)dh;
@@ -911,6 +1103,32 @@ char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n,
         info->true_rel1 = trel1;
         info->true_rel2 = trel2;
     }
+}
+
+extern "C" {
+
+char *lsh_depth_from_slices(const float *slices, lsh_mem slices_mem, uint32_t n, const float *fused3,
+                            lsh_mem fused_mem, uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth,
+                            float *conf_out, lsh_mem out_mem, float *floor_plane, lsh_depth_info *info) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(slices && fused3 && p && depth && conf_out, "lsh_depth_from_slices: null argument");
+    LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
+    const size_t k = (size_t)1 << p->scale;
+    ArraySource src(slices, slices_mem, ((w + k - 1) / k) * ((h + k - 1) / k));
+    depth_run(src, n, fused3, fused_mem, w, h, p, depth, conf_out, out_mem, floor_plane, info, DepthLog{});
+    LSH_API_END
+}
+
+char *lsh_depth_from_source(lsh_slice_fn next, void *next_user, uint32_t n, const float *fused3, lsh_mem fused_mem,
+                            uint32_t w, uint32_t h, const lsh_depth_params *p, float *depth, float *conf,
+                            lsh_mem out_mem, float *floor_plane, lsh_depth_info *info, lsh_log_fn log, void *log_user) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(next && fused3 && p && depth && conf, "lsh_depth_from_source: null argument");
+    CallbackSource src(next, next_user);
+    DepthLog lg;
+    lg.fn = log;
+    lg.user = log_user;
+    depth_run(src, n, fused3, fused_mem, w, h, p, depth, conf, out_mem, floor_plane, info, lg);
     LSH_API_END
 }
 
@@ -919,8 +1137,11 @@ char *lsh_depth_grid(const float *slices, lsh_mem slices_mem, uint32_t n, const 
                      lsh_mem out_mem, lsh_depth_info *info) {
     LSH_API_BEGIN
     LSH_REQUIRE(slices && fused3 && p && guide && depth && conf, "lsh_depth_grid: null argument");
+    LSH_REQUIRE(p->scale <= 8, "depth: scale out of range");
+    const size_t k = (size_t)1 << p->scale;
+    ArraySource src(slices, slices_mem, ((w + k - 1) / k) * ((h + k - 1) / k));
     DepthGrid G;
-    depth_grid(G, slices, slices_mem, n, fused3, fused_mem, w, h, p);
+    depth_grid(G, src, n, fused3, fused_mem, w, h, p, DepthLog{});
     const size_t M = (size_t)G.dw * G.dh;
     copy_mem(guide, out_mem, G.guide.p, LSH_DEVICE, M * sizeof(float));
     copy_mem(depth, out_mem, G.depth_w.p, LSH_DEVICE, M * sizeof(float));

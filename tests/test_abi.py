@@ -49,3 +49,6 @@ def test_errors_come_back_as_strings():
     assert msg and "empty" in msg
     msg = _lib.take_err(lib.lsh_fuser_push(None, None, 0))
     assert msg and "null" in msg
+    msg = _lib.take_err(lib.lsh_depth_from_source(_lib.SLICE_FN(), None, 2, None, 0, 8, 8, None, None, None, 0, None,
+                                                  None, _lib.LOG_FN(), None))
+    assert msg and "null" in msg
