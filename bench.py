@@ -418,8 +418,8 @@ def main():
     log(f"weighted average: {wav_ms:.1f} ms for {len(frames)} frames")
 
     # per-kernel timing pass (HIP events on the library's stream), off the timed region; the host-steered
-    # path runs the same kernels and reads back each search's passes per level, which gives the search's
-    # algorithmic bytes (a pass reads the level's reference and target once)
+    # path runs the same kernels and reads back each search's counted evaluations per level, which gives the
+    # search's algorithmic bytes (an evaluation at level L reads the level's reference and target once: 8 N_L)
     _lib.prof_reset()
     _lib.prof_enable(True)
     job.run(fp, apm, dp, nframes=args.prof_frames or None, path="host")

@@ -211,23 +211,23 @@ __device__ void grid_sum(double (&v)[NV], double *__restrict__ part, unsigned *_
 // ------------------------------------------------------- the search on device
 // multiscale_align (align.rs:896-942) as one cooperative kernel: every block
 // runs the same bounded Nelder-Mead (align.rs:796-883) on the same numbers.
-// Each cost evaluation is a grid-strided warp-and-reduce; once the arrival
-// tree says every block's partial sums are written, every block folds them in
+// Each cost evaluation is a grid-strided warp-and-reduce; once every block's
+// arrival flag says its partial sums are written, every block folds them in
 // the same fixed order, so all blocks hold the identical value and take the
 // identical next step, with no host round trip and no broadcast.
 #define NM_MAXL 16
-// the search's flag words, each on its own 128-byte line: arrivals of groups,
-// the error flag, the join count, the hand-off generation, one arrival counter
-// per group, then the closed participant count
-#define NM_GROUPS 16
+#define NM_MAXG 256      // participating blocks at most (one arrival flag each; one per CU)
+#define NM_GRID_STEP 16  // the grid is a multiple of this many blocks
+// the search's flag words: the error flag, the join count, the hand-off flag
+// and the closed participant count each on its own 128-byte line, then one
+// arrival flag per participant (the number of passes it has completed)
 #define NM_LINE 32
-#define NM_FL_TOP 0
-#define NM_FL_ERR (1 * NM_LINE)
-#define NM_FL_JOIN (2 * NM_LINE)
-#define NM_FL_HAND (3 * NM_LINE)
-#define NM_FL_GROUP (4 * NM_LINE)
-#define NM_FL_CLOSED ((4 + NM_GROUPS) * NM_LINE)
-#define NM_FL_WORDS ((5 + NM_GROUPS) * NM_LINE)
+#define NM_FL_ERR (0 * NM_LINE)
+#define NM_FL_JOIN (1 * NM_LINE)
+#define NM_FL_HAND (2 * NM_LINE)
+#define NM_FL_CLOSED (3 * NM_LINE)
+#define NM_FL_ARR (4 * NM_LINE)
+#define NM_FL_WORDS (4 * NM_LINE + NM_MAXG)
 struct NmConst {
     const float *ref[NM_MAXL];
     const float *tgt[NM_MAXL];
@@ -243,6 +243,7 @@ struct NmConst {
     // the whole grid has joined, or when none has joined for join_ticks
     uint64_t wait_ticks, join_ticks;
     int max_part;                    // at most this many blocks take part (0: the grid)
+    int phases;                      // time the passes' phases (the clock reads cost a pass about 1 us)
     double *trace;                   // non-null: the counted costs, in the reference's order
     int trace_cap;
     unsigned long long *digest;      // per participant: a digest of every cost it folded
@@ -296,8 +297,10 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_MAXP 9     // cost evaluations per collective pass
 #define NM_TILE 64    // a pass walks 64-pixel-wide destination tiles of 16, 32 or 64 rows
 #define NM_BOX 8192   // floats of a tile's staged source window
-#define NM_HELD 2     // tiles a block holds for a whole level (levels of tiles of at most 32 rows)
-#define NM_HSLOT 8192 // floats per held tile: its reference pixels, then its source window
+#ifndef NM_HB
+#define NM_HB 24576   // floats a block holds a level's tiles in (their windows, and their reference pixels where they fit)
+#endif
+#define NM_HMAXT 8    // tiles a block holds at most
 #define NM_HMARGIN 6  // pixels a held window extends past the first pass's bound
 // the state words: the Sim, the evaluation total, per level the evaluations and
 // block 0's ticks, the four phase tick totals, per level the passes, the
@@ -305,7 +308,9 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_ST_PART (13 + 3 * NM_MAXL)
 #define NM_ST_TRACE (14 + 3 * NM_MAXL)
 #define NM_ST_HAND (15 + 3 * NM_MAXL)  // block 0's Sim after the solo levels, handed to the grid
-#define NM_STATE (23 + 3 * NM_MAXL)
+#define NM_ST_LPH (23 + 3 * NM_MAXL)   // per level the phase tick totals (NM_NPH of them)
+#define NM_NPH 8
+#define NM_STATE (23 + (3 + NM_NPH) * NM_MAXL)
 
 struct NmShared {
     double S[9][8], FV[9], S2[9][8], FV2[9], XR[8], XE[8], XC[8], C[8], lo[8], hi[8], cur[8], FR;
@@ -313,8 +318,13 @@ struct NmShared {
     double red[3 * NM_MAXP][NM_WAVES], sums[3 * NM_MAXP];
     unsigned lev_evals[NM_MAXL], lev_passes[NM_MAXL];
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
+    // per level: the four phases, then the combine's parts (publishing the
+    // partials, waiting for the other blocks, folding), one spare
+    uint64_t lph[NM_MAXL][NM_NPH];
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
-    int hwin[NM_HELD][5];   // the held tiles' windows (x0, y0, w, h, staged)
+    int hwin[NM_HMAXT][7];  // per tile of a held level: its window (x0, y0, w, h, staged), held, offset in hb
+    int hoff;               // the held tiles' floats so far (the hold pass)
+    int restage;            // some tile of the block is staged every pass (it did not fit)
     int grid;               // the participating blocks (read through nm_G, as a scalar)
     int rank;               // this block's place among them (nm_B)
     double tolv[8];         // the level's convergence tolerances
@@ -356,37 +366,47 @@ __device__ __forceinline__ void zero_waves(NmShared &sh) {
     for (int i = threadIdx.x; i < 3 * NM_MAXP * NM_WAVES; i += NM_THREADS) (&sh.red[0][0])[i] = 0.0;
 }
 
-// The rows of a pass's tiles: 64 where the level has a 64 x 64 tile for every
-// block (or one block does all), fewer on smaller levels so more blocks share
-// them: down to 16 (a row per wave), then to 8 or 4 (waves left idle) while
-// every tile still has a block of its own, since a block's tile costs it
-// about the same with 16 rows as with 4 when its SIMDs take turns.
+// The rows of a pass's tiles (64, 32, 16, 8 or 4; solo: 64): the fewest
+// tiles for the busiest block (each tile costs its staging, its reference
+// loads and its sums over the waves, while more rows per thread overlap),
+// and among those the smaller tiles, which spread the level over more blocks
+// (below 16 rows a tile leaves waves idle, and the block's SIMDs then run one
+// wave each).
 __device__ __forceinline__ int nm_tile_rows(int tw, int th, bool solo, int G) {
+    if (solo) return 64;
     const int ntx = (tw + NM_TILE - 1) / NM_TILE;
-    int TH = 64;
-    if (!solo) {
-        while (TH > 16 && ntx * ((th + TH - 1) / TH) < G) TH >>= 1;
-        while (TH > 4 && ntx * ((th + TH / 2 - 1) / (TH / 2)) <= G) TH >>= 1;
+    int best = 64, fewest = 0x7fffffff;
+    for (int TH = 64; TH >= 4; TH >>= 1) {
+        const int T = (ntx * ((th + TH - 1) / TH) + G - 1) / G;
+        if (T <= fewest) {
+            fewest = T;
+            best = TH;
+        }
     }
-    return TH;
+    return best;
 }
 
-// Whether a grid level's tiles can be held (nm_tiles): they are at most 32
-// rows and no block has more than NM_HELD of them.
-__device__ __forceinline__ bool nm_holdable(int tw, int th, int G) {
+// Whether a grid level's tiles are held (no block has more than NM_HMAXT),
+// and whether their reference pixels are kept with them: when every tile of
+// the busiest block fits with a window of a few pixels' bound.
+__device__ __forceinline__ bool nm_holdable(int tw, int th, int G, bool *refh) {
     const int ntx = (tw + NM_TILE - 1) / NM_TILE, TH = nm_tile_rows(tw, th, false, G);
-    return TH <= 32 && ntx * ((th + TH - 1) / TH) <= NM_HELD * G;
+    const int T = (ntx * ((th + TH - 1) / TH) + G - 1) / G;
+    const int win = (NM_TILE + 2 * NM_HMARGIN + 7) * (TH + 2 * NM_HMARGIN + 7);
+    *refh = T * (NM_TILE * TH + win) <= NM_HB;
+    return T <= NM_HMAXT;
 }
 
 // The pass's tiles, the sums into the waves' accumulators.  MODE 1: a
 // resident solo level.  MODE 0: each tile's source window (the pass's bound)
 // staged per pass, or, on a held level, staged by the level's first pass
 // (hold) with the bound widened by NM_HMARGIN on each side for the moves to
-// come, into the block's slot of hb together with the tile's reference
-// pixels, and read from there by every later pass.
+// come, at the block's next free place in hb (together with the tile's
+// reference pixels when refh), and read from there by every later pass; a
+// tile that no longer fits there is staged by every pass instead.
 template <int MODE>
 __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np,
-                                         bool solo, bool held, bool hold) {
+                                         bool solo, bool held, bool hold, bool refh) {
     const int tw = K.lw[lvl], th = K.lh[lvl];
     const float *rf = K.ref[lvl], *tg = K.tgt[lvl];
     const double xmax = (double)(tw - 1), ymax = (double)(th - 1);
@@ -418,20 +438,37 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 rv[r] = (r < R && i < npx) ? boxs[bw * bh + i] : 0.0f;
             }
         } else {
-            const int q = held ? (t - B) / G : 0;  // a held tile's slot
-            float *hs = hb + q * NM_HSLOT;
+            const int q = held ? (t - B) / G : 0;  // the tile's place among the block's held ones
             int *hw = sh.hwin[q];
-            if (!held || hold) {
+            // a held tile after the level's first pass: its window (and its
+            // reference pixels, refh) where that pass put them in hb
+            const bool kept = held && !hold && __builtin_amdgcn_readfirstlane(hw[5]);
+            if (!kept) {
                 __syncthreads();  // the bound is written; the previous tile's window is no longer read
                 // the window may reach one sample before and two past the level
                 // on each axis (a support's reach), where it holds the edge samples
-                const int m = held ? NM_HMARGIN : 0;
+                int m = hold ? NM_HMARGIN : 0, off = -1;
                 bx0 = max(-1, X0 + sh.box[0] - m);
                 by0 = max(-1, Y0 + sh.box[2] - m);
                 bw = min(tw + 1, X1 + sh.box[1] + m) - bx0 + 1;
                 bh = min(th + 1, Y1 + sh.box[3] + m) - by0 + 1;
-                float *wb = held ? hs + NM_TILE * TH : boxs;
-                staged = sh.box[4] && bw > 0 && bh > 0 && bw * bh <= (held ? NM_HSLOT - NM_TILE * TH : NM_BOX);
+                if (hold) {
+                    // the level's first pass holds the tile at the block's next
+                    // free place in hb when it fits; otherwise the tile is staged
+                    // by every pass, with that pass's bound
+                    const int need = (refh ? NM_TILE * TH : 0) + bw * bh, at = sh.hoff;
+                    if (sh.box[4] && bw > 0 && bh > 0 && at + need <= NM_HB) {
+                        off = at;
+                    } else {
+                        m = 0;
+                        bx0 = max(-1, X0 + sh.box[0]);
+                        by0 = max(-1, Y0 + sh.box[2]);
+                        bw = min(tw + 1, X1 + sh.box[1]) - bx0 + 1;
+                        bh = min(th + 1, Y1 + sh.box[3]) - by0 + 1;
+                    }
+                }
+                float *wb = off >= 0 ? hb + off + (refh ? NM_TILE * TH : 0) : boxs;
+                staged = sh.box[4] && bw > 0 && bh > 0 && (off >= 0 || bw * bh <= NM_BOX);
                 // (issuing all of a thread's window loads before storing any measured
                 // no faster here and spilled registers)
                 if (staged)
@@ -439,21 +476,27 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                         const int r = i / bw, c = i - r * bw;
                         wb[i] = tg[(size_t)min(max(by0 + r, 0), th - 1) * tw + min(max(bx0 + c, 0), tw - 1)];
                     }
-                if (held && threadIdx.x == 0) {
+                win = wb;
+                __syncthreads();  // every thread has read sh.hoff
+                if (hold && threadIdx.x == 0) {
                     hw[0] = bx0;
                     hw[1] = by0;
                     hw[2] = bw;
                     hw[3] = bh;
                     hw[4] = staged;
+                    hw[5] = off >= 0;
+                    hw[6] = off;
+                    if (off >= 0)
+                        sh.hoff = off + (refh ? NM_TILE * TH : 0) + bw * bh;
+                    else
+                        sh.restage = 1;
                 }
-                win = wb;
-                __syncthreads();
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int x = X0 + tx, y = Y0 + ty + 16 * r;
                     rv[r] = (r < R && act && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
                     // kept for the later passes (each thread reads back its own)
-                    if (held && r < R && act) hs[(ty + 16 * r) * NM_TILE + tx] = rv[r];
+                    if (off >= 0 && refh && r < R && act) hb[off + (ty + 16 * r) * NM_TILE + tx] = rv[r];
                 }
             } else {
                 bx0 = hw[0];
@@ -461,9 +504,15 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 bw = hw[2];
                 bh = hw[3];
                 staged = hw[4];
-                win = hs + NM_TILE * TH;
+                const int off = __builtin_amdgcn_readfirstlane(hw[6]);
+                win = hb + off + (refh ? NM_TILE * TH : 0);
 #pragma unroll
-                for (int r = 0; r < 4; r++) rv[r] = r < R && act ? hs[(ty + 16 * r) * NM_TILE + tx] : 0.0f;
+                for (int r = 0; r < 4; r++) {
+                    const int x = X0 + tx, y = Y0 + ty + 16 * r;
+                    rv[r] = r < R && act ? (refh ? hb[off + (ty + 16 * r) * NM_TILE + tx]
+                                                 : (x < tw && y < th ? rf[(size_t)y * tw + x] : 0.0f))
+                                         : 0.0f;
+                }
             }
         }
         // the window is the block's (scalar registers)
@@ -554,11 +603,15 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
 // the level's own samples, and a tap outside it is read from memory); the
 // order of the sums follows the level's tiling.
 __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np, bool solo,
-                        bool resident, bool held, bool hold, double *part, unsigned *ctr, unsigned *err,
+                        bool resident, bool held, bool hold, bool refh, double *part, unsigned *ctr, unsigned *err,
                         unsigned &gen) {
     const int tw = K.lw[lvl], th = K.lh[lvl];
-    uint64_t tq = wall_clock64();
-    if (threadIdx.x == 0) sh.ph[3] += tq - sh.tmark;
+    const bool tm = K.phases != 0;  // block-uniform
+    const uint64_t tq = tm ? wall_clock64() : 0;
+    if (tm && threadIdx.x == 0) {
+        sh.ph[3] += tq - sh.tmark;
+        sh.lph[lvl][3] += tq - sh.tmark;
+    }
     if ((int)threadIdx.x < np) {
         double v[8];
         for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
@@ -566,14 +619,17 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         sim_inverse_dev<false>(v, tw, th, sh.inv[threadIdx.x]);
     }
     __syncthreads();
-    uint64_t tq1 = wall_clock64();
-    if (threadIdx.x == 0) sh.ph[0] += tq1 - tq;
+    const uint64_t tq1 = tm ? wall_clock64() : 0;
+    if (tm && threadIdx.x == 0) {
+        sh.ph[0] += tq1 - tq;
+        sh.lph[lvl][0] += tq1 - tq;
+    }
     zero_waves(sh);  // ordered before its first use by the tile loop's barriers
     // The displacement (source - destination) of an affine map is affine in the
     // pixel, so over the level it is bounded by its values at the 4 corners: one
     // bound for the pass, from which every tile's window follows.  A projective
     // point (or a wild one) turns the staging off for the pass.
-    if (threadIdx.x == 0 && !resident && (!held || hold)) {
+    if (threadIdx.x == 0 && !resident && (!held || hold || sh.restage)) {
         double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
         int ok = 1;
         for (int p = 0; p < np; p++) {
@@ -598,21 +654,25 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         sh.box[4] = ok;
     }
     if (resident)
-        nm_tiles<1>(K, sh, boxs, hb, lvl, np, solo, false, false);
+        nm_tiles<1>(K, sh, boxs, hb, lvl, np, solo, false, false, false);
     else
-        nm_tiles<0>(K, sh, boxs, hb, lvl, np, solo, held, hold);
+        nm_tiles<0>(K, sh, boxs, hb, lvl, np, solo, held, hold, refh);
     fold_waves(sh, 3 * np);
-    uint64_t tq2 = wall_clock64();
+    const uint64_t tq2 = tm ? wall_clock64() : 0;
     if (threadIdx.x == 0) {
-        sh.ph[1] += tq2 - tq1;
+        if (tm) {
+            sh.ph[1] += tq2 - tq1;
+            sh.lph[lvl][1] += tq2 - tq1;
+        }
         sh.lev_passes[lvl]++;
     }
     if (solo) {
         if ((int)threadIdx.x < np)
             sh.PC[threadIdx.x] = rms_of(sh.sums[3 * threadIdx.x], sh.sums[3 * threadIdx.x + 1], sh.sums[3 * threadIdx.x + 2]);
-        if (threadIdx.x == 0) {
+        if (tm && threadIdx.x == 0) {
             sh.tmark = wall_clock64();
             sh.ph[2] += sh.tmark - tq2;
+            sh.lph[lvl][2] += sh.tmark - tq2;
         }
         __syncthreads();
         return;
@@ -621,48 +681,76 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
     const int nv = 3 * np;
     const int G = nm_G(sh), B = nm_B(sh);
     double *pp = part + (size_t)slot * G * 3 * NM_MAXP;
-    if (threadIdx.x == 0) {
-        // The partials are agent-scope atomic stores: written through to the
-        // level every XCD reads coherently.  Their completion is waited for
-        // explicitly (s_waitcnt vmcnt(0): on gfx9 a store leaves the vector
-        // memory count only once the write is acknowledged at its scope)
-        // before the arrival count is raised; the workgroup release fence
-        // keeps the compiler from moving the stores past the count, but emits
-        // no wait of its own on this target.  Every reader loads them with
-        // agent-scope atomic loads once it has seen the count.  So no XCD's L2
-        // has to be written back or invalidated for them: an agent-scope
-        // release / acquire does both to the whole L2 on every pass, which
-        // here (the side work's dirty lines in it, the level's samples lost
-        // from it) cost more than a small level's pass.
-        for (int k = 0; k < nv; k++)
-            __hip_atomic_store(&pp[(size_t)B * 3 * NM_MAXP + k], sh.sums[k], __ATOMIC_RELAXED,
+    if (threadIdx.x < 64) {
+        // Wave 0 publishes the block's partials and waits for everyone's.  The
+        // partials are agent-scope atomic stores (one lane each, one store
+        // instruction): written through to the level every XCD reads
+        // coherently.  Their completion is waited for explicitly (s_waitcnt
+        // vmcnt(0): on gfx9 a store leaves the vector memory count only once
+        // the write is acknowledged at its scope) before the block's arrival
+        // flag is raised to the number of passes it has completed; the
+        // workgroup release fence keeps the compiler from moving the stores
+        // past the flag, but emits no wait of its own on this target.  Every
+        // reader loads the partials with agent-scope atomic loads once it has
+        // seen every flag at the pass.  So no XCD's L2 has to be written back
+        // or invalidated for them: an agent-scope release / acquire does both
+        // to the whole L2 on every pass, which here (the side work's dirty
+        // lines in it, the level's samples lost from it) cost more than a
+        // small level's pass.  A flag per block (not a shared counter) keeps
+        // the arrival a plain store, and the wave reads all flags in one
+        // round trip per poll (a lane per block, up to NM_MAXG / 64 each).
+        const int lane = threadIdx.x;
+        if (lane < nv)
+            __hip_atomic_store(&pp[(size_t)B * 3 * NM_MAXP + lane], sh.sums[lane], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(0);
-        // two-level arrival: NM_GROUPS counters (one cache line each; group g
-        // holds the ranks congruent to g), then one counter of completed
-        // groups, which every block waits on (bounded, and left at once when
-        // any block has raised the error flag).  A group's last arriver has
-        // seen every increment of its group (each made after that block's
-        // partials were complete) when it raises the top count.
-        const unsigned g = (unsigned)B % NM_GROUPS;
-        const unsigned members = (unsigned)G / NM_GROUPS + (g < (unsigned)G % NM_GROUPS ? 1u : 0u);
-        const unsigned groups = (unsigned)min(G, NM_GROUPS);
-        unsigned a = __hip_atomic_fetch_add(&ctr[NM_FL_GROUP + NM_LINE * g], 1u, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-        if (a == (gen + 1) * members - 1)
-            __hip_atomic_fetch_add(&ctr[NM_FL_TOP], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned target = (gen + 1) * groups;
-        const uint64_t t_in = wall_clock64();
+        unsigned *arr = ctr + NM_FL_ARR;
+        const unsigned target = gen + 1;
+        if (lane == 0) __hip_atomic_store(&arr[B], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tm && lane == 0) {
+            const uint64_t tp = wall_clock64();
+            sh.lph[lvl][4] += tp - tq2;
+            sh.tmark = tp;
+        }
+        // bounded, and left when any block has raised the error flag (looked
+        // at, with the clock, every 64 polls; the clock starts at the first look)
+        uint64_t t_in = 0;
         unsigned spins = 0;
-        while (__hip_atomic_load(&ctr[NM_FL_TOP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        for (;;) {
+            unsigned v[NM_MAXG / 64];
+#pragma unroll
+            for (int j = 0; j < NM_MAXG / 64; j++) {
+                const int b = lane + 64 * j;
+                v[j] = b < G ? __hip_atomic_load(&arr[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
+            }
+            int all = 1;
+#pragma unroll
+            for (int j = 0; j < NM_MAXG / 64; j++) all &= v[j] >= target;
+            if (__all(all)) break;
             __builtin_amdgcn_s_sleep(1);
-            if ((++spins & 63u) == 0 && wall_clock64() - t_in > K.wait_ticks) {
-                atomicOr(err, 1u);
-                break;
+            if ((++spins & 63u) == 0) {
+                // another block's failure, or this wait's time-out, ends the search
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (lane == 0) sh.stop = 1;
+                    break;
+                }
+                const uint64_t now = wall_clock64();
+                if (spins == 64u) t_in = now;
+                if (now - t_in > K.wait_ticks) {
+                    if (lane == 0) {
+                        atomicOr(err, 1u);
+                        sh.stop = 1;
+                    }
+                    break;
+                }
             }
         }
+    }
+    if (tm && threadIdx.x == 0) {
+        const uint64_t tw = wall_clock64();
+        sh.lph[lvl][5] += tw - sh.tmark;
+        sh.tmark = tw;
     }
     __syncthreads();
     {
@@ -716,9 +804,13 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         unsigned long long d = sh.dig;
         for (int k = 0; k < nv; k++) d = (d ^ (unsigned long long)__double_as_longlong(sh.sums[k])) * 1099511628211ull;
         sh.dig = d;
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) sh.stop = 1;
-        sh.tmark = wall_clock64();
-        sh.ph[2] += sh.tmark - tq2;
+        if (tm) {
+            const uint64_t tf = wall_clock64();
+            sh.lph[lvl][6] += tf - sh.tmark;
+            sh.tmark = tf;
+            sh.ph[2] += sh.tmark - tq2;
+            sh.lph[lvl][2] += sh.tmark - tq2;
+        }
     }
     __syncthreads();
 }
@@ -758,6 +850,8 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
 #define CLAMPK(v, k) clampd_dev((v), sh.lo[k], sh.hi[k])
     if ((int)threadIdx.x < n) sh.tolv[threadIdx.x] = K.tol[lvl][threadIdx.x];  // read by the iterations' set-up
     if (t0) {
+        sh.hoff = 0;  // the level's held tiles start over
+        sh.restage = 0;
         for (int k = 0; k < n; k++) sh.S[0][k] = sh.cur[K.free_idx[k]];
         for (int i = 0; i < n; i++) {
             for (int k = 0; k < n; k++) sh.S[i + 1][k] = sh.S[0][k];
@@ -794,10 +888,11 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
         }
     }
     __syncthreads();
-    const bool held = !solo && nm_holdable(K.lw[lvl], K.lh[lvl], nm_G(sh));
+    bool refh = false;
+    const bool held = !solo && nm_holdable(K.lw[lvl], K.lh[lvl], nm_G(sh), &refh);
     int state = INIT, np = n + 1, it = 0;
     for (;;) {
-        nm_pass(K, sh, boxs, hb, lvl, np, solo, resident, held, held && state == INIT, part, ctr, err, gen);
+        nm_pass(K, sh, boxs, hb, lvl, np, solo, resident, held, held && state == INIT, refh, part, ctr, err, gen);
         int next = -1;  // a follow-up pass within the iteration, or none
         if (state == INIT) {
             if (t0) {
@@ -953,7 +1048,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
                                                           unsigned *__restrict__ err) {
     __shared__ NmShared sh;
     __shared__ __attribute__((aligned(16))) float boxs[NM_BOX];  // a tile's source window; the fold's staging
-    __shared__ __attribute__((aligned(16))) float hb[NM_HELD * NM_HSLOT];  // a held level's tiles
+    __shared__ __attribute__((aligned(16))) float hb[NM_HB];  // a held level's tiles
     const NmConst &K = *Kp;
     const int n = K.nfree;
     unsigned gen = 0;
@@ -1017,6 +1112,8 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
             }
             for (int l = 0; l < NM_MAXL; l++) sh.lev_evals[l] = sh.lev_passes[l] = 0;
             for (int q = 0; q < 4; q++) sh.ph[q] = 0;
+            for (int l = 0; l < NM_MAXL; l++)
+                for (int q = 0; q < NM_NPH; q++) sh.lph[l][q] = 0;
             sh.dig = 14695981039346656037ull;
             sh.ntrace = 0;
             sh.tmark = wall_clock64();
@@ -1076,13 +1173,15 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
         for (int k = 0; k < 8; k++) state[k] = sh.cur[k];
         state[8] = tot;
         for (int q = 0; q < 4; q++) state[9 + 2 * NM_MAXL + q] = (double)sh.ph[q];
+        for (int l = 0; l < NM_MAXL; l++)
+            for (int q = 0; q < NM_NPH; q++) state[NM_ST_LPH + NM_NPH * l + q] = (double)sh.lph[l][q];
         for (int l = 0; l < NM_MAXL; l++) state[13 + 2 * NM_MAXL + l] = (double)sh.lev_passes[l];
         state[NM_ST_PART] = (double)nm_G(sh);
         state[NM_ST_TRACE] = (double)sh.ntrace;
     }
 }
 
-// clears the arrival counters, the join count and the hand-off flag before a
+// clears the arrival flags, the join count and the hand-off flag before a
 // search (a kernel, so the sequence can be captured in a graph), and the error
 // flag unless keep_err (a stream's later frames: one failed search fails the
 // rest at once, and the host reports it after the last frame)
@@ -1498,6 +1597,10 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
         K.join_ticks = (uint64_t)khz * 20u;
     }
     if (const char *e = std::getenv("LSH_SEARCH_MAX_BLOCKS")) K.max_part = std::max(0, std::atoi(e));
+    {
+        const char *e = std::getenv("LSH_SEARCH_PHASES");
+        K.phases = prof().on || (e && std::atoi(e) != 0);
+    }
     double W = (double)f->w, H = (double)f->h, r = std::max(W, H) / 2.0;
     const double gain[8] = {W, H, r, r, H / 2.0, H / 2.0, W / 4.0, H / 4.0};  // align.rs:766-770
     for (size_t l = 0; l < n; l++) {
@@ -1520,9 +1623,9 @@ static void search_const(lsh_frames *f, const uint8_t *free8, size_t coarsen) {
         if (const char *e = std::getenv("LSH_SEARCH_BLOCKS_PER_CU")) want = std::max(1, std::atoi(e));
         f->nm_bpc = (unsigned)std::min(per_cu, want);
         f->nm_grid = f->nm_bpc * (unsigned)cus;
-        f->nm_grid -= f->nm_grid % NM_GROUPS;
-        LSH_REQUIRE(f->nm_grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
-        if (f->nm_grid > NM_BOX / 2) f->nm_grid = NM_BOX / 2;  // the fold stages one value per block in LDS
+        f->nm_grid = std::min(f->nm_grid, (unsigned)NM_MAXG);  // an arrival flag each
+        f->nm_grid -= f->nm_grid % NM_GRID_STEP;
+        LSH_REQUIRE(f->nm_grid >= NM_GRID_STEP, "the search needs at least 16 resident blocks");
         f->nm_part.alloc((size_t)2 * 3 * NM_MAXP * f->nm_grid + 2 * NM_MAXP);
         f->nm_digest.alloc(f->nm_grid);
     }
@@ -1558,8 +1661,8 @@ static void search_launch(lsh_frames *f, bool plain = false, hipEvent_t started 
     if (const unsigned c = stream_cus()) {
         // a stream restricted to c CUs holds only that many blocks at once
         grid = std::min(grid, f->nm_bpc * c);
-        grid -= grid % NM_GROUPS;
-        LSH_REQUIRE(grid >= NM_GROUPS, "the search needs at least 16 resident blocks");
+        grid -= grid % NM_GRID_STEP;
+        LSH_REQUIRE(grid >= NM_GRID_STEP, "the search needs at least 16 resident blocks");
     }
     if (plain) {
         launch("nm_search", k_nm_search, dim3(grid), dim3(NM_THREADS), 0, kp, state, part, ctr, err);
@@ -1586,9 +1689,11 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     LSH_REQUIRE(flags[0] == 0, "search: a wait for the other blocks timed out");
     f->search_part = (unsigned)out[NM_ST_PART];
     f->trace_n = (size_t)out[NM_ST_TRACE];
+    // algorithmic bytes: each counted cost evaluation (the reference's own,
+    // align.rs:581-635) reads the level's reference and target once; the
+    // speculative costs a branch leaves unused are this kernel's overhead
     double bytes = 0.0;
-    for (size_t l = 0; l < n; l++)  // a pass reads the level's reference and target once
-        bytes += out[13 + 2 * NM_MAXL + l] * 8.0 * (double)(f->lw[l] * f->lh[l]);
+    for (size_t l = 0; l < n; l++) bytes += out[9 + l] * 8.0 * (double)(f->lw[l] * f->lh[l]);
     prof_set_last_bytes(bytes);
     Sim s;
     std::memcpy(s.v, out, 8 * sizeof(double));
@@ -1598,6 +1703,8 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
         f->lev_ticks[l] = (uint64_t)out[9 + NM_MAXL + l];
     }
     for (int q = 0; q < 4; q++) f->phase_ticks[q] = (uint64_t)out[9 + 2 * NM_MAXL + q];
+    for (int l = 0; l < NM_MAXL; l++)
+        for (int q = 0; q < NM_NPH; q++) f->lev_phase_ticks[l][q] = (uint64_t)out[NM_ST_LPH + NM_NPH * l + q];
     for (size_t l = 0; l < NM_MAXL; l++) f->lev_passes[l] = (uint64_t)out[13 + 2 * NM_MAXL + l];
     return s;
 }
@@ -1905,6 +2012,9 @@ char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, u
         if (ticks) ticks[NM_MAXL + q] = f->phase_ticks[q];
     for (uint32_t l = 0; l < NM_MAXL && NM_MAXL + l < cap; l++)
         if (evals) evals[NM_MAXL + l] = f->lev_passes[l];
+    // then per level the phases' ticks
+    for (uint32_t i = 0; i < NM_NPH * NM_MAXL && NM_MAXL + 4 + i < cap; i++)
+        if (ticks) ticks[NM_MAXL + 4 + i] = f->lev_phase_ticks[i / NM_NPH][i % NM_NPH];
     LSH_API_END
 }
 

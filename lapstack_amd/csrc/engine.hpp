@@ -66,6 +66,7 @@ struct lsh_frames {
     lsh::DBuf<double> rec_sims, rec_evals;               // per-frame Sim and evaluations of a stream
     lsh::DBuf<float> rec_gains;                          // per-frame gains of a stream
     uint64_t phase_ticks[4] = {0};
+    uint64_t lev_phase_ticks[16][8] = {};
     size_t coarsen = 0;                                  // a stream's finest searched level
 };
 
