@@ -748,9 +748,9 @@ __device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, 
         }
     }
     if (tm && threadIdx.x == 0) {
-        const uint64_t tw = wall_clock64();
-        sh.lph[lvl][5] += tw - sh.tmark;
-        sh.tmark = tw;
+        const uint64_t t_all = wall_clock64();
+        sh.lph[lvl][5] += t_all - sh.tmark;
+        sh.tmark = t_all;
     }
     __syncthreads();
     {

@@ -65,7 +65,17 @@ __device__ __forceinline__ int cdivd(int a, int b) { return (a + b - 1) / b; }
 #define B_ERR 1     // sticky error: a bounded wait ran out
 #define B_JOIN 2    // tickets taken at start; bit 31 set once the set is closed
 #define B_NPART 3   // the number of participating blocks, published at close
+#define B_XCNT 4    // per XCD: the participants' tickets taken there (B_NXCD words)
+#define B_XPUB 12   // per XCD: its participants, published at close (B_NXCD words)
+#define B_WORDS 20
+#define B_NXCD 8
+#define B_RUN 256   // items per run dealt to an XCD
 #define B_CLOSED 0x80000000u
+
+// The XCD (L2 and its CUs) the block runs on: the XCC_ID hardware register.
+__device__ __forceinline__ unsigned b_xcd() {
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & (B_NXCD - 1);  // hwreg(HW_REG_XCC_ID, 0, 4)
+}
 
 // Which blocks take part.  A cooperative launch is accepted up to the occupancy
 // the runtime computes from LDS, but the SGPRs of this kernel can allow fewer
@@ -73,15 +83,20 @@ __device__ __forceinline__ int cdivd(int a, int b) { return (a + b - 1) / b; }
 // a resident one ends -- a grid barrier would then wait for it forever.  So
 // every block takes a ticket as it starts; the first one closes the set once
 // every block has joined or none has joined for 20 us (resident blocks start
-// within a few us of each other) and publishes the count.  Ticket holders do
-// all the work with their ticket as block index and the count as grid size;
-// blocks starting after the close leave at once.  Every participant was
-// running when it joined, so the barriers below only wait for running blocks.
+// within a few us of each other) and publishes the count.  Each participant
+// also takes a ticket on its XCD (its rank there); the closer waits for those
+// (taken right after the first, by running blocks) and publishes the count per
+// XCD with the total.  Blocks starting after the close leave at once.  Every
+// participant was running when it joined, so the barriers below only wait for
+// running blocks.  sh: the ticket, the count, the XCD, the rank on it, then
+// the participants per XCD.
 __device__ void b_join(unsigned *ctl, unsigned *sh) {
     if (threadIdx.x == 0) {
         const unsigned t = atomicAdd(&ctl[B_JOIN], 1u);
-        unsigned n = 0;
+        unsigned n = 0, x = 0, rx = 0, cnt[B_NXCD] = {};
         if (!(t & B_CLOSED)) {
+            x = b_xcd();
+            rx = atomicAdd(&ctl[B_XCNT + x], 1u);
             if (t == 0) {
                 unsigned seen = 1, spins = 0;
                 unsigned long long last = __builtin_amdgcn_s_memrealtime();  // 100 MHz
@@ -99,6 +114,21 @@ __device__ void b_join(unsigned *ctl, unsigned *sh) {
                     if (++spins > (1u << 22)) break;
                 }
                 n = atomicOr(&ctl[B_JOIN], B_CLOSED) & ~B_CLOSED;
+                // the n participants' XCD tickets
+                spins = 0;
+                for (;;) {
+                    unsigned sum = 0;
+                    for (int y = 0; y < B_NXCD; y++)
+                        sum += cnt[y] = __hip_atomic_load(&ctl[B_XCNT + y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (sum >= n) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1u << 24)) {
+                        atomicOr(&ctl[B_ERR], 1u);
+                        break;
+                    }
+                }
+                for (int y = 0; y < B_NXCD; y++)
+                    __hip_atomic_store(&ctl[B_XPUB + y], cnt[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&ctl[B_NPART], n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 unsigned spins = 0;
@@ -109,10 +139,17 @@ __device__ void b_join(unsigned *ctl, unsigned *sh) {
                         break;
                     }
                 }
+                for (int y = 0; y < B_NXCD; y++)
+                    cnt[y] = __hip_atomic_load(&ctl[B_XPUB + y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
+        // a failed join (a wait that ran out) sends every block away
+        if (__hip_atomic_load(&ctl[B_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) n = 0;
         sh[0] = t;
         sh[1] = n;
+        sh[2] = x;
+        sh[3] = rx;
+        for (int y = 0; y < B_NXCD; y++) sh[4 + y] = cnt[y];
     }
     __syncthreads();
 }
@@ -213,12 +250,24 @@ template <int R, int TY>
 __global__ __launch_bounds__(256) void k_batch(const BStack *__restrict__ st, BPlan P, unsigned *ctl) {
     constexpr size_t LDS = sizeof(ReduceLds) > sizeof(FoldLds<R, TY>) ? sizeof(ReduceLds) : sizeof(FoldLds<R, TY>);
     __shared__ __attribute__((aligned(16))) unsigned char lds[LDS];
-    __shared__ unsigned ticket[2];
+    __shared__ unsigned ticket[4 + B_NXCD];
     ReduceLds &SR = *reinterpret_cast<ReduceLds *>(lds);
     FoldLds<R, TY> &SF = *reinterpret_cast<FoldLds<R, TY> *>(lds);
     b_join(ctl, ticket);
     const unsigned vb = ticket[0], nb = ticket[1];
-    if ((vb & B_CLOSED) || nb == 0 || vb >= nb) return;  // started after the close (or the count never came)
+    if ((vb & B_CLOSED) || nb == 0 || vb >= nb) return;  // started after the close (or the join failed)
+    // A phase's items are cut into runs of B_RUN (a few rows of tiles), dealt
+    // to the XCDs in turn; an XCD's participants stride over its runs.  So
+    // neighbouring tiles (items in row order), which read the same halo rows,
+    // are worked on together by one XCD and meet those rows in its L2, while
+    // every XCD gets a like share of every kind of item.
+    const unsigned xcd = ticket[2], rx = ticket[3], nx = ticket[4 + xcd];
+    unsigned before = 0;  // the participants on the XCDs numbered below this one
+    for (unsigned y = 0; y < xcd; y++) before += ticket[4 + y];
+    if (nx == 0 || rx >= nx || before + nx > nb) {  // counts that do not add up: the launch fails
+        if (threadIdx.x == 0) atomicOr(&ctl[B_ERR], 1u);
+        return;
+    }
     unsigned gen = 0;
     const int nph = P.Lmax + 1;
     for (int r = 0; r < P.rounds; r++) {
@@ -228,7 +277,9 @@ __global__ __launch_bounds__(256) void k_batch(const BStack *__restrict__ st, BP
             const long long total = pre[A];
             if (total == 0) continue;
             int s = 0;
-            for (long long it = vb; it < total; it += nb) {
+            for (long long u = rx;; u += nx) {
+                const long long it = ((u / B_RUN) * B_NXCD + xcd) * B_RUN + u % B_RUN;
+                if (it >= total) break;
                 while (pre[s + 1] <= it) s++;  // items only grow along the walk
                 long long j = it - pre[s];
                 const BStack &B = st[s];
@@ -276,7 +327,7 @@ __global__ __launch_bounds__(256) void k_batch(const BStack *__restrict__ st, BP
 }
 
 __global__ void k_batch_reset(unsigned *ctl) {
-    if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
+    if (threadIdx.x < B_WORDS) ctl[threadIdx.x] = 0;
 }
 
 static void binomial_host(size_t r, float *out) {  // fuse.rs:139-149
@@ -406,7 +457,7 @@ char *lsh_batch_fuse(lsh_fuser **fusers, const lsh_batch_stack *stacks, uint32_t
     };
     DBuf<uint8_t> dstacks(count * sizeof(BStack));
     copy_mem(dstacks.p, LSH_DEVICE, hs.data(), LSH_HOST, count * sizeof(BStack));
-    DBuf<unsigned> ctl(4);
+    DBuf<unsigned> ctl(B_WORDS);
     DBuf<int> dactive;
     DBuf<long long> dpre;
     BPlan P{};
