@@ -1325,15 +1325,18 @@ __global__ void k_frame_gains(const double *__restrict__ sums, int enabled, Fram
 
 // The final resample of a frame (align.rs:554-559) with the as-shot fallback
 // outside the frame (stack.rs:669-678), the brightness gains (brightness.rs:
-// 124-133), and optionally the warped registration luma (stack.rs:527).
+// 124-133), and optionally the warped registration luma (stack.rs:527) or,
+// FY, the luma of the resampled frame itself (the focus measure's input,
+// depth.rs:160-165, with the focus slice's own expression on the same values).
 // The warp of one pixel: the three channels (and the luma when LUMA) summed in
 // one pass over the taps, each sum in warp_sample's order, every load
 // independent of whether the point is in the frame (clamped indices are always
 // in range), so a thread's loads are all in flight together.
-template <int N, bool LUMA, bool RGB>
+template <int N, bool LUMA, bool RGB, bool FY = false>
 __device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const float *__restrict__ yin, int w, int h,
                                         size_t o, bool valid, int x0, int y0, const float *wx, const float *wy,
-                                        const Gains &g, float *__restrict__ planes, float *__restrict__ yout) {
+                                        const Gains &g, float *__restrict__ planes, float *__restrict__ yout,
+                                        float *__restrict__ fy = nullptr) {
     constexpr int start = 1 - N / 2;
     const size_t n = (size_t)w * h;
     float raw[3];
@@ -1356,20 +1359,24 @@ __device__ __forceinline__ void warp_px(const uint16_t *__restrict__ rgb, const 
         for (int c = 0; c < 3; c++) acc[c] += wy[j] * r[c];
         if (LUMA) ay += wy[j] * ry;
     }
-    if (RGB)
+    if (RGB) {
+        float v[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            float v = valid ? acc[c] : raw[c];
-            if (g.on[c]) v *= g.g[c];
-            planes[c * n + o] = v;
+            v[c] = valid ? acc[c] : raw[c];
+            if (g.on[c]) v[c] *= g.g[c];
+            planes[c * n + o] = v[c];
         }
+        if (FY) fy[o] = 0.299f * v[0] + 0.587f * v[1] + 0.114f * v[2];
+    }
     if (LUMA) yout[o] = ay;
 }
 
 template <int N>
 __global__ __launch_bounds__(256) void k_warp(const FrameRef src, const float *__restrict__ yin, int w, int h, Inv9 inv,
                                               int affine, int ik, Gains g, float *__restrict__ planes,
-                                              float *__restrict__ yout, const FrameDev *__restrict__ fp) {
+                                              float *__restrict__ yout, const FrameDev *__restrict__ fp,
+                                              float *__restrict__ flum) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
     const uint16_t *__restrict__ rgb = src.get();
@@ -1391,6 +1398,8 @@ __global__ __launch_bounds__(256) void k_warp(const FrameRef src, const float *_
         warp_px<N, true, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
     else if (yout)
         warp_px<N, true, false>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
+    else if (flum)
+        warp_px<N, false, true, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout, flum);
     else
         warp_px<N, false, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
 }
@@ -1469,21 +1478,23 @@ __global__ __launch_bounds__(256) void k_bright_sums(const FrameRef src, int w, 
 
 static int taps_n(uint32_t ik) { return ik <= 1 ? 2 : (ik <= 3 ? 4 : 6); }
 
-// luma_only: the warped registration luma alone (no frame read, no planes)
+// luma_only: the warped registration luma alone (no frame read, no planes);
+// fy (with planes, without yout): the resampled frame's luma beside its planes
 static void warp_launch(lsh_frames *f, const FrameRef &rgb, const Inv9 &inv, int affine, const Gains &g,
                         float *yout, const FrameDev *fp = nullptr, const float *yin_ = nullptr,
-                        float *planes_out = nullptr, bool luma_only = false) {
+                        float *planes_out = nullptr, bool luma_only = false, float *fy = nullptr) {
     dim3 gr(cdiv(f->w, 64), cdiv(f->h, 4)), b(64, 4);
     const float *yin = yin_ ? yin_ : f->yt.p;
     float *planes = luma_only ? nullptr : (planes_out ? planes_out : f->planes.p);
     int ik = (int)f->interp, w = (int)f->w, h = (int)f->h;
     // u16 frame read, 3 f32 planes written; the luma: luma read, warped luma written
-    const double by = (double)f->w * f->h * ((planes ? 18.0 : 0.0) + (yout ? 8.0 : 0.0));
+    if (yout || !planes) fy = nullptr;
+    const double by = (double)f->w * f->h * ((planes ? 18.0 : 0.0) + (yout ? 8.0 : 0.0) + (fy ? 4.0 : 0.0));
     const char *name = planes ? "warp" : "warp_luma";
     switch (taps_n(f->interp)) {
-    case 2: launch_b(name, by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
-    case 4: launch_b(name, by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
-    default: launch_b(name, by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp); break;
+    case 2: launch_b(name, by, k_warp<2>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp, fy); break;
+    case 4: launch_b(name, by, k_warp<4>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp, fy); break;
+    default: launch_b(name, by, k_warp<6>, gr, b, 0, rgb, yin, w, h, inv, affine, ik, g, planes, yout, fp, fy); break;
     }
 }
 
@@ -1779,7 +1790,7 @@ void frames_chain(lsh_frames *f, int slot, bool plain_search, hipEvent_t started
     f->processed++;
 }
 
-void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness) {
+void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness, float *fy_out) {
     FrameDev *fp = (FrameDev *)f->fdev.p + slot;
     const Inv9 unused{};
     const Gains ug{};
@@ -1789,7 +1800,7 @@ void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *plane
     } else {
         launch("frame_gains", k_frame_gains, dim3(1), dim3(64), 0, (const double *)nullptr, 0, fp);
     }
-    warp_launch(f, src, unused, 1, ug, nullptr, fp, nullptr, planes_out);
+    warp_launch(f, src, unused, 1, ug, nullptr, fp, nullptr, planes_out, false, fy_out);
     launch("frame_record", k_frame_record, dim3(1), dim3(64), 0, (const FrameDev *)fp, f->cursor.p, f->rec_sims.p,
            f->rec_gains.p, f->rec_evals.p);
 }

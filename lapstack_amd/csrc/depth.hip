@@ -43,6 +43,9 @@ struct TapTable {
 // averaged on the working grid.  A block covers 64 x (4 CY) grid cells: the
 // luma of its pixels and their halo is staged in LDS once (reflect-101 at the
 // borders), then each thread averages the measure over CY cells of its column.
+// ONE: the input is the luma plane itself (written with the same expression by
+// the warp that made the planes), not the three planes.
+template <bool ONE>
 __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k, int CY,
                                                      const TapTable tab, Taps T, float *__restrict__ out, int dw,
                                                      int dh, const unsigned *__restrict__ out_idx) {
@@ -66,14 +69,17 @@ __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ p
                 const int ly = q / PW, lx = q - ly * PW;
                 const size_t o = (size_t)rf101(py0 + ly, h) * w + rf101(px0 + lx, w);
                 v[u][0] = planes[o];
-                v[u][1] = planes[n + o];
-                v[u][2] = planes[2 * n + o];
+                if (!ONE) {
+                    v[u][1] = planes[n + o];
+                    v[u][2] = planes[2 * n + o];
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int q = q0 + 256 * u;
-            if (q < NQ) Y[q] = 0.299f * v[u][0] + 0.587f * v[u][1] + 0.114f * v[u][2];  // depth.rs:160-165
+            if (q < NQ)
+                Y[q] = ONE ? v[u][0] : 0.299f * v[u][0] + 0.587f * v[u][1] + 0.114f * v[u][2];  // depth.rs:160-165
         }
     }
     __syncthreads();
@@ -146,15 +152,17 @@ static void rdf_taps_ordered(uint32_t r_in, uint32_t r_out, std::vector<int32_t>
 // (uploaded when the measure changes).
 static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
                              uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
-                             float *out, const unsigned *out_idx = nullptr);
+                             float *out, const unsigned *out_idx = nullptr, bool luma = false);
 
-void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out, const unsigned *out_idx) {
-    focus_slice_with(f->taps_dy, f->taps_dx, f->taps_wt, f->ntaps, f->taps_key, f->w, f->h, planes, p, out, out_idx);
+void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out, const unsigned *out_idx,
+                 bool luma) {
+    focus_slice_with(f->taps_dy, f->taps_dx, f->taps_wt, f->ntaps, f->taps_key, f->w, f->h, planes, p, out, out_idx,
+                     luma);
 }
 
 static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
                              uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
-                             float *out, const unsigned *out_idx) {
+                             float *out, const unsigned *out_idx, bool luma) {
     LSH_REQUIRE(p.scale <= 2, "focus slice: scale above 2 is not supported");
     LSH_REQUIRE(p.focus_kind <= 1, "focus slice: unknown focus measure");
     int k = 1 << p.scale;
@@ -202,8 +210,13 @@ static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBu
             tab.wt[t] = wt[t];
         }
     }
-    launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice, dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)),
-             dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY, tab, T, out, dw, dh, out_idx);
+    if (luma)
+        launch_b("focus_slice", 4.0 * w * h + 4.0 * dw * dh, k_focus_slice<true>, dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)),
+                 dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY, tab, T, out, dw, dh, out_idx);
+    else
+        launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice<false>,
+                 dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)), dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY,
+                 tab, T, out, dw, dh, out_idx);
 }
 
 // ----------------------------------------------------------------- helpers

@@ -105,14 +105,17 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
 // another stream orders after it is dispatched after the search, so it cannot
 // take the CUs the search's grid needs at its start.
 void frames_stream_slots(lsh_frames *f);
-// out_idx (if given): the slice goes to out + out_idx[0] * (its cell count)
+// out_idx (if given): the slice goes to out + out_idx[0] * (its cell count);
+// luma: planes is the frame's luma plane (one plane), not its three planes
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out,
-                 const unsigned *out_idx = nullptr);  // depth.hip
+                 const unsigned *out_idx = nullptr, bool luma = false);  // depth.hip
 void frames_target(lsh_frames *f, const FrameRef &src, int slot);
 void frames_chain(lsh_frames *f, int slot, bool plain_search, hipEvent_t started = nullptr);
 // The rest of frame k's step, off the chain: the brightness sums and gains,
-// the RGB warp into planes_out (the map of the FrameDev at slot), the record.
-void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness);
+// the RGB warp into planes_out (the map of the FrameDev at slot) and, when
+// fy_out is given, the warped frame's luma into it; the record.
+void frames_side_warp(lsh_frames *f, const FrameRef &src, int slot, float *planes_out, bool brightness,
+                      float *fy_out = nullptr);
 void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, uint64_t *evals);
 // fuse.hip: a push whose frame index comes from device memory (a replayed graph)
 void fuser_reserve(lsh_fuser *f, size_t nframes);

@@ -246,6 +246,11 @@ char *stream_impl(lsh_frames *f, lsh_fuser *fu, const Source &src, uint32_t n, c
         const hipStream_t Ms = stream(), Ss = side_stream();
         Events E;
         float *planes[2] = {f->planes.p, f->planes_b.p};
+        // with a focus measure, the side warp also writes each frame's luma,
+        // from which the focus slice reads one plane instead of three
+        DBuf<float> fluma[2];
+        if (measure)
+            for (auto &b : fluma) b.alloc(npx);
         const size_t fbytes = 3 * npx * sizeof(uint16_t);
         // raw frames: the caller's device pointers (read through a device table
         // of them inside a replayed graph), or host frames uploaded into a ring
@@ -324,8 +329,9 @@ char *stream_impl(lsh_frames *f, lsh_fuser *fu, const Source &src, uint32_t n, c
         // frame k's side work: its gains, RGB warp and record, its focus slice
         // and its fold (the fold index is k)
         auto side = [&](uint32_t k, const FrameRef &src) {
-            frames_side_warp(f, src, (int)(k & 1), planes[k & 1], bright);
-            if (measure) focus_slice(f, planes[k & 1], *measure, slice_at(k), graph && direct ? fidx.p : nullptr);
+            frames_side_warp(f, src, (int)(k & 1), planes[k & 1], bright, measure ? fluma[k & 1].p : nullptr);
+            if (measure)
+                focus_slice(f, fluma[k & 1].p, *measure, slice_at(k), graph && direct ? fidx.p : nullptr, true);
             fuser_push_dev(fu, planes[k & 1], fidx.p);
             launch("stream_fold_next", k_counter_next, dim3(1), dim3(64), 0, fidx.p);
         };
