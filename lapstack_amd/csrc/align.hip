@@ -61,9 +61,11 @@ struct BurtLds {
     float Hs[2 * TY + 3][BT_TX + 1];
 };
 
-template <int TY, bool EDGE>
-__device__ __forceinline__ void burt_body(const float *__restrict__ src, int w, int h, float *__restrict__ dst, int ow,
-                                          int oh, int ox0, int oy0, BurtLds<TY> &S) {
+// src(gx, gy): the input plane's sample at an in-range position (a stored
+// plane, or a value computed on the spot)
+template <int TY, bool EDGE, class Src>
+__device__ __forceinline__ void burt_body(const Src &src, int w, int h, float *__restrict__ dst, int ow, int oh, int ox0,
+                                          int oy0, BurtLds<TY> &S) {
     constexpr int IW = 2 * BT_TX + 3, IH = 2 * TY + 3;
     float k[5];
     burt_k(k);
@@ -72,7 +74,7 @@ __device__ __forceinline__ void burt_body(const float *__restrict__ src, int w, 
         [&](int q, float &v) {
             const int r = q / IW, c = q - r * IW, gy = iy0 + r, gx = ix0 + c;
             if (EDGE && !(gy >= 0 && gy < h && gx >= 0 && gx < w)) return false;
-            v = src[(size_t)gy * w + gx];
+            v = src(gx, gy);
             return true;
         },
         [&](int q, float v) {
@@ -128,10 +130,11 @@ __global__ __launch_bounds__(256) void k_burt(const float *__restrict__ src, int
     const int ox0 = blockIdx.x * BT_TX, oy0 = blockIdx.y * TY;
     const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * BT_TX <= w - 1 && 2 * oy0 + 2 * TY <= h - 1 &&
                        ox0 + BT_TX <= ow && oy0 + TY <= oh;
+    const auto plane = [src, w](int gx, int gy) { return src[(size_t)gy * w + gx]; };
     if (inner)
-        burt_body<TY, false>(src, w, h, dst, ow, oh, ox0, oy0, S);
+        burt_body<TY, false>(plane, w, h, dst, ow, oh, ox0, oy0, S);
     else
-        burt_body<TY, true>(src, w, h, dst, ow, oh, ox0, oy0, S);
+        burt_body<TY, true>(plane, w, h, dst, ow, oh, ox0, oy0, S);
 }
 
 // align.rs:639-646
@@ -1404,6 +1407,54 @@ __global__ __launch_bounds__(256) void k_warp(const FrameRef src, const float *_
         warp_px<N, false, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
 }
 
+// The warped registration luma at one pixel: k_warp's luma path (warp_px's
+// LUMA sum, the same taps in the same order), for a consumer that needs it
+// only as an input sample.
+template <int N>
+__device__ __forceinline__ float warp_luma_at(const float *__restrict__ yin, int w, int h, const Inv9 &inv, int affine,
+                                              int ik, int x, int y) {
+    double sx, sy;
+    src_point(inv, affine, x, y, sx, sy);
+    const double fx = floor(sx), fy = floor(sy);
+    const int x0 = (int)fmax(fmin(fx, 1e9), -1e9), y0 = (int)fmax(fmin(fy, 1e9), -1e9);
+    float wx[6], wy[6];
+    interp_w(ik, sx - fx, wx);
+    interp_w(ik, sy - fy, wy);
+    constexpr int start = 1 - N / 2;
+    float ay = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const size_t row = (size_t)min(max(y0 + start + j, 0), h - 1) * w;
+        float ry = 0.0f;
+#pragma unroll
+        for (int i = 0; i < N; i++) ry += wx[i] * yin[row + min(max(x0 + start + i, 0), w - 1)];
+        ay += wy[j] * ry;
+    }
+    return ay;
+}
+
+// Level 1 of the registration pyramid of the warped luma straight from the
+// unwarped luma (align.rs:554-559 then :683-734): each tile's input window is
+// the warped luma computed in LDS, so the full-resolution warped plane is
+// never stored.  Only for pyramids whose level 0 nothing else reads (a search
+// that stops short of full resolution).
+template <int N>
+__global__ __launch_bounds__(256) void k_warp_burt(const float *__restrict__ yin, int w, int h, int ik,
+                                                   const FrameDev *__restrict__ fp, float *__restrict__ dst, int ow,
+                                                   int oh) {
+    __shared__ BurtLds<BT_TY> S;
+    const Inv9 inv = fp->inv;
+    const int affine = fp->affine;
+    const int ox0 = blockIdx.x * BT_TX, oy0 = blockIdx.y * BT_TY;
+    const bool inner = ox0 >= 1 && oy0 >= 1 && 2 * ox0 + 2 * BT_TX <= w - 1 && 2 * oy0 + 2 * BT_TY <= h - 1 &&
+                       ox0 + BT_TX <= ow && oy0 + BT_TY <= oh;
+    const auto warped = [&](int gx, int gy) { return warp_luma_at<N>(yin, w, h, inv, affine, ik, gx, gy); };
+    if (inner)
+        burt_body<BT_TY, false>(warped, w, h, dst, ow, oh, ox0, oy0, S);
+    else
+        burt_body<BT_TY, true>(warped, w, h, dst, ow, oh, ox0, oy0, S);
+}
+
 // The frame as decoded (io.rs:89), times the gains
 __global__ void k_ingest(const uint16_t *__restrict__ rgb, size_t n, Gains g, float *__restrict__ planes) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1782,11 +1833,28 @@ void frames_chain(lsh_frames *f, int slot, bool plain_search, hipEvent_t started
     search_launch(f, plain_search, started);
     launch("frame_inv", k_frame_inv, dim3(1), dim3(64), 0, (const double *)f->nm_state.p,
            (const unsigned *)(f->nm_flags.p + NM_FL_ERR), (int)w, (int)h, fp);
-    // the warped registration luma from the target's full-resolution luma in its slot
-    const Inv9 unused{};
-    const Gains ug{};
-    warp_launch(f, FrameRef(), unused, 1, ug, f->yr.p, fp, tgt, nullptr, true);
-    build_reg_pyramid(f, f->yr.p);
+    // the warped registration luma from the target's full-resolution luma in
+    // its slot, and its pyramid; when the search stops short of level 0, the
+    // warp is folded into the first REDUCE and level 0 is never stored
+    if (f->coarsen >= 1 && f->lw.size() >= 2) {
+        const dim3 grid(cdiv(f->lw[1], BT_TX), cdiv(f->lh[1], BT_TY));
+        const int wi = (int)w, hi = (int)h, ik = (int)f->interp, ow = (int)f->lw[1], oh = (int)f->lh[1];
+        float *l1 = f->yr.p + f->loff[1];
+        // the luma taps of the warped samples in the window (about 1.2 per level-0 pixel), level 1 written
+        const double by = 4.0 * (double)taps_n(f->interp) * taps_n(f->interp) * 1.2 * w * h + 4.0 * ow * oh;
+        switch (taps_n(f->interp)) {
+        case 2: launch_b("warp_burt", by, k_warp_burt<2>, grid, dim3(256), 0, tgt, wi, hi, ik, (const FrameDev *)fp, l1, ow, oh); break;
+        case 4: launch_b("warp_burt", by, k_warp_burt<4>, grid, dim3(256), 0, tgt, wi, hi, ik, (const FrameDev *)fp, l1, ow, oh); break;
+        default: launch_b("warp_burt", by, k_warp_burt<6>, grid, dim3(256), 0, tgt, wi, hi, ik, (const FrameDev *)fp, l1, ow, oh); break;
+        }
+        for (size_t l = 2; l < f->lw.size(); l++)
+            burt_level(f->yr.p + f->loff[l - 1], f->lw[l - 1], f->lh[l - 1], f->yr.p + f->loff[l], f->lw[l], f->lh[l]);
+    } else {
+        const Inv9 unused{};
+        const Gains ug{};
+        warp_launch(f, FrameRef(), unused, 1, ug, f->yr.p, fp, tgt, nullptr, true);
+        build_reg_pyramid(f, f->yr.p);
+    }
     f->processed++;
 }
 
