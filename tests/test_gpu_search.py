@@ -90,3 +90,24 @@ def test_search_with_part_of_the_grid():
         assert r["levels"].tolist() == lev.tolist()
         assert np.array_equal(r["sim"], sim)
         assert np.abs(r["costs"] - costs).max() <= 1e-9 * np.abs(costs).max()
+
+
+# align.rs:115-119 free_mask: the affine model frees aspect and shear as well, the
+# projective one the two perspective terms too (8 parameters, 9-point simplices;
+# a projective point turns the tile windows off for its pass)
+@pytest.mark.parametrize("model,free", [("affine", (1, 1, 1, 1, 1, 1, 0, 0)), ("projective", (1, 1, 1, 1, 1, 1, 1, 1))])
+def test_search_models_follow_the_oracle(model, free):
+    w, h, coarsen = 768, 512, 1
+    print(f"search {model} {w}x{h} coarsen {coarsen}", flush=True)
+    f0, f1, _ = _pair(w, h, seed=43)
+    dev = _device_search(f0, f1, coarsen, "bilinear", free=free)
+    y0, y1 = ref.luma(ref.u16_to_planes(f0)), ref.luma(ref.u16_to_planes(f1))
+    sim, evals, lev, costs = ref.multiscale_align_trace(y0, y1, free=free, coarsen=coarsen)
+    print("levels device", dev["levels"][:8].tolist(), "oracle", lev[:8].tolist(), flush=True)
+    assert dev["levels"].tolist() == lev.tolist()
+    assert dev["evals"] == evals
+    rel = np.abs(np.array(dev["costs"]) - costs) / np.maximum(np.abs(costs), 1e-30)
+    assert rel.max() <= 1e-9, rel.max()
+    assert np.array_equal(dev["sim"], sim), (dev["sim"], sim)
+    b = dev["blocks"]
+    assert len(b) >= 1 and all(d == b[0] for d in b)
