@@ -30,16 +30,22 @@ __global__ __launch_bounds__(256) void k_reduce3(const float *__restrict__ src, 
                 blockIdx.x * RED_TX, blockIdx.y * RED3_TY, S);
 }
 
-// one band-pass level of the fold on 64x16 tiles
+// one band-pass level of the fold on 64 x FOLD_TY tiles.  8 rows: a tile's LDS
+// (22.5 KB at radius 1) then fits on a CU beside the search's block, so in
+// the pipelined stack the fold also runs on the search's CUs (100-frame 45 MP
+// step 682.4 -> 679.2 ms; alone it runs as fast as with 16 rows)
+#ifndef FOLD_TY
+#define FOLD_TY 8
+#endif
 template <int R>
 __global__ __launch_bounds__(256) void k_fold(const float *__restrict__ G, const float *__restrict__ Cn, int w, int h,
                                               int cw, int ch, float *__restrict__ best, float *__restrict__ acc,
                                               uint16_t *__restrict__ winner, int first, unsigned idx, Wt9 wt,
                                               int chroma, float *__restrict__ en_out,
                                               const unsigned *__restrict__ idx_dev) {
-    __shared__ FoldLds<R, 16> S;
-    fold_tile<R, 16>(PlaneSrc{G, (size_t)w * h}, Cn, w, h, cw, ch, best, acc, winner, first, idx_dev ? *idx_dev : idx,
-                     wt, chroma, en_out, blockIdx.x * 64, blockIdx.y * 16, S);
+    __shared__ FoldLds<R, FOLD_TY> S;
+    fold_tile<R, FOLD_TY>(PlaneSrc{G, (size_t)w * h}, Cn, w, h, cw, ch, best, acc, winner, first,
+                          idx_dev ? *idx_dev : idx, wt, chroma, en_out, blockIdx.x * 64, blockIdx.y * FOLD_TY, S);
 }
 
 // L = G - EXPAND(next) (pyramid.rs:208-211), or the collapse step out = L + EXPAND(next)
@@ -292,7 +298,7 @@ static void fuser_push(lsh_fuser *f, const float *g0, const unsigned *idx_dev = 
     Wt9 wt{};
     binomial(f->p.energy_radius, wt.k);
     for (size_t l = 0; l < nsel; l++) {
-        dim3 g(cdiv(f->lw[l], 64), cdiv(f->lh[l], 16));
+        dim3 g(cdiv(f->lw[l], 64), cdiv(f->lh[l], FOLD_TY));
         bool is_depth = l == f->depth_level;
         float *en_out = (f->halo_on && l == f->guide) ? f->en_guide.p : nullptr;
         fold_dispatch((int)f->p.energy_radius, g, lvl(l), lvl(l + 1), (int)f->lw[l], (int)f->lh[l], (int)f->lw[l + 1],
