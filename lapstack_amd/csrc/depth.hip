@@ -37,6 +37,46 @@ struct TapTable {
     float wt[MAX_TAPS];
 };
 
+// The ring difference filter of the default measure, RDF(r_in 1, r_out 3), at
+// scale 1 (2 x 2 pixels a cell) with its rows as compile-time constants: in
+// the reference's order (depth.rs:417-451) the taps are 7 rows, each its disk
+// taps (dx = -D..D, weight wd) then its ring taps (the rest of dx = -M..M,
+// weight wr), both ascending.  The host checks the order has this form before
+// it selects this path.  A row's window samples for both pixels of a cell are
+// read from LDS once (6 or 8 instead of 2 x (2D+1 + ring)), and each pixel's
+// sum takes its taps in the reference's order.
+template <int D, int M>
+__device__ __forceinline__ void rdf13_row(const float *__restrict__ row, float wd, float wr, float (&a)[2]) {
+    float seg[2 + 2 * M];  // seg[M + dx + j]: the sample at dx of pixel j
+#pragma unroll
+    for (int q = 0; q < 2 + 2 * M; q++) seg[q] = row[q - M];
+#pragma unroll
+    for (int dx = -D; dx <= D; dx++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) a[j] += wd * seg[M + dx + j];
+#pragma unroll
+    for (int dx = -M; dx <= M; dx++)
+        if (dx < -D || dx > D)
+#pragma unroll
+            for (int j = 0; j < 2; j++) a[j] += wr * seg[M + dx + j];
+}
+
+// the cell's two pixels of window row ly from column lx: |RDF| summed in x order
+__device__ __forceinline__ float rdf13_pair(const float *__restrict__ Y, int PW, int ly, int lx, float wd, float wr) {
+    float a[2] = {0.0f, 0.0f};
+    rdf13_row<0, 2>(Y + (ly - 1) * PW + lx, wd, wr, a);
+    rdf13_row<1, 3>(Y + ly * PW + lx, wd, wr, a);
+    rdf13_row<0, 2>(Y + (ly + 1) * PW + lx, wd, wr, a);
+    rdf13_row<-1, 0>(Y + (ly - 3) * PW + lx, wd, wr, a);
+    rdf13_row<-1, 2>(Y + (ly - 2) * PW + lx, wd, wr, a);
+    rdf13_row<-1, 2>(Y + (ly + 2) * PW + lx, wd, wr, a);
+    rdf13_row<-1, 0>(Y + (ly + 3) * PW + lx, wd, wr, a);
+    float rs = 0.0f;
+    rs += fabsf(a[0]);
+    rs += fabsf(a[1]);
+    return rs;
+}
+
 // depth.rs:417-495 + :174-197: luma, ring difference filter (or SML), |.|, k x k
 // block mean, one output cell per thread, the luma tile staged in LDS.
 // depth.rs:1097-1100: the focus measure of the luma (depth.rs:160-165) block-
@@ -44,11 +84,13 @@ struct TapTable {
 // luma of its pixels and their halo is staged in LDS once (reflect-101 at the
 // borders), then each thread averages the measure over CY cells of its column.
 // ONE: the input is the luma plane itself (written with the same expression by
-// the warp that made the planes), not the three planes.
-template <bool ONE>
+// the warp that made the planes), not the three planes.  R13: the measure is
+// RDF(1, 3) at scale 1, whose whole cells go through rdf13_pair.
+template <bool ONE, bool R13>
 __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k, int CY,
-                                                     const TapTable tab, Taps T, float *__restrict__ out, int dw,
-                                                     int dh, const unsigned *__restrict__ out_idx) {
+                                                     const TapTable tab, Taps T, float wd, float wr,
+                                                     float *__restrict__ out, int dw, int dh,
+                                                     const unsigned *__restrict__ out_idx) {
     if (out_idx) out += (size_t)out_idx[0] * dw * dh;
     extern __shared__ float Y[];
     const int TX = 64, TY = 4 * CY, R = T.halo;
@@ -90,6 +132,13 @@ __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ p
         if (oy >= dh) return;
         int x0 = ox * k, y0 = oy * k, x1 = min(x0 + k, w), y1 = min(y0 + k, h);
         float o = 0.0f;
+        if (R13 && x1 - x0 == 2 && y1 - y0 == 2) {
+            const int lx = x0 - px0, ly = y0 - py0;
+            o += rdf13_pair(Y, PW, ly, lx, wd, wr);
+            o += rdf13_pair(Y, PW, ly + 1, lx, wd, wr);
+            out[(size_t)oy * dw + ox] = o / 4.0f;
+            continue;
+        }
         for (int y = y0; y < y1; y++) {
             float rs = 0.0f;
             int ly = y - py0;
@@ -146,6 +195,27 @@ static void rdf_taps_ordered(uint32_t r_in, uint32_t r_out, std::vector<int32_t>
                 dx.push_back(std::get<1>(t));
                 wt.push_back(std::get<2>(t));
             }
+}
+
+// Whether RDF(1, 3)'s taps in the reference's order are the rows rdf13_pair
+// takes (dy, disk half-width D or -1, ring half-width M), each row its disk taps
+// then its ring taps, both ascending, with one disk and one ring weight.
+static bool rdf13_order_holds() {
+    static const int rows[7][3] = {{-1, 0, 2}, {0, 1, 3}, {1, 0, 2}, {-3, -1, 0}, {-2, -1, 2}, {2, -1, 2}, {3, -1, 0}};
+    std::vector<int32_t> dy, dx;
+    std::vector<float> wt;
+    rdf_taps_ordered(1, 3, dy, dx, wt);
+    size_t t = 0;
+    for (const auto &r : rows) {
+        for (int x = -r[1]; x <= r[1]; x++, t++)
+            if (t >= wt.size() || dy[t] != r[0] || dx[t] != x || wt[t] != wt[0]) return false;
+        for (int x = -r[2]; x <= r[2]; x++)
+            if (x < -r[1] || x > r[1]) {
+                if (t >= wt.size() || dy[t] != r[0] || dx[t] != x || wt[t] != wt.back()) return false;
+                t++;
+            }
+    }
+    return t == wt.size();
 }
 
 // The focus slice of planes (w x h) with a holder of the measure's taps
@@ -210,13 +280,24 @@ static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBu
             tab.wt[t] = wt[t];
         }
     }
-    if (luma)
-        launch_b("focus_slice", 4.0 * w * h + 4.0 * dw * dh, k_focus_slice<true>, dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)),
-                 dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY, tab, T, out, dw, dh, out_idx);
+    // the specialised RDF(1, 3) at scale 1, when the reference's order has the form it assumes
+    const bool r13 = p.focus_kind == 0 && p.r_in == 1 && p.r_out == 3 && k == 2 && rdf13_order_holds();
+    float wd = 0.0f, wr = 0.0f;
+    if (r13) {
+        wd = tab.wt[0];             // the first tap is a disk tap
+        wr = tab.wt[ntaps - 1];     // the last a ring tap
+    }
+    const double by = (luma ? 4.0 : 12.0) * w * h + 4.0 * dw * dh;
+    const dim3 grid(cdiv(dw, 64), cdiv(dh, 4 * CY)), blk(64, 4);
+    const float *src = planes;
+    if (luma && r13)
+        launch_b("focus_slice", by, k_focus_slice<true, true>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
+    else if (luma)
+        launch_b("focus_slice", by, k_focus_slice<true, false>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
+    else if (r13)
+        launch_b("focus_slice", by, k_focus_slice<false, true>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
     else
-        launch_b("focus_slice", 12.0 * w * h + 4.0 * dw * dh, k_focus_slice<false>,
-                 dim3(cdiv(dw, 64), cdiv(dh, 4 * CY)), dim3(64, 4), shm, (const float *)planes, (int)w, (int)h, k, CY,
-                 tab, T, out, dw, dh, out_idx);
+        launch_b("focus_slice", by, k_focus_slice<false, false>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
 }
 
 // ----------------------------------------------------------------- helpers
