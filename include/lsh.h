@@ -162,7 +162,8 @@ uint64_t lsh_frames_last_evals(lsh_frames *f);
  * the collective passes at level l (each pass evaluates up to 9 points), and
  * ticks[16..20) the ticks per phase (matrix, pixels, combine, steering), and
  * ticks[20 + 8 l + q] those of phase q at level l (q 4-6: the combine's
- * publishing, waiting and folding; cap 148 for all); the phases
+ * publishing, waiting and folding; q 7: the steering's branch, before an
+ * iteration's set-up; cap 148 for all); the phases
  * are timed only while lsh_prof_enable is on or LSH_SEARCH_PHASES=1 is set in
  * the environment (zero otherwise: the clock reads would lengthen every pass). */
 char *lsh_frames_search_stats(lsh_frames *f, uint64_t *evals, uint64_t *ticks, uint32_t cap);

@@ -322,7 +322,8 @@ struct NmShared {
     unsigned lev_evals[NM_MAXL], lev_passes[NM_MAXL];
     uint64_t ph[4], tmark;  // block 0's wall-clock ticks per phase: inverse, pixels, combine, steering
     // per level: the four phases, then the combine's parts (publishing the
-    // partials, waiting for the other blocks, folding), one spare
+    // partials, waiting for the other blocks, folding), then the steering's
+    // branch (up to an iteration's set-up)
     uint64_t lph[NM_MAXL][NM_NPH];
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
     int hwin[NM_HMAXT][7];  // per tile of a held level: its window (x0, y0, w, h, staged), held, offset in hb
@@ -982,6 +983,7 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
             continue;
         }
         if (sh.stop || it == 200) break;
+        if (K.phases && t0) sh.lph[lvl][7] += wall_clock64() - sh.tmark;  // the branch's share of the steering
         it++;
         // the iteration's set-up, spread over threads (every step keeps the
         // reference's arithmetic and order per coordinate):
