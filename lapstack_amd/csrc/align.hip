@@ -263,8 +263,7 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
     if constexpr (CR) {
         sincos_cr(s[3], &sn, &c);
     } else {
-        c = cos(s[3]);
-        sn = sin(s[3]);
+        sincos(s[3], &sn, &c);  // one argument reduction for both, the same bits as sin and cos
     }
     double sc = s[2];
     double a = sc * c, b = sc * (c * s[5] - sn * s[4]), d = sc * sn, e = sc * (sn * s[5] + c * s[4]);
@@ -333,6 +332,7 @@ struct NmShared {
     int rank;               // this block's place among them (nm_B)
     double tolv[8];         // the level's convergence tolerances
     int stop;
+    int ctl;                // the next pass's point count (0: the level is done), from wave 0
     unsigned long long dig; // the digest of every cost this block has folded
     int ntrace;             // the costs traced so far (rank 0)
 };
@@ -354,16 +354,14 @@ __device__ __forceinline__ double rms_of(double sd, double sd2, double cnt) {
     return sqrt(vv > 0.0 ? vv : 0.0);
 }
 
-// sh.sums[k] (k < nv) = sh.red[k][0] + ... + sh.red[k][NM_WAVES-1] (the waves'
-// accumulators, in wave order)
-__device__ __forceinline__ void fold_waves(NmShared &sh, int nv) {
-    __syncthreads();
-    if ((int)threadIdx.x < nv) {
-        double t = 0.0;
-        for (int q = 0; q < NM_WAVES; q++) t += sh.red[threadIdx.x][q];
-        sh.sums[threadIdx.x] = t;
-    }
-    __syncthreads();
+// Orders LDS accesses between the lanes of wave 0, which runs the search's
+// control alone: the lanes execute in step and a wave's LDS operations complete
+// in issue order, so only the compiler has to be kept from moving accesses
+// across this point.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ void zero_waves(NmShared &sh) {
@@ -423,7 +421,8 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     const int nty = (th + TH - 1) / TH, ntiles = MODE == 1 ? 1 : ntx * nty;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const bool act = MODE == 1 || ty < TH;  // the wave has a row of the tile
-    if (MODE != 0 || (held && !hold)) __syncthreads();  // the cleared accumulators (no barrier in the loop then)
+    // (the accumulators were cleared, and the pass's maps and bound written, by
+    // wave 0 before the barrier that starts the pass)
     const int B = nm_B(sh);
     for (int t = solo ? 0 : B; t < ntiles; t += solo ? 1 : G) {
         const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
@@ -590,233 +589,201 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     }
 }
 
-// One collective pass: the costs (align.rs:581-635) of the np points
-// sh.PT[0..np) at level lvl into sh.PC[0..np).  The level is walked in 64-wide
-// destination tiles (nm_tiles); each tile's source window (the union over the
-// points of the bounding box of the tile corners' source points, plus the
-// 4 x 4 support) is staged in LDS when it fits, and every tap inside it is
-// read from there, any other from global memory.  solo: block 0 alone over
-// all tiles; otherwise every block over its grid-strided share, the blocks'
-// partial sums combined by every block in the same order.  Batching the
-// points does not change any cost: each point's sums are the ones a pass of
-// its own would form.  resident (solo levels small enough): the whole target
-// level and the reference level are staged in boxs once for the level, so a
-// pass reads no global memory.  held: a grid level whose tiles the block holds
-// in hb (nm_tiles); hold: the level's first pass, which stages them.  The
-// taps and the samples they read are the same in every case (a window holds
-// the level's own samples, and a tap outside it is read from memory); the
-// order of the sums follows the level's tiling.
-__device__ void nm_pass(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np, bool solo,
-                        bool resident, bool held, bool hold, bool refh, double *part, unsigned *ctr, unsigned *err,
-                        unsigned &gen) {
+// The search's control runs in wave 0 alone, between two block barriers per
+// pass: after the pixels, wave 0 publishes the pass's sums and waits for the
+// other participants' (nm_publish), every wave folds them (nm_fold), and wave 0
+// takes the costs (nm_costs), steers
+// the simplex (nm_steer) and prepares the next pass's maps and window bound
+// (nm_prep); the other waves wait at the barrier that starts the next pass.
+// Within wave 0 the vertices, coordinates and points are spread over lanes.
+
+// wave 0: the maps of the pass's points sh.PT[0..np) (the inverse homographies
+// the warps sample through), and, when the pass stages windows, its bound: the
+// displacement (source - destination) of an affine map is affine in the pixel,
+// so over the level it is bounded by its values at the 4 corners (a lane per
+// point and corner; min and max are exact in any order).  A projective point
+// (or a wild one) turns the staging off for the pass.
+__device__ void nm_prep(const NmConst &K, NmShared &sh, int lvl, int np, bool bound) {
+    const int lane = threadIdx.x;
     const int tw = K.lw[lvl], th = K.lh[lvl];
-    const bool tm = K.phases != 0;  // block-uniform
-    const uint64_t tq = tm ? wall_clock64() : 0;
-    if (tm && threadIdx.x == 0) {
-        sh.ph[3] += tq - sh.tmark;
-        sh.lph[lvl][3] += tq - sh.tmark;
-    }
-    if ((int)threadIdx.x < np) {
+    if (lane < np) {
         double v[8];
         for (int k = 0; k < 8; k++) v[k] = sh.cur[k];
-        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = sh.PT[threadIdx.x][k];
-        sim_inverse_dev<false>(v, tw, th, sh.inv[threadIdx.x]);
+        for (int k = 0; k < K.nfree; k++) v[K.free_idx[k]] = sh.PT[lane][k];
+        sim_inverse_dev<false>(v, tw, th, sh.inv[lane]);
     }
-    __syncthreads();
-    const uint64_t tq1 = tm ? wall_clock64() : 0;
-    if (tm && threadIdx.x == 0) {
-        sh.ph[0] += tq1 - tq;
-        sh.lph[lvl][0] += tq1 - tq;
+    if (!bound) return;
+    wsync();
+    double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
+    int ok = 1;
+    if (lane < 4 * np) {
+        const int p = lane >> 2, c = lane & 3;
+        Inv9 iv;
+        for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
+        if (!(iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0)) ok = 0;
+        const int cx = (c & 1) ? tw - 1 : 0, cy = (c & 2) ? th - 1 : 0;
+        double sx, sy;
+        src_point(iv, 1, cx, cy, sx, sy);
+        mnx = fmin(mnx, sx - cx);
+        mxx = fmax(mxx, sx - cx);
+        mny = fmin(mny, sy - cy);
+        mxy = fmax(mxy, sy - cy);
     }
-    zero_waves(sh);  // ordered before its first use by the tile loop's barriers
-    // The displacement (source - destination) of an affine map is affine in the
-    // pixel, so over the level it is bounded by its values at the 4 corners: one
-    // bound for the pass, from which every tile's window follows.  A projective
-    // point (or a wild one) turns the staging off for the pass.
-    if (threadIdx.x == 0 && !resident && (!held || hold || sh.restage)) {
-        double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
-        int ok = 1;
-        for (int p = 0; p < np; p++) {
-            Inv9 iv;
-            for (int k = 0; k < 9; k++) iv.m[k] = sh.inv[p][k];
-            if (!(iv.m[6] == 0.0 && iv.m[7] == 0.0 && iv.m[8] == 1.0)) ok = 0;
-            for (int c = 0; c < 4; c++) {
-                const int cx = (c & 1) ? tw - 1 : 0, cy = (c & 2) ? th - 1 : 0;
-                double sx, sy;
-                src_point(iv, 1, cx, cy, sx, sy);
-                mnx = fmin(mnx, sx - cx);
-                mxx = fmax(mxx, sx - cx);
-                mny = fmin(mny, sy - cy);
-                mxy = fmax(mxy, sy - cy);
-            }
-        }
-        if (!(mnx > -256.0 && mxx < 256.0 && mny > -256.0 && mxy < 256.0)) ok = 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
+        mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
+        mny = fmin(mny, __shfl_xor(mny, o, 64));
+        mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+    }
+    ok = __all(ok) && mnx <= mxx && mny <= mxy && mnx > -256.0 && mxx < 256.0 && mny > -256.0 && mxy < 256.0;
+    if (lane == 0) {
         sh.box[0] = ok ? (int)floor(mnx) - 1 : 0;  // window = tile + [lo, hi] on each axis
         sh.box[1] = ok ? (int)floor(mxx) + 2 : 0;
         sh.box[2] = ok ? (int)floor(mny) - 1 : 0;
         sh.box[3] = ok ? (int)floor(mxy) + 2 : 0;
         sh.box[4] = ok;
     }
-    if (resident)
-        nm_tiles<1>(K, sh, boxs, hb, lvl, np, solo, false, false, false);
-    else
-        nm_tiles<0>(K, sh, boxs, hb, lvl, np, solo, held, hold, refh);
-    fold_waves(sh, 3 * np);
-    const uint64_t tq2 = tm ? wall_clock64() : 0;
-    if (threadIdx.x == 0) {
-        if (tm) {
-            sh.ph[1] += tq2 - tq1;
-            sh.lph[lvl][1] += tq2 - tq1;
-        }
-        sh.lev_passes[lvl]++;
-    }
-    if (solo) {
-        if ((int)threadIdx.x < np)
-            sh.PC[threadIdx.x] = rms_of(sh.sums[3 * threadIdx.x], sh.sums[3 * threadIdx.x + 1], sh.sums[3 * threadIdx.x + 2]);
-        if (tm && threadIdx.x == 0) {
-            sh.tmark = wall_clock64();
-            sh.ph[2] += sh.tmark - tq2;
-            sh.lph[lvl][2] += sh.tmark - tq2;
-        }
-        __syncthreads();
-        return;
-    }
-    const unsigned slot = gen & 1;
+}
+
+// The combine of a pass's sums, after its pixels.  Wave 0 sums each value over
+// the block's waves (in wave order); on a solo level that is the pass's sum.
+// On the grid, wave 0 publishes the block's partial sums (one lane per value,
+// one store instruction) as agent-scope atomic stores, written through to the
+// level every XCD reads coherently, stored value-major (value k of block b at
+// k * G + b, so that one value of 64 consecutive blocks is 512 contiguous
+// bytes); their completion is waited for explicitly (s_waitcnt vmcnt(0): on
+// gfx9 a store leaves the vector memory count only once the write is
+// acknowledged at its scope) before the block's arrival flag is raised to the
+// number of passes it has completed; the workgroup release fence keeps the
+// compiler from moving the stores past the flag, but emits no wait of its own
+// on this target.  Wave 0 reads all flags in one round trip per poll (a lane
+// per block, up to NM_MAXG / 64 each).  Then (nm_fold) wave w loads value w
+// (and w + 16, ...) of every block with agent-scope atomic loads, issued after
+// the barrier that follows the wait, lane l holding blocks l, l + 64, ..., and
+// sums it over them in block order and then over the lanes by a fixed tree:
+// an order that depends on lane indices only, so every block holds the
+// identical costs with no broadcast.  A block a pass ahead writes the other
+// slot; none can be two ahead before every block has arrived at the pass
+// between.  No XCD's L2 is written back or invalidated for the exchange: an
+// agent-scope release / acquire does both to the whole L2 on every pass, which
+// here (the side work's dirty lines in it, the level's samples lost from it)
+// cost more than a small level's pass.  A flag per block (not a shared
+// counter) keeps the arrival a plain store.
+
+// wave 0: the block's sums of the pass (solo: into sh.sums); on the grid their
+// publication and the wait for every participant's.  False when the search
+// stops (another block's failure or a wait's time-out).
+__device__ bool nm_publish(const NmConst &K, NmShared &sh, int lvl, int np, bool solo, double *part, unsigned *ctr,
+                           unsigned *err, unsigned gen) {
+    const int lane = threadIdx.x;
     const int nv = 3 * np;
+    const bool tm = K.phases != 0;
+    double t = 0.0;
+    if (lane < nv) {
+        for (int q = 0; q < NM_WAVES; q++) t += sh.red[lane][q];
+        for (int q = 0; q < NM_WAVES; q++) sh.red[lane][q] = 0.0;  // cleared for the next pass
+    }
+    if (lane == 0) sh.lev_passes[lvl]++;
+    if (solo) {
+        if (lane < nv) sh.sums[lane] = t;
+        return true;
+    }
+    const uint64_t tq2 = tm ? wall_clock64() : 0;
     const int G = nm_G(sh), B = nm_B(sh);
-    double *pp = part + (size_t)slot * G * 3 * NM_MAXP;
-    if (threadIdx.x < 64) {
-        // Wave 0 publishes the block's partials and waits for everyone's.  The
-        // partials are agent-scope atomic stores (one lane each, one store
-        // instruction): written through to the level every XCD reads
-        // coherently.  Their completion is waited for explicitly (s_waitcnt
-        // vmcnt(0): on gfx9 a store leaves the vector memory count only once
-        // the write is acknowledged at its scope) before the block's arrival
-        // flag is raised to the number of passes it has completed; the
-        // workgroup release fence keeps the compiler from moving the stores
-        // past the flag, but emits no wait of its own on this target.  Every
-        // reader loads the partials with agent-scope atomic loads once it has
-        // seen every flag at the pass.  So no XCD's L2 has to be written back
-        // or invalidated for them: an agent-scope release / acquire does both
-        // to the whole L2 on every pass, which here (the side work's dirty
-        // lines in it, the level's samples lost from it) cost more than a
-        // small level's pass.  A flag per block (not a shared counter) keeps
-        // the arrival a plain store, and the wave reads all flags in one
-        // round trip per poll (a lane per block, up to NM_MAXG / 64 each).
-        const int lane = threadIdx.x;
-        if (lane < nv)
-            __hip_atomic_store(&pp[(size_t)B * 3 * NM_MAXP + lane], sh.sums[lane], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_s_waitcnt(0);
-        unsigned *arr = ctr + NM_FL_ARR;
-        const unsigned target = gen + 1;
-        if (lane == 0) __hip_atomic_store(&arr[B], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (tm && lane == 0) {
-            const uint64_t tp = wall_clock64();
-            sh.lph[lvl][4] += tp - tq2;
-            sh.tmark = tp;
+    if (lane < nv)
+        __hip_atomic_store(&part[(size_t)(gen & 1) * G * 3 * NM_MAXP + (size_t)lane * G + B], t, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    unsigned *arr = ctr + NM_FL_ARR;
+    const unsigned target = gen + 1;
+    if (lane == 0) __hip_atomic_store(&arr[B], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t tp = 0;
+    if (tm) {
+        tp = wall_clock64();
+        if (lane == 0) sh.lph[lvl][4] += tp - tq2;
+    }
+    // bounded, and left when any block has raised the error flag (looked
+    // at, with the clock, every 64 polls; the clock starts at the first look)
+    uint64_t t_in = 0;
+    unsigned spins = 0;
+    bool stop = false;
+    for (;;) {
+        unsigned v[NM_MAXG / 64];
+#pragma unroll
+        for (int j = 0; j < NM_MAXG / 64; j++) {
+            const int b = lane + 64 * j;
+            v[j] = b < G ? __hip_atomic_load(&arr[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
         }
-        // bounded, and left when any block has raised the error flag (looked
-        // at, with the clock, every 64 polls; the clock starts at the first look)
-        uint64_t t_in = 0;
-        unsigned spins = 0;
-        for (;;) {
-            unsigned v[NM_MAXG / 64];
+        int all = 1;
 #pragma unroll
-            for (int j = 0; j < NM_MAXG / 64; j++) {
-                const int b = lane + 64 * j;
-                v[j] = b < G ? __hip_atomic_load(&arr[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
+        for (int j = 0; j < NM_MAXG / 64; j++) all &= v[j] >= target;
+        if (__all(all)) break;
+        __builtin_amdgcn_s_sleep(1);
+        if ((++spins & 63u) == 0) {
+            // another block's failure, or this wait's time-out, ends the search
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                stop = true;
+                break;
             }
-            int all = 1;
-#pragma unroll
-            for (int j = 0; j < NM_MAXG / 64; j++) all &= v[j] >= target;
-            if (__all(all)) break;
-            __builtin_amdgcn_s_sleep(1);
-            if ((++spins & 63u) == 0) {
-                // another block's failure, or this wait's time-out, ends the search
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    if (lane == 0) sh.stop = 1;
-                    break;
-                }
-                const uint64_t now = wall_clock64();
-                if (spins == 64u) t_in = now;
-                if (now - t_in > K.wait_ticks) {
-                    if (lane == 0) {
-                        atomicOr(err, 1u);
-                        sh.stop = 1;
-                    }
-                    break;
-                }
+            const uint64_t now = wall_clock64();
+            if (spins == 64u) t_in = now;
+            if (now - t_in > K.wait_ticks) {
+                if (lane == 0) atomicOr(err, 1u);
+                stop = true;
+                break;
             }
         }
     }
-    if (tm && threadIdx.x == 0) {
-        const uint64_t t_all = wall_clock64();
-        sh.lph[lvl][5] += t_all - sh.tmark;
-        sh.tmark = t_all;
-    }
-    __syncthreads();
-    {
-        // Every block folds all blocks' partial sums itself, in an order that
-        // depends on thread indices only, so all hold the identical costs with
-        // no broadcast.  All partials are loaded at once (up to 4 independent
-        // loads per thread) into LDS (the window buffer: the pixels are done),
-        // then wave w sums value k = w, w + 16, ... over the blocks, lanes in
-        // block order.  A block a pass ahead writes the other slot; none can be
-        // two ahead before every block has arrived at the pass between.
-        double *stg = (double *)boxs;
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        const int kchunk = min(nv, (NM_BOX / 2) / G);
-        for (int k0 = 0; k0 < nv; k0 += kchunk) {
-            const int kn = min(kchunk, nv - k0), total = G * kn;
-            for (int i0 = 0; i0 < total; i0 += 4 * NM_THREADS) {
-                double v[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int i = i0 + u * NM_THREADS + threadIdx.x;
-                    // agent-scope atomic loads, issued after the wait has seen every arrival
-                    v[u] = i < total ? __hip_atomic_load(&pp[(size_t)(i / kn) * 3 * NM_MAXP + k0 + i % kn],
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int i = i0 + u * NM_THREADS + threadIdx.x;
-                    if (i < total) stg[(i % kn) * G + i / kn] = v[u];
-                }
-            }
-            __syncthreads();
-            for (int k = wv; k < kn; k += NM_WAVES) {
-                double acc = 0.0;
-                for (int b = lane; b < G; b += 64) acc += stg[k * G + b];
-                acc = wave_sum(acc);
-                if (lane == 0) sh.sums[k0 + k] = acc;
-            }
-            __syncthreads();
-        }
-        if ((int)threadIdx.x < np) {
-            const int q = threadIdx.x;
-            sh.PC[q] = rms_of(sh.sums[3 * q], sh.sums[3 * q + 1], sh.sums[3 * q + 2]);
-        }
-    }
-    gen++;
-    if (threadIdx.x == 0) {
-        // the digest of the sums this block folded (FNV-1a over their bits;
-        // final since the fold's last barrier): every participant must end a
-        // search with the same one
-        unsigned long long d = sh.dig;
-        for (int k = 0; k < nv; k++) d = (d ^ (unsigned long long)__double_as_longlong(sh.sums[k])) * 1099511628211ull;
-        sh.dig = d;
+    if (lane == 0) {
+        if (stop) sh.stop = 1;
         if (tm) {
-            const uint64_t tf = wall_clock64();
-            sh.lph[lvl][6] += tf - sh.tmark;
-            sh.tmark = tf;
-            sh.ph[2] += sh.tmark - tq2;
-            sh.lph[lvl][2] += sh.tmark - tq2;
+            sh.tmark = wall_clock64();
+            sh.lph[lvl][5] += sh.tmark - tp;
         }
     }
-    __syncthreads();
+    return !stop;
+}
+
+// every wave, once every participant's partials are in place: value k = w,
+// w + 16, ... of the pass summed over the blocks into sh.sums[k]
+__device__ void nm_fold(NmShared &sh, int np, const double *part, unsigned gen) {
+    const int nv = 3 * np, G = nm_G(sh);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double *pp = part + (size_t)(gen & 1) * G * 3 * NM_MAXP;
+    for (int k = wv; k < nv; k += NM_WAVES) {
+        // every load issued before the first is used: unconditional, from an
+        // address clamped into the value's run (a block past the grid is
+        // loaded and left out of the sum)
+        double v[NM_MAXG / 64];
+#pragma unroll
+        for (int u = 0; u < NM_MAXG / 64; u++)
+            v[u] = __hip_atomic_load(&pp[(size_t)k * G + min(lane + 64 * u, G - 1)], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        double a = 0.0;
+#pragma unroll
+        for (int u = 0; u < NM_MAXG / 64; u++)
+            if (lane + 64 * u < G) a += v[u];
+        a = wave_sum(a);
+        if (lane == 0) sh.sums[k] = a;
+    }
+}
+
+// wave 0: the costs (align.rs:630-634) of the pass's np points into sh.PC;
+// on the grid the digest of the sums this block folded (FNV-1a over their
+// bits): every participant must end a search with the same one
+__device__ void nm_costs(const NmConst &K, NmShared &sh, int lvl, int np, bool solo) {
+    const int lane = threadIdx.x;
+    if (!solo && lane == 0) {
+        unsigned long long d = sh.dig;
+        for (int k = 0; k < 3 * np; k++)
+            d = (d ^ (unsigned long long)__double_as_longlong(sh.sums[k])) * 1099511628211ull;
+        sh.dig = d;
+        if (K.phases) sh.lph[lvl][6] += wall_clock64() - sh.tmark;
+    }
+    wsync();
+    if (lane < np) sh.PC[lane] = rms_of(sh.sums[3 * lane], sh.sums[3 * lane + 1], sh.sums[3 * lane + 2]);
+    wsync();
 }
 
 // A counted cost into the trace (thread 0 of rank 0, when tracing), in the
@@ -833,38 +800,193 @@ __device__ __forceinline__ double clampd_dev(double v, double lo, double hi) {
     return v;
 }
 
+#define CLAMPK(v, k) clampd_dev((v), sh.lo[k], sh.hi[k])
+enum { NM_INIT, NM_ITER, NM_EXPAND, NM_CONTRACT, NM_SHRINK };
+
+// wave 0: one step of a level's bounded Nelder-Mead (align.rs:796-883) once a
+// pass's costs are in sh.PC: the branch the pass was for, and when the
+// iteration is over the next one's set-up (the stable sort, the convergence
+// test, the centroid and the trial points).  Returns the next pass's point
+// count, or 0 when the level is done.  The
+// iterates and the counted evaluations are the reference's: on the grid levels
+// where a pass costs mostly synchronisation (spec), an iteration evaluates the
+// reflection, the expansion and the contraction in one pass (the latter two
+// are fixed by the centroid and the worst vertex before the reflection's cost
+// is known) and then takes the reference's branch, and a speculative cost the
+// branch does not use is not counted; elsewhere they are evaluated one by one
+// as the reference does.  A shrink evaluates its n new vertices in one pass.
+__device__ int nm_steer(const NmConst &K, NmShared &sh, int lvl, int &state, int &it, bool spec) {
+    const int n = K.nfree, lane = threadIdx.x;
+    const bool l0 = lane == 0, tm = K.phases != 0;
+    const uint64_t t_c = tm ? wall_clock64() : 0;
+    int next = -1;
+    if (state == NM_INIT) {
+        if (lane <= n) sh.FV[lane] = sh.PC[lane];
+        if (l0) {
+            for (int i = 0; i <= n; i++) nm_trace(K, sh, sh.PC[i]);
+            sh.lev_evals[lvl] += n + 1;
+        }
+    } else if (state == NM_ITER) {
+        const double fr = sh.PC[0], fe = sh.PC[1], fc = sh.PC[2];
+        const int br = fr < sh.FV[0] ? 0 : (fr < sh.FV[n - 1] ? 1 : 2);
+        const int bc = spec && br == 2 ? (fc < sh.FV[n] ? 2 : 3) : br;
+        wsync();  // every lane has read the branch's inputs
+        const double *src = nullptr;
+        double fv = 0.0;
+        if (br == 1) {  // reflect
+            src = sh.XR;
+            fv = fr;
+        } else if (!spec) {  // the expansion or the contraction is the next pass
+            if (lane < n) sh.PT[0][lane] = br == 0 ? sh.XE[lane] : sh.XC[lane];
+        } else if (br == 0) {  // expand (align.rs:838-849)
+            src = fe < fr ? sh.XE : sh.XR;
+            fv = fe < fr ? fe : fr;
+        } else if (bc == 2) {  // contract (align.rs:853-862)
+            src = sh.XC;
+            fv = fc;
+        }
+        if (src) {
+            if (lane < n) sh.S[n][lane] = src[lane];
+            if (l0) sh.FV[n] = fv;
+        }
+        if (l0) {
+            sh.FR = fr;
+            sh.lev_evals[lvl] += 1;
+            nm_trace(K, sh, fr);
+            if (spec && br != 1) {
+                sh.lev_evals[lvl] += 1;
+                nm_trace(K, sh, br == 0 ? fe : fc);
+            }
+        }
+        if (!spec && br != 1) next = br == 0 ? NM_EXPAND : NM_CONTRACT;
+        if (spec && bc == 3) next = NM_SHRINK;
+    } else if (state == NM_EXPAND) {
+        const double fe = sh.PC[0], fr = sh.FR;
+        const double *src = fe < fr ? sh.XE : sh.XR;
+        if (lane < n) sh.S[n][lane] = src[lane];
+        if (l0) {
+            sh.FV[n] = fe < fr ? fe : fr;
+            sh.lev_evals[lvl] += 1;
+            nm_trace(K, sh, fe);
+        }
+    } else if (state == NM_CONTRACT) {
+        const double fc = sh.PC[0];
+        const bool take = fc < sh.FV[n];
+        wsync();
+        if (take) {
+            if (lane < n) sh.S[n][lane] = sh.XC[lane];
+            if (l0) sh.FV[n] = fc;
+        }
+        if (l0) {
+            sh.lev_evals[lvl] += 1;
+            nm_trace(K, sh, fc);
+        }
+        if (!take) next = NM_SHRINK;
+    } else {  // NM_SHRINK: the n new vertices' costs
+        if (lane >= 1 && lane <= n) sh.FV[lane] = sh.PC[lane - 1];
+        if (l0) {
+            for (int i = 1; i <= n; i++) nm_trace(K, sh, sh.PC[i - 1]);
+            sh.lev_evals[lvl] += n;
+        }
+    }
+    wsync();
+    if (next == NM_SHRINK && lane < n * n) {  // shrink towards the best (align.rs:863-870)
+        const int i = 1 + lane / n, k = lane % n;
+        const double v = CLAMPK(sh.S[0][k] + 0.5 * (sh.S[i][k] - sh.S[0][k]), k);
+        sh.S[i][k] = v;
+        sh.PT[i - 1][k] = v;
+    }
+    wsync();
+    const bool stop = sh.stop != 0;
+    if (next >= 0 && !stop) {
+        state = next;
+        if (tm && l0) {
+            const uint64_t t_e = wall_clock64();
+            sh.lph[lvl][7] += t_e - t_c;
+            sh.ph[3] += t_e - t_c;
+            sh.lph[lvl][3] += t_e - t_c;
+        }
+        return next == NM_SHRINK ? n : 1;
+    }
+    if (stop || it == 200) return 0;
+    if (tm && l0) sh.lph[lvl][7] += wall_clock64() - t_c;  // the branch's share of the steering
+    it++;
+    // the iteration's set-up (every step keeps the reference's arithmetic and
+    // order per coordinate):
+    // 1. the stable sort of the vertices by value (align.rs:816-818): vertex
+    //    i goes to its rank (values below it, then equal values before it)
+    if (lane <= n) {
+        const double f = sh.FV[lane];
+        int r = 0;
+        for (int j = 0; j <= n; j++) r += sh.FV[j] < f || (sh.FV[j] == f && j < lane);
+        for (int k = 0; k < n; k++) sh.S2[r][k] = sh.S[lane][k];
+        sh.FV2[r] = f;
+    }
+    wsync();
+    for (int t = lane; t < (n + 1) * 8; t += 64) {
+        const int i = t >> 3, k = t & 7;
+        if (k < n) sh.S[i][k] = sh.S2[i][k];
+        if (k == 0) sh.FV[i] = sh.FV2[i];
+    }
+    wsync();
+    // 2. convergence (align.rs:888-894): the value spread and every vertex
+    //    within tol of the best, a (vertex, coordinate) per lane
+    int ok = 1;
+    if (l0) ok = !(fabs(sh.FV[n] - sh.FV[0]) > 1e-4 * (1.0 + fabs(sh.FV[0])));
+    for (int t = lane; t < (n + 1) * 8; t += 64) {
+        const int i = t >> 3, k = t & 7;
+        if (i >= 1 && k < n && !(fabs(sh.S[i][k] - sh.S[0][k]) <= sh.tolv[k])) ok = 0;
+    }
+    // 3. per coordinate: the centroid of the best n, the reflection, the
+    //    expansion and the contraction (align.rs:823-862)
+    if (lane < n) {
+        const int k = lane;
+        double c = 0.0;
+        for (int i = 0; i < n; i++) c += sh.S[i][k];
+        c /= (double)n;
+        sh.C[k] = c;
+        const double xr = CLAMPK(c + 1.0 * (c - sh.S[n][k]), k);
+        const double xe = CLAMPK(c + 2.0 * (xr - c), k);
+        const double xc = CLAMPK(c + 0.5 * (sh.S[n][k] - c), k);
+        sh.XR[k] = sh.PT[0][k] = xr;
+        sh.XE[k] = sh.PT[1][k] = xe;
+        sh.XC[k] = sh.PT[2][k] = xc;
+    }
+    if (__all(ok)) return 0;
+    state = NM_ITER;
+    wsync();
+    if (tm && l0) {
+        const uint64_t t_e = wall_clock64();
+        sh.ph[3] += t_e - t_c;
+        sh.lph[lvl][3] += t_e - t_c;
+    }
+    return spec ? 3 : 1;
+}
+
 // One level's bounded Nelder-Mead (align.rs:796-883), collective over the
-// block (solo) or the grid; the simplex lives in LDS and thread 0 steers it.
-// The loop is a small state machine around one pass call site.  On the grid
-// levels where a pass costs mostly synchronisation (spec_max), an iteration
-// evaluates the reflection, the expansion and the contraction in one pass
-// (the latter two are fixed by the centroid and the worst vertex before the
-// reflection's cost is known) and then takes the reference's branch;
-// elsewhere it evaluates them one by one as the reference does.  A shrink
-// evaluates its n new vertices in one pass, as does the initial simplex.  The
-// iterates and the counted evaluations are the reference's: a speculative
-// cost the branch does not use is not counted.
+// block (solo) or the grid.  A pass (the costs of up to NM_MAXP points at the
+// level, align.rs:581-635) walks the level in 64-wide destination tiles
+// (nm_tiles); each tile's source window (the union over the points of the
+// bounding box of the tile corners' source points, plus the 4 x 4 support) is
+// staged in LDS when it fits, and every tap inside it is read from there, any
+// other from global memory.  solo: block 0 alone over all tiles; otherwise
+// every block over its grid-strided share, the blocks' partial sums combined by
+// every block in the same order (nm_publish, nm_fold).  Batching the points does not
+// change any cost: each point's sums are the ones a pass of its own would form.
+// resident (solo levels small enough): the whole target level and the
+// reference level are staged in boxs once for the level, so a pass reads no
+// global memory.  held: a grid level whose tiles the block holds in hb
+// (nm_tiles), staged by the level's first pass.  The taps and the samples they
+// read are the same in every case (a window holds the level's own samples, and
+// a tap outside it is read from memory); the order of the sums follows the
+// level's tiling.
 __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, bool solo, double *part,
                          unsigned *ctr, unsigned *err, unsigned &gen) {
-    enum { INIT, ITER, EXPAND, CONTRACT, SHRINK };
     const int n = K.nfree;
-    const bool t0 = threadIdx.x == 0;
+    const bool w0 = threadIdx.x < 64;
+    const int lane = threadIdx.x;  // wave 0's lanes
     const bool spec = !solo && (size_t)K.lw[lvl] * K.lh[lvl] <= (size_t)K.spec_max;
     const double *step = K.step[lvl];
-#define CLAMPK(v, k) clampd_dev((v), sh.lo[k], sh.hi[k])
-    if ((int)threadIdx.x < n) sh.tolv[threadIdx.x] = K.tol[lvl][threadIdx.x];  // read by the iterations' set-up
-    if (t0) {
-        sh.hoff = 0;  // the level's held tiles start over
-        sh.restage = 0;
-        for (int k = 0; k < n; k++) sh.S[0][k] = sh.cur[K.free_idx[k]];
-        for (int i = 0; i < n; i++) {
-            for (int k = 0; k < n; k++) sh.S[i + 1][k] = sh.S[0][k];
-            sh.S[i + 1][i] += step[i];
-            for (int k = 0; k < n; k++) sh.S[i + 1][k] = CLAMPK(sh.S[i + 1][k], k);
-        }
-        for (int i = 0; i <= n; i++)
-            for (int k = 0; k < n; k++) sh.PT[i][k] = sh.S[i][k];
-    }
     // a solo level whose target and reference fit the window buffer together
     // is staged once and read from there by every pass: the target with one
     // edge sample before and two past it on each axis ((w + 3) x (h + 3), as
@@ -891,155 +1013,90 @@ __device__ void nm_level(const NmConst &K, NmShared &sh, float *boxs, float *hb,
             if (i < np3 + npx) boxs[i] = v[u];
         }
     }
-    __syncthreads();
     bool refh = false;
     const bool held = !solo && nm_holdable(K.lw[lvl], K.lh[lvl], nm_G(sh), &refh);
-    int state = INIT, np = n + 1, it = 0;
-    for (;;) {
-        nm_pass(K, sh, boxs, hb, lvl, np, solo, resident, held, held && state == INIT, refh, part, ctr, err, gen);
-        int next = -1;  // a follow-up pass within the iteration, or none
-        if (state == INIT) {
-            if (t0) {
-                for (int i = 0; i <= n; i++) {
-                    sh.FV[i] = sh.PC[i];
-                    nm_trace(K, sh, sh.PC[i]);
+    int state = NM_INIT, it = 0, nxt = n + 1;  // wave 0's
+    if (w0) {
+        if (lane < n) sh.tolv[lane] = K.tol[lvl][lane];  // read by the iterations' set-up
+        if (lane == 0) {
+            sh.hoff = 0;  // the level's held tiles start over
+            sh.restage = 0;
+        }
+        // the initial simplex (align.rs:800-812): the start, and a step along
+        // each free axis from it, clamped to the box; a vertex per lane
+        if (lane <= n)
+            for (int k = 0; k < n; k++) {
+                double v = sh.cur[K.free_idx[k]];
+                if (lane >= 1) {
+                    if (k == lane - 1) v += step[k];
+                    v = CLAMPK(v, k);
                 }
-                sh.lev_evals[lvl] += n + 1;
+                sh.S[lane][k] = v;
+                sh.PT[lane][k] = v;
             }
-        } else if (state == ITER) {
-            // every thread reads the branch's inputs before thread 0 changes any of them
-            const double fr = sh.PC[0], fe = sh.PC[1], fc = sh.PC[2];
-            const int br = fr < sh.FV[0] ? 0 : (fr < sh.FV[n - 1] ? 1 : 2);
-            const int bc = spec && br == 2 ? (fc < sh.FV[n] ? 2 : 3) : br;
-            __syncthreads();
-            if (t0) {
-                sh.FR = fr;
-                sh.lev_evals[lvl] += 1;
-                nm_trace(K, sh, fr);
-                if (br == 1) {  // reflect
-                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XR[k];
-                    sh.FV[n] = fr;
-                } else if (!spec) {  // the expansion or the contraction is the next pass
-                    for (int k = 0; k < n; k++) sh.PT[0][k] = br == 0 ? sh.XE[k] : sh.XC[k];
-                } else if (br == 0) {  // expand (align.rs:838-849)
-                    const double *src = fe < fr ? sh.XE : sh.XR;
-                    for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
-                    sh.FV[n] = fe < fr ? fe : fr;
-                    sh.lev_evals[lvl] += 1;
-                    nm_trace(K, sh, fe);
-                } else if (bc == 2) {  // contract (align.rs:853-862)
-                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XC[k];
-                    sh.FV[n] = fc;
-                    sh.lev_evals[lvl] += 1;
-                    nm_trace(K, sh, fc);
-                } else {
-                    sh.lev_evals[lvl] += 1;
-                    nm_trace(K, sh, fc);
-                }
-            }
-            if (!spec && br != 1) next = br == 0 ? EXPAND : CONTRACT;
-            if (spec && bc == 3) next = SHRINK;
-        } else if (state == EXPAND) {
-            if (t0) {
-                const double fe = sh.PC[0], fr = sh.FR;
-                const double *src = fe < fr ? sh.XE : sh.XR;
-                for (int k = 0; k < n; k++) sh.S[n][k] = src[k];
-                sh.FV[n] = fe < fr ? fe : fr;
-                sh.lev_evals[lvl] += 1;
-                nm_trace(K, sh, fe);
-            }
-        } else if (state == CONTRACT) {
-            const double fc = sh.PC[0];
-            const bool take = fc < sh.FV[n];
-            __syncthreads();
-            if (t0) {
-                if (take) {
-                    for (int k = 0; k < n; k++) sh.S[n][k] = sh.XC[k];
-                    sh.FV[n] = fc;
-                }
-                sh.lev_evals[lvl] += 1;
-                nm_trace(K, sh, fc);
-            }
-            if (!take) next = SHRINK;
-        } else {  // SHRINK
-            if (t0) {
-                for (int i = 1; i <= n; i++) {
-                    sh.FV[i] = sh.PC[i - 1];
-                    nm_trace(K, sh, sh.PC[i - 1]);
-                }
-                sh.lev_evals[lvl] += n;
-            }
-        }
-        if (next == SHRINK && t0)  // shrink towards the best (align.rs:863-870)
-            for (int i = 1; i <= n; i++)
-                for (int k = 0; k < n; k++) {
-                    sh.S[i][k] = CLAMPK(sh.S[0][k] + 0.5 * (sh.S[i][k] - sh.S[0][k]), k);
-                    sh.PT[i - 1][k] = sh.S[i][k];
-                }
-        __syncthreads();
-        if (next >= 0 && !sh.stop) {
-            state = next;
-            np = next == SHRINK ? n : 1;
-            continue;
-        }
-        if (sh.stop || it == 200) break;
-        if (K.phases && t0) sh.lph[lvl][7] += wall_clock64() - sh.tmark;  // the branch's share of the steering
-        it++;
-        // the iteration's set-up, spread over threads (every step keeps the
-        // reference's arithmetic and order per coordinate):
-        // 1. the stable sort of the vertices by value (align.rs:816-818): vertex
-        //    i goes to its rank (values below it, then equal values before it)
-        const int tid = tid_here();
-        if (tid <= n) {
-            const double f = sh.FV[tid];
-            int r = 0;
-            for (int j = 0; j <= n; j++) r += sh.FV[j] < f || (sh.FV[j] == f && j < tid);
-            for (int k = 0; k < n; k++) sh.S2[r][k] = sh.S[tid][k];
-            sh.FV2[r] = f;
-        }
-        __syncthreads();
-        if (tid < (n + 1) * 8) {
-            const int i = tid >> 3, k = tid & 7;
-            if (k < n) sh.S[i][k] = sh.S2[i][k];
-            if (k == 0) sh.FV[i] = sh.FV2[i];
-        }
-        __syncthreads();
-        // 2. convergence (align.rs:888-894): the value spread and every vertex
-        //    within tol of the best, one (vertex, coordinate) per thread
-        int ok = 1;
-        if (tid == 0) ok = !(fabs(sh.FV[n] - sh.FV[0]) > 1e-4 * (1.0 + fabs(sh.FV[0])));
-        if (tid >= 8 && tid < (n + 1) * 8) {
-            const int i = tid >> 3, k = tid & 7;
-            if (k < n && !(fabs(sh.S[i][k] - sh.S[0][k]) <= sh.tolv[k])) ok = 0;
-        }
-        // 3. per coordinate: the centroid of the best n, the reflection, the
-        //    expansion and the contraction (align.rs:823-862)
-        if (tid < n) {
-            const int k = tid;
-            double c = 0.0;
-            for (int i = 0; i < n; i++) c += sh.S[i][k];
-            c /= (double)n;
-            sh.C[k] = c;
-            const double xr = CLAMPK(c + 1.0 * (c - sh.S[n][k]), k);
-            const double xe = CLAMPK(c + 2.0 * (xr - c), k);
-            const double xc = CLAMPK(c + 0.5 * (sh.S[n][k] - c), k);
-            sh.XR[k] = sh.PT[0][k] = xr;
-            sh.XE[k] = sh.PT[1][k] = xe;
-            sh.XC[k] = sh.PT[2][k] = xc;
-        }
-        if (__syncthreads_and(ok)) break;
-        state = ITER;
-        np = spec ? 3 : 1;
+        wsync();
     }
-#undef CLAMPK
-    if (t0) {
+    for (bool first = true;; first = false) {
+        if (w0) {
+            // the next pass's maps and window bound (the level's first pass
+            // stages the held tiles), or the end of the level
+            const uint64_t t_e = K.phases ? wall_clock64() : 0;
+            if (nxt) nm_prep(K, sh, lvl, nxt, !resident && (first || !held || sh.restage));
+            if (lane == 0) {
+                sh.ctl = nxt;
+                if (K.phases) {
+                    sh.tmark = wall_clock64();
+                    sh.ph[0] += sh.tmark - t_e;
+                    sh.lph[lvl][0] += sh.tmark - t_e;
+                }
+            }
+        }
+        __syncthreads();
+        const int np = __builtin_amdgcn_readfirstlane(sh.ctl);
+        if (np == 0) break;
+        if (resident)
+            nm_tiles<1>(K, sh, boxs, hb, lvl, np, solo, false, false, false);
+        else
+            nm_tiles<0>(K, sh, boxs, hb, lvl, np, solo, held, held && first, refh);
+        __syncthreads();  // every wave's sums are in sh.red
+        uint64_t tc = 0;
+        if (w0) {
+            if (K.phases && lane == 0) {
+                tc = wall_clock64();
+                sh.ph[1] += tc - sh.tmark;
+                sh.lph[lvl][1] += tc - sh.tmark;
+                sh.tmark = tc;
+            }
+            nm_publish(K, sh, lvl, np, solo, part, ctr, err, gen);
+        }
+        if (!solo) {
+            __syncthreads();  // every participant's partials are in place, or the search stops
+            if (!sh.stop) nm_fold(sh, np, part, gen);
+            gen++;
+            __syncthreads();
+        }
+        if (w0) {
+            nxt = 0;
+            if (!sh.stop) {
+                nm_costs(K, sh, lvl, np, solo);
+                if (K.phases && lane == 0) {
+                    const uint64_t t = wall_clock64();
+                    sh.ph[2] += t - tc;
+                    sh.lph[lvl][2] += t - tc;
+                }
+                nxt = nm_steer(K, sh, lvl, state, it, spec);
+            }
+        }
+    }
+    if (w0 && lane < n) {
         int best = 0;
         for (int i = 1; i <= n; i++)
             if (sh.FV[i] < sh.FV[best]) best = i;
-        for (int k = 0; k < n; k++) sh.cur[K.free_idx[k]] = sh.S[best][k];
+        sh.cur[K.free_idx[lane]] = sh.S[best][lane];
     }
     __syncthreads();
 }
+#undef CLAMPK
 
 // state: [0..8) the initial Sim (the guess) in, the found Sim out; [8] the
 // evaluations; [9 + l] the evaluations at level l; [9 + NM_MAXL + l] rank 0's
@@ -1058,6 +1115,7 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
     const int n = K.nfree;
     unsigned gen = 0;
     const bool t0 = threadIdx.x == 0;
+    zero_waves(sh);  // the waves' accumulators (wave 0 clears them after each pass)
     if (t0) {
         sh.stop = 0;
         sh.rank = -1;
