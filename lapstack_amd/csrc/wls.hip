@@ -115,9 +115,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ wd, const T *__restrict__ ax,
                                                  const T *__restrict__ ay, const T *__restrict__ dinv,
                                                  const T *__restrict__ x, const T *__restrict__ b, int w, int h,
-                                                 int mode, T *__restrict__ out) {
+                                                 int mode, T *__restrict__ out, const int *__restrict__ skip) {
     int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= w || y >= h) return;
+    if (i >= w || y >= h || (skip && *skip)) return;
     size_t k = (size_t)y * w + i;
     T v = apply_at(wd, ax, ay, x, w, h, i, y, k);
     if (mode == 0)
@@ -129,32 +129,90 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ wd, const
 }
 
 template <typename T>
-__global__ void k_jacobi0(const T *__restrict__ dinv, const T *__restrict__ b, size_t n, T *__restrict__ x) {
+__global__ void k_jacobi0(const T *__restrict__ dinv, const T *__restrict__ b, size_t n, T *__restrict__ x,
+                          const int *__restrict__ skip) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = (T)MG_OMEGA * dinv[i] * b[i];
+    if (i < n && !(skip && *skip)) x[i] = (T)MG_OMEGA * dinv[i] * b[i];
 }
 
-// depth.rs:911-921
+// The column and row of a grid-stride loop's index over a w-wide grid,
+// advanced by the stride with no division per step.
+struct GridWalk {
+    int i, y, di, dy, w;
+    __device__ GridWalk(size_t k0, size_t stride, int w_) : w(w_) {
+        y = (int)(k0 / (size_t)w_);
+        i = (int)(k0 - (size_t)y * w_);
+        dy = (int)(stride / (size_t)w_);
+        di = (int)(stride - (size_t)dy * w_);
+    }
+    __device__ __forceinline__ void next() {
+        i += di;
+        y += dy;
+        if (i >= w) {
+            i -= w;
+            y++;
+        }
+    }
+};
+
+// A level's descent step in one pass (depth.rs:837-847 from zero, :924-930 and
+// :911-921): per coarse cell, the 2 x 2 fine cells under it get x = w D^-1 b
+// (k_jacobi0), their residuals b - A x (k_stencil mode 1, the neighbours' x
+// formed the same way where they are read), and the coarse cell gets their sum
+// as the restriction adds them (row by row); the fine residual is never stored.
 template <typename T>
-__global__ void k_restrict(const T *__restrict__ r, int w, int h, T *__restrict__ bc, int cw, int ch) {
-    int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
-    if (xx >= cw || yy >= ch) return;
-    int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
+__device__ __forceinline__ T jac0_at(const T *__restrict__ dinv, const T *__restrict__ b, size_t k) {
+    return (T)MG_OMEGA * dinv[k] * b[k];
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_down(const T *__restrict__ wd, const T *__restrict__ ax,
+                                                 const T *__restrict__ ay, const T *__restrict__ dinv,
+                                                 const T *__restrict__ b, int w, int h, T *__restrict__ x,
+                                                 T *__restrict__ bc, int cw, int ch, const int *__restrict__ skip) {
+    const int xx = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (xx >= cw || yy >= ch || (skip && *skip)) return;
+    const int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
     T o = 0;
     for (int y = 2 * yy; y < ye; y++) {
-        T s = 0;
-        for (int x = 2 * xx; x < xe; x++) s += r[(size_t)y * w + x];
-        o += s;
+        T sr = 0;
+        for (int i = 2 * xx; i < xe; i++) {
+            const size_t k = (size_t)y * w + i;
+            const T xk = jac0_at(dinv, b, k);
+            x[k] = xk;
+            T v = wd[k] * xk;
+            if (i > 0) v += ax[k - 1] * (xk - jac0_at(dinv, b, k - 1));
+            if (i + 1 < w) v += ax[k] * (xk - jac0_at(dinv, b, k + 1));
+            if (y > 0) v += ay[k - w] * (xk - jac0_at(dinv, b, k - w));
+            if (y + 1 < h) v += ay[k] * (xk - jac0_at(dinv, b, k + w));
+            sr += b[k] - v;
+        }
+        o += sr;
     }
     bc[(size_t)yy * cw + xx] = o;
 }
 
-// depth.rs:901-908
+// A level's ascent step in one pass (depth.rs:901-908 then :837-847): the
+// post-smoothing sweep applied to x + P xc (the prolongation's sum, formed where each
+// value is read), into out; the prolonged x itself is never stored.
 template <typename T>
-__global__ void k_prolong(T *__restrict__ x, int w, int h, const T *__restrict__ xc, int cw) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= w || y >= h) return;
-    x[(size_t)y * w + i] += xc[(size_t)(y / 2) * cw + i / 2];
+__global__ __launch_bounds__(256) void k_mg_up(const T *__restrict__ wd, const T *__restrict__ ax,
+                                               const T *__restrict__ ay, const T *__restrict__ dinv,
+                                               const T *__restrict__ b, const T *__restrict__ x,
+                                               const T *__restrict__ xc, int w, int h, int cw, T *__restrict__ out,
+                                               const int *__restrict__ skip) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= w || y >= h || (skip && *skip)) return;
+    const size_t k = (size_t)y * w + i;
+    auto px = [&](int ii, int yy2) -> T {
+        return x[(size_t)yy2 * w + ii] + xc[(size_t)(yy2 / 2) * cw + ii / 2];
+    };
+    const T xk = px(i, y);
+    T v = wd[k] * xk;
+    if (i > 0) v += ax[k - 1] * (xk - px(i - 1, y));
+    if (i + 1 < w) v += ax[k] * (xk - px(i + 1, y));
+    if (y > 0) v += ay[k - w] * (xk - px(i, y - 1));
+    if (y + 1 < h) v += ay[k] * (xk - px(i, y + 1));
+    out[k] = xk + (T)MG_OMEGA * dinv[k] * (b[k] - v);
 }
 
 // depth.rs:677-702: the Thomas solve of every row (wd null: unit weights).
@@ -525,7 +583,16 @@ __device__ __forceinline__ double wsum(double v) {
 }
 
 // fixed-order grid reduction of one double per thread (per-block partials, last block folds)
+template <class Fin>
+__device__ void grid_sum1f(double v, double *__restrict__ part, unsigned *__restrict__ ticket, Fin fin);
 __device__ void grid_sum1(double v, double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ out) {
+    grid_sum1f(v, part, ticket, [=](double t) { *out = t; });
+}
+
+// Sums v over the grid: per-block partials, then the last block to finish folds
+// them in block order (deterministic) and thread 0 of it calls fin(total).
+template <class Fin>
+__device__ void grid_sum1f(double v, double *__restrict__ part, unsigned *__restrict__ ticket, Fin fin) {
     __shared__ double sh[4];
     __shared__ bool last;
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -547,7 +614,7 @@ __device__ void grid_sum1(double v, double *__restrict__ part, unsigned *__restr
     if (lane == 0) sh[wv] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        *out = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        fin(((sh[0] + sh[1]) + sh[2]) + sh[3]);
         *ticket = 0;
     }
 }
@@ -563,29 +630,74 @@ __global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *_
     grid_sum1(s, part, ticket, out);
 }
 
-// ap = A p and sum p*ap
+// One MG-PCG solve's scalars in device memory: the loop's kernels read and
+// update them, so the loop runs with no host round trip per iteration (the
+// host checks `done` between batches of iterations).  Each scalar is computed
+// by thread 0 of the last block of a reduction, with the reference's
+// arithmetic: alpha = (T)(rz / pap), beta = (T)(rz_new / rz), the relative
+// residual sqrt(rr) / bnorm (kept in f32 by the FP32 solver, depth.rs:1051).
+struct CgState {
+    double rz, pap, rr, rel, rel0, alpha, beta, bnorm, tol;
+    unsigned iters;
+    int done;  // 0: running; 1: below the tolerance; 2: stopped on p.Ap <= 0
+};
+#define CG_WORDS 16  // doubles the state takes in device memory
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_spmv_dot(const T *__restrict__ wd, const T *__restrict__ ax,
-                                                  const T *__restrict__ ay, const T *__restrict__ p, int w, int h,
-                                                  T *__restrict__ ap, double *__restrict__ part,
-                                                  unsigned *__restrict__ ticket, double *__restrict__ out) {
+__host__ __device__ __forceinline__ double cg_rel(double rr, double bnorm) {
+    return sizeof(T) == 4 ? (double)(float)(sqrt(rr) / bnorm) : sqrt(rr) / bnorm;
+}
+template <typename T>
+__host__ __device__ __forceinline__ bool cg_below(double rel, double tol) {
+    return sizeof(T) == 4 ? (float)rel < (float)tol : rel < tol;
+}
+
+// the loop's entry: rr = r.r (rz = r.z was summed into cg->rz), the residual,
+// and whether it is already below the tolerance
+__global__ void k_cg_start(CgState *cg, double bnorm, double tol, int f32) {
+    if (threadIdx.x || blockIdx.x) return;
+    cg->bnorm = bnorm;
+    cg->tol = tol;
+    cg->rel = f32 ? cg_rel<float>(cg->rr, bnorm) : cg_rel<double>(cg->rr, bnorm);
+    cg->rel0 = cg->rel;
+    cg->iters = 0;
+    cg->done = (f32 ? cg_below<float>(cg->rel, tol) : cg_below<double>(cg->rel, tol)) ? 1 : 0;
+}
+
+// an iteration's first step (depth.rs:1058-1063): ap = A p, pap = p.ap, alpha
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_cg(const T *__restrict__ wd, const T *__restrict__ ax,
+                                                 const T *__restrict__ ay, const T *__restrict__ p, int w, int h,
+                                                 T *__restrict__ ap, double *__restrict__ part,
+                                                 unsigned *__restrict__ ticket, CgState *__restrict__ cg) {
+    if (cg->done) return;
     double s = 0.0;
-    size_t n = (size_t)w * h;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
-        int y = (int)(k / w), i = (int)(k - (size_t)y * w);
+    const size_t n = (size_t)w * h, k0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+                 stride = (size_t)gridDim.x * blockDim.x;
+    GridWalk g(k0, stride, w);
+    for (size_t k = k0; k < n; k += stride, g.next()) {
+        const int y = g.y, i = g.i;
         T v = apply_at(wd, ax, ay, p, w, h, i, y, k);
         ap[k] = v;
         s += (double)p[k] * (double)v;
     }
-    grid_sum1(s, part, ticket, out);
+    grid_sum1f(s, part, ticket, [=](double pap) {
+        cg->iters++;
+        cg->pap = pap;
+        if (pap <= 0.0)
+            cg->done = 2;
+        else
+            cg->alpha = (double)(T)(cg->rz / pap);
+    });
 }
 
-// u += alpha p; r -= alpha ap; sum r*r (depth.rs:1064-1066)
+// u += alpha p; r -= alpha ap; rr = r.r; the residual and the stop test (depth.rs:1064-1070)
 template <typename T>
-__global__ __launch_bounds__(256) void k_cg_update(T *__restrict__ u, T *__restrict__ r, const T *__restrict__ p,
-                                                   const T *__restrict__ ap, T alpha, size_t n,
-                                                   double *__restrict__ part, unsigned *__restrict__ ticket,
-                                                   double *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_update_cg(T *__restrict__ u, T *__restrict__ r, const T *__restrict__ p,
+                                                   const T *__restrict__ ap, size_t n, double *__restrict__ part,
+                                                   unsigned *__restrict__ ticket, CgState *__restrict__ cg) {
+    if (cg->done) return;
+    const T alpha = (T)cg->alpha;
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         u[i] += alpha * p[i];
@@ -593,12 +705,40 @@ __global__ __launch_bounds__(256) void k_cg_update(T *__restrict__ u, T *__restr
         r[i] = ri;
         s += (double)ri * (double)ri;
     }
-    grid_sum1(s, part, ticket, out);
+    grid_sum1f(s, part, ticket, [=](double rr) {
+        cg->rr = rr;
+        cg->rel = cg_rel<T>(rr, cg->bnorm);
+        if (cg_below<T>(cg->rel, cg->tol)) cg->done = 1;
+    });
+}
+
+// rz_new = r.z, beta = rz_new / rz, rz = rz_new (depth.rs:1071-1073); rz
+// itself at the loop's entry (first)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_cg(const T *__restrict__ a, const T *__restrict__ b, size_t n,
+                                                double *__restrict__ part, unsigned *__restrict__ ticket,
+                                                CgState *__restrict__ cg, int first) {
+    if (!first && cg->done) return;
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    grid_sum1f(s, part, ticket, [=](double t) {
+        if (first == 2) {
+            cg->rr = t;
+        } else if (first == 1) {
+            cg->rz = t;
+        } else {
+            cg->beta = (double)(T)(t / cg->rz);
+            cg->rz = t;
+        }
+    });
 }
 
 // p = z + beta p (depth.rs:1074)
 template <typename T>
-__global__ void k_pupdate(T *__restrict__ p, const T *__restrict__ z, T beta, size_t n) {
+__global__ void k_pupdate_cg(T *__restrict__ p, const T *__restrict__ z, size_t n, const CgState *__restrict__ cg) {
+    if (cg->done) return;
+    const T beta = (T)cg->beta;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = z[i] + beta * p[i];
 }
@@ -619,9 +759,11 @@ __global__ __launch_bounds__(256) void k_resid64(const float *__restrict__ wd, c
                                                  double *__restrict__ part, unsigned *__restrict__ ticket,
                                                  double *__restrict__ out) {
     double s = 0.0;
-    size_t n = (size_t)w * h;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
-        int y = (int)(k / w), i = (int)(k - (size_t)y * w);
+    const size_t n = (size_t)w * h, k0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+                 stride = (size_t)gridDim.x * blockDim.x;
+    GridWalk g(k0, stride, w);
+    for (size_t k = k0; k < n; k += stride, g.next()) {
+        const int y = g.y, i = g.i;
         double uk = u[k];
         double v = (double)wd[k] * uk;
         if (i > 0) v += (double)ax[k - 1] * (uk - (double)u[k - 1]);
@@ -717,12 +859,12 @@ __global__ __launch_bounds__(256) void k_chol_factor(const T *__restrict__ wd, c
     for (int idx = tid; idx < n * st; idx += blockDim.x) Lb[idx] = S[idx];
 }
 
-// x = (L L^T)^-1 b by column-oriented substitutions in LDS (one wavefront)
+// x = (L L^T)^-1 b by column-oriented substitutions in LDS, S and y in the
+// caller's LDS (the block's threads, any count: the column work takes the
+// first p of them)
 template <typename T>
-__global__ __launch_bounds__(64) void k_chol_solve(const double *__restrict__ Lb, int cw, int ch,
-                                                   const T *__restrict__ b, T *__restrict__ x) {
-    __shared__ double S[CHOL_MAXN * CHOL_STRIDE];
-    __shared__ double y[CHOL_MAXN];
+__device__ __forceinline__ void chol_solve_body(const double *__restrict__ Lb, int cw, int ch, const T *__restrict__ b,
+                                                T *__restrict__ x, double *S, double *y) {
     const int n = cw * ch, p = cw, st = p + 1, tid = threadIdx.x;
     for (int idx = tid; idx < n * st; idx += blockDim.x) S[idx] = Lb[idx];
     for (int i = tid; i < n; i += blockDim.x) y[i] = (double)b[i];
@@ -746,6 +888,161 @@ __global__ __launch_bounds__(64) void k_chol_solve(const double *__restrict__ Lb
         __syncthreads();
     }
     for (int i = tid; i < n; i += blockDim.x) x[i] = (T)y[i];
+}
+
+// chol_solve_body by the 64 lanes of one wave (lanes in step; a wave's LDS
+// accesses complete in issue order, so the steps need only be kept in order by
+// the compiler)
+__device__ __forceinline__ void wave_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <typename T>
+__device__ __forceinline__ void chol_solve_wave(const double *__restrict__ Lb, int cw, int ch, const T *__restrict__ b,
+                                                T *__restrict__ x, double *S, double *y) {
+    const int n = cw * ch, p = cw, st = p + 1, tid = threadIdx.x;
+    for (int idx = tid; idx < n * st; idx += 64) S[idx] = Lb[idx];
+    for (int i = tid; i < n; i += 64) y[i] = (double)b[i];
+    wave_order();
+    for (int j = 0; j < n; j++) {  // L y = b
+        if (tid == 0) y[j] /= S[j * st];
+        wave_order();
+        if (tid < p && j + 1 + tid < n) {
+            int i = j + 1 + tid;
+            y[i] -= S[i * st + i - j] * y[j];
+        }
+        wave_order();
+    }
+    for (int j = n - 1; j >= 0; j--) {  // L^T x = y
+        if (tid == 0) y[j] /= S[j * st];
+        wave_order();
+        if (tid < p && j - 1 - tid >= 0) {
+            int i = j - 1 - tid;
+            y[i] -= S[j * st + j - i] * y[j];
+        }
+        wave_order();
+    }
+    for (int i = tid; i < n; i += 64) x[i] = (T)y[i];
+}
+
+// the coarsest grid solved by the FP64 factor (one wavefront)
+template <typename T>
+__global__ __launch_bounds__(64) void k_chol_solve(const double *__restrict__ Lb, int cw, int ch,
+                                                   const T *__restrict__ b, T *__restrict__ x,
+                                                   const int *__restrict__ skip) {
+    __shared__ double S[CHOL_MAXN * CHOL_STRIDE];
+    __shared__ double y[CHOL_MAXN];
+    if (skip && *skip) return;
+    chol_solve_body<T>(Lb, cw, ch, b, x, S, y);
+}
+
+// The V-cycle's coarse tail (depth.rs:924-947 from level l0 down) in one block:
+// on levels this small every step of the recursion is a few cells per thread,
+// so a launch per step (two per level, and the coarsest grid's 32 sweeps) costs
+// far more than the arithmetic.  The block walks the reference's steps in its
+// order with a barrier between them, each cell computed with the operations the
+// finer levels' kernels use (apply_at, the restriction's row sums, the
+// prolongation's add), so the values are the ones those steps give.  A level's
+// post-smoothing writes its r buffer, as k_mg_up does (the host then swaps x and
+// r of the levels, as it does after those launches).
+#define VT_MAXL 12
+#define VT_THREADS 1024
+#define VT_CELLS 16384  // the tail starts at the first level of at most this many cells
+template <typename T>
+struct VTail {
+    const T *wd[VT_MAXL], *ax[VT_MAXL], *ay[VT_MAXL], *dinv[VT_MAXL];
+    T *x[VT_MAXL], *b[VT_MAXL], *r[VT_MAXL];
+    int w[VT_MAXL], h[VT_MAXL];
+    int nl;     // levels in the tail: l0 .. l0 + nl - 1, the last the coarsest
+    int exact;  // the coarsest grid by the FP64 factor (else MG_COARSE damped-Jacobi sweeps)
+    const double *chol;
+    const int *skip;  // non-null and set: the launch does nothing (a PCG that has stopped)
+};
+
+template <typename T>
+__device__ __forceinline__ void vt_sweep(const VTail<T> &P, int l, const T *__restrict__ x, T *__restrict__ o) {
+    // one damped-Jacobi sweep of level l from x into o (k_stencil mode 2)
+    const int w = P.w[l], h = P.h[l], n = w * h;
+    const T *bb = P.b[l], *di = P.dinv[l];
+    for (int k = threadIdx.x; k < n; k += VT_THREADS) {
+        const int y = k / w, i = k - y * w;
+        const T v = apply_at(P.wd[l], P.ax[l], P.ay[l], x, w, h, i, y, (size_t)k);
+        o[k] = x[k] + (T)MG_OMEGA * di[k] * (bb[k] - v);
+    }
+    __syncthreads();
+}
+
+// k_jacobi0: x = w D^-1 b on level l
+template <typename T>
+__device__ __forceinline__ void vt_first(const VTail<T> &P, int l) {
+    const int n = P.w[l] * P.h[l];
+    for (int k = threadIdx.x; k < n; k += VT_THREADS) P.x[l][k] = (T)MG_OMEGA * P.dinv[l][k] * P.b[l][k];
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void k_vtail(const VTail<T> P) {
+    __shared__ double S[CHOL_MAXN * CHOL_STRIDE];
+    __shared__ double Y[CHOL_MAXN];
+    const int tid = threadIdx.x, L = P.nl;
+    if (P.skip && *P.skip) return;
+    for (int l = 0; l + 1 < L; l++) {
+        vt_first(P, l);
+        // the residual into r (k_stencil mode 1), restricted into the next level's b (depth.rs:911-921)
+        const int w = P.w[l], h = P.h[l], n = w * h;
+        T *rr = P.r[l];
+        for (int k = tid; k < n; k += VT_THREADS) {
+            const int y = k / w, i = k - y * w;
+            rr[k] = P.b[l][k] - apply_at(P.wd[l], P.ax[l], P.ay[l], (const T *)P.x[l], w, h, i, y, (size_t)k);
+        }
+        __syncthreads();
+        const int cw = P.w[l + 1], chh = P.h[l + 1];
+        for (int k = tid; k < cw * chh; k += VT_THREADS) {
+            const int yy = k / cw, xx = k - yy * cw;
+            const int ye = min(2 * yy + 2, h), xe = min(2 * xx + 2, w);
+            T o = 0;
+            for (int y = 2 * yy; y < ye; y++) {
+                T s = 0;
+                for (int x = 2 * xx; x < xe; x++) s += rr[(size_t)y * w + x];
+                o += s;
+            }
+            P.b[l + 1][k] = o;
+        }
+        __syncthreads();
+    }
+    // the coarsest grid; its x ends in P.x (the factor, or an even number of
+    // sweeps after the first) or in P.r
+    const T *xcoarse = P.x[L - 1];
+    if (P.exact) {
+        // by wave 0 alone: the substitutions are a chain of small steps, and
+        // a barrier of one wave is only an ordering of its LDS accesses
+        if (tid < 64) chol_solve_wave<T>(P.chol, P.w[L - 1], P.h[L - 1], (const T *)P.b[L - 1], P.x[L - 1], S, Y);
+        __syncthreads();
+    } else {
+        vt_first(P, L - 1);
+        T *a = P.x[L - 1], *c = P.r[L - 1];
+        for (int k = 1; k < MG_COARSE; k++) {
+            vt_sweep(P, L - 1, a, c);
+            T *t = a;
+            a = c;
+            c = t;
+        }
+        xcoarse = a;
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        // the prolongation into x (depth.rs:901-908), then the post-smoothing sweep from x into r (where
+        // the level's x then is)
+        const int w = P.w[l], h = P.h[l], cw = P.w[l + 1];
+        T *x = P.x[l];
+        for (int k = tid; k < w * h; k += VT_THREADS) {
+            const int y = k / w, i = k - y * w;
+            x[k] += xcoarse[(size_t)(y / 2) * cw + i / 2];
+        }
+        __syncthreads();
+        vt_sweep(P, l, (const T *)x, P.r[l]);
+        xcoarse = P.r[l];
+    }
 }
 
 // Where a solver's buffers come from: device allocations (base null, counting
@@ -797,6 +1094,7 @@ struct WlsSolver {
     DBuf<float> r, z, p, ap, e;
     DBuf<double> b64, r64, z64, p64, ap64, x64;
     DBuf<double> part, out;
+    DBuf<double> cgs;  // a PCG solve's scalars (CgState)
     DBuf<unsigned> ticket;
     bool noalloc = false;  // buffers in a caller's block: scalars come back through pageable host memory
 };
@@ -833,6 +1131,7 @@ static void wls_layout(WlsSolver &s, int w, int h, Arena &a, bool with64) {
         for (auto *b : {&s.b64, &s.r64}) a.get(*b, n);
     a.get(s.part, 1024);
     a.get(s.out, 1);
+    a.get(s.cgs, CG_WORDS);
     a.get(s.ticket, 1);
     s.L.clear();
     s.L.emplace_back();
@@ -942,43 +1241,83 @@ static void ensure_l64(WlsSolver &s) {
     s.l64_ok = true;
 }
 
+// a damped-Jacobi sweep of level t (from zero: x = w D^-1 b), b its right-hand
+// side; skip: the device flag that turns the launch into a no-op (nullable)
 template <typename T>
-static void jacobi(Lev<T> &t, bool zero) {
+static void jacobi(Lev<T> &t, bool zero, const T *b, const int *skip) {
     size_t n = (size_t)t.w * t.h;
     double sz = sizeof(T);
     if (zero) {
-        launch_b("mg_jacobi", 3.0 * sz * n, k_jacobi0<T>, dim3(g1(n)), dim3(256), 0, (const T *)t.dinv.p,
-                 (const T *)t.b.p, n, t.x.p);
+        launch_b("mg_jacobi", 3.0 * sz * n, k_jacobi0<T>, dim3(g1(n)), dim3(256), 0, (const T *)t.dinv.p, b, n, t.x.p,
+                 skip);
         return;
     }
     launch_b("mg_jacobi", 7.0 * sz * n, k_stencil<T>, g2(t.w, t.h), B2, 0, (const T *)t.wd.p, (const T *)t.ax.p,
-             (const T *)t.ay.p, (const T *)t.dinv.p, (const T *)t.x.p, (const T *)t.b.p, t.w, t.h, 2, t.r.p);
+             (const T *)t.ay.p, (const T *)t.dinv.p, (const T *)t.x.p, b, t.w, t.h, 2, t.r.p, skip);
     std::swap(t.x, t.r);
 }
 
-// depth.rs:924-947; exact: the coarsest grid solved by the FP64 factor instead of 32 sweeps
+// depth.rs:924-947; exact: the coarsest grid solved by the FP64 factor instead
+// of 32 sweeps.  b0: level 0's right-hand side (the residual the caller
+// preconditions, read where it is); skip: as for jacobi.
 template <typename T>
-static void vcycle(WlsSolver &s, std::vector<Lev<T>> &H, size_t l, bool exact) {
+static void vcycle(WlsSolver &s, std::vector<Lev<T>> &H, size_t l, bool exact, const T *b0, const int *skip) {
     Lev<T> &t = H[l];
-    if (l + 1 == H.size()) {
-        if (exact)
-            launch("mg_coarse_solve", k_chol_solve<T>, dim3(1), dim3(64), 0, (const double *)s.chol.p, t.w, t.h,
-                   (const T *)t.b.p, t.x.p);
-        else
-            for (int k = 0; k < MG_COARSE; k++) jacobi(t, k == 0);
+    const T *bl = l == 0 ? b0 : (const T *)t.b.p;
+    if ((size_t)t.w * t.h <= VT_CELLS && H.size() - l <= VT_MAXL) {
+        // the coarse tail in one block (k_vtail), then the pointer swaps its
+        // sweeps stand for
+        VTail<T> P;
+        std::memset(&P, 0, sizeof(P));
+        P.nl = (int)(H.size() - l);
+        P.exact = exact ? 1 : 0;
+        P.chol = s.chol.p;
+        P.skip = skip;
+        double bytes = 0.0;
+        for (int q = 0; q < P.nl; q++) {
+            Lev<T> &v = H[l + q];
+            P.wd[q] = v.wd.p;
+            P.ax[q] = v.ax.p;
+            P.ay[q] = v.ay.p;
+            P.dinv[q] = v.dinv.p;
+            P.x[q] = v.x.p;
+            P.b[q] = q == 0 ? (T *)bl : v.b.p;  // (level l's b is only read)
+            P.r[q] = v.r.p;
+            P.w[q] = v.w;
+            P.h[q] = v.h;
+            // the bytes the per-level kernels count for these steps
+            const double n = (double)v.w * v.h, sz = sizeof(T);
+            if (q + 1 < P.nl)
+                bytes += sz * (19.0 * n + 2.0 * (double)H[l + q + 1].w * H[l + q + 1].h);
+            else
+                bytes += exact ? 2.0 * sz * n : sz * (3.0 + 7.0 * (MG_COARSE - 1)) * n;
+        }
+        launch_b("mg_tail", bytes, k_vtail<T>, dim3(1), dim3(VT_THREADS), 0, P);
+        for (int q = 0; q + 1 < P.nl; q++) std::swap(H[l + q].x, H[l + q].r);
+        if (!exact)
+            for (int k = 1; k < MG_COARSE; k++) std::swap(H.back().x, H.back().r);
         return;
     }
-    jacobi(t, true);
-    double n = (double)t.w * t.h, sz = sizeof(T);
-    launch_b("mg_residual", 6.0 * sz * n, k_stencil<T>, g2(t.w, t.h), B2, 0, (const T *)t.wd.p, (const T *)t.ax.p,
-             (const T *)t.ay.p, (const T *)t.dinv.p, (const T *)t.x.p, (const T *)t.b.p, t.w, t.h, 1, t.r.p);
+    if (l + 1 == H.size()) {
+        if (exact)
+            launch("mg_coarse_solve", k_chol_solve<T>, dim3(1), dim3(64), 0, (const double *)s.chol.p, t.w, t.h, bl,
+                   t.x.p, skip);
+        else
+            for (int k = 0; k < MG_COARSE; k++) jacobi(t, k == 0, bl, skip);
+        return;
+    }
+    // the pre-smoothing from zero, the residual and its restriction in one
+    // pass; the coarse correction; its prolongation and the post-smoothing in one
+    const double n = (double)t.w * t.h, sz = sizeof(T);
     Lev<T> &c = H[l + 1];
-    launch_b("mg_restrict", sz * (n + (double)c.w * c.h), k_restrict<T>, g2(c.w, c.h), B2, 0, (const T *)t.r.p, t.w,
-             t.h, c.b.p, c.w, c.h);
-    vcycle(s, H, l + 1, exact);
-    launch_b("mg_prolong", sz * (2.0 * n + (double)c.w * c.h), k_prolong<T>, g2(t.w, t.h), B2, 0, t.x.p, t.w, t.h,
-             (const T *)c.x.p, c.w);
-    jacobi(t, false);
+    const double nc = (double)c.w * c.h;
+    launch_b("mg_down", sz * (6.0 * n + nc), k_mg_down<T>, g2(c.w, c.h), B2, 0, (const T *)t.wd.p, (const T *)t.ax.p,
+             (const T *)t.ay.p, (const T *)t.dinv.p, bl, t.w, t.h, t.x.p, c.b.p, c.w, c.h, skip);
+    vcycle(s, H, l + 1, exact, (const T *)nullptr, skip);
+    launch_b("mg_up", sz * (7.0 * n + nc), k_mg_up<T>, g2(t.w, t.h), B2, 0, (const T *)t.wd.p, (const T *)t.ax.p,
+             (const T *)t.ay.p, (const T *)t.dinv.p, bl, (const T *)t.x.p, (const T *)c.x.p, t.w, t.h, c.w, t.r.p,
+             skip);
+    std::swap(t.x, t.r);
 }
 
 template <typename T>
@@ -987,53 +1326,83 @@ static double dot(WlsSolver &s, const T *a, const T *b, size_t n) {
     return read_out(s);
 }
 
+// z = M^-1 r: one V-cycle on r where it is; z is the hierarchy's level-0 x
+// (valid until the next V-cycle)
 template <typename T>
-static void precond(WlsSolver &s, std::vector<Lev<T>> &H, const T *r, T *z, bool exact) {
-    size_t n = (size_t)H[0].w * H[0].h;
-    LSH_CHECK(hipMemcpyAsync(H[0].b.p, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
-    vcycle(s, H, 0, exact);
-    LSH_CHECK(hipMemcpyAsync(z, H[0].x.p, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+static const T *precond(WlsSolver &s, std::vector<Lev<T>> &H, const T *r, bool exact, const int *skip) {
+    vcycle(s, H, 0, exact, r, skip);
+    return H[0].x.p;
+}
+
+static CgState read_cg(WlsSolver &s) {
+    CgState c;
+    if (s.noalloc) {
+        LSH_CHECK(hipMemcpyAsync(&c, s.cgs.p, sizeof(CgState), hipMemcpyDeviceToHost, stream()));
+        LSH_CHECK(hipStreamSynchronize(stream()));
+        return c;
+    }
+    void *pin = pinned_scratch(sizeof(CgState));
+    LSH_CHECK(hipMemcpyAsync(pin, s.cgs.p, sizeof(CgState), hipMemcpyDeviceToHost, stream()));
+    sync();
+    std::memcpy(&c, pin, sizeof(CgState));
+    return c;
 }
 
 // MG-PCG (depth.rs:1036-1077) on H from the iterate u for the right-hand side b:
-// the reference's sequence of operations; rel is the recurrence's residual over bnorm.
+// the reference's sequence of operations; rel is the recurrence's residual over
+// bnorm.  The loop runs on the device: its scalars live in CgState, every
+// kernel of an iteration does nothing once the solve has stopped, and the host
+// queues iterations in batches and reads the state between them (the first
+// batch a few iterations, the next sized by the convergence rate so far), so
+// an iteration costs no host round trip; the iterations queued past the stop
+// cost their launches only.
 template <typename T>
 static uint32_t pcg(WlsSolver &s, std::vector<Lev<T>> &H, T *u, const T *b, double bnorm, double tol, uint32_t max_iters,
-                    bool exact, double &rel_out, T *r, T *z, T *p, T *ap) {
+                    bool exact, double &rel_out, T *r, T * /*z: the hierarchy's level-0 x instead*/, T *p, T *ap) {
     const int w = H[0].w, h = H[0].h;
     const size_t n = (size_t)w * h;
     Lev<T> &t0 = H[0];
     const double sz = sizeof(T);
+    CgState *cg = (CgState *)s.cgs.p;
+    const int *done = &cg->done;
     launch_b("cg_residual", 6.0 * sz * n, k_stencil<T>, g2(w, h), B2, 0, (const T *)t0.wd.p, (const T *)t0.ax.p,
-             (const T *)t0.ay.p, (const T *)t0.dinv.p, (const T *)u, b, w, h, 1, r);
-    precond(s, H, (const T *)r, z, exact);
+             (const T *)t0.ay.p, (const T *)t0.dinv.p, (const T *)u, b, w, h, 1, r, (const int *)nullptr);
+    const T *z = precond(s, H, (const T *)r, exact, (const int *)nullptr);
     LSH_CHECK(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
-    double rz = dot<T>(s, r, z, n);
-    // the reference keeps the residual in f32 (depth.rs:1051); so does the FP32 solver here
-    auto relv = [&](double rr) { return sizeof(T) == 4 ? (double)(float)(std::sqrt(rr) / bnorm) : std::sqrt(rr) / bnorm; };
-    auto below = [&](double rel) { return sizeof(T) == 4 ? (float)rel < (float)tol : rel < tol; };
-    double rel = relv(dot<T>(s, r, r, n));
-    uint32_t iters = 0;
-    for (uint32_t it = 0; it < max_iters; it++) {
-        if (below(rel)) break;
-        iters++;
-        launch_b("cg_spmv", 5.0 * sz * n, k_spmv_dot<T>, dim3(gred(n)), dim3(256), 0, (const T *)t0.wd.p,
-                 (const T *)t0.ax.p, (const T *)t0.ay.p, (const T *)p, w, h, ap, s.part.p, s.ticket.p, s.out.p);
-        double pap = read_out(s);
-        if (pap <= 0.0) break;
-        T alpha = (T)(rz / pap);
-        launch_b("cg_update", 6.0 * sz * n, k_cg_update<T>, dim3(gred(n)), dim3(256), 0, u, r, (const T *)p,
-                 (const T *)ap, alpha, n, s.part.p, s.ticket.p, s.out.p);
-        rel = relv(read_out(s));
-        if (below(rel)) break;
-        precond(s, H, (const T *)r, z, exact);
-        double rz_new = dot<T>(s, r, z, n);
-        T beta = (T)(rz_new / rz);
-        rz = rz_new;
-        launch_b("cg_pupdate", 3.0 * sz * n, k_pupdate<T>, dim3(g1(n)), dim3(256), 0, p, (const T *)z, beta, n);
+    launch_b("cg_dot", 2.0 * sz * n, k_dot_cg<T>, dim3(gred(n)), dim3(256), 0, (const T *)r, z, n, s.part.p, s.ticket.p,
+             cg, 1);
+    launch_b("cg_dot", 2.0 * sz * n, k_dot_cg<T>, dim3(gred(n)), dim3(256), 0, (const T *)r, (const T *)r, n, s.part.p,
+             s.ticket.p, cg, 2);
+    launch("cg_start", k_cg_start, dim3(1), dim3(64), 0, cg, bnorm, tol, sizeof(T) == 4 ? 1 : 0);
+    auto iteration = [&]() {
+        launch_b("cg_spmv", 5.0 * sz * n, k_spmv_cg<T>, dim3(gred(n)), dim3(256), 0, (const T *)t0.wd.p,
+                 (const T *)t0.ax.p, (const T *)t0.ay.p, (const T *)p, w, h, ap, s.part.p, s.ticket.p, cg);
+        launch_b("cg_update", 6.0 * sz * n, k_update_cg<T>, dim3(gred(n)), dim3(256), 0, u, r, (const T *)p,
+                 (const T *)ap, n, s.part.p, s.ticket.p, cg);
+        const T *zz = precond(s, H, (const T *)r, exact, done);
+        launch_b("cg_dot", 2.0 * sz * n, k_dot_cg<T>, dim3(gred(n)), dim3(256), 0, (const T *)r, zz, n, s.part.p,
+                 s.ticket.p, cg, 0);
+        launch_b("cg_pupdate", 3.0 * sz * n, k_pupdate_cg<T>, dim3(g1(n)), dim3(256), 0, p, zz, n,
+                 (const CgState *)cg);
+    };
+    uint32_t queued = 0, batch = std::min<uint32_t>(max_iters, 4);
+    CgState st = read_cg(s);  // the entry's state (and the initial residual for the rate)
+    while (!st.done && queued < max_iters) {
+        for (uint32_t k = 0; k < batch; k++) iteration();
+        queued += batch;
+        st = read_cg(s);
+        if (st.done || queued >= max_iters) break;
+        // the iterations still needed at the rate so far (at least one)
+        uint32_t more = 1;
+        if (st.iters > 0 && st.rel > 0.0 && st.rel0 > st.rel) {
+            const double rate = std::pow(st.rel / st.rel0, 1.0 / st.iters);
+            const double need = std::log(tol / st.rel) / std::log(rate);
+            if (need > 1.0 && need < 1e6) more = (uint32_t)std::ceil(need);
+        }
+        batch = std::min(more, max_iters - queued);
     }
-    rel_out = rel;
-    return iters;
+    rel_out = st.rel;
+    return st.iters;
 }
 
 // depth.rs:1019-1034: three rounds of row and column sweeps with the lambda_t
