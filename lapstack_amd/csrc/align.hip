@@ -173,6 +173,39 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// v from the lane DPP control CTRL names (both halves by v_mov_b32 with DPP)
+template <int CTRL>
+__device__ __forceinline__ double dpp_get(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lane_get(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The sum of v over the 64 lanes of a wave (all of them active), the same
+// value in every lane: within each row of 16 lanes a butterfly of DPP moves
+// (adjacent lanes, then pairs, then rotations of the row by 4 and by 8 lanes),
+// then the four rows' sums (lanes 0, 16, 32, 48) in row order.  A fixed tree,
+// so every wave and every block forms the same value from the same inputs;
+// each step is a register move and an add, where a shuffle through the LDS
+// crossbar waits on its latency.
+template <class Op>
+__device__ __forceinline__ double wave_reduce(double v, Op op) {
+    v = op(v, dpp_get<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+    v = op(v, dpp_get<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+    v = op(v, dpp_get<0x124>(v));  // row_ror:4
+    v = op(v, dpp_get<0x128>(v));  // row_ror:8
+    return op(op(op(lane_get(v, 0), lane_get(v, 16)), lane_get(v, 32)), lane_get(v, 48));
+}
+__device__ __forceinline__ double wave_total(double v) {
+    return wave_reduce(v, [](double a, double b) { return a + b; });
+}
+
 // Sums NV doubles per thread over the grid into out[NV]: per-block partials,
 // then the last block to finish folds them in block order (deterministic).
 template <int NV>
@@ -577,9 +610,9 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 s1 += d * d;
                 s2 += 1.0;
             }
-            s0 = wave_sum(s0);
-            s1 = wave_sum(s1);
-            s2 = wave_sum(s2);
+            s0 = wave_total(s0);
+            s1 = wave_total(s1);
+            s2 = wave_total(s2);
             if (lane == 0) {
                 sh.red[3 * p][wv] += s0;
                 sh.red[3 * p + 1][wv] += s1;
@@ -629,12 +662,12 @@ __device__ void nm_prep(const NmConst &K, NmShared &sh, int lvl, int np, bool bo
         mny = fmin(mny, sy - cy);
         mxy = fmax(mxy, sy - cy);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
-        mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
-        mny = fmin(mny, __shfl_xor(mny, o, 64));
-        mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
-    }
+    auto mn = [](double a, double b) { return fmin(a, b); };
+    auto mx = [](double a, double b) { return fmax(a, b); };
+    mnx = wave_reduce(mnx, mn);
+    mxx = wave_reduce(mxx, mx);
+    mny = wave_reduce(mny, mn);
+    mxy = wave_reduce(mxy, mx);
     ok = __all(ok) && mnx <= mxx && mny <= mxy && mnx > -256.0 && mxx < 256.0 && mny > -256.0 && mxy < 256.0;
     if (lane == 0) {
         sh.box[0] = ok ? (int)floor(mnx) - 1 : 0;  // window = tile + [lo, hi] on each axis
@@ -764,7 +797,7 @@ __device__ void nm_fold(NmShared &sh, int np, const double *part, unsigned gen) 
 #pragma unroll
         for (int u = 0; u < NM_MAXG / 64; u++)
             if (lane + 64 * u < G) a += v[u];
-        a = wave_sum(a);
+        a = wave_total(a);
         if (lane == 0) sh.sums[k] = a;
     }
 }
