@@ -54,6 +54,12 @@ struct OnStream {
 };
 unsigned stream_cus();  // the CU budget of the current stream (0: the whole device)
 void *pinned_scratch(size_t);  // a per-device pinned host block (reused; at least the size asked)
+// Device memory for the library's buffers, on the current device, from a
+// per-device cache of released blocks (dev_free); dev is the device it was
+// taken on.  A released block is handed out again only once the work the
+// library's streams had queued when it was released has run.
+void *dev_alloc(size_t bytes, int *dev);
+void dev_free(void *p, size_t bytes, int dev);
 
 // ------------------------------------------------------------- profiling
 struct Prof {
@@ -138,11 +144,12 @@ struct DBuf {
     T *p = nullptr;
     size_t n = 0;
     bool owned = false;
+    int dev = -1;  // the device an owned block was taken on
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), owned(o.owned) {
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), owned(o.owned), dev(o.dev) {
         o.p = nullptr;
         o.n = 0;
         o.owned = false;
@@ -153,6 +160,7 @@ struct DBuf {
             p = o.p;
             n = o.n;
             owned = o.owned;
+            dev = o.dev;
             o.p = nullptr;
             o.n = 0;
             o.owned = false;
@@ -162,7 +170,7 @@ struct DBuf {
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
-        if (count) LSH_CHECK(hipMalloc((void **)&p, count * sizeof(T)));
+        if (count) p = (T *)dev_alloc(count * sizeof(T), &dev);
         n = count;
         owned = true;
     }
@@ -176,7 +184,7 @@ struct DBuf {
         if (count > n) alloc(count);
     }
     void release() {
-        if (p && owned) (void)hipFree(p);
+        if (p && owned) dev_free(p, n * sizeof(T), dev);
         p = nullptr;
         n = 0;
         owned = false;

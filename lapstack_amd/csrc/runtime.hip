@@ -1,6 +1,8 @@
 // runtime.hip -- the library's stream, event timing, copies, homography math
 // and the library-level C ABI (probe, version, errors, stream, timing).
 #include <cstdio>
+#include <map>
+#include <vector>
 #include <cstdlib>
 
 #include "api.hpp"
@@ -80,6 +82,145 @@ void *pinned_scratch(size_t n) {
 }
 
 Prof &prof() { return dev_state().prof; }
+
+// ------------------------------------------------------------ device memory
+// The library's device buffers come and go with every call (a depth pass
+// takes and returns a few GB at the 45 MP working grid, in a few dozen
+// blocks).  hipFree waits for the whole device and unmaps the block, so
+// freeing them between the steps of a call leaves the device idle for
+// milliseconds each time.  Released blocks are kept instead, per device and
+// size class (powers of two up to 1 MiB, then multiples of 2 MiB), each with
+// an event recorded on each of the library's streams at its release; a block
+// is handed out again only once all of its events have completed, so no work
+// queued before the release can still be using it.  When the device has no
+// room for a new block, the cached blocks are returned to it and the
+// allocation tried again.  LSH_POOL=0 turns the cache off.
+namespace {
+struct PoolBlock {
+    void *p;
+    hipEvent_t ev[3];
+    int nev;
+};
+struct DevPool {
+    std::multimap<size_t, PoolBlock> freed;
+    std::vector<hipEvent_t> spare;
+};
+struct Pools {
+    std::mutex mu;
+    std::map<int, DevPool> dev;
+    int on = -1;
+};
+Pools &pools() {
+    static Pools *p = new Pools();  // never destroyed: buffers may be released during exit
+    return *p;
+}
+size_t size_class(size_t b) {
+    if (b <= ((size_t)1 << 20)) {
+        size_t c = 256;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    const size_t g = (size_t)2 << 20;
+    return (b + g - 1) / g * g;
+}
+bool pool_on() {
+    Pools &P = pools();
+    if (P.on < 0) {
+        const char *e = std::getenv("LSH_POOL");
+        P.on = !(e && std::atoi(e) == 0);
+    }
+    return P.on != 0;
+}
+bool block_ready(const PoolBlock &b) {
+    for (int i = 0; i < b.nev; i++)
+        if (hipEventQuery(b.ev[i]) != hipSuccess) return false;
+    return true;
+}
+// returns every cached block of the current device to it (the caller holds the lock)
+void pool_trim(DevPool &D) {
+    for (auto &kv : D.freed) {
+        for (int i = 0; i < kv.second.nev; i++) {
+            (void)hipEventSynchronize(kv.second.ev[i]);
+            D.spare.push_back(kv.second.ev[i]);
+        }
+        (void)hipFree(kv.second.p);
+    }
+    D.freed.clear();
+}
+}  // namespace
+
+void *dev_alloc(size_t bytes, int *dev) {
+    LSH_CHECK(hipGetDevice(dev));
+    if (!pool_on()) {
+        void *p = nullptr;
+        LSH_CHECK(hipMalloc(&p, bytes));
+        return p;
+    }
+    const size_t cls = size_class(bytes);
+    Pools &P = pools();
+    std::lock_guard<std::mutex> lk(P.mu);
+    DevPool &D = P.dev[*dev];
+    auto r = D.freed.equal_range(cls);
+    for (auto it = r.first; it != r.second; ++it)
+        if (block_ready(it->second)) {
+            void *p = it->second.p;
+            for (int i = 0; i < it->second.nev; i++) D.spare.push_back(it->second.ev[i]);
+            D.freed.erase(it);
+            return p;
+        }
+    void *p = nullptr;
+    if (hipMalloc(&p, cls) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim(D);
+        LSH_CHECK(hipMalloc(&p, cls));
+    }
+    return p;
+}
+
+void dev_free(void *p, size_t bytes, int dev) {
+    if (!p) return;
+    if (!pool_on()) {
+        (void)hipFree(p);
+        return;
+    }
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) return;
+    {
+        DevState &st = dev_state();
+        Pools &P = pools();
+        std::lock_guard<std::mutex> lk(P.mu);
+        DevPool &D = P.dev[dev];
+        PoolBlock b{p, {nullptr, nullptr, nullptr}, 0};
+        const hipStream_t ss[3] = {st.s, st.side, st.copy};
+        bool ok = true;
+        for (hipStream_t s : ss) {
+            if (!s) continue;
+            hipEvent_t e = nullptr;
+            if (!D.spare.empty()) {
+                e = D.spare.back();
+                D.spare.pop_back();
+            } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                ok = false;
+                break;
+            }
+            if (hipEventRecord(e, s) != hipSuccess) {
+                D.spare.push_back(e);
+                ok = false;
+                break;
+            }
+            b.ev[b.nev++] = e;
+        }
+        if (ok) {
+            D.freed.emplace(size_class(bytes), b);
+        } else {  // no event to guard the block with: give it back to the device
+            for (int i = 0; i < b.nev; i++) D.spare.push_back(b.ev[i]);
+            (void)hipGetLastError();
+            (void)hipFree(p);
+        }
+    }
+    if (cur != dev) (void)hipSetDevice(cur);
+}
 
 hipEvent_t Prof::get() {
     if (pool.empty()) {
