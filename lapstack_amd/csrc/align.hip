@@ -27,13 +27,29 @@ namespace lsh {
 
 #define INV16 (1.0f / 65535.0f)
 
-// align.rs:562-571 on the decoded frame
+// align.rs:562-571 on the decoded frame, four pixels per thread: their 24
+// bytes of interleaved u16 as three 8-byte loads and their lumas as one 16-byte
+// store (a frame whose address is not 8-byte aligned, and the last n % 4
+// pixels, one pixel at a time)
+__device__ __forceinline__ float luma_of(uint32_t r, uint32_t g, uint32_t b) {
+    return 0.299f * ((float)r * INV16) + 0.587f * ((float)g * INV16) + 0.114f * ((float)b * INV16);
+}
 __global__ void k_luma_u16(const FrameRef src, size_t n, float *__restrict__ y) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, i = 4 * t;
     if (i >= n) return;
     const uint16_t *__restrict__ rgb = src.get();
-    float r = (float)rgb[3 * i] * INV16, g = (float)rgb[3 * i + 1] * INV16, b = (float)rgb[3 * i + 2] * INV16;
-    y[i] = 0.299f * r + 0.587f * g + 0.114f * b;
+    if (i + 4 <= n && ((uintptr_t)rgb & 7) == 0 && ((uintptr_t)y & 15) == 0) {
+        const uint64_t *q = (const uint64_t *)(rgb + 3 * i);
+        const uint64_t a = q[0], b = q[1], c = q[2];
+        float4 o;
+        o.x = luma_of((uint32_t)(a & 0xffff), (uint32_t)((a >> 16) & 0xffff), (uint32_t)((a >> 32) & 0xffff));
+        o.y = luma_of((uint32_t)(a >> 48), (uint32_t)(b & 0xffff), (uint32_t)((b >> 16) & 0xffff));
+        o.z = luma_of((uint32_t)((b >> 32) & 0xffff), (uint32_t)(b >> 48), (uint32_t)(c & 0xffff));
+        o.w = luma_of((uint32_t)((c >> 16) & 0xffff), (uint32_t)((c >> 32) & 0xffff), (uint32_t)(c >> 48));
+        *(float4 *)(y + i) = o;
+        return;
+    }
+    for (size_t k = i; k < i + 4 && k < n; k++) y[k] = luma_of(rgb[3 * k], rgb[3 * k + 1], rgb[3 * k + 2]);
 }
 
 // align.rs:650-734: Burt REDUCE (a = 0.33) with the border renormalisation
@@ -1912,7 +1928,7 @@ void frames_stream_slots(lsh_frames *f) {
 void frames_target(lsh_frames *f, const FrameRef &src, int slot) {
     const size_t n = f->w * f->h;
     float *dst = f->yts[slot].p;
-    launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, dst);
+    launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 1024)), dim3(256), 0, src, n, dst);
     build_reg_pyramid(f, dst);
 }
 
@@ -2070,7 +2086,7 @@ char *lsh_frames_process(lsh_frames *f, const uint16_t *rgb, lsh_mem rgb_mem, co
     } else if (searching) {
         Sim init;
         if (guess) std::memcpy(init.v, guess, sizeof(init.v));
-        launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, f->yt.p);
+        launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 1024)), dim3(256), 0, src, n, f->yt.p);
         build_reg_pyramid(f, f->yt.p);
         sim = multiscale_align(f, init, free8, coarsen);
     }
@@ -2092,7 +2108,7 @@ char *lsh_frames_process(lsh_frames *f, const uint16_t *rgb, lsh_mem rgb_mem, co
     // the next frame's reference: this frame's luma, warped (stack.rs:527)
     if (!known_sim) {
         if (!searching) {
-            launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 256)), dim3(256), 0, src, n, f->yt.p);
+            launch_b("luma", 10.0 * n, k_luma_u16, dim3(cdiv(n, 1024)), dim3(256), 0, src, n, f->yt.p);
         }
         if (sim.is_id()) {
             std::swap(f->yt, f->yr);
