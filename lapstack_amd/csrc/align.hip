@@ -349,7 +349,7 @@ __device__ void sim_inverse_dev(const double *s, int w, int h, double *inv) {
 #define NM_TILE 64    // a pass walks 64-pixel-wide destination tiles of 16, 32 or 64 rows
 #define NM_BOX 8192   // floats of a tile's staged source window
 #ifndef NM_HB
-#define NM_HB 24576   // floats a block holds a level's tiles in (their windows, and their reference pixels where they fit)
+#define NM_HB 30688   // floats a block holds a level's tiles in (their windows, and their reference pixels where they fit): the rest of the LDS
 #endif
 #define NM_HMAXT 8    // tiles a block holds at most
 #define NM_HMARGIN 6  // pixels a held window extends past the first pass's bound
@@ -374,7 +374,7 @@ struct NmShared {
     // branch (up to an iteration's set-up)
     uint64_t lph[NM_MAXL][NM_NPH];
     int box[5];             // the pass's window margins (x lo, x hi, y lo, y hi) and whether to stage
-    int hwin[NM_HMAXT][7];  // per tile of a held level: its window (x0, y0, w, h, staged), held, offset in hb
+    int hwin[NM_HMAXT][8];  // per tile of a held level: its window (x0, y0, w, h, staged), held, window and reference offsets in hb
     int hoff;               // the held tiles' floats so far (the hold pass)
     int restage;            // some tile of the block is staged every pass (it did not fit)
     int grid;               // the participating blocks (read through nm_G, as a scalar)
@@ -454,7 +454,11 @@ __device__ __forceinline__ bool nm_holdable(int tw, int th, int G, bool *refh) {
 // (hold) with the bound widened by NM_HMARGIN on each side for the moves to
 // come, at the block's next free place in hb (together with the tile's
 // reference pixels when refh), and read from there by every later pass; a
-// tile that no longer fits there is staged by every pass instead.
+// tile that no longer fits there is staged by every pass instead.  When the
+// block's windows all fit but not every tile's reference pixels beside them,
+// the reference pixels go after the last window, tile by tile while they fit,
+// into the rest of hb and the window buffer after it (free on a level whose
+// windows are all held); the others are read from memory by every pass.
 template <int MODE>
 __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *boxs, float *hb, int lvl, int np,
                                          bool solo, bool held, bool hold, bool refh) {
@@ -473,6 +477,21 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
     // (the accumulators were cleared, and the pass's maps and bound written, by
     // wave 0 before the barrier that starts the pass)
     const int B = nm_B(sh);
+    // the hold pass without refh: the block's windows' total size, and
+    // whether they all fit (then reference pixels may follow them)
+    int wsum = 0;
+    bool allw = false;
+    if (MODE == 0 && hold && !refh && sh.box[4]) {
+        for (int t = B; t < ntiles; t += G) {
+            const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
+            const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
+            const int bx0 = max(-1, X0 + sh.box[0] - NM_HMARGIN), by0 = max(-1, Y0 + sh.box[2] - NM_HMARGIN);
+            const int bw = min(tw + 1, X1 + sh.box[1] + NM_HMARGIN) - bx0 + 1;
+            const int bh = min(th + 1, Y1 + sh.box[3] + NM_HMARGIN) - by0 + 1;
+            wsum += bw > 0 && bh > 0 ? bw * bh : NM_HB + 1;
+        }
+        allw = wsum <= NM_HB;
+    }
     for (int t = solo ? 0 : B; t < ntiles; t += solo ? 1 : G) {
         const int X0 = (t / nty) * NM_TILE, Y0 = (t % nty) * TH;
         const int X1 = min(X0 + NM_TILE - 1, tw - 1), Y1 = min(Y0 + TH - 1, th - 1);
@@ -499,7 +518,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 __syncthreads();  // the bound is written; the previous tile's window is no longer read
                 // the window may reach one sample before and two past the level
                 // on each axis (a support's reach), where it holds the edge samples
-                int m = hold ? NM_HMARGIN : 0, off = -1;
+                int m = hold ? NM_HMARGIN : 0, off = -1, roff = -1;
                 bx0 = max(-1, X0 + sh.box[0] - m);
                 by0 = max(-1, Y0 + sh.box[2] - m);
                 bw = min(tw + 1, X1 + sh.box[1] + m) - bx0 + 1;
@@ -508,9 +527,13 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                     // the level's first pass holds the tile at the block's next
                     // free place in hb when it fits; otherwise the tile is staged
                     // by every pass, with that pass's bound
-                    const int need = (refh ? NM_TILE * TH : 0) + bw * bh, at = sh.hoff;
+                    const int rsz = NM_TILE * TH, need = (refh ? rsz : 0) + bw * bh, at = sh.hoff;
                     if (sh.box[4] && bw > 0 && bh > 0 && at + need <= NM_HB) {
-                        off = at;
+                        off = refh ? at + rsz : at;
+                        if (refh)
+                            roff = at;
+                        else if (allw && wsum + (q + 1) * rsz <= NM_HB + NM_BOX)
+                            roff = wsum + q * rsz;
                     } else {
                         m = 0;
                         bx0 = max(-1, X0 + sh.box[0]);
@@ -519,7 +542,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                         bh = min(th + 1, Y1 + sh.box[3]) - by0 + 1;
                     }
                 }
-                float *wb = off >= 0 ? hb + off + (refh ? NM_TILE * TH : 0) : boxs;
+                float *wb = off >= 0 ? hb + off : boxs;
                 staged = sh.box[4] && bw > 0 && bh > 0 && (off >= 0 || bw * bh <= NM_BOX);
                 // (issuing all of a thread's window loads before storing any measured
                 // no faster here and spilled registers)
@@ -538,8 +561,9 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                     hw[4] = staged;
                     hw[5] = off >= 0;
                     hw[6] = off;
+                    hw[7] = roff;
                     if (off >= 0)
-                        sh.hoff = off + (refh ? NM_TILE * TH : 0) + bw * bh;
+                        sh.hoff = off + bw * bh;
                     else
                         sh.restage = 1;
                 }
@@ -548,7 +572,7 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                     const int x = X0 + tx, y = Y0 + ty + 16 * r;
                     rv[r] = (r < R && act && x < tw && y < th) ? rf[(size_t)y * tw + x] : 0.0f;
                     // kept for the later passes (each thread reads back its own)
-                    if (off >= 0 && refh && r < R && act) hb[off + (ty + 16 * r) * NM_TILE + tx] = rv[r];
+                    if (roff >= 0 && r < R && act) hb[roff + (ty + 16 * r) * NM_TILE + tx] = rv[r];
                 }
             } else {
                 bx0 = hw[0];
@@ -556,13 +580,13 @@ __device__ __forceinline__ void nm_tiles(const NmConst &K, NmShared &sh, float *
                 bw = hw[2];
                 bh = hw[3];
                 staged = hw[4];
-                const int off = __builtin_amdgcn_readfirstlane(hw[6]);
-                win = hb + off + (refh ? NM_TILE * TH : 0);
+                const int off = __builtin_amdgcn_readfirstlane(hw[6]), roff = __builtin_amdgcn_readfirstlane(hw[7]);
+                win = hb + off;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int x = X0 + tx, y = Y0 + ty + 16 * r;
-                    rv[r] = r < R && act ? (refh ? hb[off + (ty + 16 * r) * NM_TILE + tx]
-                                                 : (x < tw && y < th ? rf[(size_t)y * tw + x] : 0.0f))
+                    rv[r] = r < R && act ? (roff >= 0 ? hb[roff + (ty + 16 * r) * NM_TILE + tx]
+                                                      : (x < tw && y < th ? rf[(size_t)y * tw + x] : 0.0f))
                                          : 0.0f;
                 }
             }
@@ -1158,8 +1182,11 @@ __global__ __launch_bounds__(NM_THREADS) void k_nm_search(const NmConst *__restr
                                                           double *__restrict__ part, unsigned *__restrict__ ctr,
                                                           unsigned *__restrict__ err) {
     __shared__ NmShared sh;
-    __shared__ __attribute__((aligned(16))) float boxs[NM_BOX];  // a tile's source window; the fold's staging
-    __shared__ __attribute__((aligned(16))) float hb[NM_HB];  // a held level's tiles
+    // a held level's tiles (hb), then a tile's source window (boxs), one
+    // array: the reference pixels of a level whose windows are all held may
+    // run on from hb into boxs
+    __shared__ __attribute__((aligned(16))) float hbx[NM_HB + NM_BOX];
+    float *const hb = hbx, *const boxs = hbx + NM_HB;
     const NmConst &K = *Kp;
     const int n = K.nfree;
     unsigned gen = 0;
@@ -1843,6 +1870,22 @@ static void search_launch(lsh_frames *f, bool plain = false, hipEvent_t started 
     launch_coop("nm_search", 0.0, (const void *)k_nm_search, dim3(grid), dim3(NM_THREADS), args, 0);
 }
 
+// The search's record (its state words, as the kernel left them) into the
+// handle's statistics: evaluations, ticks and passes per level, participants
+static void search_state_in(lsh_frames *f, const double *out) {
+    f->search_part = (unsigned)out[NM_ST_PART];
+    f->trace_n = (size_t)out[NM_ST_TRACE];
+    f->evals = (uint64_t)out[8];
+    for (size_t l = 0; l < NM_MAXL; l++) {
+        f->lev_evals[l] = (uint64_t)out[9 + l];
+        f->lev_ticks[l] = (uint64_t)out[9 + NM_MAXL + l];
+    }
+    for (int q = 0; q < 4; q++) f->phase_ticks[q] = (uint64_t)out[9 + 2 * NM_MAXL + q];
+    for (int l = 0; l < NM_MAXL; l++)
+        for (int q = 0; q < NM_NPH; q++) f->lev_phase_ticks[l][q] = (uint64_t)out[NM_ST_LPH + NM_NPH * l + q];
+    for (size_t l = 0; l < NM_MAXL; l++) f->lev_passes[l] = (uint64_t)out[13 + 2 * NM_MAXL + l];
+}
+
 static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8, size_t coarsen) {
     const size_t n = f->lw.size();
     search_const(f, free8, coarsen);
@@ -1858,8 +1901,7 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     LSH_CHECK(hipMemcpyAsync(flags, err, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
     sync();
     LSH_REQUIRE(flags[0] == 0, "search: a wait for the other blocks timed out");
-    f->search_part = (unsigned)out[NM_ST_PART];
-    f->trace_n = (size_t)out[NM_ST_TRACE];
+    search_state_in(f, out);
     // algorithmic bytes: each counted cost evaluation (the reference's own,
     // align.rs:581-635) reads the level's reference and target once; the
     // speculative costs a branch leaves unused are this kernel's overhead
@@ -1868,15 +1910,6 @@ static Sim multiscale_align(lsh_frames *f, const Sim &init, const uint8_t *free8
     prof_set_last_bytes(bytes);
     Sim s;
     std::memcpy(s.v, out, 8 * sizeof(double));
-    f->evals = (uint64_t)out[8];
-    for (size_t l = 0; l < NM_MAXL; l++) {
-        f->lev_evals[l] = (uint64_t)out[9 + l];
-        f->lev_ticks[l] = (uint64_t)out[9 + NM_MAXL + l];
-    }
-    for (int q = 0; q < 4; q++) f->phase_ticks[q] = (uint64_t)out[9 + 2 * NM_MAXL + q];
-    for (int l = 0; l < NM_MAXL; l++)
-        for (int q = 0; q < NM_NPH; q++) f->lev_phase_ticks[l][q] = (uint64_t)out[NM_ST_LPH + NM_NPH * l + q];
-    for (size_t l = 0; l < NM_MAXL; l++) f->lev_passes[l] = (uint64_t)out[13 + 2 * NM_MAXL + l];
     return s;
 }
 
@@ -1993,6 +2026,10 @@ void frames_stream_results(lsh_frames *f, size_t n, double *sims, float *gains, 
         copy_mem(e.data(), LSH_HOST, f->rec_evals.p, LSH_DEVICE, n * sizeof(double));
         for (size_t i = 0; i < n; i++) evals[i] = (uint64_t)e[i];
     }
+    // the last search's statistics (lsh_frames_search_stats)
+    std::vector<double> st(NM_STATE);
+    copy_mem(st.data(), LSH_HOST, f->nm_state.p, LSH_DEVICE, NM_STATE * sizeof(double));
+    search_state_in(f, st.data());
 }
 
 static const uint16_t *frame_source(lsh_frames *f, const uint16_t *rgb, lsh_mem mem) {
