@@ -41,6 +41,7 @@ struct Error : std::runtime_error {
 hipStream_t stream();          // the library's stream on the current device (or the OnStream one)
 hipStream_t side_stream();     // a second stream on the current device (default priority), made on first use
 hipStream_t copy_stream();     // a third, for uploads (default priority), made on first use
+hipStream_t copy_stream2();    // a fourth, for the other half of split uploads (default priority)
 void sync();                   // wait for both (and for the OnStream stream, if another)
 // While one lives, this thread's launches and copies go to stream s; cus: the
 // number of CUs s may use when it is restricted to a subset (0: all of them).

@@ -16,6 +16,7 @@ struct DevState {
     hipStream_t s = nullptr;
     hipStream_t side = nullptr;  // a second stream, for work that runs beside the main chain
     hipStream_t copy = nullptr;  // a third, for host-to-device uploads beside both
+    hipStream_t copy2 = nullptr;  // a fourth: uploads split over two copy engines
     void *pinned = nullptr;
     size_t pinned_n = 0;
     Prof prof;
@@ -45,6 +46,12 @@ hipStream_t side_stream() {
     return st.side;
 }
 
+hipStream_t copy_stream2() {
+    DevState &st = dev_state();
+    if (!st.copy2) LSH_CHECK(hipStreamCreateWithFlags(&st.copy2, hipStreamNonBlocking));
+    return st.copy2;
+}
+
 hipStream_t copy_stream() {
     DevState &st = dev_state();
     if (!st.copy) LSH_CHECK(hipStreamCreateWithFlags(&st.copy, hipStreamNonBlocking));
@@ -66,6 +73,7 @@ void sync() {
     LSH_CHECK(hipStreamSynchronize(st.s));
     if (st.side) LSH_CHECK(hipStreamSynchronize(st.side));
     if (st.copy) LSH_CHECK(hipStreamSynchronize(st.copy));
+    if (st.copy2) LSH_CHECK(hipStreamSynchronize(st.copy2));
 }
 
 void *pinned_scratch(size_t n) {
@@ -98,7 +106,7 @@ Prof &prof() { return dev_state().prof; }
 namespace {
 struct PoolBlock {
     void *p;
-    hipEvent_t ev[3];
+    hipEvent_t ev[4];
     int nev;
 };
 struct DevPool {
@@ -191,8 +199,8 @@ void dev_free(void *p, size_t bytes, int dev) {
         Pools &P = pools();
         std::lock_guard<std::mutex> lk(P.mu);
         DevPool &D = P.dev[dev];
-        PoolBlock b{p, {nullptr, nullptr, nullptr}, 0};
-        const hipStream_t ss[3] = {st.s, st.side, st.copy};
+        PoolBlock b{p, {nullptr, nullptr, nullptr, nullptr}, 0};
+        const hipStream_t ss[4] = {st.s, st.side, st.copy, st.copy2};
         bool ok = true;
         for (hipStream_t s : ss) {
             if (!s) continue;
