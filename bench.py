@@ -252,9 +252,16 @@ def wls_precision(job, n, dp, tol=1e-6):
         _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=dp.cg_iters,
                                  warm=warm, u_out=u)
         _, p0, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=0, u_out=u)
+        base = la.WarmNet(seed=1)  # untrained: the blocks' weighted mean depths, prolongated
+        ub = torch.empty_like(d)
+        base.predict(d, c, g, dscale, out=ub)
+        _, rb, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=dp.cg_iters, warm=ub,
+                                u_out=u)
+        base.close()
         out["learned"] = {"infer_ms": round(infer_ms, 2), "iters": rep["iters"], "solve_ms": round(rep["ms"], 2),
                           "ms": round(infer_ms + rep["ms"], 2), "rel": rep["rel"], "true_rel": rep["true_rel"],
-                          "guess_true_rel": g0["true_rel"], "fgs_guess_true_rel": p0["true_rel"]}
+                          "guess_true_rel": g0["true_rel"], "fgs_guess_true_rel": p0["true_rel"],
+                          "untrained_iters": rb["iters"]}
         la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=dp.cg_iters, warm=warm, u_out=u)
         out["learned"]["max_err_vs_fp64"] = float((u.double() - oracle).abs().max())
         log(f"WLS learned: {out['learned']}")

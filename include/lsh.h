@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LSH_ABI_VERSION 2u
+#define LSH_ABI_VERSION 3u
 
 typedef enum lsh_mem { LSH_HOST = 0, LSH_DEVICE = 1 } lsh_mem;
 
@@ -110,6 +110,13 @@ void lsh_free_str(char *s);
 char *lsh_stream(void **stream_out);
 /* Wait for all queued work of the library on the current device. */
 char *lsh_sync(void);
+/* The library's device memory on the current device: bytes in blocks handed
+ * out, and bytes in released blocks it keeps for reuse (at most an eighth of
+ * the device's memory, at least 4 GiB). */
+char *lsh_mem_stats(uint64_t *live_bytes, uint64_t *cached_bytes);
+/* Return every kept block of the current device to it (waits for the work
+ * that last used them), for callers that need the memory elsewhere. */
+char *lsh_mem_release(void);
 
 /* Per-kernel timing with HIP events on the library's stream: enable, read
  * (names separated by '\n' into names; per kernel the total ms, the total
@@ -449,12 +456,18 @@ char *lsh_wls_solve_noalloc(const float *guide, const float *conf, const float *
 
 /* No counterpart in the reference, whose PCG starts from its FGS sweeps
  * (depth.rs:1019-1034): a bf16 convolutional encoder-decoder (channels
- * 16/32/64/64, 3x3 convolutions, ReLU, 254,481 parameters) mapping the depth
- * pass's grid planes (peak depth, p90-normalised confidence, guide luma;
- * lsh_depth_grid) to a starting depth for lsh_wls_solve.  Inference and
- * training run on the device (bf16 MFMA, f32 accumulation, f32 master weights,
- * Adam).  Depth enters the network divided by dscale (the frame-index range,
- * n - 1) and leaves multiplied by it; the WLS solution scales the same way.
+ * 16/32/64/64, 3x3 convolutions, ReLU, 255,489 parameters) on the grid of 8 x 8
+ * blocks of the WLS grid.  From the system's planes (peak depth, p90-normalised
+ * confidence, guide luma; lsh_depth_grid) averaged over the blocks -- the plain
+ * mean depth and the weighted means over 4 x 4, 16 x 16 and 64 x 64 blocks of
+ * blocks, each as its difference from the block's mean depth weighted by the
+ * data weight max(conf, 0) + 1e-4, that weight's mean and log, the mean edge
+ * conductances exp(-|dg| / sigma_c) -- it predicts the solution's block means
+ * as that weighted mean plus a correction; their bilinear prolongation is the
+ * start for lsh_wls_solve, in place of the FGS sweeps.  Inference and training run on the device (bf16 MFMA, f32
+ * accumulation, f32 master weights, Adam).  Depth enters divided by dscale (the
+ * frame-index range, n - 1) and leaves multiplied by it; the WLS solution
+ * scales the same way.
  *
  * Parameters travel as one flat f32 array, layer by layer in network order
  * (e0a, e0b, e1a, e1b, e2a, e2b, e3a, e3b, d2, d1, d0, out): the weights
@@ -465,21 +478,24 @@ char *lsh_warm_nparams(lsh_warm *net, uint32_t *count);
 char *lsh_warm_get(lsh_warm *net, float *params, uint32_t count);
 /* Load parameters (resets the optimiser state). */
 char *lsh_warm_set(lsh_warm *net, const float *params, uint32_t count);
-/* The training loss: 0 (the default) the mean squared error of the normalised
- * prediction; 1 the energy norm of that error under each image's own WLS
- * operator (data weights max(conf, 0) + 1e-4, edges lambda exp(-|dg| /
- * sigma_c) of the guide), the norm MG-PCG reduces, per cell. */
-char *lsh_warm_set_loss(lsh_warm *net, uint32_t mode, float lambda, float sigma_c);
-/* nimg images of w*h planes each (image-major) in in_mem; u (nimg*w*h) in out_mem. */
+/* The guide scale sigma_c of the WLS systems served (default 0.04): the edge
+ * conductances among the inputs are exp(-|dg| / sigma_c). */
+char *lsh_warm_set_sigma(lsh_warm *net, float sigma_c);
+/* nimg systems of w*h planes each (image-major) in in_mem; the start u
+ * (nimg*w*h) in out_mem. */
 char *lsh_warm_predict(lsh_warm *net, const float *depth, const float *conf, const float *guide, lsh_mem in_mem,
                        uint32_t nimg, uint32_t w, uint32_t h, float dscale, float *u, lsh_mem out_mem);
-/* The batch's mean squared error of the normalised prediction against target
- * (the WLS solution) and its gradient (host, count floats, nullable), from the
- * device's backward pass. */
+/* The batch's loss against target (the WLS solution, nimg*w*h): per system
+ * the mean squared error of the predicted block means against the target's
+ * (depth / dscale, in units of 1/32), divided by the mean square of the
+ * target's own deviation from the weighted mean depth (at least 1e-4), then
+ * averaged over the systems; and its gradient (host, count floats, nullable),
+ * from the device's backward pass. */
 char *lsh_warm_grads(lsh_warm *net, const float *depth, const float *conf, const float *guide, const float *target,
                      lsh_mem in_mem, uint32_t nimg, uint32_t w, uint32_t h, float dscale, double *loss, float *grads,
                      uint32_t count);
-/* One Adam step (beta 0.9 / 0.999) on the batch; loss before the step. */
+/* One Adam step (beta 0.9 / 0.999) on the batch; loss before the step.  A
+ * non-finite loss is reported and leaves the weights as they were. */
 char *lsh_warm_train_step(lsh_warm *net, const float *depth, const float *conf, const float *guide,
                           const float *target, lsh_mem in_mem, uint32_t nimg, uint32_t w, uint32_t h, float dscale,
                           float lr, double *loss);

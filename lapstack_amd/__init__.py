@@ -701,9 +701,11 @@ def depth_grid(slices, n: int, fused, params: Optional[DepthParams] = None, out_
 
 class WarmNet:
     """The learned warm start of the WLS solve (include/lsh.h lsh_warm_*): a
-    bf16 convolutional encoder-decoder from the grid planes (peak depth,
-    confidence, guide) to a starting depth, run and trained on the device.
-    Planes are (h, w) or (nimg, h, w), numpy or device tensors."""
+    bf16 convolutional encoder-decoder on the grid of 8 x 8 blocks that
+    predicts the solution's block means from the system's planes (peak depth,
+    confidence, guide) averaged over the blocks; the start is their bilinear
+    prolongation.  Run and trained on the device.  Planes are (h, w) or (nimg,
+    h, w), numpy or device tensors."""
 
     def __init__(self, seed: int = 0):
         self._lib = _lib.load()
@@ -725,11 +727,10 @@ class WarmNet:
             raise HipError(f"warm: {p.size} parameters, the network has {self.nparams}")
         _lib.check(self._lib.lsh_warm_set(self._h, p.ctypes.data_as(C.c_void_p), self.nparams))
 
-    def set_loss(self, mode: str = "mse", lam: float = 0.0, sigma_c: float = 1.0):
-        """Training loss: "mse", or "energy" (the error's energy norm under the
-        WLS operator with lambda ``lam`` and guide scale ``sigma_c``)."""
-        m = {"mse": 0, "energy": 1}[mode]
-        _lib.check(self._lib.lsh_warm_set_loss(self._h, m, float(lam), float(sigma_c)))
+    def set_sigma(self, sigma_c: float):
+        """The guide scale of the WLS systems served (edge conductances
+        exp(-|dg| / sigma_c) are among the inputs); 0.04 by default."""
+        _lib.check(self._lib.lsh_warm_set_sigma(self._h, float(sigma_c)))
 
     def save(self, path: str):
         self.get().tofile(path)

@@ -8,7 +8,7 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LSH_LIB", os.path.join(HERE, "liblsh.so"))
 LSH_HOST, LSH_DEVICE = 0, 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _p = C.c_void_p
 _u32 = C.c_uint32
@@ -74,6 +74,8 @@ SIGNATURES = [
     ("lsh_free_str", None, [_p]),
     ("lsh_stream", _p, [C.POINTER(_p)]),
     ("lsh_sync", _p, []),
+    ("lsh_mem_stats", _p, [_p, _p]),
+    ("lsh_mem_release", _p, []),
     ("lsh_prof_enable", _p, [_u32]),
     ("lsh_prof_read", _p, [_p, C.c_size_t, _p, _p, _p, _u32, C.POINTER(_u32)]),
     ("lsh_prof_reset", _p, []),
@@ -106,7 +108,7 @@ SIGNATURES = [
     ("lsh_warm_nparams", _p, [_p, C.POINTER(_u32)]),
     ("lsh_warm_get", _p, [_p, _p, _u32]),
     ("lsh_warm_set", _p, [_p, _p, _u32]),
-    ("lsh_warm_set_loss", _p, [_p, _u32, C.c_float, C.c_float]),
+    ("lsh_warm_set_sigma", _p, [_p, C.c_float]),
     ("lsh_warm_predict", _p, [_p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, _p, C.c_int]),
     ("lsh_warm_grads", _p, [_p, _p, _p, _p, _p, C.c_int, _u32, _u32, _u32, C.c_float, C.POINTER(C.c_double), _p,
                             _u32]),
@@ -203,6 +205,18 @@ def stream_ptr() -> int:
 
 def sync():
     check(load().lsh_sync())
+
+
+def mem_stats():
+    """(live, cached) bytes of the library's device memory on the current device."""
+    live, cached = C.c_uint64(), C.c_uint64()
+    check(load().lsh_mem_stats(C.byref(live), C.byref(cached)))
+    return live.value, cached.value
+
+
+def mem_release():
+    """Return the library's kept device blocks to the device."""
+    check(load().lsh_mem_release())
 
 
 def prof_enable(on: bool):

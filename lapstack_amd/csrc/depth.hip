@@ -90,7 +90,9 @@ template <bool ONE, bool R13>
 __global__ __launch_bounds__(256) void k_focus_slice(const float *__restrict__ planes, int w, int h, int k, int CY,
                                                      const TapTable tab, Taps T, float wd, float wr,
                                                      float *__restrict__ out, int dw, int dh,
-                                                     const unsigned *__restrict__ out_idx) {
+                                                     const unsigned *__restrict__ out_idx,
+                                                     float *const *__restrict__ out_base) {
+    if (out_base) out = out_base[0];
     if (out_idx) out += (size_t)out_idx[0] * dw * dh;
     extern __shared__ float Y[];
     const int TX = 64, TY = 4 * CY, R = T.halo;
@@ -222,17 +224,18 @@ static bool rdf13_order_holds() {
 // (uploaded when the measure changes).
 static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
                              uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
-                             float *out, const unsigned *out_idx = nullptr, bool luma = false);
+                             float *out, const unsigned *out_idx = nullptr, bool luma = false,
+                             float *const *out_base = nullptr);
 
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out, const unsigned *out_idx,
-                 bool luma) {
+                 bool luma, float *const *out_base) {
     focus_slice_with(f->taps_dy, f->taps_dx, f->taps_wt, f->ntaps, f->taps_key, f->w, f->h, planes, p, out, out_idx,
-                     luma);
+                     luma, out_base);
 }
 
 static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBuf<float> &taps_wt, uint32_t &ntaps,
                              uint32_t &taps_key, size_t w, size_t h, const float *planes, const lsh_depth_params &p,
-                             float *out, const unsigned *out_idx, bool luma) {
+                             float *out, const unsigned *out_idx, bool luma, float *const *out_base) {
     LSH_REQUIRE(p.scale <= 2, "focus slice: scale above 2 is not supported");
     LSH_REQUIRE(p.focus_kind <= 1, "focus slice: unknown focus measure");
     int k = 1 << p.scale;
@@ -291,13 +294,13 @@ static void focus_slice_with(DBuf<int32_t> &taps_dy, DBuf<int32_t> &taps_dx, DBu
     const dim3 grid(cdiv(dw, 64), cdiv(dh, 4 * CY)), blk(64, 4);
     const float *src = planes;
     if (luma && r13)
-        launch_b("focus_slice", by, k_focus_slice<true, true>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
+        launch_b("focus_slice", by, k_focus_slice<true, true>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx, out_base);
     else if (luma)
-        launch_b("focus_slice", by, k_focus_slice<true, false>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
+        launch_b("focus_slice", by, k_focus_slice<true, false>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx, out_base);
     else if (r13)
-        launch_b("focus_slice", by, k_focus_slice<false, true>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
+        launch_b("focus_slice", by, k_focus_slice<false, true>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx, out_base);
     else
-        launch_b("focus_slice", by, k_focus_slice<false, false>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx);
+        launch_b("focus_slice", by, k_focus_slice<false, false>, grid, blk, shm, src, (int)w, (int)h, k, CY, tab, T, wd, wr, out, dw, dh, out_idx, out_base);
 }
 
 // ----------------------------------------------------------------- helpers

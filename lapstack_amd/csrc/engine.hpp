@@ -106,9 +106,12 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
 // take the CUs the search's grid needs at its start.
 void frames_stream_slots(lsh_frames *f);
 // out_idx (if given): the slice goes to out + out_idx[0] * (its cell count);
-// luma: planes is the frame's luma plane (one plane), not its three planes
+// luma: planes is the frame's luma plane (one plane), not its three planes;
+// out_base (if given): out is read from device memory there, so a captured
+// graph's launch holds no pointer to the caller's buffer (the runtime keeps
+// a buffer that a graph's kernel arguments named from going back to the device)
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out,
-                 const unsigned *out_idx = nullptr, bool luma = false);  // depth.hip
+                 const unsigned *out_idx = nullptr, bool luma = false, float *const *out_base = nullptr);  // depth.hip
 void frames_target(lsh_frames *f, const FrameRef &src, int slot);
 void frames_chain(lsh_frames *f, int slot, bool plain_search, hipEvent_t started = nullptr);
 // The rest of frame k's step, off the chain: the brightness sums and gains,

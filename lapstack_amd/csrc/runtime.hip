@@ -100,9 +100,13 @@ Prof &prof() { return dev_state().prof; }
 // size class (powers of two up to 1 MiB, then multiples of 2 MiB), each with
 // an event recorded on each of the library's streams at its release; a block
 // is handed out again only once all of its events have completed, so no work
-// queued before the release can still be using it.  When the device has no
-// room for a new block, the cached blocks are returned to it and the
-// allocation tried again.  LSH_POOL=0 turns the cache off.
+// queued before the release can still be using it; a request may take a
+// cached block of up to 5/4 of its class.  The cache holds at most an eighth
+// of the device's memory (at least 4 GiB): past that, released blocks whose
+// events have completed go back to the device, largest first, so buffers whose
+// sizes change from call to call (a crop, a frame count) do not pile up.  When
+// the device has no room for a new block, every cached block is returned to it
+// and the allocation tried again.  LSH_POOL=0 turns the cache off.
 namespace {
 struct PoolBlock {
     void *p;
@@ -110,8 +114,10 @@ struct PoolBlock {
     int nev;
 };
 struct DevPool {
-    std::multimap<size_t, PoolBlock> freed;
+    std::multimap<size_t, PoolBlock> freed;  // by the block's own size
+    std::map<void *, size_t> live;           // blocks handed out: their own sizes
     std::vector<hipEvent_t> spare;
+    size_t cached = 0, cap = 0;
 };
 struct Pools {
     std::mutex mu;
@@ -154,6 +160,18 @@ void pool_trim(DevPool &D) {
         (void)hipFree(kv.second.p);
     }
     D.freed.clear();
+    D.cached = 0;
+}
+// returns completed cached blocks, largest first, until the cache is within its cap
+void pool_fit(DevPool &D) {
+    for (auto it = D.freed.end(); D.cached > D.cap && it != D.freed.begin();) {
+        --it;
+        if (!block_ready(it->second)) continue;
+        for (int i = 0; i < it->second.nev; i++) D.spare.push_back(it->second.ev[i]);
+        (void)hipFree(it->second.p);
+        D.cached -= it->first;
+        it = D.freed.erase(it);
+    }
 }
 }  // namespace
 
@@ -168,11 +186,17 @@ void *dev_alloc(size_t bytes, int *dev) {
     Pools &P = pools();
     std::lock_guard<std::mutex> lk(P.mu);
     DevPool &D = P.dev[*dev];
-    auto r = D.freed.equal_range(cls);
-    for (auto it = r.first; it != r.second; ++it)
+    if (!D.cap) {
+        size_t fr = 0, total = 0;
+        LSH_CHECK(hipMemGetInfo(&fr, &total));
+        D.cap = std::max(total / 8, (size_t)4 << 30);
+    }
+    for (auto it = D.freed.lower_bound(cls); it != D.freed.end() && it->first <= cls + cls / 4; ++it)
         if (block_ready(it->second)) {
             void *p = it->second.p;
             for (int i = 0; i < it->second.nev; i++) D.spare.push_back(it->second.ev[i]);
+            D.cached -= it->first;
+            D.live[p] = it->first;
             D.freed.erase(it);
             return p;
         }
@@ -182,6 +206,7 @@ void *dev_alloc(size_t bytes, int *dev) {
         pool_trim(D);
         LSH_CHECK(hipMalloc(&p, cls));
     }
+    D.live[p] = cls;
     return p;
 }
 
@@ -199,6 +224,9 @@ void dev_free(void *p, size_t bytes, int dev) {
         Pools &P = pools();
         std::lock_guard<std::mutex> lk(P.mu);
         DevPool &D = P.dev[dev];
+        auto lv = D.live.find(p);
+        const size_t size = lv != D.live.end() ? lv->second : size_class(bytes);
+        if (lv != D.live.end()) D.live.erase(lv);
         PoolBlock b{p, {nullptr, nullptr, nullptr, nullptr}, 0};
         const hipStream_t ss[4] = {st.s, st.side, st.copy, st.copy2};
         bool ok = true;
@@ -220,7 +248,9 @@ void dev_free(void *p, size_t bytes, int dev) {
             b.ev[b.nev++] = e;
         }
         if (ok) {
-            D.freed.emplace(size_class(bytes), b);
+            D.freed.emplace(size, b);
+            D.cached += size;
+            if (D.cached > D.cap) pool_fit(D);
         } else {  // no event to guard the block with: give it back to the device
             for (int i = 0; i < b.nev; i++) D.spare.push_back(b.ev[i]);
             (void)hipGetLastError();
@@ -356,6 +386,31 @@ char *lsh_stream(void **out) {
 char *lsh_sync(void) {
     LSH_API_BEGIN
     sync();
+    LSH_API_END
+}
+
+char *lsh_mem_stats(uint64_t *live, uint64_t *cached) {
+    LSH_API_BEGIN
+    LSH_REQUIRE(live && cached, "lsh_mem_stats: null output");
+    int dev = 0;
+    LSH_CHECK(hipGetDevice(&dev));
+    Pools &P = pools();
+    std::lock_guard<std::mutex> lk(P.mu);
+    DevPool &D = P.dev[dev];
+    size_t l = 0;
+    for (auto &kv : D.live) l += kv.second;
+    *live = l;
+    *cached = D.cached;
+    LSH_API_END
+}
+
+char *lsh_mem_release(void) {
+    LSH_API_BEGIN
+    int dev = 0;
+    LSH_CHECK(hipGetDevice(&dev));
+    Pools &P = pools();
+    std::lock_guard<std::mutex> lk(P.mu);
+    pool_trim(P.dev[dev]);
     LSH_API_END
 }
 

@@ -331,6 +331,12 @@ char *stream_impl(lsh_frames *f, lsh_fuser *fu, const Source &src, uint32_t n, c
             return graph ? sbuf[k & 1].p : sall.p + (size_t)k * M;
         };
         DBuf<unsigned> fidx(1);  // the fold's frame index (side work only)
+        // a graph writes the slices through a device copy of their address:
+        // the runtime keeps any buffer a captured launch's arguments name from
+        // going back to the device when the caller frees it, even after the
+        // graph is destroyed (on the 45 MP grid: the whole slice stack per call)
+        DBuf<float *> sbase(1);
+        if (graph && direct) copy_mem(sbase.p, LSH_DEVICE, &slices, LSH_HOST, sizeof(float *));
         launch("stream_fold_index", k_counter_set, dim3(1), dim3(64), 0, fidx.p, 1u);
         const hipEvent_t e_main = E.make();
         hipEvent_t e_target[2] = {E.make(), E.make()}, e_warp[2] = {E.make(), E.make()},
@@ -343,7 +349,8 @@ char *stream_impl(lsh_frames *f, lsh_fuser *fu, const Source &src, uint32_t n, c
         auto side = [&](uint32_t k, const FrameRef &src) {
             frames_side_warp(f, src, (int)(k & 1), planes[k & 1], bright, measure ? fluma[k & 1].p : nullptr);
             if (measure)
-                focus_slice(f, fluma[k & 1].p, *measure, slice_at(k), graph && direct ? fidx.p : nullptr, true);
+                focus_slice(f, fluma[k & 1].p, *measure, graph && direct ? nullptr : slice_at(k),
+                            graph && direct ? fidx.p : nullptr, true, graph && direct ? sbase.p : nullptr);
             fuser_push_dev(fu, planes[k & 1], fidx.p);
             launch("stream_fold_next", k_counter_next, dim3(1), dim3(64), 0, fidx.p);
         };

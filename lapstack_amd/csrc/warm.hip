@@ -1,11 +1,15 @@
 // warm.hip -- the learned warm start of the WLS solve: a small convolutional
-// encoder-decoder that maps the depth pass's grid planes (peak depth,
-// confidence, guide luma) to a starting depth for MG-PCG.  Trained and run on
-// gfx950 by the kernels here: implicit-GEMM 3x3 convolutions on bf16 MFMA with
-// f32 accumulation for the forward pass and both halves of the backward pass
-// (input gradient, weight gradient), 2x2 average pooling, nearest upsampling,
-// the MSE loss and Adam on f32 master weights.  No reference counterpart: the
-// reference starts PCG from its FGS sweeps (depth.rs:1019-1034).
+// encoder-decoder that predicts the solution's means over 8 x 8 blocks of the
+// WLS grid from the system's own planes averaged over the same blocks (peak
+// depth as differences from the block's weighted mean, data weight, edge
+// conductances, and coarser push-pull means); the
+// start MG-PCG runs from is their bilinear prolongation, in place of the FGS
+// sweeps.  Trained and run on gfx950 by the kernels here: implicit-GEMM 3x3
+// convolutions on bf16 MFMA with f32 accumulation for the forward pass and
+// both halves of the backward pass (input gradient, weight gradient), 2x2
+// average pooling, nearest upsampling, a per-image relative squared error and
+// Adam on f32 master weights.  No reference counterpart: the reference starts
+// PCG from its FGS sweeps (depth.rs:1019-1034).
 //
 // Layout.  Activations are bf16, channel-last ([image][y][x][channel]) with a
 // channel stride that may exceed the channel count, so the decoder's
@@ -15,10 +19,11 @@
 // to multiples of 16; the input-gradient pass runs the same kernel over the
 // flipped, transposed copy [8-tap][in][out].
 //
-// Network (channels): 3 -> e0 16,16 -> pool -> e1 32,32 -> pool -> e2 64,64 ->
-// pool -> e3 64,64 -> up ++ e2 -> d2 64 -> up ++ e1 -> d1 32 -> up ++ e0 -> d0
-// 16 -> out 1; ReLU after every conv but the last; the prediction is the
-// normalised peak depth plus the output.  254,481 parameters.
+// Network (channels, on the coarse grid): 10 -> e0 16,16 -> pool -> e1 32,32 ->
+// pool -> e2 64,64 -> pool -> e3 64,64 -> up ++ e2 -> d2 64 -> up ++ e1 -> d1
+// 32 -> up ++ e0 -> d0 16 -> out 1; ReLU after every conv but the last; the
+// predicted block mean (depth / dscale) is the weighted mean dm of the block's
+// peak depths plus 1/32 of the output.  255,489 parameters.
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -54,6 +59,11 @@ struct View {
     uint16_t *p;
     int H, W, C, cs;
 };
+
+constexpr int WB = 8;                  // the coarse grid's block edge, in WLS cells
+constexpr int WNFEAT = 10;             // input channels
+constexpr int WPP[3] = {4, 16, 64};    // the push-pull levels, in coarse cells
+constexpr float WKAPPA = 1.0f / 32.0f; // the output's scale, in depth / dscale
 
 // ------------------------------------------------------------------ conv
 // Implicit GEMM on a 4-row x 64-pixel tile per workgroup: wave w owns row w of
@@ -328,41 +338,155 @@ __global__ void k_up_bwd(View dfine, View dcoarse, int nimg) {
     dcoarse.p[(((size_t)b * dcoarse.H + y) * dcoarse.W + x) * dcoarse.cs + c] = f2bf(s);
 }
 
-// the network's input: depth / dscale, confidence, guide; channels 3..15 zero
-__global__ void k_pack(const float *__restrict__ d, const float *__restrict__ conf, const float *__restrict__ g,
-                       size_t n, float inv_scale, View x) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// ------------------------------------------------------------------ ends
+// The network runs on the coarse grid of WB x WB blocks of the WLS grid and
+// predicts the block means of the solution; the warm start is their bilinear
+// prolongation.  Partial blocks at the right and bottom edges average the
+// cells that exist.
+
+// Per coarse cell, over the WLS cells of its block: dm = sum(w d) / sum(w) /
+// dscale with the data weight w = max(conf, 0) + 1e-4, the mean of d / dscale,
+// the mean of w, the means of the edge conductances exp(-|g - g_right| /
+// sigma_c) (0 past the last column) and exp(-|g - g_below| / sigma_c) (0 past
+// the last row) as the solver builds them, and with a target t the mean of
+// t / dscale - dm.  cf holds the planes [dm, dmean, w, ax, ay, tc], each
+// nimg*ch*cw.
+__global__ void k_feat_blocks(const float *__restrict__ d, const float *__restrict__ conf, const float *__restrict__ g,
+                              const float *__restrict__ t, int nimg, int h, int w, int ch, int cw, float inv_scale,
+                              float inv_sigma, float *__restrict__ cf) {
+    const size_t nc = (size_t)nimg * ch * cw;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const int cx = (int)(i % cw), cy = (int)((i / cw) % ch), b = (int)(i / ((size_t)cw * ch));
+    const int x0 = cx * WB, y0 = cy * WB, x1 = min(x0 + WB, w), y1 = min(y0 + WB, h);
+    float sw = 0.0f, swd = 0.0f, sd = 0.0f, sax = 0.0f, say = 0.0f, st = 0.0f;
+    for (int y = y0; y < y1; y++) {
+        const size_t row = ((size_t)b * h + y) * w;
+        for (int x = x0; x < x1; x++) {
+            const float wi = fmaxf(conf[row + x], 0.0f) + 1e-4f, di = d[row + x], gi = g[row + x];
+            sw += wi;
+            swd += wi * di;
+            sd += di;
+            if (x + 1 < w) sax += expf(fabsf(gi - g[row + x + 1]) * inv_sigma);
+            if (y + 1 < h) say += expf(fabsf(gi - g[row + w + x]) * inv_sigma);
+            if (t) st += t[row + x];
+        }
+    }
+    const float inv_n = 1.0f / (float)((y1 - y0) * (x1 - x0));
+    const float dm = swd / sw * inv_scale;
+    cf[i] = dm;
+    cf[nc + i] = sd * inv_n * inv_scale;
+    cf[2 * nc + i] = sw * inv_n;
+    cf[3 * nc + i] = sax * inv_n;
+    cf[4 * nc + i] = say * inv_n;
+    if (t) cf[5 * nc + i] = st * inv_n * inv_scale - dm;
+}
+
+// The push-pull level k: per cell of the grid k times coarser (kh = ceil(ch /
+// k)), log10 of the mean data weight and the weight-averaged dm of the coarse
+// cells inside.
+__global__ void k_feat_pp(const float *__restrict__ cf, int nimg, int ch, int cw, int k, int kh, int kw,
+                          float *__restrict__ lpk, float *__restrict__ mk) {
+    const size_t nc = (size_t)nimg * ch * cw;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nimg * kh * kw) return;
+    const int kx = (int)(i % kw), ky = (int)((i / kw) % kh), b = (int)(i / ((size_t)kw * kh));
+    const int x1 = min(kx * k + k, cw), y1 = min(ky * k + k, ch);
+    float sw = 0.0f, swd = 0.0f;
+    for (int y = ky * k; y < y1; y++)
+        for (int x = kx * k; x < x1; x++) {
+            const size_t j = ((size_t)b * ch + y) * cw + x;
+            const float pw = cf[2 * nc + j];
+            sw += pw;
+            swd += pw * cf[j];
+        }
+    lpk[i] = log10f(sw / (float)((y1 - ky * k) * (x1 - kx * k)));
+    mk[i] = swd / sw;
+}
+
+// bilinear resampling of the (ih, iw) plane p at the cell (oy, ox) of an
+// (oh, ow) grid over the same area: source position (o + 1/2) in/out - 1/2,
+// clamped at 0, its right / lower neighbour clamped to the last cell
+__device__ __forceinline__ float resample(const float *__restrict__ p, int ih, int iw, int oh, int ow, int oy,
+                                          int ox) {
+    const float sy = fmaxf(((float)oy + 0.5f) * ((float)ih / (float)oh) - 0.5f, 0.0f);
+    const float sx = fmaxf(((float)ox + 0.5f) * ((float)iw / (float)ow) - 0.5f, 0.0f);
+    const int y0 = (int)sy, x0 = (int)sx;
+    const int y1 = y0 + (y0 < ih - 1 ? 1 : 0), x1 = x0 + (x0 < iw - 1 ? 1 : 0);
+    const float ly = sy - (float)y0, lx = sx - (float)x0;
+    return (1.0f - ly) * ((1.0f - lx) * p[(size_t)y0 * iw + x0] + lx * p[(size_t)y0 * iw + x1]) +
+           ly * ((1.0f - lx) * p[(size_t)y1 * iw + x0] + lx * p[(size_t)y1 * iw + x1]);
+}
+
+// the push-pull levels of a batch: per level l the planes log10 mean weight
+// and weighted mean dm, each nimg * kh[l] * kw[l]
+struct PushPull {
+    const float *lpk[3], *mk[3];
+    int kh[3], kw[3];
+};
+
+// The network's input on the coarse grid.  Depth enters only as differences
+// from the block's own weighted mean dm, in units of WKAPPA (the solution's
+// deviation from dm does not change when a constant is added to the depth,
+// and differences of nearly equal values would not survive bf16): dmean - dm,
+// then per push-pull level its weighted mean resampled to the coarse grid
+// minus dm; then w, log10(w) / 4 + 1, ax, ay, and per push-pull level log10
+// of its mean weight / 4 + 1 resampled.  Channels 10..15 zero.
+__global__ void k_feat_pack(const float *__restrict__ cf, PushPull pp, int nimg, int ch, int cw, View x) {
+    const size_t nc = (size_t)nimg * ch * cw;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const int cx = (int)(i % cw), cy = (int)((i / cw) % ch), b = (int)(i / ((size_t)cw * ch));
+    const float dm = cf[i], pw = cf[2 * nc + i], r = 1.0f / WKAPPA;
+    float v[WNFEAT];
+    v[0] = (cf[nc + i] - dm) * r;
+    v[4] = pw;
+    v[5] = log10f(pw) * 0.25f + 1.0f;
+    v[6] = cf[3 * nc + i];
+    v[7] = cf[4 * nc + i];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        const size_t o = (size_t)b * pp.kh[l] * pp.kw[l];
+        v[1 + l] = (resample(pp.mk[l] + o, pp.kh[l], pp.kw[l], ch, cw, cy, cx) - dm) * r;
+        if (l < 2) v[8 + l] = resample(pp.lpk[l] + o, pp.kh[l], pp.kw[l], ch, cw, cy, cx) * 0.25f + 1.0f;
+    }
     uint16_t *o = x.p + i * x.cs;
-    o[0] = f2bf(d[i] * inv_scale);
-    o[1] = f2bf(conf[i]);
-    o[2] = f2bf(g[i]);
 #pragma unroll
-    for (int c = 3; c < 16; c++) o[c] = 0;
+    for (int c = 0; c < WNFEAT; c++) o[c] = f2bf(v[c]);
+#pragma unroll
+    for (int c = WNFEAT; c < 16; c++) o[c] = 0;
 }
 
-// prediction (depth units): (d / dscale + out) * dscale
-__global__ void k_predict(const float *__restrict__ out32, const float *__restrict__ d, size_t n, float inv_scale,
-                          float scale, float *__restrict__ u) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u[i] = (d[i] * inv_scale + out32[i]) * scale;
+// the warm start on the WLS grid: dscale times the bilinear prolongation of
+// the predicted block means dm + WKAPPA out
+__global__ void k_prolong(const float *__restrict__ dm, const float *__restrict__ out32, int nimg, int ch, int cw,
+                          int h, int w, float scale, float *__restrict__ u) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nimg * h * w) return;
+    const int x = (int)(i % w), y = (int)((i / w) % h), b = (int)(i / ((size_t)w * h));
+    const size_t o = (size_t)b * ch * cw;
+    const float sy = fmaxf(((float)y + 0.5f) * ((float)ch / (float)h) - 0.5f, 0.0f);
+    const float sx = fmaxf(((float)x + 0.5f) * ((float)cw / (float)w) - 0.5f, 0.0f);
+    const int y0 = (int)sy, x0 = (int)sx;
+    const int y1 = y0 + (y0 < ch - 1 ? 1 : 0), x1 = x0 + (x0 < cw - 1 ? 1 : 0);
+    const float ly = sy - (float)y0, lx = sx - (float)x0;
+    auto pc = [&](int yy, int xx) {
+        const size_t j = o + (size_t)yy * cw + xx;
+        return dm[j] + WKAPPA * out32[j];
+    };
+    u[i] = scale * ((1.0f - ly) * ((1.0f - lx) * pc(y0, x0) + lx * pc(y0, x1)) +
+                    ly * ((1.0f - lx) * pc(y1, x0) + lx * pc(y1, x1)));
 }
 
-// MSE loss over the pixels and its gradient into channel 0 of dout (bf16);
-// per-block f64 partial sums of the squared error
-__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, const float *__restrict__ d,
-                                              const float *__restrict__ t, size_t n, float inv_scale, float gscale,
-                                              View dout, double *__restrict__ part) {
+// the loss weight of each image: 1 / max(mean((tc / WKAPPA)^2), 1e-4), so
+// that every image's error counts relative to its own target's size
+__global__ __launch_bounds__(256) void k_loss_weight(const float *__restrict__ tc, int npx, float *__restrict__ wimg) {
     __shared__ double sh[256];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float *p = tc + (size_t)blockIdx.x * npx;
     double s = 0.0;
-    if (i < n) {
-        float e = (d[i] * inv_scale + out32[i]) - t[i] * inv_scale;
-        s = (double)e * (double)e;
-        uint16_t *o = dout.p + i * dout.cs;
-        o[0] = f2bf(gscale * e);
-#pragma unroll
-        for (int c = 1; c < 16; c++) o[c] = 0;
+    for (int i = threadIdx.x; i < npx; i += 256) {
+        const double v = (double)(p[i] * (1.0f / WKAPPA));
+        s += v * v;
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -370,49 +494,23 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, c
         if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) wimg[blockIdx.x] = (float)(1.0 / fmax(sh[0] / npx, 1e-4));
 }
 
-// The error of the normalised prediction, e = d / dscale + out - t / dscale.
-__global__ void k_err(const float *__restrict__ out32, const float *__restrict__ d, const float *__restrict__ t,
-                      size_t n, float inv_scale, float *__restrict__ e) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) e[i] = (d[i] * inv_scale + out32[i]) - t[i] * inv_scale;
-}
-
-// The energy norm of the error under the WLS operator of each image (the
-// norm MG-PCG reduces): sum over cells of w e^2 + lambda a (e - e_right)^2 +
-// lambda a (e - e_below)^2, with w = max(conf, 0) + 1e-4 and a =
-// exp(-|g - g'| / sigma) as the solver builds them (depth.rs:653-672, 994),
-// over each image with free borders.  Its gradient 2 (A e) / N goes into
-// channel 0 of dout; per-block f64 partial sums of the loss.
-__global__ __launch_bounds__(256) void k_energy(const float *__restrict__ e, const float *__restrict__ conf,
-                                                const float *__restrict__ g, int H, int W, size_t n, float lambda,
-                                                float inv_sigma, float gscale, View dout,
-                                                double *__restrict__ part) {
+// The loss: the mean over the images of wimg * mean((out - tc / WKAPPA)^2),
+// its gradient into channel 0 of dout (bf16); per-block f64 partial sums
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, const float *__restrict__ tc,
+                                              const float *__restrict__ wimg, int npx, size_t n, float gscale,
+                                              View dout, double *__restrict__ part) {
     __shared__ double sh[256];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
     if (i < n) {
-        const int x = (int)(i % W), y = (int)((i / W) % H);
-        const float ei = e[i], gi = g[i];
-        const float wi = fmaxf(conf[i], 0.0f) + 1e-4f;
-        float ae = wi * ei;
-        s = (double)wi * ei * ei;
-        if (x + 1 < W) {
-            const float a = lambda * expf(fabsf(gi - g[i + 1]) * inv_sigma), de = ei - e[i + 1];
-            ae += a * de;
-            s += (double)a * de * de;
-        }
-        if (y + 1 < H) {
-            const float a = lambda * expf(fabsf(gi - g[i + W]) * inv_sigma), de = ei - e[i + W];
-            ae += a * de;
-            s += (double)a * de * de;
-        }
-        if (x > 0) ae += lambda * expf(fabsf(g[i - 1] - gi) * inv_sigma) * (ei - e[i - 1]);
-        if (y > 0) ae += lambda * expf(fabsf(g[i - W] - gi) * inv_sigma) * (ei - e[i - W]);
+        const float wi = wimg[i / npx];
+        const float e = out32[i] - tc[i] * (1.0f / WKAPPA);
+        s = (double)wi * (double)e * (double)e;
         uint16_t *o = dout.p + i * dout.cs;
-        o[0] = f2bf(gscale * ae);
+        o[0] = f2bf(gscale * wi * e);
 #pragma unroll
         for (int c = 1; c < 16; c++) o[c] = 0;
     }
@@ -455,7 +553,7 @@ struct LayerSpec {
     int cin, cout;
     bool relu;
 };
-static const LayerSpec SPECS[12] = {{3, 16, true},   {16, 16, true},  {16, 32, true}, {32, 32, true},
+static const LayerSpec SPECS[12] = {{WNFEAT, 16, true},   {16, 16, true},  {16, 32, true}, {32, 32, true},
                                     {32, 64, true},  {64, 64, true},  {64, 64, true}, {64, 64, true},
                                     {128, 64, true}, {96, 32, true},  {48, 16, true}, {16, 1, false}};
 enum { E0A, E0B, E1A, E1B, E2A, E2B, E3A, E3B, D2, D1, D0, OUT, NLAYER };
@@ -482,9 +580,8 @@ struct Acts {
 
 struct lsh_warm {
     std::vector<lsh::Layer> L;
-    int loss_mode = 0;  // 0: mean squared error, 1: the WLS operator's energy norm
-    float loss_lambda = 0.0f, loss_sigma = 1.0f;
-    lsh::DBuf<float> ebuf;
+    float sigma_c = 0.04f;  // the guide scale of the WLS systems served
+    lsh::DBuf<float> cf, pp, wimg;  // block planes, push-pull levels, loss weights
     size_t npad = 0, nnat = 0;
     lsh::DBuf<float> w32, g32, m, v;
     uint64_t step = 0;
@@ -610,13 +707,48 @@ static void ew(const char *name, size_t n, void (*k)(View, View, int), View a, V
     launch(name, k, dim3(cdiv(n, 256)), dim3(256), 0, a, b, nimg);
 }
 
-static void forward(lsh_warm &n, const float *d, const float *conf, const float *g, int nimg, float dscale) {
+// the coarse grid of an h x w WLS grid
+static int cdim(int v) { return (v + WB - 1) / WB; }
+
+// the block planes and the packed input of a batch of nimg h x w systems (the
+// activations sized for the coarse grid first); with a target t also its
+// block means
+static void features(lsh_warm &n, const float *d, const float *conf, const float *g, const float *t, int nimg, int h,
+                     int w, float dscale) {
+    Acts &a = n.A;
+    const int ch = a.H[0], cw = a.W[0];
+    const size_t nc = (size_t)nimg * ch * cw;
+    PushPull pp;
+    size_t np[3], tot = 0;
+    for (int l = 0; l < 3; l++) {
+        pp.kh[l] = (int)cdiv(ch, WPP[l]);
+        pp.kw[l] = (int)cdiv(cw, WPP[l]);
+        np[l] = (size_t)nimg * pp.kh[l] * pp.kw[l];
+        tot += 2 * np[l];
+    }
+    n.cf.ensure(6 * nc);
+    n.pp.ensure(tot);
+    launch_b("warm_feat", (t ? 16.0 : 12.0) * nimg * h * w + 24.0 * nc, k_feat_blocks, dim3(cdiv(nc, 256)), dim3(256), 0,
+             d, conf, g, t, nimg, h, w, ch, cw, 1.0f / dscale, -1.0f / n.sigma_c, n.cf.p);
+    float *q = n.pp.p;
+    for (int l = 0; l < 3; l++) {
+        float *lpk = q, *mk = q + np[l];
+        q += 2 * np[l];
+        launch("warm_feat", k_feat_pp, dim3(cdiv(np[l], 256)), dim3(256), 0, (const float *)n.cf.p, nimg, ch, cw, WPP[l],
+               pp.kh[l], pp.kw[l], lpk, mk);
+        pp.lpk[l] = lpk;
+        pp.mk[l] = mk;
+    }
+    launch("warm_feat", k_feat_pack, dim3(cdiv(nc, 256)), dim3(256), 0, (const float *)n.cf.p, pp, nimg, ch, cw,
+           view(a.X0, ch, cw, 16, 16));
+}
+
+// the network on the packed input X0
+static void forward(lsh_warm &n, int nimg) {
     Acts &a = n.A;
     const int *H = a.H, *W = a.W;
     auto &L = n.L;
-    size_t n0 = (size_t)nimg * H[0] * W[0];
     View X0 = view(a.X0, H[0], W[0], 16, 16);
-    launch("warm_pack", k_pack, dim3(cdiv(n0, 256)), dim3(256), 0, d, conf, g, n0, 1.0f / dscale, X0);
     const float *b32 = n.w32.p;
     auto B = [&](int l) { return b32 + L[l].b_off; };
     View A0a = view(a.A0a, H[0], W[0], 16, 16), S0 = view(a.C0, H[0], W[0], 16, 48, 32);
@@ -669,9 +801,9 @@ static void weight_grad(lsh_warm &n, int l, const View &in, const View &dz, int 
            n.g32.p + ly.b_off);
 }
 
-// loss and gradients of the batch (forward first); returns the mean squared error
-static double backward(lsh_warm &n, const float *d, const float *conf, const float *g, const float *t, int nimg,
-                       float dscale) {
+// loss and gradients of the batch (features with the target and forward
+// first); returns the loss
+static double backward(lsh_warm &n, int nimg) {
     Acts &a = n.A;
     const int *H = a.H, *W = a.W;
     auto &L = n.L;
@@ -679,16 +811,13 @@ static double backward(lsh_warm &n, const float *d, const float *conf, const flo
     auto px = [&](int l) { return (size_t)nimg * H[l] * W[l]; };
     View dOUT = view(a.dOUT, H[0], W[0], 16, 16);
     const int lb = (int)cdiv(n0, 256);
+    const int npx = H[0] * W[0];
+    const float *tc = n.cf.p + 5 * n0;
     n.lpart.ensure(lb);
-    if (n.loss_mode == 0) {
-        launch("warm_loss", k_loss, dim3(lb), dim3(256), 0, (const float *)a.out32.p, d, t, n0, 1.0f / dscale,
-               2.0f / (float)n0, dOUT, n.lpart.p);
-    } else {
-        n.ebuf.ensure(n0);
-        launch("warm_loss", k_err, dim3(lb), dim3(256), 0, (const float *)a.out32.p, d, t, n0, 1.0f / dscale, n.ebuf.p);
-        launch("warm_loss", k_energy, dim3(lb), dim3(256), 0, (const float *)n.ebuf.p, conf, g, H[0], W[0], n0,
-               n.loss_lambda, -1.0f / fmaxf(n.loss_sigma, 1e-6f), 2.0f / (float)n0, dOUT, n.lpart.p);
-    }
+    n.wimg.ensure(nimg);
+    launch("warm_loss", k_loss_weight, dim3(nimg), dim3(256), 0, tc, npx, n.wimg.p);
+    launch("warm_loss", k_loss, dim3(lb), dim3(256), 0, (const float *)a.out32.p, tc, (const float *)n.wimg.p, npx, n0,
+           2.0f / (float)n0, dOUT, n.lpart.p);
     View X0 = view(a.X0, H[0], W[0], 16, 16);
     View A0a = view(a.A0a, H[0], W[0], 16, 16), S0 = view(a.C0, H[0], W[0], 16, 48, 32), C0 = view(a.C0, H[0], W[0], 48, 48);
     View P0 = view(a.P0, H[1], W[1], 16, 16), A1a = view(a.A1a, H[1], W[1], 32, 32);
@@ -802,7 +931,7 @@ char *lsh_warm_new(uint64_t seed, lsh_warm **out) {
         b->zero();
     }
     // He-normal weights (std sqrt(2 / (9 cin))), zero biases; the last layer
-    // starts at zero so the first prediction is the peak depth itself
+    // starts at zero so the first prediction is the blocks' weighted mean depth
     std::vector<float> natw(nat, 0.0f);
     uint64_t s = seed;
     size_t o = 0;
@@ -857,14 +986,11 @@ char *lsh_warm_set(lsh_warm *n, const float *params, uint32_t count) {
     LSH_API_END
 }
 
-char *lsh_warm_set_loss(lsh_warm *n, uint32_t mode, float lambda, float sigma_c) {
+char *lsh_warm_set_sigma(lsh_warm *n, float sigma_c) {
     LSH_API_BEGIN
-    LSH_REQUIRE(n, "lsh_warm_set_loss: null handle");
-    LSH_REQUIRE(mode <= 1, "warm: unknown loss");
-    LSH_REQUIRE(mode == 0 || (lambda >= 0.0f && sigma_c > 0.0f), "warm: the energy loss needs lambda >= 0, sigma_c > 0");
-    n->loss_mode = (int)mode;
-    n->loss_lambda = lambda;
-    n->loss_sigma = sigma_c;
+    LSH_REQUIRE(n, "lsh_warm_set_sigma: null handle");
+    LSH_REQUIRE(sigma_c > 0.0f && std::isfinite(sigma_c), "warm: sigma_c must be positive");
+    n->sigma_c = sigma_c;
     LSH_API_END
 }
 
@@ -876,15 +1002,17 @@ char *lsh_warm_predict(lsh_warm *n, const float *d, const float *conf, const flo
     LSH_REQUIRE(dscale > 0.0f, "warm: dscale must be positive");
     const size_t cnt = (size_t)nimg * w * h;
     Inputs in = stage(*n, d, conf, guide, nullptr, in_mem, cnt);
-    ensure_acts(*n, (int)nimg, (int)h, (int)w, false);
-    forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
+    const int ch = cdim((int)h), cw = cdim((int)w);
+    ensure_acts(*n, (int)nimg, ch, cw, false);
+    features(*n, in.d, in.conf, in.g, nullptr, (int)nimg, (int)h, (int)w, dscale);
+    forward(*n, (int)nimg);
     float *uo = u;
     if (out_mem == LSH_HOST) {
         n->ubuf.ensure(cnt);
         uo = n->ubuf.p;
     }
-    launch("warm_predict", k_predict, dim3(cdiv(cnt, 256)), dim3(256), 0, (const float *)n->A.out32.p, in.d, cnt,
-           1.0f / dscale, dscale, uo);
+    launch_b("warm_prolong", 4.0 * cnt, k_prolong, dim3(cdiv(cnt, 256)), dim3(256), 0, (const float *)n->cf.p,
+             (const float *)n->A.out32.p, (int)nimg, ch, cw, (int)h, (int)w, dscale, uo);
     if (out_mem == LSH_HOST) copy_mem(u, LSH_HOST, uo, LSH_DEVICE, cnt * sizeof(float));
     sync();
     LSH_API_END
@@ -900,9 +1028,10 @@ char *lsh_warm_grads(lsh_warm *n, const float *d, const float *conf, const float
     LSH_REQUIRE(!grads || count == n->nnat, "warm: parameter count mismatch");
     const size_t cnt = (size_t)nimg * w * h;
     Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
-    ensure_acts(*n, (int)nimg, (int)h, (int)w, true);
-    forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
-    double l = backward(*n, in.d, in.conf, in.g, in.t, (int)nimg, dscale);
+    ensure_acts(*n, (int)nimg, cdim((int)h), cdim((int)w), true);
+    features(*n, in.d, in.conf, in.g, in.t, (int)nimg, (int)h, (int)w, dscale);
+    forward(*n, (int)nimg);
+    double l = backward(*n, (int)nimg);
     if (loss) *loss = l;
     if (grads) {
         std::vector<float> pad(n->npad);
@@ -920,9 +1049,12 @@ char *lsh_warm_train_step(lsh_warm *n, const float *d, const float *conf, const 
     LSH_REQUIRE(dscale > 0.0f, "warm: dscale must be positive");
     const size_t cnt = (size_t)nimg * w * h;
     Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
-    ensure_acts(*n, (int)nimg, (int)h, (int)w, true);
-    forward(*n, in.d, in.conf, in.g, (int)nimg, dscale);
-    double l = backward(*n, in.d, in.conf, in.g, in.t, (int)nimg, dscale);
+    ensure_acts(*n, (int)nimg, cdim((int)h), cdim((int)w), true);
+    features(*n, in.d, in.conf, in.g, in.t, (int)nimg, (int)h, (int)w, dscale);
+    forward(*n, (int)nimg);
+    double l = backward(*n, (int)nimg);
+    if (loss) *loss = l;
+    if (!std::isfinite(l)) return nullptr;  // a non-finite loss leaves the weights and the optimiser as they were
     n->step++;
     const float b1 = 0.9f, b2 = 0.999f;
     double c1 = 1.0 - std::pow((double)b1, (double)n->step), c2 = 1.0 - std::pow((double)b2, (double)n->step);
@@ -930,7 +1062,6 @@ char *lsh_warm_train_step(lsh_warm *n, const float *d, const float *conf, const 
     launch("warm_adam", k_adam, dim3(cdiv(n->npad, 256)), dim3(256), 0, n->w32.p, (const float *)n->g32.p, n->m.p,
            n->v.p, n->npad, lr_t, b1, b2, eps_t);
     recast(*n);
-    if (loss) *loss = l;
     LSH_API_END
 }
 

@@ -1,16 +1,19 @@
 """Training data for, training of, and the measured gain of the learned WLS warm start.
 
-A sample is one seeded focus bracket (8 frames of 1024x768 RGB u16 from
-lapstack_amd.synth, no jitter) taken through the device path: the fold and
-focus slices (HipFrames.ingest + Fuser), the depth pass up to the solve
-(depth_grid: peak depth, confidence, guide on the 512x384 working grid), and
-the plain MG-PCG solve to a 1e-6 relative residual (the target).  The network
-(WarmNet) is trained on random crops of the samples by its own device
-kernels; the evaluation solves held-out stacks from the FGS guess and from the
-network's prediction, both to the same tolerance by the same MG-PCG, and
-reports iterations, device milliseconds and the error against an FP64 solve.
+A sample is one seeded 45 MP focus bracket (8256x5504 RGB u16 from
+lapstack_amd.synth, per-frame similarity jitter, 24 to 100 frames) taken
+through the device's stacking path with the benchmark's settings
+(lsh_stack_stream: alignment, 7-level fusion, focus slices) and the depth pass
+up to the solve (depth_grid: guide, peak depth, confidence on the 4128x2752
+grid), with its WLS solution from the FP64 mode to a 1e-11 relative residual
+(the target).  The network (WarmNet) is trained on whole systems by its own
+device kernels; the evaluation solves held-out stacks from the reference's FGS
+guess and from the network's start, both to the same tolerance by the same
+MG-PCG, and reports iterations, device milliseconds (inference included) and
+the error against the FP64 solution -- and, as the ablation, the start of an
+untrained network (the blocks' weighted mean depths, prolongated).
 
-    python -m lapstack_amd.learn --stacks 2000 --steps 4000 --out lapstack_amd/warm_net.f32
+    python -m lapstack_amd.learn --stacks 128 --steps 6000 --lr 3e-4 --out lapstack_amd/warm_net.f32
 """
 from __future__ import annotations
 
@@ -24,6 +27,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "warm_net.f32")
+W45, H45 = 8256, 5504
 
 
 def depth_params():
@@ -34,8 +38,9 @@ def depth_params():
     return la.DepthParams(agg_radius=8, agg_eps=1e-3, tol=1e-6)
 
 
-def stack_planes(frames, dp, fp=None):
-    """(guide, depth, conf) on the working grid of one stack, as device tensors."""
+def stack_planes(frames, dp, fp=None, apm=None):
+    """(guide, depth, conf) on the working grid of one stack, as device
+    tensors, through the stacking path (alignment, fusion, focus slices)."""
     import torch
 
     import lapstack_amd as la
@@ -44,65 +49,71 @@ def stack_planes(frames, dp, fp=None):
     h, w, _ = tuple(frames[0].shape)
     dw, dh = dp.grid(w, h)
     slices = torch.empty((n, dh, dw), dtype=torch.float32, device="cuda")
-    fr = la.HipFrames(w, h, "bilinear")
-    fu = la.Fuser(w, h, fp or la.FuseParams(levels=5))
-    for i, f in enumerate(frames):
-        fr.ingest(f, True, dp, slices[i])
-        fu.push_frames(fr)
     image = torch.empty((3, h, w), dtype=torch.float32, device="cuda")
-    fu.finish(img_out=image)
+    la.stack_stream(frames, fp or la.FuseParams(levels=7), apm or la.AlignParams(coarsen=2, interp="bilinear"), dp,
+                    graph=True, slices_out=slices, img_out=image)
     g, d, c, _ = la.depth_grid(slices, n, image, dp, out_device=True)
-    fr.close()
-    fu.close()
-    return g, d, c
+    return g.clone(), d.clone(), c.clone()
 
 
-def make_sample(seed, dp, w=1024, h=768, n=8):
-    """(guide, depth, conf, target) device planes of one seeded stack."""
+def make_problem(seed, n, dp, w=W45, h=H45):
+    """(guide, depth, conf) device planes of one seeded jittered stack of n frames."""
     import torch
 
-    import lapstack_amd as la
     from lapstack_amd import synth
 
-    br = synth.Bracket(w, h, n, seed=seed, jitter=False, device="cuda")
+    br = synth.Bracket(w, h, n, seed=seed, jitter=True, device="cuda")
     frames = [br.frame(i) for i in range(n)]
     del br
-    g, d, c = stack_planes(frames, dp)
-    u = torch.empty_like(d)
-    la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=dp.tol, max_iters=1000, u_out=u)
-    return g, d, c, u
+    out = stack_planes(frames, dp)
+    del frames
+    torch.cuda.empty_cache()
+    return out
 
 
-def solve_pair(net, g, d, c, dscale, dp, mode="reference"):
-    """The plain solve (FGS guess) and the solve from the network's warm start,
-    to the same tolerance by the same MG-PCG; errors against an FP64 solve to 1e-12."""
+def fp64_solution(g, d, c, dp):
+    """The FP64 solve to a 1e-11 relative residual (float64 device tensor)."""
     import torch
 
     import lapstack_amd as la
 
     u = torch.empty_like(d)
     x = torch.empty(d.shape, dtype=torch.float64, device="cuda")
-    la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "fp64", tol=1e-12, max_iters=2000, u_out=u, u64_out=x)
-    oracle = x.clone()
+    la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "fp64", tol=1e-11, max_iters=2000, u_out=u, u64_out=x)
+    return x
+
+
+def solve_pair(net, g, d, c, dscale, dp, mode="reference", oracle=None, ablation=None):
+    """The plain solve (FGS guess) and the solve from the network's start, to
+    the same tolerance by the same MG-PCG; errors against an FP64 solve.  With
+    ``ablation`` (an untrained network) also the solve from its start."""
+    import torch
+
+    import lapstack_amd as la
+
+    u = torch.empty_like(d)
+    if oracle is None:
+        oracle = fp64_solution(g, d, c, dp)
     res = {}
     _, rep0, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=0, u_out=u)  # the FGS guess
-    fgs_err = float((u.double() - oracle).abs().max())
-    _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=1000, u_out=u)
+    for _ in range(2):  # the second run is timed warm
+        _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=1000, u_out=u)
     res["plain"] = dict(iters=rep["iters"], ms=rep["ms"], true_rel=rep["true_rel"],
-                        err=float((u.double() - oracle).abs().max()), guess_err=fgs_err, guess_rel=rep0["true_rel"])
-    warm = torch.empty_like(d)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    net.predict(d, c, g, dscale, out=warm)
-    e1.record()
-    torch.cuda.synchronize()
-    infer_ms = e0.elapsed_time(e1)
-    _, repw, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=0, warm=warm, u_out=u)
-    _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=1000, warm=warm, u_out=u)
-    res["learned"] = dict(guess_rel=repw["true_rel"], iters=rep["iters"], ms=rep["ms"] + infer_ms, solve_ms=rep["ms"], infer_ms=infer_ms,
-                          true_rel=rep["true_rel"], err=float((u.double() - oracle).abs().max()),
-                          guess_err=float((warm.double() - oracle).abs().max()))
+                        err=float((u.double() - oracle).abs().max()), guess_rel=rep0["true_rel"])
+    for key, m in (("learned", net), ("ablation", ablation)):
+        if m is None:
+            continue
+        warm = torch.empty_like(d)
+        m.predict(d, c, g, dscale, out=warm)  # sizes the activations
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.predict(d, c, g, dscale, out=warm)  # returns once the start is on the device
+        infer_ms = 1000.0 * (time.perf_counter() - t0)
+        _, repw, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=0, warm=warm, u_out=u)
+        for _ in range(2):
+            _, rep, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, mode, tol=dp.tol, max_iters=1000, warm=warm, u_out=u)
+        res[key] = dict(guess_rel=repw["true_rel"], iters=rep["iters"], ms=rep["ms"] + infer_ms, solve_ms=rep["ms"],
+                        infer_ms=infer_ms, true_rel=rep["true_rel"], err=float((u.double() - oracle).abs().max()))
     return res
 
 
@@ -135,59 +146,63 @@ def train(args):
 
     dp = depth_params()
     t0 = time.perf_counter()
-    G, D, Cf, U = [], [], [], []
+    rng = np.random.default_rng(args.seed)
+    G = D = Cf = U = None
     for s in range(args.stacks):
-        g, d, c, u = make_sample(args.seed + s, dp)
-        G.append(g)
-        D.append(d)
-        Cf.append(c)
-        U.append(u)
-        if (s + 1) % 50 == 0 or s + 1 == args.stacks:
+        n = int(rng.choice([24, 40, 64, 100]))
+        g, d, c = make_problem(args.seed + s, n, dp)
+        x = fp64_solution(g, d, c, dp)
+        if G is None:
+            G, D, Cf, U = (torch.empty((args.stacks,) + tuple(d.shape), dtype=torch.float32, device="cuda")
+                           for _ in range(4))
+        sc = float(n - 1)  # depth / dscale: every system in frame-range units
+        G[s], D[s], Cf[s], U[s] = g, d / sc, c, (x / sc).float()
+        del g, d, c, x
+        if (s + 1) % 16 == 0 or s + 1 == args.stacks:
             print(f"samples {s + 1}/{args.stacks} ({time.perf_counter() - t0:.1f}s)", flush=True)
-    G, D, Cf, U = (torch.stack(a) for a in (G, D, Cf, U))
     N, H, W = D.shape
-    dscale = 7.0  # 8 frames: frame index range n - 1
-    net = la.WarmNet(seed=args.seed)
-    if args.loss == "energy":
-        net.set_loss("energy", dp.lam, dp.sigma_c)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(args.seed)
-    cs, B = args.crop, args.batch
-    bufs = [torch.empty((B, cs, cs), dtype=torch.float32, device="cuda") for _ in range(4)]
+    net = la.WarmNet(seed=1)
+    net.set_sigma(dp.sigma_c)
+    gen = torch.Generator()
+    gen.manual_seed(1)
+    B = args.batch
+    bufs = [torch.empty((B, H, W), dtype=torch.float32, device="cuda") for _ in range(4)]
     t1 = time.perf_counter()
     losses = []
     for step in range(args.steps):
-        idx = torch.randint(0, N, (B,), generator=gen, device="cuda")
-        oy = torch.randint(0, H - cs + 1, (B,), generator=gen, device="cuda")
-        ox = torch.randint(0, W - cs + 1, (B,), generator=gen, device="cuda")
-        ar = torch.arange(cs, device="cuda")
-        yy = (oy[:, None] + ar[None, :])[:, :, None]
-        xx = (ox[:, None] + ar[None, :])[:, None, :]
-        ii = idx[:, None, None]
+        idx = torch.randint(0, N, (B,), generator=gen).cuda()
+        flip = [d for d, on in ((-1, step % 2), (-2, (step // 2) % 2)) if on]
         for buf, src in zip(bufs, (D, Cf, G, U)):
-            buf.copy_(src[ii, yy, xx])
+            v = src.index_select(0, idx)
+            buf.copy_(v.flip(flip) if flip else v)
         lr = args.lr * (0.5 * (1.0 + np.cos(np.pi * step / args.steps)))  # cosine decay
-        losses.append(net.train_step(bufs[0], bufs[1], bufs[2], bufs[3], dscale, lr))
+        losses.append(net.train_step(bufs[0], bufs[1], bufs[2], bufs[3], 1.0, lr))
         if (step + 1) % 250 == 0 or step + 1 == args.steps:
-            print(f"step {step + 1}/{args.steps}: loss {np.mean(losses[-250:]):.3e} "
+            print(f"step {step + 1}/{args.steps}: loss {np.mean(losses[-250:]):.4f} "
                   f"({time.perf_counter() - t1:.1f}s)", flush=True)
     net.save(args.out)
     print(f"saved {args.out}", flush=True)
-    del G, D, Cf, U
+    del G, D, Cf, U, bufs
     torch.cuda.empty_cache()
-    # held-out stacks: seeds after the training range
+    base = la.WarmNet(seed=1)  # untrained: the weighted block means
+    base.set_sigma(dp.sigma_c)
     rows = []
-    for s in range(args.eval):
-        g, d, c, _ = make_sample(args.seed + args.stacks + 10_000 + s, dp)
-        rows.append(solve_pair(net, g, d, c, dscale, dp))
+    for s in range(args.eval):  # held-out stacks: seeds after the training range
+        n = (100, 40, 64, 24)[s % 4]
+        g, d, c = make_problem(args.seed + args.stacks + 10_000 + s, n, dp)
+        r = solve_pair(net, g, d, c, float(n - 1), dp, ablation=base)
+        r["frames"] = n
+        rows.append(r)
+        print(json.dumps(r), flush=True)
     infer = infer_timing(net, args.infer_grid) if args.infer_grid else None
+    keys = ("plain", "learned", "ablation")
     summary = {
         "infer": infer,
-        "train": {"loss": args.loss, "stacks": args.stacks, "grid": [W, H], "steps": args.steps, "batch": B, "crop": cs,
+        "train": {"stacks": args.stacks, "grid": [W, H], "steps": args.steps, "batch": B,
                   "final_loss": float(np.mean(losses[-250:])), "train_s": round(time.perf_counter() - t1, 1)},
         "eval_stacks": args.eval,
-        "plain": {k: float(np.mean([r["plain"][k] for r in rows])) for k in rows[0]["plain"]},
-        "learned": {k: float(np.mean([r["learned"][k] for r in rows])) for k in rows[0]["learned"]},
+        **{k: {f: float(np.mean([r[k][f] for r in rows])) for f in rows[0][k]} for k in keys},
+        "wins": int(sum(r["learned"]["ms"] < r["plain"]["ms"] for r in rows)),
     }
     print(json.dumps(summary), flush=True)
     return summary
@@ -195,16 +210,13 @@ def train(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--stacks", type=int, default=2000)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--crop", type=int, default=128)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--stacks", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--seed", type=int, default=20_000)
     ap.add_argument("--eval", type=int, default=8)
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--loss", choices=["mse", "energy"], default="energy",
-                    help="energy: the error's energy norm under the WLS operator (what MG-PCG reduces)")
     ap.add_argument("--infer-grid", default="4128x2752", help="time one prediction on this grid ('' to skip)")
     args = ap.parse_args()
     sys.exit(0 if train(args) else 1)

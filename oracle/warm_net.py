@@ -5,25 +5,38 @@ forward and backward passes; nothing in the product imports it.
 
 The network has no counterpart in the reference (whose PCG starts from the FGS
 sweeps, crates/lapstack-core/src/depth.rs:1019-1034); this file states it in
-numpy f64: zero-padded 3x3 convolutions, ReLU, 2x2 average pooling over the
-cells inside the finer grid, nearest upsampling, channel concatenation, the
-loss of the normalised prediction (mean squared error, or the error's energy
-norm under the WLS operator of depth.rs:653-672/994), and the backward pass by
-the chain rule.  With ``emulate=True`` it rounds to bf16 exactly where the device stores
-bf16 (weights, the packed input, every activation and every activation
-gradient), so the two differ only by the order of f32 accumulation.
+numpy f64.  Its input: the WLS system's planes averaged over 8 x 8 blocks of
+the grid (partial blocks at the right and bottom edges average the cells that
+exist).  With dm the block's mean depth weighted by the data weight
+max(conf, 0) + 1e-4 (depth.rs:994), depth enters as differences from dm in
+units of 1/32: the plain mean depth, and per push-pull level (4 x 4, 16 x 16
+and 64 x 64 blocks of blocks) the weighted mean resampled bilinearly to the
+block grid; then the mean data weight and its log, the mean edge conductances
+exp(-|dg| / sigma_c) (depth.rs:653-672; 0 past the last column or row), and
+the first two levels' log mean weight, resampled.
+The network: zero-padded 3x3 convolutions, ReLU, 2x2 average pooling over the
+cells inside the finer grid, nearest upsampling, channel concatenation; its
+prediction of the solution's block means is the weighted mean plus 1/32 of
+the output, and the start is that prediction resampled bilinearly to the WLS
+grid.  The loss: per system the mean squared error against the target's block
+means, relative to the mean square of the target's own deviation from the
+weighted mean (at least 1e-4, in units of 1/32), averaged over the systems;
+the backward pass by the chain rule.  With ``emulate=True`` it rounds to bf16
+exactly where the device stores bf16 (weights, the packed input, every
+activation and every activation gradient), so the two differ only by the
+order of f32 accumulation and the f32 arithmetic of the input planes.
 
-Pinned by: central finite differences of the f64 loss, both losses, and the
-energy norm against e^T A e with A assembled as a scipy sparse matrix
-(tests/test_oracle_warm.py),
-the convolution against a direct 7-loop sum, and pooling/upsampling as
-adjoints (<P x, y> = <x, P* y>).
+Pinned by (tests/test_oracle_warm.py): central finite differences of the f64
+loss, the convolution against a direct 7-loop sum, pooling/upsampling as
+adjoints (<P x, y> = <x, P* y>), the block planes against a loop over the
+cells of each block, and the bilinear resampling against
+torch.nn.functional.interpolate (bilinear, align_corners=False).
 """
 from __future__ import annotations
 
 import numpy as np
 
-SPECS = [(3, 16, True), (16, 16, True), (16, 32, True), (32, 32, True), (32, 64, True), (64, 64, True),
+SPECS = [(10, 16, True), (16, 16, True), (16, 32, True), (32, 32, True), (32, 64, True), (64, 64, True),
          (64, 64, True), (64, 64, True), (128, 64, True), (96, 32, True), (48, 16, True), (16, 1, False)]
 NAMES = ["e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "e3a", "e3b", "d2", "d1", "d0", "out"]
 
@@ -141,21 +154,83 @@ def up_bwd(du, shape):
     return s
 
 
-def _inputs(d, conf, g, dscale, emulate):
-    d, conf, g = (np.asarray(a, np.float32) for a in (d, conf, g))
-    if d.ndim == 2:
-        d, conf, g = d[None], conf[None], g[None]
-    dn = (d * np.float32(1.0 / dscale)).astype(np.float32)
-    x = np.stack([dn, conf, g], axis=-1).astype(np.float64)
-    return (bf16(x) if emulate else x), dn.astype(np.float64)
+WB = 8                # block edge, in WLS cells
+KAPPA = 1.0 / 32.0    # the output's scale, in depth / dscale
+PP = (4, 16, 64)      # push-pull levels, in blocks
 
 
-def forward(params, d, conf, g, dscale, emulate=False):
-    """The normalised prediction d / dscale + out (B, H, W) and the activations."""
+def block_means(a, k):
+    """Means of a (B, H, W) over k x k blocks; partial blocks at the right and
+    bottom edges average the cells that exist."""
+    a = np.asarray(a, np.float64)
+    B, H, W = a.shape
+    h, w = -(-H // k), -(-W // k)
+    s = np.zeros((B, h, w))
+    cnt = np.zeros((h, w))
+    for dy in range(k):
+        for dx in range(k):
+            part = a[:, dy::k, dx::k]
+            s[:, :part.shape[1], :part.shape[2]] += part
+            cnt[:part.shape[1], :part.shape[2]] += 1
+    return s / cnt
+
+
+def resample(p, oh, ow):
+    """Bilinear resampling of p (B, ih, iw) onto an (oh, ow) grid over the same
+    area: source position (o + 1/2) in/out - 1/2, clamped at 0, the
+    right/lower neighbour clamped to the last cell."""
+    p = np.asarray(p, np.float64)
+    _, ih, iw = p.shape
+    sy = np.maximum((np.arange(oh) + 0.5) * (ih / oh) - 0.5, 0.0)
+    sx = np.maximum((np.arange(ow) + 0.5) * (iw / ow) - 0.5, 0.0)
+    y0, x0 = sy.astype(np.int64), sx.astype(np.int64)
+    y1, x1 = np.minimum(y0 + 1, ih - 1), np.minimum(x0 + 1, iw - 1)
+    ly, lx = (sy - y0)[None, :, None], (sx - x0)[None, None, :]
+    top = (1 - lx) * p[:, y0][:, :, x0] + lx * p[:, y0][:, :, x1]
+    bot = (1 - lx) * p[:, y1][:, :, x0] + lx * p[:, y1][:, :, x1]
+    return (1 - ly) * top + ly * bot
+
+
+def _planes(*a):
+    a = [np.asarray(x, np.float32) for x in a]
+    return [x[None] if x.ndim == 2 else x for x in a]
+
+
+def features(d, conf, g, dscale, sigma=0.04, t=None):
+    """The input planes X (B, ch, cw, 10), the weighted mean dm (B, ch, cw), and
+    with a target t its block means' deviation tc = mean(t) / dscale - dm."""
+    d, conf, g = (x.astype(np.float64) for x in _planes(d, conf, g))
+    wd = np.maximum(conf, 0.0) + 1e-4
+    pw = block_means(wd, WB)
+    dm = block_means(wd * d, WB) / pw / dscale
+    ax = np.zeros_like(g)
+    ax[:, :, :-1] = np.exp(-np.abs(g[:, :, :-1] - g[:, :, 1:]) / sigma)
+    ay = np.zeros_like(g)
+    ay[:, :-1] = np.exp(-np.abs(g[:, :-1] - g[:, 1:]) / sigma)
+    ch, cw = pw.shape[1:]
+    feats = [(block_means(d, WB) / dscale - dm) / KAPPA]
+    lpk = []
+    for k in PP:
+        pk = block_means(pw, k)
+        mk = block_means(pw * dm, k) / pk
+        feats.append((resample(mk, ch, cw) - dm) / KAPPA)
+        lpk.append(resample(np.log10(pk), ch, cw) / 4 + 1)
+    feats += [pw, np.log10(pw) / 4 + 1, block_means(ax, WB), block_means(ay, WB)] + lpk[:2]
+    tc = None
+    if t is not None:
+        (t,) = _planes(t)
+        tc = block_means(t, WB) / dscale - dm
+    return np.stack(feats, axis=-1), dm, tc
+
+
+def forward(params, d, conf, g, dscale, sigma=0.04, emulate=False):
+    """The predicted block means dm + out / 32 (B, ch, cw), in depth / dscale,
+    and the activations."""
     R = bf16 if emulate else (lambda a: a)
     L = [(R(W) if emulate else W, b) for W, b in unflatten(params)]
     relu = lambda a: np.maximum(a, 0.0)
-    X0, dn = _inputs(d, conf, g, dscale, emulate)
+    X, dm, _ = features(d, conf, g, dscale, sigma)
+    X0 = bf16(X) if emulate else X
     c = {"X0": X0}
     c["A0a"] = R(relu(conv(X0, *L[0])))
     c["S0"] = R(relu(conv(c["A0a"], *L[1])))
@@ -178,54 +253,35 @@ def forward(params, d, conf, g, dscale, emulate=False):
     if emulate:
         out = out.astype(np.float32).astype(np.float64)
     c["L"] = L
-    c["dn"] = dn
-    return dn + out, c
+    c["dm"] = dm
+    c["out"] = out
+    return dm + KAPPA * out, c
 
 
-def energy(e, conf, g, lam, sigma):
-    """The energy norm per cell of the error planes e (B, H, W) under each
-    image's WLS operator (data weights max(conf, 0) + 1e-4, edges
-    lam * exp(-|dg| / sigma)), and its gradient 2 (A e) / N."""
-    e = np.asarray(e, np.float64)
-    conf = np.asarray(conf, np.float64).reshape(e.shape)
-    g = np.asarray(g, np.float64).reshape(e.shape)
-    w = np.maximum(conf, 0.0) + 1e-4
-    ax = lam * np.exp(-np.abs(g[:, :, 1:] - g[:, :, :-1]) / sigma)
-    ay = lam * np.exp(-np.abs(g[:, 1:] - g[:, :-1]) / sigma)
-    dx, dy = e[:, :, :-1] - e[:, :, 1:], e[:, :-1] - e[:, 1:]
-    N = e.size
-    loss = (np.sum(w * e * e) + np.sum(ax * dx * dx) + np.sum(ay * dy * dy)) / N
-    Ae = w * e
-    Ae[:, :, :-1] += ax * dx
-    Ae[:, :, 1:] -= ax * dx
-    Ae[:, :-1] += ay * dy
-    Ae[:, 1:] -= ay * dy
-    return loss, 2.0 * Ae / N
+def predict(params, d, conf, g, dscale, sigma=0.04, emulate=False):
+    """The start on the WLS grid (B, H, W): dscale times the predicted block
+    means resampled bilinearly."""
+    (d,) = _planes(d)
+    pred, _ = forward(params, d, conf, g, dscale, sigma, emulate)
+    return dscale * resample(pred, d.shape[1], d.shape[2])
 
 
-def loss_and_grads(params, d, conf, g, target, dscale, emulate=False, loss="mse", lam=0.0, sigma=1.0):
-    """The loss of the normalised prediction against target / dscale (mean
-    squared error, or the energy norm under the WLS operator) and its gradient
-    with respect to the flat parameters."""
+def loss_and_grads(params, d, conf, g, target, dscale, sigma=0.04, emulate=False):
+    """The loss (per system the mean squared error of out against tc * 32,
+    divided by max(mean((tc * 32)^2), 1e-4), averaged over the systems) and its
+    gradient with respect to the flat parameters."""
     R = bf16 if emulate else (lambda a: a)
-    pred, c = forward(params, d, conf, g, dscale, emulate)
+    _, c = forward(params, d, conf, g, dscale, sigma, emulate)
     L = c["L"]
-    t = np.asarray(target, np.float32)
-    if t.ndim == 2:
-        t = t[None]
-    tn = (t * np.float32(1.0 / dscale)).astype(np.float64)
-    if emulate:
-        pred = pred.astype(np.float32).astype(np.float64)
-    e = pred - tn
-    N = e.size
+    _, _, tc = features(d, conf, g, dscale, sigma, t=target)
+    that = tc / KAPPA
+    e = c["out"] - that
     if emulate:
         e = e.astype(np.float32).astype(np.float64)
-    if loss == "energy":
-        cf = np.asarray(conf, np.float32)
-        gg = np.asarray(g, np.float32)
-        loss_v, de = energy(e, cf.reshape(e.shape), gg.reshape(e.shape), lam, sigma)
-    else:
-        loss_v, de = float(np.mean(e * e)), (2.0 / N) * e
+    B, ch, cw = e.shape
+    wimg = 1.0 / np.maximum(np.mean(that * that, axis=(1, 2)), 1e-4)
+    loss_v = float(np.mean(wimg * np.mean(e * e, axis=(1, 2))))
+    de = 2.0 * wimg[:, None, None] * e / (B * ch * cw)
     grads = [None] * len(L)
     pos = lambda a: (a > 0).astype(np.float64)
 

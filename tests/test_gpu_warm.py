@@ -1,11 +1,11 @@
 """The learned warm start's device kernels (lsh_warm_*) against the CPU oracle
 (oracle/warm_net.py), which rounds to bf16 where the device stores bf16.
 
-Bars: the prediction within 1e-2 of the normalised range at every pixel (and
-1e-3 in the mean: the device and the oracle differ only where f32 sums in
-another order round an activation to the neighbouring bf16 value); the loss
-within 1e-3 relative; every layer's weight gradient within 3e-2 relative
-(L2) and the whole gradient at cosine > 0.999; training lowers the loss."""
+Bars: the start within 1e-2 of the depth range at every cell (and 1e-3 in the
+mean: the device and the oracle differ only where f32 sums in another order
+round an input or an activation to the neighbouring bf16 value); the loss
+within 1e-3 relative; every layer's weight gradient within 3e-2 relative (L2)
+and the whole gradient at cosine > 0.999; training lowers the loss."""
 import numpy as np
 import pytest
 
@@ -39,7 +39,7 @@ def _batch(B, H, W, seed, dscale=7.0):
     return [np.stack([o[i] for o in out]).astype(np.float32) for i in range(4)]
 
 
-@pytest.mark.parametrize("B,H,W", [(2, 37, 70), (1, 64, 64), (3, 9, 130)])
+@pytest.mark.parametrize("B,H,W", [(2, 150, 260), (1, 64, 64), (3, 41, 300), (1, 7, 5)])
 def test_forward_matches_oracle(B, H, W):
     print(f"warm forward {B}x{H}x{W}", flush=True)
     net = la.WarmNet()
@@ -47,24 +47,23 @@ def test_forward_matches_oracle(B, H, W):
     net.set(p)
     d, conf, g, _ = _batch(B, H, W, 1)
     u = net.predict(d, conf, g, 7.0)
-    want, _ = wn.forward(p, d, conf, g, 7.0, emulate=True)
-    err = np.abs(u / 7.0 - want)
-    rng = float(np.abs(want).max())
-    assert err.max() <= 1e-2 * rng, err.max()
-    assert err.mean() <= 1e-3 * rng, err.mean()
+    want = wn.predict(p, d, conf, g, 7.0, emulate=True)
+    err = np.abs(u - want)
+    assert err.max() <= 1e-2 * 7.0, err.max()
+    assert err.mean() <= 1e-3 * 7.0, err.mean()
     net.close()
 
 
-@pytest.mark.parametrize("B,H,W,kind", [(2, 37, 70, "mse"), (1, 40, 33, "mse"), (2, 37, 70, "energy")])
-def test_gradients_match_oracle(B, H, W, kind):
-    print(f"warm gradients {B}x{H}x{W} {kind}", flush=True)
+@pytest.mark.parametrize("B,H,W", [(2, 150, 260), (1, 120, 97), (2, 600, 1100), (1, 1400, 2100)])
+def test_gradients_match_oracle(B, H, W):
+    print(f"warm gradients {B}x{H}x{W}", flush=True)
     net = la.WarmNet()
     p = wn.init(11)
     net.set(p)
-    net.set_loss(kind, 3.0, 0.2)
+    net.set_sigma(0.2)
     d, conf, g, t = _batch(B, H, W, 2)
     loss, grad = net.grads(d, conf, g, t, 7.0)
-    wloss, wgrad = wn.loss_and_grads(p, d, conf, g, t, 7.0, emulate=True, loss=kind, lam=3.0, sigma=0.2)
+    wloss, wgrad = wn.loss_and_grads(p, d, conf, g, t, 7.0, sigma=0.2, emulate=True)
     assert loss == pytest.approx(wloss, rel=1e-3)
     offs = np.cumsum([0] + [ci * co * 9 + co for ci, co, _ in wn.SPECS])
     for li, name in enumerate(wn.NAMES):
@@ -79,7 +78,7 @@ def test_gradients_match_oracle(B, H, W, kind):
 def test_training_lowers_the_loss():
     print("warm training", flush=True)
     net = la.WarmNet(seed=3)
-    d, conf, g, t = _batch(4, 64, 96, 3)
+    d, conf, g, t = _batch(4, 160, 240, 3)
     first = net.train_step(d, conf, g, t, 7.0, lr=2e-3)
     for _ in range(150):
         last = net.train_step(d, conf, g, t, 7.0, lr=2e-3)
@@ -100,10 +99,10 @@ def test_device_planes_and_errors():
 
     net = la.WarmNet()
     net.set(wn.init(5))
-    d, conf, g, _ = _batch(1, 33, 47, 4)
+    d, conf, g, _ = _batch(1, 83, 147, 4)
     host = net.predict(d[0], conf[0], g[0], 7.0)
     dev = [torch.from_numpy(a[0]).cuda() for a in (d, conf, g)]
-    out = torch.zeros((33, 47), dtype=torch.float32, device="cuda")
+    out = torch.zeros((83, 147), dtype=torch.float32, device="cuda")
     net.predict(*dev, 7.0, out=out)
     assert np.array_equal(out.cpu().numpy(), host)
     with pytest.raises(la.HipError):
@@ -112,33 +111,49 @@ def test_device_planes_and_errors():
         net.set(np.zeros(10, np.float32))
     with pytest.raises(la.HipError):
         net.predict(d, conf, g, 0.0)
+    with pytest.raises(la.HipError):
+        net.set_sigma(0.0)
     net.close()
 
 
 def test_trained_warm_start_on_fresh_stacks():
-    """What the shipped network (trained on 2000 seeded 1024x768 stacks, energy
-    loss) does on two stacks it never saw: MG-PCG from its prediction reaches
-    the same tolerance as from the reference's FGS guess, with the same error
-    against an FP64 solve; the iteration counts and the guesses' residuals are
-    printed (measured: the FGS guess is the better start on these stacks)."""
+    """What the shipped network (trained by these kernels on 128 seeded 45 MP
+    jittered stacks of 24 to 100 frames) gains on 45 MP stacks it never saw:
+    MG-PCG from its start reaches the same tolerance as from the reference's
+    FGS guess, at an error against the FP64 solution no larger, and in less
+    time summed over the stacks, the inference included.  The untrained
+    network (the blocks' weighted mean depths, prolongated) is the ablation:
+    the trained one needs fewer iterations than it summed over the stacks."""
     import os
 
     import torch
 
     from lapstack_amd import learn
 
-    if not os.path.exists(learn.WEIGHTS):
-        pytest.skip("no trained weights shipped")
+    assert os.path.exists(learn.WEIGHTS), "the trained weights ship with the package"
     dp = learn.depth_params()
     net = la.WarmNet()
     net.load(learn.WEIGHTS)
-    for seed in (90001, 90002):
-        print(f"trained warm start, stack seed {seed}", flush=True)
-        g, d, c, _ = learn.make_sample(seed, dp)
-        r = learn.solve_pair(net, g, d, c, 7.0, dp)
+    base = la.WarmNet(seed=1)
+    plain_ms = learned_ms = 0.0
+    learned_it = base_it = 0
+    for seed, n in ((90001, 40), (90002, 64)):
+        print(f"trained warm start, 45 MP stack seed {seed}, {n} frames", flush=True)
+        g, d, c = learn.make_problem(seed, n, dp)
+        r = learn.solve_pair(net, g, d, c, float(n - 1), dp, ablation=base)
         print(r, flush=True)
-        assert r["plain"]["true_rel"] < 2e-6 and r["learned"]["true_rel"] < 2e-6
-        assert r["learned"]["err"] <= 2.0 * r["plain"]["err"] + 1e-4
-        assert r["learned"]["guess_rel"] < 0.1  # far better than a zero start (relative residual 1)
+        for k in ("plain", "learned", "ablation"):
+            assert r[k]["true_rel"] < 2e-6, (k, r[k])
+        assert r["learned"]["err"] <= 1.5 * r["plain"]["err"] + 1e-4
+        plain_ms += r["plain"]["ms"]
+        learned_ms += r["learned"]["ms"]
+        learned_it += r["learned"]["iters"]
+        base_it += r["ablation"]["iters"]
+        del g, d, c
         torch.cuda.synchronize()
+    print(f"plain {plain_ms:.2f} ms, learned {learned_ms:.2f} ms; iterations learned {learned_it}, "
+          f"untrained {base_it}", flush=True)
+    assert learned_ms < plain_ms
+    assert learned_it < base_it
     net.close()
+    base.close()

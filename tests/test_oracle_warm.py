@@ -1,8 +1,9 @@
 """Pins the CPU oracle of the learned warm-start network (oracle/warm_net.py):
 the convolution against a direct loop sum, pooling, upsampling and the
-convolution's input gradient as adjoints of the forward maps, the whole
-backward pass against central finite differences of the loss, and the bf16
-rounding against its definition."""
+convolution's input gradient as adjoints of the forward maps, the block planes
+against a loop over each block's cells, the bilinear resampling against
+torch's, the whole backward pass against central finite differences of the
+loss, and the bf16 rounding against its definition."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ from oracle import warm_net as wn
 
 
 def test_parameter_count():
-    assert wn.nparams() == 254481
+    assert wn.nparams() == 255489
     p = wn.init(1)
     assert p.size == wn.nparams()
     assert np.array_equal(wn.flatten(wn.unflatten(p)), p)
@@ -52,18 +53,16 @@ def test_adjoints(H, W):
     assert np.allclose(wn.pool(np.ones((1, H, W, 2))), 1.0)
 
 
-@pytest.mark.parametrize("loss", ["mse", "energy"])
-def test_backward_matches_finite_differences(loss):
+def test_backward_matches_finite_differences():
     rng = np.random.default_rng(5)
-    B, H, W = 1, 10, 12
+    B, H, W = 2, 41, 75  # block grid 6 x 10, partial blocks on both edges
     d = rng.uniform(0, 7, (B, H, W)).astype(np.float32)
-    conf = rng.uniform(0, 1, (B, H, W)).astype(np.float32)
+    conf = rng.uniform(-0.1, 1, (B, H, W)).astype(np.float32)
     g = rng.uniform(0, 1, (B, H, W)).astype(np.float32)
     t = (d + rng.normal(0, 0.5, (B, H, W))).astype(np.float32)
     p = wn.init(3)
-    kw = dict(loss=loss, lam=3.0, sigma=0.2)
-    value, grad = wn.loss_and_grads(p, d, conf, g, t, 7.0, **kw)
-    f = lambda q: wn.loss_and_grads(q, d, conf, g, t, 7.0, **kw)[0]
+    value, grad = wn.loss_and_grads(p, d, conf, g, t, 7.0, sigma=0.2)
+    f = lambda q: wn.loss_and_grads(q, d, conf, g, t, 7.0, sigma=0.2)[0]
     offs = np.cumsum([0] + [ci * co * 9 + co for ci, co, _ in wn.SPECS])
     idx = []
     for li in range(len(wn.SPECS)):  # two weights and a bias of every layer
@@ -79,24 +78,85 @@ def test_backward_matches_finite_differences(loss):
     assert value > 0
 
 
-def test_energy_is_the_quadratic_form_of_the_wls_matrix():
-    from tests.fixtures import wls_matrix
+def test_loss_is_relative_per_system():
+    """Scaling one system's target deviation and the prediction's error with it
+    leaves the loss unchanged; a zero network predicts the weighted means, so
+    its loss is 1 for every system above the floor."""
+    rng = np.random.default_rng(6)
+    d = rng.uniform(0, 7, (2, 24, 32)).astype(np.float32)
+    conf = rng.uniform(0, 1, (2, 24, 32)).astype(np.float32)
+    g = rng.uniform(0, 1, (2, 24, 32)).astype(np.float32)
+    t = (d + rng.normal(0, 1.0, d.shape)).astype(np.float32)
+    p = np.zeros(wn.nparams())
+    loss, _ = wn.loss_and_grads(p, d, conf, g, t, 7.0)
+    assert loss == pytest.approx(1.0, rel=1e-12)
 
-    rng = np.random.default_rng(8)
-    B, H, W = 2, 7, 9
-    e = rng.standard_normal((B, H, W))
-    conf = rng.uniform(-0.2, 1.0, (B, H, W)).astype(np.float32)
-    g = rng.uniform(0, 1, (B, H, W)).astype(np.float32)
-    loss, grad = wn.energy(e, conf, g, 3.0, 0.2)
-    want, wgrad = 0.0, []
-    for b in range(B):
-        A = wls_matrix(g[b], conf[b], 3.0, 0.2)
-        v = e[b].ravel()
-        want += v @ (A @ v)
-        wgrad.append((2.0 * (A @ v) / e.size).reshape(H, W))
-    assert loss == pytest.approx(want / e.size, rel=1e-6)
-    # the fixture builds the edge weights in f32 (as the solver does), the oracle in f64
-    assert np.allclose(grad, np.stack(wgrad), rtol=1e-6, atol=1e-7)
+
+def test_block_planes_are_the_block_loops():
+    rng = np.random.default_rng(9)
+    H, W, sig, ds = 19, 26, 0.2, 5.0  # blocks of 8 x 8, 8 x 2 and 3 x 8 cells at the edges
+    d = rng.uniform(0, 5, (H, W)).astype(np.float32)
+    conf = rng.uniform(-0.3, 1, (H, W)).astype(np.float32)
+    g = rng.uniform(0, 1, (H, W)).astype(np.float32)
+    t = rng.uniform(0, 5, (H, W)).astype(np.float32)
+    X, dm, tc = wn.features(d, conf, g, ds, sig, t=t)
+    assert X.shape == (1, 3, 4, 10)
+    for by in range(3):
+        for bx in range(4):
+            sw = swd = sd = sax = say = st = 0.0
+            n = 0
+            for y in range(8 * by, min(8 * by + 8, H)):
+                for x in range(8 * bx, min(8 * bx + 8, W)):
+                    w = max(float(conf[y, x]), 0.0) + 1e-4
+                    sw += w
+                    swd += w * float(d[y, x])
+                    sd += float(d[y, x])
+                    st += float(t[y, x])
+                    if x + 1 < W:
+                        sax += np.exp(-abs(float(g[y, x]) - float(g[y, x + 1])) / sig)
+                    if y + 1 < H:
+                        say += np.exp(-abs(float(g[y, x]) - float(g[y + 1, x])) / sig)
+                    n += 1
+            want = [(sd / n / ds - swd / sw / ds) * 32, sw / n, np.log10(sw / n) / 4 + 1, sax / n, say / n]
+            assert np.allclose(X[0, by, bx, [0, 4, 5, 6, 7]], want, rtol=1e-9, atol=1e-12)
+            assert dm[0, by, bx] == pytest.approx(swd / sw / ds, rel=1e-12)
+            assert tc[0, by, bx] == pytest.approx(st / n / ds - swd / sw / ds, rel=1e-9, abs=1e-12)
+    # the push-pull levels over the 3 x 4 block grid: one cell of each holds
+    # every block, so each level's mean is the constant weighted mean m
+    pw = X[0, :, :, 4]
+    m = np.sum(pw * dm[0]) / np.sum(pw)
+    for c in (1, 2, 3):
+        assert np.allclose(X[0, :, :, c], (m - dm[0]) * 32, rtol=1e-9, atol=1e-12)
+    for c in (8, 9):
+        assert np.allclose(X[0, :, :, c], np.log10(pw.mean()) / 4 + 1, rtol=1e-12)
+    # depth enters only as differences: a constant added to it changes no input
+    X2, dm2, tc2 = wn.features(d + 3.0, conf, g, ds, sig, t=t + 3.0)
+    assert np.allclose(X2, X, rtol=1e-6, atol=1e-6) and np.allclose(tc2, tc, atol=1e-9)
+
+
+@pytest.mark.parametrize("ih,iw,oh,ow", [(3, 4, 19, 26), (5, 9, 40, 72), (1, 1, 3, 5), (7, 3, 7, 3), (6, 10, 3, 4)])
+def test_resample_is_torch_bilinear(ih, iw, oh, ow):
+    import torch
+    import torch.nn.functional as F
+
+    rng = np.random.default_rng(ih * 100 + iw)
+    p = rng.standard_normal((2, ih, iw))
+    want = F.interpolate(torch.from_numpy(p)[:, None], size=(oh, ow), mode="bilinear", align_corners=False)
+    assert np.allclose(wn.resample(p, oh, ow), want[:, 0].numpy(), rtol=1e-12, atol=1e-12)
+
+
+def test_prediction_of_a_zero_network_is_the_prolonged_weighted_mean():
+    rng = np.random.default_rng(4)
+    d = rng.uniform(0, 7, (1, 30, 45)).astype(np.float32)
+    conf = np.ones_like(d)
+    g = rng.uniform(0, 1, d.shape).astype(np.float32)
+    u = wn.predict(np.zeros(wn.nparams()), d, conf, g, 7.0)
+    assert u.shape == d.shape
+    # equal weights: the block means of d, resampled
+    assert np.allclose(u, wn.resample(wn.block_means(d, 8), 30, 45), rtol=1e-12)
+    # a constant field stays constant
+    u = wn.predict(np.zeros(wn.nparams()), np.full_like(d, 3.0), conf, g, 7.0)
+    assert np.allclose(u, 3.0, rtol=1e-12)
 
 
 def test_bf16_rounding():
