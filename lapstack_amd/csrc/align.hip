@@ -1851,7 +1851,7 @@ static void search_launch(lsh_frames *f, bool plain = false, hipEvent_t started 
     unsigned *ctr = f->nm_flags.p, *err = f->nm_flags.p + NM_FL_ERR;
     // a stream's searches (plain) keep a raised error flag: the frames after a
     // failed one leave at once
-    launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p, plain ? 1 : 0);
+    launch_b("search_reset", 4.0 * NM_FL_WORDS, k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p, plain ? 1 : 0);
     if (started) LSH_CHECK(hipEventRecord(started, stream()));
     const NmConst *kp = (const NmConst *)f->nm_const.p;
     double *state = f->nm_state.p, *part = f->nm_part.p;
@@ -1947,7 +1947,7 @@ void frames_stream_setup(lsh_frames *f, const uint8_t *free8, uint32_t coarsen, 
     f->rec_gains.ensure(nframes * 3);
     f->rec_evals.ensure(nframes);
     f->red.ensure((size_t)std::max(1u, std::min<unsigned>(cdiv(f->gw * f->gh, 256), 1024)) * 7);
-    launch("search_reset", k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p, 0);  // a fresh error flag
+    launch_b("search_reset", 4.0 * NM_FL_WORDS, k_nm_reset, dim3(1), dim3(256), 0, f->nm_flags.p, 0);  // a fresh error flag
     launch("stream_cursor", k_cursor_set, dim3(1), dim3(64), 0, f->cursor.p, (unsigned)f->processed);
     launch("stream_state", k_state_set, dim3(1), dim3(64), 0, f->nm_state.p, Sim());  // frame 0 sits at the identity
 }

@@ -1074,7 +1074,9 @@ static void depth_grid(DepthGrid &G, SliceSource &src, uint32_t n, const float *
         log.line("depth: %u slices aggregated  (%.1fs)", n, log.secs());
     }
     // close the profiles, the noise floor (depth.rs:568-580)
-    launch_b("peak_close", 72.0 * M, k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
+    // every cell reads prev and prev2; only the cells whose last slice is a
+    // rising edge also read and write their profile, which no count can know
+    launch_b("peak_close", 8.0 * M, k_peak_close, dim3(g1(M)), dim3(256), 0, P, M, n);
     DBuf<unsigned> hist;
     size_t ns = (M + 6) / 7;
     float floor_ = G.floor_ = select_kth(PB.cmin.p, M, 7, ns / 2, hist);

@@ -1148,7 +1148,7 @@ static void wls_init(WlsSolver &s, const float *guide, float lambda, float sigma
     const int w = s.w, h = s.h;
     const size_t n = (size_t)w * h;
     s.lambda = lambda;
-    launch("wls_ticket", k_fill_u32, dim3(1), dim3(64), 0, s.ticket.p, (size_t)1, 0u);
+    launch_b("wls_ticket", 4.0, k_fill_u32, dim3(1), dim3(64), 0, s.ticket.p, (size_t)1, 0u);
     float inv = -1.0f / fmaxf(sigma_c, 1e-6f);
     launch_b("wls_edges", 20.0 * n, k_edges, g2(w, h), B2, 0, guide, w, h, inv, lambda, s.ex.p, s.ey.p, s.L[0].ax.p,
              s.L[0].ay.p);
@@ -1213,7 +1213,9 @@ static void ensure_chol(WlsSolver &s) {
     Lev<float> &c = s.L.back();
     LSH_REQUIRE(c.w <= 16 && c.h <= 16, "coarsest grid larger than 16x16");
     s.chol.ensure((size_t)c.w * c.h * (c.w + 1));
-    launch("wls_coarse_factor", k_chol_factor<float>, dim3(1), dim3(256), 0, (const float *)c.wd.p,
+    // reads the three coefficient planes, writes the banded factor (f64)
+    launch_b("wls_coarse_factor", 12.0 * c.w * c.h + 8.0 * c.w * c.h * (c.w + 1), k_chol_factor<float>, dim3(1),
+             dim3(256), 0, (const float *)c.wd.p,
            (const float *)c.ax.p, (const float *)c.ay.p, c.w, c.h, s.chol.p);
     s.chol_ok = true;
 }
@@ -1232,9 +1234,9 @@ static void ensure_l64(WlsSolver &s) {
         Lev<float> &a = s.L[l];
         Lev<double> &b = s.L64[l];
         size_t n = (size_t)a.w * a.h;
-        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.wd.p, n, b.wd.p);
-        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.ax.p, n, b.ax.p);
-        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.ay.p, n, b.ay.p);
+        launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.wd.p, n, b.wd.p);
+        launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.ax.p, n, b.ax.p);
+        launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)a.ay.p, n, b.ay.p);
         launch("wls_dinv", k_dinv<double>, g2(b.w, b.h), B2, 0, (const double *)b.wd.p, (const double *)b.ax.p,
                (const double *)b.ay.p, b.w, b.h, b.dinv.p);
     }
@@ -1373,7 +1375,7 @@ static uint32_t pcg(WlsSolver &s, std::vector<Lev<T>> &H, T *u, const T *b, doub
              cg, 1);
     launch_b("cg_dot", 2.0 * sz * n, k_dot_cg<T>, dim3(gred(n)), dim3(256), 0, (const T *)r, (const T *)r, n, s.part.p,
              s.ticket.p, cg, 2);
-    launch("cg_start", k_cg_start, dim3(1), dim3(64), 0, cg, bnorm, tol, sizeof(T) == 4 ? 1 : 0);
+    launch_b("cg_start", 2.0 * sizeof(CgState), k_cg_start, dim3(1), dim3(64), 0, cg, bnorm, tol, sizeof(T) == 4 ? 1 : 0);
     auto iteration = [&]() {
         launch_b("cg_spmv", 5.0 * sz * n, k_spmv_cg<T>, dim3(gred(n)), dim3(256), 0, (const T *)t0.wd.p,
                  (const T *)t0.ax.p, (const T *)t0.ay.p, (const T *)p, w, h, ap, s.part.p, s.ticket.p, cg);
@@ -1450,7 +1452,7 @@ WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double to
             launch("fgs_rows", k_solve_rows, dim3(1), dim3(256), 0, u, d, wd, (const float *)s.ex.p, w, h, s.lambda, s.cp.p, s.dp.p);
         else
             launch("fgs_cols", k_solve_cols, dim3(1), dim3(64), 0, u, d, wd, (const float *)s.ey.p, w, h, s.lambda, s.cp.p, s.dp.p);
-        if (u64) launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, u64);
+        if (u64) launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, u64);
         return res;
     }
     if (warm)
@@ -1476,7 +1478,7 @@ WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double to
         s.e.ensure(n);
         s.x64.ensure(n);
         double *x = u64 ? u64 : s.x64.p;
-        launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, x);
+        launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, x);
         const Lev<float> &t0 = s.L[0];
         for (uint32_t outer = 0; outer < 64; outer++) {
             launch_b("wls_resid64", 36.0 * n, k_resid64<double>, dim3(gred(n)), dim3(256), 0, (const float *)t0.wd.p,
@@ -1507,7 +1509,7 @@ WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double to
     ensure_l64(s);
     for (auto *bf : {&s.z64, &s.p64, &s.ap64, &s.x64}) bf->ensure(n);
     double *x = u64 ? u64 : s.x64.p;
-    launch("wls_promote", k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, x);
+    launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, x);
     res.iters = pcg<double>(s, s.L64, x, (const double *)s.b64.p, bnorm, tol, max_iters, true, res.rel, s.r64.p, s.z64.p,
                             s.p64.p, s.ap64.p);
     launch("wls_demote", k_convert<double, float>, dim3(g1(n)), dim3(256), 0, (const double *)x, n, u);
