@@ -265,6 +265,13 @@ def wls_precision(job, n, dp, tol=1e-6):
         la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "reference", tol=tol, max_iters=dp.cg_iters, warm=warm, u_out=u)
         out["learned"]["max_err_vs_fp64"] = float((u.double() - oracle).abs().max())
         log(f"WLS learned: {out['learned']}")
+        # the same start under the refined mode (the depth pass's), against its plain run above
+        _, rr, _ = la.wls_solve(g, c, d, dp.lam, dp.sigma_c, "refined", tol=tol, max_iters=2000, warm=warm, u_out=u,
+                                u64_out=x)
+        out["learned_refined"] = {"iters": rr["iters"], "outer": rr["outer"], "solve_ms": round(rr["ms"], 2),
+                                  "ms": round(infer_ms + rr["ms"], 2), "true_rel": rr["true_rel"],
+                                  "max_err_vs_fp64": float((u.double() - oracle).abs().max())}
+        log(f"WLS learned, refined: {out['learned_refined']}")
         net.close()
     return out
 

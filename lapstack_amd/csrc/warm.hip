@@ -350,78 +350,88 @@ __global__ void k_up_bwd(View dfine, View dcoarse, int nimg) {
 // sigma_c) (0 past the last column) and exp(-|g - g_below| / sigma_c) (0 past
 // the last row) as the solver builds them, and with a target t the mean of
 // t / dscale - dm.  cf holds the planes [dm, dmean, w, ax, ay, tc], each
-// nimg*ch*cw.
-__global__ void k_feat_blocks(const float *__restrict__ d, const float *__restrict__ conf, const float *__restrict__ g,
-                              const float *__restrict__ t, int nimg, int h, int w, int ch, int cw, float inv_scale,
-                              float inv_sigma, float *__restrict__ cf) {
+// nimg*ch*cw, in f64: the inputs are rounded once, to bf16, as they enter the
+// network.  A workgroup takes 256 columns of one block row of one image
+// (grid: column groups, block rows, images): each thread sums its column over
+// the block's rows, coalesced across the wave, and the 8 lanes of each block
+// add their sums by shuffles.
+__global__ __launch_bounds__(256) void k_feat_blocks(const float *__restrict__ d, const float *__restrict__ conf,
+                                                     const float *__restrict__ g, const float *__restrict__ t, int nimg,
+                                                     int h, int w, int ch, int cw, double inv_scale, double inv_sigma,
+                                                     double *__restrict__ cf) {
     const size_t nc = (size_t)nimg * ch * cw;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc) return;
-    const int cx = (int)(i % cw), cy = (int)((i / cw) % ch), b = (int)(i / ((size_t)cw * ch));
-    const int x0 = cx * WB, y0 = cy * WB, x1 = min(x0 + WB, w), y1 = min(y0 + WB, h);
-    float sw = 0.0f, swd = 0.0f, sd = 0.0f, sax = 0.0f, say = 0.0f, st = 0.0f;
-    for (int y = y0; y < y1; y++) {
-        const size_t row = ((size_t)b * h + y) * w;
-        for (int x = x0; x < x1; x++) {
-            const float wi = fmaxf(conf[row + x], 0.0f) + 1e-4f, di = d[row + x], gi = g[row + x];
-            sw += wi;
-            swd += wi * di;
-            sd += di;
-            if (x + 1 < w) sax += expf(fabsf(gi - g[row + x + 1]) * inv_sigma);
-            if (y + 1 < h) say += expf(fabsf(gi - g[row + w + x]) * inv_sigma);
-            if (t) st += t[row + x];
+    const int x = blockIdx.x * 256 + threadIdx.x, cy = blockIdx.y, b = blockIdx.z;
+    const int y0 = cy * WB, y1 = min(y0 + WB, h);
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // w, w d, d, ax, ay, t
+    if (x < w)
+        for (int y = y0; y < y1; y++) {
+            const size_t o = ((size_t)b * h + y) * w + x;
+            const double wi = fmax((double)conf[o], 0.0) + 1e-4, di = d[o], gi = g[o];
+            s[0] += wi;
+            s[1] += wi * di;
+            s[2] += di;
+            if (x + 1 < w) s[3] += exp(fabs(gi - (double)g[o + 1]) * inv_sigma);
+            if (y + 1 < h) s[4] += exp(fabs(gi - (double)g[o + w]) * inv_sigma);
+            if (t) s[5] += t[o];
         }
-    }
-    const float inv_n = 1.0f / (float)((y1 - y0) * (x1 - x0));
-    const float dm = swd / sw * inv_scale;
+#pragma unroll
+    for (int m = 1; m < WB; m <<= 1)
+#pragma unroll
+        for (int k = 0; k < 6; k++) s[k] += __shfl_xor(s[k], m);
+    const int cx = x / WB;
+    if ((threadIdx.x & (WB - 1)) || cx >= cw) return;
+    const size_t i = ((size_t)b * ch + cy) * cw + cx;
+    const int x0 = cx * WB, x1 = min(x0 + WB, w);
+    const double inv_n = 1.0 / (double)((y1 - y0) * (x1 - x0));
+    const double dm = s[1] / s[0] * inv_scale;
     cf[i] = dm;
-    cf[nc + i] = sd * inv_n * inv_scale;
-    cf[2 * nc + i] = sw * inv_n;
-    cf[3 * nc + i] = sax * inv_n;
-    cf[4 * nc + i] = say * inv_n;
-    if (t) cf[5 * nc + i] = st * inv_n * inv_scale - dm;
+    cf[nc + i] = s[2] * inv_n * inv_scale;
+    cf[2 * nc + i] = s[0] * inv_n;
+    cf[3 * nc + i] = s[3] * inv_n;
+    cf[4 * nc + i] = s[4] * inv_n;
+    if (t) cf[5 * nc + i] = s[5] * inv_n * inv_scale - dm;
 }
 
 // The push-pull level k: per cell of the grid k times coarser (kh = ceil(ch /
 // k)), log10 of the mean data weight and the weight-averaged dm of the coarse
 // cells inside.
-__global__ void k_feat_pp(const float *__restrict__ cf, int nimg, int ch, int cw, int k, int kh, int kw,
-                          float *__restrict__ lpk, float *__restrict__ mk) {
+__global__ void k_feat_pp(const double *__restrict__ cf, int nimg, int ch, int cw, int k, int kh, int kw,
+                          double *__restrict__ lpk, double *__restrict__ mk) {
     const size_t nc = (size_t)nimg * ch * cw;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nimg * kh * kw) return;
     const int kx = (int)(i % kw), ky = (int)((i / kw) % kh), b = (int)(i / ((size_t)kw * kh));
     const int x1 = min(kx * k + k, cw), y1 = min(ky * k + k, ch);
-    float sw = 0.0f, swd = 0.0f;
+    double sw = 0.0, swd = 0.0;
     for (int y = ky * k; y < y1; y++)
         for (int x = kx * k; x < x1; x++) {
             const size_t j = ((size_t)b * ch + y) * cw + x;
-            const float pw = cf[2 * nc + j];
+            const double pw = cf[2 * nc + j];
             sw += pw;
             swd += pw * cf[j];
         }
-    lpk[i] = log10f(sw / (float)((y1 - ky * k) * (x1 - kx * k)));
+    lpk[i] = log10(sw / (double)((y1 - ky * k) * (x1 - kx * k)));
     mk[i] = swd / sw;
 }
 
 // bilinear resampling of the (ih, iw) plane p at the cell (oy, ox) of an
 // (oh, ow) grid over the same area: source position (o + 1/2) in/out - 1/2,
 // clamped at 0, its right / lower neighbour clamped to the last cell
-__device__ __forceinline__ float resample(const float *__restrict__ p, int ih, int iw, int oh, int ow, int oy,
-                                          int ox) {
-    const float sy = fmaxf(((float)oy + 0.5f) * ((float)ih / (float)oh) - 0.5f, 0.0f);
-    const float sx = fmaxf(((float)ox + 0.5f) * ((float)iw / (float)ow) - 0.5f, 0.0f);
+template <class P>
+__device__ __forceinline__ double resample(P p, int ih, int iw, int oh, int ow, int oy, int ox) {
+    const double sy = fmax(((double)oy + 0.5) * ((double)ih / (double)oh) - 0.5, 0.0);
+    const double sx = fmax(((double)ox + 0.5) * ((double)iw / (double)ow) - 0.5, 0.0);
     const int y0 = (int)sy, x0 = (int)sx;
     const int y1 = y0 + (y0 < ih - 1 ? 1 : 0), x1 = x0 + (x0 < iw - 1 ? 1 : 0);
-    const float ly = sy - (float)y0, lx = sx - (float)x0;
-    return (1.0f - ly) * ((1.0f - lx) * p[(size_t)y0 * iw + x0] + lx * p[(size_t)y0 * iw + x1]) +
-           ly * ((1.0f - lx) * p[(size_t)y1 * iw + x0] + lx * p[(size_t)y1 * iw + x1]);
+    const double ly = sy - (double)y0, lx = sx - (double)x0;
+    return (1.0 - ly) * ((1.0 - lx) * p((size_t)y0 * iw + x0) + lx * p((size_t)y0 * iw + x1)) +
+           ly * ((1.0 - lx) * p((size_t)y1 * iw + x0) + lx * p((size_t)y1 * iw + x1));
 }
 
 // the push-pull levels of a batch: per level l the planes log10 mean weight
 // and weighted mean dm, each nimg * kh[l] * kw[l]
 struct PushPull {
-    const float *lpk[3], *mk[3];
+    const double *lpk[3], *mk[3];
     int kh[3], kw[3];
 };
 
@@ -432,60 +442,54 @@ struct PushPull {
 // then per push-pull level its weighted mean resampled to the coarse grid
 // minus dm; then w, log10(w) / 4 + 1, ax, ay, and per push-pull level log10
 // of its mean weight / 4 + 1 resampled.  Channels 10..15 zero.
-__global__ void k_feat_pack(const float *__restrict__ cf, PushPull pp, int nimg, int ch, int cw, View x) {
+__global__ void k_feat_pack(const double *__restrict__ cf, PushPull pp, int nimg, int ch, int cw, View x) {
     const size_t nc = (size_t)nimg * ch * cw;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     const int cx = (int)(i % cw), cy = (int)((i / cw) % ch), b = (int)(i / ((size_t)cw * ch));
-    const float dm = cf[i], pw = cf[2 * nc + i], r = 1.0f / WKAPPA;
-    float v[WNFEAT];
+    const double dm = cf[i], pw = cf[2 * nc + i], r = 1.0 / (double)WKAPPA;
+    double v[WNFEAT];
     v[0] = (cf[nc + i] - dm) * r;
     v[4] = pw;
-    v[5] = log10f(pw) * 0.25f + 1.0f;
+    v[5] = log10(pw) * 0.25 + 1.0;
     v[6] = cf[3 * nc + i];
     v[7] = cf[4 * nc + i];
 #pragma unroll
     for (int l = 0; l < 3; l++) {
         const size_t o = (size_t)b * pp.kh[l] * pp.kw[l];
-        v[1 + l] = (resample(pp.mk[l] + o, pp.kh[l], pp.kw[l], ch, cw, cy, cx) - dm) * r;
-        if (l < 2) v[8 + l] = resample(pp.lpk[l] + o, pp.kh[l], pp.kw[l], ch, cw, cy, cx) * 0.25f + 1.0f;
+        const double *mk = pp.mk[l] + o, *lpk = pp.lpk[l] + o;
+        v[1 + l] = (resample([&](size_t j) { return mk[j]; }, pp.kh[l], pp.kw[l], ch, cw, cy, cx) - dm) * r;
+        if (l < 2)
+            v[8 + l] = resample([&](size_t j) { return lpk[j]; }, pp.kh[l], pp.kw[l], ch, cw, cy, cx) * 0.25 + 1.0;
     }
     uint16_t *o = x.p + i * x.cs;
 #pragma unroll
-    for (int c = 0; c < WNFEAT; c++) o[c] = f2bf(v[c]);
+    for (int c = 0; c < WNFEAT; c++) o[c] = f2bf((float)v[c]);
 #pragma unroll
     for (int c = WNFEAT; c < 16; c++) o[c] = 0;
 }
 
 // the warm start on the WLS grid: dscale times the bilinear prolongation of
 // the predicted block means dm + WKAPPA out
-__global__ void k_prolong(const float *__restrict__ dm, const float *__restrict__ out32, int nimg, int ch, int cw,
-                          int h, int w, float scale, float *__restrict__ u) {
+__global__ void k_prolong(const double *__restrict__ dm, const float *__restrict__ out32, int nimg, int ch, int cw,
+                          int h, int w, double scale, float *__restrict__ u) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nimg * h * w) return;
     const int x = (int)(i % w), y = (int)((i / w) % h), b = (int)(i / ((size_t)w * h));
     const size_t o = (size_t)b * ch * cw;
-    const float sy = fmaxf(((float)y + 0.5f) * ((float)ch / (float)h) - 0.5f, 0.0f);
-    const float sx = fmaxf(((float)x + 0.5f) * ((float)cw / (float)w) - 0.5f, 0.0f);
-    const int y0 = (int)sy, x0 = (int)sx;
-    const int y1 = y0 + (y0 < ch - 1 ? 1 : 0), x1 = x0 + (x0 < cw - 1 ? 1 : 0);
-    const float ly = sy - (float)y0, lx = sx - (float)x0;
-    auto pc = [&](int yy, int xx) {
-        const size_t j = o + (size_t)yy * cw + xx;
-        return dm[j] + WKAPPA * out32[j];
-    };
-    u[i] = scale * ((1.0f - ly) * ((1.0f - lx) * pc(y0, x0) + lx * pc(y0, x1)) +
-                    ly * ((1.0f - lx) * pc(y1, x0) + lx * pc(y1, x1)));
+    const double v = resample([&](size_t j) { return dm[o + j] + (double)WKAPPA * (double)out32[o + j]; }, ch, cw, h, w,
+                              y, x);
+    u[i] = (float)(scale * v);
 }
 
 // the loss weight of each image: 1 / max(mean((tc / WKAPPA)^2), 1e-4), so
 // that every image's error counts relative to its own target's size
-__global__ __launch_bounds__(256) void k_loss_weight(const float *__restrict__ tc, int npx, float *__restrict__ wimg) {
+__global__ __launch_bounds__(256) void k_loss_weight(const double *__restrict__ tc, int npx, float *__restrict__ wimg) {
     __shared__ double sh[256];
-    const float *p = tc + (size_t)blockIdx.x * npx;
+    const double *p = tc + (size_t)blockIdx.x * npx;
     double s = 0.0;
     for (int i = threadIdx.x; i < npx; i += 256) {
-        const double v = (double)(p[i] * (1.0f / WKAPPA));
+        const double v = p[i] * (1.0 / (double)WKAPPA);
         s += v * v;
     }
     sh[threadIdx.x] = s;
@@ -499,7 +503,7 @@ __global__ __launch_bounds__(256) void k_loss_weight(const float *__restrict__ t
 
 // The loss: the mean over the images of wimg * mean((out - tc / WKAPPA)^2),
 // its gradient into channel 0 of dout (bf16); per-block f64 partial sums
-__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, const float *__restrict__ tc,
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, const double *__restrict__ tc,
                                               const float *__restrict__ wimg, int npx, size_t n, float gscale,
                                               View dout, double *__restrict__ part) {
     __shared__ double sh[256];
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out32, c
     double s = 0.0;
     if (i < n) {
         const float wi = wimg[i / npx];
-        const float e = out32[i] - tc[i] * (1.0f / WKAPPA);
+        const float e = out32[i] - (float)(tc[i] * (1.0 / (double)WKAPPA));
         s = (double)wi * (double)e * (double)e;
         uint16_t *o = dout.p + i * dout.cs;
         o[0] = f2bf(gscale * wi * e);
@@ -581,7 +585,8 @@ struct Acts {
 struct lsh_warm {
     std::vector<lsh::Layer> L;
     float sigma_c = 0.04f;  // the guide scale of the WLS systems served
-    lsh::DBuf<float> cf, pp, wimg;  // block planes, push-pull levels, loss weights
+    lsh::DBuf<double> cf, pp;  // block planes, push-pull levels
+    lsh::DBuf<float> wimg;     // loss weights
     size_t npad = 0, nnat = 0;
     lsh::DBuf<float> w32, g32, m, v;
     uint64_t step = 0;
@@ -728,18 +733,18 @@ static void features(lsh_warm &n, const float *d, const float *conf, const float
     }
     n.cf.ensure(6 * nc);
     n.pp.ensure(tot);
-    launch_b("warm_feat", (t ? 16.0 : 12.0) * nimg * h * w + 24.0 * nc, k_feat_blocks, dim3(cdiv(nc, 256)), dim3(256), 0,
-             d, conf, g, t, nimg, h, w, ch, cw, 1.0f / dscale, -1.0f / n.sigma_c, n.cf.p);
-    float *q = n.pp.p;
+    launch_b("warm_feat", (t ? 16.0 : 12.0) * nimg * h * w + (t ? 48.0 : 40.0) * nc, k_feat_blocks, dim3(cdiv(w, 256), ch, nimg),
+             dim3(256), 0, d, conf, g, t, nimg, h, w, ch, cw, 1.0 / dscale, -1.0 / (double)n.sigma_c, n.cf.p);
+    double *q = n.pp.p;
     for (int l = 0; l < 3; l++) {
-        float *lpk = q, *mk = q + np[l];
+        double *lpk = q, *mk = q + np[l];
         q += 2 * np[l];
-        launch("warm_feat", k_feat_pp, dim3(cdiv(np[l], 256)), dim3(256), 0, (const float *)n.cf.p, nimg, ch, cw, WPP[l],
+        launch("warm_feat", k_feat_pp, dim3(cdiv(np[l], 256)), dim3(256), 0, (const double *)n.cf.p, nimg, ch, cw, WPP[l],
                pp.kh[l], pp.kw[l], lpk, mk);
         pp.lpk[l] = lpk;
         pp.mk[l] = mk;
     }
-    launch("warm_feat", k_feat_pack, dim3(cdiv(nc, 256)), dim3(256), 0, (const float *)n.cf.p, pp, nimg, ch, cw,
+    launch("warm_feat", k_feat_pack, dim3(cdiv(nc, 256)), dim3(256), 0, (const double *)n.cf.p, pp, nimg, ch, cw,
            view(a.X0, ch, cw, 16, 16));
 }
 
@@ -812,7 +817,7 @@ static double backward(lsh_warm &n, int nimg) {
     View dOUT = view(a.dOUT, H[0], W[0], 16, 16);
     const int lb = (int)cdiv(n0, 256);
     const int npx = H[0] * W[0];
-    const float *tc = n.cf.p + 5 * n0;
+    const double *tc = n.cf.p + 5 * n0;
     n.lpart.ensure(lb);
     n.wimg.ensure(nimg);
     launch("warm_loss", k_loss_weight, dim3(nimg), dim3(256), 0, tc, npx, n.wimg.p);
@@ -1011,8 +1016,8 @@ char *lsh_warm_predict(lsh_warm *n, const float *d, const float *conf, const flo
         n->ubuf.ensure(cnt);
         uo = n->ubuf.p;
     }
-    launch_b("warm_prolong", 4.0 * cnt, k_prolong, dim3(cdiv(cnt, 256)), dim3(256), 0, (const float *)n->cf.p,
-             (const float *)n->A.out32.p, (int)nimg, ch, cw, (int)h, (int)w, dscale, uo);
+    launch_b("warm_prolong", 4.0 * cnt, k_prolong, dim3(cdiv(cnt, 256)), dim3(256), 0, (const double *)n->cf.p,
+             (const float *)n->A.out32.p, (int)nimg, ch, cw, (int)h, (int)w, (double)dscale, uo);
     if (out_mem == LSH_HOST) copy_mem(u, LSH_HOST, uo, LSH_DEVICE, cnt * sizeof(float));
     sync();
     LSH_API_END
