@@ -357,7 +357,7 @@ __global__ void k_up_bwd(View dfine, View dcoarse, int nimg) {
 // add their sums by shuffles.
 __global__ __launch_bounds__(256) void k_feat_blocks(const float *__restrict__ d, const float *__restrict__ conf,
                                                      const float *__restrict__ g, const float *__restrict__ t, int nimg,
-                                                     int h, int w, int ch, int cw, double inv_scale, double inv_sigma,
+                                                     int h, int w, int ch, int cw, double inv_scale, float inv_sigma,
                                                      double *__restrict__ cf) {
     const size_t nc = (size_t)nimg * ch * cw;
     const int x = blockIdx.x * 256 + threadIdx.x, cy = blockIdx.y, b = blockIdx.z;
@@ -366,12 +366,12 @@ __global__ __launch_bounds__(256) void k_feat_blocks(const float *__restrict__ d
     if (x < w)
         for (int y = y0; y < y1; y++) {
             const size_t o = ((size_t)b * h + y) * w + x;
-            const double wi = fmax((double)conf[o], 0.0) + 1e-4, di = d[o], gi = g[o];
+            const double wi = fmax((double)conf[o], 0.0) + 1e-4, di = d[o];
             s[0] += wi;
             s[1] += wi * di;
             s[2] += di;
-            if (x + 1 < w) s[3] += exp(fabs(gi - (double)g[o + 1]) * inv_sigma);
-            if (y + 1 < h) s[4] += exp(fabs(gi - (double)g[o + w]) * inv_sigma);
+            if (x + 1 < w) s[3] += expf(fabsf(g[o] - g[o + 1]) * inv_sigma);
+            if (y + 1 < h) s[4] += expf(fabsf(g[o] - g[o + w]) * inv_sigma);
             if (t) s[5] += t[o];
         }
 #pragma unroll
@@ -392,35 +392,54 @@ __global__ __launch_bounds__(256) void k_feat_blocks(const float *__restrict__ d
     if (t) cf[5 * nc + i] = s[5] * inv_n * inv_scale - dm;
 }
 
-// The push-pull level k: per cell of the grid k times coarser (kh = ceil(ch /
-// k)), log10 of the mean data weight and the weight-averaged dm of the coarse
-// cells inside.
-__global__ void k_feat_pp(const double *__restrict__ cf, int nimg, int ch, int cw, int k, int kh, int kw,
-                          double *__restrict__ lpk, double *__restrict__ mk) {
-    const size_t nc = (size_t)nimg * ch * cw;
+// A push-pull level: per cell of the grid 4, 16 or 64 times coarser than the
+// block grid (4 x 4 cells of the level before, or of the blocks), the sums of
+// w and w dm over the blocks inside and their count, then log10 of the mean
+// weight and the weighted mean.  src: the level before's planes [sw, swd,
+// cnt, ...] (nimg*ph*pw each), or null to sum the blocks (cf).  out: this
+// level's planes [sw, swd, cnt, lpk, mk], nimg*kh*kw each.
+__global__ void k_feat_pp(const double *__restrict__ cf, const double *__restrict__ src, int nimg, int ch, int cw,
+                          int ph, int pw, int kh, int kw, double *__restrict__ out) {
+    const size_t np = (size_t)nimg * kh * kw;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)nimg * kh * kw) return;
+    if (i >= np) return;
     const int kx = (int)(i % kw), ky = (int)((i / kw) % kh), b = (int)(i / ((size_t)kw * kh));
-    const int x1 = min(kx * k + k, cw), y1 = min(ky * k + k, ch);
-    double sw = 0.0, swd = 0.0;
-    for (int y = ky * k; y < y1; y++)
-        for (int x = kx * k; x < x1; x++) {
-            const size_t j = ((size_t)b * ch + y) * cw + x;
-            const double pw = cf[2 * nc + j];
-            sw += pw;
-            swd += pw * cf[j];
-        }
-    lpk[i] = log10(sw / (double)((y1 - ky * k) * (x1 - kx * k)));
-    mk[i] = swd / sw;
+    double sw = 0.0, swd = 0.0, cnt = 0.0;
+    if (!src) {
+        const size_t nc = (size_t)nimg * ch * cw;
+        for (int y = ky * 4; y < min(ky * 4 + 4, ch); y++)
+            for (int x = kx * 4; x < min(kx * 4 + 4, cw); x++) {
+                const size_t j = ((size_t)b * ch + y) * cw + x;
+                const double v = cf[2 * nc + j];
+                sw += v;
+                swd += v * cf[j];
+                cnt += 1.0;
+            }
+    } else {
+        const size_t pn = (size_t)nimg * ph * pw;
+        for (int y = ky * 4; y < min(ky * 4 + 4, ph); y++)
+            for (int x = kx * 4; x < min(kx * 4 + 4, pw); x++) {
+                const size_t j = ((size_t)b * ph + y) * pw + x;
+                sw += src[j];
+                swd += src[pn + j];
+                cnt += src[2 * pn + j];
+            }
+    }
+    out[i] = sw;
+    out[np + i] = swd;
+    out[2 * np + i] = cnt;
+    out[3 * np + i] = log10(sw / cnt);
+    out[4 * np + i] = swd / sw;
 }
 
-// bilinear resampling of the (ih, iw) plane p at the cell (oy, ox) of an
-// (oh, ow) grid over the same area: source position (o + 1/2) in/out - 1/2,
-// clamped at 0, its right / lower neighbour clamped to the last cell
+// bilinear resampling of the (ih, iw) plane p at the cell (oy, ox) of a grid
+// over the same area, ry = ih / oh and rx = iw / ow its scales: source
+// position (o + 1/2) r - 1/2, clamped at 0, its right / lower neighbour
+// clamped to the last cell
 template <class P>
-__device__ __forceinline__ double resample(P p, int ih, int iw, int oh, int ow, int oy, int ox) {
-    const double sy = fmax(((double)oy + 0.5) * ((double)ih / (double)oh) - 0.5, 0.0);
-    const double sx = fmax(((double)ox + 0.5) * ((double)iw / (double)ow) - 0.5, 0.0);
+__device__ __forceinline__ double resample(P p, int ih, int iw, double ry, double rx, int oy, int ox) {
+    const double sy = fmax(((double)oy + 0.5) * ry - 0.5, 0.0);
+    const double sx = fmax(((double)ox + 0.5) * rx - 0.5, 0.0);
     const int y0 = (int)sy, x0 = (int)sx;
     const int y1 = y0 + (y0 < ih - 1 ? 1 : 0), x1 = x0 + (x0 < iw - 1 ? 1 : 0);
     const double ly = sy - (double)y0, lx = sx - (double)x0;
@@ -433,6 +452,7 @@ __device__ __forceinline__ double resample(P p, int ih, int iw, int oh, int ow, 
 struct PushPull {
     const double *lpk[3], *mk[3];
     int kh[3], kw[3];
+    double ry[3], rx[3];  // kh / ch, kw / cw
 };
 
 // The network's input on the coarse grid.  Depth enters only as differences
@@ -458,9 +478,10 @@ __global__ void k_feat_pack(const double *__restrict__ cf, PushPull pp, int nimg
     for (int l = 0; l < 3; l++) {
         const size_t o = (size_t)b * pp.kh[l] * pp.kw[l];
         const double *mk = pp.mk[l] + o, *lpk = pp.lpk[l] + o;
-        v[1 + l] = (resample([&](size_t j) { return mk[j]; }, pp.kh[l], pp.kw[l], ch, cw, cy, cx) - dm) * r;
+        v[1 + l] = (resample([&](size_t j) { return mk[j]; }, pp.kh[l], pp.kw[l], pp.ry[l], pp.rx[l], cy, cx) - dm) * r;
         if (l < 2)
-            v[8 + l] = resample([&](size_t j) { return lpk[j]; }, pp.kh[l], pp.kw[l], ch, cw, cy, cx) * 0.25 + 1.0;
+            v[8 + l] =
+                resample([&](size_t j) { return lpk[j]; }, pp.kh[l], pp.kw[l], pp.ry[l], pp.rx[l], cy, cx) * 0.25 + 1.0;
     }
     uint16_t *o = x.p + i * x.cs;
 #pragma unroll
@@ -472,12 +493,12 @@ __global__ void k_feat_pack(const double *__restrict__ cf, PushPull pp, int nimg
 // the warm start on the WLS grid: dscale times the bilinear prolongation of
 // the predicted block means dm + WKAPPA out
 __global__ void k_prolong(const double *__restrict__ dm, const float *__restrict__ out32, int nimg, int ch, int cw,
-                          int h, int w, double scale, float *__restrict__ u) {
+                          int h, int w, double ry, double rx, double scale, float *__restrict__ u) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nimg * h * w) return;
     const int x = (int)(i % w), y = (int)((i / w) % h), b = (int)(i / ((size_t)w * h));
     const size_t o = (size_t)b * ch * cw;
-    const double v = resample([&](size_t j) { return dm[o + j] + (double)WKAPPA * (double)out32[o + j]; }, ch, cw, h, w,
+    const double v = resample([&](size_t j) { return dm[o + j] + (double)WKAPPA * (double)out32[o + j]; }, ch, cw, ry, rx,
                               y, x);
     u[i] = (float)(scale * v);
 }
@@ -728,21 +749,25 @@ static void features(lsh_warm &n, const float *d, const float *conf, const float
     for (int l = 0; l < 3; l++) {
         pp.kh[l] = (int)cdiv(ch, WPP[l]);
         pp.kw[l] = (int)cdiv(cw, WPP[l]);
+        pp.ry[l] = (double)pp.kh[l] / (double)ch;
+        pp.rx[l] = (double)pp.kw[l] / (double)cw;
         np[l] = (size_t)nimg * pp.kh[l] * pp.kw[l];
-        tot += 2 * np[l];
+        tot += 5 * np[l];
     }
     n.cf.ensure(6 * nc);
     n.pp.ensure(tot);
     launch_b("warm_feat", (t ? 16.0 : 12.0) * nimg * h * w + (t ? 48.0 : 40.0) * nc, k_feat_blocks, dim3(cdiv(w, 256), ch, nimg),
-             dim3(256), 0, d, conf, g, t, nimg, h, w, ch, cw, 1.0 / dscale, -1.0 / (double)n.sigma_c, n.cf.p);
+             dim3(256), 0, d, conf, g, t, nimg, h, w, ch, cw, 1.0 / dscale, -1.0f / n.sigma_c, n.cf.p);
     double *q = n.pp.p;
+    const double *prev = nullptr;
     for (int l = 0; l < 3; l++) {
-        double *lpk = q, *mk = q + np[l];
-        q += 2 * np[l];
-        launch("warm_feat", k_feat_pp, dim3(cdiv(np[l], 256)), dim3(256), 0, (const double *)n.cf.p, nimg, ch, cw, WPP[l],
-               pp.kh[l], pp.kw[l], lpk, mk);
-        pp.lpk[l] = lpk;
-        pp.mk[l] = mk;
+        static_assert(WPP[0] == 4 && WPP[1] == 16 && WPP[2] == 64, "push-pull levels 4 times apart");
+        launch("warm_feat", k_feat_pp, dim3(cdiv(np[l], 256)), dim3(256), 0, (const double *)n.cf.p, prev, nimg, ch, cw,
+               l ? pp.kh[l - 1] : 0, l ? pp.kw[l - 1] : 0, pp.kh[l], pp.kw[l], q);
+        pp.lpk[l] = q + 3 * np[l];
+        pp.mk[l] = q + 4 * np[l];
+        prev = q;
+        q += 5 * np[l];
     }
     launch("warm_feat", k_feat_pack, dim3(cdiv(nc, 256)), dim3(256), 0, (const double *)n.cf.p, pp, nimg, ch, cw,
            view(a.X0, ch, cw, 16, 16));
@@ -1017,7 +1042,8 @@ char *lsh_warm_predict(lsh_warm *n, const float *d, const float *conf, const flo
         uo = n->ubuf.p;
     }
     launch_b("warm_prolong", 4.0 * cnt, k_prolong, dim3(cdiv(cnt, 256)), dim3(256), 0, (const double *)n->cf.p,
-             (const float *)n->A.out32.p, (int)nimg, ch, cw, (int)h, (int)w, (double)dscale, uo);
+             (const float *)n->A.out32.p, (int)nimg, ch, cw, (int)h, (int)w, (double)ch / (double)h,
+             (double)cw / (double)w, (double)dscale, uo);
     if (out_mem == LSH_HOST) copy_mem(u, LSH_HOST, uo, LSH_DEVICE, cnt * sizeof(float));
     sync();
     LSH_API_END

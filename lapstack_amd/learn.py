@@ -208,6 +208,34 @@ def train(args):
     return summary
 
 
+def evaluate(args):
+    """The shipped weights on held-out 45 MP stacks (the seeds after the
+    training range): plain, learned and untrained starts, each to the same
+    tolerance by the same MG-PCG."""
+    import lapstack_amd as la
+
+    dp = depth_params()
+    net = la.WarmNet()
+    net.load(args.weights)
+    net.set_sigma(dp.sigma_c)
+    base = la.WarmNet(seed=1)
+    base.set_sigma(dp.sigma_c)
+    rows = []
+    for s in range(args.eval):
+        n = (100, 40, 64, 24)[s % 4]
+        g, d, c = make_problem(args.seed + args.stacks + 10_000 + s, n, dp)
+        r = solve_pair(net, g, d, c, float(n - 1), dp, ablation=base)
+        r["frames"] = n
+        rows.append(r)
+        print(json.dumps(r), flush=True)
+    keys = ("plain", "learned", "ablation")
+    summary = {"eval_stacks": args.eval,
+               **{k: {f: float(np.mean([r[k][f] for r in rows])) for f in rows[0][k]} for k in keys},
+               "wins": int(sum(r["learned"]["ms"] < r["plain"]["ms"] for r in rows))}
+    print(json.dumps(summary), flush=True)
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--stacks", type=int, default=128)
@@ -218,8 +246,11 @@ def main():
     ap.add_argument("--eval", type=int, default=8)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--infer-grid", default="4128x2752", help="time one prediction on this grid ('' to skip)")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="no training: evaluate --weights on --eval held-out stacks")
+    ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
-    sys.exit(0 if train(args) else 1)
+    sys.exit(0 if (evaluate(args) if args.eval_only else train(args)) else 1)
 
 
 if __name__ == "__main__":
