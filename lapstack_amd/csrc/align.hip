@@ -1355,33 +1355,42 @@ __device__ double lanczos3_k(double d) {
     }
     return 0.0;
 }
+// N: the kernel's tap count (2: ik 0-1, 4: ik 2-3, 6: ik 4-5), so that each
+// warp's instantiation holds only the kernels of its width (the 6-tap ones
+// bring sines and divisions whose registers the 2-tap warps would pay for)
+template <int N>
 __device__ __forceinline__ void interp_w(int ik, double t, float *wf) {
-    double w[6];
-    int N = ik <= 1 ? 2 : (ik <= 3 ? 4 : 6);
-    switch (ik) {
-    case 0:
-        w[0] = t < 0.5 ? 1.0 : 0.0;
-        w[1] = t < 0.5 ? 0.0 : 1.0;
-        break;
-    case 1:
-        w[0] = 1.0 - t;
-        w[1] = t;
-        break;
-    case 3:
-        w[0] = ((-1.0 / 3.0 * t + 0.8) * t - 0.46666667) * t;
-        w[1] = ((t - 1.8) * t - 0.2) * t + 1.0;
-        w[2] = ((1.2 - t) * t + 0.8) * t;
-        w[3] = ((1.0 / 3.0 * t - 0.2) * t - 0.13333334) * t;
-        break;
-    default: {
+    double w[N];
+    if constexpr (N == 2) {
+        if (ik == 0) {
+            w[0] = t < 0.5 ? 1.0 : 0.0;
+            w[1] = t < 0.5 ? 0.0 : 1.0;
+        } else {
+            w[0] = 1.0 - t;
+            w[1] = t;
+        }
+    } else if constexpr (N == 4) {
+        if (ik == 3) {
+            w[0] = ((-1.0 / 3.0 * t + 0.8) * t - 0.46666667) * t;
+            w[1] = ((t - 1.8) * t - 0.2) * t + 1.0;
+            w[2] = ((1.2 - t) * t + 0.8) * t;
+            w[3] = ((1.0 / 3.0 * t - 0.2) * t - 0.13333334) * t;
+        } else {
+            double sum = 0.0;
+            for (int i = 0; i < N; i++) {
+                w[i] = keys_k(fabs(t - ((double)i + 1.0 - (double)(N / 2))));
+                sum += w[i];
+            }
+            for (int i = 0; i < N; i++) w[i] /= sum;
+        }
+    } else {
         double sum = 0.0;
         for (int i = 0; i < N; i++) {
             double d = fabs(t - ((double)i + 1.0 - (double)(N / 2)));
-            w[i] = ik == 2 ? keys_k(d) : (ik == 4 ? spline36_k(d) : lanczos3_k(d));
+            w[i] = ik == 4 ? spline36_k(d) : lanczos3_k(d);
             sum += w[i];
         }
         for (int i = 0; i < N; i++) w[i] /= sum;
-    }
     }
     for (int i = 0; i < N; i++) wf[i] = (float)w[i];
 }
@@ -1531,8 +1540,8 @@ __global__ __launch_bounds__(256) void k_warp(const FrameRef src, const float *_
     double fx = floor(sx), fy = floor(sy);
     int x0 = (int)fmax(fmin(fx, 1e9), -1e9), y0 = (int)fmax(fmin(fy, 1e9), -1e9);
     float wx[6], wy[6];
-    interp_w(ik, sx - fx, wx);
-    interp_w(ik, sy - fy, wy);
+    interp_w<N>(ik, sx - fx, wx);
+    interp_w<N>(ik, sy - fy, wy);
     if (yout && planes)
         warp_px<N, true, true>(rgb, yin, w, h, o, valid, x0, y0, wx, wy, g, planes, yout);
     else if (yout)
@@ -1554,8 +1563,8 @@ __device__ __forceinline__ float warp_luma_at(const float *__restrict__ yin, int
     const double fx = floor(sx), fy = floor(sy);
     const int x0 = (int)fmax(fmin(fx, 1e9), -1e9), y0 = (int)fmax(fmin(fy, 1e9), -1e9);
     float wx[6], wy[6];
-    interp_w(ik, sx - fx, wx);
-    interp_w(ik, sy - fy, wy);
+    interp_w<N>(ik, sx - fx, wx);
+    interp_w<N>(ik, sy - fy, wy);
     constexpr int start = 1 - N / 2;
     float ay = 0.0f;
 #pragma unroll
@@ -1649,8 +1658,8 @@ __global__ __launch_bounds__(256) void k_bright_sums(const FrameRef src, int w, 
             double fx = floor(sx), fy = floor(sy);
             int x0 = (int)fmax(fmin(fx, 1e9), -1e9), y0 = (int)fmax(fmin(fy, 1e9), -1e9);
             float wx[6], wy[6];
-            interp_w(ik, sx - fx, wx);
-            interp_w(ik, sy - fy, wy);
+            interp_w<N>(ik, sx - fx, wx);
+            interp_w<N>(ik, sy - fy, wy);
             for (int c = 0; c < 3; c++)
                 val[c] = valid ? warp_sample<N>(rgb, nullptr, c, w, h, x0, y0, wx, wy) : (float)rgb[3 * o + c] * INV16;
         }
@@ -2294,8 +2303,8 @@ __global__ __launch_bounds__(256) void k_resize(const float *__restrict__ src, i
     const double fx = floor(sx), fy = floor(sy);
     const int x0 = (int)fmax(fmin(fx, 1e9), -1e9), y0 = (int)fmax(fmin(fy, 1e9), -1e9);
     float wx[6], wy[6];
-    interp_w(ik, sx - fx, wx);
-    interp_w(ik, sy - fy, wy);
+    interp_w<N>(ik, sx - fx, wx);
+    interp_w<N>(ik, sy - fy, wy);
     const size_t pn = (size_t)pw * ph, on = (size_t)ow * oh, o = (size_t)y * ow + x;
     for (int c = 0; c < np; c++) dst[c * on + o] = warp_sample<N>(nullptr, src + c * pn, -1, pw, ph, x0, y0, wx, wy);
 }
