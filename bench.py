@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--host-only", action="store_true",
                     help="only the host-steered path (one stream, cooperative search launches) over --frames frames, "
                          "twice, then the per-kernel timings: the run the HBM traffic counters are read on")
+    ap.add_argument("--wls-start", choices=["learned", "fgs"], default="learned",
+                    help="the WLS solves' start in the depth pass: the shipped learned warm start (the first solve "
+                         "from its prediction, the reweighted one from the first's result) or the reference's FGS "
+                         "sweeps; MG-PCG takes either to the same tolerance")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (twice) and exit: the calibration run of the "
                          "HBM traffic counters")
@@ -366,6 +370,14 @@ def main():
         raise SystemExit(f"HIP backend unavailable: {why}")
     w, h, n = args.width, args.height, args.frames
     fp, apm, dp = params()
+    from lapstack_amd import learn
+
+    wls_start = "fgs"
+    if args.wls_start == "learned" and os.path.exists(learn.WEIGHTS):
+        dp.warm = la.WarmNet()
+        dp.warm.load(learn.WEIGHTS)
+        dp.warm.set_sigma(dp.sigma_c)
+        wls_start = "learned"
     t0 = time.perf_counter()
     br = synth.Bracket(w, h, n, seed=args.seed, jitter=True, device="cuda")
     frames = []
@@ -473,9 +485,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{n} frames {w}x{h} RGB u16 focus bracket with similarity jitter: align (NM on "
                                f"1/4-res luma, coarsen 2, bilinear resample) + fuse (7 levels, 3x3 energy) + depth "
-                               f"(RDF, guided filter r=8 eps=1e-3, MG-PCG WLS in FP64 refinement to a 1e-6 true relative residual); frames 1..{n - 1} each one "
-                               f"replay of a captured hipGraph",
-                   "frames": n, "width": w, "height": h, "inputs": "resident in HBM", "per_frame_path": "graph"},
+                               f"(RDF, guided filter r=8 eps=1e-3, MG-PCG WLS in FP64 refinement to a 1e-6 true relative residual, "
+                               f"started from {'the learned warm start' if wls_start == 'learned' else 'the FGS sweeps'}); "
+                               f"frames 1..{n - 1} each one replay of a captured hipGraph",
+                   "frames": n, "width": w, "height": h, "inputs": "resident in HBM", "per_frame_path": "graph",
+                   "wls_start": wls_start},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,

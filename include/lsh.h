@@ -79,6 +79,11 @@ typedef struct lsh_depth_params {
     float robust;
     float tol;
     uint32_t precision;
+    /* nullable (no counterpart in the reference): a learned warm start
+     * (lsh_warm_*); the first WLS solve then starts from its prediction for
+     * the system, the reweighted solve from the first solve's result, instead
+     * of the FGS sweeps (depth.rs:1019-1034); MG-PCG still takes each to tol */
+    struct lsh_warm *warm;
 } lsh_depth_params;
 
 /* depth.rs:140-156 DepthMap's scalars plus the WLS report (depth.rs:1160,

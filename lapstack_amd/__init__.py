@@ -124,6 +124,8 @@ class DepthParams:
     robust: float = 1.0
     tol: float = 1e-5
     precision: str = "reference"  # the WLS mode: "reference" (FP32 MG-PCG) or "refined" (inside FP64 refinement)
+    # a WarmNet: the first WLS solve starts from its prediction, the reweighted one from the first's result
+    warm: Optional["WarmNet"] = None
 
     def _c(self):
         if self.precision not in ("reference", "refined"):
@@ -131,7 +133,8 @@ class DepthParams:
         return _lib.DepthParamsC(self.scale, 0 if self.focus == "rdf" else 1, self.r_in, self.r_out, self.sml_step,
                                  self.agg_radius, self.agg_eps, self.lam, self.sigma_c, self.cg_iters,
                                  int(self.median), 1 if self.upsample == "guided" else 0, self.up_radius,
-                                 self.up_eps, self.gate, self.robust, self.tol, WLS_MODES[self.precision])
+                                 self.up_eps, self.gate, self.robust, self.tol, WLS_MODES[self.precision],
+                                 self.warm._h if self.warm is not None else None)
 
     def grid(self, w, h):
         k = 1 << self.scale

@@ -28,7 +28,7 @@ class DepthParamsC(C.Structure):
                 ("agg_radius", _u32), ("agg_eps", C.c_float), ("lambda_", C.c_float), ("sigma_c", C.c_float),
                 ("cg_iters", _u32), ("median", _u32), ("upsample_kind", _u32), ("up_radius", _u32),
                 ("up_eps", C.c_float), ("gate", C.c_float), ("robust", C.c_float), ("tol", C.c_float),
-                ("precision", _u32)]
+                ("precision", _u32), ("warm", _p)]
 
 
 class DepthInfoC(C.Structure):

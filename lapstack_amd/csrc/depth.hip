@@ -1129,7 +1129,16 @@ static void depth_run(SliceSource &src, uint32_t n, const float *fused3, lsh_mem
         wls_set_weights(s.get(), conf.p, nullptr, nullptr, 0.0f);
         u.alloc(M);
         u.zero();
-        WlsResult r1 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, nullptr, u.p, nullptr);
+        // with a learned start the first solve begins at its prediction for
+        // this system (depth in frames, so dscale is the frame range), the
+        // reweighted one at the first solve's result; else both at the FGS sweeps
+        const float *start = nullptr;
+        if (p->warm) {
+            warm_predict_dev(p->warm, depth_w.p, conf.p, guide.p, dw, dh, (float)std::max<uint32_t>(n - 1, 1),
+                             p->sigma_c, u.p);
+            start = u.p;
+        }
+        WlsResult r1 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, start, u.p, nullptr);
         it1 = it2 = r1.iters;
         rel1 = rel2 = (float)r1.rel;
         trel1 = trel2 = wls_true_rel(s.get(), depth_w.p, u.p);
@@ -1137,7 +1146,7 @@ static void depth_run(SliceSource &src, uint32_t n, const float *fused3, lsh_mem
             log.line("depth: robust reweighting (tau=%s frames), first solve %u iterations, residual %s",
                      rs_f32(p->robust).c_str(), it1, rs_e1(rel1).c_str());
             wls_set_weights(s.get(), conf.p, depth_w.p, u.p, p->robust);
-            WlsResult r2 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, nullptr, u.p, nullptr);
+            WlsResult r2 = wls_solve(s.get(), depth_w.p, p->cg_iters, p->tol, mode, start, u.p, nullptr);
             it2 = r2.iters;
             rel2 = (float)r2.rel;
             trel2 = wls_true_rel(s.get(), depth_w.p, u.p);

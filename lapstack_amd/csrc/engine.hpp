@@ -110,6 +110,9 @@ void frames_stream_slots(lsh_frames *f);
 // out_base (if given): out is read from device memory there, so a captured
 // graph's launch holds no pointer to the caller's buffer (the runtime keeps
 // a buffer that a graph's kernel arguments named from going back to the device)
+// the learned start for one w x h WLS system, on the library's stream (warm.hip)
+void warm_predict_dev(lsh_warm *n, const float *d, const float *conf, const float *g, int w, int h, float dscale,
+                      float sigma_c, float *u);
 void focus_slice(lsh_frames *f, const float *planes, const lsh_depth_params &p, float *out,
                  const unsigned *out_idx = nullptr, bool luma = false, float *const *out_base = nullptr);  // depth.hip
 void frames_target(lsh_frames *f, const FrameRef &src, int slot);

@@ -740,7 +740,7 @@ static int cdim(int v) { return (v + WB - 1) / WB; }
 // activations sized for the coarse grid first); with a target t also its
 // block means
 static void features(lsh_warm &n, const float *d, const float *conf, const float *g, const float *t, int nimg, int h,
-                     int w, float dscale) {
+                     int w, float dscale, float sigma_c) {
     Acts &a = n.A;
     const int ch = a.H[0], cw = a.W[0];
     const size_t nc = (size_t)nimg * ch * cw;
@@ -757,7 +757,7 @@ static void features(lsh_warm &n, const float *d, const float *conf, const float
     n.cf.ensure(6 * nc);
     n.pp.ensure(tot);
     launch_b("warm_feat", (t ? 16.0 : 12.0) * nimg * h * w + (t ? 48.0 : 40.0) * nc, k_feat_blocks, dim3(cdiv(w, 256), ch, nimg),
-             dim3(256), 0, d, conf, g, t, nimg, h, w, ch, cw, 1.0 / dscale, -1.0f / n.sigma_c, n.cf.p);
+             dim3(256), 0, d, conf, g, t, nimg, h, w, ch, cw, 1.0 / dscale, -1.0f / sigma_c, n.cf.p);
     double *q = n.pp.p;
     const double *prev = nullptr;
     for (int l = 0; l < 3; l++) {
@@ -926,6 +926,20 @@ static Inputs stage(lsh_warm &n, const float *d, const float *conf, const float 
     return Inputs{n.dbuf.p, n.cbuf.p, n.gbuf.p, t ? n.tbuf.p : nullptr};
 }
 
+// the start for one system of the depth pass, on the library's stream (no
+// synchronisation): dscale times the prolongated predicted block means
+void warm_predict_dev(lsh_warm *n, const float *d, const float *conf, const float *g, int w, int h, float dscale,
+                      float sigma_c, float *u) {
+    const int ch = cdim(h), cw = cdim(w);
+    ensure_acts(*n, 1, ch, cw, false);
+    features(*n, d, conf, g, nullptr, 1, h, w, dscale, sigma_c);
+    forward(*n, 1);
+    const size_t cnt = (size_t)w * h;
+    launch_b("warm_prolong", 4.0 * cnt, k_prolong, dim3(cdiv(cnt, 256)), dim3(256), 0, (const double *)n->cf.p,
+             (const float *)n->A.out32.p, 1, ch, cw, h, w, (double)ch / (double)h, (double)cw / (double)w,
+             (double)dscale, u);
+}
+
 }  // namespace lsh
 
 using namespace lsh;
@@ -1034,7 +1048,7 @@ char *lsh_warm_predict(lsh_warm *n, const float *d, const float *conf, const flo
     Inputs in = stage(*n, d, conf, guide, nullptr, in_mem, cnt);
     const int ch = cdim((int)h), cw = cdim((int)w);
     ensure_acts(*n, (int)nimg, ch, cw, false);
-    features(*n, in.d, in.conf, in.g, nullptr, (int)nimg, (int)h, (int)w, dscale);
+    features(*n, in.d, in.conf, in.g, nullptr, (int)nimg, (int)h, (int)w, dscale, n->sigma_c);
     forward(*n, (int)nimg);
     float *uo = u;
     if (out_mem == LSH_HOST) {
@@ -1060,7 +1074,7 @@ char *lsh_warm_grads(lsh_warm *n, const float *d, const float *conf, const float
     const size_t cnt = (size_t)nimg * w * h;
     Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
     ensure_acts(*n, (int)nimg, cdim((int)h), cdim((int)w), true);
-    features(*n, in.d, in.conf, in.g, in.t, (int)nimg, (int)h, (int)w, dscale);
+    features(*n, in.d, in.conf, in.g, in.t, (int)nimg, (int)h, (int)w, dscale, n->sigma_c);
     forward(*n, (int)nimg);
     double l = backward(*n, (int)nimg);
     if (loss) *loss = l;
@@ -1081,7 +1095,7 @@ char *lsh_warm_train_step(lsh_warm *n, const float *d, const float *conf, const 
     const size_t cnt = (size_t)nimg * w * h;
     Inputs in = stage(*n, d, conf, guide, target, in_mem, cnt);
     ensure_acts(*n, (int)nimg, cdim((int)h), cdim((int)w), true);
-    features(*n, in.d, in.conf, in.g, in.t, (int)nimg, (int)h, (int)w, dscale);
+    features(*n, in.d, in.conf, in.g, in.t, (int)nimg, (int)h, (int)w, dscale, n->sigma_c);
     forward(*n, (int)nimg);
     double l = backward(*n, (int)nimg);
     if (loss) *loss = l;

@@ -1455,10 +1455,11 @@ WlsResult wls_solve(WlsSolver *sp, const float *d, uint32_t max_iters, double to
         if (u64) launch_b("wls_promote", 12.0 * n, k_convert<float, double>, dim3(g1(n)), dim3(256), 0, (const float *)u, n, u64);
         return res;
     }
-    if (warm)
-        LSH_CHECK(hipMemcpyAsync(u, warm, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-    else
+    if (warm) {
+        if (warm != u) LSH_CHECK(hipMemcpyAsync(u, warm, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    } else {
         fgs_guess(s, d, u);
+    }
     if (mode == WLS_REFERENCE) {
         float *b = s.f.p;
         launch_b("wls_rhs", 12.0 * n, k_rhs<float>, dim3(g1(n)), dim3(256), 0, wd, d, n, b);

@@ -159,3 +159,33 @@ def test_config4_batched_stacks_match_the_oracle_fold():
         wi, ww = f.finish()[:2]
         assert np.array_equal(img, wi) and np.array_equal(win, ww)
     assert math.isfinite(rep["ms"]) and rep["rounds"] >= 1
+
+
+def test_config3_depth_pass_with_the_learned_start(config3):
+    """The depth pass of the 45 MP stack with the shipped learned warm start
+    (the first solve from its prediction, the reweighted one from the first's
+    result), in the refined mode: both solves reach the tolerance (true
+    relative residual), and the depth stands against the oracle's (FGS start,
+    FP32 MG-PCG) at the depth bar, like the FGS-started pass."""
+    import os
+
+    from lapstack_amd import learn
+
+    got, want, n = config3
+    assert os.path.exists(learn.WEIGHTS)
+    dp = la.DepthParams(agg_radius=8, agg_eps=1e-3, tol=1e-6, precision="refined")
+    plain = la.depth_from_slices(got["slices"], n, got["fused"], dp)
+    net = la.WarmNet()
+    net.load(learn.WEIGHTS)
+    net.set_sigma(dp.sigma_c)
+    dp.warm = net
+    dm = la.depth_from_slices(got["slices"], n, got["fused"], dp)
+    info = dm.info
+    print(f"learned start: iterations {info['iters1']}/{info['iters2']} (FGS {plain.info['iters1']}/"
+          f"{plain.info['iters2']}), true residuals {info['true_rel1']:.2e}/{info['true_rel2']:.2e}", flush=True)
+    assert info["true_rel1"] <= 1e-6 and info["true_rel2"] <= 1e-6
+    for d in (dm.depth, plain.depth):
+        dd = np.abs(d - want["depth"])
+        assert (dd <= 1e-3 * (n - 1)).mean() >= 0.999, dd.max()
+    assert np.abs(dm.depth - plain.depth).max() <= 1e-3 * (n - 1)
+    net.close()
