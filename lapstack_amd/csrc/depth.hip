@@ -484,12 +484,18 @@ __global__ __launch_bounds__(256) void k_box_v2_direct(const float *__restrict__
 // The same two passes on tiles staged in LDS: BH_RB row segments of BH_TX
 // outputs and their radius-r halo (horizontal), or BV_TX columns of BV_TY outputs and
 // their 2r halo rows (vertical), loaded once with every load of a thread's
-// chunk issued before it is stored; each output then sums its window from LDS
-// with the same f64 additions in the same order as the direct kernels.
+// chunk issued before it is stored.  Each thread then takes a run of
+// consecutive outputs along the pass (BH_RUN in a row, BV_RUN in a column):
+// the first window summed in f64 from its start, each next one by adding the
+// sample that enters and subtracting the one that leaves, as the reference's
+// running sum does (depth.rs:293-331).  The f64 sums of f32 samples are exact
+// (see box_h2), so every window's value is the direct kernels' to the bit.
 #define BH_TX 256
-#define BH_RB 4  // rows per horizontal tile
+#define BH_RB 4   // rows per horizontal tile
+#define BH_RUN 4  // outputs per thread along a row: BH_RB * BH_TX / BH_RUN = 256 threads
 #define BV_TX 64
 #define BV_TY 32
+#define BV_RUN 8  // outputs per thread down a column: BV_TX * BV_TY / BV_RUN = 256 threads
 #define BOX_TILE_MAX_R 32
 __global__ __launch_bounds__(256) void k_box_h2_tile(const float *__restrict__ a, const float *__restrict__ b,
                                                      const float *__restrict__ g, int mode, int w, int h, int r,
@@ -522,26 +528,42 @@ __global__ __launch_bounds__(256) void k_box_h2_tile(const float *__restrict__ a
         }
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x;
-    if (x >= w) return;
-    const int lo = max(x - r, 0), hi = min(x + r, w - 1);
-    // the rows' sums are independent chains, interleaved
-    double sa[BH_RB], sb[BH_RB];
+    constexpr int TPR = BH_TX / BH_RUN;  // threads per row
+    const int j = threadIdx.x / TPR, xs = x0 + (threadIdx.x % TPR) * BH_RUN;
+    if (j >= nr || xs >= w) return;
+    const float *ra = Sa + j * L - c0, *rb = Sb + j * L - c0;  // indexed by the column
+    double sa = 0.0, sb = 0.0;
+    for (int i = max(xs - r, 0); i <= min(xs + r, w - 1); i++) {
+        sa += (double)ra[i];
+        sb += (double)rb[i];
+    }
+    float va[BH_RUN], vb[BH_RUN];
+    va[0] = (float)sa;
+    vb[0] = (float)sb;
 #pragma unroll
-    for (int j = 0; j < BH_RB; j++) sa[j] = sb[j] = 0.0;
-    for (int i = lo; i <= hi; i++) {
-#pragma unroll
-        for (int j = 0; j < BH_RB; j++) {
-            sa[j] += (double)Sa[j * L + i - c0];
-            sb[j] += (double)Sb[j * L + i - c0];
+    for (int u = 1; u < BH_RUN; u++) {
+        const int x = xs + u;
+        if (x + r <= w - 1) {
+            sa += (double)ra[x + r];
+            sb += (double)rb[x + r];
+        }
+        if (x - r - 1 >= 0) {
+            sa -= (double)ra[x - r - 1];
+            sb -= (double)rb[x - r - 1];
+        }
+        va[u] = (float)sa;
+        vb[u] = (float)sb;
+    }
+    const size_t o = (size_t)(y0 + j) * w + xs;
+    if (xs + BH_RUN <= w && (((uintptr_t)(oa + o) | (uintptr_t)(ob + o)) & 15) == 0) {
+        *(float4 *)(oa + o) = make_float4(va[0], va[1], va[2], va[3]);
+        *(float4 *)(ob + o) = make_float4(vb[0], vb[1], vb[2], vb[3]);
+    } else {
+        for (int u = 0; u < BH_RUN && xs + u < w; u++) {
+            oa[o + u] = va[u];
+            ob[o + u] = vb[u];
         }
     }
-#pragma unroll
-    for (int j = 0; j < BH_RB; j++)
-        if (j < nr) {
-            oa[(size_t)(y0 + j) * w + x] = (float)sa[j];
-            ob[(size_t)(y0 + j) * w + x] = (float)sb[j];
-        }
 }
 
 __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ ta, const float *__restrict__ tb,
@@ -576,38 +598,26 @@ __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ t
             }
         }
     __syncthreads();
-    if (x >= w) return;
-    // two rows at a time (y and y + 4): where their windows are as long, the
-    // two sums are interleaved, each still added from its window's top down
-    for (int k = 0; k < BV_TY / 4; k += 2) {
-        const int ya = y0 + ty + 4 * k, yb = ya + 4;
-        if (ya >= h) break;
-        const bool two = yb < h;
-        const int loa = max(ya - r, 0), hia = min(ya + r, h - 1);
-        const int lob = max(yb - r, 0), hib = min(yb + r, h - 1);
-        double sa = 0.0, sb = 0.0, ua = 0.0, ub = 0.0;
-        if (two && hia - loa == hib - lob) {
-            const float *pa = Sa + (loa - r0) * BV_TX + tx, *pb = Sb + (loa - r0) * BV_TX + tx;
-            const int d4 = (lob - loa) * BV_TX;
-            for (int i = 0; i <= hia - loa; i++) {
-                sa += (double)pa[i * BV_TX];
-                sb += (double)pb[i * BV_TX];
-                ua += (double)pa[i * BV_TX + d4];
-                ub += (double)pb[i * BV_TX + d4];
-            }
-        } else {
-            for (int i = loa; i <= hia; i++) {
-                sa += (double)Sa[(i - r0) * BV_TX + tx];
-                sb += (double)Sb[(i - r0) * BV_TX + tx];
-            }
-            if (two)
-                for (int i = lob; i <= hib; i++) {
-                    ua += (double)Sa[(i - r0) * BV_TX + tx];
-                    ub += (double)Sb[(i - r0) * BV_TX + tx];
-                }
+    const int ys = y0 + ty * BV_RUN;
+    if (x >= w || ys >= h) return;
+    const float *ca = Sa + tx - r0 * BV_TX, *cb = Sb + tx - r0 * BV_TX;  // indexed by the row
+    double sa = 0.0, sb = 0.0;
+    for (int i = max(ys - r, 0); i <= min(ys + r, h - 1); i++) {
+        sa += (double)ca[i * BV_TX];
+        sb += (double)cb[i * BV_TX];
+    }
+    box_v_tail(x, ys, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
+    for (int u = 1; u < BV_RUN && ys + u < h; u++) {
+        const int y = ys + u;
+        if (y + r <= h - 1) {
+            sa += (double)ca[(y + r) * BV_TX];
+            sb += (double)cb[(y + r) * BV_TX];
         }
-        box_v_tail(x, ya, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
-        if (two) box_v_tail(x, yb, w, h, r, mode, ua, ub, guide, mean_i, var_i, eps, oa, ob, P, m);
+        if (y - r - 1 >= 0) {
+            sa -= (double)ca[(y - r - 1) * BV_TX];
+            sb -= (double)cb[(y - r - 1) * BV_TX];
+        }
+        box_v_tail(x, y, w, h, r, mode, sa, sb, guide, mean_i, var_i, eps, oa, ob, P, m);
     }
 }
 
@@ -825,7 +835,10 @@ static void gf_step2(GF &g, int mode, float *oa, float *ob, const Peaks &P, uint
     double M = (double)g.w * g.h;
     box_h2("box_h", 16.0 * M, g.s1.p, g.s2.p, nullptr, 0, g.w, g.h, g.r, g.t1.p, g.t2.p);
     // mode 2: t1, t2 and the guide read, the nine-plane peak state read and written
-    double by = mode == 2 ? (12.0 + 72.0) * M : 16.0 * M;
+    // the peak push reads and writes prev, prev2, sum and cmin at every cell;
+    // only where the previous slice was a peak does it also touch the profile
+    // (c1, c2, i1, l1, r1), which no count can know beforehand
+    double by = mode == 2 ? (12.0 + 32.0) * M : 16.0 * M;
     box_v2(mode == 2 ? "box_v_peaks" : "box_v", by, g.t1.p, g.t2.p, g.w, g.h, g.r, mode, g.guide, nullptr, nullptr,
            0.0f, oa, ob, P, m);
 }
