@@ -770,11 +770,12 @@ static const dim3 B2(64, 4);
 static unsigned g1(size_t n) { return cdiv(n, 256); }
 
 // The two box passes: the LDS-tiled kernels when the radius fits their tiles,
-// else the direct ones (the same window sums either way).  Each window is
-// summed directly in f64, where the reference keeps an f64 running sum along
-// the row (depth.rs:293-331); the two agree whenever the f64 sums are exact,
-// which they are for f32 planes whose values span fewer than about 2^24 in
-// magnitude over a window's row (the rounding to f32 then sees the same value).
+// else the direct ones (the same window sums either way).  The direct kernels
+// sum each window in f64; the tiled ones slide an f64 sum over a run of a few
+// windows; the reference keeps an f64 running sum along the whole row
+// (depth.rs:293-331).  All three agree whenever the f64 sums are exact, which
+// they are for f32 planes whose values span fewer than about 2^24 in
+// magnitude over a row (the rounding to f32 then sees the same value).
 static void box_h2(const char *name, double bytes, const float *a, const float *b, const float *g, int mode, int w,
                    int h, int r, float *oa, float *ob) {
     if (r <= BOX_TILE_MAX_R && cdiv(h, BH_RB) <= 65535) {
