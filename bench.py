@@ -319,7 +319,7 @@ def upload_inclusive(job, fp, apm, dp):
             "bytes_uploaded_per_step": n * frame_bytes,
             "h2d_GBps_in_the_step": round(n * frame_bytes / step_s / 1e9, 2),
             "h2d_GBps_link_alone": round(n * frame_bytes / link_s / 1e9, 2),
-            "what": "frames in pinned host memory, each uploaded in two halves on two copy streams through a four-slot ring beside "
+            "what": "frames in pinned host memory, each uploaded in two halves on two copy streams through an eight-slot ring beside "
                     "the replayed graph (one step after a warm-up step)"}
 
 
