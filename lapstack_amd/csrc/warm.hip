@@ -729,8 +729,11 @@ static void conv(const View &in, const uint16_t *w, const float *bias, int cop, 
         conv_kc<16>(in, w, bias, cop, out, out32, mask, relu, nimg);
 }
 
-static void ew(const char *name, size_t n, void (*k)(View, View, int), View a, View b, int nimg) {
-    launch(name, k, dim3(cdiv(n, 256)), dim3(256), 0, a, b, nimg);
+// an element-wise pass over n outputs of bf16 (2 bytes): bpe the bytes it
+// moves per output (pooling and the upsampling's adjoint read four inputs per
+// output, 10; the upsampling reads one per four outputs, 2.5)
+static void ew(const char *name, size_t n, double bpe, void (*k)(View, View, int), View a, View b, int nimg) {
+    launch_b(name, bpe * (double)n, k, dim3(cdiv(n, 256)), dim3(256), 0, a, b, nimg);
 }
 
 // the coarse grid of an h x w WLS grid
@@ -793,20 +796,20 @@ static void forward(lsh_warm &n, int nimg) {
     auto px = [&](int l) { return (size_t)nimg * H[l] * W[l]; };
     conv(X0, L[E0A].wf.p, B(E0A), L[E0A].cop, A0a, nullptr, none, true, nimg);
     conv(A0a, L[E0B].wf.p, B(E0B), L[E0B].cop, S0, nullptr, none, true, nimg);
-    ew("warm_pool", px(1) * 16, k_pool, S0, P0, nimg);
+    ew("warm_pool", px(1) * 16, 10.0, k_pool, S0, P0, nimg);
     conv(P0, L[E1A].wf.p, B(E1A), L[E1A].cop, A1a, nullptr, none, true, nimg);
     conv(A1a, L[E1B].wf.p, B(E1B), L[E1B].cop, S1, nullptr, none, true, nimg);
-    ew("warm_pool", px(2) * 32, k_pool, S1, P1, nimg);
+    ew("warm_pool", px(2) * 32, 10.0, k_pool, S1, P1, nimg);
     conv(P1, L[E2A].wf.p, B(E2A), L[E2A].cop, A2a, nullptr, none, true, nimg);
     conv(A2a, L[E2B].wf.p, B(E2B), L[E2B].cop, S2, nullptr, none, true, nimg);
-    ew("warm_pool", px(3) * 64, k_pool, S2, P2, nimg);
+    ew("warm_pool", px(3) * 64, 10.0, k_pool, S2, P2, nimg);
     conv(P2, L[E3A].wf.p, B(E3A), L[E3A].cop, A3a, nullptr, none, true, nimg);
     conv(A3a, L[E3B].wf.p, B(E3B), L[E3B].cop, B3, nullptr, none, true, nimg);
-    ew("warm_up", px(2) * 64, k_up, B3, U2, nimg);
+    ew("warm_up", px(2) * 64, 2.5, k_up, B3, U2, nimg);
     conv(C2, L[D2].wf.p, B(D2), L[D2].cop, D2v, nullptr, none, true, nimg);
-    ew("warm_up", px(1) * 64, k_up, D2v, U1, nimg);
+    ew("warm_up", px(1) * 64, 2.5, k_up, D2v, U1, nimg);
     conv(C1, L[D1].wf.p, B(D1), L[D1].cop, D1v, nullptr, none, true, nimg);
-    ew("warm_up", px(0) * 32, k_up, D1v, U0, nimg);
+    ew("warm_up", px(0) * 32, 2.5, k_up, D1v, U0, nimg);
     conv(C0, L[D0].wf.p, B(D0), L[D0].cop, D0v, nullptr, none, true, nimg);
     View dummy = View{nullptr, H[0], W[0], 1, 1};
     conv(D0v, L[OUT].wf.p, B(OUT), L[OUT].cop, dummy, a.out32.p, none, false, nimg);
@@ -873,13 +876,13 @@ static double backward(lsh_warm &n, int nimg) {
     conv(dOUT, L[OUT].wt.p, nullptr, L[OUT].cip, dD0, nullptr, D0v, false, nimg);
     weight_grad(n, D0, C0, dD0, nimg);
     conv(dD0, L[D0].wt.p, nullptr, L[D0].cip, dC0, nullptr, C0, false, nimg);
-    ew("warm_up_bwd", px(1) * 32, k_up_bwd, dU0, dD1, nimg);
+    ew("warm_up_bwd", px(1) * 32, 10.0, k_up_bwd, dU0, dD1, nimg);
     weight_grad(n, D1, C1, dD1, nimg);
     conv(dD1, L[D1].wt.p, nullptr, L[D1].cip, dC1, nullptr, C1, false, nimg);
-    ew("warm_up_bwd", px(2) * 64, k_up_bwd, dU1, dD2, nimg);
+    ew("warm_up_bwd", px(2) * 64, 10.0, k_up_bwd, dU1, dD2, nimg);
     weight_grad(n, D2, C2, dD2, nimg);
     conv(dD2, L[D2].wt.p, nullptr, L[D2].cip, dC2, nullptr, C2, false, nimg);
-    ew("warm_up_bwd", px(3) * 64, k_up_bwd, dU2, dB3, nimg);
+    ew("warm_up_bwd", px(3) * 64, 10.0, k_up_bwd, dU2, dB3, nimg);
     weight_grad(n, E3B, A3a, dB3, nimg);
     conv(dB3, L[E3B].wt.p, nullptr, L[E3B].cip, dA3a, nullptr, A3a, false, nimg);
     weight_grad(n, E3A, P2, dA3a, nimg);
