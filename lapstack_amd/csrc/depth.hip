@@ -431,6 +431,38 @@ __device__ __forceinline__ void peak_push(const Peaks &P, size_t i, float v, uin
     P.prev[i] = v;
 }
 
+// peak_push of slice m's value v1 then slice m + 1's v2, the state kept in
+// registers between the two (the same operations in the same order)
+__device__ __forceinline__ void peak_push2(const Peaks &P, size_t i, float v1, float v2, uint32_t m) {
+    float prev = P.prev[i], prev2 = P.prev2[i], sum = P.sum[i], cmin = P.cmin[i];
+    float c1 = 0.0f, c2 = 0.0f;
+    bool have = false;
+    auto one = [&](float v, uint32_t mm) {
+        if (mm >= 1 && (mm == 1 || prev >= prev2) && prev > v) {
+            if (!have) {
+                c1 = P.c1[i];
+                c2 = P.c2[i];
+                have = true;
+            }
+            px_register(P, i, prev, mm - 1, mm >= 2 ? prev2 : -1.0f, v, c1, c2);
+        }
+        sum += v;
+        cmin = fminf(cmin, v);
+        prev2 = prev;
+        prev = v;
+    };
+    one(v1, m);
+    one(v2, m + 1);
+    if (have) {
+        P.c1[i] = c1;
+        P.c2[i] = c2;
+    }
+    P.sum[i] = sum;
+    P.cmin[i] = cmin;
+    P.prev2[i] = prev2;
+    P.prev[i] = prev;
+}
+
 // Vertical window sums of two planes, then per mode:
 //  0  guide statistics: oa = mean(I), ob = var(I) = max(mean(I^2) - mean(I)^2, 0)
 //  1  guided filter step 1 (depth.rs:382-389): oa = a, ob = b from mean(p), mean(I p)
@@ -621,6 +653,66 @@ __global__ __launch_bounds__(256) void k_box_v2_tile(const float *__restrict__ t
     }
 }
 
+// The guided filter's last vertical pass (mode 2 of k_box_v2_tile) for two
+// slices at once, m (ta, tb) and m + 1 (tc, td), fused with both peak pushes,
+// so the peak state is read and written once per pair of slices
+__global__ __launch_bounds__(256) void k_box_v4_peaks_tile(const float *__restrict__ ta, const float *__restrict__ tb,
+                                                           const float *__restrict__ tc, const float *__restrict__ td,
+                                                           int w, int h, int r, const float *__restrict__ guide,
+                                                           Peaks P, uint32_t m) {
+    extern __shared__ float S[];  // [4][BV_TY + 2 r][BV_TX]
+    const int RS = (BV_TY + 2 * r) * BV_TX;
+    float *Sp[4] = {S, S + RS, S + 2 * RS, S + 3 * RS};
+    const float *src[4] = {ta, tb, tc, td};
+    const int x0 = blockIdx.x * BV_TX, y0 = blockIdx.y * BV_TY;
+    const int tx = threadIdx.x & (BV_TX - 1), ty = threadIdx.x / BV_TX, x = x0 + tx;
+    const int r0 = max(y0 - r, 0), nr = min(y0 + BV_TY - 1 + r, h - 1) - r0 + 1;
+    if (x < w)
+        for (int q0 = ty; q0 < nr; q0 += 4 * 2) {
+            float v[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int q = q0 + 4 * u;
+                if (q < nr) {
+                    const size_t o = (size_t)(r0 + q) * w + x;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) v[u][c] = src[c][o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int q = q0 + 4 * u;
+                if (q < nr)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) Sp[c][q * BV_TX + tx] = v[u][c];
+            }
+        }
+    __syncthreads();
+    const int ys = y0 + ty * BV_RUN;
+    if (x >= w || ys >= h) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = max(ys - r, 0); i <= min(ys + r, h - 1); i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) s[c] += (double)Sp[c][(i - r0) * BV_TX + tx];
+    for (int u = 0; u < BV_RUN && ys + u < h; u++) {
+        const int y = ys + u;
+        if (u > 0) {
+            if (y + r <= h - 1)
+#pragma unroll
+                for (int c = 0; c < 4; c++) s[c] += (double)Sp[c][(y + r - r0) * BV_TX + tx];
+            if (y - r - 1 >= 0)
+#pragma unroll
+                for (int c = 0; c < 4; c++) s[c] -= (double)Sp[c][(y - r - 1 - r0) * BV_TX + tx];
+        }
+        const float ic = inv_count(x, y, w, h, r);
+        const size_t o = (size_t)y * w + x;
+        const float g = guide[o];
+        const float q1 = ((float)s[0] * ic) * g + (float)s[1] * ic;
+        const float q2 = ((float)s[2] * ic) * g + (float)s[3] * ic;
+        peak_push2(P, o, fmaxf(q1, 0.0f), fmaxf(q2, 0.0f), m);
+    }
+}
+
 __global__ void k_peak_push_raw(const float *__restrict__ c, size_t n, Peaks P, uint32_t m) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) peak_push(P, i, c[i], m);
@@ -807,6 +899,7 @@ struct GF {
     float eps;
     const float *guide;
     DBuf<float> mean_i, var_i, t1, t2, s1, s2;
+    DBuf<float> t3, t4, s3, s4;  // the second slice of a pair
 };
 
 static void gf_init(GF &g, const float *guide, int w, int h, int r, float eps) {
@@ -822,21 +915,35 @@ static void gf_init(GF &g, const float *guide, int w, int h, int r, float eps) {
     box_v2("box_v", 0.0, g.t1.p, g.t2.p, w, h, r, 0, nullptr, nullptr, nullptr, 0.0f, g.mean_i.p, g.var_i.p, none, 0u);
 }
 
-// step 1 of a filtering of p: s1 = a, s2 = b
-static void gf_step1(GF &g, const float *p) {
+// step 1 of a filtering of p: sa = a, sb = b (s1, s2 unless given)
+static void gf_step1(GF &g, const float *p, float *sa = nullptr, float *sb = nullptr) {
     Peaks none{};
     double M = (double)g.w * g.h;
     box_h2("box_h", 16.0 * M, p, nullptr, g.guide, 1, g.w, g.h, g.r, g.t1.p, g.t2.p);
-    box_v2("box_v", 24.0 * M, g.t1.p, g.t2.p, g.w, g.h, g.r, 1, g.guide, g.mean_i.p, g.var_i.p, g.eps, g.s1.p, g.s2.p,
-           none, 0u);
+    box_v2("box_v", 24.0 * M, g.t1.p, g.t2.p, g.w, g.h, g.r, 1, g.guide, g.mean_i.p, g.var_i.p, g.eps, sa ? sa : g.s1.p,
+           sb ? sb : g.s2.p, none, 0u);
+}
+
+// the filterings of slices m (s1, s2 from step 1) and m + 1 (s3, s4) to their
+// ends, their peak pushes in one pass over the peak state
+static void gf_step2_pair(GF &g, const Peaks &P, uint32_t m) {
+    double M = (double)g.w * g.h;
+    box_h2("box_h", 16.0 * M, g.s1.p, g.s2.p, nullptr, 0, g.w, g.h, g.r, g.t1.p, g.t2.p);
+    box_h2("box_h", 16.0 * M, g.s3.p, g.s4.p, nullptr, 0, g.w, g.h, g.r, g.t3.p, g.t4.p);
+    LSH_REQUIRE(g.r <= 16, "depth: paired pass radius");
+    const size_t shm = 4 * (size_t)(BV_TY + 2 * g.r) * BV_TX * sizeof(float);
+    // four planes and the guide read; the peak state's four planes read and
+    // written once for the two slices (the profile only where a peak lands)
+    launch_b("box_v_peaks2", (20.0 + 32.0) * M, k_box_v4_peaks_tile, dim3(cdiv(g.w, BV_TX), cdiv(g.h, BV_TY)), dim3(256),
+             shm, (const float *)g.t1.p, (const float *)g.t2.p, (const float *)g.t3.p, (const float *)g.t4.p, g.w, g.h,
+             g.r, g.guide, P, m);
 }
 
 // step 2: the box means of a and b, then per mode (2 peak push, 3 coefficients into oa/ob, 4 filtered into oa)
 static void gf_step2(GF &g, int mode, float *oa, float *ob, const Peaks &P, uint32_t m) {
     double M = (double)g.w * g.h;
     box_h2("box_h", 16.0 * M, g.s1.p, g.s2.p, nullptr, 0, g.w, g.h, g.r, g.t1.p, g.t2.p);
-    // mode 2: t1, t2 and the guide read, the nine-plane peak state read and written
-    // the peak push reads and writes prev, prev2, sum and cmin at every cell;
+    // mode 2: t1, t2 and the guide read; the peak push reads and writes prev, prev2, sum and cmin at every cell;
     // only where the previous slice was a peak does it also touch the profile
     // (c1, c2, i1, l1, r1), which no count can know beforehand
     double by = mode == 2 ? (12.0 + 32.0) * M : 16.0 * M;
@@ -859,6 +966,8 @@ struct SliceSource {
     virtual const float *get(uint32_t m, uint64_t &cells, lsh_mem &mem) = 0;
     // the pass has read slice m
     virtual void done(uint32_t, lsh_mem) {}
+    // every slice's memory stays the caller's until the pass ends (two may be read at once)
+    virtual bool stable() const { return false; }
 };
 
 struct ArraySource final : SliceSource {
@@ -866,6 +975,7 @@ struct ArraySource final : SliceSource {
     lsh_mem mem;
     size_t M;
     ArraySource(const float *b, lsh_mem m, size_t cells) : base(b), mem(m), M(cells) {}
+    bool stable() const override { return mem == LSH_DEVICE; }
     const float *get(uint32_t m, uint64_t &cells, lsh_mem &mm) override {
         cells = M;
         mm = mem;
@@ -1062,9 +1172,9 @@ static void depth_grid(DepthGrid &G, SliceSource &src, uint32_t n, const float *
                  focus, dw, dh, k, p->agg_radius, rs_f32(p->agg_eps).c_str(), log.secs());
     }
     DBuf<float> stage;
-    for (uint32_t m = 0; m < n; m++) {
+    auto fetch = [&](uint32_t m, lsh_mem &mem) {
         uint64_t cells = 0;
-        lsh_mem mem = LSH_HOST;
+        mem = LSH_HOST;
         const float *c = src.get(m, cells, mem);
         if (!c) throw Error("depth: slice " + std::to_string(m) + " of " + std::to_string(n) + " missing");
         if (cells != M)
@@ -1075,6 +1185,27 @@ static void depth_grid(DepthGrid &G, SliceSource &src, uint32_t n, const float *
             copy_mem(stage.p, LSH_DEVICE, c, LSH_HOST, M * sizeof(float));
             c = stage.p;
         }
+        return c;
+    };
+    // slices in pairs where the source keeps both in place (a device array):
+    // the last pass of both filterings updates the peak state once
+    // (radius up to 16: the paired pass's four staged planes then fit 64 KB of LDS)
+    const bool pairs = p->agg_radius > 0 && p->agg_radius <= 16 && src.stable();
+    if (pairs && n >= 2)
+        for (auto *b : {&gf.t3, &gf.t4, &gf.s3, &gf.s4}) b->alloc(M);
+    for (uint32_t m = 0; m < n;) {
+        lsh_mem mem = LSH_HOST, mem2 = LSH_HOST;
+        const float *c = fetch(m, mem);
+        if (pairs && m + 1 < n) {
+            const float *c2 = fetch(m + 1, mem2);
+            gf_step1(gf, c);
+            gf_step1(gf, c2, gf.s3.p, gf.s4.p);
+            gf_step2_pair(gf, P, m);
+            src.done(m, mem);
+            src.done(m + 1, mem2);
+            m += 2;
+            continue;
+        }
         if (p->agg_radius > 0) {
             gf_step1(gf, c);
             gf_step2(gf, 2, nullptr, nullptr, P, m);
@@ -1082,6 +1213,7 @@ static void depth_grid(DepthGrid &G, SliceSource &src, uint32_t n, const float *
             launch_b("peak_push", 76.0 * M, k_peak_push_raw, dim3(g1(M)), dim3(256), 0, c, M, P, m);
         }
         src.done(m, mem);
+        m++;
     }
     if (log.on()) {
         sync();
