@@ -157,3 +157,40 @@ def test_trained_warm_start_on_fresh_stacks():
     assert learned_it < base_it
     net.close()
     base.close()
+
+
+@pytest.mark.parametrize("w,h,n,precision", [(384, 288, 6, "reference"), (520, 390, 5, "refined")])
+def test_depth_pass_with_the_learned_start_on_small_stacks(w, h, n, precision):
+    """lsh_depth_params.warm on stacks far smaller than the network was trained
+    on (and with an odd slice count, so the paired peak pass ends on a single
+    slice): both solves still reach the tolerance, and the depth stays within
+    1e-3 of the frame range of the FGS-started pass's."""
+    import os
+
+    import torch
+
+    from lapstack_amd import learn, synth
+
+    print(f"learned start in the depth pass, {n} x {w}x{h} {precision}", flush=True)
+    assert os.path.exists(learn.WEIGHTS)
+    br = synth.Bracket(w, h, n, seed=41, jitter=True, device="cuda")
+    frames = [br.frame(i) for i in range(n)]
+    dp = la.DepthParams(agg_radius=8, agg_eps=1e-3, tol=1e-6, precision=precision)
+    dw, dh = dp.grid(w, h)
+    sl = torch.empty((n, dh, dw), dtype=torch.float32, device="cuda")
+    out = la.stack_stream(frames, la.FuseParams(levels=5), la.AlignParams(coarsen=1), dp, graph=True, slices_out=sl)
+    img = torch.from_numpy(np.ascontiguousarray(out["image"])).cuda()
+    plain = la.depth_from_slices(sl, n, img, dp)
+    net = la.WarmNet()
+    net.load(learn.WEIGHTS)
+    dp.warm = net
+    dm = la.depth_from_slices(sl, n, img, dp)
+    info = dm.info
+    print(info, flush=True)
+    if precision == "refined":
+        assert info["true_rel1"] <= 1e-6 and info["true_rel2"] <= 1e-6
+    else:
+        assert info["rel1"] <= 1e-6 and info["rel2"] <= 1e-6
+    dd = np.abs(dm.depth - plain.depth)
+    assert (dd <= 1e-3 * (n - 1)).mean() >= 0.999, dd.max()
+    net.close()
