@@ -44,9 +44,11 @@ def _per_frame(frames, fp, ap, dp):
     return dict(image=image, winner=winner, sims=np.array(sims), evals=np.array(evals), slices=slices)
 
 
-# 5 and 4 frames: each parity's graph captured once; 9 frames: both replayed (frames 2..7 replay, 8 runs eagerly)
+# 5 and 4 frames: each parity's graph captured once; 9 frames: both replayed (frames 2..7 replay, 8 runs eagerly);
+# ragged sizes (odd, no multiple of any tile) with the 6- and 4-tap kernels, through every edge tile
 @pytest.mark.parametrize("w,h,n,interp", [(384, 288, 5, "bilinear"), (320, 240, 4, "spline4x4"),
-                                          (256, 192, 9, "bilinear")])
+                                          (256, 192, 9, "bilinear"), (301, 217, 6, "lanczos3"),
+                                          (257, 163, 5, "bicubic")])
 def test_stream_matches_the_per_frame_path(w, h, n, interp):
     print(f"stream {w}x{h}x{n} {interp}", flush=True)
     b = synth.Bracket(w, h, n, seed=17, jitter=True, device="cuda")
